@@ -1,0 +1,136 @@
+/*
+ * ntx_hip.h -- C ABI of the MI355X (gfx950) backend for NVTT's BC1/BC3/BC4/BC5
+ * block compressors and mipmap downsampler.
+ *
+ * This is the seam that replaces NVTT 2.1.2's CUDA backend
+ * (src/nvtt/cuda/CudaCompressorDXT.{h,cpp}, src/nvtt/cuda/CudaUtils.cpp) and
+ * the single-threaded FloatImage mip filters behind Surface::buildNextMipmap.
+ * Plain C types only: pointers, sizes and ints.  Every call returns an
+ * ntx_status (0 = success).
+ *
+ * Threading: one context owns one HIP stream.  Calls on one context are
+ * serialised by the caller (as Compressor::compress is in the reference,
+ * whose CUDA compressor shares one result buffer: CudaCompressorDXT.cpp:67-118).
+ * Entry points suffixed _device take device pointers and only enqueue work on
+ * the context's stream; the others take host pointers and return when the
+ * result is in host memory.
+ */
+#ifndef NTX_HIP_H
+#define NTX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define NTX_API __attribute__((visibility("default")))
+#else
+#define NTX_API
+#endif
+
+typedef enum {
+    NTX_OK = 0,
+    NTX_ERR_INVALID_ARGUMENT = 1, /* maps to nvtt::Error_InvalidInput */
+    NTX_ERR_UNSUPPORTED = 2,      /* maps to nvtt::Error_UnsupportedFeature */
+    NTX_ERR_DEVICE = 3,           /* a HIP runtime error; maps to nvtt::Error_CudaError (value 3, kept for ABI) */
+    NTX_ERR_NO_DEVICE = 4,
+    NTX_ERR_OUT_OF_MEMORY = 5
+} ntx_status;
+
+/* nvtt::Format values (nvtt.h:77-97) accepted by the compressor. */
+enum { NTX_FORMAT_BC1 = 1, NTX_FORMAT_BC3 = 4, NTX_FORMAT_BC4 = 6, NTX_FORMAT_BC5 = 7 };
+/* nvtt::Quality (nvtt.h:148-154). */
+enum { NTX_QUALITY_FASTEST = 0, NTX_QUALITY_NORMAL = 1, NTX_QUALITY_PRODUCTION = 2, NTX_QUALITY_HIGHEST = 3 };
+/* nvtt::AlphaMode (nvtt.h:274-279). */
+enum { NTX_ALPHA_NONE = 0, NTX_ALPHA_TRANSPARENCY = 1, NTX_ALPHA_PREMULTIPLIED = 2 };
+/* nvtt::MipmapFilter (nvtt.h:236-241) and nvtt::WrapMode (nvtt.h:209-214). */
+enum { NTX_FILTER_BOX = 0, NTX_FILTER_TRIANGLE = 1, NTX_FILTER_KAISER = 2 };
+enum { NTX_WRAP_CLAMP = 0, NTX_WRAP_REPEAT = 1, NTX_WRAP_MIRROR = 2 };
+
+typedef struct ntx_hip_context ntx_hip_context;
+
+/* ---------------------------------------------------------------- devices --
+ * Replace cuda::isHardwarePresent / cuda::getFastestDevice
+ * (src/nvtt/cuda/CudaUtils.cpp:136-224, called from Context.cpp:61-73). */
+NTX_API int ntx_hip_is_hardware_present(void);
+NTX_API ntx_status ntx_hip_device_count(int *count);
+NTX_API const char *ntx_hip_status_string(ntx_status status);
+
+/* --------------------------------------------------------------- contexts --
+ * Replace CudaContext (CudaCompressorDXT.cpp:67-118), created by
+ * Compressor::enableCudaAcceleration (Context.cpp:81-98).  Uploads the cluster
+ * and single-colour tables and creates the stream. */
+NTX_API ntx_status ntx_hip_create(int device, ntx_hip_context **ctx);
+NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx);
+/* Text of the last HIP error seen by this context ("" if none). */
+NTX_API const char *ntx_hip_last_error(const ntx_hip_context *ctx);
+/* The hipStream_t the context launches on (as a void*). */
+NTX_API void *ntx_hip_stream(ntx_hip_context *ctx);
+
+/* Device memory staging for callers that do not own a HIP allocator. */
+NTX_API ntx_status ntx_hip_malloc(ntx_hip_context *ctx, size_t bytes, void **dptr);
+NTX_API ntx_status ntx_hip_free(ntx_hip_context *ctx, void *dptr);
+NTX_API ntx_status ntx_hip_upload(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes);
+NTX_API ntx_status ntx_hip_download(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes);
+NTX_API ntx_status ntx_hip_synchronize(ntx_hip_context *ctx);
+
+/* Stream-ordered timer: records an event pair around the work enqueued
+ * between begin and end; end waits and returns the elapsed milliseconds. */
+NTX_API ntx_status ntx_hip_timer_begin(ntx_hip_context *ctx);
+NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms);
+
+/* ------------------------------------------------------------- compression --
+ * Size in bytes of the block-compressed image (Surface.cpp:210-218
+ * computeImageSize for block formats): ceil(w/4)*ceil(h/4)*{8,16}. */
+NTX_API size_t ntx_hip_compressed_size(int format, int w, int h);
+
+/* Replaces CudaCompressor::compress (CudaCompressorDXT.cpp:128-206) behind
+ * nv::CompressorInterface::compress (src/nvtt/Compressor.h:34-38), extended
+ * from BC1 to BC1/BC3/BC4/BC5 at Quality_Normal:
+ *   BC1: icbc compress_dxt1 Level8 per block (BlockCompressor.cpp:116-222),
+ *   BC3: QuickCompress alpha + squish weighted cluster fit colour (CompressorDX9.cpp:145-176),
+ *   BC4/BC5: QuickCompress::compressDXT5A on R / R,G (CompressorDX10.cpp:42-60).
+ * rgba: planar FP32, four planes R,G,B,A of w*h floats (Surface::data(),
+ * Surface.cpp:492-495).  out: ntx_hip_compressed_size() bytes in block-raster
+ * order.  color_weights: CompressionOptions::setColorWeights (r,g,b,a).
+ * Quality_Fastest and other formats return NTX_ERR_UNSUPPORTED. */
+NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                    const float *color_weights, const float *rgba, int w, int h,
+                                    void *out, size_t out_bytes);
+NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                           const float *color_weights, const float *d_rgba, int w, int h,
+                                           void *d_out);
+/* BC1 only: also writes, per block, the colour count after reduce_colors (n)
+ * and the count seen by the three-colour fit (m, -1 if not run) as int16
+ * pairs; these fix the algorithmic flop count of each block. */
+NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int alpha_mode,
+                                                     const float *color_weights, const float *d_rgba,
+                                                     int w, int h, void *d_out, int16_t *d_nm);
+
+/* ---------------------------------------------------------------- mipmaps --
+ * Replaces Surface::buildNextMipmap's filtering (Surface.cpp:1344-1406):
+ * Box with width 0.5 -> FloatImage::fastDownSample (FloatImage.cpp:559-737);
+ * otherwise FloatImage::downSample -> resize with PolyphaseKernel weights
+ * (FloatImage.cpp:740-808, 1115-1176; Filter.cpp:563-608).  params: Kaiser
+ * (alpha, stretch), may be NULL.  Output is max(1,W/2) x max(1,H/2), four
+ * planes.  The alpha mode only changes channel order in the reference
+ * (FloatImage.cpp:877-925), so results do not depend on it. */
+NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params,
+                                      int wrap, const float *rgba, int W, int H, float *out);
+NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, float width, const float *params,
+                                             int wrap, const float *d_rgba, int W, int H, float *d_out);
+
+/* Surface::toLinearFromSrgb (Surface.cpp:1562-1592) on the R,G,B planes, in place. */
+NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
+/* Surface::setImage(InputFormat_BGRA_8UB, ...) (Surface.cpp:745-761):
+ * 8-bit B,G,R,A texels to planar FP32 (value / 255). */
+NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
+                                               float *d_rgba);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NTX_HIP_H */

@@ -1,0 +1,201 @@
+// mip.hip -- mipmap downsampling for gfx950, bit-exact with NVTT's
+// single-threaded FloatImage filters (same tap order, no fused multiply-add):
+//   Box, width 0.5: FloatImage::fastDownSample (FloatImage.cpp:559-737)
+//   otherwise: FloatImage::resize (FloatImage.cpp:761-808) = applyKernelX into
+//   a w x H temporary, then applyKernelY (FloatImage.cpp:1115-1176), with the
+//   host-built PolyphaseKernel weights (Filter.cpp:563-608).
+// Planar FP32 RGBA, four planes.  Every kernel is a coalesced elementwise pass
+// (one output texel per lane, consecutive lanes on consecutive x).
+#include "ntx_common.h"
+
+#include <cmath>
+
+namespace {
+
+__global__ __launch_bounds__(256) void box_kernel(const float *__restrict__ src, int W, int H, float *__restrict__ dst,
+                                                  int w, int h)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int c = blockIdx.z;
+    if (x >= w) return;
+    const float *s = src + size_t(c) * size_t(W) * size_t(H);
+    float *d = dst + size_t(c) * size_t(w) * size_t(h);
+    float r;
+    if (W == 1 || H == 1) {
+        // 1D case: n outputs along the longer axis.
+        const int n = w * h;
+        const int i = x + y * w;  // one of w, h is 1
+        if (i >= n) return;
+        const float *p = s + 2 * size_t(i);
+        if ((W * H) & 1) {
+            const float scale = 1.0f / float(2 * n + 1);
+            const float w0 = float(n - i), w1 = float(n), w2 = float(1 + i);
+            r = scale * ((w0 * p[0] + w1 * p[1]) + w2 * p[2]);
+        } else {
+            r = 0.5f * (p[0] + p[1]);
+        }
+        d[i] = r;
+        return;
+    }
+    const float *p = s + size_t(2 * y) * size_t(W) + 2 * size_t(x);
+    if ((W & 1) == 0 && (H & 1) == 0) {
+        r = 0.25f * (((p[0] + p[1]) + p[W]) + p[W + 1]);
+    } else if ((W & 1) && (H & 1)) {
+        const float scale = 1.0f / float(W * H);
+        const float v0 = float(h - y), v1 = float(h), v2 = float(1 + y);
+        const float w0 = float(w - x), w1 = float(w), w2 = float(1 + x);
+        float f = 0.0f;
+        f += v0 * ((w0 * p[0] + w1 * p[1]) + w2 * p[2]);
+        f += v1 * ((w0 * p[W] + w1 * p[W + 1]) + w2 * p[W + 2]);
+        f += v2 * ((w0 * p[2 * W] + w1 * p[2 * W + 1]) + w2 * p[2 * W + 2]);
+        r = f * scale;
+    } else if (W & 1) {
+        const float scale = 1.0f / float(2 * W);
+        const float w0 = float(w - x), w1 = float(w), w2 = float(1 + x);
+        float f = 0.0f;
+        f += w0 * (p[0] + p[W]);
+        f += w1 * (p[1] + p[W + 1]);
+        f += w2 * (p[2] + p[W + 2]);
+        r = f * scale;
+    } else {
+        const float scale = 1.0f / float(2 * H);
+        const float v0 = float(h - y), v1 = float(h), v2 = float(1 + y);
+        float f = 0.0f;
+        f += v0 * (p[0] + p[1]);
+        f += v1 * (p[W] + p[W + 1]);
+        f += v2 * (p[2 * W] + p[2 * W + 1]);
+        r = f * scale;
+    }
+    d[size_t(y) * size_t(w) + x] = r;
+}
+
+// FloatImage.h:298-330 wrap helpers.
+__device__ __forceinline__ int wrap_index(int x, int n, int wm)
+{
+    if (x >= 0 && x < n) return x;
+    if (wm == NTX_WRAP_CLAMP) return x < 0 ? 0 : n - 1;
+    if (wm == NTX_WRAP_REPEAT) return x >= 0 ? x % n : (x + 1) % n + n - 1;
+    if (n == 1) return 0;
+    x = abs(x);
+    while (x >= n) x = abs(n + n - x - 2);
+    return x;
+}
+
+// applyKernelX: tmp(i, y) = sum_j k(i, j) * src(wrap(left_i + j), y).
+__global__ __launch_bounds__(256) void poly_x_kernel(const float *__restrict__ src, int W, int H, float *__restrict__ tmp,
+                                                     int w, const float *__restrict__ kw, const int *__restrict__ left,
+                                                     int window, int wrap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int c = blockIdx.z;
+    if (i >= w) return;
+    const float *row = src + (size_t(c) * size_t(H) + size_t(y)) * size_t(W);
+    const float *k = kw + size_t(i) * window;
+    const int l = left[i];
+    float sum = 0.0f;
+    if (l >= 0 && l + window <= W) {
+        for (int j = 0; j < window; ++j) sum += k[j] * row[l + j];
+    } else {
+        for (int j = 0; j < window; ++j) sum += k[j] * row[wrap_index(l + j, W, wrap)];
+    }
+    tmp[(size_t(c) * size_t(H) + size_t(y)) * size_t(w) + i] = sum;
+}
+
+// applyKernelY on the temporary: dst(x, i) = sum_j k(i, j) * tmp(x, wrap(left_i + j)).
+__global__ __launch_bounds__(256) void poly_y_kernel(const float *__restrict__ tmp, int w, int H, float *__restrict__ dst,
+                                                     int h, const float *__restrict__ kw, const int *__restrict__ left,
+                                                     int window, int wrap)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const int c = blockIdx.z;
+    if (x >= w) return;
+    const float *col = tmp + size_t(c) * size_t(w) * size_t(H) + x;
+    const float *k = kw + size_t(i) * window;
+    const int l = left[i];
+    float sum = 0.0f;
+    if (l >= 0 && l + window <= H) {
+        for (int j = 0; j < window; ++j) sum += k[j] * col[size_t(l + j) * size_t(w)];
+    } else {
+        for (int j = 0; j < window; ++j) sum += k[j] * col[size_t(wrap_index(l + j, H, wrap)) * size_t(w)];
+    }
+    dst[(size_t(c) * size_t(h) + size_t(i)) * size_t(w) + x] = sum;
+}
+
+// Surface.cpp:1562-1569 fromSrgb.  Values that are exactly k/255 (every
+// texel of an 8-bit source) take the host-computed table entry, so they match
+// the host powf bit for bit; other values use the device powf.
+__global__ __launch_bounds__(256) void srgb_kernel(float *__restrict__ d, size_t n, const float *__restrict__ lut)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float f = d[i];
+    const float k = rintf(f * 255.0f);
+    if (k >= 0.0f && k <= 255.0f && __float_as_uint(k / 255.0f) == __float_as_uint(f)) {
+        d[i] = lut[int(k)];
+        return;
+    }
+    if (f < 0.0f) f = 0.0f;
+    else if (f < 0.04045f) f = f / 12.92f;
+    else if (f <= 1.0f) f = powf((f + 0.055f) / 1.055f, 2.4f);
+    else f = 1.0f;
+    d[i] = f;
+}
+
+// Surface.cpp:745-761: BGRA8 texels to planar float / 255.
+__global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ src, size_t n, float *__restrict__ dst)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uchar4 c = src[i];  // x=b, y=g, z=r, w=a
+    dst[i] = float(c.z) / 255.0f;
+    dst[i + n] = float(c.y) / 255.0f;
+    dst[i + 2 * n] = float(c.x) / 255.0f;
+    dst[i + 3 * n] = float(c.w) / 255.0f;
+}
+
+}  // namespace
+
+namespace ntx {
+
+hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    if (W == 1 || H == 1) {
+        const int n = w * h;
+        hipLaunchKernelGGL(box_kernel, dim3((n + 255) / 256, 1, 4), dim3(256), 0, stream, d_src, W, H, d_dst, n, 1);
+    } else {
+        hipLaunchKernelGGL(box_kernel, dim3((w + 255) / 256, h, 4), dim3(256), 0, stream, d_src, W, H, d_dst, w, h);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_poly(const float *d_src, int W, int H, float *d_tmp, float *d_dst, const float *d_kx, const int *d_lx,
+                       int wx, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    hipLaunchKernelGGL(poly_x_kernel, dim3((w + 255) / 256, H, 4), dim3(256), 0, stream, d_src, W, H, d_tmp, w, d_kx, d_lx,
+                       wx, wrap);
+    hipLaunchKernelGGL(poly_y_kernel, dim3((w + 255) / 256, h, 4), dim3(256), 0, stream, d_tmp, w, H, d_dst, h, d_ky, d_ly,
+                       wy, wrap);
+    return hipGetLastError();
+}
+
+hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(srgb_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, d, n, d_lut);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(unpack_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const uchar4 *>(d_bgra), n, d_dst);
+    return hipGetLastError();
+}
+
+}  // namespace ntx

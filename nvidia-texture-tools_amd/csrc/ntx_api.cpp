@@ -1,0 +1,424 @@
+// ntx_api.cpp -- the C ABI declared in include/ntx_hip.h: contexts, staging,
+// dispatch to the gfx950 kernels.  No CPU compression path exists here: when
+// the device is missing or a HIP call fails the entry points return an error
+// status, which the nvtt front end reports as Error_CudaError.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ntx_common.h"
+
+namespace ntx {
+hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
+                      int16_t *d_nm, hipStream_t stream);
+hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
+hipError_t launch_bc3_color(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
+                            void *d_out, hipStream_t stream);
+hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
+hipError_t launch_poly(const float *d_src, int W, int H, float *d_tmp, float *d_dst, const float *d_kx, const int *d_lx,
+                       int wx, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
+hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
+hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
+}  // namespace ntx
+
+namespace {
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct PolyDev {
+    float *weights = nullptr;
+    int *left = nullptr;
+    int window = 0;
+};
+
+using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
+
+uint32_t fbits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// Surface.cpp:1562-1569 fromSrgb, evaluated on the host for the 256 values an
+// 8-bit source can hold.
+float from_srgb(float f)
+{
+    if (f < 0.0f) f = 0.0f;
+    else if (f < 0.04045f) f = f / 12.92f;
+    else if (f <= 1.0f) f = std::pow((f + 0.055f) / 1.055f, 2.4f);
+    else f = 1.0f;
+    return f;
+}
+
+}  // namespace
+
+struct ntx_hip_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ntx::Tables *d_tab = nullptr;
+    float *d_srgb = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf in, out, tmp, dst;
+    std::map<PolyKey, PolyDev> poly;
+    std::string err;
+};
+
+namespace {
+
+ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what)
+{
+    if (ctx) ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? NTX_ERR_OUT_OF_MEMORY : NTX_ERR_DEVICE;
+}
+
+#define NTX_HIP(ctx, call)                                \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return fail(ctx, e_, #call); \
+    } while (0)
+
+ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes) return NTX_OK;
+    if (b.ptr) NTX_HIP(ctx, hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.cap = 0;
+    NTX_HIP(ctx, hipMalloc(&b.ptr, bytes));
+    b.cap = bytes;
+    return NTX_OK;
+}
+
+ntx_status bind(ntx_hip_context *ctx)
+{
+    if (!ctx) return NTX_ERR_INVALID_ARGUMENT;
+    NTX_HIP(ctx, hipSetDevice(ctx->device));
+    return NTX_OK;
+}
+
+// Quality levels handled exactly by the kernels (qualityLevel in
+// BlockCompressor.cpp:202-209; chooseCpuCompressor, Context.cpp:1038-1163).
+bool supported(int format, int quality)
+{
+    switch (format) {
+    case NTX_FORMAT_BC1: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_HIGHEST;
+    case NTX_FORMAT_BC3: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
+    case NTX_FORMAT_BC4:
+    case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
+    default: return false;
+    }
+}
+
+ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out)
+{
+    const float alpha = (filter == NTX_FILTER_KAISER && params) ? params[0] : 4.0f;
+    const float stretch = (filter == NTX_FILTER_KAISER && params) ? params[1] : 1.0f;
+    PolyKey key(filter, fbits(width), fbits(alpha), fbits(stretch), src, dst);
+    auto it = ctx->poly.find(key);
+    if (it != ctx->poly.end()) { *out = &it->second; return NTX_OK; }
+    ntx::Kernel1D k;
+    const float p[2] = { alpha, stretch };
+    ntx::build_polyphase(filter, width, p, src, dst, &k);
+    // applyKernelX/Y (FloatImage.cpp:1118-1133): first source tap per output.
+    std::vector<int> left(dst);
+    for (int i = 0; i < dst; i++) left[i] = int(std::floor((0.5f + i) * k.iscale - k.width));
+    PolyDev d;
+    d.window = k.window;
+    hipError_t e = hipMalloc(&d.weights, sizeof(float) * size_t(k.window) * dst);
+    if (e == hipSuccess) e = hipMalloc(&d.left, sizeof(int) * dst);
+    if (e == hipSuccess) e = hipMemcpy(d.weights, k.weights, sizeof(float) * size_t(k.window) * dst, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.left, left.data(), sizeof(int) * dst, hipMemcpyHostToDevice);
+    ntx::free_polyphase(&k);
+    if (e != hipSuccess) {
+        if (d.weights) (void)hipFree(d.weights);
+        if (d.left) (void)hipFree(d.left);
+        return fail(ctx, e, "polyphase table upload");
+    }
+    auto res = ctx->poly.emplace(key, d);
+    *out = &res.first->second;
+    return NTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+NTX_API int ntx_hip_is_hardware_present(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0;
+}
+
+NTX_API ntx_status ntx_hip_device_count(int *count)
+{
+    if (!count) return NTX_ERR_INVALID_ARGUMENT;
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) { *count = 0; return NTX_ERR_NO_DEVICE; }
+    if (e != hipSuccess) return NTX_ERR_DEVICE;
+    return *count > 0 ? NTX_OK : NTX_ERR_NO_DEVICE;
+}
+
+NTX_API const char *ntx_hip_status_string(ntx_status s)
+{
+    switch (s) {
+    case NTX_OK: return "ok";
+    case NTX_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case NTX_ERR_UNSUPPORTED: return "unsupported format or quality";
+    case NTX_ERR_DEVICE: return "HIP device error";
+    case NTX_ERR_NO_DEVICE: return "no HIP device";
+    case NTX_ERR_OUT_OF_MEMORY: return "out of device memory";
+    }
+    return "unknown status";
+}
+
+NTX_API ntx_status ntx_hip_create(int device, ntx_hip_context **out)
+{
+    if (!out) return NTX_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return NTX_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return NTX_ERR_INVALID_ARGUMENT;
+    ntx_hip_context *ctx = new ntx_hip_context;
+    ctx->device = device;
+    auto cleanup = [&](ntx_status s) { ntx_hip_destroy(ctx); return s; };
+    if (hipSetDevice(device) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
+    ntx::Tables *tab = new ntx::Tables;
+    ntx::build_tables(tab);
+    float lut[256];
+    for (int k = 0; k < 256; k++) lut[k] = from_srgb(float(k) / 255.0f);
+    hipError_t e = hipMalloc(&ctx->d_tab, sizeof(ntx::Tables));
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_tab, tab, sizeof(ntx::Tables), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_srgb, sizeof(lut));
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_srgb, lut, sizeof(lut), hipMemcpyHostToDevice);
+    delete tab;
+    if (e != hipSuccess) return cleanup(e == hipErrorOutOfMemory ? NTX_ERR_OUT_OF_MEMORY : NTX_ERR_DEVICE);
+    *out = ctx;
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
+{
+    if (!ctx) return NTX_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->poly) {
+        (void)hipFree(kv.second.weights);
+        (void)hipFree(kv.second.left);
+    }
+    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst })
+        if (b->ptr) (void)hipFree(b->ptr);
+    if (ctx->d_tab) (void)hipFree(ctx->d_tab);
+    if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return NTX_OK;
+}
+
+NTX_API const char *ntx_hip_last_error(const ntx_hip_context *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+NTX_API void *ntx_hip_stream(ntx_hip_context *ctx) { return ctx ? reinterpret_cast<void *>(ctx->stream) : nullptr; }
+
+NTX_API ntx_status ntx_hip_malloc(ntx_hip_context *ctx, size_t bytes, void **dptr)
+{
+    if (!dptr) return NTX_ERR_INVALID_ARGUMENT;
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_free(ntx_hip_context *ctx, void *dptr)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (dptr) NTX_HIP(ctx, hipFree(dptr));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_upload(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_download(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_synchronize(ntx_hip_context *ctx)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_timer_begin(ntx_hip_context *ctx)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    NTX_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0.0f;
+    NTX_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t;
+    return NTX_OK;
+}
+
+NTX_API size_t ntx_hip_compressed_size(int format, int w, int h)
+{
+    if (w <= 0 || h <= 0) return 0;
+    const size_t bs = (format == NTX_FORMAT_BC1 || format == NTX_FORMAT_BC4) ? 8 : 16;
+    return size_t((w + 3) / 4) * size_t((h + 3) / 4) * bs;
+}
+
+NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                           const float *cw, const float *d_rgba, int w, int h, void *d_out)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_rgba || !d_out))) return NTX_ERR_INVALID_ARGUMENT;
+    if (!supported(format, quality)) return NTX_ERR_UNSUPPORTED;
+    if (size_t(w) * size_t(h) == 0) return NTX_OK;
+    const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
+    if (!cw) cw = ones;
+    hipError_t e = hipSuccess;
+    if (format == NTX_FORMAT_BC1) {
+        e = ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream);
+    } else if (format == NTX_FORMAT_BC3) {
+        e = ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream);
+        if (e == hipSuccess)
+            e = ntx::launch_bc3_color(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, ctx->d_tab, d_out, ctx->stream);
+    } else {
+        e = ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream);
+    }
+    if (e != hipSuccess) return fail(ctx, e, "compress launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int alpha_mode, const float *cw,
+                                                     const float *d_rgba, int w, int h, void *d_out, int16_t *d_nm)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || !d_nm || (size_t(w) * size_t(h) > 0 && (!d_rgba || !d_out))) return NTX_ERR_INVALID_ARGUMENT;
+    const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
+    hipError_t e = ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw ? cw : ones, ctx->d_tab, d_out, d_nm, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, e, "bc1 launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int quality, int alpha_mode, const float *cw,
+                                    const float *rgba, int w, int h, void *out, size_t out_bytes)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0) return NTX_ERR_INVALID_ARGUMENT;
+    if (!supported(format, quality)) return NTX_ERR_UNSUPPORTED;
+    const size_t need = ntx_hip_compressed_size(format, w, h);
+    if (need == 0) return NTX_OK;
+    if (!rgba || !out || out_bytes < need) return NTX_ERR_INVALID_ARGUMENT;
+    const size_t in_bytes = size_t(w) * size_t(h) * 4 * sizeof(float);
+    if ((s = ensure(ctx, ctx->in, in_bytes)) || (s = ensure(ctx, ctx->out, need))) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(ctx->in.ptr, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    s = ntx_hip_compress_device(ctx, format, quality, alpha_mode, cw, static_cast<const float *>(ctx->in.ptr), w, h,
+                                ctx->out.ptr);
+    if (s) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(out, ctx->out.ptr, need, hipMemcpyDeviceToHost, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,
+                                             const float *d_rgba, int W, int H, float *d_out)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (W < 1 || H < 1 || (W == 1 && H == 1) || !d_rgba || !d_out) return NTX_ERR_INVALID_ARGUMENT;
+    if (filter < NTX_FILTER_BOX || filter > NTX_FILTER_KAISER || wrap < NTX_WRAP_CLAMP || wrap > NTX_WRAP_MIRROR)
+        return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e;
+    if (filter == NTX_FILTER_BOX && width == 0.5f) {
+        e = ntx::launch_box(d_rgba, W, H, d_out, ctx->stream);
+    } else {
+        const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+        PolyDev *kx = nullptr, *ky = nullptr;
+        if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
+        if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
+        if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
+        e = ntx::launch_poly(d_rgba, W, H, static_cast<float *>(ctx->tmp.ptr), d_out, kx->weights, kx->left, kx->window,
+                             ky->weights, ky->left, ky->window, wrap, ctx->stream);
+    }
+    if (e != hipSuccess) return fail(ctx, e, "downsample launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,
+                                      const float *rgba, int W, int H, float *out)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (W < 1 || H < 1 || (W == 1 && H == 1) || !rgba || !out) return NTX_ERR_INVALID_ARGUMENT;
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    const size_t in_bytes = size_t(W) * size_t(H) * 4 * sizeof(float);
+    const size_t out_bytes = size_t(w) * size_t(h) * 4 * sizeof(float);
+    if ((s = ensure(ctx, ctx->in, in_bytes)) || (s = ensure(ctx, ctx->dst, out_bytes))) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(ctx->in.ptr, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    s = ntx_hip_downsample_device(ctx, filter, width, params, wrap, static_cast<const float *>(ctx->in.ptr), W, H,
+                                  static_cast<float *>(ctx->dst.ptr));
+    if (s) return s;
+    NTX_HIP(ctx, hipMemcpyAsync(out, ctx->dst.ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_rgba, int w, int h)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && !d_rgba)) return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e = ntx::launch_srgb(d_rgba, size_t(w) * size_t(h) * 3, ctx->d_srgb, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, e, "srgb launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h, float *d_rgba)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e = ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, e, "unpack launch");
+    return NTX_OK;
+}
+
+}  // extern "C"

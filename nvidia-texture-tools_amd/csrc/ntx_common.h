@@ -1,0 +1,66 @@
+// ntx_common.h -- shared definitions for the gfx950 BCn / mipmap kernels.
+//
+// All kernels are compiled with -ffp-contract=off: the reference CPU path has
+// no fused multiply-add, and every BCn decision (partition winner, 565
+// rounding, index choice) must see the same float roundings it does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ntx_hip.h"
+
+namespace ntx {
+
+// Tables shared by host set-up and kernels.  Layout is fixed here so the host
+// builders and device code agree.
+struct Tables {
+    // icbc cluster tables (icbc.h:1906-2012): packed c0 | c1<<8 | c2<<16.
+    uint32_t four[976];
+    int four_total[16];
+    uint32_t three[160];
+    int three_total[16];
+    // icbc single-colour tables for the D3D10 decoder (icbc.h:3166-3262): mx | mn<<8.
+    uint16_t match5[256];
+    uint16_t match6[256];
+    // nvtt OMatch5/6 (SingleColorLookup.cpp:34-89): mx | mn<<8.
+    uint16_t omatch5[256];
+    uint16_t omatch6[256];
+    // 565 bit-expansion midpoints as tabulated by icbc (icbc.h:1513-1526).
+    float mid5[32];
+    float mid6[64];
+    // squish Compress4 loop order (weightedclusterfit.cpp:495-553) for a point
+    // count n: entries [sq_off[n], sq_off[n+1]) hold c0 | c1<<8 | c2<<16.
+    int sq_off[18];
+    uint32_t sq[4845];
+};
+
+void build_tables(Tables *t);
+
+// Polyphase weights (Filter.cpp:563-608) built on the host.
+struct Kernel1D {
+    int length = 0;      // output samples
+    int window = 0;      // taps per output
+    float width = 0.0f;  // kernel half-width in source samples
+    float iscale = 0.0f; // source samples per output sample
+    float *weights = nullptr; // length * window
+};
+void build_polyphase(int filter, float width, const float *params, int src_len, int dst_len, Kernel1D *k);
+void free_polyphase(Kernel1D *k);
+int polyphase_window(float width, int src_len, int dst_len);
+
+}  // namespace ntx
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define NTX_DEVICE 1
+#endif
+
+// Synchronisation of the lanes of one wavefront: orders LDS traffic between
+// lanes of the same wave (a compiler fence; the hardware keeps one wave's LDS
+// operations in order).
+__device__ __forceinline__ void ntx_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}

@@ -1,0 +1,193 @@
+// ntx_tables.cpp -- host-built lookup tables uploaded to the device once per
+// context, and the polyphase filter weights built per mip level.
+//
+// Built with -ffp-contract=off like the kernels: the polyphase weights feed
+// the mip kernels and must round exactly as Filter.cpp's do.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "ntx_common.h"
+
+namespace ntx {
+
+static int expand5(int v) { return (v << 3) | (v >> 2); }
+static int expand6(int v) { return (v << 2) | (v >> 4); }
+
+void build_tables(Tables *t)
+{
+    std::memset(t, 0, sizeof(*t));
+
+    // icbc.h:1906-2012.  Level k of the table holds the cumulative partitions
+    // (c0, c0+c1, k): every partition of fewer points already appears at an
+    // earlier level.  The tail replicates the last entry 8 times.
+    int i = 0;
+    for (int k = 1; k <= 16; k++) {
+        for (int a = 0; a <= k; a++)
+            for (int b = a; b <= k; b++) t->four[i++] = uint32_t(a) | uint32_t(b) << 8 | uint32_t(k) << 16;
+        t->four_total[k - 1] = i;
+    }
+    for (int r = 0; r < 8; r++) t->four[968 + r] = t->four[967];
+    i = 0;
+    for (int k = 1; k <= 16; k++) {
+        for (int a = 0; a <= k; a++) t->three[i++] = uint32_t(a) | uint32_t(k) << 8;
+        t->three_total[k - 1] = i;
+    }
+    for (int r = 0; r < 8; r++) t->three[152 + r] = t->three[151];
+
+    // icbc.h:3166-3262, Decoder_D3D10: worst of the AMD and NVIDIA interpolators.
+    for (int v = 0; v < 256; v++) {
+        int best5 = 25600, best6 = 25600;
+        for (int lo = 0; lo < 32; lo++)
+            for (int hi = 0; hi < 32; hi++) {
+                int el = expand5(lo), eh = expand5(hi);
+                int amd = std::abs(((43 * eh + 21 * el + 32) >> 6) - v);
+                int nv = std::abs(((2 * hi + lo) * 22) / 8 - v);
+                int e = amd > nv ? amd : nv;
+                if (e < best5) { best5 = e; t->match5[v] = uint16_t(hi | lo << 8); }
+            }
+        for (int lo = 0; lo < 64; lo++)
+            for (int hi = 0; hi < 64; hi++) {
+                int el = expand6(lo), eh = expand6(hi), d = eh - el;
+                int amd = std::abs(((43 * eh + 21 * el + 32) >> 6) - v);
+                int nv = std::abs((256 * el + d / 4 + 128 + d * 80) / 256 - v);
+                int e = amd > nv ? amd : nv;
+                if (e < best6) { best6 = e; t->match6[v] = uint16_t(hi | lo << 8); }
+            }
+    }
+
+    // SingleColorLookup.cpp:34-89 (colour tables): 100*|lerp13 - v| + 3*|hi-lo|.
+    for (int v = 0; v < 256; v++) {
+        int b5 = 25600, b6 = 25600;
+        for (int lo = 0; lo < 32; lo++)
+            for (int hi = 0; hi < 32; hi++) {
+                int e = std::abs((expand5(hi) * 2 + expand5(lo)) / 3 - v) * 100 + std::abs(hi - lo) * 3;
+                if (e < b5) { b5 = e; t->omatch5[v] = uint16_t(hi | lo << 8); }
+            }
+        for (int lo = 0; lo < 64; lo++)
+            for (int hi = 0; hi < 64; hi++) {
+                int e = std::abs((expand6(hi) * 2 + expand6(lo)) / 3 - v) * 100 + std::abs(hi - lo) * 3;
+                if (e < b6) { b6 = e; t->omatch6[v] = uint16_t(hi | lo << 8); }
+            }
+    }
+
+    // icbc.h:1516-1526: midpoints between consecutive bit-expanded levels,
+    // printed to six decimals in the reference table.
+    char buf[64];
+    for (int k = 0; k < 31; k++) {
+        float m = (float(expand5(k)) / 255.0f + float(expand5(k + 1)) / 255.0f) * 0.5f;
+        std::snprintf(buf, sizeof buf, "%f", double(m));
+        t->mid5[k] = std::strtof(buf, nullptr);
+    }
+    t->mid5[31] = FLT_MAX;
+    for (int k = 0; k < 63; k++) {
+        float m = (float(expand6(k)) / 255.0f + float(expand6(k + 1)) / 255.0f) * 0.5f;
+        std::snprintf(buf, sizeof buf, "%f", double(m));
+        t->mid6[k] = std::strtof(buf, nullptr);
+    }
+    t->mid6[63] = FLT_MAX;
+
+    // weightedclusterfit.cpp:495-553: (c0, c1, c2) with c0+c1+c2 <= n, c0
+    // outermost, in loop order, for each point count n.
+    i = 0;
+    for (int n = 0; n <= 16; n++) {
+        t->sq_off[n] = i;
+        for (int a = 0; a <= n; a++)
+            for (int b = 0; b <= n - a; b++)
+                for (int c = 0; c <= n - a - b; c++) t->sq[i++] = uint32_t(a) | uint32_t(b) << 8 | uint32_t(c) << 16;
+    }
+    t->sq_off[17] = i;
+}
+
+// ---------------------------------------------------------------- filters --
+
+static const float kPi = float(3.1415926535897932384626433833);
+
+// Filter.cpp:50-58 sincf.
+static float sinc(float x)
+{
+    if (std::fabs(x) < 0.0001f) return 1.0f + x * x * (-1.0f / 6.0f + x * x * 1.0f / 120.0f);
+    return std::sin(x) / x;
+}
+
+// Filter.cpp:63-82 bessel0 (series until the term drops below 1e-6 of the sum).
+static float bessel_i0(float x)
+{
+    float half = 0.5f * x, sum = 1.0f, term = 1.0f, sq = 1.0f;
+    int k = 0;
+    while (sq > sum * 1e-6f) {
+        ++k;
+        term = term * (half / k);
+        sq = term * term;
+        sum = sum + sq;
+    }
+    return sum;
+}
+
+struct FilterFn {
+    int type;
+    float width, alpha, stretch;
+    // Filter.cpp:157-175, 257-263.
+    float operator()(float x) const
+    {
+        if (type == NTX_FILTER_BOX) return std::fabs(x) <= width ? 1.0f : 0.0f;
+        if (type == NTX_FILTER_TRIANGLE) {
+            x = std::fabs(x);
+            return x < width ? width - x : 0.0f;
+        }
+        const float s = sinc(kPi * x * stretch);
+        const float t = x / width;
+        if ((1 - t * t) >= 0) return s * bessel_i0(alpha * std::sqrt(1 - t * t)) / bessel_i0(alpha);
+        return 0.0f;
+    }
+};
+
+int polyphase_window(float width, int src_len, int dst_len)
+{
+    const float scale = float(dst_len) / float(src_len);
+    const float iscale = 1.0f / scale;
+    return int(std::ceil(width * iscale * 2)) + 1;
+}
+
+// Filter.cpp:563-608 PolyphaseKernel: per output sample, box-integrate the
+// filter over each source texel with 32 point samples (double accumulator,
+// Filter.cpp:115-131), then normalise the row.
+void build_polyphase(int filter, float width, const float *params, int src_len, int dst_len, Kernel1D *k)
+{
+    FilterFn f{ filter, width, 4.0f, 1.0f };
+    if (filter == NTX_FILTER_KAISER && params) { f.alpha = params[0]; f.stretch = params[1]; }
+    float scale = float(dst_len) / float(src_len);
+    const float iscale = 1.0f / scale;
+    int samples = 32;
+    if (scale > 1) { samples = 1; scale = 1; }
+    k->length = dst_len;
+    k->iscale = iscale;
+    k->width = width * iscale;
+    k->window = int(std::ceil(k->width * 2)) + 1;
+    k->weights = new float[size_t(k->window) * dst_len];
+    const float inv_samples = 1.0f / float(samples);
+    for (int i = 0; i < dst_len; i++) {
+        const float center = (0.5f + i) * iscale;
+        const int left = int(std::floor(center - k->width));
+        float total = 0.0f;
+        float *row = k->weights + size_t(i) * k->window;
+        for (int j = 0; j < k->window; j++) {
+            const float x = left + j - center;
+            double acc = 0;
+            for (int s = 0; s < samples; s++) acc += f((x + (float(s) + 0.5f) * inv_samples) * scale);
+            row[j] = float(acc * inv_samples);
+            total += row[j];
+        }
+        for (int j = 0; j < k->window; j++) row[j] /= total;
+    }
+}
+
+void free_polyphase(Kernel1D *k)
+{
+    delete[] k->weights;
+    k->weights = nullptr;
+}
+
+}  // namespace ntx

@@ -1,0 +1,244 @@
+"""ctypes binding of the gfx950 backend's C ABI (include/ntx_hip.h).
+
+This is how the Python tests and bench.py reach the HIP kernels: every call
+goes through lib/libntx_hip.so.  There is no fallback: if the library is not
+built, or no HIP device is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libntx_hip.so")
+NVTT_LIB_PATH = os.path.join(LIB_DIR, "libnvtt_amd.so")
+
+# nvtt::Format / Quality / AlphaMode / MipmapFilter / WrapMode values (nvtt.h).
+FORMAT_BC1, FORMAT_BC3, FORMAT_BC4, FORMAT_BC5 = 1, 4, 6, 7
+QUALITY_FASTEST, QUALITY_NORMAL, QUALITY_PRODUCTION, QUALITY_HIGHEST = 0, 1, 2, 3
+ALPHA_NONE, ALPHA_TRANSPARENCY = 0, 1
+FILTER_BOX, FILTER_TRIANGLE, FILTER_KAISER = 0, 1, 2
+WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = 0, 1, 2
+
+# Surface.cpp:1121-1141 getDefaultFilterWidthAndParams.
+DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_KAISER: (3.0, (4.0, 1.0))}
+
+# Every symbol include/ntx_hip.h declares.
+EXPORTS = (
+    "ntx_hip_is_hardware_present", "ntx_hip_device_count", "ntx_hip_status_string",
+    "ntx_hip_create", "ntx_hip_destroy", "ntx_hip_last_error", "ntx_hip_stream",
+    "ntx_hip_malloc", "ntx_hip_free", "ntx_hip_upload", "ntx_hip_download", "ntx_hip_synchronize",
+    "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_compressed_size",
+    "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
+    "ntx_hip_downsample", "ntx_hip_downsample_device",
+    "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device",
+)
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_S = ctypes.c_size_t
+_lib = None
+
+
+class NtxError(RuntimeError):
+    def __init__(self, status: int, what: str, detail: str = ""):
+        self.status = status
+        super().__init__(f"{what} failed: status {status}" + (f" ({detail})" if detail else ""))
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libntx_hip.so and declare its signatures; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    sig = {
+        "ntx_hip_is_hardware_present": (_I, []),
+        "ntx_hip_device_count": (_I, [ctypes.POINTER(_I)]),
+        "ntx_hip_status_string": (ctypes.c_char_p, [_I]),
+        "ntx_hip_create": (_I, [_I, ctypes.POINTER(_P)]),
+        "ntx_hip_destroy": (_I, [_P]),
+        "ntx_hip_last_error": (ctypes.c_char_p, [_P]),
+        "ntx_hip_stream": (_P, [_P]),
+        "ntx_hip_malloc": (_I, [_P, _S, ctypes.POINTER(_P)]),
+        "ntx_hip_free": (_I, [_P, _P]),
+        "ntx_hip_upload": (_I, [_P, _P, _P, _S]),
+        "ntx_hip_download": (_I, [_P, _P, _P, _S]),
+        "ntx_hip_synchronize": (_I, [_P]),
+        "ntx_hip_timer_begin": (_I, [_P]),
+        "ntx_hip_timer_end": (_I, [_P, ctypes.POINTER(_F)]),
+        "ntx_hip_compressed_size": (_S, [_I, _I, _I]),
+        "ntx_hip_compress": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P, _S]),
+        "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
+        "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
+        "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _P, _I, _I, _P]),
+        "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _P, _I, _I, _P]),
+        "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
+        "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def compressed_size(fmt: int, w: int, h: int) -> int:
+    bs = 8 if fmt in (FORMAT_BC1, FORMAT_BC4) else 16
+    return ((w + 3) // 4) * ((h + 3) // 4) * bs
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _f32c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _P()
+        ctx._check(ctx.lib.ntx_hip_malloc(ctx.handle, self.nbytes, ctypes.byref(p)), "ntx_hip_malloc")
+        self.ptr = p.value
+
+    def upload(self, host: np.ndarray) -> "DeviceBuffer":
+        host = np.ascontiguousarray(host)
+        assert host.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.ntx_hip_upload(self.ctx.handle, self.ptr, _ptr(host), host.nbytes), "upload")
+        return self
+
+    def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        self.ctx._check(self.ctx.lib.ntx_hip_download(self.ctx.handle, _ptr(out), self.ptr, out.nbytes), "download")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.lib.ntx_hip_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """One HIP stream + device tables (ntx_hip_create)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _P()
+        st = self.lib.ntx_hip_create(device, ctypes.byref(h))
+        if st != 0:
+            raise NtxError(st, "ntx_hip_create", self.lib.ntx_hip_status_string(st).decode())
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.ntx_hip_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            detail = self.lib.ntx_hip_last_error(self.handle).decode() if self.handle else ""
+            raise NtxError(st, what, detail or self.lib.ntx_hip_status_string(st).decode())
+
+    # -- host-pointer entry points ------------------------------------------
+    def compress(self, fmt: int, rgba_planar: np.ndarray, w: int, h: int, quality: int = QUALITY_NORMAL,
+                 alpha_mode: int = ALPHA_NONE, color_weights: Sequence[float] = (1, 1, 1, 1)) -> np.ndarray:
+        src = _f32c(rgba_planar).reshape(-1)
+        assert src.size == 4 * w * h
+        out = np.zeros(compressed_size(fmt, w, h), np.uint8)
+        cw = _f32c(color_weights)
+        st = self.lib.ntx_hip_compress(self.handle, fmt, quality, alpha_mode, _ptr(cw), _ptr(src), w, h,
+                                       _ptr(out), out.nbytes)
+        self._check(st, "ntx_hip_compress")
+        return out
+
+    def downsample(self, filt: int, rgba_planar: np.ndarray, W: int, H: int, width: Optional[float] = None,
+                   params: Optional[Sequence[float]] = None, wrap: int = WRAP_MIRROR) -> np.ndarray:
+        dw, dp = DEFAULT_FILTER[filt]
+        width = dw if width is None else width
+        params = dp if params is None else params
+        src = _f32c(rgba_planar).reshape(-1)
+        w, h = max(1, W // 2), max(1, H // 2)
+        out = np.empty(4 * w * h, np.float32)
+        pp = _f32c(params) if params is not None else None
+        st = self.lib.ntx_hip_downsample(self.handle, filt, width, _ptr(pp) if pp is not None else None, wrap,
+                                         _ptr(src), W, H, _ptr(out))
+        self._check(st, "ntx_hip_downsample")
+        return out
+
+    # -- device-pointer entry points -----------------------------------------
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def compress_device(self, fmt: int, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
+                        quality: int = QUALITY_NORMAL, alpha_mode: int = ALPHA_NONE,
+                        color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        cw = _f32c(color_weights)
+        st = self.lib.ntx_hip_compress_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), d_rgba.ptr, w, h,
+                                              d_out.ptr)
+        self._check(st, "ntx_hip_compress_device")
+
+    def compress_bc1_stats_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
+                                  d_nm: DeviceBuffer, alpha_mode: int = ALPHA_NONE,
+                                  color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        cw = _f32c(color_weights)
+        st = self.lib.ntx_hip_compress_bc1_stats_device(self.handle, alpha_mode, _ptr(cw), d_rgba.ptr, w, h,
+                                                        d_out.ptr, d_nm.ptr)
+        self._check(st, "ntx_hip_compress_bc1_stats_device")
+
+    def downsample_device(self, filt: int, d_src: DeviceBuffer, W: int, H: int, d_dst: DeviceBuffer,
+                          width: Optional[float] = None, params: Optional[Sequence[float]] = None,
+                          wrap: int = WRAP_MIRROR) -> None:
+        dw, dp = DEFAULT_FILTER[filt]
+        width = dw if width is None else width
+        params = dp if params is None else params
+        pp = _f32c(params) if params is not None else None
+        st = self.lib.ntx_hip_downsample_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
+                                                wrap, d_src.ptr, W, H, d_dst.ptr)
+        self._check(st, "ntx_hip_downsample_device")
+
+    def srgb_to_linear_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
+        self._check(self.lib.ntx_hip_srgb_to_linear_device(self.handle, d_rgba.ptr, w, h), "srgb_to_linear")
+
+    def unpack_bgra8_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
+        self._check(self.lib.ntx_hip_unpack_bgra8_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack")
+
+    def synchronize(self) -> None:
+        self._check(self.lib.ntx_hip_synchronize(self.handle), "synchronize")
+
+    def timer_begin(self) -> None:
+        self._check(self.lib.ntx_hip_timer_begin(self.handle), "timer_begin")
+
+    def timer_end(self) -> float:
+        ms = _F()
+        self._check(self.lib.ntx_hip_timer_end(self.handle, ctypes.byref(ms)), "timer_end")
+        return float(ms.value)
+
+
+def mip_sizes(w: int, h: int):
+    """Extents of the full chain down to 1x1 (Surface.cpp:155-207 countMipmaps)."""
+    out = [(w, h)]
+    while w != 1 or h != 1:
+        w, h = max(1, w // 2), max(1, h // 2)
+        out.append((w, h))
+    return out

@@ -1,0 +1,95 @@
+"""Writes the committed golden vectors of tests/golden/ from the reference's
+own encoders, compiled from the reference sources by oracle/Makefile into
+oracle/_ref/libntxref.so (icbc.h with ICBC_SIMD=2, nvsquish with
+SQUISH_USE_SSE=0).  Run in the container that holds the reference tree:
+
+    make -C oracle && python tests/golden/make_golden.py
+
+bc1_blocks.npz        inputs (colors, weights, colour weights) and icbc's 8-byte
+                      output + returned error for adversarial BC1 blocks.
+bc3_color_blocks.npz  8-bit BGRA blocks and squish's 8-byte colour block.
+chain_c1_64.npz       the 64x64 noise+rectangles texture (seed 7), and icbc's
+                      BC1 bytes for every level of its Box chain; the levels
+                      are built by the oracle's fastDownSample (FloatImage is
+                      not buildable from the reference without its cmake step).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "nvidia-texture-tools_amd"))
+
+import blocks  # noqa: E402
+import oracle_lib  # noqa: E402
+from ntx_amd import synth  # noqa: E402
+
+META = "reference castano/nvidia-texture-tools 2.1.2; g++ 11.4 -O2 -std=c++11 -msse4.1 -ffp-contract=off -DICBC_SIMD=2 -DSQUISH_USE_SSE=0"
+
+
+def gather_bc1(planar, w, h):
+    """FloatColorCompressorTask (BlockCompressor.cpp:116-166), AlphaMode_None."""
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    p = planar.reshape(4, h, w)
+    cols = np.zeros((bw * bh, 16, 4), np.float32)
+    wts = np.zeros((bw * bh, 16), np.float32)
+    for by in range(bh):
+        for bx in range(bw):
+            i = by * bw + bx
+            for y in range(4):
+                for x in range(4):
+                    X, Y = 4 * bx + x, 4 * by + y
+                    if X < w and Y < h:
+                        cols[i, 4 * y + x] = p[:, Y, X]
+                        wts[i, 4 * y + x] = 1.0
+    return cols, wts
+
+
+def main():
+    ref = oracle_lib.Reference.load()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libntxref.so missing: build it with `make -C oracle` next to the reference tree")
+    orc = oracle_lib.Oracle()
+
+    cols, wts, kinds = blocks.bc1_set(seed=2024, count=2048)
+    cws = np.ones((len(cols), 3), np.float32)
+    cws[1::7] = np.array([0.5, 1.0, 0.25], np.float32)
+    cws[3::11] = np.array([0.2126, 0.7152, 0.0722], np.float32)
+    out = np.zeros((len(cols), 8), np.uint8)
+    err = np.zeros(len(cols), np.float32)
+    for i in range(len(cols)):
+        out[i], err[i] = ref.bc1_block(cols[i], wts[i], cws[i])
+    np.savez_compressed(os.path.join(HERE, "bc1_blocks.npz"), colors=cols, weights=wts, color_weights=cws,
+                        kinds=kinds, expected=out, error=err, meta=META)
+
+    px = blocks.bgra_set(seed=2025, count=2048)
+    wba = (np.arange(len(px)) % 3 == 2).astype(np.int32)
+    out3 = np.zeros((len(px), 8), np.uint8)
+    for i in range(len(px)):
+        single = np.all(px[i, :, :3] == px[i, 0, :3])
+        out3[i] = orc.bc3_color_block(px[i], wba[i]) if single else ref.bc3_color_squish(px[i], wba[i])
+    np.savez_compressed(os.path.join(HERE, "bc3_color_blocks.npz"), bgra=px, weight_by_alpha=wba, expected=out3,
+                        meta=META + "; single-colour blocks use the oracle's OptimalCompress path")
+
+    img = synth.noise_rect_rgba(64, 64, seed=7)
+    planar = synth.to_planar(img).reshape(-1)
+    w = h = 64
+    levels = {}
+    while True:
+        c, wt = gather_bc1(planar, w, h)
+        b = np.stack([ref.bc1_block(c[i], wt[i])[0] for i in range(len(c))])
+        levels[f"bc1_{w}x{h}"] = b
+        if w == 1 and h == 1:
+            break
+        planar = orc.fast_downsample(planar, w, h)
+        w, h = max(1, w // 2), max(1, h // 2)
+    np.savez_compressed(os.path.join(HERE, "chain_c1_64.npz"), image=img, meta=META, **levels)
+    for f in ("bc1_blocks.npz", "bc3_color_blocks.npz", "chain_c1_64.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+"""ctypes access to the oracle (oracle/liboracle.so) and to the reference's
+own icbc/squish build (oracle/_ref/libntxref.so).  Test infrastructure."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libntxref.so")
+_P = ctypes.c_void_p
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built; run `make -C oracle`")
+        L = self.lib = ctypes.CDLL(path)
+        L.ntxo_bc1_block.restype = ctypes.c_float
+        L.ntxo_output_size.restype = ctypes.c_size_t
+        L.ntxo_output_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ntxo_kernel_window.restype = ctypes.c_int
+        L.ntxo_kernel_window.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, _P]
+        L.ntxo_polyphase_kernel.argtypes = [ctypes.c_int, ctypes.c_float, _P, ctypes.c_int, ctypes.c_int, _P]
+        L.ntxo_downsample.argtypes = [ctypes.c_int, ctypes.c_float, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
+        L.ntxo_build_next_mipmap.argtypes = L.ntxo_downsample.argtypes
+        L.ntxo_srgb_to_linear.argtypes = [ctypes.c_size_t, _P]
+        L.ntxo_init()
+
+    # ---- tables
+    def cluster_table4(self):
+        t = np.zeros(976 * 3, np.uint8); tot = np.zeros(16, np.int32)
+        self.lib.ntxo_cluster_table4(_p(t), _p(tot)); return t.reshape(-1, 3), tot
+
+    def cluster_table3(self):
+        t = np.zeros(160 * 2, np.uint8); tot = np.zeros(16, np.int32)
+        self.lib.ntxo_cluster_table3(_p(t), _p(tot)); return t.reshape(-1, 2), tot
+
+    def match_tables(self):
+        a = np.zeros(512, np.uint8); b = np.zeros(512, np.uint8)
+        self.lib.ntxo_icbc_match_tables(_p(a), _p(b)); return a.reshape(256, 2), b.reshape(256, 2)
+
+    def omatch_tables(self):
+        a = np.zeros(512, np.uint8); b = np.zeros(512, np.uint8)
+        self.lib.ntxo_omatch_tables(_p(a), _p(b)); return a.reshape(256, 2), b.reshape(256, 2)
+
+    def midpoints(self):
+        a = np.zeros(32, np.float32); b = np.zeros(64, np.float32)
+        self.lib.ntxo_midpoints(_p(a), _p(b)); return a, b
+
+    # ---- blocks
+    def bc1_block(self, colors, weights, cw=(1, 1, 1)):
+        c = f32(colors).reshape(16, 4); w = f32(weights).reshape(16); cwv = f32(cw)
+        out = np.zeros(8, np.uint8); n = ctypes.c_int(); m = ctypes.c_int()
+        e = self.lib.ntxo_bc1_block(_p(c), _p(w), _p(cwv), _p(out), ctypes.byref(n), ctypes.byref(m))
+        return out, float(e), n.value, m.value
+
+    def bc1_decode(self, blk):
+        blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
+        self.lib.ntxo_bc1_decode(_p(blk), _p(out)); return out.reshape(16, 4)
+
+    def bc3_color_block(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc3_color_block(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out
+
+    def bc4_channel(self, v):
+        v = np.ascontiguousarray(v, np.uint8).reshape(16); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc4_channel(_p(v), _p(out)); return out
+
+    def bc4_decode(self, blk):
+        blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(16, np.uint8)
+        self.lib.ntxo_bc4_decode(_p(blk), _p(out)); return out
+
+    # ---- images
+    def compress(self, fmt, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1), threads=8):
+        src = f32(planar).reshape(-1); assert src.size == 4 * w * h
+        out = np.zeros(self.lib.ntxo_output_size(fmt, w, h), np.uint8); cwv = f32(cw)
+        self.lib.ntxo_compress_image(int(fmt), int(alpha_mode), _p(cwv), int(w), int(h), _p(src), _p(out), int(threads))
+        return out
+
+    def compress_bc1_counts(self, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1), threads=8):
+        src = f32(planar).reshape(-1)
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        out = np.zeros(nb * 8, np.uint8); nm = np.zeros(2 * nb, np.int16); cwv = f32(cw)
+        self.lib.ntxo_compress_bc1_counts(int(alpha_mode), _p(cwv), int(w), int(h), _p(src), _p(out), _p(nm), int(threads))
+        return out, nm.reshape(-1, 2)
+
+    def fast_downsample(self, planar, W, H):
+        src = f32(planar).reshape(-1); w, h = max(1, W // 2), max(1, H // 2)
+        out = np.zeros(4 * w * h, np.float32)
+        self.lib.ntxo_fast_downsample(int(W), int(H), _p(src), _p(out)); return out
+
+    def polyphase(self, filt, width, params, src_len, dst_len):
+        ws = self.lib.ntxo_kernel_window(int(filt), float(width), int(src_len), int(dst_len), None)
+        out = np.zeros(ws * dst_len, np.float32)
+        pp = f32(params) if params is not None else None
+        self.lib.ntxo_polyphase_kernel(int(filt), float(width), _p(pp) if pp is not None else None, int(src_len), int(dst_len), _p(out))
+        return out.reshape(dst_len, ws)
+
+    def downsample(self, filt, width, params, wrap, planar, W, H):
+        src = f32(planar).reshape(-1); w, h = max(1, W // 2), max(1, H // 2)
+        out = np.zeros(4 * w * h, np.float32)
+        pp = f32(params) if params is not None else None
+        self.lib.ntxo_downsample(int(filt), float(width), _p(pp) if pp is not None else None, int(wrap), int(W), int(H), _p(src), _p(out))
+        return out
+
+    def build_next_mipmap(self, filt, width, params, wrap, planar, W, H):
+        src = f32(planar).reshape(-1); w, h = max(1, W // 2), max(1, H // 2)
+        out = np.zeros(4 * w * h, np.float32)
+        pp = f32(params) if params is not None else None
+        self.lib.ntxo_build_next_mipmap(int(filt), float(width), _p(pp) if pp is not None else None, int(wrap), int(W), int(H), _p(src), _p(out))
+        return out
+
+    def srgb_to_linear(self, x):
+        a = f32(x).copy(); self.lib.ntxo_srgb_to_linear(a.size, _p(a)); return a
+
+
+class Reference:
+    """The reference's own icbc.h (ICBC_SIMD=2) and nvsquish (scalar)."""
+
+    @staticmethod
+    def load(path=REF_SO):
+        if not os.path.exists(path):
+            return None
+        return Reference(path)
+
+    def __init__(self, path):
+        L = self.lib = ctypes.CDLL(path)
+        L.ntxref_bc1_block.restype = ctypes.c_float
+        L.ntxref_init()
+
+    def bc1_block(self, colors, weights, cw=(1, 1, 1)):
+        c = f32(colors).reshape(16, 4); w = f32(weights).reshape(16); cwv = f32(cw)
+        out = np.zeros(8, np.uint8)
+        e = self.lib.ntxref_bc1_block(_p(c), _p(w), _p(cwv), _p(out))
+        return out, float(e)
+
+    def bc1_decode(self, blk):
+        blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
+        self.lib.ntxref_bc1_decode(_p(blk), _p(out)); return out.reshape(16, 4)
+
+    def bc3_color_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
+        self.lib.ntxref_bc3_color_squish(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out

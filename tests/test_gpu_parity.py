@@ -1,0 +1,204 @@
+"""GPU parity: the gfx950 kernels, called through the C ABI, against the
+golden vectors written by the reference's own encoders and against the
+oracle on the same seeded inputs.  The bar is bit-exact output."""
+import os
+
+import numpy as np
+import pytest
+
+import blocks
+import ntx_amd as N
+from ntx_amd import synth
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+ODD_SIZES = [(1, 1), (2, 3), (3, 5), (5, 4), (7, 1), (1, 6), (37, 23), (130, 66)]
+
+
+def _blocks_as_image(cols, wts):
+    """Lay blocks side by side in a 4 x 4n image; weights go to alpha so that
+    AlphaMode_Transparency reproduces them (weight = saturate(a))."""
+    n = len(cols)
+    img = np.zeros((4, 4, 4 * n), np.float32)
+    for i in range(n):
+        for t in range(16):
+            y, x = divmod(t, 4)
+            img[:3, y, 4 * i + x] = cols[i, t, :3]
+            img[3, y, 4 * i + x] = wts[i, t]
+    return img, 4 * n, 4
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def test_bc1_golden_blocks(gpu):
+    g = np.load(os.path.join(GOLD, "bc1_blocks.npz"))
+    cws = g["color_weights"]
+    keys = {tuple(c) for c in cws}
+    bad = 0
+    for key in keys:
+        sel = np.nonzero(np.all(cws == np.array(key, np.float32), axis=1))[0]
+        img, w, h = _blocks_as_image(g["colors"][sel], g["weights"][sel])
+        out = gpu.compress(N.FORMAT_BC1, img, w, h, alpha_mode=N.ALPHA_TRANSPARENCY,
+                           color_weights=(key[0], key[1], key[2], 1.0)).reshape(-1, 8)
+        bad += int(np.sum(np.any(out != g["expected"][sel], axis=1)))
+    assert bad == 0, f"{bad} of {len(cws)} golden BC1 blocks differ"
+
+
+def test_bc1_box_chain_golden(gpu):
+    g = np.load(os.path.join(GOLD, "chain_c1_64.npz"))
+    planar = synth.to_planar(g["image"]).reshape(-1)
+    w = h = 64
+    while True:
+        out = gpu.compress(N.FORMAT_BC1, planar, w, h).reshape(-1, 8)
+        assert np.array_equal(out, g[f"bc1_{w}x{h}"]), (w, h)
+        if w == 1 and h == 1:
+            break
+        planar = gpu.downsample(N.FILTER_BOX, planar, w, h)
+        w, h = max(1, w // 2), max(1, h // 2)
+
+
+@pytest.mark.parametrize("seed,size", [(1, (256, 256)), (2, (128, 64))] + [(3, s) for s in ODD_SIZES])
+def test_bc1_chain_matches_oracle(gpu, oracle, seed, size):
+    w, h = size
+    planar = synth.to_planar(synth.noise_rect_rgba(w, h, seed=seed)).reshape(-1)
+    while True:
+        a = gpu.compress(N.FORMAT_BC1, planar, w, h)
+        b = oracle.compress(N.FORMAT_BC1, planar, w, h)
+        assert np.array_equal(a, b), (w, h, int(np.sum(np.any(a.reshape(-1, 8) != b.reshape(-1, 8), axis=1))))
+        if w == 1 and h == 1:
+            break
+        nxt = gpu.downsample(N.FILTER_BOX, planar, w, h)
+        assert np.array_equal(_bits(nxt), _bits(oracle.build_next_mipmap(N.FILTER_BOX, 0.5, None, N.WRAP_MIRROR, planar, w, h)))
+        planar = nxt
+        w, h = max(1, w // 2), max(1, h // 2)
+
+
+@pytest.mark.parametrize("kind", range(len(blocks.KINDS)))
+def test_bc1_adversarial_blocks(gpu, oracle, kind):
+    rng = np.random.default_rng(100 + kind)
+    cols = np.zeros((512, 16, 4), np.float32)
+    wts = np.zeros((512, 16), np.float32)
+    for i in range(512):
+        cols[i], wts[i] = blocks.bc1_block(rng, blocks.KINDS[kind])
+    img, w, h = _blocks_as_image(cols, wts)
+    for cw in ((1, 1, 1, 1), (0.5, 1.0, 0.25, 1.0)):
+        a = gpu.compress(N.FORMAT_BC1, img, w, h, alpha_mode=N.ALPHA_TRANSPARENCY, color_weights=cw)
+        b = oracle.compress(N.FORMAT_BC1, img, w, h, alpha_mode=N.ALPHA_TRANSPARENCY, cw=cw)
+        assert np.array_equal(a, b), (blocks.KINDS[kind], cw, int(np.sum(np.any(a.reshape(-1, 8) != b.reshape(-1, 8), axis=1))))
+
+
+def test_bc3_color_golden_blocks(gpu):
+    g = np.load(os.path.join(GOLD, "bc3_color_blocks.npz"))
+    px, wba, exp = g["bgra"], g["weight_by_alpha"], g["expected"]
+    for mode in (0, 1):
+        sel = np.nonzero(wba == mode)[0]
+        n = len(sel)
+        img = np.zeros((4, 4, 4 * n), np.float32)
+        for j, i in enumerate(sel):
+            for t in range(16):
+                y, x = divmod(t, 4)
+                b, gg, r, a = px[i, t].astype(np.float32)
+                # (v + 0.5) / 255 truncates back to v in ColorBlock::init
+                img[:, y, 4 * j + x] = (np.array([r, gg, b, a], np.float32) + np.float32(0.5)) / np.float32(255)
+        out = gpu.compress(N.FORMAT_BC3, img, 4 * n, 4,
+                           alpha_mode=N.ALPHA_TRANSPARENCY if mode else N.ALPHA_NONE).reshape(-1, 16)
+        assert np.array_equal(out[:, 8:], exp[sel]), int(np.sum(np.any(out[:, 8:] != exp[sel], axis=1)))
+
+
+@pytest.mark.parametrize("fmt", [N.FORMAT_BC3, N.FORMAT_BC4, N.FORMAT_BC5])
+@pytest.mark.parametrize("size", [(256, 256), (64, 128)] + ODD_SIZES)
+def test_bc345_images_match_oracle(gpu, oracle, fmt, size):
+    w, h = size
+    if fmt == N.FORMAT_BC3:
+        img = synth.noise_rect_rgba(w, h, seed=11, transparent=True)
+    elif fmt == N.FORMAT_BC5:
+        img = synth.normal_map_rgba(w, h, seed=12)
+    else:
+        img = synth.single_channel_rgba(w, h, seed=13)
+    planar = synth.to_planar(img).reshape(-1)
+    for alpha_mode in ((N.ALPHA_NONE, N.ALPHA_TRANSPARENCY) if fmt == N.FORMAT_BC3 else (N.ALPHA_NONE,)):
+        a = gpu.compress(fmt, planar, w, h, alpha_mode=alpha_mode)
+        b = oracle.compress(fmt, planar, w, h, alpha_mode=alpha_mode)
+        assert np.array_equal(a, b), (fmt, size, alpha_mode)
+
+
+def test_bc345_random_float_blocks(gpu, oracle):
+    rng = np.random.default_rng(21)
+    w, h = 64, 48
+    planar = rng.random(4 * w * h, dtype=np.float32) * np.float32(1.2) - np.float32(0.1)
+    for fmt in (N.FORMAT_BC3, N.FORMAT_BC4, N.FORMAT_BC5):
+        assert np.array_equal(gpu.compress(fmt, planar, w, h), oracle.compress(fmt, planar, w, h)), fmt
+
+
+@pytest.mark.parametrize("filt", [N.FILTER_BOX, N.FILTER_TRIANGLE, N.FILTER_KAISER])
+@pytest.mark.parametrize("wrap", [N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR])
+@pytest.mark.parametrize("size", [(64, 64), (37, 23), (1, 7), (6, 1), (2, 2), (5, 4), (130, 66), (3, 3)])
+def test_mip_filters_match_oracle(gpu, oracle, filt, wrap, size):
+    W, H = size
+    rng = np.random.default_rng(W * 131 + H)
+    planar = rng.random(4 * W * H, dtype=np.float32)
+    width, params = N.DEFAULT_FILTER[filt]
+    a = gpu.downsample(filt, planar, W, H, wrap=wrap)
+    b = oracle.build_next_mipmap(filt, width, params, wrap, planar, W, H)
+    assert np.array_equal(_bits(a), _bits(b))
+    if filt == N.FILTER_BOX:
+        # the polyphase Box (width != 0.5 path is fastDownSample-free) too
+        a2 = gpu.downsample(filt, planar, W, H, width=1.0, wrap=wrap)
+        b2 = oracle.build_next_mipmap(filt, 1.0, None, wrap, planar, W, H)
+        assert np.array_equal(_bits(a2), _bits(b2))
+
+
+def test_kaiser_custom_parameters(gpu, oracle):
+    W, H = 96, 40
+    planar = np.random.default_rng(5).random(4 * W * H, dtype=np.float32)
+    for width, params in ((2.0, (3.0, 1.0)), (3.0, (4.0, 0.75)), (4.0, (6.0, 1.0))):
+        a = gpu.downsample(N.FILTER_KAISER, planar, W, H, width=width, params=params)
+        b = oracle.build_next_mipmap(N.FILTER_KAISER, width, params, N.WRAP_MIRROR, planar, W, H)
+        assert np.array_equal(_bits(a), _bits(b)), (width, params)
+
+
+def test_srgb_to_linear(gpu, oracle):
+    w, h = 256, 3
+    v = np.concatenate([np.arange(256, dtype=np.float32) / np.float32(255)] * 12)
+    d = gpu.alloc(v.nbytes).upload(v)
+    gpu.srgb_to_linear_device(d, w, h)
+    got = d.download(np.float32)
+    exp = oracle.srgb_to_linear(v[: 3 * w * h])
+    assert np.array_equal(_bits(got[: 3 * w * h]), _bits(exp))
+    assert np.array_equal(got[3 * w * h:], v[3 * w * h:])  # alpha plane untouched
+    # arbitrary floats: device powf, within 2 ulp of the host result
+    r = np.random.default_rng(3).random(4 * 4096, dtype=np.float32) * np.float32(1.2) - np.float32(0.1)
+    d2 = gpu.alloc(r.nbytes).upload(r)
+    gpu.srgb_to_linear_device(d2, 4096, 1)
+    got = d2.download(np.float32)[: 3 * 4096]
+    exp = oracle.srgb_to_linear(r[: 3 * 4096])
+    ulps = np.abs(_bits(got).astype(np.int64) - _bits(exp).astype(np.int64))
+    assert ulps.max() <= 2
+    d.free(); d2.free()
+
+
+def test_unpack_bgra8(gpu):
+    img = synth.noise_rect_rgba(40, 24, seed=9)
+    bgra = synth.to_bgra(img)
+    d_in = gpu.alloc(bgra.nbytes).upload(bgra)
+    d_out = gpu.alloc(4 * 40 * 24 * 4)
+    gpu.unpack_bgra8_device(d_in, 40, 24, d_out)
+    got = d_out.download(np.float32)
+    assert np.array_equal(got, synth.to_planar(img).reshape(-1))
+    d_in.free(); d_out.free()
+
+
+def test_unsupported_and_invalid_calls_report_status(gpu):
+    planar = np.zeros(4 * 16, np.float32)
+    with pytest.raises(N.NtxError) as e:
+        gpu.compress(N.FORMAT_BC1, planar, 4, 4, quality=N.QUALITY_FASTEST)
+    assert e.value.status == 2
+    with pytest.raises(N.NtxError) as e:
+        gpu.compress(2, planar, 4, 4)  # Format_DXT1a: not on this path
+    assert e.value.status == 2
+    with pytest.raises(N.NtxError) as e:
+        gpu.downsample(N.FILTER_BOX, planar[:4], 1, 1)
+    assert e.value.status == 1
