@@ -1,0 +1,301 @@
+#!/usr/bin/env python3
+"""Benchmark of the NVTT hot path on one MI355X.
+
+Workload (one step): a 4096x4096 RGBA8 texture from the seeded
+noise+rectangles generator, alpha thresholded into 30% transparent texels,
+already resident in HBM -> Surface::setImage(BGRA_8UB) unpack ->
+toLinearFromSrgb -> the full 13-level Kaiser(width 3, alpha 4, stretch 1)
+mip chain in FP32 -> BC3 (QuickCompress alpha + squish cluster-fit colour)
+for every level.  value = texels compressed per second over the chain.
+
+Also measured on the same chain: BC1 (icbc Level8) Mtexels/s and the Kaiser
+mip chain's algorithmic HBM GB/s.  The CPU baseline is the oracle (CPU
+restatement, threaded like nv::ParallelFor, mips single-threaded like the
+reference) on the same full workload, plus the reference's own icbc (compiled
+from the reference sources into oracle/_ref) on the BC1 chain.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from math import comb
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nvidia-texture-tools_amd"))
+
+METRIC = "BC1 Mtexels/s at PSNR within 0.05 dB of ref; mip-chain HBM GB/s"
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+# Algorithmic flops (SURVEY 8d-style accounting):
+#  BC1 icbc Level8 block: 134*N4(n) + 117*N3(m) + 3500 (n >= 2), 200 otherwise,
+#  N4(n) = C(n+3,3)-1, N3(m) = C(m+2,2)-1.
+#  BC3 colour (squish Compress4): 137 flops per candidate (the loop body of
+#  weightedclusterfit.cpp:507-541, every add/mul/div/min/max/floor/compare
+#  counted once) x C(k+3,3) candidates for k distinct colours, + 1500 set-up;
+#  single-colour blocks take the table path (0).
+SQUISH_PER_CANDIDATE = 137
+SQUISH_SETUP = 1500
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bc1_flops(nm: np.ndarray) -> float:
+    n = nm[:, 0].astype(np.int64)
+    m = nm[:, 1].astype(np.int64)
+    n4 = np.where(n >= 2, (n + 3) * (n + 2) * (n + 1) // 6 - 1, 0)
+    n3 = np.where(m >= 1, (m + 2) * (m + 1) // 2 - 1, 0)
+    f = np.where(n >= 2, 134 * n4 + 117 * n3 + 3500, 200)
+    return float(f.sum())
+
+
+def bc3_color_flops(planar: np.ndarray, w: int, h: int) -> float:
+    """Distinct 8-bit RGB colours per block (ColorBlock::init truncation and
+    edge repetition, ColourSet's minimal set) -> squish candidate count."""
+    p = planar.reshape(4, h, w)[:3]
+    u8 = (np.float32(255) * np.clip(p, np.float32(0), np.float32(1))).astype(np.uint8).astype(np.uint32)
+    rgb = u8[0] | (u8[1] << 8) | (u8[2] << 16)
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    ys = np.array([[4 * by + (i % min(4, h - 4 * by)) for i in range(4)] for by in range(bh)])
+    xs = np.array([[4 * bx + (e % min(4, w - 4 * bx)) for e in range(4)] for bx in range(bw)])
+    blk = rgb[ys[:, None, :, None], xs[None, :, None, :]].reshape(bh * bw, 16)
+    s = np.sort(blk, axis=1)
+    k = 1 + np.count_nonzero(np.diff(s, axis=1), axis=1)
+    cand = (k + 3) * (k + 2) * (k + 1) // 6
+    f = np.where(k >= 2, SQUISH_PER_CANDIDATE * cand + SQUISH_SETUP, 0)
+    return float(f.sum())
+
+
+def psnr(a: np.ndarray, b: np.ndarray) -> float:
+    mse = np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2) * 3
+    return float("inf") if mse == 0 else float(20 * np.log10(255.0 / np.sqrt(mse)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build uses a single MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    S = args.size
+    levels = N.mip_sizes(S, S)
+    texels = sum(w * h for w, h in levels)
+    kw, kp = 3.0, (4.0, 1.0)
+
+    log(f"generating {S}x{S} input (seed {args.seed})")
+    img = synth.noise_rect_rgba(S, S, seed=args.seed, transparent=True)
+    bgra = synth.to_bgra(img)
+
+    ctx = N.Context(0)
+    d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
+    d_lv = [ctx.alloc(16 * w * h) for w, h in levels]
+    d_bc3 = [ctx.alloc(N.compressed_size(N.FORMAT_BC3, w, h)) for w, h in levels]
+    d_bc1 = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels]
+
+    def mips():
+        for k in range(len(levels) - 1):
+            w, h = levels[k]
+            ctx.downsample_device(N.FILTER_KAISER, d_lv[k], w, h, d_lv[k + 1], kw, kp)
+
+    def step_bc3():
+        ctx.unpack_bgra8_device(d_bgra, S, S, d_lv[0])
+        ctx.srgb_to_linear_device(d_lv[0], S, S)
+        for k, (w, h) in enumerate(levels):
+            ctx.compress_device(N.FORMAT_BC3, d_lv[k], w, h, d_bc3[k])
+            if k + 1 < len(levels):
+                ctx.downsample_device(N.FILTER_KAISER, d_lv[k], w, h, d_lv[k + 1], kw, kp)
+
+    def step_bc1():
+        for k, (w, h) in enumerate(levels):
+            ctx.compress_device(N.FORMAT_BC1, d_lv[k], w, h, d_bc1[k])
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        ctx.synchronize()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        t0 = time.perf_counter()
+        ctx.timer_begin()
+        for _ in range(steps):
+            fn()
+        ms = ctx.timer_end()
+        wall = time.perf_counter() - t0
+        prof = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        return ms, wall, prof
+
+    log(f"BC3 pipeline: {args.warmup} warm-up + {args.steps} timed steps")
+    ms, wall, prof = timed(step_bc3, args.steps, args.warmup)
+    value = texels * args.steps / (ms / 1e3) / 1e6
+    log(f"  {ms / args.steps:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step, {value:.1f} Mtexels/s")
+
+    # Dominant kernel of the step and its algorithmic flops.
+    dom = max(prof, key=lambda k: prof[k][0])
+    lv_host = [d.download(np.float32) for d in d_lv]
+    flops_bc3 = sum(bc3_color_flops(lv_host[k], w, h) for k, (w, h) in enumerate(levels))
+    dom_ms, dom_n = prof[dom]
+    launches_per_step = dom_n / args.steps
+    if dom == "bc3_color_kernel":
+        achieved = flops_bc3 / (dom_ms / args.steps / 1e3) / 1e12
+        roofline = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "avg_launch_ms": round(dom_ms / dom_n, 5), "launches_per_step": launches_per_step,
+                    "algorithmic_flops_per_step": flops_bc3}
+    else:
+        roofline = {"kernel": dom, "bound": "unknown", "avg_launch_ms": dom_ms / dom_n}
+
+    log("mip chain (Kaiser) alone")
+    ms_mip, _, prof_mip = timed(mips, args.steps, 1)
+    mip_bytes = sum(16 * (w * h + levels[k + 1][0] * levels[k + 1][1]) for k, (w, h) in enumerate(levels[:-1]))
+    mip_gbs = mip_bytes * args.steps / (ms_mip / 1e3) / 1e9
+
+    log("BC1 over the same chain")
+    nm = [ctx.alloc(4 * ((w + 3) // 4) * ((h + 3) // 4)) for w, h in levels]
+    for k, (w, h) in enumerate(levels):
+        ctx.compress_bc1_stats_device(d_lv[k], w, h, d_bc1[k], nm[k])
+    ctx.synchronize()
+    flops_bc1 = sum(bc1_flops(x.download(np.int16).reshape(-1, 2)) for x in nm)
+    ms_bc1, _, prof_bc1 = timed(step_bc1, args.steps, 1)
+    bc1_mtex = texels * args.steps / (ms_bc1 / 1e3) / 1e6
+    bc1_kernel_ms = prof_bc1["bc1_kernel"][0] / args.steps
+    bc1_tflops = flops_bc1 / (bc1_kernel_ms / 1e3) / 1e12
+
+    gpu_bc3 = [d.download(np.uint8) for d in d_bc3]
+    gpu_bc1 = [d.download(np.uint8) for d in d_bc1]
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mtexels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{S}x{S} RGBA8 noise+rectangles, 30% transparent alpha (seed {args.seed}) -> "
+                        f"sRGB->linear FP32 -> {len(levels)}-level Kaiser(3; 4, 1) mip chain -> BC3 per level",
+            "texels_per_step": texels,
+            "levels": len(levels),
+            "input_resident_in_hbm": True,
+        },
+        "roofline": roofline,
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+        "mip_chain_gbs": round(mip_gbs, 1),
+        "mip_roofline": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(mip_gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes_per_chain": mip_bytes,
+                         "kernels_ms_per_chain": {k: round(v[0] / args.steps, 4) for k, v in prof_mip.items()}},
+        "bc1_mtexels_s": round(bc1_mtex, 2),
+        "bc1_roofline": {"kernel": "bc1_kernel", "bound": "valu", "achieved": round(bc1_tflops, 3),
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(bc1_tflops / PEAK_FP32_TFLOPS, 4),
+                         "algorithmic_flops_per_chain": flops_bc1, "kernel_ms_per_chain": round(bc1_kernel_ms, 4)},
+    }
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        orc = oracle_lib.Oracle()
+        log(f"CPU baseline: oracle, {threads} threads, full workload")
+        t0 = time.perf_counter()
+        x = synth.to_planar(img).reshape(-1)
+        x = np.concatenate([orc.srgb_to_linear(x[: 3 * S * S]), x[3 * S * S:]])
+        cpu_lv = [x]
+        cpu_bc3 = []
+        t_mip = 0.0
+        for k, (w, h) in enumerate(levels):
+            cpu_bc3.append(orc.compress(N.FORMAT_BC3, cpu_lv[k], w, h, threads=threads))
+            if k + 1 < len(levels):
+                tm = time.perf_counter()
+                cpu_lv.append(orc.build_next_mipmap(N.FILTER_KAISER, kw, kp, N.WRAP_MIRROR, cpu_lv[k], w, h))
+                t_mip += time.perf_counter() - tm
+        cpu_s = time.perf_counter() - t0
+        same = [np.array_equal(a.reshape(-1, 16), b.reshape(-1, 16)) for a, b in zip(gpu_bc3, cpu_bc3)]
+        ident = sum(int(np.sum(np.all(a.reshape(-1, 16) == b.reshape(-1, 16), axis=1))) for a, b in zip(gpu_bc3, cpu_bc3))
+        nblk = sum(len(a) // 16 for a in cpu_bc3)
+        mips_equal = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(lv_host, cpu_lv))
+        result["cpu_baseline"] = {
+            "value": round(texels / cpu_s / 1e6, 3), "unit": "Mtexels/s", "cores": threads, "kind": "port",
+            "sample": f"the full workload ({texels} texels, {len(levels)} levels): oracle BC3 with {threads} threads "
+                      f"(one block per task, atomic counter), Kaiser mips single-threaded as in the reference "
+                      f"({t_mip:.2f} s of {cpu_s:.2f} s)",
+            "seconds": round(cpu_s, 3),
+        }
+        result["parity"] = {"bc3_identical_blocks": ident / nblk, "bc3_levels_identical": all(same),
+                            "mip_levels_bit_identical": bool(mips_equal)}
+        ref = oracle_lib.Reference.load()
+        if ref is not None:
+            import ctypes
+            log(f"CPU reference: icbc (reference sources) BC1 on the chain, {threads} threads")
+            fn = ref.lib.ntxref_compress_bc1_image
+            fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+            t0 = time.perf_counter()
+            ref_bc1 = []
+            for k, (w, h) in enumerate(levels):
+                out = np.zeros(N.compressed_size(N.FORMAT_BC1, w, h), np.uint8)
+                src = np.ascontiguousarray(lv_host[k], np.float32)
+                fn(src.ctypes.data, w, h, out.ctypes.data, threads)
+                ref_bc1.append(out)
+            ref_s = time.perf_counter() - t0
+            ident1 = sum(int(np.sum(np.all(a.reshape(-1, 8) == b.reshape(-1, 8), axis=1))) for a, b in zip(gpu_bc1, ref_bc1))
+            nb1 = sum(len(a) // 8 for a in ref_bc1)
+            # PSNR of both encodings against the FP32 level 0 quantised to 8 bits
+            lvl0 = lv_host[0].reshape(4, S, S)[:3]
+            src8 = np.floor(np.clip(lvl0, 0, 1) * 255 + 0.5).astype(np.uint8)
+            dec = lambda blk: _decode_bc1(blk, S, S)
+            p_gpu, p_ref = psnr(dec(gpu_bc1[0]), src8), psnr(dec(ref_bc1[0]), src8)
+            result["cpu_reference_bc1"] = {
+                "value": round(texels / ref_s / 1e6, 3), "unit": "Mtexels/s", "cores": threads, "kind": "reference",
+                "sample": "icbc Level8 from the reference sources over the same 13 FP32 levels",
+                "bc1_identical_blocks": ident1 / nb1, "psnr_level0_gpu_db": round(p_gpu, 4),
+                "psnr_level0_ref_db": round(p_ref, 4)}
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+def _decode_bc1(blk: np.ndarray, w: int, h: int) -> np.ndarray:
+    """D3D10 BC1 decode of a w x h level into (3, h, w) uint8 (BlockDXT.cpp:43-98)."""
+    b = blk.reshape(-1, 8)
+    c0 = b[:, 0].astype(np.int32) | (b[:, 1].astype(np.int32) << 8)
+    c1 = b[:, 2].astype(np.int32) | (b[:, 3].astype(np.int32) << 8)
+    idx = b[:, 4:8].copy().view("<u4")[:, 0].astype(np.int64)
+
+    def ex(c):
+        r, g, bb = c >> 11, (c >> 5) & 63, c & 31
+        return np.stack([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (bb << 3) | (bb >> 2)], 1)
+    p0, p1 = ex(c0), ex(c1)
+    four = (c0 > c1)[:, None]
+    p2 = np.where(four, (2 * p0 + p1) // 3, (p0 + p1) // 2)
+    p3 = np.where(four, (p0 + 2 * p1) // 3, 0)
+    pal = np.stack([p0, p1, p2, p3], 1)  # (nb, 4, 3)
+    sel = (idx[:, None] >> (2 * np.arange(16))[None]) & 3  # (nb, 16)
+    px = np.take_along_axis(pal, sel[:, :, None].repeat(3, 2), 1)  # (nb, 16, 3)
+    bw, bh = (w + 3) // 4, (h + 3) // 4
+    img = px.reshape(bh, bw, 4, 4, 3).transpose(4, 0, 2, 1, 3).reshape(3, 4 * bh, 4 * bw)
+    return img[:, :h, :w].astype(np.uint8)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,27 @@ NTX_API ntx_status ntx_hip_synchronize(ntx_hip_context *ctx);
 NTX_API ntx_status ntx_hip_timer_begin(ntx_hip_context *ctx);
 NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms);
 
+/* Per-kernel timing: while enabled, every kernel the context launches is
+ * bracketed by a pair of HIP events on the context's stream.  read()
+ * waits for the stream and returns, per kernel, the summed milliseconds and
+ * launch count since the last reset.  No counterpart in the reference (its
+ * CUDA timing is commented out, CudaCompressorDXT.cpp:173-174). */
+enum {
+    NTX_KERNEL_BC1 = 0,
+    NTX_KERNEL_BC3_COLOR = 1,
+    NTX_KERNEL_CHANNEL = 2, /* BC4 / BC5 / BC3-alpha blocks */
+    NTX_KERNEL_BOX = 3,
+    NTX_KERNEL_POLY_X = 4,
+    NTX_KERNEL_POLY_Y = 5,
+    NTX_KERNEL_SRGB = 6,
+    NTX_KERNEL_UNPACK = 7,
+    NTX_KERNEL_COUNT = 8
+};
+NTX_API const char *ntx_hip_kernel_name(int kernel);
+NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
+NTX_API ntx_status ntx_hip_profile_read(ntx_hip_context *ctx, float *total_ms /* NTX_KERNEL_COUNT */,
+                                        int *launches /* NTX_KERNEL_COUNT */, int reset);
+
 /* ------------------------------------------------------------- compression --
  * Size in bytes of the block-compressed image (Surface.cpp:210-218
  * computeImageSize for block formats): ceil(w/4)*ceil(h/4)*{8,16}. */

@@ -172,12 +172,19 @@ hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_poly(const float *d_src, int W, int H, float *d_tmp, float *d_dst, const float *d_kx, const int *d_lx,
-                       int wx, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
+hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
+                         hipStream_t stream)
 {
-    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    const int w = W / 2 > 1 ? W / 2 : 1;
     hipLaunchKernelGGL(poly_x_kernel, dim3((w + 255) / 256, H, 4), dim3(256), 0, stream, d_src, W, H, d_tmp, w, d_kx, d_lx,
                        wx, wrap);
+    return hipGetLastError();
+}
+
+hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
+                         hipStream_t stream)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
     hipLaunchKernelGGL(poly_y_kernel, dim3((w + 255) / 256, h, 4), dim3(256), 0, stream, d_tmp, w, H, d_dst, h, d_ky, d_ly,
                        wy, wrap);
     return hipGetLastError();

@@ -19,8 +19,10 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
 hipError_t launch_bc3_color(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
                             void *d_out, hipStream_t stream);
 hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
-hipError_t launch_poly(const float *d_src, int W, int H, float *d_tmp, float *d_dst, const float *d_kx, const int *d_lx,
-                       int wx, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
+hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
+                         hipStream_t stream);
+hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
+                         hipStream_t stream);
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
 hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
 }  // namespace ntx
@@ -69,6 +71,11 @@ struct ntx_hip_context {
     DevBuf in, out, tmp, dst;
     std::map<PolyKey, PolyDev> poly;
     std::string err;
+    // per-kernel event timing (ntx_hip_profile_*)
+    bool profiling = false;
+    struct Mark { int kernel; hipEvent_t start, stop; };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> spare;
 };
 
 namespace {
@@ -84,6 +91,32 @@ ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what)
         hipError_t e_ = (call);                           \
         if (e_ != hipSuccess) return fail(ctx, e_, #call); \
     } while (0)
+
+hipEvent_t take_event(ntx_hip_context *ctx)
+{
+    if (!ctx->spare.empty()) {
+        hipEvent_t e = ctx->spare.back();
+        ctx->spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs one launch; brackets it with events when the context is profiling.
+template <typename F>
+hipError_t timed(ntx_hip_context *ctx, int kernel, F &&launch)
+{
+    if (!ctx->profiling) return launch();
+    hipEvent_t a = take_event(ctx), b = take_event(ctx);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    ctx->marks.push_back({ kernel, a, b });
+    return e;
+}
 
 ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes)
 {
@@ -220,6 +253,8 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
+    for (auto &m : ctx->marks) { (void)hipEventDestroy(m.start); (void)hipEventDestroy(m.stop); }
+    for (hipEvent_t e : ctx->spare) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -294,6 +329,43 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms)
     return NTX_OK;
 }
 
+NTX_API const char *ntx_hip_kernel_name(int kernel)
+{
+    static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel", "bc3_color_kernel", "channel_kernel", "box_kernel",
+                                                   "poly_x_kernel", "poly_y_kernel", "srgb_kernel", "unpack_kernel" };
+    return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
+}
+
+NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    ctx->profiling = enable != 0;
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_profile_read(ntx_hip_context *ctx, float *total_ms, int *launches, int reset)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < NTX_KERNEL_COUNT; k++) {
+        if (total_ms) total_ms[k] = 0.0f;
+        if (launches) launches[k] = 0;
+    }
+    for (auto &m : ctx->marks) {
+        float ms = 0.0f;
+        NTX_HIP(ctx, hipEventElapsedTime(&ms, m.start, m.stop));
+        if (total_ms) total_ms[m.kernel] += ms;
+        if (launches) launches[m.kernel] += 1;
+    }
+    if (reset) {
+        for (auto &m : ctx->marks) { ctx->spare.push_back(m.start); ctx->spare.push_back(m.stop); }
+        ctx->marks.clear();
+    }
+    return NTX_OK;
+}
+
 NTX_API size_t ntx_hip_compressed_size(int format, int w, int h)
 {
     if (w <= 0 || h <= 0) return 0;
@@ -313,13 +385,15 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     if (!cw) cw = ones;
     hipError_t e = hipSuccess;
     if (format == NTX_FORMAT_BC1) {
-        e = ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream);
+        e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream); });
     } else if (format == NTX_FORMAT_BC3) {
-        e = ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream);
+        e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
         if (e == hipSuccess)
-            e = ntx::launch_bc3_color(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, ctx->d_tab, d_out, ctx->stream);
+            e = timed(ctx, NTX_KERNEL_BC3_COLOR, [&] {
+                return ntx::launch_bc3_color(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, ctx->d_tab, d_out, ctx->stream);
+            });
     } else {
-        e = ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream);
+        e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
     }
     if (e != hipSuccess) return fail(ctx, e, "compress launch");
     return NTX_OK;
@@ -332,7 +406,7 @@ NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int a
     if (s) return s;
     if (w < 0 || h < 0 || !d_nm || (size_t(w) * size_t(h) > 0 && (!d_rgba || !d_out))) return NTX_ERR_INVALID_ARGUMENT;
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
-    hipError_t e = ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw ? cw : ones, ctx->d_tab, d_out, d_nm, ctx->stream);
+    hipError_t e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw ? cw : ones, ctx->d_tab, d_out, d_nm, ctx->stream); });
     if (e != hipSuccess) return fail(ctx, e, "bc1 launch");
     return NTX_OK;
 }
@@ -368,15 +442,17 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
         return NTX_ERR_INVALID_ARGUMENT;
     hipError_t e;
     if (filter == NTX_FILTER_BOX && width == 0.5f) {
-        e = ntx::launch_box(d_rgba, W, H, d_out, ctx->stream);
+        e = timed(ctx, NTX_KERNEL_BOX, [&] { return ntx::launch_box(d_rgba, W, H, d_out, ctx->stream); });
     } else {
         const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
         PolyDev *kx = nullptr, *ky = nullptr;
         if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
         if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
         if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
-        e = ntx::launch_poly(d_rgba, W, H, static_cast<float *>(ctx->tmp.ptr), d_out, kx->weights, kx->left, kx->window,
-                             ky->weights, ky->left, ky->window, wrap, ctx->stream);
+        float *tmp = static_cast<float *>(ctx->tmp.ptr);
+        e = timed(ctx, NTX_KERNEL_POLY_X, [&] { return ntx::launch_poly_x(d_rgba, W, H, tmp, kx->weights, kx->left, kx->window, wrap, ctx->stream); });
+        if (e == hipSuccess)
+            e = timed(ctx, NTX_KERNEL_POLY_Y, [&] { return ntx::launch_poly_y(tmp, W, H, d_out, ky->weights, ky->left, ky->window, wrap, ctx->stream); });
     }
     if (e != hipSuccess) return fail(ctx, e, "downsample launch");
     return NTX_OK;
@@ -406,7 +482,7 @@ NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_
     ntx_status s = bind(ctx);
     if (s) return s;
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && !d_rgba)) return NTX_ERR_INVALID_ARGUMENT;
-    hipError_t e = ntx::launch_srgb(d_rgba, size_t(w) * size_t(h) * 3, ctx->d_srgb, ctx->stream);
+    hipError_t e = timed(ctx, NTX_KERNEL_SRGB, [&] { return ntx::launch_srgb(d_rgba, size_t(w) * size_t(h) * 3, ctx->d_srgb, ctx->stream); });
     if (e != hipSuccess) return fail(ctx, e, "srgb launch");
     return NTX_OK;
 }
@@ -416,7 +492,7 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8
     ntx_status s = bind(ctx);
     if (s) return s;
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
-    hipError_t e = ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->stream);
+    hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] { return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->stream); });
     if (e != hipSuccess) return fail(ctx, e, "unpack launch");
     return NTX_OK;
 }

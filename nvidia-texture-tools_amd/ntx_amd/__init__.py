@@ -27,12 +27,17 @@ WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = 0, 1, 2
 # Surface.cpp:1121-1141 getDefaultFilterWidthAndParams.
 DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_KAISER: (3.0, (4.0, 1.0))}
 
+# ntx_hip.h kernel ids for ntx_hip_profile_read.
+KERNELS = ("bc1_kernel", "bc3_color_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
+           "srgb_kernel", "unpack_kernel")
+
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
     "ntx_hip_is_hardware_present", "ntx_hip_device_count", "ntx_hip_status_string",
     "ntx_hip_create", "ntx_hip_destroy", "ntx_hip_last_error", "ntx_hip_stream",
     "ntx_hip_malloc", "ntx_hip_free", "ntx_hip_upload", "ntx_hip_download", "ntx_hip_synchronize",
-    "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_compressed_size",
+    "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
+    "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device",
@@ -74,6 +79,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_synchronize": (_I, [_P]),
         "ntx_hip_timer_begin": (_I, [_P]),
         "ntx_hip_timer_end": (_I, [_P, ctypes.POINTER(_F)]),
+        "ntx_hip_kernel_name": (ctypes.c_char_p, [_I]),
+        "ntx_hip_profile_enable": (_I, [_P, _I]),
+        "ntx_hip_profile_read": (_I, [_P, _P, _P, _I]),
         "ntx_hip_compressed_size": (_S, [_I, _I, _I]),
         "ntx_hip_compress": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P, _S]),
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
@@ -228,6 +236,16 @@ class Context:
 
     def timer_begin(self) -> None:
         self._check(self.lib.ntx_hip_timer_begin(self.handle), "timer_begin")
+
+    def profile(self, enable: bool) -> None:
+        self._check(self.lib.ntx_hip_profile_enable(self.handle, int(bool(enable))), "profile_enable")
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{kernel name: (total ms, launches)} since the last reset."""
+        ms = np.zeros(len(KERNELS), np.float32)
+        n = np.zeros(len(KERNELS), np.int32)
+        self._check(self.lib.ntx_hip_profile_read(self.handle, _ptr(ms), _ptr(n), int(bool(reset))), "profile_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS) if n[i]}
 
     def timer_end(self) -> float:
         ms = _F()

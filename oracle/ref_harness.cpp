@@ -43,3 +43,48 @@ void ntxref_bc3_color_squish(const uint8_t *bgra, int weight_by_alpha, const flo
 }
 
 }
+
+// Whole-image BC1 with the reference encoder, for the CPU baseline.  The
+// gather restates FloatColorCompressorTask (BlockCompressor.cpp:116-166,
+// AlphaMode_None); workers pull one block at a time from an atomic counter
+// like nv::ParallelFor (ParallelFor.cpp:14-60).
+#include <atomic>
+#include <thread>
+#include <vector>
+
+extern "C" void ntxref_compress_bc1_image(const float *planar, int w, int h, uint8_t *out, int threads)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4, count = bw * bh;
+    const size_t plane = size_t(w) * size_t(h);
+    const float cw[3] = { 1.0f, 1.0f, 1.0f };
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count) return;
+            const int bx = i % bw, by = i / bw;
+            float colors[64], weights[16];
+            for (int y = 0; y < 4; y++)
+                for (int x = 0; x < 4; x++) {
+                    const int k = 4 * y + x, X = 4 * bx + x, Y = 4 * by + y;
+                    if (X < w && Y < h) {
+                        const size_t s = size_t(Y) * w + X;
+                        colors[4 * k] = planar[s];
+                        colors[4 * k + 1] = planar[s + plane];
+                        colors[4 * k + 2] = planar[s + 2 * plane];
+                        colors[4 * k + 3] = planar[s + 3 * plane];
+                        weights[k] = 1.0f;
+                    } else {
+                        colors[4 * k] = colors[4 * k + 1] = colors[4 * k + 2] = colors[4 * k + 3] = 0.0f;
+                        weights[k] = 0.0f;
+                    }
+                }
+            icbc::compress_dxt1(icbc::Quality_Level8, colors, weights, cw, true, true, out + size_t(i) * 8);
+        }
+    };
+    if (threads < 1) threads = 1;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
