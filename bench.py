@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="short run for rocprofv3 counter passes: the BC3 chain, the Kaiser chain, the BC1 chain "
+                         "(2 steps each) and a 256 MiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build uses a single MI355X)",
@@ -141,6 +144,20 @@ def main():
         prof = ctx.profile_read(reset=True)
         ctx.profile(False)
         return ms, wall, prof
+
+    if args.profile_pass:
+        for fn in (step_bc3, mips, step_bc1):
+            for _ in range(2):
+                fn()
+            ctx.synchronize()
+        nbytes = 256 << 20
+        a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        for _ in range(3):
+            ctx.lib.ntx_hip_copy_device(ctx.handle, b.ptr, a.ptr, nbytes)
+        ctx.synchronize()
+        log(f"profile pass done: BC3/Kaiser/BC1 chains x2, 3 copies of {nbytes} bytes")
+        ctx.close()
+        return
 
     log(f"BC3 pipeline: {args.warmup} warm-up + {args.steps} timed steps")
     ms, wall, prof = timed(step_bc3, args.steps, args.warmup)

@@ -76,6 +76,11 @@ NTX_API ntx_status ntx_hip_free(ntx_hip_context *ctx, void *dptr);
 NTX_API ntx_status ntx_hip_upload(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes);
 NTX_API ntx_status ntx_hip_download(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes);
 NTX_API ntx_status ntx_hip_synchronize(ntx_hip_context *ctx);
+/* Device-to-device copy and byte fill on the context's stream; both wait for
+ * completion.  Used by the Surface front end for copy-on-write
+ * (Surface.cpp:354-363 detach) and for Surface::setImage(w, h, d). */
+NTX_API ntx_status ntx_hip_copy_device(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes);
+NTX_API ntx_status ntx_hip_memset_device(ntx_hip_context *ctx, void *dst, int value, size_t bytes);
 
 /* Stream-ordered timer: records an event pair around the work enqueued
  * between begin and end; end waits and returns the elapsed milliseconds. */
@@ -96,7 +101,8 @@ enum {
     NTX_KERNEL_POLY_Y = 5,
     NTX_KERNEL_SRGB = 6,
     NTX_KERNEL_UNPACK = 7,
-    NTX_KERNEL_COUNT = 8
+    NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
+    NTX_KERNEL_COUNT = 9
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -133,16 +139,19 @@ NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int a
 
 /* ---------------------------------------------------------------- mipmaps --
  * Replaces Surface::buildNextMipmap's filtering (Surface.cpp:1344-1406):
- * Box with width 0.5 -> FloatImage::fastDownSample (FloatImage.cpp:559-737);
- * otherwise FloatImage::downSample -> resize with PolyphaseKernel weights
- * (FloatImage.cpp:740-808, 1115-1176; Filter.cpp:563-608).  params: Kaiser
- * (alpha, stretch), may be NULL.  Output is max(1,W/2) x max(1,H/2), four
- * planes.  The alpha mode only changes channel order in the reference
- * (FloatImage.cpp:877-925), so results do not depend on it. */
+ * AlphaMode_None with Box of width 0.5 -> FloatImage::fastDownSample
+ * (FloatImage.cpp:559-737); every other case -> FloatImage::downSample ->
+ * resize with PolyphaseKernel weights (FloatImage.cpp:740-808, 1115-1176;
+ * Filter.cpp:563-608).  With AlphaMode_Transparency the reference filters the
+ * alpha plane first (FloatImage.cpp:877-925), which changes no value.
+ * params: Kaiser (alpha, stretch), may be NULL.  Output is
+ * max(1,W/2) x max(1,H/2), four planes.  1x1 input is rejected (the reference
+ * makes no further mipmap: canMakeNextMipmap, Surface.cpp:155-167). */
 NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params,
-                                      int wrap, const float *rgba, int W, int H, float *out);
+                                      int wrap, int alpha_mode, const float *rgba, int W, int H, float *out);
 NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, float width, const float *params,
-                                             int wrap, const float *d_rgba, int W, int H, float *d_out);
+                                             int wrap, int alpha_mode, const float *d_rgba, int W, int H,
+                                             float *d_out);
 
 /* Surface::toLinearFromSrgb (Surface.cpp:1562-1592) on the R,G,B planes, in place. */
 NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);

@@ -124,6 +124,108 @@ __global__ __launch_bounds__(256) void poly_y_kernel(const float *__restrict__ t
     dst[(size_t(c) * size_t(h) + size_t(i)) * size_t(w) + x] = sum;
 }
 
+// Fused separable resize: one workgroup per kTX x kTY output tile.  The
+// source rows/columns the tile's taps touch are staged in LDS (with the wrap
+// applied while loading), the X taps fill an LDS temporary for every source
+// row the Y taps need, then the Y taps produce the outputs.  Each temporary
+// value is exactly the reference's applyKernelX result for that (wrapped)
+// row, and each output sums its Y taps in the reference order, so the result
+// is bit-identical to the two-pass form while the w x H temporary never
+// touches HBM.
+constexpr int kTX = 64, kTY = 16;
+constexpr int kSrcW = 160, kSrcH = 56;  // largest staged source tile
+constexpr int kMaxWin = 16;             // largest window held in registers
+
+__global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
+                                                         float *__restrict__ dst, int w, int h,
+                                                         const float *__restrict__ kx, const int *__restrict__ lx, int wx,
+                                                         const float *__restrict__ ky, const int *__restrict__ ly, int wy,
+                                                         int wrap)
+{
+    __shared__ float s_src[kSrcH * kSrcW];
+    __shared__ float s_tmp[kSrcH * kTX];
+    const int t = threadIdx.x;
+    const int ox0 = blockIdx.x * kTX, oy0 = blockIdx.y * kTY;
+    const int nx = min(kTX, w - ox0), ny = min(kTY, h - oy0);
+    const int cx0 = lx[ox0], ry0 = ly[oy0];
+    const int sw = lx[ox0 + nx - 1] - cx0 + wx;  // staged columns
+    const int sh = ly[oy0 + ny - 1] - ry0 + wy;  // staged rows
+
+    // Per-thread X-pass column and its weights, reused for every row and plane.
+    const int xi = t & (kTX - 1);
+    const bool xlive = xi < nx;
+    float kxr[kMaxWin];
+    int xoff = 0;
+    if (xlive) {
+        xoff = lx[ox0 + xi] - cx0;
+#pragma unroll
+        for (int j = 0; j < kMaxWin; j++) kxr[j] = j < wx ? kx[size_t(ox0 + xi) * wx + j] : 0.0f;
+    }
+
+    // Staging: lane (t & 63) owns source columns (t & 63) + 64k of the tile,
+    // whose wrapped image columns are fixed; wave (t >> 6) owns rows
+    // (t >> 6) + 4i.  All of a thread's loads for one plane are issued before
+    // any LDS store (out-of-tile slots load a valid in-tile address and are
+    // not stored), so their latencies overlap.
+    constexpr int kColSlots = (kSrcW + kTX - 1) / kTX;
+    constexpr int kRowSlots = (kSrcH + 3) / 4;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    int gx[kColSlots];
+    bool cvalid[kColSlots];
+#pragma unroll
+    for (int k = 0; k < kColSlots; k++) {
+        const int col = xi + kTX * k;
+        cvalid[k] = col < sw;
+        gx[k] = wrap_index(cx0 + (cvalid[k] ? col : 0), W, wrap);
+    }
+    size_t rowoff[kRowSlots];
+#pragma unroll
+    for (int i = 0; i < kRowSlots; i++) {
+        const int r = min(wave + 4 * i, sh - 1);
+        rowoff[i] = size_t(wrap_index(ry0 + r, H, wrap)) * size_t(W);
+    }
+
+    for (int c = 0; c < 4; c++) {
+        const float *plane = src + size_t(c) * size_t(W) * size_t(H);
+        float v[kRowSlots][kColSlots];
+#pragma unroll
+        for (int i = 0; i < kRowSlots; i++)
+#pragma unroll
+            for (int k = 0; k < kColSlots; k++) v[i][k] = plane[rowoff[i] + gx[k]];
+#pragma unroll
+        for (int i = 0; i < kRowSlots; i++) {
+            const int r = wave + 4 * i;
+#pragma unroll
+            for (int k = 0; k < kColSlots; k++)
+                if (r < sh && cvalid[k]) s_src[r * kSrcW + xi + kTX * k] = v[i][k];
+        }
+        __syncthreads();
+        if (xlive) {
+            for (int r = wave; r < sh; r += 4) {
+                const float *row = s_src + r * kSrcW + xoff;
+                float sum = 0.0f;
+#pragma unroll
+                for (int j = 0; j < kMaxWin; ++j)
+                    if (j < wx) sum += kxr[j] * row[j];
+                s_tmp[r * kTX + xi] = sum;
+            }
+        }
+        __syncthreads();
+        if (xlive) {
+            float *out = dst + size_t(c) * size_t(w) * size_t(h);
+            for (int o = wave; o < ny; o += 4) {
+                const int oy = oy0 + o;
+                const float *k = ky + size_t(oy) * wy;  // wave-uniform row: scalar loads
+                const float *col = s_tmp + (ly[oy] - ry0) * kTX + xi;
+                float sum = 0.0f;
+                for (int j = 0; j < wy; ++j) sum += k[j] * col[j * kTX];
+                out[size_t(oy) * size_t(w) + ox0 + xi] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Surface.cpp:1562-1569 fromSrgb.  Values that are exactly k/255 (every
 // texel of an 8-bit source) take the host-computed table entry, so they match
 // the host powf bit for bit; other values use the device powf.
@@ -187,6 +289,25 @@ hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const f
     const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
     hipLaunchKernelGGL(poly_y_kernel, dim3((w + 255) / 256, h, 4), dim3(256), 0, stream, d_tmp, w, H, d_dst, h, d_ky, d_ly,
                        wy, wrap);
+    return hipGetLastError();
+}
+
+// Whether the fused kernel's staging fits: sw/sh are the largest staged
+// extents over all tiles (computed on the host from the tap tables).
+bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy)
+{
+    return max_sw <= kSrcW && max_sh <= kSrcH && wx <= kMaxWin && wy <= kMaxWin;
+}
+
+int poly_tile_x() { return kTX; }
+int poly_tile_y() { return kTY; }
+
+hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
+                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    hipLaunchKernelGGL(poly_fused_kernel, dim3((w + kTX - 1) / kTX, (h + kTY - 1) / kTY), dim3(256), 0, stream, d_src, W, H,
+                       d_dst, w, h, d_kx, d_lx, wx, d_ky, d_ly, wy, wrap);
     return hipGetLastError();
 }
 

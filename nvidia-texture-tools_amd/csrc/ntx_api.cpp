@@ -2,6 +2,7 @@
 // dispatch to the gfx950 kernels.  No CPU compression path exists here: when
 // the device is missing or a HIP call fails the entry points return an error
 // status, which the nvtt front end reports as Error_CudaError.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -23,6 +24,11 @@ hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const f
                          hipStream_t stream);
 hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
                          hipStream_t stream);
+hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
+                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
+bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
+int poly_tile_x();
+int poly_tile_y();
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
 hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
 }  // namespace ntx
@@ -38,7 +44,19 @@ struct PolyDev {
     float *weights = nullptr;
     int *left = nullptr;
     int window = 0;
+    int span_x = 0;  // largest source span of a run of poly_tile_x() outputs
+    int span_y = 0;  // same for poly_tile_y() outputs
 };
+
+int max_span(const std::vector<int> &left, int window, int tile)
+{
+    int best = 0;
+    for (size_t i = 0; i < left.size(); i += tile) {
+        const size_t last = std::min(left.size(), i + size_t(tile)) - 1;
+        best = std::max(best, left[last] - left[i] + window);
+    }
+    return best;
+}
 
 using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
 
@@ -164,6 +182,8 @@ ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const floa
     for (int i = 0; i < dst; i++) left[i] = int(std::floor((0.5f + i) * k.iscale - k.width));
     PolyDev d;
     d.window = k.window;
+    d.span_x = max_span(left, k.window, ntx::poly_tile_x());
+    d.span_y = max_span(left, k.window, ntx::poly_tile_y());
     hipError_t e = hipMalloc(&d.weights, sizeof(float) * size_t(k.window) * dst);
     if (e == hipSuccess) e = hipMalloc(&d.left, sizeof(int) * dst);
     if (e == hipSuccess) e = hipMemcpy(d.weights, k.weights, sizeof(float) * size_t(k.window) * dst, hipMemcpyHostToDevice);
@@ -301,6 +321,26 @@ NTX_API ntx_status ntx_hip_download(ntx_hip_context *ctx, void *dst, const void 
     return NTX_OK;
 }
 
+NTX_API ntx_status ntx_hip_copy_device(ntx_hip_context *ctx, void *dst, const void *src, size_t bytes)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (bytes && (!dst || !src)) return NTX_ERR_INVALID_ARGUMENT;
+    NTX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_memset_device(ntx_hip_context *ctx, void *dst, int value, size_t bytes)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (bytes && !dst) return NTX_ERR_INVALID_ARGUMENT;
+    NTX_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return NTX_OK;
+}
+
 NTX_API ntx_status ntx_hip_synchronize(ntx_hip_context *ctx)
 {
     ntx_status s = bind(ctx);
@@ -331,8 +371,9 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms)
 
 NTX_API const char *ntx_hip_kernel_name(int kernel)
 {
-    static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel", "bc3_color_kernel", "channel_kernel", "box_kernel",
-                                                   "poly_x_kernel", "poly_y_kernel", "srgb_kernel", "unpack_kernel" };
+    static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_color_kernel", "channel_kernel",
+                                                   "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
+                                                   "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -433,7 +474,7 @@ NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int qualit
 }
 
 NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,
-                                             const float *d_rgba, int W, int H, float *d_out)
+                                             int alpha_mode, const float *d_rgba, int W, int H, float *d_out)
 {
     ntx_status s = bind(ctx);
     if (s) return s;
@@ -441,25 +482,33 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
     if (filter < NTX_FILTER_BOX || filter > NTX_FILTER_KAISER || wrap < NTX_WRAP_CLAMP || wrap > NTX_WRAP_MIRROR)
         return NTX_ERR_INVALID_ARGUMENT;
     hipError_t e;
-    if (filter == NTX_FILTER_BOX && width == 0.5f) {
+    if (filter == NTX_FILTER_BOX && width == 0.5f && alpha_mode != NTX_ALPHA_TRANSPARENCY) {
         e = timed(ctx, NTX_KERNEL_BOX, [&] { return ntx::launch_box(d_rgba, W, H, d_out, ctx->stream); });
     } else {
         const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
         PolyDev *kx = nullptr, *ky = nullptr;
         if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
         if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
-        if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
-        float *tmp = static_cast<float *>(ctx->tmp.ptr);
-        e = timed(ctx, NTX_KERNEL_POLY_X, [&] { return ntx::launch_poly_x(d_rgba, W, H, tmp, kx->weights, kx->left, kx->window, wrap, ctx->stream); });
-        if (e == hipSuccess)
-            e = timed(ctx, NTX_KERNEL_POLY_Y, [&] { return ntx::launch_poly_y(tmp, W, H, d_out, ky->weights, ky->left, ky->window, wrap, ctx->stream); });
+        if (ntx::poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
+            e = timed(ctx, NTX_KERNEL_POLY_FUSED, [&] {
+                return ntx::launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, ky->weights, ky->left,
+                                              ky->window, wrap, ctx->stream);
+            });
+        } else {
+            // Small images with wide windows: X pass into a temporary, then Y.
+            if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
+            float *tmp = static_cast<float *>(ctx->tmp.ptr);
+            e = timed(ctx, NTX_KERNEL_POLY_X, [&] { return ntx::launch_poly_x(d_rgba, W, H, tmp, kx->weights, kx->left, kx->window, wrap, ctx->stream); });
+            if (e == hipSuccess)
+                e = timed(ctx, NTX_KERNEL_POLY_Y, [&] { return ntx::launch_poly_y(tmp, W, H, d_out, ky->weights, ky->left, ky->window, wrap, ctx->stream); });
+        }
     }
     if (e != hipSuccess) return fail(ctx, e, "downsample launch");
     return NTX_OK;
 }
 
 NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,
-                                      const float *rgba, int W, int H, float *out)
+                                      int alpha_mode, const float *rgba, int W, int H, float *out)
 {
     ntx_status s = bind(ctx);
     if (s) return s;
@@ -469,8 +518,8 @@ NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float wi
     const size_t out_bytes = size_t(w) * size_t(h) * 4 * sizeof(float);
     if ((s = ensure(ctx, ctx->in, in_bytes)) || (s = ensure(ctx, ctx->dst, out_bytes))) return s;
     NTX_HIP(ctx, hipMemcpyAsync(ctx->in.ptr, rgba, in_bytes, hipMemcpyHostToDevice, ctx->stream));
-    s = ntx_hip_downsample_device(ctx, filter, width, params, wrap, static_cast<const float *>(ctx->in.ptr), W, H,
-                                  static_cast<float *>(ctx->dst.ptr));
+    s = ntx_hip_downsample_device(ctx, filter, width, params, wrap, alpha_mode, static_cast<const float *>(ctx->in.ptr), W,
+                                  H, static_cast<float *>(ctx->dst.ptr));
     if (s) return s;
     NTX_HIP(ctx, hipMemcpyAsync(out, ctx->dst.ptr, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
     NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));

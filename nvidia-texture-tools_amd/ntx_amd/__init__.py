@@ -29,13 +29,14 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 
 # ntx_hip.h kernel ids for ntx_hip_profile_read.
 KERNELS = ("bc1_kernel", "bc3_color_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
-           "srgb_kernel", "unpack_kernel")
+           "srgb_kernel", "unpack_kernel", "poly_fused_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
     "ntx_hip_is_hardware_present", "ntx_hip_device_count", "ntx_hip_status_string",
     "ntx_hip_create", "ntx_hip_destroy", "ntx_hip_last_error", "ntx_hip_stream",
     "ntx_hip_malloc", "ntx_hip_free", "ntx_hip_upload", "ntx_hip_download", "ntx_hip_synchronize",
+    "ntx_hip_copy_device", "ntx_hip_memset_device",
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
@@ -77,6 +78,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_upload": (_I, [_P, _P, _P, _S]),
         "ntx_hip_download": (_I, [_P, _P, _P, _S]),
         "ntx_hip_synchronize": (_I, [_P]),
+        "ntx_hip_copy_device": (_I, [_P, _P, _P, _S]),
+        "ntx_hip_memset_device": (_I, [_P, _P, _I, _S]),
         "ntx_hip_timer_begin": (_I, [_P]),
         "ntx_hip_timer_end": (_I, [_P, ctypes.POINTER(_F)]),
         "ntx_hip_kernel_name": (ctypes.c_char_p, [_I]),
@@ -86,8 +89,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P, _S]),
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
-        "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _P, _I, _I, _P]),
-        "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _P, _I, _I, _P]),
+        "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
+        "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
     }
@@ -181,7 +184,8 @@ class Context:
         return out
 
     def downsample(self, filt: int, rgba_planar: np.ndarray, W: int, H: int, width: Optional[float] = None,
-                   params: Optional[Sequence[float]] = None, wrap: int = WRAP_MIRROR) -> np.ndarray:
+                   params: Optional[Sequence[float]] = None, wrap: int = WRAP_MIRROR,
+                   alpha_mode: int = ALPHA_NONE) -> np.ndarray:
         dw, dp = DEFAULT_FILTER[filt]
         width = dw if width is None else width
         params = dp if params is None else params
@@ -190,7 +194,7 @@ class Context:
         out = np.empty(4 * w * h, np.float32)
         pp = _f32c(params) if params is not None else None
         st = self.lib.ntx_hip_downsample(self.handle, filt, width, _ptr(pp) if pp is not None else None, wrap,
-                                         _ptr(src), W, H, _ptr(out))
+                                         alpha_mode, _ptr(src), W, H, _ptr(out))
         self._check(st, "ntx_hip_downsample")
         return out
 
@@ -216,13 +220,13 @@ class Context:
 
     def downsample_device(self, filt: int, d_src: DeviceBuffer, W: int, H: int, d_dst: DeviceBuffer,
                           width: Optional[float] = None, params: Optional[Sequence[float]] = None,
-                          wrap: int = WRAP_MIRROR) -> None:
+                          wrap: int = WRAP_MIRROR, alpha_mode: int = ALPHA_NONE) -> None:
         dw, dp = DEFAULT_FILTER[filt]
         width = dw if width is None else width
         params = dp if params is None else params
         pp = _f32c(params) if params is not None else None
         st = self.lib.ntx_hip_downsample_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
-                                                wrap, d_src.ptr, W, H, d_dst.ptr)
+                                                wrap, alpha_mode, d_src.ptr, W, H, d_dst.ptr)
         self._check(st, "ntx_hip_downsample_device")
 
     def srgb_to_linear_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:

@@ -1,0 +1,214 @@
+// nvtt.h -- the part of NVTT 2.1.2's public API (src/nvtt/nvtt.h) that this
+// backend serves: Compressor::compress for BC1/BC3/BC4/BC5, Surface with its
+// mipmap builder, and the output/error handler plumbing.  Names, enum values,
+// argument order and error behaviour follow the reference so code written
+// against it compiles and runs unchanged for this path.
+//
+// Differences, all deliberate:
+//  - Compression and mip filtering run on the MI355X through the C ABI in
+//    include/ntx_hip.h.  There is no CPU compressor: without a device, or for
+//    a format/quality the device path does not implement, compress() reports
+//    through ErrorHandler (Error_CudaError / Error_UnsupportedFeature) and
+//    writes no data.  The reference's "w*h < 512 -> CPU" threshold
+//    (Context.cpp:494) therefore does not apply.
+//  - Surface keeps its FP32 planes in device memory; data()/channel() copy
+//    them back on demand.
+//  - Container headers (outputHeader), InputOptions/process, file IO and the
+//    Surface transforms outside this path are not provided.
+#ifndef NVTT_AMD_H
+#define NVTT_AMD_H
+
+#define NVTT_VERSION 20102
+
+#if defined(__GNUC__)
+#define NVTT_API __attribute__((visibility("default")))
+#else
+#define NVTT_API
+#endif
+
+namespace nvtt {
+
+// nvtt.h:77-115 (values are ABI).
+enum Format {
+    Format_RGB,
+    Format_RGBA = Format_RGB,
+    Format_DXT1,
+    Format_DXT1a,
+    Format_DXT3,
+    Format_DXT5,
+    Format_DXT5n,
+    Format_BC1 = Format_DXT1,
+    Format_BC1a = Format_DXT1a,
+    Format_BC2 = Format_DXT3,
+    Format_BC3 = Format_DXT5,
+    Format_BC3n = Format_DXT5n,
+    Format_BC4,
+    Format_BC5,
+    Format_DXT1n,
+    Format_CTX1,
+    Format_BC6,
+    Format_BC7,
+    Format_BC3_RGBM,
+    Format_ETC1,
+    Format_ETC2_R,
+    Format_ETC2_RG,
+    Format_ETC2_RGB,
+    Format_ETC2_RGBA,
+    Format_ETC2_RGB_A1,
+    Format_ETC2_RGBM,
+    Format_PVR_2BPP_RGB,
+    Format_PVR_4BPP_RGB,
+    Format_PVR_2BPP_RGBA,
+    Format_PVR_4BPP_RGBA,
+    Format_Count
+};
+
+// nvtt.h:148-154.
+enum Quality { Quality_Fastest, Quality_Normal, Quality_Production, Quality_Highest };
+
+// nvtt.h:209-214, 217-223, 226-233, 236-241, 274-279.
+enum WrapMode { WrapMode_Clamp, WrapMode_Repeat, WrapMode_Mirror };
+enum TextureType { TextureType_2D, TextureType_Cube, TextureType_3D, TextureType_Array };
+enum InputFormat { InputFormat_BGRA_8UB, InputFormat_RGBA_16F, InputFormat_RGBA_32F, InputFormat_R_32F };
+enum MipmapFilter { MipmapFilter_Box, MipmapFilter_Triangle, MipmapFilter_Kaiser };
+enum AlphaMode { AlphaMode_None, AlphaMode_Transparency, AlphaMode_Premultiplied };
+
+// nvtt.h:345-355.
+enum Error {
+    Error_Unknown,
+    Error_InvalidInput,
+    Error_UnsupportedFeature,
+    Error_CudaError,  // reported for any HIP device error
+    Error_FileOpen,
+    Error_FileWrite,
+    Error_UnsupportedOutputFormat,
+    Error_Count
+};
+
+struct Surface;
+
+// nvtt.h:159-196 (the settings this path reads).
+struct CompressionOptions {
+    struct Private;
+    NVTT_API CompressionOptions();
+    NVTT_API ~CompressionOptions();
+    NVTT_API void reset();
+    NVTT_API void setFormat(Format format);
+    NVTT_API void setQuality(Quality quality);
+    NVTT_API void setColorWeights(float red, float green, float blue, float alpha = 1.0f);
+    NVTT_API Format format() const;
+    Private &m;
+
+private:
+    CompressionOptions(const CompressionOptions &);
+    void operator=(const CompressionOptions &);
+};
+
+// nvtt.h:330-342.
+struct OutputHandler {
+    virtual ~OutputHandler() {}
+    virtual void beginImage(int size, int width, int height, int depth, int face, int miplevel) = 0;
+    virtual bool writeData(const void *data, int size) = 0;
+    virtual void endImage() = 0;
+};
+
+// nvtt.h:358-364.
+struct ErrorHandler {
+    virtual ~ErrorHandler() {}
+    virtual void error(Error e) = 0;
+};
+
+// nvtt.h:378-399 (handler plumbing only).
+struct OutputOptions {
+    struct Private;
+    NVTT_API OutputOptions();
+    NVTT_API ~OutputOptions();
+    NVTT_API void reset();
+    NVTT_API void setOutputHandler(OutputHandler *outputHandler);
+    NVTT_API void setErrorHandler(ErrorHandler *errorHandler);
+    Private &m;
+
+private:
+    OutputOptions(const OutputOptions &);
+    void operator=(const OutputOptions &);
+};
+
+// nvtt.h:402-410.
+typedef void Task(void *context, int id);
+struct TaskDispatcher {
+    virtual ~TaskDispatcher() {}
+    virtual void dispatch(Task *task, void *context, int count) = 0;
+};
+
+// nvtt.h:413-448 (Surface and raw APIs).
+struct Compressor {
+    struct Private;
+    NVTT_API Compressor();
+    NVTT_API ~Compressor();
+
+    // Device selection: enabling takes the first HIP device (the reference's
+    // CUDA switch; the method names are kept because they are ABI).
+    NVTT_API void enableCudaAcceleration(bool enable);
+    NVTT_API bool isCudaAccelerationEnabled() const;
+    // Accepted for source compatibility; the device path does not use it.
+    NVTT_API void setTaskDispatcher(TaskDispatcher *disp);
+
+    NVTT_API bool compress(const Surface &img, int face, int mipmap, const CompressionOptions &compressionOptions,
+                           const OutputOptions &outputOptions) const;
+    NVTT_API int estimateSize(const Surface &img, int mipmapCount, const CompressionOptions &compressionOptions) const;
+    NVTT_API bool compress(int w, int h, int d, int face, int mipmap, const float *rgba,
+                           const CompressionOptions &compressionOptions, const OutputOptions &outputOptions) const;
+    NVTT_API int estimateSize(int w, int h, int d, int mipmapCount, const CompressionOptions &compressionOptions) const;
+    Private &m;
+
+private:
+    Compressor(const Compressor &);
+    void operator=(const Compressor &);
+};
+typedef Compressor Context;
+
+// nvtt.h:476-600 (the members on this path).  Copies share storage until one
+// of them is modified (copy-on-write, Surface.cpp:354-363).
+struct Surface {
+    NVTT_API Surface();
+    NVTT_API Surface(const Surface &img);
+    NVTT_API ~Surface();
+    NVTT_API void operator=(const Surface &img);
+
+    NVTT_API void setWrapMode(WrapMode mode);
+    NVTT_API void setAlphaMode(AlphaMode alphaMode);
+    NVTT_API void setNormalMap(bool isNormalMap);
+
+    NVTT_API bool isNull() const;
+    NVTT_API int width() const;
+    NVTT_API int height() const;
+    NVTT_API int depth() const;
+    NVTT_API TextureType type() const;
+    NVTT_API WrapMode wrapMode() const;
+    NVTT_API AlphaMode alphaMode() const;
+    NVTT_API bool isNormalMap() const;
+    NVTT_API int countMipmaps() const;
+    NVTT_API int countMipmaps(int min_size) const;
+    NVTT_API const float *data() const;
+    NVTT_API const float *channel(int i) const;
+
+    NVTT_API bool setImage(int w, int h, int d);
+    NVTT_API bool setImage(InputFormat format, int w, int h, int d, const void *data);
+
+    NVTT_API bool buildNextMipmap(MipmapFilter filter, int min_size = 1);
+    NVTT_API bool buildNextMipmap(MipmapFilter filter, float filterWidth, const float *params = 0, int min_size = 1);
+    NVTT_API bool canMakeNextMipmap(int min_size = 1);
+
+    NVTT_API void toLinearFromSrgb();
+
+    // The device copy of the planes (4 planes of width*height floats), for
+    // callers that keep work on the device.  Null for a null surface.
+    NVTT_API const float *deviceData() const;
+
+    struct Private;
+    Private *m;
+};
+
+}  // namespace nvtt
+
+#endif  // NVTT_AMD_H

@@ -44,3 +44,18 @@ def test_null_arguments_rejected_without_device():
     assert lib.ntx_hip_device_count(None) == 1
     assert lib.ntx_hip_create(0, None) == 1
     assert lib.ntx_hip_destroy(None) == 0
+
+
+def test_front_end_exports_nvtt_api():
+    """libnvtt_amd.so carries the nvtt:: entry points of the drop-in path."""
+    import subprocess
+    out = subprocess.run(["nm", "-DC", N.NVTT_LIB_PATH], capture_output=True, text=True, check=True).stdout
+    for sym in ("nvtt::Compressor::compress(nvtt::Surface const&, int, int, nvtt::CompressionOptions const&, nvtt::OutputOptions const&) const",
+                "nvtt::Compressor::enableCudaAcceleration(bool)",
+                "nvtt::Surface::buildNextMipmap(nvtt::MipmapFilter, float, float const*, int)",
+                "nvtt::Surface::setImage(nvtt::InputFormat, int, int, int, void const*)",
+                "nvtt::Surface::toLinearFromSrgb()",
+                "nvtt::OutputOptions::setOutputHandler(nvtt::OutputHandler*)",
+                "nvtt::CompressionOptions::setFormat(nvtt::Format)"):
+        assert sym in out, sym
+    assert os.path.exists(os.path.join(N.LIB_DIR, "nvtt_api_test"))

@@ -202,3 +202,26 @@ def test_unsupported_and_invalid_calls_report_status(gpu):
     with pytest.raises(N.NtxError) as e:
         gpu.downsample(N.FILTER_BOX, planar[:4], 1, 1)
     assert e.value.status == 1
+
+
+@pytest.mark.parametrize("size", [(257, 129), (1000, 333), (4096, 64), (64, 4096), (21, 1023)])
+@pytest.mark.parametrize("filt", [N.FILTER_TRIANGLE, N.FILTER_KAISER])
+def test_polyphase_large_and_odd(gpu, oracle, size, filt):
+    W, H = size
+    planar = np.random.default_rng(W + 7 * H).random(4 * W * H, dtype=np.float32)
+    width, params = N.DEFAULT_FILTER[filt]
+    for wrap in (N.WRAP_MIRROR, N.WRAP_REPEAT):
+        a = gpu.downsample(filt, planar, W, H, wrap=wrap)
+        b = oracle.downsample(filt, width, params, wrap, planar, W, H)
+        assert np.array_equal(_bits(a), _bits(b)), (size, wrap)
+
+
+@pytest.mark.parametrize("size", [(64, 64), (37, 23), (5, 4)])
+def test_box_with_transparency_uses_polyphase_box(gpu, oracle, size):
+    """Surface.cpp:1356-1362: with AlphaMode_Transparency a Box mip goes through
+    downSample(BoxFilter) (polyphase), not fastDownSample."""
+    W, H = size
+    planar = np.random.default_rng(2).random(4 * W * H, dtype=np.float32)
+    a = gpu.downsample(N.FILTER_BOX, planar, W, H, alpha_mode=N.ALPHA_TRANSPARENCY)
+    b = oracle.downsample(N.FILTER_BOX, 0.5, None, N.WRAP_MIRROR, planar, W, H)
+    assert np.array_equal(_bits(a), _bits(b))
