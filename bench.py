@@ -170,7 +170,7 @@ def main():
     flops_bc3 = sum(bc3_color_flops(lv_host[k], w, h) for k, (w, h) in enumerate(levels))
     dom_ms, dom_n = prof[dom]
     launches_per_step = dom_n / args.steps
-    if dom == "bc3_color_kernel":
+    if dom == "bc3_search_kernel":
         achieved = flops_bc3 / (dom_ms / args.steps / 1e3) / 1e12
         roofline = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,

@@ -94,7 +94,7 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms);
  * CUDA timing is commented out, CudaCompressorDXT.cpp:173-174). */
 enum {
     NTX_KERNEL_BC1 = 0,
-    NTX_KERNEL_BC3_COLOR = 1,
+    NTX_KERNEL_BC3_SEARCH = 1, /* BC3 colour: weighted cluster fit over bucketed blocks */
     NTX_KERNEL_CHANNEL = 2, /* BC4 / BC5 / BC3-alpha blocks */
     NTX_KERNEL_BOX = 3,
     NTX_KERNEL_POLY_X = 4,
@@ -102,7 +102,8 @@ enum {
     NTX_KERNEL_SRGB = 6,
     NTX_KERNEL_UNPACK = 7,
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
-    NTX_KERNEL_COUNT = 9
+    NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 colour: single-colour blocks + bucketing by colour count */
+    NTX_KERNEL_COUNT = 10
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);

@@ -151,27 +151,6 @@ __global__ __launch_bounds__(256) void channel_kernel(const float *__restrict__ 
 
 // ---------------------------------------------------------- BC3 colour ----
 
-constexpr int kGroup = 16;
-constexpr int kBlocksPerWG = 16;
-constexpr int kTri = 153;  // partial-sum rows s=0..16, lengths 0..16-s
-
-__device__ __forceinline__ int tri_index(int s, int len) { return s * 17 - (s * (s - 1)) / 2 + len; }
-
-struct SqShared {
-    float4 part[kBlocksPerWG][kTri];  // left-to-right partial sums of weighted points (x,y,z,w)
-    float4 wp[kBlocksPerWG][16];      // weighted points in projection order
-    float dps[kBlocksPerWG][16];
-    int order[kBlocksPerWG][16];
-};
-
-__device__ __forceinline__ float gshfl(float v, int s) { return __shfl(v, s, kGroup); }
-__device__ __forceinline__ int gshfl(int v, int s) { return __shfl(v, s, kGroup); }
-__device__ __forceinline__ uint32_t gballot(bool p)
-{
-    const unsigned long long m = __ballot(p);
-    return uint32_t(m >> (threadIdx.x & 48)) & 0xffffu;
-}
-
 // std::min / std::max as used by squish's Vec3 Min/Max (maths.h).
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
 __device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
@@ -180,14 +159,87 @@ struct SqCand {
     float ax, ay, az, bx, by, bz;
 };
 
-// One candidate of the scalar Compress4 loop body (weightedclusterfit.cpp:507-541).
-__device__ __forceinline__ float sq_candidate(const float4 *part, uint32_t pk, float4 xsum, float mx, float my, float mz,
-                                              SqCand *out)
+// colourblock.cpp:30-54 FloatToInt / FloatTo565.
+__device__ __forceinline__ int float_to_int(float a, int limit)
 {
-    const int c0 = int(pk & 0xff), c1 = int((pk >> 8) & 0xff), c2 = int((pk >> 16) & 0xff);
-    const float4 x0 = part[tri_index(0, c0)];
-    const float4 x1 = part[tri_index(c0, c1)];
-    const float4 x2 = part[tri_index(c0 + c1, c2)];
+    int i = int(a + 0.5f);
+    return i < 0 ? 0 : (i > limit ? limit : i);
+}
+
+// ------------------------------------------ BC3 colour, blocks bucketed by colour count --
+//
+// Pass 1 (bc3_classify_kernel, one thread per block) finishes single-colour
+// blocks and appends every other block to the list of its distinct-colour
+// count k (2..16).  Pass 2 (bc3_search_kernel, one block per lane) walks the
+// lists from k = 16 down: the lanes of a wave share k, so they run the
+// reference's nested (c0, c1, c2) loops with identical trip counts and the
+// reference's own running sums, with no cross-lane work.
+
+constexpr int kCountSlots = 17;
+
+__device__ __forceinline__ void bc3_single_colour(const ntx::Tables *__restrict__ tab, uint32_t r8, uint32_t g8, uint32_t b8,
+                                                  uint2 *dst)
+{
+    // OptimalCompress::compressDXT1 (OptimalCompressDXT.cpp:254-269).
+    uint32_t c0 = uint32_t(tab->omatch5[r8] & 0xff) << 11 | uint32_t(tab->omatch6[g8] & 0xff) << 5 | uint32_t(tab->omatch5[b8] & 0xff);
+    uint32_t c1 = uint32_t(tab->omatch5[r8] >> 8) << 11 | uint32_t(tab->omatch6[g8] >> 8) << 5 | uint32_t(tab->omatch5[b8] >> 8);
+    uint32_t idx = 0xaaaaaaaau;
+    if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+    *dst = make_uint2(c0 | c1 << 16, idx);
+}
+
+__global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
+{
+    if (threadIdx.x < kCountSlots) counts[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
+                                                           const ntx::Tables *__restrict__ tab, int stride, int offset,
+                                                           uint8_t *__restrict__ out, int *__restrict__ counts,
+                                                           int *__restrict__ lists, int cap)
+{
+    __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
+    const int t = threadIdx.x;
+    if (t < kCountSlots) s_cnt[t] = 0;
+    __syncthreads();
+    const int blk = blockIdx.x * blockDim.x + t;
+    int k = 0, local = -1;
+    if (blk < nblocks) {
+        const int bx = blk % bw, by = blk / bw;
+        const size_t plane = size_t(w) * size_t(h);
+        uint32_t rgb[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const size_t o = block_texel(w, h, bx, by, i);
+            rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
+        }
+        bool single = true;
+#pragma unroll
+        for (int i = 1; i < 16; i++) single = single && rgb[i] == rgb[0];
+        if (single) {
+            bc3_single_colour(tab, rgb[0] & 0xff, (rgb[0] >> 8) & 0xff, rgb[0] >> 16,
+                              reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                bool d = true;
+#pragma unroll
+                for (int j = 0; j < i; j++) d = d && rgb[j] != rgb[i];
+                k += d ? 1 : 0;
+            }
+            local = atomicAdd(&s_cnt[k], 1);
+        }
+    }
+    __syncthreads();
+    if (t < kCountSlots && s_cnt[t] > 0) s_base[t] = atomicAdd(&counts[t], s_cnt[t]);
+    __syncthreads();
+    if (local >= 0) lists[size_t(k) * cap + s_base[k] + local] = blk;
+}
+
+// The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums.
+__device__ __forceinline__ float sq_eval(float4 x0, float4 x1, float4 x2, float4 xsum, float mx, float my, float mz,
+                                         SqCand *out)
+{
     const float w3 = ((xsum.w - x0.w) - x1.w) - x2.w;
     const float alpha2 = (x0.w + x1.w * (4.0f / 9.0f)) + x2.w * (1.0f / 9.0f);
     const float beta2 = (w3 + x2.w * (4.0f / 9.0f)) + x1.w * (1.0f / 9.0f);
@@ -214,100 +266,95 @@ __device__ __forceinline__ float sq_candidate(const float4 *part, uint32_t pk, f
     const float e1r = ((ar * ar) * alpha2 + (br * br) * beta2) + 2.0f * ((((ar * br) * ab) - ar * axr) - br * bxr);
     const float e1g = ((ag * ag) * alpha2 + (bg * bg) * beta2) + 2.0f * ((((ag * bg) * ab) - ag * axg) - bg * bxg);
     const float e1b = ((abl * abl) * alpha2 + (bb * bb) * beta2) + 2.0f * ((((abl * bb) * ab) - abl * axb) - bb * bxb);
-    out->ax = ar; out->ay = ag; out->az = abl;
-    out->bx = br; out->by = bg; out->bz = bb;
+    if (out) {
+        out->ax = ar; out->ay = ag; out->az = abl;
+        out->bx = br; out->by = bg; out->bz = bb;
+    }
     return e1r * mx + e1g * my + e1b * mz;
 }
 
-// colourblock.cpp:30-54 FloatToInt / FloatTo565.
-__device__ __forceinline__ int float_to_int(float a, int limit)
+__device__ __forceinline__ float4 add4(float4 a, float4 b)
 {
-    int i = int(a + 0.5f);
-    return i < 0 ? 0 : (i > limit ? limit : i);
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-__global__ __launch_bounds__(256) void bc3_color_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
+__global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict__ src, int w, int h, int bw,
                                                         int weight_by_alpha, float cwx, float cwy, float cwz,
-                                                        const ntx::Tables *__restrict__ tab, int stride, int offset,
-                                                        uint8_t *__restrict__ out)
+                                                        int stride, int offset, uint8_t *__restrict__ out,
+                                                        const int *__restrict__ counts, const int *__restrict__ lists, int cap)
 {
-    __shared__ SqShared sh;
-    const int g = threadIdx.x >> 4;
-    const int lane = threadIdx.x & 15;
-    const int blk = blockIdx.x * kBlocksPerWG + g;
-    if (blk >= nblocks) return;
+    // Column `lane` holds this lane's points; row 16 stays zero (the
+    // reference's loops read one element past the end and discard it).
+    __shared__ float4 s_p[17][64];
+    const int lane = threadIdx.x;
+
+    // Work item -> (colour count k, position in list k), k = 16 first.
+    int item = blockIdx.x * blockDim.x + lane;
+    int k = 0, pos_in = 0;
+    for (int b = 16; b >= 2; b--) {
+        const int c = counts[b];
+        if (k == 0) {
+            if (item < c) { k = b; pos_in = item; }
+            else item -= c;
+        }
+    }
+    if (k == 0) return;
+    const int blk = lists[size_t(k) * cap + pos_in];
     const int bx = blk % bw, by = blk / bw;
     const size_t plane = size_t(w) * size_t(h);
-    const size_t o = block_texel(w, h, bx, by, lane);
-    const uint32_t r8 = to_u8(src[o]), g8 = to_u8(src[o + plane]), b8 = to_u8(src[o + 2 * plane]),
-                   a8 = to_u8(src[o + 3 * plane]);
-    uint2 *dst = reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset);
 
-    // Single RGB colour (ColorBlock::isSingleColor, ColorBlock.cpp:136-149).
-    const uint32_t rgb = r8 | g8 << 8 | b8 << 16;
-    const uint32_t rgb0 = uint32_t(gshfl(int(rgb), 0));
-    if (gballot(rgb != rgb0) == 0) {
-        if (lane == 0) {
-            // OptimalCompress::compressDXT1 (OptimalCompressDXT.cpp:254-269).
-            uint32_t c0 = uint32_t(tab->omatch5[r8] & 0xff) << 11 | uint32_t(tab->omatch6[g8] & 0xff) << 5 |
-                          uint32_t(tab->omatch5[b8] & 0xff);
-            uint32_t c1 = uint32_t(tab->omatch5[r8] >> 8) << 11 | uint32_t(tab->omatch6[g8] >> 8) << 5 |
-                          uint32_t(tab->omatch5[b8] >> 8);
-            uint32_t idx = 0xaaaaaaaau;
-            if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
-            *dst = make_uint2(c0 | c1 << 16, idx);
-        }
-        return;
-    }
-
-    // ColourSet with createMinimalSet (colourset.cpp:35-101): texels sharing
-    // RGB bytes map to the point of their first occurrence.
-    int first = lane;
-    for (int j = 15; j >= 0; j--) {
-        const uint32_t o2 = uint32_t(gshfl(int(rgb), j));
-        if (j < lane && o2 == rgb) first = j;
-    }
-    const bool distinct = first == lane;
-    const uint32_t dmask = gballot(distinct);
-    const int count = __builtin_popcount(dmask);
-    // Point id of every texel: rank of its first occurrence among distinct texels.
-    const int my_point = __builtin_popcount(dmask & ((1u << first) - 1u));
-    const float tw = weight_by_alpha ? float(a8 + 1) / 256.0f : 1.0f;
-    // Lane p becomes point p: colour of the p-th distinct texel, weights summed in texel order.
-    int src_lane = 0;
-    {
-        uint32_t mm = dmask;
-        for (int d = 0; d < lane && mm; d++) mm &= mm - 1u;
-        src_lane = mm ? __builtin_ctz(mm) : 0;
-    }
-    const bool live = lane < count;
-    const float px = float(gshfl(int(r8), src_lane)) / 255.0f;
-    const float py = float(gshfl(int(g8), src_lane)) / 255.0f;
-    const float pz = float(gshfl(int(b8), src_lane)) / 255.0f;
-    float pw = 0.0f;
+    uint32_t rgb[16];
+    float wt[16];
+#pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int pi = gshfl(my_point, i);
-        const float wi = gshfl(tw, i);
-        if (pi == lane) pw += wi;
+        const size_t o = block_texel(w, h, bx, by, i);
+        rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
+        wt[i] = weight_by_alpha ? float(to_u8(src[o + 3 * plane]) + 1) / 256.0f : 1.0f;
+    }
+
+    // ColourSet minimal set (colourset.cpp:35-101): a texel whose RGB bytes
+    // repeat an earlier texel adds its weight to that point, in texel order.
+    int pidx[16];
+    int count = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int p = count;
+        bool d = true;
+#pragma unroll
+        for (int j = i - 1; j >= 0; j--)
+            if (rgb[j] == rgb[i]) { d = false; p = pidx[j]; }
+        pidx[i] = p;
+        if (d) {
+            s_p[p][lane] = make_float4(float(rgb[i] & 0xff) / 255.0f, float((rgb[i] >> 8) & 0xff) / 255.0f,
+                                       float(rgb[i] >> 16) / 255.0f, wt[i]);
+            count++;
+        } else {
+            s_p[p][lane].w += wt[i];
+        }
     }
 
     // ComputeWeightedCovariance (maths.cpp:32-59), metric on the deltas.
+    float4 P[16];
+#pragma unroll
+    for (int p = 0; p < 16; p++) P[p] = s_p[p][lane];
     float total = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    const float qx = px * pw, qy = py * pw, qz = pz * pw;
-    for (int j = 0; j < count; j++) {
-        total += gshfl(pw, j);
-        cx += gshfl(qx, j); cy += gshfl(qy, j); cz += gshfl(qz, j);
-    }
+#pragma unroll
+    for (int p = 0; p < 16; p++)
+        if (p < count) {
+            total += P[p].w;
+            cx += P[p].x * P[p].w; cy += P[p].y * P[p].w; cz += P[p].z * P[p].w;
+        }
     const float tinv = 1.0f / total;
     cx *= tinv; cy *= tinv; cz *= tinv;
-    const float ax = (px - cx) * cwx, ay = (py - cy) * cwy, az = (pz - cz) * cwz;
-    const float bxw = ax * pw, byw = ay * pw, bzw = az * pw;
-    const float k0 = ax * bxw, k1 = ax * byw, k2 = ax * bzw, k3 = ay * byw, k4 = ay * bzw, k5 = az * bzw;
     float m0 = 0, m1 = 0, m2 = 0, m3 = 0, m4 = 0, m5 = 0;
-    for (int j = 0; j < count; j++) {
-        m0 += gshfl(k0, j); m1 += gshfl(k1, j); m2 += gshfl(k2, j);
-        m3 += gshfl(k3, j); m4 += gshfl(k4, j); m5 += gshfl(k5, j);
-    }
+#pragma unroll
+    for (int p = 0; p < 16; p++)
+        if (p < count) {
+            const float ax = (P[p].x - cx) * cwx, ay = (P[p].y - cy) * cwy, az = (P[p].z - cz) * cwz;
+            const float bxw = ax * P[p].w, byw = ay * P[p].w, bzw = az * P[p].w;
+            m0 += ax * bxw; m1 += ax * byw; m2 += ax * bzw;
+            m3 += ay * byw; m4 += ay * bzw; m5 += az * bzw;
+        }
     // ComputePrincipleComponent, scalar build (maths.cpp:109-131).
     float vx, vy, vz;
     {
@@ -327,90 +374,90 @@ __global__ __launch_bounds__(256) void bc3_color_kernel(const float *__restrict_
             vx = x * iv; vy = y * iv; vz = z * iv;
         }
     }
-    // Stable insertion sort by projection (weightedclusterfit.cpp:62-77).
-    const float dp = px * vx + py * vy + pz * vz;
-    const bool has_nan = gballot(live && dp != dp) != 0;
-    int rank = lane;
-    if (!has_nan) {
-        rank = 0;
-        for (int j = 0; j < count; j++) {
-            const float o2 = gshfl(dp, j);
-            rank += (o2 < dp || (o2 == dp && j < lane)) ? 1 : 0;
+    // Stable insertion sort by projection (weightedclusterfit.cpp:62-77),
+    // written as the network it executes: each pass bubbles element i down
+    // while it is strictly smaller than its left neighbour.
+    float dps[16];
+    int op[16];
+#pragma unroll
+    for (int p = 0; p < 16; p++) {
+        dps[p] = P[p].x * vx + P[p].y * vy + P[p].z * vz;
+        op[p] = p;
+    }
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+        bool go = i < count;
+#pragma unroll
+        for (int j = i; j > 0; j--) {
+            go = go && dps[j] < dps[j - 1];
+            if (go) {
+                const float td = dps[j]; dps[j] = dps[j - 1]; dps[j - 1] = td;
+                const int to = op[j]; op[j] = op[j - 1]; op[j - 1] = to;
+            }
         }
-        if (live) sh.order[g][rank] = lane;
-        ntx_wave_sync();
-    } else {
-        if (live) { sh.dps[g][lane] = dp; sh.order[g][lane] = lane; }
-        ntx_wave_sync();
-        if (lane == 0) {
-            for (int i = 0; i < count; i++)
-                for (int j = i; j > 0 && sh.dps[g][j] < sh.dps[g][j - 1]; j--) {
-                    const float t = sh.dps[g][j]; sh.dps[g][j] = sh.dps[g][j - 1]; sh.dps[g][j - 1] = t;
-                    const int q = sh.order[g][j]; sh.order[g][j] = sh.order[g][j - 1]; sh.order[g][j - 1] = q;
-                }
-        }
-        ntx_wave_sync();
-        for (int i = 0; i < count; i++)
-            if (sh.order[g][i] == lane) rank = i;
     }
     // Weighted points in sorted order (weightedclusterfit.cpp:92-104).
-    const int sp = live ? sh.order[g][lane] : 0;
-    const float sw = gshfl(pw, sp);
-    const float wx = sw * gshfl(px, sp), wy = sw * gshfl(py, sp), wz = sw * gshfl(pz, sp);
-    if (live) sh.wp[g][lane] = make_float4(wx, wy, wz, sw);
+    float4 wp[16];
     float4 xsum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int j = 0; j < count; j++) {
-        xsum.x += gshfl(wx, j); xsum.y += gshfl(wy, j); xsum.z += gshfl(wz, j); xsum.w += gshfl(sw, j);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float4 q = s_p[op[i]][lane];
+        wp[i] = make_float4(q.w * q.x, q.w * q.y, q.w * q.z, q.w);
+        if (i < count) xsum = add4(xsum, wp[i]);
     }
-    ntx_wave_sync();
-    // Partial sums: lane s writes row s, T[s][len] = sum of wp[s .. s+len-1]
-    // accumulated left to right from zero, as the nested loops do.
-    if (lane <= count) {
-        float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        sh.part[g][tri_index(lane, 0)] = acc;
-        for (int len = 1; lane + len <= count; len++) {
-            const float4 v = sh.wp[g][lane + len - 1];
-            acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
-            sh.part[g][tri_index(lane, len)] = acc;
-        }
-    }
-    if (lane == 0 && count == 16) sh.part[g][tri_index(16, 0)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ntx_wave_sync();
+#pragma unroll
+    for (int i = 0; i < 16; i++) s_p[i][lane] = i < count ? wp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s_p[16][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-    // Compress4 search: candidates in loop order q; the first strict minimum wins.
+    // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
     const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
-    const int qbeg = tab->sq_off[count], qn = tab->sq_off[count + 1] - qbeg;
-    float be = FLT_MAX;
-    int bq = 0;
-    SqCand c;
-    for (int q = lane; q < qn; q += kGroup) {
-        const float e = sq_candidate(sh.part[g], tab->sq[qbeg + q], xsum, msx, msy, msz, &c);
-        if (e < be) { be = e; bq = q; }
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float besterror = FLT_MAX;
+    int b0 = 0, b1 = 0, b2 = 0;
+    float4 x0 = zero;
+    for (int c0 = 0; c0 <= count; c0++) {
+        float4 x1 = zero;
+        for (int c1 = 0; c1 <= count - c0; c1++) {
+            float4 x2 = zero;
+            for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
+                const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
+                if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
+                x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+            }
+            x1 = add4(x1, s_p[c0 + c1][lane]);
+        }
+        x0 = add4(x0, s_p[c0][lane]);
     }
-    for (int off = 8; off >= 1; off >>= 1) {
-        const float oe = gshfl(be, lane ^ off);
-        const int oq = gshfl(bq, lane ^ off);
-        if (oe < be || (oe == be && oq < bq)) { be = oe; bq = oq; }
-    }
-    if (!(be < FLT_MAX)) {
-        // The reference never writes the block in this case; emit zeros.
-        if (lane == 0) *dst = make_uint2(0u, 0u);
+    uint2 *dst = reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset);
+    if (!(besterror < FLT_MAX)) {
+        *dst = make_uint2(0u, 0u);  // the reference leaves the block unwritten here
         return;
     }
-    const uint32_t pk = tab->sq[qbeg + bq];
-    sq_candidate(sh.part[g], pk, xsum, msx, msy, msz, &c);
-    const int b0 = int(pk & 0xff), b1 = int((pk >> 8) & 0xff), b2 = int((pk >> 16) & 0xff);
-    // Index of this texel: by the sorted position of its point.
-    const int pos = gshfl(rank, my_point);
-    uint32_t idx = pos < b0 ? 0u : pos < b0 + b1 ? 2u : pos < b0 + b1 + b2 ? 3u : 1u;
+    // The winner's endpoints from the same running sums.
+    float4 y0 = zero, y1 = zero, y2 = zero;
+    for (int i = 0; i < b0; i++) y0 = add4(y0, s_p[i][lane]);
+    for (int i = b0; i < b0 + b1; i++) y1 = add4(y1, s_p[i][lane]);
+    for (int i = b0 + b1; i < b0 + b1 + b2; i++) y2 = add4(y2, s_p[i][lane]);
+    SqCand c;
+    sq_eval(y0, y1, y2, xsum, msx, msy, msz, &c);
+
+    // Indices by sorted position (remap through the minimal set), then
     // WriteColourBlock4 (colourblock.cpp:108-138).
     int ca = (float_to_int(31.0f * c.ax, 31) << 11) | (float_to_int(63.0f * c.ay, 63) << 5) | float_to_int(31.0f * c.az, 31);
     int cb = (float_to_int(31.0f * c.bx, 31) << 11) | (float_to_int(63.0f * c.by, 63) << 5) | float_to_int(31.0f * c.bz, 31);
-    if (ca < cb) { const int t = ca; ca = cb; cb = t; idx = (idx ^ 1u) & 3u; }
-    else if (ca == cb) { idx = 0u; }
-    uint32_t bits = idx << (2 * lane);
-    for (int off = 8; off >= 1; off >>= 1) bits |= uint32_t(gshfl(int(bits), lane ^ off));
-    if (lane == 0) *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
+    uint32_t bits = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        int pos = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pos += (op[i] == pidx[t]) ? i : 0;
+        uint32_t idx = pos < b0 ? 0u : pos < b0 + b1 ? 2u : pos < b0 + b1 + b2 ? 3u : 1u;
+        if (ca < cb) idx = (idx ^ 1u) & 3u;
+        else if (ca == cb) idx = 0u;
+        bits |= idx << (2 * t);
+    }
+    if (ca < cb) { const int t = ca; ca = cb; cb = t; }
+    *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
 }
 
 }  // namespace
@@ -431,15 +478,30 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
     return hipGetLastError();
 }
 
-hipError_t launch_bc3_color(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
-                            void *d_out, hipStream_t stream)
+// BC3 colour, pass 1: clear the per-count tallies, finish single-colour
+// blocks, bucket the rest.  counts: kCountSlots ints; lists: kCountSlots x cap ints.
+hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
+                               int *d_lists, int cap, hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(bc3_color_kernel, dim3((nblocks + kBlocksPerWG - 1) / kBlocksPerWG), dim3(kBlocksPerWG * kGroup), 0,
-                       stream, d_src, w, h, bw, nblocks, weight_by_alpha, cw[0], cw[1], cw[2], d_tab, 16, 8,
-                       reinterpret_cast<uint8_t *>(d_out));
+    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, stream, d_counts);
+    hipLaunchKernelGGL(bc3_classify_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
+                       16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
     return hipGetLastError();
 }
+
+// BC3 colour, pass 2: one block per lane over the bucketed lists.
+hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
+                             const int *d_counts, const int *d_lists, int cap, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, d_src, w, h, bw, weight_by_alpha,
+                       cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
+    return hipGetLastError();
+}
+
+int bc3_scratch_ints(int nblocks) { return 32 + kCountSlots * nblocks; }
 
 }  // namespace ntx

@@ -17,8 +17,11 @@ namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
                       int16_t *d_nm, hipStream_t stream);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
-hipError_t launch_bc3_color(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
-                            void *d_out, hipStream_t stream);
+hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
+                               int *d_lists, int cap, hipStream_t stream);
+hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
+                             const int *d_counts, const int *d_lists, int cap, hipStream_t stream);
+int bc3_scratch_ints(int nblocks);
 hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
 hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
                          hipStream_t stream);
@@ -86,7 +89,7 @@ struct ntx_hip_context {
     ntx::Tables *d_tab = nullptr;
     float *d_srgb = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf in, out, tmp, dst;
+    DevBuf in, out, tmp, dst, bucket;
     std::map<PolyKey, PolyDev> poly;
     std::string err;
     // per-kernel event timing (ntx_hip_profile_*)
@@ -269,7 +272,7 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         (void)hipFree(kv.second.weights);
         (void)hipFree(kv.second.left);
     }
-    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst })
+    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket })
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
@@ -371,9 +374,10 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms)
 
 NTX_API const char *ntx_hip_kernel_name(int kernel)
 {
-    static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_color_kernel", "channel_kernel",
+    static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_search_kernel", "channel_kernel",
                                                    "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
-                                                   "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel" };
+                                                   "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
+                                                   "bc3_classify_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -428,10 +432,18 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     if (format == NTX_FORMAT_BC1) {
         e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream); });
     } else if (format == NTX_FORMAT_BC3) {
+        const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+        if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
+        int *counts = static_cast<int *>(ctx->bucket.ptr), *lists = counts + 32;
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
         if (e == hipSuccess)
-            e = timed(ctx, NTX_KERNEL_BC3_COLOR, [&] {
-                return ntx::launch_bc3_color(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, ctx->d_tab, d_out, ctx->stream);
+            e = timed(ctx, NTX_KERNEL_BC3_CLASSIFY, [&] {
+                return ntx::launch_bc3_classify(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, ctx->stream);
+            });
+        if (e == hipSuccess)
+            e = timed(ctx, NTX_KERNEL_BC3_SEARCH, [&] {
+                return ntx::launch_bc3_search(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, d_out, counts, lists,
+                                              nblocks, ctx->stream);
             });
     } else {
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });

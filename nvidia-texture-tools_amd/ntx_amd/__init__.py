@@ -28,8 +28,8 @@ WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = 0, 1, 2
 DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_KAISER: (3.0, (4.0, 1.0))}
 
 # ntx_hip.h kernel ids for ntx_hip_profile_read.
-KERNELS = ("bc1_kernel", "bc3_color_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
-           "srgb_kernel", "unpack_kernel", "poly_fused_kernel")
+KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
+           "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_classify_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
