@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-workload leg")
+    ap.add_argument("--batch-textures", type=int, default=256)
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprofv3 counter passes: the BC3 chain, the Kaiser chain, the BC1 chain "
                          "(2 steps each) and a 256 MiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -230,6 +232,9 @@ def main():
                          "algorithmic_flops_per_chain": flops_bc1, "kernel_ms_per_chain": round(bc1_kernel_ms, 4)},
     }
 
+    if not args.no_batch:
+        result["batch"] = bench_batch(ctx, args)
+
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
@@ -290,6 +295,97 @@ def main():
                 "psnr_level0_ref_db": round(p_ref, 4)}
     print(json.dumps(result), flush=True)
     ctx.close()
+
+
+def bench_batch(ctx, args) -> dict:
+    """Batched uneven workload: n textures with power-of-two sizes drawn
+    uniformly from {64..2048} x {64..2048}; half RGBA noise+rectangles -> BC1,
+    a quarter normal maps (packed XY) -> BC5, a quarter single-channel noise
+    -> BC4; full Triangle mip chains.  One persistent launch per run, replayed
+    from a HIP graph, with the autotuned work decomposition.  Each texture is
+    cut from one of three 2048x2048 generated images at a seeded offset."""
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    n = args.batch_textures
+    rng = np.random.default_rng(args.seed + 1000)
+    sizes = [64, 128, 256, 512, 1024, 2048]
+    log(f"batch: generating {n} textures")
+    base = {N.FORMAT_BC1: synth.noise_rect_rgba(2048, 2048, seed=args.seed + 1),
+            N.FORMAT_BC5: synth.normal_map_rgba(2048, 2048, seed=args.seed + 2),
+            N.FORMAT_BC4: synth.single_channel_rgba(2048, 2048, seed=args.seed + 3)}
+    kinds = [N.FORMAT_BC1] * (n // 2) + [N.FORMAT_BC5] * (n // 4) + [N.FORMAT_BC4] * (n - n // 2 - n // 4)
+    texs = []
+    texels = 0
+    for i, fmt in enumerate(kinds):
+        w, h = int(rng.choice(sizes)), int(rng.choice(sizes))
+        x0, y0 = int(rng.integers(0, 2048 - w + 1)), int(rng.integers(0, 2048 - h + 1))
+        planar = synth.to_planar(base[fmt][y0:y0 + h, x0:x0 + w]).reshape(-1)
+        texs.append(dict(d_rgba=ctx.alloc(planar.nbytes).upload(planar), width=w, height=h, format=fmt,
+                         filter=N.FILTER_TRIANGLE, wrap=N.WRAP_MIRROR))
+        texels += sum(a * b for a, b in N.mip_sizes(w, h))
+    b = N.Batch(ctx, texs)
+    log("batch: autotuning")
+    t0 = time.perf_counter()
+    tuning, tune_ms = b.autotune(reps=2)
+    tune_s = time.perf_counter() - t0
+    steps = max(3, args.steps // 4)
+    b.run()
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    ctx.timer_begin()
+    for _ in range(steps):
+        b.submit()
+    ms = ctx.timer_end() / steps
+    b.wait()
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+
+    # The same work through the per-image entry points (one launch per level
+    # and kernel), and a byte-for-byte comparison of every level.
+    log("batch: per-image path for comparison")
+    chains = []
+    for t in texs:
+        w, h = t["width"], t["height"]
+        lv = [t["d_rgba"]] + [ctx.alloc(16 * a * c) for a, c in N.mip_sizes(w, h)[1:]]
+        out = [ctx.alloc(N.compressed_size(t["format"], a, c)) for a, c in N.mip_sizes(w, h)]
+        chains.append((lv, out))
+
+    def per_image():
+        for t, (lv, out) in zip(texs, chains):
+            sz = N.mip_sizes(t["width"], t["height"])
+            for k, (a, c) in enumerate(sz):
+                ctx.compress_device(t["format"], lv[k], a, c, out[k])
+                if k + 1 < len(sz):
+                    ctx.downsample_device(N.FILTER_TRIANGLE, lv[k], a, c, lv[k + 1])
+    per_image()
+    ctx.synchronize()
+    ctx.timer_begin()
+    per_image()
+    ms_seq = ctx.timer_end()
+    same = 0
+    nlev = 0
+    for i, (t, (lv, out)) in enumerate(zip(texs, chains)):
+        for k in range(b.levels(i)):
+            nlev += 1
+            same += int(np.array_equal(b.blocks(i, k), out[k].download(np.uint8)))
+    b.close()
+    return {
+        "workload": f"{n} textures, sizes uniform over {{64..2048}}^2 (powers of two), {n // 2} BC1 noise+rect, "
+                    f"{n // 4} BC5 normal maps, {n - n // 2 - n // 4} BC4 single-channel; full Triangle mip chains; "
+                    f"one persistent launch per run from a HIP graph",
+        "texels_per_run": texels,
+        "mtexels_s": round(texels / (ms / 1e3) / 1e6, 2),
+        "ms_per_run": round(ms, 4),
+        "runs": steps,
+        "tuning": tuning,
+        "autotune_s": round(tune_s, 3),
+        "per_image_path_ms": round(ms_seq, 4),
+        "speedup_vs_per_image_path": round(ms_seq / ms, 3),
+        "levels_identical_to_per_image_path": f"{same}/{nlev}",
+        "kernels_ms_per_run": {k: round(v[0] / steps, 4) for k, v in prof.items()},
+    }
 
 
 def _decode_bc1(blk: np.ndarray, w: int, h: int) -> np.ndarray:

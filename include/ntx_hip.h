@@ -103,7 +103,8 @@ enum {
     NTX_KERNEL_UNPACK = 7,
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 colour: single-colour blocks + bucketing by colour count */
-    NTX_KERNEL_COUNT = 10
+    NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
+    NTX_KERNEL_COUNT = 11
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -160,6 +161,72 @@ NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_
  * 8-bit B,G,R,A texels to planar FP32 (value / 255). */
 NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
                                                float *d_rgba);
+
+/* ---------------------------------------------------------------- batches --
+ * No counterpart in the reference, which compresses one image per
+ * Compressor::compress call (Context.cpp:146-190, 486-516).  A batch is many
+ * textures of any sizes, each with a full or partial mip chain
+ * (Surface::buildNextMipmap semantics, Surface.cpp:1335-1406) and every level
+ * block-compressed (Compressor::compress semantics), run as one persistent
+ * kernel launch captured in a HIP graph.  Every output byte equals what the
+ * per-image entry points above produce for the same level.
+ *
+ * Formats: BC1 (Normal/Highest), BC4 and BC5 (Fastest/Normal).  BC3 returns
+ * NTX_ERR_UNSUPPORTED here; use ntx_hip_compress_device per level. */
+typedef struct {
+    const float *d_rgba;   /* level 0: planar FP32 R,G,B,A planes of width*height, device memory */
+    int width, height;
+    int format;            /* NTX_FORMAT_BC1 / BC4 / BC5 */
+    int quality;           /* NTX_QUALITY_* */
+    int alpha_mode;        /* NTX_ALPHA_*: BC1 weights and the mip filter choice */
+    float color_weights[4];/* CompressionOptions::setColorWeights; all zero = (1,1,1,1) */
+    int mip_levels;        /* levels to produce and compress, 0 = full chain (nv::countMipmaps) */
+    int filter;            /* NTX_FILTER_* */
+    float filter_width;    /* <= 0: the filter's default (Surface.cpp:1121-1141) */
+    float filter_params[2];/* Kaiser alpha, stretch (ignored unless filter_width > 0) */
+    int wrap;              /* NTX_WRAP_* */
+} ntx_texture_desc;
+
+/* Work decomposition of the persistent kernel; ntx_hip_batch_autotune
+ * chooses it by timing candidates on the batch itself. */
+typedef struct {
+    int bc1_blocks_per_item;   /* multiple of 16 */
+    int channel_jobs_per_item; /* multiple of 256 (a BC5 block is two jobs) */
+    int mip_tiles_per_item;
+    int mip_tile_width;        /* 16, 32, 64, 128 or 256 outputs; tile height = 256 / width */
+    int kernel_variant;        /* 0: 4 waves/SIMD, 1: 5, 2: 6 (register budget of the kernel) */
+    int workgroups;            /* persistent workgroups, 0 = as many as are resident */
+} ntx_batch_tuning;
+
+typedef struct ntx_hip_batch ntx_hip_batch;
+
+/* Plans a batch: level buffers for levels >= 1, output buffers, polyphase
+ * tables, the work list and the graph.  descs[i].d_rgba must stay valid and
+ * unchanged while the batch is used.  The tuning last chosen by
+ * ntx_hip_batch_autotune for an identical list of shapes on this context is
+ * reused, otherwise defaults apply. */
+NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_desc *descs, int n,
+                                        ntx_hip_batch **batch);
+NTX_API ntx_status ntx_hip_batch_destroy(ntx_hip_batch *batch);
+/* Enqueues one run of the whole batch on the context's stream. */
+NTX_API ntx_status ntx_hip_batch_submit(ntx_hip_batch *batch);
+/* Waits for the submitted runs; NTX_ERR_DEVICE if the kernel reported a
+ * dependency wait that did not complete. */
+NTX_API ntx_status ntx_hip_batch_wait(ntx_hip_batch *batch);
+/* Number of levels texture `tex` has in this batch. */
+NTX_API int ntx_hip_batch_levels(const ntx_hip_batch *batch, int tex);
+/* Device pointers to one level: its planar FP32 image (w x h) and its
+ * compressed blocks (bytes long), valid until the batch is destroyed. */
+NTX_API ntx_status ntx_hip_batch_level(const ntx_hip_batch *batch, int tex, int level, const float **d_rgba, int *w,
+                                       int *h, const void **d_blocks, size_t *bytes);
+NTX_API ntx_status ntx_hip_batch_get_tuning(const ntx_hip_batch *batch, ntx_batch_tuning *tuning);
+/* Replaces the work decomposition (rebuilds the work list and graph). */
+NTX_API ntx_status ntx_hip_batch_set_tuning(ntx_hip_batch *batch, const ntx_batch_tuning *tuning);
+/* Times candidate decompositions on this batch (each run `reps` times after
+ * one warm-up; coordinate descent over the fields of ntx_batch_tuning),
+ * keeps the fastest, records it on the context for identical batches, and
+ * returns it with its time in milliseconds per run. */
+NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *batch, int reps, ntx_batch_tuning *best, float *best_ms);
 
 #ifdef __cplusplus
 }

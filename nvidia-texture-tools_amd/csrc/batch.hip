@@ -1,0 +1,227 @@
+// batch.hip -- one persistent kernel for a whole batch of textures: every
+// texture's mip chain (fastDownSample or polyphase resize) and the BC1 / BC4 /
+// BC5 compression of every level, taken from one work list.
+//
+// Workgroups take items from a queue in list order (one atomic per item).
+// Each item reads one level; levels produced inside the launch are guarded by
+// a ready counter that every mip item writing the level increments once its
+// outputs are globally visible.  The host orders the list so an item only
+// ever depends on items before it: the earliest unfinished item in the list
+// always has its inputs complete, so the queue drains whatever the number of
+// resident workgroups.  A wait that exceeds its poll limit sets a status bit
+// and continues, so every wave reaches the end of the kernel.
+//
+// Per-item arithmetic is the per-image kernels' own device code (bc1_block.hpp,
+// channel_block.hpp, mip_taps.hpp), so results are bit-identical to them.
+#include "bc1_block.hpp"
+#include "channel_block.hpp"
+#include "mip_taps.hpp"
+#include "ntx_batch.h"
+
+namespace {
+
+using ntx::BatchArgs;
+using ntx::BatchItem;
+using ntx::BatchLevel;
+
+// A descriptor every lane reads at the same address, copied into scalar
+// registers.  The loads themselves are vector loads: descriptors are read
+// after other workgroups' writes in the same launch, which the scalar cache
+// does not observe.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor of whole dwords");
+    T r;
+    const int *src = reinterpret_cast<const int *>(p);
+    int *dst = reinterpret_cast<int *>(&r);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return r;
+}
+
+// Level buffers are device (global) memory: say so, so that accesses through
+// them are global rather than flat instructions.
+template <typename T>
+__device__ __forceinline__ T *global_ptr(T *p)
+{
+    return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
+__device__ __forceinline__ void wait_ready(int *counter, int target, int spin_limit, int *status)
+{
+    // Gives up after spin_limit polls, or at once when another wait already
+    // gave up (the launch is failing; later waits need not spin as well).
+    int polls = 0;
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++polls > spin_limit || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            atomicOr(status, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+}
+
+// Tiles of the next level: thread t of a (1 << shift) x (256 >> shift) tile
+// computes one output texel on all four planes.
+__device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLevel &src, int t)
+{
+    const BatchLevel dst = uniform_copy(a.levels + src.next);
+    const int W = src.w, H = src.h, w = dst.w, h = dst.h;
+    const int tw = 1 << a.tile_shift, th = 256 >> a.tile_shift;
+    const int tiles_x = (w + tw - 1) / tw;
+    const size_t splane = size_t(W) * size_t(H), dplane = size_t(w) * size_t(h);
+    const float *s = global_ptr(src.data);
+    float *d = global_ptr(dst.data);
+    for (int k = 0; k < it.count; k++) {
+        const int tile = it.first + k;
+        const int x = (tile % tiles_x) * tw + (t & (tw - 1));
+        const int y = (tile / tiles_x) * th + (t >> a.tile_shift);
+        if (x >= w || y >= h) continue;
+        const size_t o = size_t(y) * size_t(w) + size_t(x);
+        if (src.mip_kind == 0) {
+            const bool line = W == 1 || H == 1;
+            const int bx = line ? x + y : x, by = line ? 0 : y;  // one of x, y is 0 for a line
+#pragma unroll
+            for (int c = 0; c < 4; c++) d[c * dplane + o] = ntx_mip::box_texel(s + c * splane, W, H, bx, by);
+        } else {
+#pragma unroll 1
+            for (int c = 0; c < 4; c++)
+                d[c * dplane + o] = ntx_mip::poly_texel(s + c * splane, W, H, x, y, src.kx, src.lx, src.wx, src.ky, src.ly,
+                                                        src.wy, src.wrap);
+        }
+    }
+}
+
+// BC1 blocks [first, first + count): block first + 16 i + g on group g.
+__device__ void bc1_item(const BatchArgs &a, const BatchItem &it, const BatchLevel &lv, ntx_bc1::Bc1Shared &sh, int t)
+{
+    const int bw = (lv.w + 3) / 4;
+    const int g = t >> 4, lane = t & 15;
+    const float *data = global_ptr(lv.data);
+    uint2 *out = global_ptr(reinterpret_cast<uint2 *>(lv.out));
+    for (int k = g; k < it.count; k += ntx_bc1::kBlocksPerWG) {
+        const int blk = it.first + k;
+        int n, m;
+        const uint2 r = ntx_bc1::encode_block(sh, data, lv.w, lv.h, blk % bw, blk / bw, lv.alpha_mode, lv.cw[0],
+                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m);
+        if (lane == 0) out[blk] = r;
+    }
+}
+
+// BC4 (R) or BC5 (R then G) channel jobs, one per thread.
+__device__ void channel_item(const BatchItem &it, const BatchLevel &lv, int t)
+{
+    const float *data = global_ptr(lv.data);
+    uint8_t *out = global_ptr(lv.out);
+    const int bw = (lv.w + 3) / 4;
+    const int njobs = lv.format == NTX_FORMAT_BC5 ? 2 : 1;
+    const int stride = 8 * njobs;
+    const size_t plane = size_t(lv.w) * size_t(lv.h);
+    for (int k = t; k < it.count; k += 256) {
+        const int job = it.first + k;
+        const int blk = job / njobs, part = job - blk * njobs;
+        const int bx = blk % bw, by = blk / bw;
+        const float *p = data + size_t(part) * plane;
+        uint32_t vq[4];
+        ntx_bc4::gather_channel(p, lv.w, lv.h, bx, by, vq);
+        const uint64_t r = ntx_bc4::compress_channel(vq);
+        *reinterpret_cast<uint2 *>(out + size_t(blk) * stride + 8 * part) = make_uint2(uint32_t(r), uint32_t(r >> 32));
+    }
+}
+
+// kWaves: waves per SIMD the register allocation is held to (0: the
+// compiler's choice); the autotuner picks among the instances.
+template <int kWaves>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves > 0 ? kWaves : 1)))
+void batch_kernel(BatchArgs a)
+{
+    __shared__ ntx_bc1::Bc1Shared sh;
+    __shared__ int s_item;
+    const int t = threadIdx.x;
+    for (int i = t; i < 976; i += 256) sh.four[i] = a.tab->four[i];
+    for (int i = t; i < 160; i += 256) sh.three[i] = a.tab->three[i];
+    for (;;) {
+        __syncthreads();  // the previous item is finished by every thread
+        if (t == 0) {
+            const int it = atomicAdd(a.head, 1);
+            if (it < a.nitems) {
+                const BatchLevel &lv = a.levels[a.items[it].level];
+                if (lv.ready >= 0) wait_ready(a.counters + lv.ready, lv.target, a.spin_limit, a.status);
+            }
+            s_item = it;
+        }
+        __syncthreads();
+        const int idx = __builtin_amdgcn_readfirstlane(s_item);
+        if (idx >= a.nitems) break;
+        const BatchItem it = uniform_copy(a.items + idx);
+        const BatchLevel lv = uniform_copy(a.levels + it.level);
+        if (lv.ready >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (it.kind == ntx::kItemMip) {
+            mip_item(a, it, lv, t);
+            // Every thread makes its stores visible device-wide before the
+            // workgroup counts the item as done.
+            __threadfence();
+            __syncthreads();
+            if (t == 0) {
+                const int ready = __builtin_amdgcn_readfirstlane(a.levels[lv.next].ready);
+                __hip_atomic_fetch_add(a.counters + ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (it.kind == ntx::kItemBc1) {
+            bc1_item(a, it, lv, sh, t);
+        } else {
+            channel_item(it, lv, t);
+        }
+    }
+}
+
+// Clears the ready counters, the queue position and the status word (a
+// kernel rather than a memset node, so the pair can be captured in a graph).
+__global__ __launch_bounds__(256) void batch_reset_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+namespace ntx {
+
+const void *batch_kernel_variant(int variant)
+{
+    switch (variant) {
+    case 1: return reinterpret_cast<const void *>(&batch_kernel<5>);
+    case 2: return reinterpret_cast<const void *>(&batch_kernel<6>);
+    default: return reinterpret_cast<const void *>(&batch_kernel<0>);
+    }
+}
+
+int batch_kernel_variants() { return 3; }
+
+hipError_t launch_batch(const BatchArgs &args, int variant, int workgroups, hipStream_t stream)
+{
+    switch (variant) {
+    case 1: hipLaunchKernelGGL(batch_kernel<5>, dim3(workgroups), dim3(256), 0, stream, args); break;
+    case 2: hipLaunchKernelGGL(batch_kernel<6>, dim3(workgroups), dim3(256), 0, stream, args); break;
+    default: hipLaunchKernelGGL(batch_kernel<0>, dim3(workgroups), dim3(256), 0, stream, args); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(int *p, int n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+// Workgroups of one batch_kernel instance that fit on the device at once.
+hipError_t batch_resident_workgroups(int device, int variant, int *count)
+{
+    int per_cu = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_variant(variant), 256, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    *count = per_cu * cus;
+    return e;
+}
+
+}  // namespace ntx

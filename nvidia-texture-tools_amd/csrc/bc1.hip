@@ -1,328 +1,13 @@
 // bc1.hip -- BC1 encoder for gfx950, bit-exact with NVTT's CPU path
-// (icbc v1.05 compress_dxt1 at Quality_Level8, the SSE4.1 build: no FMA,
-// round-half-even, 4-lane tie-break), icbc.h:3485-3559.
+// (icbc v1.05 compress_dxt1 at Quality_Level8), icbc.h:3485-3559.
 //
 // Mapping: one 4x4 block per 16-lane group, four blocks per wave64, sixteen
-// per 256-thread workgroup.  Lane l of a group owns texel l (gather, index
-// selection, error terms) and, after colour reduction, cluster l.  The
-// partition search spreads partition p over lane p % 16; the SSE tie-break
-// (per-lane strict '<' over p = lane + 4k, then lowest lane, icbc.h:533-545)
-// is reproduced by ordering candidates by (error, p & 3, p >> 2).
-#include "ntx_common.h"
-
-#include <cfloat>
+// per 256-thread workgroup; the per-block device code is in bc1_block.hpp.
+#include "bc1_block.hpp"
 
 namespace {
 
-constexpr int kGroup = 16;
-constexpr int kBlocksPerWG = 16;
-
-struct Bc1Shared {
-    float4 sat[kBlocksPerWG][16];    // (r, g, b, w) prefix sums in projection order
-    float4 col[kBlocksPerWG][16];    // scratch: sorted products for the SAT scan
-    float dps[kBlocksPerWG][16];
-    int order[kBlocksPerWG][16];
-    uint32_t four[976];
-    uint32_t three[160];
-};
-
-__device__ __forceinline__ float sse_sat(float a)
-{
-    // minps(maxps(a, 0), 1): NaN -> 0 (icbc.h:474-478).
-    float t = (a > 0.0f) ? a : 0.0f;
-    return (t < 1.0f) ? t : 1.0f;
-}
-__device__ __forceinline__ float round5(float x) { return rintf(x * 31.0f) * (1.0f / 31.0f); }
-__device__ __forceinline__ float round6(float x) { return rintf(x * 63.0f) * (1.0f / 63.0f); }
-
-__device__ __forceinline__ float gshfl(float v, int src) { return __shfl(v, src, kGroup); }
-__device__ __forceinline__ int gshfl(int v, int src) { return __shfl(v, src, kGroup); }
-
-// Mask of the lanes of this group for which pred holds, bit i = lane i.
-__device__ __forceinline__ uint32_t gballot(bool pred)
-{
-    const unsigned long long m = __ballot(pred);
-    const int base = threadIdx.x & 48;  // group offset inside the wave
-    return uint32_t(m >> base) & 0xffffu;
-}
-
-struct Cand {
-    float ax, ay, az, bx, by, bz;
-};
-
-// One least-squares candidate, icbc.h:2501-2538 (four colours) or
-// :2201-2236 (three colours).  FLT_MAX SAT entries past the colour count are
-// read exactly as the SSE build reads them.
-template <bool FOUR>
-__device__ __forceinline__ float eval_candidate(const float4 *sat, uint32_t packed, float4 sum, float mx, float my,
-                                                float mz, Cand *out)
-{
-    const int i0 = int(packed & 0xff) - 1;
-    const int i1 = int((packed >> 8) & 0xff) - 1;
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 x0 = i0 >= 0 ? sat[i0] : z;
-    float4 x1 = i1 >= 0 ? sat[i1] : z;
-    float alpha2, beta2, ab, axr, axg, axb;
-    if (FOUR) {
-        const int i2 = int((packed >> 16) & 0xff) - 1;
-        float4 x2 = i2 >= 0 ? sat[i2] : z;
-        const float w3 = sum.w - x2.w;
-        const float x2r = x2.x - x1.x, x2g = x2.y - x1.y, x2b = x2.z - x1.z;
-        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
-        const float w2 = x2.w - x1.w;
-        const float w1 = x1.w - x0.w;
-        const float w0 = x0.w;
-        alpha2 = w2 * (1.0f / 9.0f) + (w1 * (4.0f / 9.0f) + w0);
-        beta2 = w1 * (1.0f / 9.0f) + (w2 * (4.0f / 9.0f) + w3);
-        ab = (w1 + w2) * (2.0f / 9.0f);
-        axr = x2r * (1.0f / 3.0f) + (x1r * (2.0f / 3.0f) + x0.x);
-        axg = x2g * (1.0f / 3.0f) + (x1g * (2.0f / 3.0f) + x0.y);
-        axb = x2b * (1.0f / 3.0f) + (x1b * (2.0f / 3.0f) + x0.z);
-    } else {
-        const float w2 = sum.w - x1.w;
-        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
-        const float w1 = x1.w - x0.w;
-        ab = w1 * 0.25f;
-        alpha2 = x0.w + ab;
-        beta2 = w2 + ab;
-        axr = x0.x + x1r * 0.5f;
-        axg = x0.y + x1g * 0.5f;
-        axb = x0.z + x1b * 0.5f;
-    }
-    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
-    const float bxr = sum.x - axr, bxg = sum.y - axg, bxb = sum.z - axb;
-    float ar = (axr * beta2 - bxr * ab) * factor;
-    float ag = (axg * beta2 - bxg * ab) * factor;
-    float abl = (axb * beta2 - bxb * ab) * factor;
-    float br = (bxr * alpha2 - axr * ab) * factor;
-    float bg = (bxg * alpha2 - axg * ab) * factor;
-    float bb = (bxb * alpha2 - axb * ab) * factor;
-    ar = round5(sse_sat(ar)); ag = round6(sse_sat(ag)); abl = round5(sse_sat(abl));
-    br = round5(sse_sat(br)); bg = round6(sse_sat(bg)); bb = round5(sse_sat(bb));
-    const float e2r = (ar * (br * ab - axr) - br * bxr) * 2.0f;
-    const float e2g = (ag * (bg * ab - axg) - bg * bxg) * 2.0f;
-    const float e2b = (abl * (bb * ab - axb) - bb * bxb) * 2.0f;
-    const float e1r = (ar * ar) * alpha2 + ((br * br) * beta2 + e2r);
-    const float e1g = (ag * ag) * alpha2 + ((bg * bg) * beta2 + e2g);
-    const float e1b = (abl * abl) * alpha2 + ((bb * bb) * beta2 + e2b);
-    out->ax = ar; out->ay = ag; out->az = abl;
-    out->bx = br; out->by = bg; out->bz = bb;
-    return e1r * mx + e1g * my + e1b * mz;
-}
-
-// (error, p & 3, p >> 2) ordering: true if candidate (e, p) beats (be, bp).
-__device__ __forceinline__ bool key_less(float e, int p, float be, int bp)
-{
-    if (e < be) return true;
-    if (be < e) return false;
-    if (!(e == be)) return false;  // NaN never wins
-    const int lp = p & 3, lb = bp & 3;
-    if (lp != lb) return lp < lb;
-    return p < bp;
-}
-
-// Partition search over [0, roundup4(total)) for the block of this group.
-template <bool FOUR>
-__device__ void cluster_fit(const float4 *sat, const uint32_t *table, int total, int lane, float4 sum, float mx,
-                            float my, float mz, Cand *best)
-{
-    const int limit = (total + 3) & ~3;
-    float be = FLT_MAX;
-    int bp = 0;
-    Cand c;
-    for (int p = lane; p < limit; p += kGroup) {
-        const float e = eval_candidate<FOUR>(sat, table[p], sum, mx, my, mz, &c);
-        if (e < be) { be = e; bp = p; }
-    }
-    // Each lane saw p & 3 == lane & 3 in increasing order; combine lanes.
-    for (int off = 8; off >= 1; off >>= 1) {
-        const float oe = gshfl(be, lane ^ off);
-        const int op = gshfl(bp, lane ^ off);
-        if (key_less(oe, op, be, bp)) { be = oe; bp = op; }
-    }
-    if (!(be < FLT_MAX)) {
-        // No lane improved on FLT_MAX: the SSE build returns lane 0's initial zeros.
-        best->ax = best->ay = best->az = best->bx = best->by = best->bz = 0.0f;
-        return;
-    }
-    eval_candidate<FOUR>(sat, table[bp], sum, mx, my, mz, best);
-}
-
-// icbc.h:1580-1595 vector3_to_color16.
-__device__ __forceinline__ uint32_t to565(float x, float y, float z, const float *mid5, const float *mid6)
-{
-    float fr = x * 31.0f, fg = y * 63.0f, fb = z * 31.0f;
-    fr = (0.0f < fr) ? fr : 0.0f; fr = (fr < 31.0f) ? fr : 31.0f;
-    fg = (0.0f < fg) ? fg : 0.0f; fg = (fg < 63.0f) ? fg : 63.0f;
-    fb = (0.0f < fb) ? fb : 0.0f; fb = (fb < 31.0f) ? fb : 31.0f;
-    uint32_t r = uint32_t(fr), g = uint32_t(fg), b = uint32_t(fb);
-    r += (x > mid5[r]) ? 1u : 0u;
-    g += (y > mid6[g]) ? 1u : 0u;
-    b += (z > mid5[b]) ? 1u : 0u;
-    return (r << 11) | (g << 5) | b;
-}
-
-__device__ __forceinline__ void expand565(uint32_t c, int &r, int &g, int &b)
-{
-    const int r5 = (c >> 11) & 31, g6 = (c >> 5) & 63, b5 = c & 31;
-    r = (r5 << 3) | (r5 >> 2);
-    g = (g6 << 2) | (g6 >> 4);
-    b = (b5 << 3) | (b5 >> 2);
-}
-
-// D3D10 palette (icbc.h:2557-2585) as floats in [0,1] (color_to_vector3).
-__device__ __forceinline__ void palette(uint32_t c0, uint32_t c1, float p[4][3])
-{
-    int q[4][3];
-    expand565(c0, q[0][0], q[0][1], q[0][2]);
-    expand565(c1, q[1][0], q[1][1], q[1][2]);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        if (c0 > c1) {
-            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
-            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
-        } else {
-            q[2][k] = (q[0][k] + q[1][k]) / 2;
-            q[3][k] = 0;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) p[i][k] = float(q[i][k]) / 255.0f;
-}
-
-// output_block4 / output_block3 (icbc.h:2944-2980) for the group: lane l
-// classifies texel l; the weighted MSE is summed in texel order.
-template <bool FOUR>
-__device__ float output_block(const Cand &c, float tr, float tg, float tb, float tw, float cwx, float cwy, float cwz,
-                              const float *mid5, const float *mid6, int lane, uint2 *blk)
-{
-    uint32_t c0 = to565(c.ax, c.ay, c.az, mid5, mid6);
-    uint32_t c1 = to565(c.bx, c.by, c.bz, mid5, mid6);
-    if (FOUR ? (c0 < c1) : (c0 > c1)) { const uint32_t t = c0; c0 = c1; c1 = t; }
-    float p[4][3];
-    palette(c0, c1, p);
-    const float vr = tr * cwx, vg = tg * cwy, vb = tb * cwz;
-    float d[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float pr = p[k][0] * cwx, pg = p[k][1] * cwy, pb = p[k][2] * cwz;
-        float dr, dg, db;
-        if (FOUR) { dr = vr - pr; dg = vg - pg; db = vb - pb; }
-        else { dr = pr - vr; dg = pg - vg; db = pb - vb; }
-        d[k] = dr * dr + dg * dg + db * db;
-    }
-    uint32_t idx;
-    if (FOUR) {
-        // compute_indices4 (icbc.h:2803-2825).
-        const bool x0 = ((d[1] > d[2]) && (d[0] > d[2])) || ((d[0] > d[3]) && (d[1] > d[3]));
-        const bool x1 = (d[0] > d[3]) && (d[2] > d[3]);
-        idx = (x1 ? 1u : 0u) | (x0 ? 2u : 0u);
-    } else {
-        // compute_indices3 with allow_transparent_black (icbc.h:2836-2856).
-        const float d3 = vr * vr + vg * vg + vb * vb;
-        const bool i1 = d[1] < d[2];
-        const bool i2 = (d[2] <= d[0]) && (d[2] <= d[1]);
-        const bool i3 = (d3 <= d[0]) && (d3 <= d[1]) && (d3 <= d[2]);
-        idx = ((i1 || i3) ? 1u : 0u) | ((i2 || i3) ? 2u : 0u);
-    }
-    // evaluate_mse (icbc.h:2738-2749) term for this texel.
-    const float er = ((p[idx][0] - tr) * cwx) * 255.0f;
-    const float eg = ((p[idx][1] - tg) * cwy) * 255.0f;
-    const float eb = ((p[idx][2] - tb) * cwz) * 255.0f;
-    const float term = tw * (er * er + eg * eg + eb * eb);
-    float err = 0.0f;
-    for (int i = 0; i < 16; i++) err += gshfl(term, i);
-    uint32_t bits = idx << (2 * lane);
-    for (int off = 8; off >= 1; off >>= 1) bits |= uint32_t(gshfl(int(bits), lane ^ off));
-    blk->x = c0 | (c1 << 16);
-    blk->y = bits;
-    return err;
-}
-
-// icbc.h:1710-1882: PCA (power method) of the n clusters held one per lane,
-// stable sort by projection, and the weighted prefix-sum table in LDS.
-// Returns false only for an empty set.
-__device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, float kg, float kb, float kw)
-{
-    const bool live = lane < n;
-    // Centroid: sequential sums over clusters 0..n-1 (computeCentroid).
-    const float pr = kr * kw, pg = kg * kw, pb = kb * kw;
-    float total = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    for (int j = 0; j < n; j++) {
-        total += gshfl(kw, j);
-        cr += gshfl(pr, j);
-        cg += gshfl(pg, j);
-        cb += gshfl(pb, j);
-    }
-    const float inv = 1.0f / total;
-    cr *= inv; cg *= inv; cb *= inv;
-    // Covariance (computeCovariance).
-    const float ar = kr - cr, ag = kg - cg, ab = kb - cb;
-    const float br = ar * kw, bg = ag * kw, bb = ab * kw;
-    const float q0 = ar * br, q1 = ar * bg, q2 = ar * bb, q3 = ag * bg, q4 = ag * bb, q5 = ab * bb;
-    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f, m5 = 0.0f;
-    for (int j = 0; j < n; j++) {
-        m0 += gshfl(q0, j); m1 += gshfl(q1, j); m2 += gshfl(q2, j);
-        m3 += gshfl(q3, j); m4 += gshfl(q4, j); m5 += gshfl(q5, j);
-    }
-    // firstEigenVector_PowerMethod (icbc.h:1767-1791).
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    if (!(m0 == 0 && m3 == 0 && m5 == 0)) {
-        const float r0 = m0 * m0 + m1 * m1 + m2 * m2;
-        const float r1 = m1 * m1 + m3 * m3 + m4 * m4;
-        const float r2 = m2 * m2 + m4 * m4 + m5 * m5;
-        if (r0 > r1 && r0 > r2) { vx = m0; vy = m1; vz = m2; }
-        else if (r1 > r2) { vx = m1; vy = m3; vz = m4; }
-        else { vx = m2; vy = m4; vz = m5; }
-        for (int it = 0; it < 8; it++) {
-            const float x = vx * m0 + vy * m1 + vz * m2;
-            const float y = vx * m1 + vy * m3 + vz * m4;
-            const float z = vx * m2 + vy * m4 + vz * m5;
-            float norm = (y < x) ? x : y;           // icbc max(a,b) = (b<a)?a:b
-            norm = (z < norm) ? norm : z;
-            const float iv = 1.0f / norm;
-            vx = x * iv; vy = y * iv; vz = z * iv;
-        }
-    }
-    const float dp = kr * vx + kg * vy + kb * vz;
-    // Stable insertion sort by projection.  Without NaNs its result is the
-    // stable order by key, i.e. rank = #smaller + #equal-with-lower-index.
-    const bool has_nan = gballot(live && dp != dp) != 0;
-    int rank = 0;
-    if (!has_nan) {
-        for (int j = 0; j < n; j++) {
-            const float o = gshfl(dp, j);
-            rank += (o < dp || (o == dp && j < lane)) ? 1 : 0;
-        }
-        if (live) sh.order[g][rank] = lane;
-    } else {
-        if (live) { sh.dps[g][lane] = dp; sh.order[g][lane] = lane; }
-        ntx_wave_sync();
-        if (lane == 0) {
-            for (int i = 0; i < n; i++)
-                for (int j = i; j > 0 && sh.dps[g][j] < sh.dps[g][j - 1]; j--) {
-                    const float t = sh.dps[g][j]; sh.dps[g][j] = sh.dps[g][j - 1]; sh.dps[g][j - 1] = t;
-                    const int o = sh.order[g][j]; sh.order[g][j] = sh.order[g][j - 1]; sh.order[g][j - 1] = o;
-                }
-        }
-    }
-    ntx_wave_sync();
-    // Lane i takes the cluster at sorted position i and its weighted colour.
-    const int src = live ? sh.order[g][lane] : 0;
-    const float sr = gshfl(kr, src), sg = gshfl(kg, src), sb = gshfl(kb, src), sw = gshfl(kw, src);
-    if (live) sh.col[g][lane] = make_float4(sr * sw, sg * sw, sb * sw, sw);
-    ntx_wave_sync();
-    // Prefix sums in sorted order (compute_sat's sequential loop).
-    float4 acc = sh.col[g][0];
-    for (int i = 1; i <= lane && i < n; i++) {
-        const float4 v = sh.col[g][i];
-        acc.x = acc.x + v.x; acc.y = acc.y + v.y; acc.z = acc.z + v.z; acc.w = acc.w + v.w;
-    }
-    sh.sat[g][lane] = live ? acc : make_float4(FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX);
-    ntx_wave_sync();
-}
+using namespace ntx_bc1;
 
 __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                   int alpha_mode, float cwx, float cwy, float cwz,
@@ -338,113 +23,11 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
     const int lane = threadIdx.x & 15;
     const int blk = blockIdx.x * kBlocksPerWG + g;
     if (blk >= nblocks) return;  // whole group leaves together
-
-    // Gather (BlockCompressor.cpp:116-166): texel l; outside the image -> colour 0, weight 0.
-    const int bx = blk % bw, by = blk / bw;
-    const int x = bx * 4 + (lane & 3), y = by * 4 + (lane >> 2);
-    const size_t plane = size_t(w) * size_t(h);
-    float tr = 0.0f, tg = 0.0f, tb = 0.0f, tw = 0.0f;
-    if (x < w && y < h) {
-        const size_t o = size_t(y) * size_t(w) + size_t(x);
-        tr = src[o];
-        tg = src[o + plane];
-        tb = src[o + 2 * plane];
-        if (alpha_mode == NTX_ALPHA_TRANSPARENCY) {
-            const float a = src[o + 3 * plane];
-            float s = (0.0f < a) ? a : 0.0f;
-            tw = (s < 1.0f) ? s : 1.0f;
-        } else {
-            tw = 1.0f;
-        }
-    }
-
-    // reduce_colors (icbc.h:1638-1683): lane j holds cluster j.
-    const float thr = 1.0f / 256;
-    float kr = 0.0f, kg = 0.0f, kb = 0.0f, kw = 0.0f;
-    int n = 0;
-    bool any_black = false;
-    for (int i = 0; i < 16; i++) {
-        const float ir = gshfl(tr, i), ig = gshfl(tg, i), ib = gshfl(tb, i), iw = gshfl(tw, i);
-        if (iw > 0) {
-            const bool match = lane < n && fabsf(kr - ir) < thr && fabsf(kg - ig) < thr && fabsf(kb - ib) < thr;
-            const uint32_t m = gballot(match);
-            if (m) {
-                if (lane == __builtin_ctz(m)) {
-                    const float s = kw + iw;
-                    const float nr = kr * kw + ir * iw, ng = kg * kw + ig * iw, nb = kb * kw + ib * iw;
-                    kr = nr / s; kg = ng / s; kb = nb / s;
-                    kw = s;
-                }
-            } else {
-                if (lane == n) { kr = ir; kg = ig; kb = ib; kw = iw; }
-                n++;
-            }
-            if (ir < 1.0f / 8 && ig < 1.0f / 8 && ib < 1.0f / 8) any_black = true;
-        }
-    }
-
-    const float cwsq_x = cwx * cwx, cwsq_y = cwy * cwy, cwsq_z = cwz * cwz;
-    const float *mid5 = tab->mid5;
-    const float *mid6 = tab->mid6;
-    uint2 result;
-    int m_count = -1;
-    if (n == 0) {
-        result = make_uint2(0u, 0u);
-    } else if (n == 1) {
-        // compress_dxt1_single_color_optimal (icbc.h:3274-3289) on the
-        // rounded colour (vector3_to_color32).
-        const int r = int(uint8_t(sse_sat(kr) * 255 + 0.5f));
-        const int gg = int(uint8_t(sse_sat(kg) * 255 + 0.5f));
-        const int b = int(uint8_t(sse_sat(kb) * 255 + 0.5f));
-        uint32_t c0 = uint32_t(tab->match5[r] & 0xff) << 11 | uint32_t(tab->match6[gg] & 0xff) << 5 | uint32_t(tab->match5[b] & 0xff);
-        uint32_t c1 = uint32_t(tab->match5[r] >> 8) << 11 | uint32_t(tab->match6[gg] >> 8) << 5 | uint32_t(tab->match5[b] >> 8);
-        uint32_t idx = 0xaaaaaaaau;
-        if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
-        result = make_uint2(c0 | (c1 << 16), idx);
-    } else {
-        // compress_dxt1_cluster_fit (icbc.h:3292-3326).
-        compute_sat(sh, g, lane, n, kr, kg, kb, kw);
-        float4 sum = sh.sat[g][n - 1];
-        Cand c;
-        cluster_fit<true>(sh.sat[g], sh.four, tab->four_total[n - 1], lane, sum, cwsq_x, cwsq_y, cwsq_z, &c);
-        uint2 b4;
-        const float e4 = output_block<true>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, lane, &b4);
-        float best = e4;
-        result = b4;
-        bool run3 = true;
-        int m = n;
-        if (any_black) {
-            // skip_blacks (icbc.h:1685-1703): compact the non-black clusters.
-            const bool keep = lane < n && !(kr < 1.0f / 8 && kg < 1.0f / 8 && kb < 1.0f / 8);
-            const uint32_t km = gballot(keep);
-            m = __builtin_popcount(km);
-            if (m == 0) {
-                run3 = false;
-            } else {
-                // Lane d takes the d-th kept cluster, preserving order.
-                int srcl = 0;
-                if (lane < m) {
-                    uint32_t mm = km;
-                    for (int d = 0; d < lane; d++) mm &= mm - 1u;
-                    srcl = __builtin_ctz(mm);
-                }
-                const float nr = gshfl(kr, srcl), ng = gshfl(kg, srcl), nb = gshfl(kb, srcl), nw = gshfl(kw, srcl);
-                compute_sat(sh, g, lane, m, nr, ng, nb, nw);
-                sum = sh.sat[g][m - 1];
-            }
-        }
-        if (run3) {
-            m_count = m;
-            cluster_fit<false>(sh.sat[g], sh.three, tab->three_total[m - 1], lane, sum, cwsq_x, cwsq_y, cwsq_z, &c);
-            uint2 b3;
-            const float e3 = output_block<false>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, lane, &b3);
-            if (e3 < best) { best = e3; result = b3; }
-        }
-        if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
-    }
+    int n, m;
+    const uint2 result = encode_block(sh, src, w, h, blk % bw, blk / bw, alpha_mode, cwx, cwy, cwz, tab, g, lane, &n, &m);
     if (lane == 0) {
         out[blk] = result;
-        if (nm) nm[blk] = make_short2(short(n), short(m_count));
+        if (nm) nm[blk] = make_short2(short(n), short(m));
     }
 }
 

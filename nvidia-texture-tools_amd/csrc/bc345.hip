@@ -5,130 +5,13 @@
 //     or OptimalCompress::compressDXT1 for single-colour blocks (CompressorDX9.cpp:158-176).
 // Input is gathered as ColorBlock::init does (ColorBlock.cpp:80-110):
 // truncating float->u8 conversion, edge blocks filled by repeating pixels.
-#include "ntx_common.h"
+#include "channel_block.hpp"
 
 #include <cfloat>
 
 namespace {
 
-// nv::max / nv::min (nvcore/Utils.h:160-188).
-__device__ __forceinline__ float nvmaxf(float a, float b) { return (b < a) ? a : b; }
-__device__ __forceinline__ float nvminf(float a, float b) { return (a < b) ? a : b; }
-
-__device__ __forceinline__ uint32_t to_u8(float v)
-{
-    return uint32_t(uint8_t(255 * nvminf(nvmaxf(v, 0.0f), 1.0f)));
-}
-
-// Texel (e, i) of block (bx, by) with pixel repetition at the edges.
-__device__ __forceinline__ size_t block_texel(int w, int h, int bx, int by, int t)
-{
-    const int x = 4 * bx, y = 4 * by;
-    const int bw = min(w - x, 4), bh = min(h - y, 4);
-    const int e = t & 3, i = t >> 2;
-    return size_t(y + i % bh) * size_t(w) + size_t(x + e % bw);
-}
-
-// ------------------------------------------------------------- BC4 channel --
-
-// BlockDXT.cpp:336-378 palettes (d3d9 = false).
-__device__ __forceinline__ void alpha_palette(uint32_t a0, uint32_t a1, uint32_t pal[8])
-{
-    pal[0] = a0;
-    pal[1] = a1;
-    if (a0 > a1) {
-#pragma unroll
-        for (int k = 1; k <= 6; k++) pal[1 + k] = ((7 - k) * a0 + k * a1) / 7;
-    } else {
-#pragma unroll
-        for (int k = 1; k <= 4; k++) pal[1 + k] = ((5 - k) * a0 + k * a1) / 5;
-        pal[6] = 0;
-        pal[7] = 255;
-    }
-}
-
-// computeAlphaIndices (QuickCompressDXT.cpp:505-536): nearest palette entry,
-// first on ties; rewrites the 48 index bits and returns the squared error.
-__device__ __forceinline__ uint32_t alpha_indices(const uint32_t v[16], uint64_t &blk)
-{
-    uint32_t pal[8];
-    alpha_palette(uint32_t(blk & 0xff), uint32_t((blk >> 8) & 0xff), pal);
-    uint32_t total = 0;
-    uint64_t bits = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        uint32_t be = 256 * 256, bp = 8;
-#pragma unroll
-        for (uint32_t p = 0; p < 8; p++) {
-            const int d = int(pal[p]) - int(v[i]);
-            const uint32_t e = uint32_t(d * d);
-            if (e < be) { be = e; bp = p; }
-        }
-        total += be;
-        bits |= uint64_t(bp) << (16 + 3 * i);
-    }
-    blk = (blk & 0xffffull) | bits;
-    return total;
-}
-
-// optimizeAlpha8 (QuickCompressDXT.cpp:538-592): least-squares endpoints for
-// the current indices; truncating conversion, NaN -> 0 through nv::max.
-__device__ __forceinline__ void optimize_alpha8(const uint32_t v[16], uint64_t &blk)
-{
-    float a2 = 0, b2 = 0, ab = 0, ax = 0, bx = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const uint32_t idx = uint32_t(blk >> (16 + 3 * i)) & 7u;
-        const float alpha = (idx < 2) ? 1.0f - float(idx) : (8.0f - float(idx)) / 7.0f;
-        const float beta = 1 - alpha;
-        a2 += alpha * alpha;
-        b2 += beta * beta;
-        ab += alpha * beta;
-        ax += alpha * float(v[i]);
-        bx += beta * float(v[i]);
-    }
-    const float factor = 1.0f / (a2 * b2 - ab * ab);
-    const float a = (ax * b2 - bx * ab) * factor;
-    const float b = (bx * a2 - ax * ab) * factor;
-    uint32_t a0 = uint32_t(nvminf(nvmaxf(a, 0.0f), 255.0f));
-    uint32_t a1 = uint32_t(nvminf(nvmaxf(b, 0.0f), 255.0f));
-    uint64_t bits = blk & ~0xffffull;
-    if (a0 < a1) {
-        const uint32_t t = a0; a0 = a1; a1 = t;
-        uint64_t nb = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t idx = uint32_t(bits >> (16 + 3 * i)) & 7u;
-            nb |= uint64_t(idx < 2 ? 1 - idx : 9 - idx) << (16 + 3 * i);
-        }
-        bits = nb;
-    } else if (a0 == a1) {
-        bits = 0;
-    }
-    blk = bits | uint64_t(a0) | uint64_t(a1) << 8;
-}
-
-__device__ uint64_t compress_channel(const uint32_t v[16])
-{
-    uint32_t mx = 0, mn = 255;
-#pragma unroll
-    for (int i = 0; i < 16; i++) { mx = v[i] > mx ? v[i] : mx; mn = v[i] < mn ? v[i] : mn; }
-    const uint32_t d = (mx - mn) / 34;
-    uint64_t blk = uint64_t((mx - d) & 0xff) | uint64_t((mn + d) & 0xff) << 8;
-    uint32_t besterr = alpha_indices(v, blk);
-    uint64_t best = blk;
-    for (int it = 0; it < 8; it++) {
-        optimize_alpha8(v, blk);
-        const uint32_t err = alpha_indices(v, blk);
-        if (err >= besterr) break;
-        // sameIndices (QuickCompressDXT.cpp:643-647) ORs ~0xFFFF into both
-        // words, so only the endpoint bytes are compared.
-        if ((blk & 0xffffull) == (best & 0xffffull)) { best = blk; break; }
-        besterr = err;
-        best = blk;
-    }
-    return best;
-}
+using namespace ntx_bc4;
 
 // One thread per (block, channel job).  Channel plane: 0 = R, 1 = G, 3 = A.
 __global__ __launch_bounds__(256) void channel_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
@@ -141,10 +24,9 @@ __global__ __launch_bounds__(256) void channel_kernel(const float *__restrict__ 
     const int ch = job == 0 ? ch0 : ch1;
     const int bx = blk % bw, by = blk / bw;
     const float *plane = src + size_t(ch) * size_t(w) * size_t(h);
-    uint32_t v[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) v[i] = to_u8(plane[block_texel(w, h, bx, by, i)]);
-    const uint64_t r = compress_channel(v);
+    uint32_t vq[4];
+    gather_channel(plane, w, h, bx, by, vq);
+    const uint64_t r = compress_channel(vq);
     uint2 *dst = reinterpret_cast<uint2 *>(out + size_t(blk) * stride + (job == 0 ? 0 : off1));
     *dst = make_uint2(uint32_t(r), uint32_t(r >> 32));
 }
@@ -278,31 +160,16 @@ __device__ __forceinline__ float4 add4(float4 a, float4 b)
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-__global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                        int weight_by_alpha, float cwx, float cwy, float cwz,
-                                                        int stride, int offset, uint8_t *__restrict__ out,
-                                                        const int *__restrict__ counts, const int *__restrict__ lists, int cap)
+// ColorBlock::init gather, ColourSet minimal set, weighted PCA and the
+// stable sort of WeightedClusterFit (weightedclusterfit.cpp:39-106) for block
+// (bx, by).  Leaves the weighted points in sorted order in column `lane` of
+// s_p (rows from `count` on are zero) and returns the point count, the sum
+// of the weighted points, the sorted order (op) and each texel's point (pidx).
+__device__ __forceinline__ int bc3_prepare(float4 (*s_p)[64], int lane, const float *__restrict__ src, int w, int h,
+                                           int bx, int by, int weight_by_alpha, float cwx, float cwy, float cwz,
+                                           float4 &xsum_out, int op[16], int pidx[16])
 {
-    // Column `lane` holds this lane's points; row 16 stays zero (the
-    // reference's loops read one element past the end and discard it).
-    __shared__ float4 s_p[17][64];
-    const int lane = threadIdx.x;
-
-    // Work item -> (colour count k, position in list k), k = 16 first.
-    int item = blockIdx.x * blockDim.x + lane;
-    int k = 0, pos_in = 0;
-    for (int b = 16; b >= 2; b--) {
-        const int c = counts[b];
-        if (k == 0) {
-            if (item < c) { k = b; pos_in = item; }
-            else item -= c;
-        }
-    }
-    if (k == 0) return;
-    const int blk = lists[size_t(k) * cap + pos_in];
-    const int bx = blk % bw, by = blk / bw;
     const size_t plane = size_t(w) * size_t(h);
-
     uint32_t rgb[16];
     float wt[16];
 #pragma unroll
@@ -314,7 +181,6 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
 
     // ColourSet minimal set (colourset.cpp:35-101): a texel whose RGB bytes
     // repeat an earlier texel adds its weight to that point, in texel order.
-    int pidx[16];
     int count = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -378,7 +244,6 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
     // written as the network it executes: each pass bubbles element i down
     // while it is strictly smaller than its left neighbour.
     float dps[16];
-    int op[16];
 #pragma unroll
     for (int p = 0; p < 16; p++) {
         dps[p] = P[p].x * vx + P[p].y * vy + P[p].z * vz;
@@ -409,26 +274,18 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
     for (int i = 0; i < 16; i++) s_p[i][lane] = i < count ? wp[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     s_p[16][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-    // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
-    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+    xsum_out = xsum;
+    return count;
+}
+
+// Endpoints of the winning partition from the same running sums, indices
+// by sorted position through the minimal set, WriteColourBlock4
+// (colourblock.cpp:108-138).
+__device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int op[16], const int pidx[16], float4 xsum,
+                                          float msx, float msy, float msz, float besterror, int b0, int b1, int b2,
+                                          uint2 *dst)
+{
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float besterror = FLT_MAX;
-    int b0 = 0, b1 = 0, b2 = 0;
-    float4 x0 = zero;
-    for (int c0 = 0; c0 <= count; c0++) {
-        float4 x1 = zero;
-        for (int c1 = 0; c1 <= count - c0; c1++) {
-            float4 x2 = zero;
-            for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
-                const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
-                if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
-                x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
-            }
-            x1 = add4(x1, s_p[c0 + c1][lane]);
-        }
-        x0 = add4(x0, s_p[c0][lane]);
-    }
-    uint2 *dst = reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset);
     if (!(besterror < FLT_MAX)) {
         *dst = make_uint2(0u, 0u);  // the reference leaves the block unwritten here
         return;
@@ -458,6 +315,140 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
     }
     if (ca < cb) { const int t = ca; ca = cb; cb = t; }
     *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
+}
+
+__global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                        int weight_by_alpha, float cwx, float cwy, float cwz,
+                                                        int stride, int offset, uint8_t *__restrict__ out,
+                                                        const int *__restrict__ counts, const int *__restrict__ lists, int cap)
+{
+    // Column `lane` holds this lane's points; row 16 stays zero (the
+    // reference's loops read one element past the end and discard it).
+    __shared__ float4 s_p[17][64];
+    const int lane = threadIdx.x;
+
+    // Work item -> (colour count k, position in list k), k = 16 first.
+    int item = blockIdx.x * blockDim.x + lane;
+    int k = 0, pos_in = 0;
+    for (int b = 16; b >= 2; b--) {
+        const int c = counts[b];
+        if (k == 0) {
+            if (item < c) { k = b; pos_in = item; }
+            else item -= c;
+        }
+    }
+    if (k == 0) return;
+    const int blk = lists[size_t(k) * cap + pos_in];
+    int op[16], pidx[16];
+    float4 xsum;
+    const int count = bc3_prepare(s_p, lane, src, w, h, blk % bw, blk / bw, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+
+    // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
+    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float besterror = FLT_MAX;
+    int b0 = 0, b1 = 0, b2 = 0;
+    float4 x0 = zero;
+    for (int c0 = 0; c0 <= count; c0++) {
+        float4 x1 = zero;
+        for (int c1 = 0; c1 <= count - c0; c1++) {
+            float4 x2 = zero;
+            for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
+                const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
+                if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
+                x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+            }
+            x1 = add4(x1, s_p[c0 + c1][lane]);
+        }
+        x0 = add4(x0, s_p[c0][lane]);
+    }
+    bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
+              reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
+}
+
+// The same search with 2^lpb_shift lanes per block, for images too small to
+// fill the device one block per lane: lane `part` of a block scans a
+// contiguous slice of the reference's (c0, c1, c2) loop order.  Its running
+// sums start as sequential sums from zero, which is what the reference's
+// running sums are at that point of the loop, and then advance exactly as
+// the nested loops do; the slices' winners are combined preferring the
+// earlier slice on equal error, which is the first strict minimum overall.
+__global__ __launch_bounds__(64) void bc3_search_split_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                              int weight_by_alpha, float cwx, float cwy, float cwz,
+                                                              int stride, int offset, uint8_t *__restrict__ out,
+                                                              const int *__restrict__ counts,
+                                                              const int *__restrict__ lists, int cap,
+                                                              const ntx::Tables *__restrict__ tab, int lpb_shift)
+{
+    __shared__ float4 s_p[17][64];
+    const int lane = threadIdx.x;
+    const int part = lane & ((1 << lpb_shift) - 1);
+    int item = (blockIdx.x * blockDim.x + lane) >> lpb_shift;
+    int k = 0, pos_in = 0;
+    for (int b = 16; b >= 2; b--) {
+        const int c = counts[b];
+        if (k == 0) {
+            if (item < c) { k = b; pos_in = item; }
+            else item -= c;
+        }
+    }
+    const bool live = k != 0;
+    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float besterror = FLT_MAX;
+    int best = 0;  // c0 | c1 << 8 | c2 << 16
+    int blk = 0;
+    int op[16], pidx[16];
+    float4 xsum = zero;
+    if (live) {
+        blk = lists[size_t(k) * cap + pos_in];
+        const int count = bc3_prepare(s_p, lane, src, w, h, blk % bw, blk / bw, weight_by_alpha, cwx, cwy, cwz, xsum,
+                                      op, pidx);
+        const int first = tab->sq_off[count], total = tab->sq_off[count + 1] - first;
+        const int begin = (part * total) >> lpb_shift, end = ((part + 1) * total) >> lpb_shift;
+        if (begin < end) {
+            const uint32_t pk = tab->sq[first + begin];
+            int c0 = int(pk & 0xff), c1 = int((pk >> 8) & 0xff), c2 = int((pk >> 16) & 0xff);
+            float4 x0 = zero, x1 = zero, x2 = zero;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                if (i < c0) x0 = add4(x0, s_p[i][lane]);
+                else if (i < c0 + c1) x1 = add4(x1, s_p[i][lane]);
+                else if (i < c0 + c1 + c2) x2 = add4(x2, s_p[i][lane]);
+            }
+            for (int n = begin; n < end; n++) {
+                const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
+                if (e < besterror) { besterror = e; best = c0 | c1 << 8 | c2 << 16; }
+                // Next (c0, c1, c2) in the nested loops' order, with their updates.
+                if (c0 + c1 + c2 < count) {
+                    x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+                    c2++;
+                } else if (c0 + c1 < count) {
+                    x1 = add4(x1, s_p[c0 + c1][lane]);
+                    c1++;
+                    c2 = 0;
+                    x2 = zero;
+                } else {
+                    x0 = add4(x0, s_p[c0][lane]);
+                    c0++;
+                    c1 = c2 = 0;
+                    x1 = x2 = zero;
+                }
+            }
+        }
+    }
+    // Butterfly over the block's lanes, narrowest first: before the step with
+    // offset `off` a lane holds the winner of the off aligned consecutive
+    // slices around it, and the partner's group lies entirely before its own
+    // in loop order exactly when this lane's `off` bit is set.
+    for (int off = 1; off < (1 << lpb_shift); off <<= 1) {
+        const float oe = __shfl_xor(besterror, off);
+        const int ob = __shfl_xor(best, off);
+        if (oe < besterror || (oe == besterror && (part & off))) { besterror = oe; best = ob; }
+    }
+    if (!live || part != 0) return;
+    bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, best & 0xff, (best >> 8) & 0xff, best >> 16,
+              reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
 }
 
 }  // namespace
@@ -491,14 +482,24 @@ hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d
     return hipGetLastError();
 }
 
-// BC3 colour, pass 2: one block per lane over the bucketed lists.
+// BC3 colour, pass 2: 2^lpb_shift lanes per block over the bucketed lists
+// (one block per lane when lpb_shift is 0).
 hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
-                             const int *d_counts, const int *d_lists, int cap, hipStream_t stream)
+                             const int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
+                             hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, d_src, w, h, bw, weight_by_alpha,
-                       cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
+    if (lpb_shift <= 0) {
+        hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, d_src, w, h, bw,
+                           weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts,
+                           d_lists, cap);
+    } else {
+        const size_t lanes = size_t(nblocks) << lpb_shift;
+        hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, d_src, w, h,
+                           bw, weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out),
+                           d_counts, d_lists, cap, d_tab, lpb_shift);
+    }
     return hipGetLastError();
 }
 

@@ -6,11 +6,13 @@
 //   host-built PolyphaseKernel weights (Filter.cpp:563-608).
 // Planar FP32 RGBA, four planes.  Every kernel is a coalesced elementwise pass
 // (one output texel per lane, consecutive lanes on consecutive x).
-#include "ntx_common.h"
+#include "mip_taps.hpp"
 
 #include <cmath>
 
 namespace {
+
+using ntx_mip::wrap_index;
 
 __global__ __launch_bounds__(256) void box_kernel(const float *__restrict__ src, int W, int H, float *__restrict__ dst,
                                                   int w, int h)
@@ -21,65 +23,12 @@ __global__ __launch_bounds__(256) void box_kernel(const float *__restrict__ src,
     if (x >= w) return;
     const float *s = src + size_t(c) * size_t(W) * size_t(H);
     float *d = dst + size_t(c) * size_t(w) * size_t(h);
-    float r;
     if (W == 1 || H == 1) {
-        // 1D case: n outputs along the longer axis.
-        const int n = w * h;
-        const int i = x + y * w;  // one of w, h is 1
-        if (i >= n) return;
-        const float *p = s + 2 * size_t(i);
-        if ((W * H) & 1) {
-            const float scale = 1.0f / float(2 * n + 1);
-            const float w0 = float(n - i), w1 = float(n), w2 = float(1 + i);
-            r = scale * ((w0 * p[0] + w1 * p[1]) + w2 * p[2]);
-        } else {
-            r = 0.5f * (p[0] + p[1]);
-        }
-        d[i] = r;
+        // 1D case: the w outputs (h == 1 here) run along the longer axis.
+        d[x] = ntx_mip::box_texel(s, W, H, x, 0);
         return;
     }
-    const float *p = s + size_t(2 * y) * size_t(W) + 2 * size_t(x);
-    if ((W & 1) == 0 && (H & 1) == 0) {
-        r = 0.25f * (((p[0] + p[1]) + p[W]) + p[W + 1]);
-    } else if ((W & 1) && (H & 1)) {
-        const float scale = 1.0f / float(W * H);
-        const float v0 = float(h - y), v1 = float(h), v2 = float(1 + y);
-        const float w0 = float(w - x), w1 = float(w), w2 = float(1 + x);
-        float f = 0.0f;
-        f += v0 * ((w0 * p[0] + w1 * p[1]) + w2 * p[2]);
-        f += v1 * ((w0 * p[W] + w1 * p[W + 1]) + w2 * p[W + 2]);
-        f += v2 * ((w0 * p[2 * W] + w1 * p[2 * W + 1]) + w2 * p[2 * W + 2]);
-        r = f * scale;
-    } else if (W & 1) {
-        const float scale = 1.0f / float(2 * W);
-        const float w0 = float(w - x), w1 = float(w), w2 = float(1 + x);
-        float f = 0.0f;
-        f += w0 * (p[0] + p[W]);
-        f += w1 * (p[1] + p[W + 1]);
-        f += w2 * (p[2] + p[W + 2]);
-        r = f * scale;
-    } else {
-        const float scale = 1.0f / float(2 * H);
-        const float v0 = float(h - y), v1 = float(h), v2 = float(1 + y);
-        float f = 0.0f;
-        f += v0 * (p[0] + p[1]);
-        f += v1 * (p[W] + p[W + 1]);
-        f += v2 * (p[2 * W] + p[2 * W + 1]);
-        r = f * scale;
-    }
-    d[size_t(y) * size_t(w) + x] = r;
-}
-
-// FloatImage.h:298-330 wrap helpers.
-__device__ __forceinline__ int wrap_index(int x, int n, int wm)
-{
-    if (x >= 0 && x < n) return x;
-    if (wm == NTX_WRAP_CLAMP) return x < 0 ? 0 : n - 1;
-    if (wm == NTX_WRAP_REPEAT) return x >= 0 ? x % n : (x + 1) % n + n - 1;
-    if (n == 1) return 0;
-    x = abs(x);
-    while (x >= n) x = abs(n + n - x - 2);
-    return x;
+    d[size_t(y) * size_t(w) + x] = ntx_mip::box_texel(s, W, H, x, y);
 }
 
 // applyKernelX: tmp(i, y) = sum_j k(i, j) * src(wrap(left_i + j), y).

@@ -11,45 +11,9 @@
 #include <tuple>
 #include <vector>
 
-#include "ntx_common.h"
-
-namespace ntx {
-hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
-                      int16_t *d_nm, hipStream_t stream);
-hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
-hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                               int *d_lists, int cap, hipStream_t stream);
-hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
-                             const int *d_counts, const int *d_lists, int cap, hipStream_t stream);
-int bc3_scratch_ints(int nblocks);
-hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
-hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
-                         hipStream_t stream);
-hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
-                         hipStream_t stream);
-hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
-                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
-bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
-int poly_tile_x();
-int poly_tile_y();
-hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
-hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
-}  // namespace ntx
+#include "ntx_context.h"
 
 namespace {
-
-struct DevBuf {
-    void *ptr = nullptr;
-    size_t cap = 0;
-};
-
-struct PolyDev {
-    float *weights = nullptr;
-    int *left = nullptr;
-    int window = 0;
-    int span_x = 0;  // largest source span of a run of poly_tile_x() outputs
-    int span_y = 0;  // same for poly_tile_y() outputs
-};
 
 int max_span(const std::vector<int> &left, int window, int tile)
 {
@@ -60,8 +24,6 @@ int max_span(const std::vector<int> &left, int window, int tile)
     }
     return best;
 }
-
-using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
 
 uint32_t fbits(float f)
 {
@@ -83,23 +45,7 @@ float from_srgb(float f)
 
 }  // namespace
 
-struct ntx_hip_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    ntx::Tables *d_tab = nullptr;
-    float *d_srgb = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf in, out, tmp, dst, bucket;
-    std::map<PolyKey, PolyDev> poly;
-    std::string err;
-    // per-kernel event timing (ntx_hip_profile_*)
-    bool profiling = false;
-    struct Mark { int kernel; hipEvent_t start, stop; };
-    std::vector<Mark> marks;
-    std::vector<hipEvent_t> spare;
-};
-
-namespace {
+namespace ntx {
 
 ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what)
 {
@@ -107,11 +53,6 @@ ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what)
     return e == hipErrorOutOfMemory ? NTX_ERR_OUT_OF_MEMORY : NTX_ERR_DEVICE;
 }
 
-#define NTX_HIP(ctx, call)                                \
-    do {                                                  \
-        hipError_t e_ = (call);                           \
-        if (e_ != hipSuccess) return fail(ctx, e_, #call); \
-    } while (0)
 
 hipEvent_t take_event(ntx_hip_context *ctx)
 {
@@ -125,19 +66,6 @@ hipEvent_t take_event(ntx_hip_context *ctx)
     return e;
 }
 
-// Runs one launch; brackets it with events when the context is profiling.
-template <typename F>
-hipError_t timed(ntx_hip_context *ctx, int kernel, F &&launch)
-{
-    if (!ctx->profiling) return launch();
-    hipEvent_t a = take_event(ctx), b = take_event(ctx);
-    if (!a || !b) return hipErrorOutOfMemory;
-    hipError_t e = hipEventRecord(a, ctx->stream);
-    if (e == hipSuccess) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
-    ctx->marks.push_back({ kernel, a, b });
-    return e;
-}
 
 ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes)
 {
@@ -168,6 +96,18 @@ bool supported(int format, int quality)
     case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
     default: return false;
     }
+}
+
+// Lanes per block of the BC3 colour search (as a power of two): one block
+// per lane while the image alone fills the device, otherwise enough lanes per
+// block to reach about 512K lanes, up to a whole wavefront per block.  The
+// search time of a small image is set by the longest block, so splitting each
+// block's candidates over more lanes shortens it.
+int bc3_lanes_shift(int nblocks)
+{
+    int s = 0;
+    while (s < 6 && (size_t(nblocks) << s) < (size_t(1) << 19)) s++;
+    return s;
 }
 
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out)
@@ -202,7 +142,9 @@ ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const floa
     return NTX_OK;
 }
 
-}  // namespace
+}  // namespace ntx
+
+using namespace ntx;
 
 extern "C" {
 
@@ -377,7 +319,7 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
     static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_search_kernel", "channel_kernel",
                                                    "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
                                                    "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
-                                                   "bc3_classify_kernel" };
+                                                   "bc3_classify_kernel", "batch_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -443,7 +385,7 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
         if (e == hipSuccess)
             e = timed(ctx, NTX_KERNEL_BC3_SEARCH, [&] {
                 return ntx::launch_bc3_search(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, d_out, counts, lists,
-                                              nblocks, ctx->stream);
+                                              nblocks, ctx->d_tab, bc3_lanes_shift(nblocks), ctx->stream);
             });
     } else {
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });

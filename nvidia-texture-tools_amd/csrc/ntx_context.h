@@ -1,0 +1,101 @@
+// ntx_context.h -- internal: the context behind the C ABI handle and the
+// helpers shared by the translation units of libntx_hip (ntx_api.cpp,
+// batch_api.cpp).  Not installed; the public interface is include/ntx_hip.h.
+#pragma once
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ntx_common.h"
+
+namespace ntx {
+hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
+                      int16_t *d_nm, hipStream_t stream);
+hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
+hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
+                               int *d_lists, int cap, hipStream_t stream);
+hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
+                             const int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
+                             hipStream_t stream);
+int bc3_scratch_ints(int nblocks);
+hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
+hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
+                         hipStream_t stream);
+hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
+                         hipStream_t stream);
+hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
+                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
+bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
+int poly_tile_x();
+int poly_tile_y();
+hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
+hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct PolyDev {
+    float *weights = nullptr;
+    int *left = nullptr;
+    int window = 0;
+    int span_x = 0;  // largest source span of a run of poly_tile_x() outputs
+    int span_y = 0;  // same for poly_tile_y() outputs
+};
+
+using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
+
+}  // namespace ntx
+
+struct ntx_hip_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ntx::Tables *d_tab = nullptr;
+    float *d_srgb = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ntx::DevBuf in, out, tmp, dst, bucket;
+    std::map<ntx::PolyKey, ntx::PolyDev> poly;
+    std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
+    std::string err;
+    // per-kernel event timing (ntx_hip_profile_*)
+    bool profiling = false;
+    struct Mark { int kernel; hipEvent_t start, stop; };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> spare;
+};
+
+namespace ntx {
+
+ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what);
+ntx_status bind(ntx_hip_context *ctx);
+ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes);
+hipEvent_t take_event(ntx_hip_context *ctx);
+// Quality levels handled exactly by the kernels.
+bool supported(int format, int quality);
+// Polyphase weights and first-tap indices for src -> dst samples, cached per context.
+ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out);
+
+// Runs one launch; brackets it with events when the context is profiling.
+template <typename F>
+hipError_t timed(ntx_hip_context *ctx, int kernel, F &&launch)
+{
+    if (!ctx->profiling) return launch();
+    hipEvent_t a = take_event(ctx), b = take_event(ctx);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    ctx->marks.push_back({ kernel, a, b });
+    return e;
+}
+
+}  // namespace ntx
+
+#define NTX_HIP(ctx, call)                                      \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return ntx::fail(ctx, e_, #call); \
+    } while (0)

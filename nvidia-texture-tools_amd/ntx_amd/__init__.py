@@ -29,7 +29,7 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 
 # ntx_hip.h kernel ids for ntx_hip_profile_read.
 KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
-           "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_classify_kernel")
+           "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_classify_kernel", "batch_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
@@ -42,6 +42,9 @@ EXPORTS = (
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device",
+    "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
+    "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
+    "ntx_hip_batch_autotune",
 )
 
 _P = ctypes.c_void_p
@@ -49,6 +52,24 @@ _I = ctypes.c_int
 _F = ctypes.c_float
 _S = ctypes.c_size_t
 _lib = None
+
+
+class TextureDesc(ctypes.Structure):
+    """ntx_texture_desc (include/ntx_hip.h)."""
+    _fields_ = [("d_rgba", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("format", ctypes.c_int), ("quality", ctypes.c_int), ("alpha_mode", ctypes.c_int),
+                ("color_weights", ctypes.c_float * 4), ("mip_levels", ctypes.c_int), ("filter", ctypes.c_int),
+                ("filter_width", ctypes.c_float), ("filter_params", ctypes.c_float * 2), ("wrap", ctypes.c_int)]
+
+
+class BatchTuning(ctypes.Structure):
+    """ntx_batch_tuning (include/ntx_hip.h)."""
+    _fields_ = [("bc1_blocks_per_item", ctypes.c_int), ("channel_jobs_per_item", ctypes.c_int),
+                ("mip_tiles_per_item", ctypes.c_int), ("mip_tile_width", ctypes.c_int),
+                ("kernel_variant", ctypes.c_int), ("workgroups", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 class NtxError(RuntimeError):
@@ -93,6 +114,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
+        "ntx_hip_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_P)]),
+        "ntx_hip_batch_destroy": (_I, [_P]),
+        "ntx_hip_batch_submit": (_I, [_P]),
+        "ntx_hip_batch_wait": (_I, [_P]),
+        "ntx_hip_batch_levels": (_I, [_P, _I]),
+        "ntx_hip_batch_level": (_I, [_P, _I, _I, ctypes.POINTER(_P), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                     ctypes.POINTER(_P), ctypes.POINTER(_S)]),
+        "ntx_hip_batch_get_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
+        "ntx_hip_batch_set_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
+        "ntx_hip_batch_autotune": (_I, [_P, _I, ctypes.POINTER(BatchTuning), ctypes.POINTER(_F)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -255,6 +286,104 @@ class Context:
         ms = _F()
         self._check(self.lib.ntx_hip_timer_end(self.handle, ctypes.byref(ms)), "timer_end")
         return float(ms.value)
+
+
+class Batch:
+    """Many textures, each with a mip chain and a block format, run as one
+    persistent launch (ntx_hip_batch_*).  `textures` is a sequence of dicts
+    with keys d_rgba (DeviceBuffer of planar FP32 level 0), width, height,
+    format and optionally quality, alpha_mode, color_weights, mip_levels,
+    filter, filter_width, filter_params, wrap."""
+
+    def __init__(self, ctx: Context, textures: Sequence[dict]):
+        self.ctx = ctx
+        self._keep = list(textures)  # the level-0 buffers must outlive the batch
+        arr = (TextureDesc * len(textures))()
+        for i, t in enumerate(textures):
+            d = arr[i]
+            d.d_rgba = t["d_rgba"].ptr
+            d.width, d.height, d.format = int(t["width"]), int(t["height"]), int(t["format"])
+            d.quality = int(t.get("quality", QUALITY_NORMAL))
+            d.alpha_mode = int(t.get("alpha_mode", ALPHA_NONE))
+            for k, v in enumerate(t.get("color_weights", (1, 1, 1, 1))):
+                d.color_weights[k] = float(v)
+            d.mip_levels = int(t.get("mip_levels", 0))
+            d.filter = int(t.get("filter", FILTER_BOX))
+            d.filter_width = float(t.get("filter_width", 0.0))
+            for k, v in enumerate(t.get("filter_params", (0.0, 0.0))):
+                d.filter_params[k] = float(v)
+            d.wrap = int(t.get("wrap", WRAP_MIRROR))
+        h = _P()
+        ctx._check(ctx.lib.ntx_hip_batch_create(ctx.handle, ctypes.cast(arr, _P), len(textures), ctypes.byref(h)),
+                   "ntx_hip_batch_create")
+        self.handle = h.value
+        self.n = len(textures)
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.ntx_hip_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self) -> None:
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_submit(self.handle), "ntx_hip_batch_submit")
+
+    def wait(self) -> None:
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_wait(self.handle), "ntx_hip_batch_wait")
+
+    def run(self) -> None:
+        self.submit()
+        self.wait()
+
+    def levels(self, tex: int) -> int:
+        return int(self.ctx.lib.ntx_hip_batch_levels(self.handle, tex))
+
+    def level_info(self, tex: int, level: int):
+        """(device image pointer, w, h, device blocks pointer, block bytes)."""
+        img, blocks = _P(), _P()
+        w, h = _I(), _I()
+        nb = _S()
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_level(self.handle, tex, level, ctypes.byref(img), ctypes.byref(w),
+                                                         ctypes.byref(h), ctypes.byref(blocks), ctypes.byref(nb)),
+                        "ntx_hip_batch_level")
+        return img.value, w.value, h.value, blocks.value, nb.value
+
+    def blocks(self, tex: int, level: int) -> np.ndarray:
+        _, _, _, ptr, nb = self.level_info(tex, level)
+        out = np.empty(nb, np.uint8)
+        self.ctx._check(self.ctx.lib.ntx_hip_download(self.ctx.handle, _ptr(out), ptr, nb), "download")
+        return out
+
+    def image(self, tex: int, level: int) -> np.ndarray:
+        ptr, w, h, _, _ = self.level_info(tex, level)
+        out = np.empty(4 * w * h, np.float32)
+        self.ctx._check(self.ctx.lib.ntx_hip_download(self.ctx.handle, _ptr(out), ptr, out.nbytes), "download")
+        return out
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_get_tuning(self.handle, ctypes.byref(t)), "batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **fields) -> None:
+        t = BatchTuning()
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_get_tuning(self.handle, ctypes.byref(t)), "batch_get_tuning")
+        for k, v in fields.items():
+            setattr(t, k, int(v))
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_set_tuning(self.handle, ctypes.byref(t)), "batch_set_tuning")
+
+    def autotune(self, reps: int = 2):
+        """Returns (chosen tuning dict, ms per run)."""
+        t = BatchTuning()
+        ms = _F()
+        self.ctx._check(self.ctx.lib.ntx_hip_batch_autotune(self.handle, reps, ctypes.byref(t), ctypes.byref(ms)),
+                        "ntx_hip_batch_autotune")
+        return t.as_dict(), float(ms.value)
 
 
 def mip_sizes(w: int, h: int):

@@ -1,0 +1,158 @@
+"""GPU parity of the batch path (ntx_hip_batch_*): one persistent launch that
+builds every texture's mip chain and compresses every level.  Its outputs
+must equal, byte for byte and FP32 bit for bit, the per-image entry points
+(themselves pinned to the oracle in test_gpu_parity.py) and the oracle."""
+import numpy as np
+import pytest
+
+import ntx_amd as N
+from ntx_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+# (w, h, format, filter, alpha_mode, wrap, mip_levels) -- uneven sizes, all
+# three filters, both mip paths (fastDownSample, polyphase), lines, odd sizes.
+MIXED = [
+    (64, 32, N.FORMAT_BC1, N.FILTER_TRIANGLE, N.ALPHA_NONE, N.WRAP_MIRROR, 0),
+    (37, 23, N.FORMAT_BC1, N.FILTER_BOX, N.ALPHA_NONE, N.WRAP_MIRROR, 0),
+    (16, 64, N.FORMAT_BC5, N.FILTER_TRIANGLE, N.ALPHA_NONE, N.WRAP_CLAMP, 0),
+    (50, 50, N.FORMAT_BC4, N.FILTER_KAISER, N.ALPHA_NONE, N.WRAP_REPEAT, 0),
+    (1, 9, N.FORMAT_BC1, N.FILTER_BOX, N.ALPHA_NONE, N.WRAP_MIRROR, 0),
+    (128, 8, N.FORMAT_BC4, N.FILTER_BOX, N.ALPHA_NONE, N.WRAP_MIRROR, 3),
+    (33, 17, N.FORMAT_BC1, N.FILTER_BOX, N.ALPHA_TRANSPARENCY, N.WRAP_MIRROR, 0),
+    (1, 1, N.FORMAT_BC5, N.FILTER_TRIANGLE, N.ALPHA_NONE, N.WRAP_MIRROR, 0),
+]
+
+
+def _image(w, h, fmt, seed):
+    if fmt == N.FORMAT_BC5:
+        img = synth.normal_map_rgba(w, h, seed)
+    elif fmt == N.FORMAT_BC4:
+        img = synth.single_channel_rgba(w, h, seed)
+    else:
+        img = synth.noise_rect_rgba(w, h, seed, transparent=True, rects=4)
+    return synth.to_planar(img).reshape(-1)
+
+
+def _make(gpu, specs, seed0=10):
+    texs, hosts = [], []
+    for i, (w, h, fmt, filt, am, wrap, lv) in enumerate(specs):
+        planar = _image(w, h, fmt, seed0 + i)
+        buf = gpu.alloc(planar.nbytes).upload(planar)
+        hosts.append(planar)
+        texs.append(dict(d_rgba=buf, width=w, height=h, format=fmt, alpha_mode=am, filter=filt, wrap=wrap,
+                         mip_levels=lv))
+    return texs, hosts
+
+
+def _per_image(gpu, spec, planar):
+    """Levels and blocks from the per-image entry points (the reference's
+    buildNextMipmap / compress loop)."""
+    w, h, fmt, filt, am, wrap, lv = spec
+    nlev = lv or len(N.mip_sizes(w, h))
+    imgs, blks = [], []
+    cur = planar
+    for k in range(nlev):
+        imgs.append(cur)
+        blks.append(gpu.compress(fmt, cur, w, h, alpha_mode=am))
+        if k + 1 < nlev:
+            cur = gpu.downsample(filt, cur, w, h, wrap=wrap, alpha_mode=am)
+            w, h = max(1, w // 2), max(1, h // 2)
+    return imgs, blks
+
+
+def _check_batch(batch, specs, refs):
+    for t, spec in enumerate(specs):
+        imgs, blks = refs[t]
+        assert batch.levels(t) == len(imgs), (t, spec)
+        for k in range(len(imgs)):
+            got = batch.image(t, k)
+            assert np.array_equal(got.view(np.uint32), np.asarray(imgs[k], np.float32).view(np.uint32)), (t, k, spec)
+            assert np.array_equal(batch.blocks(t, k), blks[k]), (t, k, spec)
+
+
+def test_batch_matches_per_image_path(gpu):
+    texs, hosts = _make(gpu, MIXED)
+    refs = [_per_image(gpu, s, p) for s, p in zip(MIXED, hosts)]
+    with N.Batch(gpu, texs) as b:
+        b.run()
+        _check_batch(b, MIXED, refs)
+
+
+def test_batch_matches_oracle(gpu, oracle):
+    specs = MIXED[:2] + MIXED[4:5]
+    texs, hosts = _make(gpu, specs, seed0=30)
+    with N.Batch(gpu, texs) as b:
+        b.run()
+        for t, (w, h, fmt, filt, am, wrap, lv) in enumerate(specs):
+            width, params = N.DEFAULT_FILTER[filt]
+            cur = hosts[t]
+            for k in range(b.levels(t)):
+                assert np.array_equal(b.image(t, k).view(np.uint32), cur.view(np.uint32)), (t, k)
+                assert np.array_equal(b.blocks(t, k), oracle.compress(fmt, cur, w, h, alpha_mode=am)), (t, k)
+                if k + 1 < b.levels(t):
+                    cur = oracle.build_next_mipmap(filt, width, params, wrap, cur, w, h)
+                    w, h = max(1, w // 2), max(1, h // 2)
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(workgroups=1),
+    dict(workgroups=3, mip_tile_width=16, mip_tiles_per_item=1, bc1_blocks_per_item=16, channel_jobs_per_item=256),
+    dict(kernel_variant=0, mip_tile_width=256, mip_tiles_per_item=8, bc1_blocks_per_item=256),
+    dict(kernel_variant=2, mip_tile_width=32, channel_jobs_per_item=4096),
+])
+def test_batch_tunings_are_bit_identical(gpu, tuning):
+    texs, hosts = _make(gpu, MIXED, seed0=50)
+    refs = [_per_image(gpu, s, p) for s, p in zip(MIXED, hosts)]
+    with N.Batch(gpu, texs) as b:
+        b.set_tuning(**tuning)
+        assert all(b.tuning()[k] == v for k, v in tuning.items())
+        b.run()
+        _check_batch(b, MIXED, refs)
+
+
+def test_batch_graph_replays(gpu):
+    """The captured graph re-reads level 0 on every replay: replays with
+    host work between them, and after the inputs change, all match."""
+    specs = MIXED[:4]
+    texs, hosts = _make(gpu, specs, seed0=70)
+    with N.Batch(gpu, texs) as b:
+        for rep in range(3):
+            b.run()
+            refs = [_per_image(gpu, s, p) for s, p in zip(specs, hosts)]
+            _check_batch(b, specs, refs)
+            # new level-0 contents for the next replay
+            for t, (w, h, fmt, *_r) in enumerate(specs):
+                hosts[t] = _image(w, h, fmt, 100 + 10 * rep + t)
+                texs[t]["d_rgba"].upload(hosts[t])
+
+
+def test_batch_autotune_keeps_results(gpu):
+    texs, hosts = _make(gpu, MIXED, seed0=90)
+    refs = [_per_image(gpu, s, p) for s, p in zip(MIXED, hosts)]
+    with N.Batch(gpu, texs) as b:
+        best, ms = b.autotune(reps=1)
+        assert ms > 0 and b.tuning() == best
+        b.run()
+        _check_batch(b, MIXED, refs)
+    # a second batch of the same shapes starts from the tuned decomposition
+    with N.Batch(gpu, texs) as b2:
+        assert b2.tuning() == best
+
+
+def test_batch_rejects_bad_input(gpu):
+    planar = _image(8, 8, N.FORMAT_BC1, 1)
+    buf = gpu.alloc(planar.nbytes).upload(planar)
+    base = dict(d_rgba=buf, width=8, height=8, format=N.FORMAT_BC1)
+    with pytest.raises(N.NtxError) as e:
+        N.Batch(gpu, [dict(base, format=N.FORMAT_BC3)])
+    assert e.value.status == 2
+    with pytest.raises(N.NtxError) as e:
+        N.Batch(gpu, [dict(base, mip_levels=5)])  # 8x8 has 4 levels
+    assert e.value.status == 1
+    with pytest.raises(N.NtxError) as e:
+        N.Batch(gpu, [dict(base, quality=N.QUALITY_FASTEST)])
+    assert e.value.status == 2
+    with N.Batch(gpu, [base]) as b:
+        with pytest.raises(N.NtxError):
+            b.set_tuning(bc1_blocks_per_item=24)

@@ -225,3 +225,22 @@ def test_box_with_transparency_uses_polyphase_box(gpu, oracle, size):
     a = gpu.downsample(N.FILTER_BOX, planar, W, H, alpha_mode=N.ALPHA_TRANSPARENCY)
     b = oracle.downsample(N.FILTER_BOX, 0.5, None, N.WRAP_MIRROR, planar, W, H)
     assert np.array_equal(_bits(a), _bits(b))
+
+
+@pytest.mark.parametrize("size", [8, 64, 256, 512, 1024, 2048])
+def test_bc3_lane_split_widths(gpu, oracle, size):
+    """The BC3 colour search splits each block over 64, 32, ... 2 lanes as the
+    image shrinks (one block per lane at 4096x4096).  Colours from a small
+    palette give many partitions of equal error, so the winner on ties (the
+    first in the reference's loop order) is exercised at every width."""
+    rng = np.random.default_rng(size)
+    pal = rng.integers(0, 256, (6, 3))
+    img = np.empty((size, size, 4), np.uint8)
+    img[:, :, :3] = pal[rng.integers(0, 6, (size, size))]
+    img[:, :, 3] = rng.integers(0, 256, (size, size))
+    planar = synth.to_planar(img).reshape(-1)
+    for am in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
+        got = gpu.compress(N.FORMAT_BC3, planar, size, size, alpha_mode=am)
+        exp = oracle.compress(N.FORMAT_BC3, planar, size, size, alpha_mode=am)
+        diff = np.nonzero(np.any(got.reshape(-1, 16) != exp.reshape(-1, 16), axis=1))[0]
+        assert diff.size == 0, f"{diff.size} blocks differ (alpha mode {am}), first {diff[:5]}"
