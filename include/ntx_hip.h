@@ -104,7 +104,10 @@ enum {
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 colour: single-colour blocks + bucketing by colour count */
     NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
-    NTX_KERNEL_COUNT = 11
+    NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
+    NTX_KERNEL_BC1_LIST = 12,    /* BC1 cluster fit on the blocks the proposal pass queued */
+    NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
+    NTX_KERNEL_COUNT = 14
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -227,6 +230,57 @@ NTX_API ntx_status ntx_hip_batch_set_tuning(ntx_hip_batch *batch, const ntx_batc
  * keeps the fastest, records it on the context for identical batches, and
  * returns it with its time in milliseconds per run. */
 NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *batch, int reps, ntx_batch_tuning *best, float *best_ms);
+
+/* ------------------------------------------------ learned BC1 endpoints --
+ * No counterpart in the reference.  A 48-64-64-25 ReLU network (BF16
+ * weights and activations, FP32 accumulation on the matrix cores) proposes
+ * four candidate endpoint pairs per 4x4 block plus the RMSE the full cluster
+ * fit would reach; each candidate is refined by least squares and quantised
+ * as icbc does, the best is kept, and blocks the network is unsure about are
+ * encoded by the full cluster fit (the ntx_hip_compress BC1 path), which is
+ * therefore both the fallback and the measure the proposals are trained on.
+ * Parameters: NTX_MLP_PARAMS floats, W1[64][48] b1[64] W2[64][64] b2[64]
+ * W3[25][64] b3[25].  Features: per block 48 BF16 values (R, G, B planes of
+ * the 16 texels, centred on the block mean and divided by the largest
+ * deviation or 1/32, whichever is larger) and 4 floats (mean R, G, B, scale).  Labels: 8 floats per block
+ * (cluster-fit endpoints A, B and RMSE in that frame). */
+#define NTX_MLP_PARAMS 8921
+typedef struct ntx_hip_mlp ntx_hip_mlp;
+
+/* params NULL: He-uniform initialisation from `seed`, zero biases. */
+NTX_API ntx_status ntx_hip_mlp_create(ntx_hip_context *ctx, const float *params, uint32_t seed, ntx_hip_mlp **mlp);
+NTX_API ntx_status ntx_hip_mlp_destroy(ntx_hip_mlp *mlp);
+NTX_API ntx_status ntx_hip_mlp_get_params(ntx_hip_mlp *mlp, float *params);
+/* Also resets the optimiser state. */
+NTX_API ntx_status ntx_hip_mlp_set_params(ntx_hip_mlp *mlp, const float *params);
+/* Features of every block of a w x h planar FP32 image (BC1 block order). */
+NTX_API ntx_status ntx_hip_mlp_features_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h,
+                                               void *d_features, float *d_aux);
+/* Training labels from the image's BC1 cluster-fit blocks. */
+NTX_API ntx_status ntx_hip_mlp_labels_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h, const void *d_bc1,
+                                             const float *d_aux, float *d_labels);
+/* Network outputs for n blocks: 32 floats per block, the first 25 used. */
+NTX_API ntx_status ntx_hip_mlp_forward_device(ntx_hip_mlp *mlp, const void *d_features, int n, float *d_out);
+/* One training step on n blocks (min-of-4 endpoint loss + RMSE head): the
+ * mean loss per block and, when grad is not NULL, the mean gradient
+ * (NTX_MLP_PARAMS floats, host memory).  lr > 0 applies an Adam step. */
+NTX_API ntx_status ntx_hip_mlp_train_step(ntx_hip_mlp *mlp, const void *d_features, const float *d_labels, int n,
+                                          float lr, float *loss, float *grad);
+/* BC1 (AlphaMode_None, colour weights 1) of a w x h planar FP32 image from
+ * the network's proposals.  A block falls back to the full cluster fit when
+ * its predicted cluster-fit RMSE exceeds tau_pred, or when its best refined
+ * candidate's RMSE exceeds that prediction by more than tau_gap (disabled
+ * if negative); both in 8-bit units.  refine_iters: least-squares passes
+ * per candidate; greedy_rounds: rounds of the search that moves one 565
+ * channel of one endpoint by one step while the block error drops.  flags: NTX_MLP_BOX_CANDIDATES starts from the four
+ * diagonals of each block's colour bounding box instead of the network (the
+ * same refinement without the learned part; only tau_pred < 0, which sends
+ * every block to the fallback, applies).  fallback_blocks (may be NULL)
+ * receives the count, which makes the call wait for the device. */
+#define NTX_MLP_BOX_CANDIDATES 1
+NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *mlp, const float *d_rgba, int w, int h, float tau_pred,
+                                                   float tau_gap, int refine_iters, int greedy_rounds, int flags,
+                                                   void *d_out, int *fallback_blocks);
 
 #ifdef __cplusplus
 }

@@ -326,6 +326,20 @@ __device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, flo
     ntx_wave_sync();
 }
 
+// compress_dxt1_single_color_optimal (icbc.h:3274-3289) on the rounded
+// colour (vector3_to_color32).
+__device__ __forceinline__ uint2 single_color_block(const ntx::Tables *tab, float kr, float kg, float kb)
+{
+    const int r = int(uint8_t(sse_sat(kr) * 255 + 0.5f));
+    const int gg = int(uint8_t(sse_sat(kg) * 255 + 0.5f));
+    const int b = int(uint8_t(sse_sat(kb) * 255 + 0.5f));
+    uint32_t c0 = uint32_t(tab->match5[r] & 0xff) << 11 | uint32_t(tab->match6[gg] & 0xff) << 5 | uint32_t(tab->match5[b] & 0xff);
+    uint32_t c1 = uint32_t(tab->match5[r] >> 8) << 11 | uint32_t(tab->match6[gg] >> 8) << 5 | uint32_t(tab->match5[b] >> 8);
+    uint32_t idx = 0xaaaaaaaau;
+    if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+    return make_uint2(c0 | (c1 << 16), idx);
+}
+
 // Encodes block (bx, by) of the w x h planar RGBA image for this group;
 // every lane of the group returns the 8-byte block, the colour count n
 // after reduce_colors and the count m seen by the three-colour fit (-1 if
@@ -385,16 +399,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
     if (n == 0) {
         result = make_uint2(0u, 0u);
     } else if (n == 1) {
-        // compress_dxt1_single_color_optimal (icbc.h:3274-3289) on the
-        // rounded colour (vector3_to_color32).
-        const int r = int(uint8_t(sse_sat(kr) * 255 + 0.5f));
-        const int gg = int(uint8_t(sse_sat(kg) * 255 + 0.5f));
-        const int b = int(uint8_t(sse_sat(kb) * 255 + 0.5f));
-        uint32_t c0 = uint32_t(tab->match5[r] & 0xff) << 11 | uint32_t(tab->match6[gg] & 0xff) << 5 | uint32_t(tab->match5[b] & 0xff);
-        uint32_t c1 = uint32_t(tab->match5[r] >> 8) << 11 | uint32_t(tab->match6[gg] >> 8) << 5 | uint32_t(tab->match5[b] >> 8);
-        uint32_t idx = 0xaaaaaaaau;
-        if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
-        result = make_uint2(c0 | (c1 << 16), idx);
+        result = single_color_block(tab, kr, kg, kb);
     } else {
         // compress_dxt1_cluster_fit (icbc.h:3292-3326).
         compute_sat(sh, g, lane, n, kr, kg, kb, kw);

@@ -1,0 +1,954 @@
+// mlp.hip -- learned endpoint proposal for BC1 on gfx950: a 48-64-64-25
+// ReLU network in BF16 with FP32 accumulation on the matrix cores
+// (v_mfma_f32_16x16x16_bf16), its training step, and the compressor that
+// refines the network's four candidate endpoint pairs by least squares,
+// keeps the best, and sends blocks the network is unsure about to the full
+// cluster fit (the per-image BC1 encoder's own device code).
+//
+// This part has no counterpart in the reference.  Matrix layout: the weights
+// are the A operand (rows = outputs, k = inputs) and 16 blocks are the N
+// columns, so a layer's 16x16 FP32 result fragment (lane l: rows 4(l/16)+j of
+// column l%16) is exactly the next layer's B fragment after bias, ReLU and
+// rounding to BF16: the forward pass never leaves registers.
+#include "bc1_block.hpp"
+#include "ntx_mlp.h"
+
+#include <cfloat>
+
+namespace {
+
+using namespace ntx_mlp;
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t bf16_bits(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf16_round(float x) { return __uint_as_float(uint32_t(bf16_bits(x)) << 16); }
+__device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+
+__device__ __forceinline__ v4s pack4(float a, float b, float c, float d)
+{
+    v4s r;
+    r[0] = short(bf16_bits(a));
+    r[1] = short(bf16_bits(b));
+    r[2] = short(bf16_bits(c));
+    r[3] = short(bf16_bits(d));
+    return r;
+}
+
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// Fragment slots in LDS: forward A fragments of W1 (4 x 3), W2 (4 x 4), W3
+// (2 x 4), then the transposed W3^T (4 x 2) and W2^T (4 x 4) for backward.
+constexpr int kF1 = 0, kF2 = 12, kF3 = 28, kF3T = 36, kF2T = 44, kFrags = 60;
+
+struct Net {
+    v4s frag[kFrags][64];
+    float b1[kHidden], b2[kHidden], b3[kOutPad];
+};
+
+// A fragment (tm, ts) of a rows x cols row-major matrix M (or of its
+// transpose): lane l holds M[16 tm + l % 16][16 ts + 4 (l / 16) + j].
+__device__ __forceinline__ v4s load_frag(const float *M, int rows, int cols, bool transpose, int tm, int ts, int lane)
+{
+    const int r = 16 * tm + (lane & 15);
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int c = 16 * ts + 4 * (lane >> 4) + j;
+        float x = 0.0f;
+        if (!transpose && r < rows && c < cols) x = M[r * cols + c];
+        if (transpose && c < rows && r < cols) x = M[c * cols + r];  // M^T[r][c] = M[c][r]
+        v[j] = x;
+    }
+    return pack4(v[0], v[1], v[2], v[3]);
+}
+
+// Builds the workgroup's copy of the network from the FP32 parameters.
+__device__ void load_net(Net &net, const float *__restrict__ p, int t, int nthreads)
+{
+    for (int i = t; i < kFrags * 64; i += nthreads) {
+        const int f = i / 64, lane = i % 64;
+        v4s v;
+        if (f < kF2) v = load_frag(p + kW1, kHidden, kIn, false, f / 3, f % 3, lane);
+        else if (f < kF3) v = load_frag(p + kW2, kHidden, kHidden, false, (f - kF2) / 4, (f - kF2) % 4, lane);
+        else if (f < kF3T) v = load_frag(p + kW3, kOut, kHidden, false, (f - kF3) / 4, (f - kF3) % 4, lane);
+        else if (f < kF2T) v = load_frag(p + kW3, kOut, kHidden, true, (f - kF3T) / 2, (f - kF3T) % 2, lane);
+        else v = load_frag(p + kW2, kHidden, kHidden, true, (f - kF2T) / 4, (f - kF2T) % 4, lane);
+        net.frag[f][lane] = v;
+    }
+    for (int i = t; i < kHidden; i += nthreads) {
+        net.b1[i] = p[kB1 + i];
+        net.b2[i] = p[kB2 + i];
+    }
+    for (int i = t; i < kOutPad; i += nthreads) net.b3[i] = i < kOut ? p[kB3 + i] : 0.0f;
+}
+
+// Forward pass for the 16 blocks whose input fragments are bx[0..2] (k-step
+// s = colour channel s; lane l holds texels 4(l/16)..+3 of block l % 16).
+// Returns the post-ReLU hidden activations as FP32 values of their BF16
+// roundings (what the next layer and the weight gradients consume) and the
+// 32 output rows (25 used).
+__device__ __forceinline__ void forward(const Net &net, const v4s bx[3], int lane, v4f h1[4], v4f h2[4], v4f y[2])
+{
+    const int g = lane >> 4;
+    const v4f zero = { 0.0f, 0.0f, 0.0f, 0.0f };
+    v4s hb[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        v4f acc = zero;
+#pragma unroll
+        for (int s = 0; s < 3; s++) acc = mfma(net.frag[kF1 + 3 * m + s][lane], bx[s], acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float v = acc[j] + net.b1[16 * m + 4 * g + j];
+            h1[m][j] = bf16_round(v > 0.0f ? v : 0.0f);
+        }
+        hb[m] = pack4(h1[m][0], h1[m][1], h1[m][2], h1[m][3]);
+    }
+    v4s hb2[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        v4f acc = zero;
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc = mfma(net.frag[kF2 + 4 * m + s][lane], hb[s], acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float v = acc[j] + net.b2[16 * m + 4 * g + j];
+            h2[m][j] = bf16_round(v > 0.0f ? v : 0.0f);
+        }
+        hb2[m] = pack4(h2[m][0], h2[m][1], h2[m][2], h2[m][3]);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        v4f acc = zero;
+#pragma unroll
+        for (int s = 0; s < 4; s++) acc = mfma(net.frag[kF3 + 4 * m + s][lane], hb2[s], acc);
+#pragma unroll
+        for (int j = 0; j < 4; j++) y[m][j] = acc[j] + net.b3[16 * m + 4 * g + j];
+    }
+}
+
+// Block features: the 16 texels' R, G, B (outside the image: the mean),
+// minus the mean of the texels inside, times 1 / max(largest deviation,
+// kMinSpread).
+// aux = (mean R, G, B, scale).
+__device__ __forceinline__ void block_features(const float *src, int w, int h, int bx, int by, float f[kIn], float4 &aux)
+{
+    const size_t plane = size_t(w) * size_t(h);
+    float c[3][16];
+    bool in[16];
+    float sum[3] = { 0.0f, 0.0f, 0.0f };
+    int cnt = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int x = 4 * bx + (t & 3), y = 4 * by + (t >> 2);
+        in[t] = x < w && y < h;
+        const size_t o = in[t] ? size_t(y) * size_t(w) + size_t(x) : 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            c[k][t] = in[t] ? src[o + k * plane] : 0.0f;
+            sum[k] += c[k][t];
+        }
+        cnt += in[t] ? 1 : 0;
+    }
+    float m[3], d = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) m[k] = sum[k] / float(cnt);
+    // All texels inside equal (texel 0 always is inside): exactly zero
+    // features, whatever the rounding of the mean.
+    bool uni = true;
+#pragma unroll
+    for (int t = 1; t < 16; t++)
+        if (in[t] && (c[0][t] != c[0][0] || c[1][t] != c[1][0] || c[2][t] != c[2][0])) uni = false;
+    if (uni) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) m[k] = c[k][0];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; t++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (in[t]) d = fmaxf(d, fabsf(c[k][t] - m[k]));
+    const float s = 1.0f / fmaxf(d, kMinSpread);
+#pragma unroll
+    for (int t = 0; t < 16; t++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) f[k * 16 + t] = in[t] ? (c[k][t] - m[k]) * s : 0.0f;
+    aux = make_float4(m[0], m[1], m[2], s);
+}
+
+__global__ __launch_bounds__(256) void mlp_features_kernel(const float *__restrict__ src, int w, int h, int bw, int n,
+                                                           uint16_t *__restrict__ feat, float4 *__restrict__ aux)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    float f[kIn];
+    float4 a;
+    block_features(src, w, h, b % bw, b / bw, f, a);
+#pragma unroll
+    for (int i = 0; i < kIn; i++) feat[size_t(b) * kIn + i] = bf16_bits(f[i]);
+    aux[b] = a;
+}
+
+// D3D10 BC1 palette of (c0, c1) as floats in [0, 1].
+__device__ __forceinline__ void bc1_palette(uint32_t c0, uint32_t c1, float p[4][3])
+{
+    int q[4][3];
+    ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
+    ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (c0 > c1) {
+            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
+            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
+        } else {
+            q[2][k] = (q[0][k] + q[1][k]) / 2;
+            q[3][k] = 0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) p[i][k] = float(q[i][k]) / 255.0f;
+}
+
+// Training label of a block from its cluster-fit BC1 block: both endpoints
+// and the decoded RMSE over the texels inside the image, in the block frame.
+__global__ __launch_bounds__(256) void mlp_labels_kernel(const float *__restrict__ src, int w, int h, int bw, int n,
+                                                         const uint2 *__restrict__ blocks, const float4 *__restrict__ aux,
+                                                         float *__restrict__ labels)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const uint2 blk = blocks[b];
+    const uint32_t c0 = blk.x & 0xffff, c1 = blk.x >> 16;
+    float p[4][3];
+    bc1_palette(c0, c1, p);
+    const float4 a = aux[b];
+    const int bx = b % bw, by = b / bw;
+    const size_t plane = size_t(w) * size_t(h);
+    float se = 0.0f;
+    int cnt = 0;
+    for (int t = 0; t < 16; t++) {
+        const int x = 4 * bx + (t & 3), y = 4 * by + (t >> 2);
+        if (x >= w || y >= h) continue;
+        const size_t o = size_t(y) * size_t(w) + size_t(x);
+        const int idx = (blk.y >> (2 * t)) & 3;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float d = p[idx][k] - src[o + k * plane];
+            se += d * d;
+        }
+        cnt++;
+    }
+    float *L = labels + size_t(b) * kLabel;
+    const float m[3] = { a.x, a.y, a.z };
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        L[k] = (p[0][k] - m[k]) * a.w;
+        L[3 + k] = (p[1][k] - m[k]) * a.w;
+    }
+    L[6] = sqrtf(se / float(3 * cnt)) * a.w;
+    L[7] = 0.0f;
+}
+
+// Loss of one block and its gradient with respect to the 25 outputs:
+// min over candidates of the unordered endpoint-pair error, plus a kSpread
+// share of every candidate's error, plus the squared RMSE-head error.
+__device__ __forceinline__ float block_loss(const float *y, const float *L, float *dy)
+{
+    float lk[kCandidates];
+    bool swp[kCandidates];
+#pragma unroll
+    for (int k = 0; k < kCandidates; k++) {
+        float ds = 0.0f, dw = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float a = y[6 * k + i], b = y[6 * k + 3 + i];
+            ds += (a - L[i]) * (a - L[i]) + (b - L[3 + i]) * (b - L[3 + i]);
+            dw += (a - L[3 + i]) * (a - L[3 + i]) + (b - L[i]) * (b - L[i]);
+        }
+        swp[k] = dw < ds;
+        lk[k] = swp[k] ? dw : ds;
+    }
+    int kb = 0;
+#pragma unroll
+    for (int k = 1; k < kCandidates; k++)
+        if (lk[k] < lk[kb]) kb = k;
+    float loss = lk[kb];
+#pragma unroll
+    for (int k = 0; k < kCandidates; k++) {
+        loss += kSpread * lk[k] / kCandidates;
+        const float coef = (k == kb ? 1.0f : 0.0f) + kSpread / kCandidates;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float ta = swp[k] ? L[3 + i] : L[i], tb = swp[k] ? L[i] : L[3 + i];
+            dy[6 * k + i] = coef * 2.0f * (y[6 * k + i] - ta);
+            dy[6 * k + 3 + i] = coef * 2.0f * (y[6 * k + 3 + i] - tb);
+        }
+    }
+    const float dr = y[24] - L[6];
+    loss += kRmseWeight * dr * dr;
+    dy[24] = kRmseWeight * 2.0f * dr;
+#pragma unroll
+    for (int i = kOut; i < kOutPad; i++) dy[i] = 0.0f;
+    return loss;
+}
+
+// Input fragments of 16 blocks from the feature array: k-step s = channel s,
+// lane l: texels 4(l/16)..+3 of block base + l % 16 (zero past n).
+__device__ __forceinline__ void input_frags(const uint16_t *__restrict__ feat, int base, int n, int lane, v4s bx[3])
+{
+    const int b = base + (lane & 15), g = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        v4s v = { 0, 0, 0, 0 };
+        if (b < n) {
+            const uint16_t *f = feat + size_t(b) * kIn + 16 * s + 4 * g;
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = short(f[j]);
+        }
+        bx[s] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void mlp_forward_kernel(const float *__restrict__ params, const uint16_t *__restrict__ feat,
+                                                         int n, float *__restrict__ out)
+{
+    __shared__ Net net;
+    const int lane = threadIdx.x;
+    load_net(net, params, lane, 64);
+    __syncthreads();
+    const int base = blockIdx.x * 16;
+    v4s bx[3];
+    input_frags(feat, base, n, lane, bx);
+    v4f h1[4], h2[4], y[2];
+    forward(net, bx, lane, h1, h2, y);
+    const int b = base + (lane & 15), g = lane >> 4;
+    if (b < n)
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) out[size_t(b) * kOutPad + 16 * m + 4 * g + j] = y[m][j];
+}
+
+// One wave: forward, loss, backward over `tiles` tiles of 16 blocks from
+// block tile0 * 16, accumulating this wave's weight gradients (summed over
+// its blocks) into partial row `blockIdx.x`.  Deterministic: the partials
+// are reduced in a fixed order by mlp_adam_kernel.
+__global__ __launch_bounds__(64) void mlp_train_kernel(const float *__restrict__ params, const uint16_t *__restrict__ feat,
+                                                       const float *__restrict__ labels, int n, int tiles,
+                                                       float *__restrict__ partials, float *__restrict__ loss_out)
+{
+    __shared__ Net net;
+    __shared__ float s_h1[16][kHidden + 1], s_h2[16][kHidden + 1], s_d1[16][kHidden + 1], s_d2[16][kHidden + 1];
+    __shared__ float s_y[16][kOutPad + 1], s_dy[16][kOutPad + 1];
+    const int lane = threadIdx.x, g = lane >> 4, col = lane & 15;
+    load_net(net, params, lane, 64);
+    __syncthreads();
+    const v4f zero = { 0.0f, 0.0f, 0.0f, 0.0f };
+    v4f gW1[4][3], gW2[4][4], gW3[2][4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) gW1[m][t] = zero;
+#pragma unroll
+        for (int t = 0; t < 4; t++) gW2[m][t] = zero;
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) gW3[m][t] = zero;
+    float gb1[4][4] = {}, gb2[4][4] = {}, gb3[2][4] = {};
+    float loss = 0.0f;
+
+    for (int tile = 0; tile < tiles; tile++) {
+        const int base = (blockIdx.x * tiles + tile) * 16;
+        if (base >= n) break;
+        const int b = base + col;
+        const bool valid = b < n;
+        v4s bx[3];
+        input_frags(feat, base, n, lane, bx);
+        v4f h1[4], h2[4], y[2];
+        forward(net, bx, lane, h1, h2, y);
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                s_h1[col][16 * m + 4 * g + j] = valid ? h1[m][j] : 0.0f;
+                s_h2[col][16 * m + 4 * g + j] = valid ? h2[m][j] : 0.0f;
+            }
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) s_y[col][16 * m + 4 * g + j] = y[m][j];
+        __syncthreads();
+        if (lane < 16) {
+            float dy[kOutPad];
+            if (valid) {
+                float yy[kOutPad];
+#pragma unroll
+                for (int i = 0; i < kOutPad; i++) yy[i] = s_y[lane][i];
+                loss += block_loss(yy, labels + size_t(b) * kLabel, dy);
+            } else {
+#pragma unroll
+                for (int i = 0; i < kOutPad; i++) dy[i] = 0.0f;
+            }
+            // The backward pass consumes dy as BF16.
+#pragma unroll
+            for (int i = 0; i < kOutPad; i++) s_dy[lane][i] = bf16_round(dy[i]);
+        }
+        __syncthreads();
+        // delta2 = (W3^T dy) * relu'(h2); delta1 = (W2^T delta2) * relu'(h1).
+        v4s bdy[2];
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            bdy[s] = pack4(s_dy[col][16 * s + 4 * g], s_dy[col][16 * s + 4 * g + 1], s_dy[col][16 * s + 4 * g + 2],
+                           s_dy[col][16 * s + 4 * g + 3]);
+        v4s bd2[4];
+        float d2[4][4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            v4f acc = zero;
+#pragma unroll
+            for (int s = 0; s < 2; s++) acc = mfma(net.frag[kF3T + 2 * m + s][lane], bdy[s], acc);
+#pragma unroll
+            for (int j = 0; j < 4; j++) d2[m][j] = bf16_round(h2[m][j] > 0.0f ? acc[j] : 0.0f);
+            bd2[m] = pack4(d2[m][0], d2[m][1], d2[m][2], d2[m][3]);
+        }
+        float d1[4][4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            v4f acc = zero;
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc = mfma(net.frag[kF2T + 4 * m + s][lane], bd2[s], acc);
+#pragma unroll
+            for (int j = 0; j < 4; j++) d1[m][j] = bf16_round(h1[m][j] > 0.0f ? acc[j] : 0.0f);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                s_d2[col][16 * m + 4 * g + j] = valid ? d2[m][j] : 0.0f;
+                s_d1[col][16 * m + 4 * g + j] = valid ? d1[m][j] : 0.0f;
+                gb2[m][j] += valid ? d2[m][j] : 0.0f;
+                gb1[m][j] += valid ? d1[m][j] : 0.0f;
+            }
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) gb3[m][j] += s_dy[col][16 * m + 4 * g + j];
+        __syncthreads();
+        // Weight gradients, k = the tile's 16 blocks: A = delta (row = output
+        // index l % 16, k = block 4(l/16)+j), B = activation (k = block,
+        // column = input index l % 16).
+        v4s bh2[4], bh1[4], bxT[3];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            bh2[t] = pack4(s_h2[4 * g][16 * t + col], s_h2[4 * g + 1][16 * t + col], s_h2[4 * g + 2][16 * t + col],
+                           s_h2[4 * g + 3][16 * t + col]);
+            bh1[t] = pack4(s_h1[4 * g][16 * t + col], s_h1[4 * g + 1][16 * t + col], s_h1[4 * g + 2][16 * t + col],
+                           s_h1[4 * g + 3][16 * t + col]);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            v4s v;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int bb = base + 4 * g + j;
+                v[j] = bb < n ? short(feat[size_t(bb) * kIn + 16 * t + col]) : short(0);
+            }
+            bxT[t] = v;
+        }
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const v4s a = pack4(s_dy[4 * g][16 * m + col], s_dy[4 * g + 1][16 * m + col], s_dy[4 * g + 2][16 * m + col],
+                                s_dy[4 * g + 3][16 * m + col]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) gW3[m][t] = mfma(a, bh2[t], gW3[m][t]);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const v4s a2 = pack4(s_d2[4 * g][16 * m + col], s_d2[4 * g + 1][16 * m + col], s_d2[4 * g + 2][16 * m + col],
+                                 s_d2[4 * g + 3][16 * m + col]);
+            const v4s a1 = pack4(s_d1[4 * g][16 * m + col], s_d1[4 * g + 1][16 * m + col], s_d1[4 * g + 2][16 * m + col],
+                                 s_d1[4 * g + 3][16 * m + col]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) gW2[m][t] = mfma(a2, bh1[t], gW2[m][t]);
+#pragma unroll
+            for (int t = 0; t < 3; t++) gW1[m][t] = mfma(a1, bxT[t], gW1[m][t]);
+        }
+        __syncthreads();
+    }
+    // Partial row: the parameter layout, D fragments (row 16m + 4(l/16) + j,
+    // column 16t + l % 16).
+    float *P = partials + size_t(blockIdx.x) * kParams;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 16 * m + 4 * g + j;
+#pragma unroll
+            for (int t = 0; t < 3; t++) P[kW1 + r * kIn + 16 * t + col] = gW1[m][t][j];
+#pragma unroll
+            for (int t = 0; t < 4; t++) P[kW2 + r * kHidden + 16 * t + col] = gW2[m][t][j];
+        }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 16 * m + 4 * g + j;
+            if (r < kOut)
+#pragma unroll
+                for (int t = 0; t < 4; t++) P[kW3 + r * kHidden + 16 * t + col] = gW3[m][t][j];
+        }
+    // Bias gradients: sums over the 16 columns (lanes l % 16) of each row.
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                gb1[m][j] += __shfl_xor(gb1[m][j], off);
+                gb2[m][j] += __shfl_xor(gb2[m][j], off);
+            }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) gb3[m][j] += __shfl_xor(gb3[m][j], off);
+    if (col == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                P[kB1 + 16 * m + 4 * g + j] = gb1[m][j];
+                P[kB2 + 16 * m + 4 * g + j] = gb2[m][j];
+            }
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (16 * m + 4 * g + j < kOut) P[kB3 + 16 * m + 4 * g + j] = gb3[m][j];
+    }
+    for (int off = 1; off < 64; off <<= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) loss_out[blockIdx.x] = loss;
+}
+
+// Mean gradient over the n blocks (partials summed in row order) and, when
+// lr > 0, one Adam step on the FP32 parameters.
+__global__ __launch_bounds__(256) void mlp_adam_kernel(float *__restrict__ params, float *__restrict__ m1,
+                                                       float *__restrict__ m2, const float *__restrict__ partials,
+                                                       int rows, float inv_n, float lr, float beta1, float beta2,
+                                                       float eps, float c1, float c2, float *__restrict__ grad_out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kParams) return;
+    float gsum = 0.0f;
+    for (int r = 0; r < rows; r++) gsum += partials[size_t(r) * kParams + p];
+    const float gr = gsum * inv_n;
+    if (grad_out) grad_out[p] = gr;
+    if (lr > 0.0f) {
+        const float a = beta1 * m1[p] + (1.0f - beta1) * gr;
+        const float b = beta2 * m2[p] + (1.0f - beta2) * gr * gr;
+        m1[p] = a;
+        m2[p] = b;
+        params[p] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
+    }
+}
+
+// ------------------------------------------------------------ compression --
+
+// Least-squares refinement of one endpoint pair for the block's texels
+// (4-colour index by projection onto the segment, then the 2x2 normal
+// equations), `iters` times; endpoints stay in [0, 1].
+__device__ __forceinline__ void refine(const float *tb, float a[3], float b[3], int iters)
+{
+    for (int it = 0; it < iters; it++) {
+        const float d[3] = { b[0] - a[0], b[1] - a[1], b[2] - a[2] };
+        const float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const float il = len2 > 0.0f ? 1.0f / len2 : 0.0f;
+        float saa = 0.0f, sbb = 0.0f, sab = 0.0f, sac[3] = { 0, 0, 0 }, sbc[3] = { 0, 0, 0 };
+        for (int t = 0; t < 16; t++) {
+            const float t0 = tb[3 * t], t1 = tb[3 * t + 1], t2 = tb[3 * t + 2];
+            if (t0 < 0.0f) continue;
+            const float u = ((t0 - a[0]) * d[0] + (t1 - a[1]) * d[1] + (t2 - a[2]) * d[2]) * il;
+            const float q = fminf(fmaxf(rintf(3.0f * u), 0.0f), 3.0f);
+            const float be = q * (1.0f / 3.0f), al = 1.0f - be;
+            saa += al * al; sbb += be * be; sab += al * be;
+            sac[0] += al * t0; sac[1] += al * t1; sac[2] += al * t2;
+            sbc[0] += be * t0; sbc[1] += be * t1; sbc[2] += be * t2;
+        }
+        const float det = saa * sbb - sab * sab;
+        if (!(det > 1e-8f)) break;
+        const float id = 1.0f / det;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            a[k] = fminf(fmaxf((sbb * sac[k] - sab * sbc[k]) * id, 0.0f), 1.0f);
+            b[k] = fminf(fmaxf((saa * sbc[k] - sab * sac[k]) * id, 0.0f), 1.0f);
+        }
+    }
+}
+
+// Error (squared, 8-bit units, evaluate_mse's measure) and index bits of the
+// block with 565 endpoints c0, c1: four-colour order when c0 > c1 (the
+// caller orders them), each texel's nearest palette entry.
+__device__ __forceinline__ float eval_pair(const float *tb, uint32_t c0, uint32_t c1, uint32_t *bits_out)
+{
+    float p[4][3];
+    bc1_palette(c0, c1, p);
+    float err = 0.0f;
+    uint32_t bits = 0;
+    for (int t = 0; t < 16; t++) {
+        const float t0 = tb[3 * t], t1 = tb[3 * t + 1], t2 = tb[3 * t + 2];
+        if (t0 < 0.0f) continue;
+        float best = FLT_MAX;
+        uint32_t bi = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float dr = (p[i][0] - t0) * 255.0f, dg = (p[i][1] - t1) * 255.0f, db = (p[i][2] - t2) * 255.0f;
+            const float e = dr * dr + dg * dg + db * db;
+            if (e < best) { best = e; bi = uint32_t(i); }
+        }
+        err += best;
+        bits |= bi << (2 * t);
+    }
+    if (bits_out) *bits_out = bits;
+    return err;
+}
+
+// Quantises (a, b) as icbc does (vector3_to_color16), orders the endpoints
+// for four colours and returns the block and its error.
+__device__ __forceinline__ float encode_pair(const float *tb, const float a[3], const float b[3], const float *mid5,
+                                             const float *mid6, uint2 *blk)
+{
+    uint32_t c0 = ntx_bc1::to565(a[0], a[1], a[2], mid5, mid6);
+    uint32_t c1 = ntx_bc1::to565(b[0], b[1], b[2], mid5, mid6);
+    if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; }
+    uint32_t bits;
+    const float err = eval_pair(tb, c0, c1, &bits);
+    blk->x = c0 | (c1 << 16);
+    blk->y = bits;
+    return err;
+}
+
+// One wave per 16 blocks at a time, 4 waves per workgroup.  Lane (g, n) of a
+// tile gathers row g of block n, the network runs on the matrix cores, and
+// then lane (g, n) refines candidate g of block n; the four candidates of a
+// block are compared across lanes.  A block whose predicted cluster-fit RMSE
+// exceeds tau_pred, or whose best candidate is worse than that prediction by
+// more than tau_gap (both in 8-bit units), is queued for the cluster fit.
+// box_candidates: start instead from the four diagonals of the block's
+// colour bounding box (the same refinement without the network; no
+// prediction, so only tau_pred < 0 sends blocks to the fallback).
+__global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
+                                                      int w, int h, int bw, int n, const ntx::Tables *__restrict__ tab,
+                                                      float tau_pred, float tau_gap, int refine_iters, int greedy_rounds,
+                                                      int box_candidates,
+                                                      uint2 *__restrict__ out, int *__restrict__ fb_count,
+                                                      int *__restrict__ fb_list, float *__restrict__ err_out)
+{
+    __shared__ Net net;
+    __shared__ float s_tex[4][16][49];  // 16 texels x RGB per block, padded row
+    __shared__ float s_y[4][16][kOutPad + 1];
+    __shared__ float4 s_aux[4][16];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, col = lane & 15;
+    load_net(net, params, t, 256);
+    __syncthreads();
+    const size_t plane = size_t(w) * size_t(h);
+    const int tiles = (n + 15) / 16;
+    // Trip count uniform over the workgroup (the loop body synchronises it);
+    // a wave past the last tile computes on no valid block.
+    for (int group = blockIdx.x; group * 4 < tiles; group += gridDim.x) {
+        const int tile = group * 4 + wave;
+        const int b = tile * 16 + col;
+        const bool valid = b < n;
+        const int bx = valid ? b % bw : 0, by = valid ? b / bw : 0;
+        // Row g of block b, and the block mean / scale over the texels inside.
+        float c[3][4];
+        bool inr[4];
+        float sum[3] = { 0, 0, 0 };
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int x = 4 * bx + j, y = 4 * by + g;
+            inr[j] = valid && x < w && y < h;
+            const size_t o = inr[j] ? size_t(y) * size_t(w) + size_t(x) : 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                c[k][j] = inr[j] ? src[o + k * plane] : 0.0f;
+                sum[k] += c[k][j];
+            }
+            cnt += inr[j] ? 1 : 0;
+        }
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sum[k] += __shfl_xor(sum[k], off);
+            cnt += __shfl_xor(cnt, off);
+        }
+        float m[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) m[k] = cnt > 0 ? sum[k] / float(cnt) : 0.0f;
+        float d = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (inr[j]) d = fmaxf(d, fabsf(c[k][j] - m[k]));
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) d = fmaxf(d, __shfl_xor(d, off));
+        const float s = 1.0f / fmaxf(d, kMinSpread);
+        v4s bx3[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            bx3[k] = pack4(inr[0] ? (c[k][0] - m[k]) * s : 0.0f, inr[1] ? (c[k][1] - m[k]) * s : 0.0f,
+                           inr[2] ? (c[k][2] - m[k]) * s : 0.0f, inr[3] ? (c[k][3] - m[k]) * s : 0.0f);
+        v4f h1[4], h2[4], yv[2];
+        forward(net, bx3, lane, h1, h2, yv);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) s_tex[wave][col][3 * (4 * g + j) + k] = inr[j] ? c[k][j] : -1.0f;
+#pragma unroll
+            for (int mm = 0; mm < 2; mm++) s_y[wave][col][16 * mm + 4 * g + j] = yv[mm][j];
+        }
+        if (g == 0) s_aux[wave][col] = make_float4(m[0], m[1], m[2], s);
+        __syncthreads();
+        // Lane (g, col): candidate g of block col, texels read from LDS.
+        const float *tb = s_tex[wave][col];
+        const float4 ax = s_aux[wave][col];
+        const float mm3[3] = { ax.x, ax.y, ax.z };
+        const float is = 1.0f / ax.w;
+        float a[3], bb[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            a[k] = fminf(fmaxf(mm3[k] + s_y[wave][col][6 * g + k] * is, 0.0f), 1.0f);
+            bb[k] = fminf(fmaxf(mm3[k] + s_y[wave][col][6 * g + 3 + k] * is, 0.0f), 1.0f);
+        }
+        float pred = fmaxf(s_y[wave][col][24], 0.0f) * is * 255.0f;  // predicted RMSE, 8-bit units
+        if (box_candidates) {
+            float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
+            for (int i = 0; i < 16; i++)
+                if (tb[3 * i] >= 0.0f)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) { lo[k] = fminf(lo[k], tb[3 * i + k]); hi[k] = fmaxf(hi[k], tb[3 * i + k]); }
+            // diagonal g: channel g - 1 runs against the others (g = 0: all together)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const bool flip = g == k + 1;
+                a[k] = flip ? hi[k] : lo[k];
+                bb[k] = flip ? lo[k] : hi[k];
+            }
+            pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
+        }
+        refine(tb, a, bb, refine_iters);
+        uint2 blk;
+        float err = valid ? encode_pair(tb, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
+        // Best of the four candidates (lower candidate on equal error).
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) {
+            const float oe = __shfl_xor(err, off);
+            const uint32_t ox = __shfl_xor(blk.x, off), oy = __shfl_xor(blk.y, off);
+            const bool lower = ((lane ^ off) >> 4) < g;
+            if (oe < err || (oe == err && lower)) { err = oe; blk.x = ox; blk.y = oy; }
+        }
+        // A block whose texels inside the image are all equal gets what the
+        // cluster fit gives it: reduce_colors' running weighted mean of the
+        // equal colours (icbc.h:1638-1683, weights 1) and the single-colour
+        // tables.
+        // A block whose texels inside the image are all equal gets what the
+        // cluster fit gives it: reduce_colors' running weighted mean of the
+        // equal colours (icbc.h:1638-1683, weights 1) and the single-colour
+        // tables.  Texel 0 is always inside the image.
+        bool uniform = valid;
+        for (int i = 1; i < 16; i++)
+            if (tb[3 * i] >= 0.0f && (tb[3 * i] != tb[0] || tb[3 * i + 1] != tb[1] || tb[3 * i + 2] != tb[2]))
+                uniform = false;
+        if (uniform) {
+            float kr = tb[0], kg = tb[1], kb = tb[2], kw = 1.0f;
+            for (int i = 1; i < 16; i++) {
+                if (tb[3 * i] < 0.0f) continue;
+                const float s1 = kw + 1.0f;
+                kr = (kr * kw + tb[3 * i]) / s1;
+                kg = (kg * kw + tb[3 * i + 1]) / s1;
+                kb = (kb * kw + tb[3 * i + 2]) / s1;
+                kw = s1;
+            }
+            blk = ntx_bc1::single_color_block(tab, kr, kg, kb);
+        }
+        // Greedy search on the quantised endpoints (+-1 in one 565 channel of
+        // one endpoint; the 12 moves spread 3 per lane over the block's lanes),
+        // taking the best improving move each round, lowest move on ties.
+        // q[3e + c]: channel c (5, 6, 5 bits) of endpoint e; move mv changes
+        // q[mv >> 1] by -1 (mv even) or +1 (mv odd).
+        int q[6];
+        {
+            const uint32_t e0 = blk.x & 0xffffu, e1 = blk.x >> 16;
+            q[0] = int(e0 >> 11); q[1] = int((e0 >> 5) & 63u); q[2] = int(e0 & 31u);
+            q[3] = int(e1 >> 11); q[4] = int((e1 >> 5) & 63u); q[5] = int(e1 & 31u);
+        }
+        const int rounds = uniform ? 0 : greedy_rounds;  // uniform across the block's lanes
+        for (int round = 0; round < rounds; round++) {
+            float be = err;
+            int bm = 64;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int mv = 3 * g + i, slot = mv >> 1, step = (mv & 1) ? 1 : -1;
+                int t2[6];
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    t2[k] = q[k] + (k == slot ? step : 0);
+                    ok = ok && t2[k] >= 0 && t2[k] <= ((k % 3) == 1 ? 63 : 31);
+                }
+                if (!ok) continue;
+                uint32_t c0 = uint32_t(t2[0]) << 11 | uint32_t(t2[1]) << 5 | uint32_t(t2[2]);
+                uint32_t c1 = uint32_t(t2[3]) << 11 | uint32_t(t2[4]) << 5 | uint32_t(t2[5]);
+                if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
+                const float e = valid ? eval_pair(tb, c0, c1, nullptr) : FLT_MAX;
+                if (e < be) { be = e; bm = mv; }
+            }
+#pragma unroll
+            for (int off = 16; off < 64; off <<= 1) {
+                const float oe = __shfl_xor(be, off);
+                const int om = __shfl_xor(bm, off);
+                if (oe < be || (oe == be && om < bm)) { be = oe; bm = om; }
+            }
+            if (bm < 64) {
+#pragma unroll
+                for (int k = 0; k < 6; k++) q[k] += k == (bm >> 1) ? ((bm & 1) ? 1 : -1) : 0;
+                err = be;
+            }
+        }
+        if (rounds > 0) {
+            uint32_t c0 = uint32_t(q[0]) << 11 | uint32_t(q[1]) << 5 | uint32_t(q[2]);
+            uint32_t c1 = uint32_t(q[3]) << 11 | uint32_t(q[4]) << 5 | uint32_t(q[5]);
+            if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
+            uint32_t bits = 0;
+            if (valid) err = eval_pair(tb, c0, c1, &bits);
+            blk.x = c0 | (c1 << 16);
+            blk.y = bits;
+        }
+        if (g == 0 && valid) {
+            int cntb = 0;
+            for (int i = 0; i < 16; i++) cntb += tb[3 * i] >= 0.0f ? 1 : 0;
+            const float rmse = sqrtf(err / float(3 * cntb));
+            const bool fallback = !uniform && (pred > tau_pred || (tau_gap >= 0.0f && rmse > pred + tau_gap));
+            out[b] = blk;
+            if (err_out) err_out[b] = rmse;
+            if (fallback) fb_list[atomicAdd(fb_count, 1)] = b;
+        }
+        __syncthreads();
+    }
+}
+
+// The full cluster fit on a list of blocks (the fallback), 16 blocks per
+// workgroup pass; count is read on the device.
+__global__ __launch_bounds__(256) void bc1_list_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                       const int *__restrict__ count, const int *__restrict__ list,
+                                                       const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out)
+{
+    __shared__ ntx_bc1::Bc1Shared sh;
+    for (int i = threadIdx.x; i < 976; i += blockDim.x) sh.four[i] = tab->four[i];
+    for (int i = threadIdx.x; i < 160; i += blockDim.x) sh.three[i] = tab->three[i];
+    __syncthreads();
+    const int total = *count;
+    const int g = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    for (int i = blockIdx.x * 16 + g; i < total; i += gridDim.x * 16) {
+        const int b = list[i];
+        int nn, mm;
+        const uint2 r = ntx_bc1::encode_block(sh, src, w, h, b % bw, b / bw, NTX_ALPHA_NONE, 1.0f, 1.0f, 1.0f, tab, g,
+                                              lane, &nn, &mm);
+        if (lane == 0) out[b] = r;
+    }
+}
+
+}  // namespace
+
+namespace ntx {
+
+hipError_t launch_mlp_features(const float *d_src, int w, int h, void *d_feat, float *d_aux, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, n = bw * ((h + 3) / 4);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_features_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, n,
+                       static_cast<uint16_t *>(d_feat), reinterpret_cast<float4 *>(d_aux));
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_labels(const float *d_src, int w, int h, const void *d_bc1, const float *d_aux, float *d_labels,
+                             hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, n = bw * ((h + 3) / 4);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_labels_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, n,
+                       static_cast<const uint2 *>(d_bc1), reinterpret_cast<const float4 *>(d_aux), d_labels);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float *d_params, const void *d_feat, int n, float *d_out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 15) / 16), dim3(64), 0, stream, d_params,
+                       static_cast<const uint16_t *>(d_feat), n, d_out);
+    return hipGetLastError();
+}
+
+// Rows of partial gradients a training step over n blocks writes.
+int mlp_train_rows(int n, int tiles_per_wave) { return (((n + 15) / 16) + tiles_per_wave - 1) / tiles_per_wave; }
+
+hipError_t launch_mlp_train(const float *d_params, const void *d_feat, const float *d_labels, int n, int tiles_per_wave,
+                            float *d_partials, float *d_loss, hipStream_t stream)
+{
+    const int rows = mlp_train_rows(n, tiles_per_wave);
+    hipLaunchKernelGGL(mlp_train_kernel, dim3(rows), dim3(64), 0, stream, d_params, static_cast<const uint16_t *>(d_feat),
+                       d_labels, n, tiles_per_wave, d_partials, d_loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float *d_params, float *d_m1, float *d_m2, const float *d_partials, int rows, int n, float lr,
+                           int step, float *d_grad, hipStream_t stream)
+{
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f / (1.0f - powf(beta1, float(step))), c2 = 1.0f / (1.0f - powf(beta2, float(step)));
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((ntx_mlp::kParams + 255) / 256), dim3(256), 0, stream, d_params, d_m1, d_m2,
+                       d_partials, rows, 1.0f / float(n), lr, beta1, beta2, eps, c1, c2, d_grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc1_mlp(const float *d_params, const float *d_src, int w, int h, const Tables *d_tab, float tau_pred,
+                          float tau_gap, int refine_iters, int greedy_rounds, int box_candidates, void *d_out,
+                          int *d_fb_count, int *d_fb_list, float *d_err, int workgroups, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, n = bw * ((h + 3) / 4);
+    if (n == 0) return hipSuccess;
+    const int tiles = (n + 15) / 16;
+    const int grid = std::max(1, std::min(workgroups, (tiles + 3) / 4));
+    hipLaunchKernelGGL(bc1_mlp_kernel, dim3(grid), dim3(256), 0, stream, d_params, d_src, w, h, bw, n, d_tab, tau_pred,
+                       tau_gap, refine_iters, greedy_rounds, box_candidates, static_cast<uint2 *>(d_out), d_fb_count,
+                       d_fb_list, d_err);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc1_list(const float *d_src, int w, int h, const int *d_count, const int *d_list, int max_blocks,
+                           const Tables *d_tab, void *d_out, int workgroups, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4;
+    if (max_blocks <= 0) return hipSuccess;
+    const int grid = std::max(1, std::min(workgroups, (max_blocks + 15) / 16));
+    hipLaunchKernelGGL(bc1_list_kernel, dim3(grid), dim3(256), 0, stream, d_src, w, h, bw, d_count, d_list, d_tab,
+                       static_cast<uint2 *>(d_out));
+    return hipGetLastError();
+}
+
+}  // namespace ntx

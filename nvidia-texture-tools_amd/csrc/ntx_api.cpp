@@ -319,7 +319,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
     static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_search_kernel", "channel_kernel",
                                                    "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
                                                    "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
-                                                   "bc3_classify_kernel", "batch_kernel" };
+                                                   "bc3_classify_kernel", "batch_kernel", "bc1_mlp_kernel",
+                                                   "bc1_list_kernel", "mlp_train_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 

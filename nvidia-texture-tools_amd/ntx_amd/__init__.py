@@ -29,7 +29,8 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 
 # ntx_hip.h kernel ids for ntx_hip_profile_read.
 KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
-           "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_classify_kernel", "batch_kernel")
+           "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_classify_kernel", "batch_kernel",
+           "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
@@ -45,7 +46,13 @@ EXPORTS = (
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
     "ntx_hip_batch_autotune",
+    "ntx_hip_mlp_create", "ntx_hip_mlp_destroy", "ntx_hip_mlp_get_params", "ntx_hip_mlp_set_params",
+    "ntx_hip_mlp_features_device", "ntx_hip_mlp_labels_device", "ntx_hip_mlp_forward_device", "ntx_hip_mlp_train_step",
+    "ntx_hip_compress_bc1_mlp_device",
 )
+
+MLP_PARAMS = 8921          # NTX_MLP_PARAMS
+MLP_BOX_CANDIDATES = 1     # NTX_MLP_BOX_CANDIDATES
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -124,6 +131,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_batch_get_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
         "ntx_hip_batch_set_tuning": (_I, [_P, ctypes.POINTER(BatchTuning)]),
         "ntx_hip_batch_autotune": (_I, [_P, _I, ctypes.POINTER(BatchTuning), ctypes.POINTER(_F)]),
+        "ntx_hip_mlp_create": (_I, [_P, _P, ctypes.c_uint32, ctypes.POINTER(_P)]),
+        "ntx_hip_mlp_destroy": (_I, [_P]),
+        "ntx_hip_mlp_get_params": (_I, [_P, _P]),
+        "ntx_hip_mlp_set_params": (_I, [_P, _P]),
+        "ntx_hip_mlp_features_device": (_I, [_P, _P, _I, _I, _P, _P]),
+        "ntx_hip_mlp_labels_device": (_I, [_P, _P, _I, _I, _P, _P, _P]),
+        "ntx_hip_mlp_forward_device": (_I, [_P, _P, _I, _P]),
+        "ntx_hip_mlp_train_step": (_I, [_P, _P, _P, _I, _F, ctypes.POINTER(_F), _P]),
+        "ntx_hip_compress_bc1_mlp_device": (_I, [_P, _P, _I, _I, _F, _F, _I, _I, _I, _P, ctypes.POINTER(_I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -384,6 +400,82 @@ class Batch:
         self.ctx._check(self.ctx.lib.ntx_hip_batch_autotune(self.handle, reps, ctypes.byref(t), ctypes.byref(ms)),
                         "ntx_hip_batch_autotune")
         return t.as_dict(), float(ms.value)
+
+
+class Mlp:
+    """The BC1 endpoint-proposal network (ntx_hip_mlp_*): parameters and Adam
+    state live on the device; training and inference run the gfx950 kernels."""
+
+    def __init__(self, ctx: Context, params: Optional[np.ndarray] = None, seed: int = 1):
+        self.ctx = ctx
+        h = _P()
+        pp = None if params is None else _f32c(params)
+        if pp is not None:
+            assert pp.size == MLP_PARAMS
+        ctx._check(ctx.lib.ntx_hip_mlp_create(ctx.handle, _ptr(pp) if pp is not None else None, seed, ctypes.byref(h)),
+                   "ntx_hip_mlp_create")
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.ntx_hip_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(MLP_PARAMS, np.float32)
+        self.ctx._check(self.ctx.lib.ntx_hip_mlp_get_params(self.handle, _ptr(out)), "mlp_get_params")
+        return out
+
+    def set_params(self, params: np.ndarray) -> None:
+        pp = _f32c(params)
+        assert pp.size == MLP_PARAMS
+        self.ctx._check(self.ctx.lib.ntx_hip_mlp_set_params(self.handle, _ptr(pp)), "mlp_set_params")
+
+    def forward(self, d_features: int, n: int) -> np.ndarray:
+        """Network outputs (n, 32) for n blocks' features at device address d_features."""
+        out = self.ctx.alloc(4 * 32 * max(n, 1))
+        self.ctx._check(self.ctx.lib.ntx_hip_mlp_forward_device(self.handle, d_features, n, out.ptr), "mlp_forward")
+        res = out.download(np.float32, 32 * n).reshape(n, 32)
+        out.free()
+        return res
+
+    def train_step(self, d_features: int, d_labels: int, n: int, lr: float, want_grad: bool = False):
+        """(mean loss, mean gradient or None); lr = 0 computes without updating."""
+        loss = _F()
+        grad = np.empty(MLP_PARAMS, np.float32) if want_grad else None
+        self.ctx._check(self.ctx.lib.ntx_hip_mlp_train_step(self.handle, d_features, d_labels, n, float(lr),
+                                                            ctypes.byref(loss), _ptr(grad) if want_grad else None),
+                        "mlp_train_step")
+        return float(loss.value), grad
+
+    def compress_bc1_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer, tau_pred: float = 4.0,
+                            tau_gap: float = -1.0, refine_iters: int = 2, greedy_rounds: int = 2,
+                            box_candidates: bool = False, count_fallback: bool = True) -> Optional[int]:
+        fb = _I()
+        st = self.ctx.lib.ntx_hip_compress_bc1_mlp_device(self.handle, d_rgba.ptr, w, h, float(tau_pred), float(tau_gap),
+                                                          int(refine_iters), int(greedy_rounds),
+                                                          MLP_BOX_CANDIDATES if box_candidates else 0,
+                                                          d_out.ptr, ctypes.byref(fb) if count_fallback else None)
+        self.ctx._check(st, "compress_bc1_mlp")
+        return fb.value if count_fallback else None
+
+
+def mlp_features_device(ctx: Context, d_rgba: DeviceBuffer, w: int, h: int, d_features: DeviceBuffer,
+                        d_aux: DeviceBuffer) -> None:
+    ctx._check(ctx.lib.ntx_hip_mlp_features_device(ctx.handle, d_rgba.ptr, w, h, d_features.ptr, d_aux.ptr),
+               "mlp_features")
+
+
+def mlp_labels_device(ctx: Context, d_rgba: DeviceBuffer, w: int, h: int, d_bc1: DeviceBuffer, d_aux: DeviceBuffer,
+                      d_labels: DeviceBuffer) -> None:
+    ctx._check(ctx.lib.ntx_hip_mlp_labels_device(ctx.handle, d_rgba.ptr, w, h, d_bc1.ptr, d_aux.ptr, d_labels.ptr),
+               "mlp_labels")
 
 
 def mip_sizes(w: int, h: int):
