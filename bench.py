@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-workload leg")
     ap.add_argument("--batch-textures", type=int, default=256)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal BC1 and cube-map legs")
+    ap.add_argument("--learned-size", type=int, default=16384)
     ap.add_argument("--profile-pass", action="store_true",
                     help="short run for rocprofv3 counter passes: the BC3 chain, the Kaiser chain, the BC1 chain "
                          "(2 steps each) and a 256 MiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -234,6 +236,10 @@ def main():
 
     if not args.no_batch:
         result["batch"] = bench_batch(ctx, args)
+    if not args.no_learned:
+        result["learned_bc1"], net = bench_learned(ctx, args)
+        result["cube_map"] = bench_cube(ctx, args, net)
+        net.close()
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -386,6 +392,194 @@ def bench_batch(ctx, args) -> dict:
         "levels_identical_to_per_image_path": f"{same}/{nlev}",
         "kernels_ms_per_run": {k: round(v[0] / steps, 4) for k, v in prof.items()},
     }
+
+
+def bench_learned(ctx, args):
+    """Learned endpoint proposals for BC1: the 48-64-64-25 network (BF16, FP32
+    accumulation) trained on the device on ~8M blocks cut from seeded
+    noise+rectangle textures (8 x 2048^2, each in 4 rotations) labelled by the
+    cluster fit; then a 16384^2 input compressed by the plain cluster fit and
+    by the proposal form, PSNR of both measured on the device."""
+    import ntx_amd as N
+    from ntx_amd import learned, synth
+
+    t0 = time.perf_counter()
+    texs = []
+    for i in range(8):
+        img = synth.noise_rect_rgba(2048, 2048, seed=args.seed + 2000 + i)
+        for r in range(4):
+            texs.append((synth.to_planar(np.ascontiguousarray(np.rot90(img, r))).reshape(-1), 2048, 2048))
+    cut = sum(((w + 3) // 4) * ((h + 3) // 4) for _, w, h in texs)
+    data = learned.make_dataset(ctx, texs, seed=args.seed)
+    del texs
+    data_s = time.perf_counter() - t0
+    net = N.Mlp(ctx, seed=args.seed)
+    log(f"learned: training on {data.n} non-uniform blocks of {cut} cut")
+    first, _ = net.train_step(data.feat(0), data.label(0), min(data.n, 65536), 0.0)
+    t0 = time.perf_counter()
+    epochs, batch = 3, 16384
+    losses = learned.train(net, data, epochs=epochs, batch=batch, lr=3e-3)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    steps = epochs * ((data.n + batch - 1) // batch)
+    data.free()
+
+    S = args.learned_size
+    T = S // 4
+    log(f"learned: {S}x{S} input from 4 generated {T}x{T} textures in 4 rotations")
+    base = [synth.noise_rect_rgba(T, T, seed=args.seed + 3000 + i) for i in range(4)]
+    big = np.empty((S, S, 4), np.uint8)
+    for k in range(16):
+        ty, tx = divmod(k, 4)
+        big[ty * T:(ty + 1) * T, tx * T:(tx + 1) * T] = np.rot90(base[k % 4], k // 4)
+    del base
+    bgra = synth.to_bgra(big)
+    del big
+    d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
+    del bgra
+    d = ctx.alloc(16 * S * S)
+    ctx.unpack_bgra8_device(d_bgra, S, S, d)
+    d_bgra.free()
+    d_plain = ctx.alloc(N.compressed_size(N.FORMAT_BC1, S, S))
+    d_mlp = ctx.alloc(N.compressed_size(N.FORMAT_BC1, S, S))
+    texels = S * S
+
+    def timed(fn, reps=3):
+        fn()
+        ctx.synchronize()
+        best = None
+        for _ in range(reps):
+            ctx.timer_begin()
+            fn()
+            ms = ctx.timer_end()
+            best = ms if best is None else min(best, ms)
+        return best
+
+    ms_plain = timed(lambda: ctx.compress_device(N.FORMAT_BC1, d, S, S, d_plain))
+    psnr_plain = ctx.bc1_psnr_device(d, S, S, d_plain)
+    runs = []
+    for cfg in (dict(greedy_rounds=1), dict(greedy_rounds=1, refine_iters=1), dict(greedy_rounds=1, refine_iters=0),
+                dict(greedy_rounds=1, tau_gap=0.5), dict(greedy_rounds=2),
+                dict(greedy_rounds=1, box_candidates=True), dict(greedy_rounds=1, refine_iters=1, box_candidates=True),
+                dict(greedy_rounds=1, refine_iters=0, box_candidates=True), dict(greedy_rounds=2, box_candidates=True)):
+        kw = dict(tau_pred=1e9, tau_gap=-1.0, refine_iters=2)
+        kw.update(cfg)
+        fb = net.compress_bc1_device(d, S, S, d_mlp, **kw)
+        ms = timed(lambda: net.compress_bc1_device(d, S, S, d_mlp, count_fallback=False, **kw))
+        p = ctx.bc1_psnr_device(d, S, S, d_mlp)
+        runs.append({"proposals": "box diagonals (no network)" if kw.get("box_candidates") else "network",
+                     "refine_iters": kw["refine_iters"], "greedy_rounds": kw["greedy_rounds"], "tau_gap": kw["tau_gap"],
+                     "ms": round(ms, 3),
+                     "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1), "speedup": round(ms_plain / ms, 3),
+                     "psnr_db": round(p, 4), "delta_db": round(p - psnr_plain, 4),
+                     "fallback_fraction": round(fb / (texels / 16), 4)})
+        log(f"  {runs[-1]}")
+    d.free()
+    d_plain.free()
+    d_mlp.free()
+    # Headline: the fastest network setting within 0.05 dB of the plain fit
+    # (else the best PSNR); the same for the box method without the network.
+    def pick(rs):
+        ok = [r for r in rs if r["delta_db"] >= -0.05]
+        return max(ok, key=lambda r: r["speedup"]) if ok else max(rs, key=lambda r: r["psnr_db"])
+    head = pick([r for r in runs if r["proposals"] == "network"])
+    nonet = pick([r for r in runs if r["proposals"] != "network"])
+    return {
+        "workload": f"{S}x{S} RGBA8 noise+rectangles (16 tiles: 4 seeded {T}^2 textures x 4 rotations) -> BC1; "
+                    f"network 48-64-64-25 BF16/FP32 trained on the device on {cut} blocks cut from 32 seeded 2048^2 "
+                    f"textures ({data.n} non-uniform used)",
+        "plain_cluster_fit": {"ms": round(ms_plain, 3), "mtexels_s": round(texels / (ms_plain / 1e3) / 1e6, 1),
+                              "psnr_db": round(psnr_plain, 4)},
+        "learned": head,
+        "without_network": nonet,
+        "target": "learned >= 3x plain at PSNR within 0.05 dB",
+        "meets_target": bool(head["speedup"] >= 3.0 and head["delta_db"] >= -0.05),
+        "settings": runs,
+        "training": {"blocks": data.n, "epochs": epochs, "batch": batch, "steps": steps, "seconds": round(train_s, 3),
+                     "data_seconds": round(data_s, 2), "loss_before": round(first, 5),
+                     "loss_per_epoch": [round(x, 5) for x in losses]},
+    }, net
+
+
+def _cube_face(face: int, S: int, seed: int) -> np.ndarray:
+    """One 4096^2-style cube face: seeded value noise plus smooth gradients of
+    the unit direction, which are continuous across face edges."""
+    from ntx_amd import synth
+    c = (np.arange(S, dtype=np.float32) + 0.5) / S * 2 - 1
+    u, v = np.meshgrid(c, c)
+    one = np.ones_like(u)
+    # (+x, -x, +y, -y, +z, -z) faces, directions as in the D3D cube map layout
+    d = [np.stack([one, -v, -u]), np.stack([-one, -v, u]), np.stack([u, one, v]),
+         np.stack([u, -one, -v]), np.stack([u, -v, one]), np.stack([-u, -v, -one])][face]
+    d = d / np.sqrt((d * d).sum(axis=0, keepdims=True))
+    rng = np.random.default_rng(seed + face)
+    img = np.empty((S, S, 4), np.uint8)
+    axes = ((1.0, 0.3, 0.0), (0.0, 1.0, 0.5), (0.4, 0.0, 1.0))
+    for ch in range(3):
+        grad = 0.5 + 0.5 * (axes[ch][0] * d[0] + axes[ch][1] * d[1] + axes[ch][2] * d[2]) / np.linalg.norm(axes[ch])
+        noise = synth.fbm(rng, S, S)
+        img[:, :, ch] = np.floor(np.clip(0.6 * grad + 0.4 * noise, 0, 1) * 255 + 0.5).astype(np.uint8)
+    img[:, :, 3] = 255
+    return img
+
+
+def bench_cube(ctx, args, net):
+    """Cube map: 6 faces of 4096^2, full 13-level Kaiser chains per face,
+    every level compressed to BC1 by the plain cluster fit and by the learned
+    proposal form; per-level PSNR (mean over faces) and Mtexels/s of both."""
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    S = 4096
+    levels = N.mip_sizes(S, S)
+    log("cube map: generating 6 faces")
+    lv = []
+    for f in range(6):
+        bgra = synth.to_bgra(_cube_face(f, S, args.seed + 4000))
+        d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
+        chain = [ctx.alloc(16 * w * h) for w, h in levels]
+        ctx.unpack_bgra8_device(d_bgra, S, S, chain[0])
+        d_bgra.free()
+        lv.append(chain)
+    ctx.synchronize()
+    ctx.timer_begin()
+    for chain in lv:
+        for k in range(len(levels) - 1):
+            w, h = levels[k]
+            ctx.downsample_device(N.FILTER_KAISER, chain[k], w, h, chain[k + 1], 3.0, (4.0, 1.0))
+    ms_mips = ctx.timer_end()
+    outs = [[ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels] for _ in range(6)]
+    texels = 6 * sum(w * h for w, h in levels)
+
+    def run(learned_form):
+        for f in range(6):
+            for k, (w, h) in enumerate(levels):
+                if learned_form:
+                    net.compress_bc1_device(lv[f][k], w, h, outs[f][k], tau_pred=1e9, greedy_rounds=1,
+                                            count_fallback=False)
+                else:
+                    ctx.compress_device(N.FORMAT_BC1, lv[f][k], w, h, outs[f][k])
+
+    res = {}
+    for name, flag in (("plain", False), ("learned", True)):
+        run(flag)
+        ctx.synchronize()
+        ctx.timer_begin()
+        run(flag)
+        ms = ctx.timer_end()
+        psnr = [float(np.mean([ctx.bc1_psnr_device(lv[f][k], w, h, outs[f][k]) for f in range(6)]))
+                for k, (w, h) in enumerate(levels)]
+        res[name] = {"ms": round(ms, 3), "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1),
+                     "psnr_db_per_level": [round(p, 3) for p in psnr]}
+        log(f"  cube {name}: {res[name]['mtexels_s']} Mtexels/s")
+    for chain in lv + outs:
+        for b in chain:
+            b.free()
+    return {"workload": "6 faces x 4096^2 RGBA8 (seeded noise + direction gradients continuous across faces), "
+                        "13-level Kaiser(3; 4, 1) chains per face, BC1 per level: plain cluster fit and learned "
+                        "proposals (1 greedy round, no fallback)",
+            "texels": texels, "mip_chains_ms": round(ms_mips, 3), **res,
+            "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3)}
 
 
 def _decode_bc1(blk: np.ndarray, w: int, h: int) -> np.ndarray:

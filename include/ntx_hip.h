@@ -165,6 +165,13 @@ NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_
 NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
                                                float *d_rgba);
 
+/* Error of a BC1 level against its FP32 source: D3D10 decode
+ * (BlockDXT.cpp:43-98) and the sum over texels of the squared R, G, B
+ * differences in 8-bit units (bc1enc.cpp:73-81); PSNR = 20 log10(255 /
+ * sqrt(sse / (w h))).  Deterministic; waits for the device. */
+NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h, const void *d_blocks,
+                                          double *sse);
+
 /* ---------------------------------------------------------------- batches --
  * No counterpart in the reference, which compresses one image per
  * Compressor::compress call (Context.cpp:146-190, 486-516).  A batch is many
@@ -234,8 +241,9 @@ NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *batch, int reps, ntx_ba
 /* ------------------------------------------------ learned BC1 endpoints --
  * No counterpart in the reference.  A 48-64-64-25 ReLU network (BF16
  * weights and activations, FP32 accumulation on the matrix cores) proposes
- * four candidate endpoint pairs per 4x4 block plus the RMSE the full cluster
- * fit would reach; each candidate is refined by least squares and quantised
+ * four candidate endpoint pairs per 4x4 block, as offsets from the four
+ * diagonals of the block's colour bounding box, plus the RMSE the full
+ * cluster fit would reach; each candidate is refined by least squares and quantised
  * as icbc does, the best is kept, and blocks the network is unsure about are
  * encoded by the full cluster fit (the ntx_hip_compress BC1 path), which is
  * therefore both the fallback and the measure the proposals are trained on.
@@ -247,7 +255,8 @@ NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *batch, int reps, ntx_ba
 #define NTX_MLP_PARAMS 8921
 typedef struct ntx_hip_mlp ntx_hip_mlp;
 
-/* params NULL: He-uniform initialisation from `seed`, zero biases. */
+/* params NULL: He-uniform hidden layers from `seed`, zero biases and zero
+ * output layer (the untrained network proposes the box diagonals). */
 NTX_API ntx_status ntx_hip_mlp_create(ntx_hip_context *ctx, const float *params, uint32_t seed, ntx_hip_mlp **mlp);
 NTX_API ntx_status ntx_hip_mlp_destroy(ntx_hip_mlp *mlp);
 NTX_API ntx_status ntx_hip_mlp_get_params(ntx_hip_mlp *mlp, float *params);

@@ -5,6 +5,10 @@
 // keeps the best, and sends blocks the network is unsure about to the full
 // cluster fit (the per-image BC1 encoder's own device code).
 //
+// The network predicts offsets from the four diagonals of each block's
+// colour bounding box, so with a zero output layer it is exactly the
+// bounding-box method it refines.
+//
 // This part has no counterpart in the reference.  Matrix layout: the weights
 // are the A operand (rows = outputs, k = inputs) and 16 blocks are the N
 // columns, so a layer's 16x16 FP32 result fragment (lane l: rows 4(l/16)+j of
@@ -260,19 +264,44 @@ __global__ __launch_bounds__(256) void mlp_labels_kernel(const float *__restrict
     L[7] = 0.0f;
 }
 
+// Candidate k's starting pair before the network's offsets: diagonal k of
+// the block's colour bounding box (k = 0: every channel low -> high; k =
+// 1..3: channel k - 1 runs high -> low against the others), given the box
+// corners lo, hi.
+__device__ __forceinline__ void box_diagonal(int k, const float lo[3], const float hi[3], float a[3], float b[3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const bool flip = k == c + 1;
+        a[c] = flip ? hi[c] : lo[c];
+        b[c] = flip ? lo[c] : hi[c];
+    }
+}
+
 // Loss of one block and its gradient with respect to the 25 outputs:
+// candidate k's endpoints are diagonal k of the block's bounding box (in
+// the feature frame, corners lo, hi) plus outputs 6k..6k+5; the loss is the
 // min over candidates of the unordered endpoint-pair error, plus a kSpread
 // share of every candidate's error, plus the squared RMSE-head error.
-__device__ __forceinline__ float block_loss(const float *y, const float *L, float *dy)
+__device__ __forceinline__ float block_loss(const float *y, const float *L, const float lo[3], const float hi[3],
+                                            float *dy)
 {
     float lk[kCandidates];
     bool swp[kCandidates];
+    float ya[kCandidates][6];
 #pragma unroll
     for (int k = 0; k < kCandidates; k++) {
+        float ba[3], bb[3];
+        box_diagonal(k, lo, hi, ba, bb);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            ya[k][i] = ba[i] + y[6 * k + i];
+            ya[k][3 + i] = bb[i] + y[6 * k + 3 + i];
+        }
         float ds = 0.0f, dw = 0.0f;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const float a = y[6 * k + i], b = y[6 * k + 3 + i];
+            const float a = ya[k][i], b = ya[k][3 + i];
             ds += (a - L[i]) * (a - L[i]) + (b - L[3 + i]) * (b - L[3 + i]);
             dw += (a - L[3 + i]) * (a - L[3 + i]) + (b - L[i]) * (b - L[i]);
         }
@@ -291,8 +320,8 @@ __device__ __forceinline__ float block_loss(const float *y, const float *L, floa
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const float ta = swp[k] ? L[3 + i] : L[i], tb = swp[k] ? L[i] : L[3 + i];
-            dy[6 * k + i] = coef * 2.0f * (y[6 * k + i] - ta);
-            dy[6 * k + 3 + i] = coef * 2.0f * (y[6 * k + 3 + i] - tb);
+            dy[6 * k + i] = coef * 2.0f * (ya[k][i] - ta);
+            dy[6 * k + 3 + i] = coef * 2.0f * (ya[k][3 + i] - tb);
         }
     }
     const float dr = y[24] - L[6];
@@ -397,7 +426,20 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(const float *__restrict__
                 float yy[kOutPad];
 #pragma unroll
                 for (int i = 0; i < kOutPad; i++) yy[i] = s_y[lane][i];
-                loss += block_loss(yy, labels + size_t(b) * kLabel, dy);
+                // Box corners in the feature frame (outside texels are 0, the
+                // mean, which lies inside the box anyway).
+                float lo[3], hi[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = FLT_MAX;
+                    hi[c] = -FLT_MAX;
+                    for (int t = 0; t < 16; t++) {
+                        const float f = bf16_float(feat[size_t(b) * kIn + 16 * c + t]);
+                        lo[c] = fminf(lo[c], f);
+                        hi[c] = fmaxf(hi[c], f);
+                    }
+                }
+                loss += block_loss(yy, labels + size_t(b) * kLabel, lo, hi, dy);
             } else {
 #pragma unroll
                 for (int i = 0; i < kOutPad; i++) dy[i] = 0.0f;
@@ -604,18 +646,34 @@ __device__ __forceinline__ void refine(const float *tb, float a[3], float b[3], 
 // caller orders them), each texel's nearest palette entry.
 __device__ __forceinline__ float eval_pair(const float *tb, uint32_t c0, uint32_t c1, uint32_t *bits_out)
 {
+    // Palette in 8-bit units (the D3D10 integer palette), texels scaled once.
+    int q[4][3];
+    ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
+    ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
     float p[4][3];
-    bc1_palette(c0, c1, p);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (c0 > c1) {
+            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
+            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
+        } else {
+            q[2][k] = (q[0][k] + q[1][k]) / 2;
+            q[3][k] = 0;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) p[i][k] = float(q[i][k]);
+    }
     float err = 0.0f;
     uint32_t bits = 0;
     for (int t = 0; t < 16; t++) {
-        const float t0 = tb[3 * t], t1 = tb[3 * t + 1], t2 = tb[3 * t + 2];
-        if (t0 < 0.0f) continue;
+        const float r = tb[3 * t];
+        if (r < 0.0f) continue;
+        const float t0 = r * 255.0f, t1 = tb[3 * t + 1] * 255.0f, t2 = tb[3 * t + 2] * 255.0f;
         float best = FLT_MAX;
         uint32_t bi = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float dr = (p[i][0] - t0) * 255.0f, dg = (p[i][1] - t1) * 255.0f, db = (p[i][2] - t2) * 255.0f;
+            const float dr = p[i][0] - t0, dg = p[i][1] - t1, db = p[i][2] - t2;
             const float e = dr * dr + dg * dg + db * db;
             if (e < best) { best = e; bi = uint32_t(i); }
         }
@@ -647,8 +705,9 @@ __device__ __forceinline__ float encode_pair(const float *tb, const float a[3], 
 // block are compared across lanes.  A block whose predicted cluster-fit RMSE
 // exceeds tau_pred, or whose best candidate is worse than that prediction by
 // more than tau_gap (both in 8-bit units), is queued for the cluster fit.
-// box_candidates: start instead from the four diagonals of the block's
-// colour bounding box (the same refinement without the network; no
+// Lane g refines diagonal g of the block's colour bounding box both as it is
+// and moved by the network's offsets, keeping the better; box_candidates
+// drops the moved candidates (the same method without the network; no
 // prediction, so only tau_pred < 0 sends blocks to the fallback).
 __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
                                                       int w, int h, int bw, int n, const ntx::Tables *__restrict__ tab,
@@ -727,33 +786,39 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         // Lane (g, col): candidate g of block col, texels read from LDS.
         const float *tb = s_tex[wave][col];
         const float4 ax = s_aux[wave][col];
-        const float mm3[3] = { ax.x, ax.y, ax.z };
         const float is = 1.0f / ax.w;
+        // Candidate g: diagonal g of the block's colour bounding box, plus
+        // the network's offsets (in the block frame, hence divided by the
+        // scale) unless box_candidates.
+        float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
+        for (int i = 0; i < 16; i++)
+            if (tb[3 * i] >= 0.0f)
+#pragma unroll
+                for (int k = 0; k < 3; k++) { lo[k] = fminf(lo[k], tb[3 * i + k]); hi[k] = fmaxf(hi[k], tb[3 * i + k]); }
         float a[3], bb[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            a[k] = fminf(fmaxf(mm3[k] + s_y[wave][col][6 * g + k] * is, 0.0f), 1.0f);
-            bb[k] = fminf(fmaxf(mm3[k] + s_y[wave][col][6 * g + 3 + k] * is, 0.0f), 1.0f);
-        }
+        box_diagonal(g, lo, hi, a, bb);
         float pred = fmaxf(s_y[wave][col][24], 0.0f) * is * 255.0f;  // predicted RMSE, 8-bit units
-        if (box_candidates) {
-            float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
-            for (int i = 0; i < 16; i++)
-                if (tb[3 * i] >= 0.0f)
-#pragma unroll
-                    for (int k = 0; k < 3; k++) { lo[k] = fminf(lo[k], tb[3 * i + k]); hi[k] = fmaxf(hi[k], tb[3 * i + k]); }
-            // diagonal g: channel g - 1 runs against the others (g = 0: all together)
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const bool flip = g == k + 1;
-                a[k] = flip ? hi[k] : lo[k];
-                bb[k] = flip ? lo[k] : hi[k];
-            }
-            pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
-        }
+        // The diagonal as it is, refined and encoded ...
         refine(tb, a, bb, refine_iters);
         uint2 blk;
         float err = valid ? encode_pair(tb, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
+        if (box_candidates) {
+            pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
+        } else {
+            // ... and moved by the network's offsets; the better one is kept
+            // (the diagonal on equal error), so the network can only help.
+            float na[3], nb[3];
+            box_diagonal(g, lo, hi, na, nb);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                na[k] = fminf(fmaxf(na[k] + s_y[wave][col][6 * g + k] * is, 0.0f), 1.0f);
+                nb[k] = fminf(fmaxf(nb[k] + s_y[wave][col][6 * g + 3 + k] * is, 0.0f), 1.0f);
+            }
+            refine(tb, na, nb, refine_iters);
+            uint2 nblk;
+            const float nerr = valid ? encode_pair(tb, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
+            if (nerr < err) { err = nerr; blk = nblk; }
+        }
         // Best of the four candidates (lower candidate on equal error).
 #pragma unroll
         for (int off = 16; off < 64; off <<= 1) {

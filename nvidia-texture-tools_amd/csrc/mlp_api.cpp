@@ -39,7 +39,9 @@ namespace {
 
 constexpr int kTilesPerWave = 8;  // training: 128 blocks per wave
 
-// He-uniform weights from a xorshift stream, zero biases.
+// He-uniform hidden weights from a xorshift stream; zero biases and a zero
+// output layer, so that the untrained network proposes the bounding-box
+// diagonals unchanged.
 void init_params(uint32_t seed, std::vector<float> &p)
 {
     using namespace ntx_mlp;
@@ -57,7 +59,6 @@ void init_params(uint32_t seed, std::vector<float> &p)
     };
     fill(kW1, kHidden, kIn);
     fill(kW2, kHidden, kHidden);
-    fill(kW3, kOut, kHidden);
 }
 
 ntx_status upload_params(ntx_hip_mlp *m, const float *params)

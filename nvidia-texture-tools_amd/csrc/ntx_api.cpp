@@ -214,7 +214,7 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         (void)hipFree(kv.second.weights);
         (void)hipFree(kv.second.left);
     }
-    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket })
+    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket, &ctx->metric })
         if (b->ptr) (void)hipFree(b->ptr);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
@@ -392,6 +392,21 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
     }
     if (e != hipSuccess) return fail(ctx, e, "compress launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h, const void *d_blocks,
+                                          double *sse)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 1 || h < 1 || !d_rgba || !d_blocks || !sse) return NTX_ERR_INVALID_ARGUMENT;
+    if ((s = ensure(ctx, ctx->metric, sizeof(double) * (size_t(ntx::bc1_sse_partials(w, h)) + 1)))) return s;
+    double *part = static_cast<double *>(ctx->metric.ptr);
+    double *total = part + ntx::bc1_sse_partials(w, h);
+    NTX_HIP(ctx, ntx::launch_bc1_sse(d_rgba, w, h, d_blocks, part, total, ctx->stream));
+    NTX_HIP(ctx, hipMemcpyAsync(sse, total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return NTX_OK;
 }
 

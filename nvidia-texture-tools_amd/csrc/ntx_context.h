@@ -32,6 +32,9 @@ int poly_tile_x();
 int poly_tile_y();
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
 hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
+hipError_t launch_bc1_sse(const float *d_src, int w, int h, const void *d_blocks, double *d_partial, double *d_out,
+                          hipStream_t stream);
+int bc1_sse_partials(int w, int h);
 
 struct DevBuf {
     void *ptr = nullptr;
@@ -56,7 +59,7 @@ struct ntx_hip_context {
     ntx::Tables *d_tab = nullptr;
     float *d_srgb = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    ntx::DevBuf in, out, tmp, dst, bucket;
+    ntx::DevBuf in, out, tmp, dst, bucket, metric;
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;

@@ -1,8 +1,9 @@
 // ntx_mlp.h -- layout of the BC1 endpoint-proposal network shared by host and
 // device code: 48 inputs (a 4x4 block's R, G, B planes, centred on the block
-// mean and scaled by the largest deviation, floored), two hidden layers of 64 with
-// ReLU, 25 outputs (four candidate endpoint pairs in the same frame and the
-// predicted RMSE of the cluster fit, scaled alike).
+// mean and scaled by the largest deviation, floored), two hidden layers of 64
+// with ReLU, 25 outputs: for each of four candidates, offsets of its endpoint
+// pair from one diagonal of the block's colour bounding box (same frame), and
+// the predicted RMSE of the cluster fit, scaled alike.
 #pragma once
 
 namespace ntx_mlp {

@@ -48,7 +48,7 @@ EXPORTS = (
     "ntx_hip_batch_autotune",
     "ntx_hip_mlp_create", "ntx_hip_mlp_destroy", "ntx_hip_mlp_get_params", "ntx_hip_mlp_set_params",
     "ntx_hip_mlp_features_device", "ntx_hip_mlp_labels_device", "ntx_hip_mlp_forward_device", "ntx_hip_mlp_train_step",
-    "ntx_hip_compress_bc1_mlp_device",
+    "ntx_hip_compress_bc1_mlp_device", "ntx_hip_bc1_sse_device",
 )
 
 MLP_PARAMS = 8921          # NTX_MLP_PARAMS
@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_mlp_forward_device": (_I, [_P, _P, _I, _P]),
         "ntx_hip_mlp_train_step": (_I, [_P, _P, _P, _I, _F, ctypes.POINTER(_F), _P]),
         "ntx_hip_compress_bc1_mlp_device": (_I, [_P, _P, _I, _I, _F, _F, _I, _I, _I, _P, ctypes.POINTER(_I)]),
+        "ntx_hip_bc1_sse_device": (_I, [_P, _P, _I, _I, _P, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -281,6 +282,14 @@ class Context:
 
     def unpack_bgra8_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
         self._check(self.lib.ntx_hip_unpack_bgra8_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack")
+
+    def bc1_psnr_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
+        """PSNR (dB) of a BC1 level against its FP32 source, computed on the device."""
+        sse = ctypes.c_double()
+        self._check(self.lib.ntx_hip_bc1_sse_device(self.handle, d_rgba.ptr, w, h, d_blocks.ptr, ctypes.byref(sse)),
+                    "bc1_sse")
+        mse = sse.value / (w * h)
+        return float("inf") if mse == 0 else float(20.0 * np.log10(255.0 / np.sqrt(mse)))
 
     def synchronize(self) -> None:
         self._check(self.lib.ntx_hip_synchronize(self.handle), "synchronize")

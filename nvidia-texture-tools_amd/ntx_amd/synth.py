@@ -40,9 +40,16 @@ def value_noise(rng: np.random.Generator, h: int, w: int, period: int) -> np.nda
     fx = xs - ix
     sy = (fy * fy * (3 - 2 * fy))[:, None]
     sx = (fx * fx * (3 - 2 * fx))[None, :]
-    top = grid[iy][:, ix] * (1 - sx) + grid[iy][:, ix + 1] * sx
-    bot = grid[iy + 1][:, ix] * (1 - sx) + grid[iy + 1][:, ix + 1] * sx
-    return (top * (1 - sy) + bot * sy).astype(np.float32)
+    # x interpolation once per lattice row, then rows gathered and blended in
+    # y: the same operations (float64, as the weights are) as interpolating
+    # every texel directly, rounded to float32 at the end.
+    rowx = grid[:, ix] * (1 - sx) + grid[:, ix + 1] * sx
+    top = rowx[iy]
+    bot = rowx[iy + 1]
+    top *= 1 - sy
+    bot *= sy
+    top += bot
+    return top.astype(np.float32)
 
 
 def fbm(rng: np.random.Generator, h: int, w: int) -> np.ndarray:

@@ -14,9 +14,12 @@ checked:
             with rounding=True, x, the weights and the hidden activations are
             rounded to BF16 (round to nearest even) where the kernels round
             them, and sums are FP32-accurate (here float64).
-  loss      per block: min over the 4 candidate pairs of the unordered pair
-            error |a-A|^2+|b-B|^2 (or swapped), plus kSpread/4 times every
-            candidate's error, plus (y24 - rmse)^2.  Mean over blocks.
+  loss      per block: candidate k is diagonal k of the bounding box of the
+            block's features (k = 0: all channels low -> high, k = 1..3:
+            channel k-1 high -> low) plus outputs 6k..6k+5; min over the 4
+            candidates of the unordered pair error |a-A|^2+|b-B|^2 (or
+            swapped), plus kSpread/4 times every candidate's error, plus
+            (y24 - rmse)^2.  Mean over blocks.
   gradient  backpropagation with dy, delta2, delta1 rounded to BF16 (as the
             kernels feed them to the matrix cores), summed over blocks,
             divided by n.
@@ -85,14 +88,26 @@ def features(planar, w, h):
     return feat, aux
 
 
-def block_loss(y, L):
-    """Loss of one block (float64) and d loss / d y[0:25]."""
+def box_diagonals(X):
+    """Bounding-box corners of each block's features ([n, 48] -> lo, hi [n, 3])."""
+    f = np.asarray(X, np.float64).reshape(-1, 3, 16)
+    return f.min(axis=2), f.max(axis=2)
+
+
+def block_loss(y, L, lo=None, hi=None):
+    """Loss of one block (float64) and d loss / d y[0:25]; lo, hi are the
+    block's feature bounding box (default: the origin)."""
     y = np.asarray(y, np.float64)
     L = np.asarray(L, np.float64)
+    lo = np.zeros(3) if lo is None else np.asarray(lo, np.float64)
+    hi = np.zeros(3) if hi is None else np.asarray(hi, np.float64)
     A, B, r = L[0:3], L[3:6], L[6]
-    lk, swp = [], []
+    lk, swp, ca = [], [], []
     for k in range(4):
-        a, b = y[6 * k:6 * k + 3], y[6 * k + 3:6 * k + 6]
+        flip = np.arange(3) == k - 1
+        a = np.where(flip, hi, lo) + y[6 * k:6 * k + 3]
+        b = np.where(flip, lo, hi) + y[6 * k + 3:6 * k + 6]
+        ca.append((a, b))
         ds = np.sum((a - A) ** 2) + np.sum((b - B) ** 2)
         dw = np.sum((a - B) ** 2) + np.sum((b - A) ** 2)
         swp.append(dw < ds)
@@ -103,8 +118,8 @@ def block_loss(y, L):
     for k in range(4):
         coef = (1.0 if k == kb else 0.0) + SPREAD / 4
         ta, tb = (B, A) if swp[k] else (A, B)
-        dy[6 * k:6 * k + 3] = coef * 2 * (y[6 * k:6 * k + 3] - ta)
-        dy[6 * k + 3:6 * k + 6] = coef * 2 * (y[6 * k + 3:6 * k + 6] - tb)
+        dy[6 * k:6 * k + 3] = coef * 2 * (ca[k][0] - ta)
+        dy[6 * k + 3:6 * k + 6] = coef * 2 * (ca[k][1] - tb)
     dy[24] = RMSE_WEIGHT * 2 * (y[24] - r)
     loss += RMSE_WEIGHT * (y[24] - r) ** 2
     return loss, dy
@@ -132,12 +147,13 @@ def loss_and_grad(p, X, labels, rounding=True):
     if rounding:
         w2, w3 = bf16(w2).astype(np.float64), bf16(w3).astype(np.float64)
     y, h1, h2, Xr = forward(p, X, rounding)
+    lo, hi = box_diagonals(Xr)
     n = len(y)
     r = (lambda v: bf16(v.astype(np.float32)).astype(np.float64)) if rounding else (lambda v: v)
     total = 0.0
     dy = np.zeros((n, 25))
     for i in range(n):
-        l, d = block_loss(y[i], labels[i])
+        l, d = block_loss(y[i], labels[i], lo[i], hi[i])
         total += l
         dy[i] = d
     dy = r(dy)

@@ -164,25 +164,39 @@ def test_uniform_blocks_match_cluster_fit(gpu):
 
 
 def test_training_improves_the_proposals(gpu):
-    """What the trained network gains over the same method with an untrained
-    one, and how it compares with bounding-box starts: proposals refined
-    without the greedy search (so the starting endpoints matter), on a
-    held-out texture."""
+    """What the trained network gains over the method without it.  Each of a
+    block's four lanes encodes one bounding-box diagonal both as it is and
+    moved by the network's offsets and keeps the better, so an untrained
+    network (zero output layer) gives byte for byte the box-candidate method
+    and a trained one can only add.  Measured with the candidates used as
+    they are (no least-squares pass, no greedy search) on a held-out
+    texture; the benchmark reports the refined settings."""
     data = learned.make_dataset(gpu, [(_texture(512, 512, s), 512, 512) for s in (21, 22, 23)])
     w = h = 256
     planar = _texture(w, h, 40)
     d = gpu.alloc(planar.nbytes).upload(planar)
     out = gpu.alloc(N.compressed_size(N.FORMAT_BC1, w, h))
 
-    def psnr(net, **kw):
-        net.compress_bc1_device(d, w, h, out, tau_pred=1e9, greedy_rounds=0, **kw)
-        return learned.bc1_psnr(out.download(np.uint8), planar, w, h)
+    def blocks(net, **kw):
+        net.compress_bc1_device(d, w, h, out, tau_pred=1e9, refine_iters=0, greedy_rounds=0, **kw)
+        return out.download(np.uint8)
     with N.Mlp(gpu, seed=4) as net:
-        untrained = psnr(net)
-        box = psnr(net, box_candidates=True)
+        untrained = blocks(net)
+        box = blocks(net, box_candidates=True)
+        assert np.array_equal(untrained, box)
         learned.train(net, data, epochs=10, batch=4096, lr=3e-3)
-        trained = psnr(net)
-    ref = learned.bc1_psnr(gpu.compress(N.FORMAT_BC1, planar, w, h), planar, w, h)
-    print(f"psnr: untrained {untrained:.3f}, box {box:.3f}, trained {trained:.3f}, cluster fit {ref:.3f}")
-    assert trained > untrained + 1.0
-    assert trained > ref - 3.0
+        trained = blocks(net)
+    p_box = learned.bc1_psnr(box, planar, w, h)
+    p_trained = learned.bc1_psnr(trained, planar, w, h)
+    p_ref = learned.bc1_psnr(gpu.compress(N.FORMAT_BC1, planar, w, h), planar, w, h)
+    print(f"psnr: box diagonals {p_box:.3f}, trained network {p_trained:.3f}, cluster fit {p_ref:.3f}")
+    assert p_trained > p_box
+
+
+def test_device_psnr_matches_host(gpu):
+    w, h = 123, 77
+    planar = _texture(w, h, 31)
+    d = gpu.alloc(planar.nbytes).upload(planar)
+    blocks = gpu.compress(N.FORMAT_BC1, planar, w, h)
+    db = gpu.alloc(blocks.nbytes).upload(blocks)
+    assert gpu.bc1_psnr_device(d, w, h, db) == pytest.approx(learned.bc1_psnr(blocks, planar, w, h), abs=1e-9)
