@@ -78,7 +78,8 @@ __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 __global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                            const ntx::Tables *__restrict__ tab, int stride, int offset,
                                                            uint8_t *__restrict__ out, int *__restrict__ counts,
-                                                           int *__restrict__ lists, int cap)
+                                                           int *__restrict__ lists, int cap, int weight_by_alpha,
+                                                           uint4 *__restrict__ packed)
 {
     __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
     const int t = threadIdx.x;
@@ -110,6 +111,17 @@ __global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restri
                 k += d ? 1 : 0;
             }
             local = atomicAdd(&s_cnt[k], 1);
+            // The block's 8-bit texels for the search pass, one 64-byte line
+            // per block: R | G << 8 | B << 16 | A << 24 (A only when the
+            // search weights by alpha).
+            uint32_t a8[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) a8[i] = weight_by_alpha ? to_u8(src[block_texel(w, h, bx, by, i) + 3 * plane]) : 255u;
+            uint4 *dst = packed + size_t(blk) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                dst[q] = make_uint4(rgb[4 * q] | a8[4 * q] << 24, rgb[4 * q + 1] | a8[4 * q + 1] << 24,
+                                    rgb[4 * q + 2] | a8[4 * q + 2] << 24, rgb[4 * q + 3] | a8[4 * q + 3] << 24);
         }
     }
     __syncthreads();
@@ -160,23 +172,27 @@ __device__ __forceinline__ float4 add4(float4 a, float4 b)
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
-// ColorBlock::init gather, ColourSet minimal set, weighted PCA and the
-// stable sort of WeightedClusterFit (weightedclusterfit.cpp:39-106) for block
-// (bx, by).  Leaves the weighted points in sorted order in column `lane` of
+// ColourSet minimal set, weighted PCA and the stable sort of
+// WeightedClusterFit (weightedclusterfit.cpp:39-106) for a block whose 8-bit
+// texels (ColorBlock::init's gather, stored by the classify pass) are
+// `texels`.  Leaves the weighted points in sorted order in column `lane` of
 // s_p (rows from `count` on are zero) and returns the point count, the sum
 // of the weighted points, the sorted order (op) and each texel's point (pidx).
-__device__ __forceinline__ int bc3_prepare(float4 (*s_p)[64], int lane, const float *__restrict__ src, int w, int h,
-                                           int bx, int by, int weight_by_alpha, float cwx, float cwy, float cwz,
-                                           float4 &xsum_out, int op[16], int pidx[16])
+__device__ __forceinline__ int bc3_prepare(float4 (*s_p)[64], int lane, const uint4 *__restrict__ texels,
+                                           int weight_by_alpha, float cwx, float cwy, float cwz, float4 &xsum_out,
+                                           int op[16], int pidx[16])
 {
-    const size_t plane = size_t(w) * size_t(h);
     uint32_t rgb[16];
     float wt[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const size_t o = block_texel(w, h, bx, by, i);
-        rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
-        wt[i] = weight_by_alpha ? float(to_u8(src[o + 3 * plane]) + 1) / 256.0f : 1.0f;
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = texels[q];
+        const uint32_t t4[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            rgb[4 * q + j] = t4[j] & 0xffffffu;
+            wt[4 * q + j] = weight_by_alpha ? float((t4[j] >> 24) + 1) / 256.0f : 1.0f;
+        }
     }
 
     // ColourSet minimal set (colourset.cpp:35-101): a texel whose RGB bytes
@@ -317,10 +333,10 @@ __device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int
     *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
 }
 
-__global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                        int weight_by_alpha, float cwx, float cwy, float cwz,
-                                                        int stride, int offset, uint8_t *__restrict__ out,
-                                                        const int *__restrict__ counts, const int *__restrict__ lists, int cap)
+__global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
+                                                        float cwx, float cwy, float cwz, int stride, int offset,
+                                                        uint8_t *__restrict__ out, const int *__restrict__ counts,
+                                                        const int *__restrict__ lists, int cap)
 {
     // Column `lane` holds this lane's points; row 16 stays zero (the
     // reference's loops read one element past the end and discard it).
@@ -341,7 +357,7 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
     const int blk = lists[size_t(k) * cap + pos_in];
     int op[16], pidx[16];
     float4 xsum;
-    const int count = bc3_prepare(s_p, lane, src, w, h, blk % bw, blk / bw, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+    const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
 
     // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
     const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
@@ -373,10 +389,9 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const float *__restrict_
 // running sums are at that point of the loop, and then advance exactly as
 // the nested loops do; the slices' winners are combined preferring the
 // earlier slice on equal error, which is the first strict minimum overall.
-__global__ __launch_bounds__(64) void bc3_search_split_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                              int weight_by_alpha, float cwx, float cwy, float cwz,
-                                                              int stride, int offset, uint8_t *__restrict__ out,
-                                                              const int *__restrict__ counts,
+__global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
+                                                              float cwx, float cwy, float cwz, int stride, int offset,
+                                                              uint8_t *__restrict__ out, const int *__restrict__ counts,
                                                               const int *__restrict__ lists, int cap,
                                                               const ntx::Tables *__restrict__ tab, int lpb_shift)
 {
@@ -402,8 +417,8 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const float *__res
     float4 xsum = zero;
     if (live) {
         blk = lists[size_t(k) * cap + pos_in];
-        const int count = bc3_prepare(s_p, lane, src, w, h, blk % bw, blk / bw, weight_by_alpha, cwx, cwy, cwz, xsum,
-                                      op, pidx);
+        const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op,
+                                      pidx);
         const int first = tab->sq_off[count], total = tab->sq_off[count + 1] - first;
         const int begin = (part * total) >> lpb_shift, end = ((part + 1) * total) >> lpb_shift;
         if (begin < end) {
@@ -472,37 +487,42 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
 // BC3 colour, pass 1: clear the per-count tallies, finish single-colour
 // blocks, bucket the rest.  counts: kCountSlots ints; lists: kCountSlots x cap ints.
 hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                               int *d_lists, int cap, hipStream_t stream)
+                               int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
     hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, stream, d_counts);
     hipLaunchKernelGGL(bc3_classify_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
-                       16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
+                       16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap, weight_by_alpha,
+                       reinterpret_cast<uint4 *>(d_packed));
     return hipGetLastError();
 }
 
 // BC3 colour, pass 2: 2^lpb_shift lanes per block over the bucketed lists
 // (one block per lane when lpb_shift is 0).
-hipError_t launch_bc3_search(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
+hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
                              const int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
                              hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
+    const uint4 *packed = reinterpret_cast<const uint4 *>(d_packed);
     if (lpb_shift <= 0) {
-        hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, d_src, w, h, bw,
-                           weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts,
-                           d_lists, cap);
+        hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, packed, weight_by_alpha,
+                           cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
     } else {
         const size_t lanes = size_t(nblocks) << lpb_shift;
-        hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, d_src, w, h,
-                           bw, weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out),
-                           d_counts, d_lists, cap, d_tab, lpb_shift);
+        hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, packed,
+                           weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts,
+                           d_lists, cap, d_tab, lpb_shift);
     }
     return hipGetLastError();
 }
 
-int bc3_scratch_ints(int nblocks) { return 32 + kCountSlots * nblocks; }
+// Scratch of the BC3 colour passes, in 32-bit words: 32 count slots, the
+// kCountSlots lists of up to nblocks entries, then (16-byte aligned) 16
+// packed texels per block.
+int bc3_packed_offset(int nblocks) { return (32 + kCountSlots * nblocks + 3) & ~3; }
+int bc3_scratch_ints(int nblocks) { return bc3_packed_offset(nblocks) + 16 * nblocks; }
 
 }  // namespace ntx

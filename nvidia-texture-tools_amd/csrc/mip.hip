@@ -134,7 +134,11 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
         rowoff[i] = size_t(wrap_index(ry0 + r, H, wrap)) * size_t(W);
     }
 
-    for (int c = 0; c < 4; c++) {
+    // Planes [blockIdx.z * planes, +planes): all four per workgroup on large
+    // images, one per workgroup (four times the workgroups, a quarter of the
+    // serial work each) on images too small to fill the device.
+    const int planes = 4 / gridDim.z;
+    for (int c = blockIdx.z * planes; c < (blockIdx.z + 1) * planes; c++) {
         const float *plane = src + size_t(c) * size_t(W) * size_t(H);
         float v[kRowSlots][kColSlots];
 #pragma unroll
@@ -255,8 +259,10 @@ hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, con
                              const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
 {
     const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
-    hipLaunchKernelGGL(poly_fused_kernel, dim3((w + kTX - 1) / kTX, (h + kTY - 1) / kTY), dim3(256), 0, stream, d_src, W, H,
-                       d_dst, w, h, d_kx, d_lx, wx, d_ky, d_ly, wy, wrap);
+    const int tx = (w + kTX - 1) / kTX, ty = (h + kTY - 1) / kTY;
+    const int split = tx * ty < 1024 ? 4 : 1;  // workgroups per tile (planes split between them)
+    hipLaunchKernelGGL(poly_fused_kernel, dim3(tx, ty, split), dim3(256), 0, stream, d_src, W, H, d_dst, w, h, d_kx, d_lx,
+                       wx, d_ky, d_ly, wy, wrap);
     return hipGetLastError();
 }
 

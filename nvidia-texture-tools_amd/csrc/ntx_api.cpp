@@ -100,13 +100,13 @@ bool supported(int format, int quality)
 
 // Lanes per block of the BC3 colour search (as a power of two): one block
 // per lane while the image alone fills the device, otherwise enough lanes per
-// block to reach about 512K lanes, up to a whole wavefront per block.  The
+// block to reach about 256K lanes, up to a whole wavefront per block.  The
 // search time of a small image is set by the longest block, so splitting each
 // block's candidates over more lanes shortens it.
 int bc3_lanes_shift(int nblocks)
 {
     int s = 0;
-    while (s < 6 && (size_t(nblocks) << s) < (size_t(1) << 19)) s++;
+    while (s < 6 && (size_t(nblocks) << s) < (size_t(1) << 18)) s++;
     return s;
 }
 
@@ -378,15 +378,18 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
         int *counts = static_cast<int *>(ctx->bucket.ptr), *lists = counts + 32;
+        uint32_t *packed = reinterpret_cast<uint32_t *>(counts + ntx::bc3_packed_offset(nblocks));
+        const int wba = alpha_mode == NTX_ALPHA_TRANSPARENCY ? 1 : 0;
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
         if (e == hipSuccess)
             e = timed(ctx, NTX_KERNEL_BC3_CLASSIFY, [&] {
-                return ntx::launch_bc3_classify(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, ctx->stream);
+                return ntx::launch_bc3_classify(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed,
+                                                ctx->stream);
             });
         if (e == hipSuccess)
             e = timed(ctx, NTX_KERNEL_BC3_SEARCH, [&] {
-                return ntx::launch_bc3_search(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY, cw, d_out, counts, lists,
-                                              nblocks, ctx->d_tab, bc3_lanes_shift(nblocks), ctx->stream);
+                return ntx::launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab,
+                                              bc3_lanes_shift(nblocks), ctx->stream);
             });
     } else {
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
