@@ -40,6 +40,13 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 SQUISH_PER_CANDIDATE = 137
 SQUISH_SETUP = 1500
 
+# HBM bytes per launch of bc3_search_kernel on the 4096^2 chain (mean of its
+# two launches, levels 0 and 1): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the
+# rocprofv3 --pmc runs of `bench.py --profile-pass` summarised in
+# profiles/bc3_kaiser_chain_4096_hbm_traffic.md (the factors come from a
+# 256 MiB device copy in the same run).
+SEARCH_TRAFFIC_BYTES = {"bc3_search_kernel": round((70.99e6 + 18.00e6 + 15.70e6 + 3.62e6) / 2)}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -168,20 +175,30 @@ def main():
     value = texels * args.steps / (ms / 1e3) / 1e6
     log(f"  {ms / args.steps:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step, {value:.1f} Mtexels/s")
 
-    # Dominant kernel of the step and its algorithmic flops.
+    # Dominant kernel of the step and its algorithmic flops.  Levels of at
+    # least 2^18 blocks run bc3_search_kernel (one block per lane), smaller
+    # ones bc3_search_split_kernel (several lanes per block).
     dom = max(prof, key=lambda k: prof[k][0])
     lv_host = [d.download(np.float32) for d in d_lv]
-    flops_bc3 = sum(bc3_color_flops(lv_host[k], w, h) for k, (w, h) in enumerate(levels))
-    dom_ms, dom_n = prof[dom]
-    launches_per_step = dom_n / args.steps
-    if dom == "bc3_search_kernel":
-        achieved = flops_bc3 / (dom_ms / args.steps / 1e3) / 1e12
-        roofline = {"kernel": dom, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                    "avg_launch_ms": round(dom_ms / dom_n, 5), "launches_per_step": launches_per_step,
-                    "algorithmic_flops_per_step": flops_bc3}
+    lv_flops = [bc3_color_flops(lv_host[k], w, h) for k, (w, h) in enumerate(levels)]
+    big = [((w + 3) // 4) * ((h + 3) // 4) >= (1 << 18) for w, h in levels]
+    flops_of = {"bc3_search_kernel": sum(f for f, b in zip(lv_flops, big) if b),
+                "bc3_search_split_kernel": sum(f for f, b in zip(lv_flops, big) if not b)}
+
+    def valu_roofline(kernel):
+        k_ms, k_n = prof[kernel]
+        achieved = flops_of[kernel] / (k_ms / args.steps / 1e3) / 1e12
+        return {"kernel": kernel, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": SEARCH_TRAFFIC_BYTES.get(kernel), "avg_launch_ms": round(k_ms / k_n, 5),
+                "launches_per_step": k_n / args.steps, "algorithmic_flops_per_step": flops_of[kernel],
+                "algorithmic_flops_per_launch": flops_of[kernel] / (k_n / args.steps)}
+
+    if dom in flops_of:
+        roofline = valu_roofline(dom)
     else:
-        roofline = {"kernel": dom, "bound": "unknown", "avg_launch_ms": dom_ms / dom_n}
+        roofline = {"kernel": dom, "bound": "unknown", "avg_launch_ms": prof[dom][0] / prof[dom][1]}
+    split_roofline = valu_roofline("bc3_search_split_kernel") if "bc3_search_split_kernel" in prof else None
 
     log("mip chain (Kaiser) alone")
     ms_mip, _, prof_mip = timed(mips, args.steps, 1)
@@ -223,6 +240,7 @@ def main():
             "input_resident_in_hbm": True,
         },
         "roofline": roofline,
+        "split_search_roofline": split_roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "mip_chain_gbs": round(mip_gbs, 1),
         "mip_roofline": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

@@ -94,7 +94,7 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms);
  * CUDA timing is commented out, CudaCompressorDXT.cpp:173-174). */
 enum {
     NTX_KERNEL_BC1 = 0,
-    NTX_KERNEL_BC3_SEARCH = 1, /* BC3 colour: weighted cluster fit over bucketed blocks */
+    NTX_KERNEL_BC3_SEARCH = 1, /* BC3 colour: weighted cluster fit over bucketed blocks, one per lane */
     NTX_KERNEL_CHANNEL = 2, /* BC4 / BC5 / BC3-alpha blocks */
     NTX_KERNEL_BOX = 3,
     NTX_KERNEL_POLY_X = 4,
@@ -107,7 +107,8 @@ enum {
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
     NTX_KERNEL_BC1_LIST = 12,    /* BC1 cluster fit on the blocks the proposal pass queued */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
-    NTX_KERNEL_COUNT = 14
+    NTX_KERNEL_BC3_SEARCH_SPLIT = 14, /* BC3 colour search of a small image: 2..64 lanes per block */
+    NTX_KERNEL_COUNT = 15
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);

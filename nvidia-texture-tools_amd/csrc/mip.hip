@@ -85,20 +85,54 @@ constexpr int kTX = 64, kTY = 16;
 constexpr int kSrcW = 160, kSrcH = 56;  // largest staged source tile
 constexpr int kMaxWin = 16;             // largest window held in registers
 
+// Output tile of workgroup `tile` among tx x ty tiles.  Workgroups are
+// dispatched round-robin over the eight XCDs (each with its own L2), so when
+// tx divides into eight strips XCD k takes strip k, walked row by row: the
+// source rows shared by vertically adjacent tiles are then read by the same
+// L2 at about the same time instead of twice from HBM.
+__device__ __forceinline__ void tile_coords(int tile, int tx, int &tX, int &tY)
+{
+    if ((tx & 7) == 0) {
+        const int strip = tx >> 3, local = tile >> 3;
+        tX = (tile & 7) * strip + local % strip;
+        tY = local / strip;
+    } else {
+        tX = tile % tx;
+        tY = tile / tx;
+    }
+}
+
+// kPair: every output's first tap is two source columns after its left
+// neighbour's (downsampling by exactly 2), so lane xi's taps start at column
+// 2*xi of the staged row and are read two at a time with 8-byte LDS reads
+// (consecutive lanes on consecutive 8-byte words: no bank conflicts).
+template <bool kPair>
 __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
-                                                         float *__restrict__ dst, int w, int h,
+                                                         float *__restrict__ dst, int w, int h, int tx,
                                                          const float *__restrict__ kx, const int *__restrict__ lx, int wx,
                                                          const float *__restrict__ ky, const int *__restrict__ ly, int wy,
                                                          int wrap)
 {
-    __shared__ float s_src[kSrcH * kSrcW];
+    static_assert(2 * (kTX - 1) + kMaxWin <= kSrcW, "paired X reads stay inside a staged row");
+    __shared__ __align__(16) float s_src[kSrcH * kSrcW];
     __shared__ float s_tmp[kSrcH * kTX];
+    __shared__ float s_ky[kTY * kMaxWin];
+    __shared__ int s_ly[kTY];
     const int t = threadIdx.x;
-    const int ox0 = blockIdx.x * kTX, oy0 = blockIdx.y * kTY;
+    int tX, tY;
+    tile_coords(blockIdx.x, tx, tX, tY);
+    const int ox0 = tX * kTX, oy0 = tY * kTY;
     const int nx = min(kTX, w - ox0), ny = min(kTY, h - oy0);
     const int cx0 = lx[ox0], ry0 = ly[oy0];
     const int sw = lx[ox0 + nx - 1] - cx0 + wx;  // staged columns
     const int sh = ly[oy0 + ny - 1] - ry0 + wy;  // staged rows
+
+    // The tile's Y weights and first rows, shared by every plane.
+    if (t < ny * kMaxWin) {
+        const int o = t / kMaxWin, j = t % kMaxWin;
+        s_ky[t] = j < wy ? ky[size_t(oy0 + o) * wy + j] : 0.0f;
+    }
+    if (t < ny) s_ly[t] = ly[oy0 + t] - ry0;
 
     // Per-thread X-pass column and its weights, reused for every row and plane.
     const int xi = t & (kTX - 1);
@@ -106,16 +140,17 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
     float kxr[kMaxWin];
     int xoff = 0;
     if (xlive) {
-        xoff = lx[ox0 + xi] - cx0;
+        xoff = kPair ? 2 * xi : lx[ox0 + xi] - cx0;
 #pragma unroll
         for (int j = 0; j < kMaxWin; j++) kxr[j] = j < wx ? kx[size_t(ox0 + xi) * wx + j] : 0.0f;
     }
 
     // Staging: lane (t & 63) owns source columns (t & 63) + 64k of the tile,
     // whose wrapped image columns are fixed; wave (t >> 6) owns rows
-    // (t >> 6) + 4i.  All of a thread's loads for one plane are issued before
-    // any LDS store (out-of-tile slots load a valid in-tile address and are
-    // not stored), so their latencies overlap.
+    // (t >> 6) + 4i.  A plane's loads are all issued before its LDS stores
+    // (out-of-tile slots load a valid in-tile address and are not stored),
+    // and the next plane's loads are issued before this plane's taps run, so
+    // their latency hides behind the arithmetic.
     constexpr int kColSlots = (kSrcW + kTX - 1) / kTX;
     constexpr int kRowSlots = (kSrcH + 3) / 4;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -138,13 +173,16 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
     // images, one per workgroup (four times the workgroups, a quarter of the
     // serial work each) on images too small to fill the device.
     const int planes = 4 / gridDim.z;
-    for (int c = blockIdx.z * planes; c < (blockIdx.z + 1) * planes; c++) {
-        const float *plane = src + size_t(c) * size_t(W) * size_t(H);
-        float v[kRowSlots][kColSlots];
+    const int c0 = blockIdx.z * planes;
+    float v[kRowSlots][kColSlots];
+    {
+        const float *plane = src + size_t(c0) * size_t(W) * size_t(H);
 #pragma unroll
         for (int i = 0; i < kRowSlots; i++)
 #pragma unroll
             for (int k = 0; k < kColSlots; k++) v[i][k] = plane[rowoff[i] + gx[k]];
+    }
+    for (int c = c0; c < c0 + planes; c++) {
 #pragma unroll
         for (int i = 0; i < kRowSlots; i++) {
             const int r = wave + 4 * i;
@@ -153,13 +191,36 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
                 if (r < sh && cvalid[k]) s_src[r * kSrcW + xi + kTX * k] = v[i][k];
         }
         __syncthreads();
+        if (c + 1 < c0 + planes) {
+            const float *plane = src + size_t(c + 1) * size_t(W) * size_t(H);
+#pragma unroll
+            for (int i = 0; i < kRowSlots; i++)
+#pragma unroll
+                for (int k = 0; k < kColSlots; k++) v[i][k] = plane[rowoff[i] + gx[k]];
+        }
         if (xlive) {
             for (int r = wave; r < sh; r += 4) {
                 const float *row = s_src + r * kSrcW + xoff;
                 float sum = 0.0f;
+                if (kPair) {
+                    // All kMaxWin columns from 2*xi are inside the staged row
+                    // (2 * (kTX - 1) + kMaxWin <= kSrcW), so the reads need no
+                    // guard; only the taps below wx are summed.
+                    // 64-bit words, so each pair is one ds_read_b64.
+                    const uint64_t *row2 = reinterpret_cast<const uint64_t *>(row);
+                    uint64_t q[kMaxWin / 2];
 #pragma unroll
-                for (int j = 0; j < kMaxWin; ++j)
-                    if (j < wx) sum += kxr[j] * row[j];
+                    for (int p = 0; p < kMaxWin / 2; ++p) q[p] = row2[p];
+#pragma unroll
+                    for (int p = 0; p < kMaxWin / 2; ++p) {
+                        if (2 * p < wx) sum += kxr[2 * p] * __uint_as_float(uint32_t(q[p]));
+                        if (2 * p + 1 < wx) sum += kxr[2 * p + 1] * __uint_as_float(uint32_t(q[p] >> 32));
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kMaxWin; ++j)
+                        if (j < wx) sum += kxr[j] * row[j];
+                }
                 s_tmp[r * kTX + xi] = sum;
             }
         }
@@ -167,12 +228,13 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
         if (xlive) {
             float *out = dst + size_t(c) * size_t(w) * size_t(h);
             for (int o = wave; o < ny; o += 4) {
-                const int oy = oy0 + o;
-                const float *k = ky + size_t(oy) * wy;  // wave-uniform row: scalar loads
-                const float *col = s_tmp + (ly[oy] - ry0) * kTX + xi;
+                const float *k = s_ky + o * kMaxWin;
+                const float *col = s_tmp + s_ly[o] * kTX + xi;
                 float sum = 0.0f;
-                for (int j = 0; j < wy; ++j) sum += k[j] * col[j * kTX];
-                out[size_t(oy) * size_t(w) + ox0 + xi] = sum;
+#pragma unroll
+                for (int j = 0; j < kMaxWin; ++j)
+                    if (j < wy) sum += k[j] * col[j * kTX];
+                out[size_t(oy0 + o) * size_t(w) + ox0 + xi] = sum;
             }
         }
         __syncthreads();
@@ -182,27 +244,63 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
 // Surface.cpp:1562-1569 fromSrgb.  Values that are exactly k/255 (every
 // texel of an 8-bit source) take the host-computed table entry, so they match
 // the host powf bit for bit; other values use the device powf.
-__global__ __launch_bounds__(256) void srgb_kernel(float *__restrict__ d, size_t n, const float *__restrict__ lut)
+__device__ __forceinline__ float srgb_to_linear(float f, const float *lut)
 {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float f = d[i];
     const float k = rintf(f * 255.0f);
-    if (k >= 0.0f && k <= 255.0f && __float_as_uint(k / 255.0f) == __float_as_uint(f)) {
-        d[i] = lut[int(k)];
-        return;
-    }
-    if (f < 0.0f) f = 0.0f;
-    else if (f < 0.04045f) f = f / 12.92f;
-    else if (f <= 1.0f) f = powf((f + 0.055f) / 1.055f, 2.4f);
-    else f = 1.0f;
-    d[i] = f;
+    if (k >= 0.0f && k <= 255.0f && __float_as_uint(k / 255.0f) == __float_as_uint(f)) return lut[int(k)];
+    if (f < 0.0f) return 0.0f;
+    if (f < 0.04045f) return f / 12.92f;
+    if (f <= 1.0f) return powf((f + 0.055f) / 1.055f, 2.4f);
+    return 1.0f;
 }
 
-// Surface.cpp:745-761: BGRA8 texels to planar float / 255.
-__global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ src, size_t n, float *__restrict__ dst)
+// vec: four consecutive values per lane (16-byte loads and stores; d is
+// 16-byte aligned), the n % 4 values after the last whole quad one per lane
+// on the first lanes.  Otherwise one value per lane.
+__global__ __launch_bounds__(256) void srgb_kernel(float *__restrict__ d, size_t n, const float *__restrict__ lut, int vec)
 {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (!vec) {
+        if (i < n) d[i] = srgb_to_linear(d[i], lut);
+        return;
+    }
+    const size_t quads = n / 4;
+    if (i < quads) {
+        float4 v = reinterpret_cast<float4 *>(d)[i];
+        v.x = srgb_to_linear(v.x, lut);
+        v.y = srgb_to_linear(v.y, lut);
+        v.z = srgb_to_linear(v.z, lut);
+        v.w = srgb_to_linear(v.w, lut);
+        reinterpret_cast<float4 *>(d)[i] = v;
+    }
+    if (i < n % 4) d[4 * quads + i] = srgb_to_linear(d[4 * quads + i], lut);
+}
+
+// Surface.cpp:745-761: BGRA8 texels to planar float / 255.  vec: four
+// texels per lane (source and planes 16-byte aligned, n % 4 == 0), else one.
+__global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ src, size_t n, float *__restrict__ dst,
+                                                     int vec)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (vec) {
+        if (4 * i >= n) return;
+        const uint4 q = reinterpret_cast<const uint4 *>(src)[i];  // 4 texels, bytes b, g, r, a
+        const uint32_t t[4] = { q.x, q.y, q.z, q.w };
+        float4 r, g, b, a;
+        float *pr = &r.x, *pg = &g.x, *pb = &b.x, *pa = &a.x;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            pb[k] = float(t[k] & 0xffu) / 255.0f;
+            pg[k] = float((t[k] >> 8) & 0xffu) / 255.0f;
+            pr[k] = float((t[k] >> 16) & 0xffu) / 255.0f;
+            pa[k] = float(t[k] >> 24) / 255.0f;
+        }
+        reinterpret_cast<float4 *>(dst)[i] = r;
+        reinterpret_cast<float4 *>(dst + n)[i] = g;
+        reinterpret_cast<float4 *>(dst + 2 * n)[i] = b;
+        reinterpret_cast<float4 *>(dst + 3 * n)[i] = a;
+        return;
+    }
     if (i >= n) return;
     const uchar4 c = src[i];  // x=b, y=g, z=r, w=a
     dst[i] = float(c.z) / 255.0f;
@@ -256,28 +354,38 @@ int poly_tile_x() { return kTX; }
 int poly_tile_y() { return kTY; }
 
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
-                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
+                             bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)
 {
     const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
     const int tx = (w + kTX - 1) / kTX, ty = (h + kTY - 1) / kTY;
     const int split = tx * ty < 1024 ? 4 : 1;  // workgroups per tile (planes split between them)
-    hipLaunchKernelGGL(poly_fused_kernel, dim3(tx, ty, split), dim3(256), 0, stream, d_src, W, H, d_dst, w, h, d_kx, d_lx,
-                       wx, d_ky, d_ly, wy, wrap);
+    const dim3 grid(unsigned(tx * ty), 1, split);
+    if (pair_x)
+        hipLaunchKernelGGL(poly_fused_kernel<true>, grid, dim3(256), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx,
+                           d_ky, d_ly, wy, wrap);
+    else
+        hipLaunchKernelGGL(poly_fused_kernel<false>, grid, dim3(256), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx,
+                           wx, d_ky, d_ly, wy, wrap);
     return hipGetLastError();
 }
 
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(srgb_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, d, n, d_lut);
+    const int vec = (reinterpret_cast<uintptr_t>(d) & 15) == 0 && n >= 4;
+    const size_t lanes = vec ? n / 4 : n;
+    hipLaunchKernelGGL(srgb_kernel, dim3(unsigned((lanes + 255) / 256)), dim3(256), 0, stream, d, n, d_lut, vec);
     return hipGetLastError();
 }
 
 hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(unpack_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream,
-                       reinterpret_cast<const uchar4 *>(d_bgra), n, d_dst);
+    const int vec = (n & 3) == 0 && (reinterpret_cast<uintptr_t>(d_bgra) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(d_dst) & 15) == 0;
+    const size_t lanes = vec ? n / 4 : n;
+    hipLaunchKernelGGL(unpack_kernel, dim3(unsigned((lanes + 255) / 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const uchar4 *>(d_bgra), n, d_dst, vec);
     return hipGetLastError();
 }
 

@@ -127,6 +127,8 @@ ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const floa
     d.window = k.window;
     d.span_x = max_span(left, k.window, ntx::poly_tile_x());
     d.span_y = max_span(left, k.window, ntx::poly_tile_y());
+    d.stride2 = true;
+    for (int i = 0; i < dst; i++) d.stride2 = d.stride2 && left[i] == left[0] + 2 * i;
     hipError_t e = hipMalloc(&d.weights, sizeof(float) * size_t(k.window) * dst);
     if (e == hipSuccess) e = hipMalloc(&d.left, sizeof(int) * dst);
     if (e == hipSuccess) e = hipMemcpy(d.weights, k.weights, sizeof(float) * size_t(k.window) * dst, hipMemcpyHostToDevice);
@@ -320,7 +322,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
                                                    "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
                                                    "bc3_classify_kernel", "batch_kernel", "bc1_mlp_kernel",
-                                                   "bc1_list_kernel", "mlp_train_kernel" };
+                                                   "bc1_list_kernel", "mlp_train_kernel",
+                                                   "bc3_search_split_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -386,10 +389,11 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
                 return ntx::launch_bc3_classify(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed,
                                                 ctx->stream);
             });
+        const int shift = bc3_lanes_shift(nblocks);
         if (e == hipSuccess)
-            e = timed(ctx, NTX_KERNEL_BC3_SEARCH, [&] {
-                return ntx::launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab,
-                                              bc3_lanes_shift(nblocks), ctx->stream);
+            e = timed(ctx, shift ? NTX_KERNEL_BC3_SEARCH_SPLIT : NTX_KERNEL_BC3_SEARCH, [&] {
+                return ntx::launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab, shift,
+                                              ctx->stream);
             });
     } else {
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
@@ -464,8 +468,8 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
         if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
         if (ntx::poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
             e = timed(ctx, NTX_KERNEL_POLY_FUSED, [&] {
-                return ntx::launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, ky->weights, ky->left,
-                                              ky->window, wrap, ctx->stream);
+                return ntx::launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, kx->stride2,
+                                              ky->weights, ky->left, ky->window, wrap, ctx->stream);
             });
         } else {
             // Small images with wide windows: X pass into a temporary, then Y.

@@ -27,7 +27,7 @@ hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const f
 hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
                          hipStream_t stream);
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
-                             const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
+                             bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
 bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
 int poly_tile_x();
 int poly_tile_y();
@@ -48,6 +48,7 @@ struct PolyDev {
     int window = 0;
     int span_x = 0;  // largest source span of a run of poly_tile_x() outputs
     int span_y = 0;  // same for poly_tile_y() outputs
+    bool stride2 = false;  // left[i] == left[0] + 2 i for every output i
 };
 
 using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;

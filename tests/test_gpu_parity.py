@@ -191,6 +191,30 @@ def test_unpack_bgra8(gpu):
     d_in.free(); d_out.free()
 
 
+@pytest.mark.parametrize("w,h,offset", [(40, 24, 0), (37, 21, 0), (40, 24, 4), (37, 21, 4), (1, 1, 0), (3, 1, 12)])
+def test_front_end_sizes_and_alignment(gpu, oracle, w, h, offset):
+    """Unpack and sRGB conversion on texel counts that are not a multiple of
+    four and on buffers that start off a 16-byte boundary (the kernels take
+    four values per lane only when both allow it)."""
+    img = synth.noise_rect_rgba(w, h, seed=w * h)
+    bgra = synth.to_bgra(img)
+    n = w * h
+    d_in = gpu.alloc(bgra.nbytes + 16)
+    d_out = gpu.alloc(16 * n + 16)
+    src = np.zeros(bgra.nbytes + 16, np.uint8)
+    src[offset:offset + bgra.nbytes] = bgra.reshape(-1)
+    d_in.upload(src)
+    gpu._check(gpu.lib.ntx_hip_unpack_bgra8_device(gpu.handle, d_in.ptr + offset, w, h, d_out.ptr + offset), "unpack")
+    raw = d_out.download(np.uint8)[offset:offset + 16 * n].copy().view(np.float32)
+    planar = synth.to_planar(img).reshape(-1)
+    assert np.array_equal(raw, planar)
+    gpu._check(gpu.lib.ntx_hip_srgb_to_linear_device(gpu.handle, d_out.ptr + offset, w, h), "srgb")
+    raw = d_out.download(np.uint8)[offset:offset + 16 * n].copy().view(np.float32)
+    assert np.array_equal(_bits(raw[: 3 * n]), _bits(oracle.srgb_to_linear(planar[: 3 * n])))
+    assert np.array_equal(raw[3 * n:], planar[3 * n:])
+    d_in.free(); d_out.free()
+
+
 def test_unsupported_and_invalid_calls_report_status(gpu):
     planar = np.zeros(4 * 16, np.float32)
     with pytest.raises(N.NtxError) as e:
@@ -204,7 +228,7 @@ def test_unsupported_and_invalid_calls_report_status(gpu):
     assert e.value.status == 1
 
 
-@pytest.mark.parametrize("size", [(257, 129), (1000, 333), (4096, 64), (64, 4096), (21, 1023)])
+@pytest.mark.parametrize("size", [(257, 129), (1000, 333), (4096, 64), (64, 4096), (21, 1023), (2048, 2048), (1030, 516)])
 @pytest.mark.parametrize("filt", [N.FILTER_TRIANGLE, N.FILTER_KAISER])
 def test_polyphase_large_and_odd(gpu, oracle, size, filt):
     W, H = size
