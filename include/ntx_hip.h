@@ -95,14 +95,14 @@ NTX_API ntx_status ntx_hip_timer_end(ntx_hip_context *ctx, float *ms);
 enum {
     NTX_KERNEL_BC1 = 0,
     NTX_KERNEL_BC3_SEARCH = 1, /* BC3 colour: weighted cluster fit over bucketed blocks, one per lane */
-    NTX_KERNEL_CHANNEL = 2, /* BC4 / BC5 / BC3-alpha blocks */
+    NTX_KERNEL_CHANNEL = 2, /* BC4 / BC5 blocks */
     NTX_KERNEL_BOX = 3,
     NTX_KERNEL_POLY_X = 4,
     NTX_KERNEL_POLY_Y = 5,
     NTX_KERNEL_SRGB = 6,
     NTX_KERNEL_UNPACK = 7,
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
-    NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 colour: single-colour blocks + bucketing by colour count */
+    NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 pass 1: alpha halves, single-colour blocks, bucketing by colour count */
     NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
     NTX_KERNEL_BC1_LIST = 12,    /* BC1 cluster fit on the blocks the proposal pass queued */

@@ -70,16 +70,31 @@ __device__ __forceinline__ void bc3_single_colour(const ntx::Tables *__restrict_
     *dst = make_uint2(c0 | c1 << 16, idx);
 }
 
+// Scratch layout (ints): counts[0, kCountSlots) per-count tallies;
+// counts[kTopSlot] and kTicketGroups ticket counters kTicketStride ints
+// apart from kTicketBase (own cache lines, so their atomics do not queue
+// behind one another); the lists start at kListsOffset.
+constexpr int kTopSlot = 31;
+constexpr int kTicketGroups = 16, kTicketBase = 32, kTicketStride = 32;
+constexpr int kListsOffset = kTicketBase + kTicketGroups * kTicketStride;
+
 __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 {
-    if (threadIdx.x < kCountSlots) counts[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < kListsOffset; i += 64) counts[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
-                                                           const ntx::Tables *__restrict__ tab, int stride, int offset,
-                                                           uint8_t *__restrict__ out, int *__restrict__ counts,
-                                                           int *__restrict__ lists, int cap, int weight_by_alpha,
-                                                           uint4 *__restrict__ packed)
+// BC3 pass 1, one thread per block (CompressorDX9.cpp:145-176): the alpha
+// half by QuickCompress::compressDXT5A (as channel_kernel does for BC4), then
+// the colour: single-colour blocks are finished with OptimalCompress, every
+// other block is appended to the list of its distinct-colour count k and its
+// 8-bit texels are stored for the search as one 64-byte line per block,
+// R | G << 8 | B << 16 | A << 24 (A = 255 unless alpha weights the fit).  A
+// finished block is written as one 16-byte store when the output is 16-byte
+// aligned.
+__global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
+                                                        const ntx::Tables *__restrict__ tab, uint8_t *__restrict__ out,
+                                                        int wide, int *__restrict__ counts, int *__restrict__ lists, int cap,
+                                                        int weight_by_alpha, uint4 *__restrict__ packed)
 {
     __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
     const int t = threadIdx.x;
@@ -90,6 +105,10 @@ __global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restri
     if (blk < nblocks) {
         const int bx = blk % bw, by = blk / bw;
         const size_t plane = size_t(w) * size_t(h);
+        uint32_t av[4];
+        gather_channel(src + 3 * plane, w, h, bx, by, av);
+        const uint64_t alpha = compress_channel(av);
+        uint8_t *dst = out + size_t(blk) * 16;
         uint32_t rgb[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -100,9 +119,16 @@ __global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restri
 #pragma unroll
         for (int i = 1; i < 16; i++) single = single && rgb[i] == rgb[0];
         if (single) {
-            bc3_single_colour(tab, rgb[0] & 0xff, (rgb[0] >> 8) & 0xff, rgb[0] >> 16,
-                              reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
+            uint2 colour;
+            bc3_single_colour(tab, rgb[0] & 0xff, (rgb[0] >> 8) & 0xff, rgb[0] >> 16, &colour);
+            if (wide) {
+                *reinterpret_cast<uint4 *>(dst) = make_uint4(uint32_t(alpha), uint32_t(alpha >> 32), colour.x, colour.y);
+            } else {
+                reinterpret_cast<uint2 *>(dst)[0] = make_uint2(uint32_t(alpha), uint32_t(alpha >> 32));
+                reinterpret_cast<uint2 *>(dst)[1] = colour;
+            }
         } else {
+            *reinterpret_cast<uint2 *>(dst) = make_uint2(uint32_t(alpha), uint32_t(alpha >> 32));
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 bool d = true;
@@ -111,23 +137,44 @@ __global__ __launch_bounds__(256) void bc3_classify_kernel(const float *__restri
                 k += d ? 1 : 0;
             }
             local = atomicAdd(&s_cnt[k], 1);
-            // The block's 8-bit texels for the search pass, one 64-byte line
-            // per block: R | G << 8 | B << 16 | A << 24 (A only when the
-            // search weights by alpha).
-            uint32_t a8[16];
+            uint4 *line = packed + size_t(blk) * 4;
 #pragma unroll
-            for (int i = 0; i < 16; i++) a8[i] = weight_by_alpha ? to_u8(src[block_texel(w, h, bx, by, i) + 3 * plane]) : 255u;
-            uint4 *dst = packed + size_t(blk) * 4;
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                dst[q] = make_uint4(rgb[4 * q] | a8[4 * q] << 24, rgb[4 * q + 1] | a8[4 * q + 1] << 24,
-                                    rgb[4 * q + 2] | a8[4 * q + 2] << 24, rgb[4 * q + 3] | a8[4 * q + 3] << 24);
+            for (int q = 0; q < 4; q++) {
+                const uint32_t a = weight_by_alpha ? av[q] : 0xffffffffu;  // byte b of av[q] = alpha of texel 4q + b
+                line[q] = make_uint4(rgb[4 * q] | (a & 0xffu) << 24, rgb[4 * q + 1] | ((a >> 8) & 0xffu) << 24,
+                                     rgb[4 * q + 2] | ((a >> 16) & 0xffu) << 24, rgb[4 * q + 3] | (a >> 24) << 24);
+            }
         }
     }
     __syncthreads();
     if (t < kCountSlots && s_cnt[t] > 0) s_base[t] = atomicAdd(&counts[t], s_cnt[t]);
     __syncthreads();
     if (local >= 0) lists[size_t(k) * cap + s_base[k] + local] = blk;
+}
+
+// End of a search workgroup (one wavefront): the last one to finish clears
+// the per-count tallies for the next image.  Every workgroup has used the
+// tallies it loaded before it takes its ticket, so once all tickets are
+// taken no one reads them again.  Two levels of tickets: workgroup b counts
+// into group b % kTicketGroups, the last of a group into the top counter,
+// which keeps the number of atomics on any one address small when thousands
+// of workgroups end together.  No fence: nothing is published through the
+// tickets, and the zeros reach the next kernel at the kernel boundary.
+__device__ __forceinline__ void bc3_release_counts(int *counts)
+{
+    if (threadIdx.x == 0) {
+        const int groups = min(int(gridDim.x), kTicketGroups);
+        const int g = int(blockIdx.x) % kTicketGroups;
+        const int members = (int(gridDim.x) - g + kTicketGroups - 1) / kTicketGroups;
+        int *ticket = counts + kTicketBase + g * kTicketStride;
+        if (atomicAdd(ticket, 1) == members - 1) {
+            *ticket = 0;
+            if (atomicAdd(&counts[kTopSlot], 1) == groups - 1) {
+                for (int i = 0; i < kCountSlots; i++) counts[i] = 0;
+                counts[kTopSlot] = 0;
+            }
+        }
+    }
 }
 
 // The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums.
@@ -335,7 +382,7 @@ __device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int
 
 __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
                                                         float cwx, float cwy, float cwz, int stride, int offset,
-                                                        uint8_t *__restrict__ out, const int *__restrict__ counts,
+                                                        uint8_t *__restrict__ out, int *__restrict__ counts,
                                                         const int *__restrict__ lists, int cap)
 {
     // Column `lane` holds this lane's points; row 16 stays zero (the
@@ -353,33 +400,35 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             else item -= c;
         }
     }
-    if (k == 0) return;
-    const int blk = lists[size_t(k) * cap + pos_in];
-    int op[16], pidx[16];
-    float4 xsum;
-    const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+    if (k != 0) {
+        const int blk = lists[size_t(k) * cap + pos_in];
+        int op[16], pidx[16];
+        float4 xsum;
+        const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
 
-    // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
-    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float besterror = FLT_MAX;
-    int b0 = 0, b1 = 0, b2 = 0;
-    float4 x0 = zero;
-    for (int c0 = 0; c0 <= count; c0++) {
-        float4 x1 = zero;
-        for (int c1 = 0; c1 <= count - c0; c1++) {
-            float4 x2 = zero;
-            for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
-                const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
-                if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
-                x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+        // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
+        const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float besterror = FLT_MAX;
+        int b0 = 0, b1 = 0, b2 = 0;
+        float4 x0 = zero;
+        for (int c0 = 0; c0 <= count; c0++) {
+            float4 x1 = zero;
+            for (int c1 = 0; c1 <= count - c0; c1++) {
+                float4 x2 = zero;
+                for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
+                    const float e = sq_eval(x0, x1, x2, xsum, msx, msy, msz, nullptr);
+                    if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
+                    x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+                }
+                x1 = add4(x1, s_p[c0 + c1][lane]);
             }
-            x1 = add4(x1, s_p[c0 + c1][lane]);
+            x0 = add4(x0, s_p[c0][lane]);
         }
-        x0 = add4(x0, s_p[c0][lane]);
+        bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
+                  reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
     }
-    bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
-              reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
+    bc3_release_counts(counts);
 }
 
 // The same search with 2^lpb_shift lanes per block, for images too small to
@@ -391,7 +440,7 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
 // earlier slice on equal error, which is the first strict minimum overall.
 __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
                                                               float cwx, float cwy, float cwz, int stride, int offset,
-                                                              uint8_t *__restrict__ out, const int *__restrict__ counts,
+                                                              uint8_t *__restrict__ out, int *__restrict__ counts,
                                                               const int *__restrict__ lists, int cap,
                                                               const ntx::Tables *__restrict__ tab, int lpb_shift)
 {
@@ -461,6 +510,7 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
         const int ob = __shfl_xor(best, off);
         if (oe < besterror || (oe == besterror && (part & off))) { besterror = oe; best = ob; }
     }
+    bc3_release_counts(counts);
     if (!live || part != 0) return;
     bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, best & 0xff, (best >> 8) & 0xff, best >> 16,
               reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
@@ -484,24 +534,36 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
     return hipGetLastError();
 }
 
-// BC3 colour, pass 1: clear the per-count tallies, finish single-colour
-// blocks, bucket the rest.  counts: kCountSlots ints; lists: kCountSlots x cap ints.
-hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                               int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream)
+// BC3 pass 1: alpha halves, single-colour blocks, bucketing of the rest.
+// counts: the bc3_lists_offset() ints before the lists, zero on entry
+// (cleared by the previous search or by launch_bc3_clear_counts); lists:
+// kCountSlots x cap ints.
+hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
+                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, stream, d_counts);
-    hipLaunchKernelGGL(bc3_classify_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
-                       16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap, weight_by_alpha,
+    const int wide = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    hipLaunchKernelGGL(bc3_front_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
+                       reinterpret_cast<uint8_t *>(d_out), wide, d_counts, d_lists, cap, weight_by_alpha,
                        reinterpret_cast<uint4 *>(d_packed));
     return hipGetLastError();
 }
 
+// Zeroes the tallies and the ticket (a fill kernel, so the sequence stays
+// capturable in a graph); needed only for fresh scratch memory or after an
+// interrupted sequence.
+hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream)
+{
+    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, stream, d_counts);
+    return hipGetLastError();
+}
+
 // BC3 colour, pass 2: 2^lpb_shift lanes per block over the bucketed lists
-// (one block per lane when lpb_shift is 0).
+// (one block per lane when lpb_shift is 0); its last workgroup clears the
+// tallies.
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
-                             const int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
+                             int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
                              hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
@@ -522,7 +584,8 @@ hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_
 // Scratch of the BC3 colour passes, in 32-bit words: 32 count slots, the
 // kCountSlots lists of up to nblocks entries, then (16-byte aligned) 16
 // packed texels per block.
-int bc3_packed_offset(int nblocks) { return (32 + kCountSlots * nblocks + 3) & ~3; }
+int bc3_lists_offset() { return kListsOffset; }
+int bc3_packed_offset(int nblocks) { return (kListsOffset + kCountSlots * nblocks + 3) & ~3; }
 int bc3_scratch_ints(int nblocks) { return bc3_packed_offset(nblocks) + 16 * nblocks; }
 
 }  // namespace ntx

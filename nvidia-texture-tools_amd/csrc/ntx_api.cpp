@@ -321,7 +321,7 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
     static const char *names[NTX_KERNEL_COUNT] = { "bc1_kernel",    "bc3_search_kernel", "channel_kernel",
                                                    "box_kernel",    "poly_x_kernel",    "poly_y_kernel",
                                                    "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
-                                                   "bc3_classify_kernel", "batch_kernel", "bc1_mlp_kernel",
+                                                   "bc3_front_kernel", "batch_kernel", "bc1_mlp_kernel",
                                                    "bc1_list_kernel", "mlp_train_kernel",
                                                    "bc3_search_split_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
@@ -379,22 +379,26 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
         e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream); });
     } else if (format == NTX_FORMAT_BC3) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+        const size_t cap_before = ctx->bucket.cap;
         if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
-        int *counts = static_cast<int *>(ctx->bucket.ptr), *lists = counts + 32;
+        if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
+        int *counts = static_cast<int *>(ctx->bucket.ptr), *lists = counts + ntx::bc3_lists_offset();
         uint32_t *packed = reinterpret_cast<uint32_t *>(counts + ntx::bc3_packed_offset(nblocks));
         const int wba = alpha_mode == NTX_ALPHA_TRANSPARENCY ? 1 : 0;
-        e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
+        const int shift = bc3_lanes_shift(nblocks);
+        if (!ctx->bc3_counts_zero) e = ntx::launch_bc3_clear_counts(counts, ctx->stream);
+        ctx->bc3_counts_zero = false;
         if (e == hipSuccess)
             e = timed(ctx, NTX_KERNEL_BC3_CLASSIFY, [&] {
-                return ntx::launch_bc3_classify(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed,
-                                                ctx->stream);
+                return ntx::launch_bc3_front(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed,
+                                             ctx->stream);
             });
-        const int shift = bc3_lanes_shift(nblocks);
         if (e == hipSuccess)
             e = timed(ctx, shift ? NTX_KERNEL_BC3_SEARCH_SPLIT : NTX_KERNEL_BC3_SEARCH, [&] {
                 return ntx::launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab, shift,
                                               ctx->stream);
             });
+        if (e == hipSuccess) ctx->bc3_counts_zero = true;
     } else {
         e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
     }

@@ -14,12 +14,14 @@ namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
                       int16_t *d_nm, hipStream_t stream);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
-hipError_t launch_bc3_classify(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                               int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream);
+hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
+                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream);
+hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream);
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
-                             const int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
+                             int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
                              hipStream_t stream);
 int bc3_scratch_ints(int nblocks);
+int bc3_lists_offset();
 int bc3_packed_offset(int nblocks);
 hipError_t launch_box(const float *d_src, int W, int H, float *d_dst, hipStream_t stream);
 hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const float *d_kx, const int *d_lx, int wx, int wrap,
@@ -62,6 +64,9 @@ struct ntx_hip_context {
     float *d_srgb = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ntx::DevBuf in, out, tmp, dst, bucket, metric;
+    // The BC3 tallies at the start of `bucket` are zero: set after a complete
+    // front + search sequence (the search clears them), cleared before one.
+    bool bc3_counts_zero = false;
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;
