@@ -244,15 +244,16 @@ NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *batch, int reps, ntx_ba
  * weights and activations, FP32 accumulation on the matrix cores) proposes
  * four candidate endpoint pairs per 4x4 block, as offsets from the four
  * diagonals of the block's colour bounding box, plus the RMSE the full
- * cluster fit would reach; each candidate is refined by least squares and quantised
- * as icbc does, the best is kept, and blocks the network is unsure about are
- * encoded by the full cluster fit (the ntx_hip_compress BC1 path), which is
- * therefore both the fallback and the measure the proposals are trained on.
- * Parameters: NTX_MLP_PARAMS floats, W1[64][48] b1[64] W2[64][64] b2[64]
- * W3[25][64] b3[25].  Features: per block 48 BF16 values (R, G, B planes of
- * the 16 texels, centred on the block mean and divided by the largest
- * deviation or 1/32, whichever is larger) and 4 floats (mean R, G, B, scale).  Labels: 8 floats per block
- * (cluster-fit endpoints A, B and RMSE in that frame). */
+ * cluster fit would reach; each candidate is refined by least squares and
+ * quantised as icbc does, the best is kept, and blocks the network is unsure
+ * about are encoded by the full cluster fit (the ntx_hip_compress BC1 path),
+ * which is therefore both the fallback and the measure the proposals are
+ * trained on.  Parameters: NTX_MLP_PARAMS floats, W1[64][48] b1[64]
+ * W2[64][64] b2[64] W3[25][64] b3[25].  Features: per block 48 BF16 values
+ * (R, G, B planes of the 16 texels, centred on the block mean and divided by
+ * the largest deviation or 1/32, whichever is larger) and 4 floats (mean R,
+ * G, B, scale).  Labels: 8 floats per block (cluster-fit endpoints A, B and
+ * RMSE in that frame). */
 #define NTX_MLP_PARAMS 8921
 typedef struct ntx_hip_mlp ntx_hip_mlp;
 
@@ -282,11 +283,12 @@ NTX_API ntx_status ntx_hip_mlp_train_step(ntx_hip_mlp *mlp, const void *d_featur
  * candidate's RMSE exceeds that prediction by more than tau_gap (disabled
  * if negative); both in 8-bit units.  refine_iters: least-squares passes
  * per candidate; greedy_rounds: rounds of the search that moves one 565
- * channel of one endpoint by one step while the block error drops.  flags: NTX_MLP_BOX_CANDIDATES starts from the four
- * diagonals of each block's colour bounding box instead of the network (the
- * same refinement without the learned part; only tau_pred < 0, which sends
- * every block to the fallback, applies).  fallback_blocks (may be NULL)
- * receives the count, which makes the call wait for the device. */
+ * channel of one endpoint by one step while the block error drops.  flags:
+ * NTX_MLP_BOX_CANDIDATES starts from the four diagonals of each block's
+ * colour bounding box instead of the network (the same refinement without
+ * the learned part; only tau_pred < 0, which sends every block to the
+ * fallback, applies).  fallback_blocks (may be NULL) receives the count,
+ * which makes the call wait for the device. */
 #define NTX_MLP_BOX_CANDIDATES 1
 NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *mlp, const float *d_rgba, int w, int h, float tau_pred,
                                                    float tau_gap, int refine_iters, int greedy_rounds, int flags,
