@@ -129,6 +129,14 @@ def main():
             ctx.downsample_device(N.FILTER_KAISER, d_lv[k], w, h, d_lv[k + 1], kw, kp)
 
     def step_bc3():
+        """The workload through the chain entry point: each level's BC3
+        compression overlaps the filtering and compression of the others;
+        setImage and toLinearFromSrgb as one pass."""
+        ctx.unpack_bgra8_srgb_device(d_bgra, S, S, d_lv[0])
+        ctx.compress_chain_device(N.FORMAT_BC3, N.FILTER_KAISER, d_lv, S, S, d_bc3, width=kw, params=kp)
+
+    def step_bc3_per_level():
+        """The same workload as nvtt's per-level calls, one after another."""
         ctx.unpack_bgra8_device(d_bgra, S, S, d_lv[0])
         ctx.srgb_to_linear_device(d_lv[0], S, S)
         for k, (w, h) in enumerate(levels):
@@ -157,7 +165,7 @@ def main():
         return ms, wall, prof
 
     if args.profile_pass:
-        for fn in (step_bc3, mips, step_bc1):
+        for fn in (step_bc3_per_level, step_bc3, mips, step_bc1):
             for _ in range(2):
                 fn()
             ctx.synchronize()
@@ -166,7 +174,7 @@ def main():
         for _ in range(3):
             ctx.lib.ntx_hip_copy_device(ctx.handle, b.ptr, a.ptr, nbytes)
         ctx.synchronize()
-        log(f"profile pass done: BC3/Kaiser/BC1 chains x2, 3 copies of {nbytes} bytes")
+        log(f"profile pass done: BC3 per level/BC3 chain/Kaiser/BC1 x2, 3 copies of {nbytes} bytes")
         ctx.close()
         return
 
@@ -174,19 +182,30 @@ def main():
     ms, wall, prof = timed(step_bc3, args.steps, args.warmup)
     value = texels * args.steps / (ms / 1e3) / 1e6
     log(f"  {ms / args.steps:.3f} ms/step (events), wall {wall * 1e3 / args.steps:.3f} ms/step, {value:.1f} Mtexels/s")
+    gpu_bc3_chain = [d.download(np.uint8) for d in d_bc3]
+    log("BC3 pipeline, per-level calls")
+    ms_pl, _, prof_pl = timed(step_bc3_per_level, args.steps, 1)
+    same_as_per_level = all(np.array_equal(a, d.download(np.uint8)) for a, d in zip(gpu_bc3_chain, d_bc3))
 
-    # Dominant kernel of the step and its algorithmic flops.  Levels of at
-    # least 2^18 blocks run bc3_search_kernel (one block per lane), smaller
-    # ones bc3_search_split_kernel (several lanes per block).
-    dom = max(prof, key=lambda k: prof[k][0])
+    # Dominant kernel and its algorithmic flops.  A level runs
+    # bc3_search_kernel (one block per lane) when it has at least 2^t blocks,
+    # else bc3_search_split_kernel (several lanes per block): t = 18 for
+    # per-level calls and for level 0 of a chain, 16 for the other levels of
+    # a chain (they run beside other work).  The roofline is taken from the
+    # per-level pass, where each kernel has the device to itself; the same
+    # kernels' durations inside the chain (sharing the device) are reported
+    # beside it.
     lv_host = [d.download(np.float32) for d in d_lv]
     lv_flops = [bc3_color_flops(lv_host[k], w, h) for k, (w, h) in enumerate(levels)]
-    big = [((w + 3) // 4) * ((h + 3) // 4) >= (1 << 18) for w, h in levels]
-    flops_of = {"bc3_search_kernel": sum(f for f, b in zip(lv_flops, big) if b),
-                "bc3_search_split_kernel": sum(f for f, b in zip(lv_flops, big) if not b)}
+    nblk = [((w + 3) // 4) * ((h + 3) // 4) for w, h in levels]
 
-    def valu_roofline(kernel):
-        k_ms, k_n = prof[kernel]
+    def flops_by_kernel(chain):
+        one = [n >= (1 << (16 if chain and k else 18)) for k, n in enumerate(nblk)]
+        return {"bc3_search_kernel": sum(f for f, o in zip(lv_flops, one) if o),
+                "bc3_search_split_kernel": sum(f for f, o in zip(lv_flops, one) if not o)}
+
+    def valu_roofline(kernel, pr, flops_of):
+        k_ms, k_n = pr[kernel]
         achieved = flops_of[kernel] / (k_ms / args.steps / 1e3) / 1e12
         return {"kernel": kernel, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -194,11 +213,18 @@ def main():
                 "launches_per_step": k_n / args.steps, "algorithmic_flops_per_step": flops_of[kernel],
                 "algorithmic_flops_per_launch": flops_of[kernel] / (k_n / args.steps)}
 
-    if dom in flops_of:
-        roofline = valu_roofline(dom)
+    dom = max(prof_pl, key=lambda k: prof_pl[k][0])
+    fl_pl, fl_chain = flops_by_kernel(False), flops_by_kernel(True)
+    if dom in fl_pl:
+        roofline = valu_roofline(dom, prof_pl, fl_pl)
+        roofline["measured_in"] = "per-level pass (kernel alone on the device)"
+        if dom in prof:
+            ch = valu_roofline(dom, prof, fl_chain)
+            roofline["in_chain"] = {k: ch[k] for k in ("achieved", "frac", "avg_launch_ms", "launches_per_step")}
     else:
-        roofline = {"kernel": dom, "bound": "unknown", "avg_launch_ms": prof[dom][0] / prof[dom][1]}
-    split_roofline = valu_roofline("bc3_search_split_kernel") if "bc3_search_split_kernel" in prof else None
+        roofline = {"kernel": dom, "bound": "unknown", "avg_launch_ms": prof_pl[dom][0] / prof_pl[dom][1]}
+    split_roofline = (valu_roofline("bc3_search_split_kernel", prof_pl, fl_pl)
+                      if "bc3_search_split_kernel" in prof_pl else None)
 
     log("mip chain (Kaiser) alone")
     ms_mip, _, prof_mip = timed(mips, args.steps, 1)
@@ -241,6 +267,10 @@ def main():
         },
         "roofline": roofline,
         "split_search_roofline": split_roofline,
+        "per_level_calls": {"ms_per_step": round(ms_pl / args.steps, 4),
+                            "mtexels_s": round(texels * args.steps / (ms_pl / 1e3) / 1e6, 2),
+                            "blocks_identical_to_chain": bool(same_as_per_level),
+                            "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof_pl.items()}},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
         "mip_chain_gbs": round(mip_gbs, 1),
         "mip_roofline": {"bound": "hbm", "achieved": round(mip_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

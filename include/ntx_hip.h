@@ -159,12 +159,34 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
                                              int wrap, int alpha_mode, const float *d_rgba, int W, int H,
                                              float *d_out);
 
+/* A whole mip chain as the per-mipmap loop of Compressor::Private::compress
+ * runs it (Context.cpp:288-343: compress the level, then build the next one
+ * with Surface::buildNextMipmap), with the gamma and quantisation steps left
+ * to the caller.  d_levels[0] is level 0 (w x h, read only); for
+ * 1 <= k < levels, d_levels[k] receives level k (max(1, w >> k) x
+ * max(1, h >> k), as ntx_hip_downsample_device makes it from level k - 1);
+ * d_out[k] receives level k compressed as ntx_hip_compress_device does it.
+ * levels: 1 .. the full chain length (nv::countMipmaps).  The bytes are those
+ * of the per-level calls; the work is scheduled differently: the levels'
+ * compressions run concurrently on the context's stream and two internal
+ * streams (the small levels' latency-bound launches overlap the large ones),
+ * and all of it is ordered before later work on the context's stream. */
+NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                                 const float *color_weights, int filter, float filter_width,
+                                                 const float *filter_params, int wrap, int w, int h, int levels,
+                                                 float *const *d_levels, void *const *d_out);
+
 /* Surface::toLinearFromSrgb (Surface.cpp:1562-1592) on the R,G,B planes, in place. */
 NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
 /* Surface::setImage(InputFormat_BGRA_8UB, ...) (Surface.cpp:745-761):
  * 8-bit B,G,R,A texels to planar FP32 (value / 255). */
 NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
                                                float *d_rgba);
+/* Surface::setImage(InputFormat_BGRA_8UB, ...) followed by
+ * Surface::toLinearFromSrgb() (Surface.cpp:745-761, 1562-1592) in one pass
+ * over the texels; the same values as the two calls above. */
+NTX_API ntx_status ntx_hip_unpack_bgra8_srgb_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
+                                                    float *d_rgba);
 
 /* Error of a BC1 level against its FP32 source: D3D10 decode
  * (BlockDXT.cpp:43-98) and the sum over texels of the squared R, G, B

@@ -94,8 +94,9 @@ __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                         const ntx::Tables *__restrict__ tab, uint8_t *__restrict__ out,
                                                         int wide, int *__restrict__ counts, int *__restrict__ lists, int cap,
-                                                        int weight_by_alpha, uint4 *__restrict__ packed)
+                                                        int weight_by_alpha, uint4 *__restrict__ packed, int urgent)
 {
+    if (urgent) __builtin_amdgcn_s_setprio(3);
     __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
     const int t = threadIdx.x;
     if (t < kCountSlots) s_cnt[t] = 0;
@@ -442,8 +443,10 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
                                                               float cwx, float cwy, float cwz, int stride, int offset,
                                                               uint8_t *__restrict__ out, int *__restrict__ counts,
                                                               const int *__restrict__ lists, int cap,
-                                                              const ntx::Tables *__restrict__ tab, int lpb_shift)
+                                                              const ntx::Tables *__restrict__ tab, int lpb_shift,
+                                                              int urgent)
 {
+    if (urgent) __builtin_amdgcn_s_setprio(3);
     __shared__ float4 s_p[17][64];
     const int lane = threadIdx.x;
     const int part = lane & ((1 << lpb_shift) - 1);
@@ -535,18 +538,22 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
 }
 
 // BC3 pass 1: alpha halves, single-colour blocks, bucketing of the rest.
+// urgent: the waves raise their issue priority (s_setprio), for small
+// latency-bound levels running beside large searches that would otherwise
+// take most of each SIMD's issue slots.
 // counts: the bc3_lists_offset() ints before the lists, zero on entry
 // (cleared by the previous search or by launch_bc3_clear_counts); lists:
 // kCountSlots x cap ints.
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream)
+                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
+                            hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
     const int wide = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
     hipLaunchKernelGGL(bc3_front_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
                        reinterpret_cast<uint8_t *>(d_out), wide, d_counts, d_lists, cap, weight_by_alpha,
-                       reinterpret_cast<uint4 *>(d_packed));
+                       reinterpret_cast<uint4 *>(d_packed), urgent ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -564,7 +571,7 @@ hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream)
 // tallies.
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
                              int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
-                             hipStream_t stream)
+                             bool urgent, hipStream_t stream)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
@@ -576,7 +583,7 @@ hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_
         const size_t lanes = size_t(nblocks) << lpb_shift;
         hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, packed,
                            weight_by_alpha, cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts,
-                           d_lists, cap, d_tab, lpb_shift);
+                           d_lists, cap, d_tab, lpb_shift, urgent ? 1 : 0);
     }
     return hipGetLastError();
 }

@@ -276,10 +276,18 @@ __global__ __launch_bounds__(256) void srgb_kernel(float *__restrict__ d, size_t
     if (i < n % 4) d[4 * quads + i] = srgb_to_linear(d[4 * quads + i], lut);
 }
 
-// Surface.cpp:745-761: BGRA8 texels to planar float / 255.  vec: four
-// texels per lane (source and planes 16-byte aligned, n % 4 == 0), else one.
+// Surface.cpp:745-761: BGRA8 texels to planar float / 255.  With lut, also
+// Surface::toLinearFromSrgb on R, G, B (Surface.cpp:1562-1592): every value
+// is exactly c / 255, whose conversion is lut[c] (srgb_kernel's table path),
+// so the result equals the two passes'.  vec: four texels per lane (source
+// and planes 16-byte aligned, n % 4 == 0), else one.
+__device__ __forceinline__ float unpack_rgb(uint32_t c, const float *lut)
+{
+    return lut ? lut[c] : float(c) / 255.0f;
+}
+
 __global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ src, size_t n, float *__restrict__ dst,
-                                                     int vec)
+                                                     const float *__restrict__ lut, int vec)
 {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (vec) {
@@ -290,9 +298,9 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ 
         float *pr = &r.x, *pg = &g.x, *pb = &b.x, *pa = &a.x;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            pb[k] = float(t[k] & 0xffu) / 255.0f;
-            pg[k] = float((t[k] >> 8) & 0xffu) / 255.0f;
-            pr[k] = float((t[k] >> 16) & 0xffu) / 255.0f;
+            pb[k] = unpack_rgb(t[k] & 0xffu, lut);
+            pg[k] = unpack_rgb((t[k] >> 8) & 0xffu, lut);
+            pr[k] = unpack_rgb((t[k] >> 16) & 0xffu, lut);
             pa[k] = float(t[k] >> 24) / 255.0f;
         }
         reinterpret_cast<float4 *>(dst)[i] = r;
@@ -303,9 +311,9 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uchar4 *__restrict__ 
     }
     if (i >= n) return;
     const uchar4 c = src[i];  // x=b, y=g, z=r, w=a
-    dst[i] = float(c.z) / 255.0f;
-    dst[i + n] = float(c.y) / 255.0f;
-    dst[i + 2 * n] = float(c.x) / 255.0f;
+    dst[i] = unpack_rgb(c.z, lut);
+    dst[i + n] = unpack_rgb(c.y, lut);
+    dst[i + 2 * n] = unpack_rgb(c.x, lut);
     dst[i + 3 * n] = float(c.w) / 255.0f;
 }
 
@@ -378,14 +386,14 @@ hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t strea
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream)
+hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, const float *d_srgb_lut, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     const int vec = (n & 3) == 0 && (reinterpret_cast<uintptr_t>(d_bgra) & 15) == 0 &&
                     (reinterpret_cast<uintptr_t>(d_dst) & 15) == 0;
     const size_t lanes = vec ? n / 4 : n;
     hipLaunchKernelGGL(unpack_kernel, dim3(unsigned((lanes + 255) / 256)), dim3(256), 0, stream,
-                       reinterpret_cast<const uchar4 *>(d_bgra), n, d_dst, vec);
+                       reinterpret_cast<const uchar4 *>(d_bgra), n, d_dst, d_srgb_lut, vec);
     return hipGetLastError();
 }
 

@@ -99,14 +99,15 @@ bool supported(int format, int quality)
 }
 
 // Lanes per block of the BC3 colour search (as a power of two): one block
-// per lane while the image alone fills the device, otherwise enough lanes per
-// block to reach about 256K lanes, up to a whole wavefront per block.  The
+// per lane while the image fills 2^target_log2 lanes, otherwise enough lanes
+// per block to reach about that many, up to a whole wavefront per block.  The
 // search time of a small image is set by the longest block, so splitting each
-// block's candidates over more lanes shortens it.
-int bc3_lanes_shift(int nblocks)
+// block's candidates over more lanes shortens it; each extra lane repeats the
+// block's set-up, so a level that runs beside other work (a chain) aims lower.
+int bc3_lanes_shift(int nblocks, int target_log2)
 {
     int s = 0;
-    while (s < 6 && (size_t(nblocks) << s) < (size_t(1) << 18)) s++;
+    while (s < 6 && (size_t(nblocks) << s) < (size_t(1) << target_log2)) s++;
     return s;
 }
 
@@ -216,8 +217,13 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         (void)hipFree(kv.second.weights);
         (void)hipFree(kv.second.left);
     }
-    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket, &ctx->metric })
+    for (hipStream_t st : ctx->side)
+        if (st) (void)hipStreamSynchronize(st);
+    for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket, &ctx->metric, &ctx->chain })
         if (b->ptr) (void)hipFree(b->ptr);
+    for (hipEvent_t e : ctx->sync_events) (void)hipEventDestroy(e);
+    for (hipStream_t st : ctx->side)
+        if (st) (void)hipStreamDestroy(st);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
     for (auto &m : ctx->marks) { (void)hipEventDestroy(m.start); (void)hipEventDestroy(m.stop); }
@@ -364,6 +370,104 @@ NTX_API size_t ntx_hip_compressed_size(int format, int w, int h)
     return size_t((w + 3) / 4) * size_t((h + 3) / 4) * bs;
 }
 
+}  // extern "C"
+
+namespace ntx {
+
+// One level's compression enqueued on `stream`.  BC3 uses `bucket`
+// (bc3_scratch_ints(blocks) ints) whose tallies are zero when *counts_zero;
+// the flag is left true only after a complete front + search sequence.
+// lanes_log2: the BC3 search's lane target (bc3_lanes_shift); urgent: its
+// small launches raise their wave priority.
+static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int format, int alpha_mode, const float *cw,
+                              const float *d_rgba, int w, int h, void *d_out, int *bucket, bool *counts_zero,
+                              int lanes_log2 = 18, bool urgent = false)
+{
+    hipError_t e = hipSuccess;
+    if (format == NTX_FORMAT_BC1) {
+        e = timed_on(ctx, stream, NTX_KERNEL_BC1,
+                     [&] { return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream); });
+    } else if (format == NTX_FORMAT_BC3) {
+        const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+        int *counts = bucket, *lists = counts + bc3_lists_offset();
+        uint32_t *packed = reinterpret_cast<uint32_t *>(counts + bc3_packed_offset(nblocks));
+        const int wba = alpha_mode == NTX_ALPHA_TRANSPARENCY ? 1 : 0;
+        const int shift = bc3_lanes_shift(nblocks, lanes_log2);
+        if (!*counts_zero) e = launch_bc3_clear_counts(counts, stream);
+        *counts_zero = false;
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, NTX_KERNEL_BC3_CLASSIFY, [&] {
+                return launch_bc3_front(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed, urgent,
+                                        stream);
+            });
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, shift ? NTX_KERNEL_BC3_SEARCH_SPLIT : NTX_KERNEL_BC3_SEARCH, [&] {
+                return launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab, shift, urgent,
+                                         stream);
+            });
+        if (e == hipSuccess) *counts_zero = true;
+    } else {
+        e = timed_on(ctx, stream, NTX_KERNEL_CHANNEL, [&] { return launch_channels(d_rgba, w, h, format, d_out, stream); });
+    }
+    return e;
+}
+
+// Surface::buildNextMipmap of one level enqueued on `stream` (the fallback
+// two-pass form uses the context's temporary, so only one stream may run it).
+static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int filter, float width, const float *params,
+                                int wrap, int alpha_mode, const float *d_rgba, int W, int H, float *d_out)
+{
+    ntx_status s = NTX_OK;
+    hipError_t e;
+    if (filter == NTX_FILTER_BOX && width == 0.5f && alpha_mode != NTX_ALPHA_TRANSPARENCY) {
+        e = timed_on(ctx, stream, NTX_KERNEL_BOX, [&] { return launch_box(d_rgba, W, H, d_out, stream); });
+    } else {
+        const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+        PolyDev *kx = nullptr, *ky = nullptr;
+        if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
+        if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
+        if (poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
+            e = timed_on(ctx, stream, NTX_KERNEL_POLY_FUSED, [&] {
+                return launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, kx->stride2, ky->weights,
+                                         ky->left, ky->window, wrap, stream);
+            });
+        } else {
+            // Small images with wide windows: X pass into a temporary, then Y.
+            if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
+            float *tmp = static_cast<float *>(ctx->tmp.ptr);
+            e = timed_on(ctx, stream, NTX_KERNEL_POLY_X, [&] {
+                return launch_poly_x(d_rgba, W, H, tmp, kx->weights, kx->left, kx->window, wrap, stream);
+            });
+            if (e == hipSuccess)
+                e = timed_on(ctx, stream, NTX_KERNEL_POLY_Y, [&] {
+                    return launch_poly_y(tmp, W, H, d_out, ky->weights, ky->left, ky->window, wrap, stream);
+                });
+        }
+    }
+    if (e != hipSuccess) return fail(ctx, e, "downsample launch");
+    return NTX_OK;
+}
+
+static bool valid_mip_args(int filter, int wrap)
+{
+    return filter >= NTX_FILTER_BOX && filter <= NTX_FILTER_KAISER && wrap >= NTX_WRAP_CLAMP && wrap <= NTX_WRAP_MIRROR;
+}
+
+// A timing-free event from the context's pool of fork/join events.
+static hipEvent_t sync_event(ntx_hip_context *ctx, size_t i)
+{
+    while (ctx->sync_events.size() <= i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        ctx->sync_events.push_back(e);
+    }
+    return ctx->sync_events[i];
+}
+
+}  // namespace ntx
+
+extern "C" {
+
 NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
                                            const float *cw, const float *d_rgba, int w, int h, void *d_out)
 {
@@ -374,35 +478,110 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     if (size_t(w) * size_t(h) == 0) return NTX_OK;
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
     if (!cw) cw = ones;
-    hipError_t e = hipSuccess;
-    if (format == NTX_FORMAT_BC1) {
-        e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, ctx->stream); });
-    } else if (format == NTX_FORMAT_BC3) {
+    int *bucket = nullptr;
+    if (format == NTX_FORMAT_BC3) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         const size_t cap_before = ctx->bucket.cap;
         if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
         if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
-        int *counts = static_cast<int *>(ctx->bucket.ptr), *lists = counts + ntx::bc3_lists_offset();
-        uint32_t *packed = reinterpret_cast<uint32_t *>(counts + ntx::bc3_packed_offset(nblocks));
-        const int wba = alpha_mode == NTX_ALPHA_TRANSPARENCY ? 1 : 0;
-        const int shift = bc3_lanes_shift(nblocks);
-        if (!ctx->bc3_counts_zero) e = ntx::launch_bc3_clear_counts(counts, ctx->stream);
-        ctx->bc3_counts_zero = false;
-        if (e == hipSuccess)
-            e = timed(ctx, NTX_KERNEL_BC3_CLASSIFY, [&] {
-                return ntx::launch_bc3_front(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed,
-                                             ctx->stream);
-            });
-        if (e == hipSuccess)
-            e = timed(ctx, shift ? NTX_KERNEL_BC3_SEARCH_SPLIT : NTX_KERNEL_BC3_SEARCH, [&] {
-                return ntx::launch_bc3_search(packed, w, h, wba, cw, d_out, counts, lists, nblocks, ctx->d_tab, shift,
-                                              ctx->stream);
-            });
-        if (e == hipSuccess) ctx->bc3_counts_zero = true;
-    } else {
-        e = timed(ctx, NTX_KERNEL_CHANNEL, [&] { return ntx::launch_channels(d_rgba, w, h, format, d_out, ctx->stream); });
+        bucket = static_cast<int *>(ctx->bucket.ptr);
     }
+    hipError_t e = ntx::compress_on(ctx, ctx->stream, format, alpha_mode, cw, d_rgba, w, h, d_out, bucket,
+                                    &ctx->bc3_counts_zero);
     if (e != hipSuccess) return fail(ctx, e, "compress launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                                 const float *cw, int filter, float filter_width,
+                                                 const float *filter_params, int wrap, int w, int h, int levels,
+                                                 float *const *d_levels, void *const *d_out)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 1 || h < 1 || levels < 1 || !d_levels || !d_out || !valid_mip_args(filter, wrap))
+        return NTX_ERR_INVALID_ARGUMENT;
+    int full = 1;
+    for (int a = w, b = h; a > 1 || b > 1; a = a / 2 > 1 ? a / 2 : 1, b = b / 2 > 1 ? b / 2 : 1) full++;
+    if (levels > full) return NTX_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < levels; k++)
+        if (!d_levels[k] || !d_out[k]) return NTX_ERR_INVALID_ARGUMENT;
+    if (!supported(format, quality)) return NTX_ERR_UNSUPPORTED;
+    const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
+    if (!cw) cw = ones;
+    std::vector<int> lw(levels), lh(levels);
+    std::vector<size_t> region(levels, 0);
+    size_t total = 0;
+    for (int k = 0; k < levels; k++) {
+        lw[k] = k ? (lw[k - 1] / 2 > 1 ? lw[k - 1] / 2 : 1) : w;
+        lh[k] = k ? (lh[k - 1] / 2 > 1 ? lh[k - 1] / 2 : 1) : h;
+        region[k] = total;
+        if (format == NTX_FORMAT_BC3)
+            total += (size_t(ntx::bc3_scratch_ints(((lw[k] + 3) / 4) * ((lh[k] + 3) / 4))) + 63) & ~size_t(63);
+    }
+    if (total) {
+        const size_t cap_before = ctx->chain.cap;
+        if ((s = ensure(ctx, ctx->chain, sizeof(int) * total))) return s;
+        // The tallies are known to be zero only in the same memory with the
+    // same region layout as the last complete chain.
+        const std::tuple<int, int, int> layout{ w, h, levels };
+        if (ctx->chain.cap != cap_before || ctx->chain_layout != layout) ctx->chain_counts_zero = false;
+        ctx->chain_layout = layout;
+    }
+    for (int i = 0; i < ntx_hip_context::kSideStreams; i++)
+        if (!ctx->side[i]) NTX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+
+    // Schedule.  The main stream builds every level first: the filter's
+    // workgroups need 50 KB of LDS each and would wait behind any search
+    // running beside them.  Then the levels compress concurrently: level 0 on
+    // the main stream, levels 1 and 2 on side stream 1, levels 3 to n-1
+    // (short, latency-bound) on side stream 0, beside the large ones.  (Three
+    // streams: with the default four hardware queues per process, one of
+    // which the null stream takes, more streams would share queues and run
+    // one after another.)  The main stream then waits for both side streams.
+    hipStream_t small = ctx->side[0], second = ctx->side[1];
+    bool zero = ctx->chain_counts_zero;
+    ctx->chain_counts_zero = false;
+    bool all_zero = true;
+    // Levels below level 0 run beside other work, so their searches aim at
+    // 2^16 lanes rather than filling the device alone; the small levels'
+    // (latency-bound) launches also raise their waves' issue priority, or the
+    // large searches sharing every SIMD would stretch each of them.
+    auto compress_level = [&](hipStream_t st, int k) -> ntx_status {
+        int *bucket = format == NTX_FORMAT_BC3 ? static_cast<int *>(ctx->chain.ptr) + region[k] : nullptr;
+        bool level_zero = zero;
+        hipError_t e = ntx::compress_on(ctx, st, format, alpha_mode, cw, d_levels[k], lw[k], lh[k], d_out[k], bucket,
+                                        &level_zero, k ? 16 : 18, st == small);
+        if (e != hipSuccess) return fail(ctx, e, "chain compress launch");
+        all_zero = all_zero && level_zero;
+        return NTX_OK;
+    };
+    hipEvent_t ev_all = ntx::sync_event(ctx, 0), ev_small = ntx::sync_event(ctx, 1), ev_second = ntx::sync_event(ctx, 2);
+    if (!ev_all || !ev_small || !ev_second) return fail(ctx, hipErrorOutOfMemory, "chain events");
+    for (int k = 1; k < levels; k++)
+        if ((s = ntx::downsample_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode,
+                                    d_levels[k - 1], lw[k - 1], lh[k - 1], d_levels[k])))
+            return s;
+    if (levels > 1) NTX_HIP(ctx, hipEventRecord(ev_all, ctx->stream));
+    // Level 0 is enqueued first: the host takes a while to enqueue the many
+    // small launches, and the longest compression should not wait for that.
+    if ((s = compress_level(ctx->stream, 0))) return s;
+    if (levels > 1) {
+        NTX_HIP(ctx, hipStreamWaitEvent(second, ev_all, 0));
+        for (int k = 1; k < levels && k <= 2; k++)
+            if ((s = compress_level(second, k))) return s;
+        NTX_HIP(ctx, hipEventRecord(ev_second, second));
+    }
+    if (levels > 3) {
+        NTX_HIP(ctx, hipStreamWaitEvent(small, ev_all, 0));
+        for (int k = 3; k < levels; k++)
+            if ((s = compress_level(small, k))) return s;
+        NTX_HIP(ctx, hipEventRecord(ev_small, small));
+    }
+    if (levels > 1) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_second, 0));
+    if (levels > 3) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_small, 0));
+    // Every level's search cleared its own region.
+    ctx->chain_counts_zero = all_zero;
     return NTX_OK;
 }
 
@@ -460,32 +639,8 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
     ntx_status s = bind(ctx);
     if (s) return s;
     if (W < 1 || H < 1 || (W == 1 && H == 1) || !d_rgba || !d_out) return NTX_ERR_INVALID_ARGUMENT;
-    if (filter < NTX_FILTER_BOX || filter > NTX_FILTER_KAISER || wrap < NTX_WRAP_CLAMP || wrap > NTX_WRAP_MIRROR)
-        return NTX_ERR_INVALID_ARGUMENT;
-    hipError_t e;
-    if (filter == NTX_FILTER_BOX && width == 0.5f && alpha_mode != NTX_ALPHA_TRANSPARENCY) {
-        e = timed(ctx, NTX_KERNEL_BOX, [&] { return ntx::launch_box(d_rgba, W, H, d_out, ctx->stream); });
-    } else {
-        const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
-        PolyDev *kx = nullptr, *ky = nullptr;
-        if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
-        if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
-        if (ntx::poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
-            e = timed(ctx, NTX_KERNEL_POLY_FUSED, [&] {
-                return ntx::launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, kx->stride2,
-                                              ky->weights, ky->left, ky->window, wrap, ctx->stream);
-            });
-        } else {
-            // Small images with wide windows: X pass into a temporary, then Y.
-            if ((s = ensure(ctx, ctx->tmp, size_t(w) * size_t(H) * 4 * sizeof(float)))) return s;
-            float *tmp = static_cast<float *>(ctx->tmp.ptr);
-            e = timed(ctx, NTX_KERNEL_POLY_X, [&] { return ntx::launch_poly_x(d_rgba, W, H, tmp, kx->weights, kx->left, kx->window, wrap, ctx->stream); });
-            if (e == hipSuccess)
-                e = timed(ctx, NTX_KERNEL_POLY_Y, [&] { return ntx::launch_poly_y(tmp, W, H, d_out, ky->weights, ky->left, ky->window, wrap, ctx->stream); });
-        }
-    }
-    if (e != hipSuccess) return fail(ctx, e, "downsample launch");
-    return NTX_OK;
+    if (!ntx::valid_mip_args(filter, wrap)) return NTX_ERR_INVALID_ARGUMENT;
+    return ntx::downsample_on(ctx, ctx->stream, filter, width, params, wrap, alpha_mode, d_rgba, W, H, d_out);
 }
 
 NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,
@@ -522,7 +677,22 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8
     ntx_status s = bind(ctx);
     if (s) return s;
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
-    hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] { return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->stream); });
+    hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] {
+        return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, nullptr, ctx->stream);
+    });
+    if (e != hipSuccess) return fail(ctx, e, "unpack launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_unpack_bgra8_srgb_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
+                                                    float *d_rgba)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] {
+        return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->d_srgb, ctx->stream);
+    });
     if (e != hipSuccess) return fail(ctx, e, "unpack launch");
     return NTX_OK;
 }

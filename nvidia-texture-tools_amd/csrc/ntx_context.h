@@ -15,11 +15,12 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
                       int16_t *d_nm, hipStream_t stream);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
-                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, hipStream_t stream);
+                            int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
+                            hipStream_t stream);
 hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream);
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
                              int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
-                             hipStream_t stream);
+                             bool urgent, hipStream_t stream);
 int bc3_scratch_ints(int nblocks);
 int bc3_lists_offset();
 int bc3_packed_offset(int nblocks);
@@ -34,7 +35,8 @@ bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
 int poly_tile_x();
 int poly_tile_y();
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
-hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, hipStream_t stream);
+// d_srgb_lut: NULL, or the 256-entry sRGB->linear table to convert R, G, B as well.
+hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, const float *d_srgb_lut, hipStream_t stream);
 hipError_t launch_bc1_sse(const float *d_src, int w, int h, const void *d_blocks, double *d_partial, double *d_out,
                           hipStream_t stream);
 int bc1_sse_partials(int w, int h);
@@ -67,6 +69,15 @@ struct ntx_hip_context {
     // The BC3 tallies at the start of `bucket` are zero: set after a complete
     // front + search sequence (the search clears them), cleared before one.
     bool bc3_counts_zero = false;
+    // Mip chains (ntx_hip_compress_chain_device): streams the levels'
+    // compression runs on, fork/join events, and per-level BC3 scratch
+    // (every region's tallies zero when chain_counts_zero).
+    static constexpr int kSideStreams = 2;
+    hipStream_t side[kSideStreams] = {};
+    std::vector<hipEvent_t> sync_events;
+    ntx::DevBuf chain;
+    bool chain_counts_zero = false;
+    std::tuple<int, int, int> chain_layout{ 0, 0, 0 };  // (w, h, levels) the regions were laid out for
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;
@@ -89,17 +100,25 @@ bool supported(int format, int quality);
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out);
 
 // Runs one launch; brackets it with events when the context is profiling.
+// Runs `launch` (which enqueues on `stream`), bracketed by profiling events
+// on that stream when profiling is on.
 template <typename F>
-hipError_t timed(ntx_hip_context *ctx, int kernel, F &&launch)
+hipError_t timed_on(ntx_hip_context *ctx, hipStream_t stream, int kernel, F &&launch)
 {
     if (!ctx->profiling) return launch();
     hipEvent_t a = take_event(ctx), b = take_event(ctx);
     if (!a || !b) return hipErrorOutOfMemory;
-    hipError_t e = hipEventRecord(a, ctx->stream);
+    hipError_t e = hipEventRecord(a, stream);
     if (e == hipSuccess) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(b, stream);
     ctx->marks.push_back({ kernel, a, b });
     return e;
+}
+
+template <typename F>
+hipError_t timed(ntx_hip_context *ctx, int kernel, F &&launch)
+{
+    return timed_on(ctx, ctx->stream, kernel, static_cast<F &&>(launch));
 }
 
 }  // namespace ntx

@@ -41,8 +41,9 @@ EXPORTS = (
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
+    "ntx_hip_compress_chain_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device",
-    "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device",
+    "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
     "ntx_hip_batch_autotune",
@@ -117,10 +118,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P, _S]),
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
+        "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
+        "ntx_hip_unpack_bgra8_srgb_device": (_I, [_P, _P, _I, _I, _P]),
         "ntx_hip_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_P)]),
         "ntx_hip_batch_destroy": (_I, [_P]),
         "ntx_hip_batch_submit": (_I, [_P]),
@@ -258,6 +261,27 @@ class Context:
                                               d_out.ptr)
         self._check(st, "ntx_hip_compress_device")
 
+    def compress_chain_device(self, fmt: int, filt: int, d_levels: Sequence[DeviceBuffer], w: int, h: int,
+                              d_out: Sequence[DeviceBuffer], quality: int = QUALITY_NORMAL,
+                              alpha_mode: int = ALPHA_NONE, color_weights: Sequence[float] = (1, 1, 1, 1),
+                              width: Optional[float] = None, params: Optional[Sequence[float]] = None,
+                              wrap: int = WRAP_MIRROR) -> None:
+        """Level 0 in d_levels[0]; levels 1.. are written to d_levels[1..] and
+        every level's blocks to d_out[k] (ntx_hip_compress_chain_device)."""
+        if len(d_levels) != len(d_out):
+            raise ValueError("one output buffer per level")
+        dw, dp = DEFAULT_FILTER[filt]
+        width = dw if width is None else width
+        params = dp if params is None else params
+        pp = _f32c(params) if params is not None else None
+        cw = _f32c(color_weights)
+        lv = (_P * len(d_levels))(*[d.ptr for d in d_levels])
+        out = (_P * len(d_out))(*[d.ptr for d in d_out])
+        st = self.lib.ntx_hip_compress_chain_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), filt, width,
+                                                    _ptr(pp) if pp is not None else None, wrap, w, h, len(d_levels),
+                                                    ctypes.cast(lv, _P), ctypes.cast(out, _P))
+        self._check(st, "ntx_hip_compress_chain_device")
+
     def compress_bc1_stats_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
                                   d_nm: DeviceBuffer, alpha_mode: int = ALPHA_NONE,
                                   color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
@@ -282,6 +306,10 @@ class Context:
 
     def unpack_bgra8_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
         self._check(self.lib.ntx_hip_unpack_bgra8_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack")
+
+    def unpack_bgra8_srgb_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
+        """unpack_bgra8_device followed by srgb_to_linear_device, in one pass."""
+        self._check(self.lib.ntx_hip_unpack_bgra8_srgb_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack_srgb")
 
     def bc1_psnr_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
         """PSNR (dB) of a BC1 level against its FP32 source, computed on the device."""

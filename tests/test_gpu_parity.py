@@ -212,7 +212,34 @@ def test_front_end_sizes_and_alignment(gpu, oracle, w, h, offset):
     raw = d_out.download(np.uint8)[offset:offset + 16 * n].copy().view(np.float32)
     assert np.array_equal(_bits(raw[: 3 * n]), _bits(oracle.srgb_to_linear(planar[: 3 * n])))
     assert np.array_equal(raw[3 * n:], planar[3 * n:])
+    # both steps in one pass: the same bits
+    d_out.upload(np.zeros(16 * n + 16, np.uint8))
+    gpu._check(gpu.lib.ntx_hip_unpack_bgra8_srgb_device(gpu.handle, d_in.ptr + offset, w, h, d_out.ptr + offset),
+               "unpack_srgb")
+    fused = d_out.download(np.uint8)[offset:offset + 16 * n].copy().view(np.float32)
+    assert np.array_equal(_bits(fused), _bits(raw))
     d_in.free(); d_out.free()
+
+
+def test_unpack_srgb_every_byte_value(gpu, oracle):
+    """All 256 byte values in every channel: the one-pass conversion equals
+    setImage followed by toLinearFromSrgb."""
+    w, h = 256, 4
+    bgra = np.zeros((h, w, 4), np.uint8)
+    v = np.arange(256, dtype=np.uint8)
+    for c in range(4):
+        bgra[c, :, :] = np.roll(v, 37 * c)[:, None]
+    d_in = gpu.alloc(bgra.nbytes).upload(bgra)
+    d_a, d_b = gpu.alloc(16 * w * h), gpu.alloc(16 * w * h)
+    gpu.unpack_bgra8_device(d_in, w, h, d_a)
+    gpu.srgb_to_linear_device(d_a, w, h)
+    gpu.unpack_bgra8_srgb_device(d_in, w, h, d_b)
+    a, b = d_a.download(np.float32), d_b.download(np.float32)
+    assert np.array_equal(_bits(a), _bits(b))
+    n = w * h
+    planar = (bgra.reshape(n, 4)[:, [2, 1, 0, 3]].T.astype(np.float32) / np.float32(255)).reshape(-1)
+    assert np.array_equal(_bits(b[: 3 * n]), _bits(oracle.srgb_to_linear(planar[: 3 * n])))
+    d_in.free(); d_a.free(); d_b.free()
 
 
 def test_unsupported_and_invalid_calls_report_status(gpu):
