@@ -178,15 +178,23 @@ __device__ __forceinline__ void bc3_release_counts(int *counts)
     }
 }
 
-// The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums.
-__device__ __forceinline__ float sq_eval(float4 x0, float4 x1, float4 x2, float4 xsum, float mx, float my, float mz,
-                                         SqCand *out)
+// The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums,
+// in two parts: the cluster weights' terms (alpha2, beta2, alphabeta and
+// the inverse determinant), then the endpoints and the error.
+__device__ __forceinline__ float4 sq_weights(float x0w, float x1w, float x2w, float sumw)
 {
-    const float w3 = ((xsum.w - x0.w) - x1.w) - x2.w;
-    const float alpha2 = (x0.w + x1.w * (4.0f / 9.0f)) + x2.w * (1.0f / 9.0f);
-    const float beta2 = (w3 + x2.w * (4.0f / 9.0f)) + x1.w * (1.0f / 9.0f);
-    const float ab = (x1.w + x2.w) * (2.0f / 9.0f);
+    const float w3 = ((sumw - x0w) - x1w) - x2w;
+    const float alpha2 = (x0w + x1w * (4.0f / 9.0f)) + x2w * (1.0f / 9.0f);
+    const float beta2 = (w3 + x2w * (4.0f / 9.0f)) + x1w * (1.0f / 9.0f);
+    const float ab = (x1w + x2w) * (2.0f / 9.0f);
     const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+    return make_float4(alpha2, beta2, ab, factor);
+}
+
+__device__ __forceinline__ float sq_eval_t(float4 x0, float4 x1, float4 x2, float4 xsum, float4 t, float mx, float my,
+                                           float mz, SqCand *out)
+{
+    const float alpha2 = t.x, beta2 = t.y, ab = t.z, factor = t.w;
     const float axr = (x0.x + x1.x * (2.0f / 3.0f)) + x2.x * (1.0f / 3.0f);
     const float axg = (x0.y + x1.y * (2.0f / 3.0f)) + x2.y * (1.0f / 3.0f);
     const float axb = (x0.z + x1.z * (2.0f / 3.0f)) + x2.z * (1.0f / 3.0f);
@@ -213,6 +221,12 @@ __device__ __forceinline__ float sq_eval(float4 x0, float4 x1, float4 x2, float4
         out->bx = br; out->by = bg; out->bz = bb;
     }
     return e1r * mx + e1g * my + e1b * mz;
+}
+
+__device__ __forceinline__ float sq_eval(float4 x0, float4 x1, float4 x2, float4 xsum, float mx, float my, float mz,
+                                         SqCand *out)
+{
+    return sq_eval_t(x0, x1, x2, xsum, sq_weights(x0.w, x1.w, x2.w, xsum.w), mx, my, mz, out);
 }
 
 __device__ __forceinline__ float4 add4(float4 a, float4 b)
@@ -384,7 +398,8 @@ __device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int
 __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
                                                         float cwx, float cwy, float cwz, int stride, int offset,
                                                         uint8_t *__restrict__ out, int *__restrict__ counts,
-                                                        const int *__restrict__ lists, int cap)
+                                                        const int *__restrict__ lists, int cap,
+                                                        const ntx::Tables *__restrict__ tab)
 {
     // Column `lane` holds this lane's points; row 16 stays zero (the
     // reference's loops read one element past the end and discard it).
@@ -401,17 +416,44 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             else item -= c;
         }
     }
-    if (k != 0) {
-        const int blk = lists[size_t(k) * cap + pos_in];
-        int op[16], pidx[16];
-        float4 xsum;
-        const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+    const bool live = k != 0;
+    int blk = 0, count = 0;
+    int op[16], pidx[16];
+    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 xsum = zero;
+    if (live) {
+        blk = lists[size_t(k) * cap + pos_in];
+        count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+    }
 
-        // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
-        const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
-        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float besterror = FLT_MAX;
-        int b0 = 0, b1 = 0, b2 = 0;
+    // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
+    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+    float besterror = FLT_MAX;
+    int b0 = 0, b1 = 0, b2 = 0;
+    // A wave whose live lanes all hold 16 points of weight 1 takes the
+    // weight terms of each candidate from sq16 (scalar loads shared by the
+    // wave); idle lanes run along on whatever their LDS column holds and
+    // write nothing.
+    const uint64_t live_mask = __builtin_amdgcn_ballot_w64(live);
+    const uint64_t unit_mask = __builtin_amdgcn_ballot_w64(live && count == 16 && !weight_by_alpha);
+    if (live_mask != 0 && unit_mask == live_mask) {
+        const float4 *__restrict__ tw = reinterpret_cast<const float4 *>(tab->sq16);
+        int idx = 0;
+        float4 x0 = zero;
+        for (int c0 = 0; c0 <= 16; c0++) {
+            float4 x1 = zero;
+            for (int c1 = 0; c1 <= 16 - c0; c1++) {
+                float4 x2 = zero;
+                for (int c2 = 0; c2 <= 16 - c0 - c1; c2++, idx++) {
+                    const float e = sq_eval_t(x0, x1, x2, xsum, tw[idx], msx, msy, msz, nullptr);
+                    if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
+                    x2 = add4(x2, s_p[c0 + c1 + c2][lane]);
+                }
+                x1 = add4(x1, s_p[c0 + c1][lane]);
+            }
+            x0 = add4(x0, s_p[c0][lane]);
+        }
+    } else if (live) {
         float4 x0 = zero;
         for (int c0 = 0; c0 <= count; c0++) {
             float4 x1 = zero;
@@ -426,9 +468,10 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             }
             x0 = add4(x0, s_p[c0][lane]);
         }
+    }
+    if (live)
         bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
                   reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
-    }
     bc3_release_counts(counts);
 }
 
@@ -578,7 +621,8 @@ hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_
     const uint4 *packed = reinterpret_cast<const uint4 *>(d_packed);
     if (lpb_shift <= 0) {
         hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, packed, weight_by_alpha,
-                           cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap);
+                           cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap,
+                           d_tab);
     } else {
         const size_t lanes = size_t(nblocks) << lpb_shift;
         hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, packed,

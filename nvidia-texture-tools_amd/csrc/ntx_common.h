@@ -33,6 +33,12 @@ struct Tables {
     // count n: entries [sq_off[n], sq_off[n+1]) hold c0 | c1<<8 | c2<<16.
     int sq_off[18];
     uint32_t sq[4845];
+    // For 16 points of weight 1 (an unweighted block with 16 distinct
+    // colours) the cluster weights of candidate (c0, c1, c2) are the counts,
+    // so the fit's alpha2, beta2, alphabeta and 1 / (alpha2 beta2 - ab^2)
+    // (weightedclusterfit.cpp:511-518) depend on the candidate only; here
+    // they are, in loop order, computed with the kernel's own operations.
+    float sq16[969][4];
 };
 
 void build_tables(Tables *t);

@@ -99,6 +99,25 @@ void build_tables(Tables *t)
                 for (int c = 0; c <= n - a - b; c++) t->sq[i++] = uint32_t(a) | uint32_t(b) << 8 | uint32_t(c) << 16;
     }
     t->sq_off[17] = i;
+
+    // The n = 16 candidates with unit weights: the operations of the
+    // kernel's sq_weights on exact integer counts (float arithmetic, no
+    // contraction, IEEE division), hence the same values.
+    i = 0;
+    for (int a = 0; a <= 16; a++)
+        for (int b = 0; b <= 16 - a; b++)
+            for (int c = 0; c <= 16 - a - b; c++, i++) {
+                const float x0 = float(a), x1 = float(b), x2 = float(c), sum = 16.0f;
+                const float w3 = ((sum - x0) - x1) - x2;
+                const float alpha2 = (x0 + x1 * (4.0f / 9.0f)) + x2 * (1.0f / 9.0f);
+                const float beta2 = (w3 + x2 * (4.0f / 9.0f)) + x1 * (1.0f / 9.0f);
+                const float ab = (x1 + x2) * (2.0f / 9.0f);
+                const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+                t->sq16[i][0] = alpha2;
+                t->sq16[i][1] = beta2;
+                t->sq16[i][2] = ab;
+                t->sq16[i][3] = factor;
+            }
 }
 
 // ---------------------------------------------------------------- filters --
