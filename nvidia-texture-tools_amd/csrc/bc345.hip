@@ -453,6 +453,30 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             }
             x0 = add4(x0, s_p[c0][lane]);
         }
+    } else if (live && !weight_by_alpha) {
+        // Integer weights (point multiplicities): each candidate's terms come
+        // from sqw by the exact cluster weights; the next candidate's entry
+        // is loaded while this one is evaluated.
+        const float4 *__restrict__ tw = reinterpret_cast<const float4 *>(tab->sqw);
+        float4 x0 = zero;
+        for (int c0 = 0; c0 <= count; c0++) {
+            float4 x1 = zero;
+            for (int c1 = 0; c1 <= count - c0; c1++) {
+                const int base = (int(x0.w) * 17 + int(x1.w)) * 17;
+                float4 x2 = zero;
+                float4 t = tw[base];
+                for (int c2 = 0; c2 <= count - c0 - c1; c2++) {
+                    const float4 pn = s_p[c0 + c1 + c2][lane];  // row `count` is zero
+                    const float4 tn = tw[base + int(x2.w + pn.w)];
+                    const float e = sq_eval_t(x0, x1, x2, xsum, t, msx, msy, msz, nullptr);
+                    if (e < besterror) { besterror = e; b0 = c0; b1 = c1; b2 = c2; }
+                    x2 = add4(x2, pn);
+                    t = tn;
+                }
+                x1 = add4(x1, s_p[c0 + c1][lane]);
+            }
+            x0 = add4(x0, s_p[c0][lane]);
+        }
     } else if (live) {
         float4 x0 = zero;
         for (int c0 = 0; c0 <= count; c0++) {

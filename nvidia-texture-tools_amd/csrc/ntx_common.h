@@ -39,6 +39,10 @@ struct Tables {
     // (weightedclusterfit.cpp:511-518) depend on the candidate only; here
     // they are, in loop order, computed with the kernel's own operations.
     float sq16[969][4];
+    // The same terms for any integer cluster weights (an unweighted block:
+    // weights are point multiplicities, 16 in all): entry
+    // (w0 * 17 + w1) * 17 + w2 for w0 + w1 + w2 <= 16, others zero.
+    float sqw[17 * 17 * 17][4];
 };
 
 void build_tables(Tables *t);

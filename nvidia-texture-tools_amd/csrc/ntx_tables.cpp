@@ -100,9 +100,12 @@ void build_tables(Tables *t)
     }
     t->sq_off[17] = i;
 
-    // The n = 16 candidates with unit weights: the operations of the
-    // kernel's sq_weights on exact integer counts (float arithmetic, no
-    // contraction, IEEE division), hence the same values.
+    // Cluster-weight terms for integer weights (a, b, c) of 16 in all: the
+    // operations of the kernel's sq_weights on exact integer values (float
+    // arithmetic, no contraction, IEEE division), hence the same values.
+    // sq16 lists them in the n = 16 loop order (unit weights: the weights
+    // are the counts), sqw by (a, b, c).
+    std::memset(t->sqw, 0, sizeof t->sqw);
     i = 0;
     for (int a = 0; a <= 16; a++)
         for (int b = 0; b <= 16 - a; b++)
@@ -113,10 +116,11 @@ void build_tables(Tables *t)
                 const float beta2 = (w3 + x2 * (4.0f / 9.0f)) + x1 * (1.0f / 9.0f);
                 const float ab = (x1 + x2) * (2.0f / 9.0f);
                 const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
-                t->sq16[i][0] = alpha2;
-                t->sq16[i][1] = beta2;
-                t->sq16[i][2] = ab;
-                t->sq16[i][3] = factor;
+                const float v[4] = { alpha2, beta2, ab, factor };
+                for (int j = 0; j < 4; j++) {
+                    t->sq16[i][j] = v[j];
+                    t->sqw[(a * 17 + b) * 17 + c][j] = v[j];
+                }
             }
 }
 
