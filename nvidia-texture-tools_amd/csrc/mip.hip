@@ -84,6 +84,7 @@ __global__ __launch_bounds__(256) void poly_y_kernel(const float *__restrict__ t
 constexpr int kTX = 64, kTY = 16;
 constexpr int kSrcW = 160, kSrcH = 56;  // largest staged source tile
 constexpr int kMaxWin = 16;             // largest window held in registers
+constexpr int kWaves = 8;               // wavefronts per workgroup
 
 // Output tile of workgroup `tile` among tx x ty tiles.  Workgroups are
 // dispatched round-robin over the eight XCDs (each with its own L2), so when
@@ -107,7 +108,7 @@ __device__ __forceinline__ void tile_coords(int tile, int tx, int &tX, int &tY)
 // 2*xi of the staged row and are read two at a time with 8-byte LDS reads
 // (consecutive lanes on consecutive 8-byte words: no bank conflicts).
 template <bool kPair>
-__global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
+__global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
                                                          float *__restrict__ dst, int w, int h, int tx,
                                                          const float *__restrict__ kx, const int *__restrict__ lx, int wx,
                                                          const float *__restrict__ ky, const int *__restrict__ ly, int wy,
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
     const int sh = ly[oy0 + ny - 1] - ry0 + wy;  // staged rows
 
     // The tile's Y weights and first rows, shared by every plane.
+    static_assert(kTY * kMaxWin <= 64 * kWaves, "one thread per staged Y weight");
     if (t < ny * kMaxWin) {
         const int o = t / kMaxWin, j = t % kMaxWin;
         s_ky[t] = j < wy ? ky[size_t(oy0 + o) * wy + j] : 0.0f;
@@ -147,12 +149,12 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
 
     // Staging: lane (t & 63) owns source columns (t & 63) + 64k of the tile,
     // whose wrapped image columns are fixed; wave (t >> 6) owns rows
-    // (t >> 6) + 4i.  A plane's loads are all issued before its LDS stores
+    // (t >> 6) + kWaves i.  A plane's loads are all issued before its LDS stores
     // (out-of-tile slots load a valid in-tile address and are not stored),
     // and the next plane's loads are issued before this plane's taps run, so
     // their latency hides behind the arithmetic.
     constexpr int kColSlots = (kSrcW + kTX - 1) / kTX;
-    constexpr int kRowSlots = (kSrcH + 3) / 4;
+    constexpr int kRowSlots = (kSrcH + kWaves - 1) / kWaves;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     int gx[kColSlots];
     bool cvalid[kColSlots];
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
     size_t rowoff[kRowSlots];
 #pragma unroll
     for (int i = 0; i < kRowSlots; i++) {
-        const int r = min(wave + 4 * i, sh - 1);
+        const int r = min(wave + kWaves * i, sh - 1);
         rowoff[i] = size_t(wrap_index(ry0 + r, H, wrap)) * size_t(W);
     }
 
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
     for (int c = c0; c < c0 + planes; c++) {
 #pragma unroll
         for (int i = 0; i < kRowSlots; i++) {
-            const int r = wave + 4 * i;
+            const int r = wave + kWaves * i;
 #pragma unroll
             for (int k = 0; k < kColSlots; k++)
                 if (r < sh && cvalid[k]) s_src[r * kSrcW + xi + kTX * k] = v[i][k];
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
                 for (int k = 0; k < kColSlots; k++) v[i][k] = plane[rowoff[i] + gx[k]];
         }
         if (xlive) {
-            for (int r = wave; r < sh; r += 4) {
+            for (int r = wave; r < sh; r += kWaves) {
                 const float *row = s_src + r * kSrcW + xoff;
                 float sum = 0.0f;
                 if (kPair) {
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(256) void poly_fused_kernel(const float *__restrict
         __syncthreads();
         if (xlive) {
             float *out = dst + size_t(c) * size_t(w) * size_t(h);
-            for (int o = wave; o < ny; o += 4) {
+            for (int o = wave; o < ny; o += kWaves) {
                 const float *k = s_ky + o * kMaxWin;
                 const float *col = s_tmp + s_ly[o] * kTX + xi;
                 float sum = 0.0f;
@@ -369,10 +371,10 @@ hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, con
     const int split = tx * ty < 1024 ? 4 : 1;  // workgroups per tile (planes split between them)
     const dim3 grid(unsigned(tx * ty), 1, split);
     if (pair_x)
-        hipLaunchKernelGGL(poly_fused_kernel<true>, grid, dim3(256), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx,
+        hipLaunchKernelGGL(poly_fused_kernel<true>, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx,
                            d_ky, d_ly, wy, wrap);
     else
-        hipLaunchKernelGGL(poly_fused_kernel<false>, grid, dim3(256), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx,
+        hipLaunchKernelGGL(poly_fused_kernel<false>, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx,
                            wx, d_ky, d_ly, wy, wrap);
     return hipGetLastError();
 }
