@@ -124,9 +124,7 @@ def main():
     d_bc1 = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels]
 
     def mips():
-        for k in range(len(levels) - 1):
-            w, h = levels[k]
-            ctx.downsample_device(N.FILTER_KAISER, d_lv[k], w, h, d_lv[k + 1], kw, kp)
+        ctx.downsample_chain_device(N.FILTER_KAISER, d_lv, S, S, kw, kp)
 
     def step_bc3():
         """The workload through the chain entry point: each level's BC3

@@ -159,6 +159,14 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
                                              int wrap, int alpha_mode, const float *d_rgba, int W, int H,
                                              float *d_out);
 
+/* Surface::buildNextMipmap repeated (Surface.cpp:1344-1406): d_levels[0] is
+ * level 0 (w x h, read only); d_levels[k], 1 <= k < levels, receives level k
+ * (max(1, w >> k) x max(1, h >> k)), each exactly as ntx_hip_downsample_device
+ * makes it from level k - 1. */
+NTX_API ntx_status ntx_hip_downsample_chain_device(ntx_hip_context *ctx, int filter, float filter_width,
+                                                   const float *filter_params, int wrap, int alpha_mode, int w, int h,
+                                                   int levels, float *const *d_levels);
+
 /* A whole mip chain as the per-mipmap loop of Compressor::Private::compress
  * runs it (Context.cpp:288-343: compress the level, then build the next one
  * with Surface::buildNextMipmap), with the gamma and quantisation steps left

@@ -448,6 +448,19 @@ static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int fi
     return NTX_OK;
 }
 
+// Levels 1 .. levels-1 of a chain on `stream`, one launch per level.
+static ntx_status downsample_chain_on(ntx_hip_context *ctx, hipStream_t stream, int filter, float width,
+                                      const float *params, int wrap, int alpha_mode, const int *lw, const int *lh,
+                                      int levels, float *const *d_levels)
+{
+    for (int k = 1; k < levels; k++) {
+        const ntx_status s = downsample_on(ctx, stream, filter, width, params, wrap, alpha_mode, d_levels[k - 1],
+                                           lw[k - 1], lh[k - 1], d_levels[k]);
+        if (s) return s;
+    }
+    return NTX_OK;
+}
+
 static bool valid_mip_args(int filter, int wrap)
 {
     return filter >= NTX_FILTER_BOX && filter <= NTX_FILTER_KAISER && wrap >= NTX_WRAP_CLAMP && wrap <= NTX_WRAP_MIRROR;
@@ -558,10 +571,9 @@ NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int forma
     };
     hipEvent_t ev_all = ntx::sync_event(ctx, 0), ev_small = ntx::sync_event(ctx, 1), ev_second = ntx::sync_event(ctx, 2);
     if (!ev_all || !ev_small || !ev_second) return fail(ctx, hipErrorOutOfMemory, "chain events");
-    for (int k = 1; k < levels; k++)
-        if ((s = ntx::downsample_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode,
-                                    d_levels[k - 1], lw[k - 1], lh[k - 1], d_levels[k])))
-            return s;
+    if ((s = ntx::downsample_chain_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode,
+                                      lw.data(), lh.data(), levels, d_levels)))
+        return s;
     if (levels > 1) NTX_HIP(ctx, hipEventRecord(ev_all, ctx->stream));
     // Level 0 is enqueued first: the host takes a while to enqueue the many
     // small launches, and the longest compression should not wait for that.
@@ -641,6 +653,25 @@ NTX_API ntx_status ntx_hip_downsample_device(ntx_hip_context *ctx, int filter, f
     if (W < 1 || H < 1 || (W == 1 && H == 1) || !d_rgba || !d_out) return NTX_ERR_INVALID_ARGUMENT;
     if (!ntx::valid_mip_args(filter, wrap)) return NTX_ERR_INVALID_ARGUMENT;
     return ntx::downsample_on(ctx, ctx->stream, filter, width, params, wrap, alpha_mode, d_rgba, W, H, d_out);
+}
+
+NTX_API ntx_status ntx_hip_downsample_chain_device(ntx_hip_context *ctx, int filter, float filter_width,
+                                                   const float *filter_params, int wrap, int alpha_mode, int w, int h,
+                                                   int levels, float *const *d_levels)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 1 || h < 1 || levels < 1 || !d_levels || !ntx::valid_mip_args(filter, wrap)) return NTX_ERR_INVALID_ARGUMENT;
+    std::vector<int> lw(1, w), lh(1, h);
+    while (lw.back() > 1 || lh.back() > 1) {
+        lw.push_back(lw.back() / 2 > 1 ? lw.back() / 2 : 1);
+        lh.push_back(lh.back() / 2 > 1 ? lh.back() / 2 : 1);
+    }
+    if (levels > int(lw.size())) return NTX_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < levels; k++)
+        if (!d_levels[k]) return NTX_ERR_INVALID_ARGUMENT;
+    return ntx::downsample_chain_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode, lw.data(),
+                                    lh.data(), levels, d_levels);
 }
 
 NTX_API ntx_status ntx_hip_downsample(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap,

@@ -32,6 +32,7 @@ hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const f
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
                              bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
 bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
+
 int poly_tile_x();
 int poly_tile_y();
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);

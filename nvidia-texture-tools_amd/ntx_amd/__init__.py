@@ -42,7 +42,7 @@ EXPORTS = (
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
     "ntx_hip_compress_chain_device",
-    "ntx_hip_downsample", "ntx_hip_downsample_device",
+    "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
+        "ntx_hip_downsample_chain_device": (_I, [_P, _I, _F, _P, _I, _I, _I, _I, _I, _P]),
         "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
         "ntx_hip_unpack_bgra8_srgb_device": (_I, [_P, _P, _I, _I, _P]),
@@ -300,6 +301,19 @@ class Context:
         st = self.lib.ntx_hip_downsample_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
                                                 wrap, alpha_mode, d_src.ptr, W, H, d_dst.ptr)
         self._check(st, "ntx_hip_downsample_device")
+
+    def downsample_chain_device(self, filt: int, d_levels: Sequence[DeviceBuffer], w: int, h: int,
+                                width: Optional[float] = None, params: Optional[Sequence[float]] = None,
+                                wrap: int = WRAP_MIRROR, alpha_mode: int = ALPHA_NONE) -> None:
+        """Levels 1.. of the chain from d_levels[0] (ntx_hip_downsample_chain_device)."""
+        dw, dp = DEFAULT_FILTER[filt]
+        width = dw if width is None else width
+        params = dp if params is None else params
+        pp = _f32c(params) if params is not None else None
+        lv = (_P * len(d_levels))(*[d.ptr for d in d_levels])
+        st = self.lib.ntx_hip_downsample_chain_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
+                                                      wrap, alpha_mode, w, h, len(d_levels), ctypes.cast(lv, _P))
+        self._check(st, "ntx_hip_downsample_chain_device")
 
     def srgb_to_linear_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
         self._check(self.lib.ntx_hip_srgb_to_linear_device(self.handle, d_rgba.ptr, w, h), "srgb_to_linear")

@@ -91,3 +91,26 @@ def test_chain_rejects_bad_arguments(gpu):
         gpu.compress_chain_device(N.FORMAT_BC1, N.FILTER_BOX, [d], 8, 8, [out], quality=N.QUALITY_FASTEST)
     assert e.value.status == 2
     d.free(); out.free()
+
+
+@pytest.mark.parametrize("filt", [N.FILTER_BOX, N.FILTER_TRIANGLE, N.FILTER_KAISER])
+@pytest.mark.parametrize("wrap", [N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR])
+@pytest.mark.parametrize("size", [(512, 256), (130, 66), (37, 23), (1, 300), (300, 1), (96, 1), (2, 2), (1, 1)])
+def test_downsample_chain_matches_per_level(gpu, filt, wrap, size):
+    """ntx_hip_downsample_chain_device against per-level downsampling, FP32
+    bit patterns, including 1-D images and the Transparency box (polyphase)
+    case."""
+    w, h = size
+    planar = np.random.default_rng(w * 7 + h).random(4 * w * h, dtype=np.float32)
+    sizes = N.mip_sizes(w, h)
+    for alpha_mode in ((N.ALPHA_NONE, N.ALPHA_TRANSPARENCY) if filt == N.FILTER_BOX else (N.ALPHA_NONE,)):
+        ref = [planar]
+        for k in range(1, len(sizes)):
+            ref.append(gpu.downsample(filt, ref[-1], *sizes[k - 1], wrap=wrap, alpha_mode=alpha_mode))
+        d_lv = [gpu.alloc(16 * a * b) for a, b in sizes]
+        d_lv[0].upload(planar)
+        gpu.downsample_chain_device(filt, d_lv, w, h, wrap=wrap, alpha_mode=alpha_mode)
+        for k, d in enumerate(d_lv):
+            got = d.download(np.float32)
+            assert np.array_equal(got.view(np.uint32), ref[k].view(np.uint32)), (k, sizes[k], alpha_mode)
+            d.free()
