@@ -147,17 +147,23 @@ def main():
             ctx.compress_device(N.FORMAT_BC1, d_lv[k], w, h, d_bc1[k])
 
     def timed(fn, steps, warmup):
+        """Time `steps` runs with nothing else on the streams (the value),
+        then the same number of runs with per-kernel event pairs (the
+        breakdown and the rooflines; the events themselves cost queue time,
+        so they stay out of the timed runs)."""
         for _ in range(warmup):
             fn()
         ctx.synchronize()
-        ctx.profile(True)
-        ctx.profile_read(reset=True)
         t0 = time.perf_counter()
         ctx.timer_begin()
         for _ in range(steps):
             fn()
         ms = ctx.timer_end()
         wall = time.perf_counter() - t0
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        for _ in range(steps):
+            fn()
         prof = ctx.profile_read(reset=True)
         ctx.profile(False)
         return ms, wall, prof
