@@ -286,6 +286,7 @@ def main():
                          "algorithmic_flops_per_chain": flops_bc1, "kernel_ms_per_chain": round(bc1_kernel_ms, 4)},
     }
 
+    result["c1_box_chain_bc1"] = bench_c1(ctx, args, not args.no_cpu)
     if not args.no_batch:
         result["batch"] = bench_batch(ctx, args)
     if not args.no_learned:
@@ -573,6 +574,78 @@ def _cube_face(face: int, S: int, seed: int) -> np.ndarray:
         img[:, :, ch] = np.floor(np.clip(0.6 * grad + 0.4 * noise, 0, 1) * 255 + 0.5).astype(np.uint8)
     img[:, :, 3] = 255
     return img
+
+
+def bench_c1(ctx, args, cpu: bool) -> dict:
+    """The reference's CPU-runnable case: one 1024^2 RGBA8 noise+rectangles
+    texture (alpha 255), its 11-level Box chain (fastDownSample), BC1 at
+    Normal quality per level, through the chain entry point.  With cpu, the
+    same chain on the host: the oracle port and the reference's own icbc
+    (oracle/_ref), both checked byte for byte against the device output."""
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    S = 1024
+    levels = N.mip_sizes(S, S)
+    texels = sum(w * h for w, h in levels)
+    img = synth.noise_rect_rgba(S, S, seed=args.seed)
+    planar = synth.to_planar(img).reshape(-1)
+    d_lv = [ctx.alloc(16 * w * h) for w, h in levels]
+    d_lv[0].upload(planar)
+    d_out = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels]
+
+    def step():
+        ctx.compress_chain_device(N.FORMAT_BC1, N.FILTER_BOX, d_lv, S, S, d_out)
+
+    for _ in range(3):
+        step()
+    ctx.synchronize()
+    ctx.timer_begin()
+    for _ in range(args.steps):
+        step()
+    ms = ctx.timer_end() / args.steps
+    gpu_blocks = [d.download(np.uint8) for d in d_out]
+    gpu_levels = [d.download(np.float32) for d in d_lv]
+    res = {"workload": f"{S}x{S} RGBA8 noise+rectangles (seed {args.seed}), {len(levels)}-level Box chain, BC1 Normal "
+                       f"per level (ntx_hip_compress_chain_device)",
+           "texels_per_step": texels, "ms_per_step": round(ms, 4), "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        orc = oracle_lib.Oracle()
+        t0 = time.perf_counter()
+        x = planar
+        cpu_blocks = []
+        cpu_levels = [x]
+        for k, (w, h) in enumerate(levels):
+            cpu_blocks.append(orc.compress(N.FORMAT_BC1, cpu_levels[k], w, h, threads=threads))
+            if k + 1 < len(levels):
+                cpu_levels.append(orc.build_next_mipmap(N.FILTER_BOX, 0.5, None, N.WRAP_MIRROR, cpu_levels[k], w, h))
+        cpu_s = time.perf_counter() - t0
+        res["cpu_port"] = {"mtexels_s": round(texels / cpu_s / 1e6, 3), "cores": threads,
+                           "blocks_identical": all(np.array_equal(a, b) for a, b in zip(gpu_blocks, cpu_blocks)),
+                           "levels_bit_identical": all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                                                       for a, b in zip(gpu_levels, cpu_levels))}
+        ref = oracle_lib.Reference.load()
+        if ref is not None:
+            import ctypes
+            fn = ref.lib.ntxref_compress_bc1_image
+            fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+            t0 = time.perf_counter()
+            ref_blocks = []
+            for k, (w, h) in enumerate(levels):
+                out = np.zeros(N.compressed_size(N.FORMAT_BC1, w, h), np.uint8)
+                src = np.ascontiguousarray(gpu_levels[k], np.float32)
+                fn(src.ctypes.data, w, h, out.ctypes.data, threads)
+                ref_blocks.append(out)
+            ref_s = time.perf_counter() - t0
+            res["cpu_reference_icbc"] = {
+                "mtexels_s": round(texels / ref_s / 1e6, 3), "cores": threads,
+                "blocks_identical": all(np.array_equal(a, b) for a, b in zip(gpu_blocks, ref_blocks))}
+    for d in d_lv + d_out:
+        d.free()
+    return res
 
 
 def bench_cube(ctx, args, net):

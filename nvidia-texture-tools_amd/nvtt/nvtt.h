@@ -140,6 +140,28 @@ struct TaskDispatcher {
     virtual void dispatch(Task *task, void *context, int count) = 0;
 };
 
+// Extension with no counterpart in NVTT 2.1.2 (the shape of the BatchList
+// of later NVTT versions): many surfaces compressed by one call.  Append adds
+// one surface, output as (face, mipmap); AppendChain adds a surface and the
+// mip chain Surface::buildNextMipmap(filter) would build from it, mipmapCount
+// levels (0: all, countMipmaps()), output as mip levels mipmap, mipmap + 1,
+// ...  The surfaces must stay alive and unchanged until compress returns.
+struct BatchList {
+    struct Private;
+    NVTT_API BatchList();
+    NVTT_API ~BatchList();
+    NVTT_API void Clear();
+    NVTT_API void Append(const Surface *img, int face, int mipmap, const OutputOptions *outputOptions);
+    NVTT_API void AppendChain(const Surface *img, int face, int mipmap, MipmapFilter filter, int mipmapCount,
+                              const OutputOptions *outputOptions);
+    NVTT_API unsigned GetSize() const;
+    Private *m;
+
+private:
+    BatchList(const BatchList &);
+    void operator=(const BatchList &);
+};
+
 // nvtt.h:413-448 (Surface and raw APIs).
 struct Compressor {
     struct Private;
@@ -159,6 +181,13 @@ struct Compressor {
     NVTT_API bool compress(int w, int h, int d, int face, int mipmap, const float *rgba,
                            const CompressionOptions &compressionOptions, const OutputOptions &outputOptions) const;
     NVTT_API int estimateSize(int w, int h, int d, int mipmapCount, const CompressionOptions &compressionOptions) const;
+    // Every image of the list (BatchList above), delivered to each entry's
+    // OutputOptions in list order, level by level, with the same
+    // beginImage / writeData / endImage / error sequence and the same bytes
+    // as compress(Surface) per image.  BC1, BC4 and BC5 run as one persistent
+    // device launch for the whole list (mip chains included); BC3 runs each
+    // entry's chain through the chain scheduler.
+    NVTT_API bool compress(const BatchList &list, const CompressionOptions &compressionOptions) const;
     Private &m;
 
 private:

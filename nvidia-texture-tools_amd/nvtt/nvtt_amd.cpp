@@ -455,6 +455,53 @@ struct Compressor::Private {
     }
 };
 
+struct BatchList::Private {
+    struct Entry {
+        const Surface *img;
+        int face, mipmap;
+        bool chain;
+        MipmapFilter filter;
+        int levels;
+        const OutputOptions *oo;
+    };
+    std::vector<Entry> items;
+};
+
+BatchList::BatchList() : m(new Private) {}
+BatchList::~BatchList() { delete m; }
+void BatchList::Clear() { m->items.clear(); }
+void BatchList::Append(const Surface *img, int face, int mipmap, const OutputOptions *oo)
+{
+    m->items.push_back({ img, face, mipmap, false, MipmapFilter_Box, 1, oo });
+}
+void BatchList::AppendChain(const Surface *img, int face, int mipmap, MipmapFilter filter, int mipmapCount,
+                            const OutputOptions *oo)
+{
+    m->items.push_back({ img, face, mipmap, true, filter, mipmapCount, oo });
+}
+unsigned BatchList::GetSize() const { return unsigned(m->items.size()); }
+
+namespace {
+
+// Surface::buildNextMipmap(filter)'s defaults (Surface.cpp:1121-1141).
+void filter_defaults(MipmapFilter f, float *width, float params[2])
+{
+    *width = f == MipmapFilter_Kaiser ? 3.0f : f == MipmapFilter_Triangle ? 1.0f : 0.5f;
+    params[0] = f == MipmapFilter_Kaiser ? 4.0f : 0.0f;
+    params[1] = f == MipmapFilter_Kaiser ? 1.0f : 0.0f;
+}
+
+// Level count of an entry: the chain length asked for, bounded by the
+// surface's own mip count; one for a single image.
+int entry_levels(const BatchList::Private::Entry &e)
+{
+    if (!e.chain) return 1;
+    const int full = e.img->countMipmaps();
+    return e.levels <= 0 || e.levels > full ? full : e.levels;
+}
+
+}  // namespace
+
 // Context.cpp:61-73: acceleration is enabled when a device is present.
 Compressor::Compressor() : m(*new Private) { enableCudaAcceleration(true); }
 Compressor::~Compressor() { delete &m; }
@@ -480,6 +527,136 @@ bool Compressor::compress(int w, int h, int d, int face, int mipmap, const float
                           const OutputOptions &oo) const
 {
     return m.compress(AlphaMode_None, w, h, d, face, mipmap, rgba, false, co.m, oo.m);
+}
+
+bool Compressor::compress(const BatchList &list, const CompressionOptions &co) const
+{
+    typedef BatchList::Private::Entry Entry;
+    const std::vector<Entry> &items = list.m->items;
+    // Every image of every entry, in delivery order: (entry, level, width, height).
+    struct Image { int entry, level, w, h; };
+    std::vector<Image> images;
+    for (int i = 0; i < int(items.size()); i++) {
+        int w = items[i].img->width(), h = items[i].img->height();
+        for (int k = 0; k < entry_levels(items[i]); k++) {
+            images.push_back({ i, k, w, h });
+            w = w / 2 > 1 ? w / 2 : 1;
+            h = h / 2 > 1 ? h / 2 : 1;
+        }
+    }
+    const int fmt = int(co.m.format);
+    auto fail_all = [&](Error e) {
+        for (const Image &im : images) {
+            const OutputOptions::Private &oo = items[im.entry].oo->m;
+            oo.beginImage(int(ntx_hip_compressed_size(fmt, im.w, im.h)), im.w, im.h, 1, items[im.entry].face,
+                          items[im.entry].mipmap + im.level);
+            oo.error(e);
+            oo.endImage();
+        }
+        return true;
+    };
+    if (!m.hipEnabled || !backend()) return fail_all(Error_CudaError);
+    CompressorInterface *probe = m.chooseCompressor(co.m, true);
+    if (!probe) return fail_all(Error_UnsupportedFeature);
+    delete probe;
+    for (const Entry &e : items)
+        if (!e.img || e.img->isNull() || e.img->depth() != 1 || !e.oo) return fail_all(Error_InvalidInput);
+
+    // The compressed levels, gathered on the host before delivery.
+    std::vector<std::vector<uint8_t>> bytes(images.size());
+    ntx_status s = NTX_OK;
+    {
+        std::lock_guard<std::mutex> g(backend_lock());
+        ntx_hip_context *ctx = backend();
+        if (co.m.format == Format_BC1 || co.m.format == Format_BC4 || co.m.format == Format_BC5) {
+            // One persistent launch for the whole list (ntx_hip_batch_*).
+            std::vector<ntx_texture_desc> descs(items.size());
+            for (size_t i = 0; i < items.size(); i++) {
+                const Entry &e = items[i];
+                ntx_texture_desc &d = descs[i];
+                std::memset(&d, 0, sizeof d);
+                d.d_rgba = e.img->deviceData();
+                d.width = e.img->width();
+                d.height = e.img->height();
+                d.format = fmt;
+                d.quality = int(co.m.quality);
+                d.alpha_mode = int(e.img->alphaMode());
+                for (int c = 0; c < 4; c++) d.color_weights[c] = co.m.colorWeight[c];
+                d.mip_levels = entry_levels(e);
+                d.filter = int(e.filter);
+                d.filter_width = 0.0f;
+                d.wrap = int(e.img->wrapMode());
+            }
+            ntx_hip_batch *b = nullptr;
+            s = ntx_hip_batch_create(ctx, descs.data(), int(descs.size()), &b);
+            if (s == NTX_OK) s = ntx_hip_batch_submit(b);
+            if (s == NTX_OK) s = ntx_hip_batch_wait(b);
+            for (size_t j = 0; s == NTX_OK && j < images.size(); j++) {
+                const float *lv = nullptr;
+                const void *blocks = nullptr;
+                int lw = 0, lh = 0;
+                size_t n = 0;
+                s = ntx_hip_batch_level(b, images[j].entry, images[j].level, &lv, &lw, &lh, &blocks, &n);
+                if (s == NTX_OK) {
+                    bytes[j].resize(n);
+                    s = ntx_hip_download(ctx, bytes[j].data(), blocks, n);
+                }
+            }
+            if (b) ntx_hip_batch_destroy(b);
+        } else {
+            // BC3: each entry's chain through the chain scheduler.
+            size_t j = 0;
+            for (size_t i = 0; s == NTX_OK && i < items.size(); i++) {
+                const Entry &e = items[i];
+                const int levels = entry_levels(e);
+                std::vector<float *> lv(levels, nullptr);
+                std::vector<void *> out(levels, nullptr);
+                lv[0] = const_cast<float *>(e.img->deviceData());
+                for (int k = 0; s == NTX_OK && k < levels; k++) {
+                    const Image &im = images[j + k];
+                    if (k > 0) {
+                        void *p = nullptr;
+                        s = ntx_hip_malloc(ctx, 4 * size_t(im.w) * size_t(im.h) * sizeof(float), &p);
+                        lv[k] = static_cast<float *>(p);
+                    }
+                    if (s == NTX_OK) s = ntx_hip_malloc(ctx, ntx_hip_compressed_size(fmt, im.w, im.h), &out[k]);
+                }
+                float width, params[2];
+                filter_defaults(e.filter, &width, params);
+                if (s == NTX_OK)
+                    s = ntx_hip_compress_chain_device(ctx, fmt, int(co.m.quality), int(e.img->alphaMode()), co.m.colorWeight,
+                                                      int(e.filter), width, params, int(e.img->wrapMode()),
+                                                      e.img->width(), e.img->height(), levels, lv.data(), out.data());
+                if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+                for (int k = 0; s == NTX_OK && k < levels; k++) {
+                    const Image &im = images[j + k];
+                    bytes[j + k].resize(ntx_hip_compressed_size(fmt, im.w, im.h));
+                    s = ntx_hip_download(ctx, bytes[j + k].data(), out[k], bytes[j + k].size());
+                }
+                for (int k = 0; k < levels; k++) {
+                    if (k > 0 && lv[k]) ntx_hip_free(ctx, lv[k]);
+                    if (out[k]) ntx_hip_free(ctx, out[k]);
+                }
+                j += size_t(levels);
+            }
+        }
+    }
+    if (s != NTX_OK) {
+        report("compress(BatchList)", s);
+        return fail_all(s == NTX_ERR_INVALID_ARGUMENT ? Error_InvalidInput
+                        : s == NTX_ERR_UNSUPPORTED   ? Error_UnsupportedFeature
+                                                     : Error_CudaError);
+    }
+    // Context.cpp:486-516, per image, on this thread.
+    for (size_t j = 0; j < images.size(); j++) {
+        const Image &im = images[j];
+        const Entry &e = items[im.entry];
+        const OutputOptions::Private &oo = e.oo->m;
+        oo.beginImage(int(bytes[j].size()), im.w, im.h, 1, e.face, e.mipmap + im.level);
+        oo.writeData(bytes[j].data(), int(bytes[j].size()));
+        oo.endImage();
+    }
+    return true;
 }
 
 // Context.cpp:192-213 estimateSize for block formats.

@@ -8,6 +8,7 @@
 //       out.bin: per level int32 w, h, size, then size bytes; then int32 event count
 //                and the handler events (int32 code, six int32 arguments)
 //   nvtt_api_test api      checks error reporting, handler order and Surface semantics
+//   nvtt_api_test batch    Compressor::compress(BatchList) against per-image compress
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -198,11 +199,95 @@ int run_api()
     return 0;
 }
 
+// A deterministic BGRA8 image: gradients in B and G, hashed noise in R and A.
+bool make_surface(nvtt::Surface &s, int w, int h, uint32_t seed)
+{
+    std::vector<uint8_t> bgra(size_t(w) * h * 4);
+    uint32_t x = seed * 2654435761u + 1u;
+    for (int i = 0; i < w * h; i++) {
+        x = x * 1664525u + 1013904223u;
+        const int px = i % w, py = i / w;
+        bgra[4 * i + 0] = uint8_t((px * 255) / (w > 1 ? w - 1 : 1));
+        bgra[4 * i + 1] = uint8_t((py * 255) / (h > 1 ? h - 1 : 1));
+        bgra[4 * i + 2] = uint8_t(x >> 24);
+        bgra[4 * i + 3] = uint8_t(255 - ((x >> 16) & 0x3f));
+    }
+    return s.setImage(nvtt::InputFormat_BGRA_8UB, w, h, 1, bgra.data());
+}
+
+// Compressor::compress(BatchList) against compress(Surface) per image and
+// buildNextMipmap per level: same bytes and the same handler events, entry
+// by entry, for the persistent-batch formats and for BC3.
+int run_batch()
+{
+    const int sizes[4][2] = { { 128, 64 }, { 37, 90 }, { 256, 256 }, { 1, 16 } };
+    const nvtt::MipmapFilter filters[4] = { nvtt::MipmapFilter_Box, nvtt::MipmapFilter_Box, nvtt::MipmapFilter_Kaiser,
+                                            nvtt::MipmapFilter_Triangle };
+    const int chain_levels[4] = { 0, 1, 0, 3 };  // entry 1 is a single image (Append)
+    const nvtt::Format formats[4] = { nvtt::Format_BC1, nvtt::Format_BC4, nvtt::Format_BC5, nvtt::Format_BC3 };
+    nvtt::Compressor compressor;
+    CHECK(compressor.isCudaAccelerationEnabled());
+    for (nvtt::Format format : formats) {
+        nvtt::Surface surf[4];
+        for (int i = 0; i < 4; i++) CHECK(make_surface(surf[i], sizes[i][0], sizes[i][1], uint32_t(17 * i + 3)));
+        surf[3].setWrapMode(nvtt::WrapMode_Repeat);
+        nvtt::CompressionOptions co;
+        co.setFormat(format);
+        co.setQuality(nvtt::Quality_Normal);
+
+        Capture expect[4], got[4];
+        for (int i = 0; i < 4; i++) {
+            nvtt::Surface t = surf[i];
+            nvtt::OutputOptions oo;
+            oo.setOutputHandler(&expect[i]);
+            int levels = i == 1 ? 1 : t.countMipmaps();
+            if (chain_levels[i] > 0 && chain_levels[i] < levels) levels = chain_levels[i];
+            for (int m = 0; m < levels; m++) {
+                CHECK(compressor.compress(t, i, 5 + m, co, oo));
+                if (m + 1 < levels) CHECK(t.buildNextMipmap(filters[i]));
+            }
+        }
+        nvtt::OutputOptions oos[4];
+        Errors err;
+        nvtt::BatchList list;
+        for (int i = 0; i < 4; i++) {
+            oos[i].setOutputHandler(&got[i]);
+            oos[i].setErrorHandler(&err);
+            if (i == 1) list.Append(&surf[i], i, 5, &oos[i]);
+            else list.AppendChain(&surf[i], i, 5, filters[i], chain_levels[i], &oos[i]);
+        }
+        CHECK(list.GetSize() == 4);
+        CHECK(compressor.compress(list, co));
+        CHECK(err.seen.empty());
+        for (int i = 0; i < 4; i++) {
+            CHECK(got[i].bytes == expect[i].bytes);
+            CHECK(got[i].events == expect[i].events);
+        }
+    }
+    // Unsupported format: every image reports, none writes.
+    nvtt::Surface s;
+    CHECK(make_surface(s, 16, 16, 9));
+    nvtt::CompressionOptions co;
+    co.setFormat(nvtt::Format_BC7);
+    Capture cap;
+    Errors err;
+    nvtt::OutputOptions oo;
+    oo.setOutputHandler(&cap);
+    oo.setErrorHandler(&err);
+    nvtt::BatchList list;
+    list.AppendChain(&s, 0, 0, nvtt::MipmapFilter_Box, 0, &oo);
+    CHECK(compressor.compress(list, co));
+    CHECK(err.seen.size() == 5 && err.seen[0] == nvtt::Error_UnsupportedFeature && cap.bytes.empty());
+    std::printf("batch ok\n");
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc >= 2 && std::strcmp(argv[1], "chain") == 0) return run_chain(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "api") == 0) return run_api();
-    return fail("usage: nvtt_api_test chain|api ...");
+    if (argc >= 2 && std::strcmp(argv[1], "batch") == 0) return run_batch();
+    return fail("usage: nvtt_api_test chain|api|batch ...");
 }

@@ -94,3 +94,12 @@ def test_front_end_chain_matches_oracle(tmp_path, oracle, case):
 def test_front_end_api_semantics():
     r = subprocess.run([DRIVER, "api"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "api ok" in r.stdout, r.stderr
+
+
+def test_front_end_batch_list():
+    """Compressor::compress(BatchList): one call for several surfaces and
+    their mip chains gives every image's bytes and handler events exactly as
+    compress(Surface) per image does (BC1, BC4, BC5 through the persistent
+    batch launch; BC3 through the chain scheduler)."""
+    r = subprocess.run([DRIVER, "batch"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "batch ok" in r.stdout, r.stdout + r.stderr
