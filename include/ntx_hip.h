@@ -184,6 +184,17 @@ NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int forma
                                                  const float *filter_params, int wrap, int w, int h, int levels,
                                                  float *const *d_levels, void *const *d_out);
 
+/* n independent images (d_rgba[k]: w[k] x h[k] planar FP32), each compressed
+ * into d_out[k] exactly as ntx_hip_compress_device does it, the way
+ * Compressor::compress would be called once per image (Context.cpp:486-516)
+ * but scheduled together: the largest on the context's stream, the others on
+ * two internal streams beside it (the small images' latency-bound launches
+ * overlap the large ones).  All of it is ordered before later work on the
+ * context's stream. */
+NTX_API ntx_status ntx_hip_compress_images_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                                  const float *color_weights, int n, const float *const *d_rgba,
+                                                  const int *w, const int *h, void *const *d_out);
+
 /* Surface::toLinearFromSrgb (Surface.cpp:1562-1592) on the R,G,B planes, in place. */
 NTX_API ntx_status ntx_hip_srgb_to_linear_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
 /* Surface::setImage(InputFormat_BGRA_8UB, ...) (Surface.cpp:745-761):

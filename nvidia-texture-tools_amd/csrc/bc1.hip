@@ -12,8 +12,9 @@ using namespace ntx_bc1;
 __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                   int alpha_mode, float cwx, float cwy, float cwz,
                                                   const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
-                                                  short2 *__restrict__ nm)
+                                                  short2 *__restrict__ nm, int urgent)
 {
+    if (urgent) __builtin_amdgcn_s_setprio(3);
     __shared__ Bc1Shared sh;
     for (int i = threadIdx.x; i < 976; i += blockDim.x) sh.four[i] = tab->four[i];
     for (int i = threadIdx.x; i < 160; i += blockDim.x) sh.three[i] = tab->three[i];
@@ -35,8 +36,10 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
 
 namespace ntx {
 
+// urgent: the waves raise their issue priority (s_setprio), for small
+// latency-bound images compressed beside large ones.
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
-                      void *d_out, int16_t *d_nm, hipStream_t stream)
+                      void *d_out, int16_t *d_nm, hipStream_t stream, bool urgent)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4;
     const int nblocks = bw * bh;
@@ -44,7 +47,7 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
     const int grid = (nblocks + kBlocksPerWG - 1) / kBlocksPerWG;
     hipLaunchKernelGGL(bc1_kernel, dim3(grid), dim3(kBlocksPerWG * kGroup), 0, stream, d_src, w, h, bw, nblocks,
                        alpha_mode, cw[0], cw[1], cw[2], d_tab, reinterpret_cast<uint2 *>(d_out),
-                       reinterpret_cast<short2 *>(d_nm));
+                       reinterpret_cast<short2 *>(d_nm), urgent ? 1 : 0);
     return hipGetLastError();
 }
 

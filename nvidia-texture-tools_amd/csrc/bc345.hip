@@ -16,8 +16,9 @@ using namespace ntx_bc4;
 // One thread per (block, channel job).  Channel plane: 0 = R, 1 = G, 3 = A.
 __global__ __launch_bounds__(256) void channel_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                      int njobs, int ch0, int ch1, int stride, int off1,
-                                                     uint8_t *__restrict__ out)
+                                                     uint8_t *__restrict__ out, int urgent)
 {
+    if (urgent) __builtin_amdgcn_s_setprio(3);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nblocks * njobs) return;
     const int blk = t / njobs, job = t - blk * njobs;
@@ -591,7 +592,7 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
 namespace ntx {
 
 // BC4 (R), BC5 (R then G) or the BC3 alpha half (A), one thread per channel block.
-hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream)
+hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream, bool urgent)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
@@ -600,7 +601,7 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
     else if (format == NTX_FORMAT_BC3) { ch0 = 3; stride = 16; }
     const int threads = nblocks * njobs;
     hipLaunchKernelGGL(channel_kernel, dim3((threads + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
-                       njobs, ch0, ch1, stride, off1, reinterpret_cast<uint8_t *>(d_out));
+                       njobs, ch0, ch1, stride, off1, reinterpret_cast<uint8_t *>(d_out), urgent ? 1 : 0);
     return hipGetLastError();
 }
 

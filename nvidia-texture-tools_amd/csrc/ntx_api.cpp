@@ -385,8 +385,9 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
 {
     hipError_t e = hipSuccess;
     if (format == NTX_FORMAT_BC1) {
-        e = timed_on(ctx, stream, NTX_KERNEL_BC1,
-                     [&] { return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream); });
+        e = timed_on(ctx, stream, NTX_KERNEL_BC1, [&] {
+            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent);
+        });
     } else if (format == NTX_FORMAT_BC3) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         int *counts = bucket, *lists = counts + bc3_lists_offset();
@@ -407,7 +408,8 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
             });
         if (e == hipSuccess) *counts_zero = true;
     } else {
-        e = timed_on(ctx, stream, NTX_KERNEL_CHANNEL, [&] { return launch_channels(d_rgba, w, h, format, d_out, stream); });
+        e = timed_on(ctx, stream, NTX_KERNEL_CHANNEL,
+                     [&] { return launch_channels(d_rgba, w, h, format, d_out, stream, urgent); });
     }
     return e;
 }
@@ -505,6 +507,88 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     return NTX_OK;
 }
 
+}  // extern "C"
+
+namespace ntx {
+
+// n independent images compressed concurrently, in the caller's order (the
+// largest first): image 0 on the context's stream, images 1 and 2 on side
+// stream 1, the rest (short, latency-bound) one after another on side stream
+// 0 with raised wave priority, so their launches fill the gaps the large
+// ones leave rather than being stretched by the large searches sharing every
+// SIMD.  Images after the first aim their BC3 search at 2^16 lanes, since
+// they run beside other work.  The side streams start once `ready` (recorded
+// on the context's stream) has passed; the context's stream then waits for
+// both.  Three streams: with the default four hardware queues per process,
+// one of which the null stream takes, more streams would share queues and
+// run one after another.  BC3 scratch: one region per image in ctx->chain,
+// laid out by the image sizes.
+static ntx_status compress_many(ntx_hip_context *ctx, int format, int alpha_mode, const float *cw, int n,
+                                const float *const *d_in, const int *w, const int *h, void *const *d_out)
+{
+    ntx_status s = NTX_OK;
+    std::vector<size_t> region(n, 0);
+    std::vector<int> layout;
+    size_t total = 0;
+    for (int k = 0; k < n; k++) {
+        region[k] = total;
+        layout.push_back(w[k]);
+        layout.push_back(h[k]);
+        if (format == NTX_FORMAT_BC3)
+            total += (size_t(bc3_scratch_ints(((w[k] + 3) / 4) * ((h[k] + 3) / 4))) + 63) & ~size_t(63);
+    }
+    if (total) {
+        const size_t cap_before = ctx->chain.cap;
+        if ((s = ensure(ctx, ctx->chain, sizeof(int) * total))) return s;
+        // The tallies are known to be zero only in the same memory with the
+        // same region layout as the last complete run.
+        if (ctx->chain.cap != cap_before || ctx->chain_layout != layout) ctx->chain_counts_zero = false;
+        ctx->chain_layout = layout;
+    }
+    for (int i = 0; i < ntx_hip_context::kSideStreams; i++)
+        if (!ctx->side[i]) NTX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+    hipStream_t small = ctx->side[0], second = ctx->side[1];
+    hipEvent_t ready = sync_event(ctx, 0), ev_small = sync_event(ctx, 1), ev_second = sync_event(ctx, 2);
+    if (!ready || !ev_small || !ev_second) return fail(ctx, hipErrorOutOfMemory, "schedule events");
+    bool zero = ctx->chain_counts_zero;
+    ctx->chain_counts_zero = false;
+    bool all_zero = true;
+    auto compress_one = [&](hipStream_t st, int k) -> ntx_status {
+        int *bucket = format == NTX_FORMAT_BC3 ? static_cast<int *>(ctx->chain.ptr) + region[k] : nullptr;
+        bool z = zero;
+        hipError_t e = compress_on(ctx, st, format, alpha_mode, cw, d_in[k], w[k], h[k], d_out[k], bucket, &z,
+                                   k ? 16 : 18, st == small);
+        if (e != hipSuccess) return fail(ctx, e, "compress launch");
+        all_zero = all_zero && z;
+        return NTX_OK;
+    };
+    if (n > 1) NTX_HIP(ctx, hipEventRecord(ready, ctx->stream));
+    // Image 0 is enqueued first: the host takes a while to enqueue the many
+    // small launches, and the longest compression should not wait for that.
+    if ((s = compress_one(ctx->stream, 0))) return s;
+    if (n > 1) {
+        NTX_HIP(ctx, hipStreamWaitEvent(second, ready, 0));
+        for (int k = 1; k < n && k <= 2; k++)
+            if ((s = compress_one(second, k))) return s;
+        NTX_HIP(ctx, hipEventRecord(ev_second, second));
+    }
+    if (n > 3) {
+        NTX_HIP(ctx, hipStreamWaitEvent(small, ready, 0));
+        for (int k = 3; k < n; k++)
+            if ((s = compress_one(small, k))) return s;
+        NTX_HIP(ctx, hipEventRecord(ev_small, small));
+    }
+    if (n > 1) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_second, 0));
+    if (n > 3) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_small, 0));
+    // Every image's search cleared its own region.
+    ctx->chain_counts_zero = all_zero;
+    return NTX_OK;
+}
+
+}  // namespace ntx
+
+extern "C" {
+
 NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
                                                  const float *cw, int filter, float filter_width,
                                                  const float *filter_params, int wrap, int w, int h, int levels,
@@ -523,78 +607,48 @@ NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int forma
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
     if (!cw) cw = ones;
     std::vector<int> lw(levels), lh(levels);
-    std::vector<size_t> region(levels, 0);
-    size_t total = 0;
     for (int k = 0; k < levels; k++) {
         lw[k] = k ? (lw[k - 1] / 2 > 1 ? lw[k - 1] / 2 : 1) : w;
         lh[k] = k ? (lh[k - 1] / 2 > 1 ? lh[k - 1] / 2 : 1) : h;
-        region[k] = total;
-        if (format == NTX_FORMAT_BC3)
-            total += (size_t(ntx::bc3_scratch_ints(((lw[k] + 3) / 4) * ((lh[k] + 3) / 4))) + 63) & ~size_t(63);
     }
-    if (total) {
-        const size_t cap_before = ctx->chain.cap;
-        if ((s = ensure(ctx, ctx->chain, sizeof(int) * total))) return s;
-        // The tallies are known to be zero only in the same memory with the
-    // same region layout as the last complete chain.
-        const std::tuple<int, int, int> layout{ w, h, levels };
-        if (ctx->chain.cap != cap_before || ctx->chain_layout != layout) ctx->chain_counts_zero = false;
-        ctx->chain_layout = layout;
-    }
-    for (int i = 0; i < ntx_hip_context::kSideStreams; i++)
-        if (!ctx->side[i]) NTX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
-
-    // Schedule.  The main stream builds every level first: the filter's
+    // Every level is built first on the context's stream: the filter's
     // workgroups need 50 KB of LDS each and would wait behind any search
-    // running beside them.  Then the levels compress concurrently: level 0 on
-    // the main stream, levels 1 and 2 on side stream 1, levels 3 to n-1
-    // (short, latency-bound) on side stream 0, beside the large ones.  (Three
-    // streams: with the default four hardware queues per process, one of
-    // which the null stream takes, more streams would share queues and run
-    // one after another.)  The main stream then waits for both side streams.
-    hipStream_t small = ctx->side[0], second = ctx->side[1];
-    bool zero = ctx->chain_counts_zero;
-    ctx->chain_counts_zero = false;
-    bool all_zero = true;
-    // Levels below level 0 run beside other work, so their searches aim at
-    // 2^16 lanes rather than filling the device alone; the small levels'
-    // (latency-bound) launches also raise their waves' issue priority, or the
-    // large searches sharing every SIMD would stretch each of them.
-    auto compress_level = [&](hipStream_t st, int k) -> ntx_status {
-        int *bucket = format == NTX_FORMAT_BC3 ? static_cast<int *>(ctx->chain.ptr) + region[k] : nullptr;
-        bool level_zero = zero;
-        hipError_t e = ntx::compress_on(ctx, st, format, alpha_mode, cw, d_levels[k], lw[k], lh[k], d_out[k], bucket,
-                                        &level_zero, k ? 16 : 18, st == small);
-        if (e != hipSuccess) return fail(ctx, e, "chain compress launch");
-        all_zero = all_zero && level_zero;
-        return NTX_OK;
-    };
-    hipEvent_t ev_all = ntx::sync_event(ctx, 0), ev_small = ntx::sync_event(ctx, 1), ev_second = ntx::sync_event(ctx, 2);
-    if (!ev_all || !ev_small || !ev_second) return fail(ctx, hipErrorOutOfMemory, "chain events");
+    // running beside them.  Then the levels compress concurrently, largest
+    // first (compress_many).
     if ((s = ntx::downsample_chain_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode,
                                       lw.data(), lh.data(), levels, d_levels)))
         return s;
-    if (levels > 1) NTX_HIP(ctx, hipEventRecord(ev_all, ctx->stream));
-    // Level 0 is enqueued first: the host takes a while to enqueue the many
-    // small launches, and the longest compression should not wait for that.
-    if ((s = compress_level(ctx->stream, 0))) return s;
-    if (levels > 1) {
-        NTX_HIP(ctx, hipStreamWaitEvent(second, ev_all, 0));
-        for (int k = 1; k < levels && k <= 2; k++)
-            if ((s = compress_level(second, k))) return s;
-        NTX_HIP(ctx, hipEventRecord(ev_second, second));
+    return ntx::compress_many(ctx, format, alpha_mode, cw, levels, d_levels, lw.data(), lh.data(), d_out);
+}
+
+NTX_API ntx_status ntx_hip_compress_images_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                                  const float *cw, int n, const float *const *d_rgba, const int *w,
+                                                  const int *h, void *const *d_out)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (n < 0 || (n > 0 && (!d_rgba || !w || !h || !d_out))) return NTX_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < n; k++)
+        if (w[k] < 1 || h[k] < 1 || !d_rgba[k] || !d_out[k]) return NTX_ERR_INVALID_ARGUMENT;
+    if (!supported(format, quality)) return NTX_ERR_UNSUPPORTED;
+    if (n == 0) return NTX_OK;
+    const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
+    if (!cw) cw = ones;
+    // Largest first (stable), the order compress_many schedules by.
+    std::vector<int> order(n);
+    for (int k = 0; k < n; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return size_t(w[a]) * size_t(h[a]) > size_t(w[b]) * size_t(h[b]); });
+    std::vector<const float *> in(n);
+    std::vector<int> sw(n), sh(n);
+    std::vector<void *> out(n);
+    for (int k = 0; k < n; k++) {
+        in[k] = d_rgba[order[k]];
+        sw[k] = w[order[k]];
+        sh[k] = h[order[k]];
+        out[k] = d_out[order[k]];
     }
-    if (levels > 3) {
-        NTX_HIP(ctx, hipStreamWaitEvent(small, ev_all, 0));
-        for (int k = 3; k < levels; k++)
-            if ((s = compress_level(small, k))) return s;
-        NTX_HIP(ctx, hipEventRecord(ev_small, small));
-    }
-    if (levels > 1) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_second, 0));
-    if (levels > 3) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_small, 0));
-    // Every level's search cleared its own region.
-    ctx->chain_counts_zero = all_zero;
-    return NTX_OK;
+    return ntx::compress_many(ctx, format, alpha_mode, cw, n, in.data(), sw.data(), sh.data(), out.data());
 }
 
 NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h, const void *d_blocks,

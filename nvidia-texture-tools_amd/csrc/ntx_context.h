@@ -12,8 +12,9 @@
 
 namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
-                      int16_t *d_nm, hipStream_t stream);
-hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream);
+                      int16_t *d_nm, hipStream_t stream, bool urgent = false);
+hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
+                           bool urgent = false);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
                             int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
                             hipStream_t stream);
@@ -70,15 +71,15 @@ struct ntx_hip_context {
     // The BC3 tallies at the start of `bucket` are zero: set after a complete
     // front + search sequence (the search clears them), cleared before one.
     bool bc3_counts_zero = false;
-    // Mip chains (ntx_hip_compress_chain_device): streams the levels'
-    // compression runs on, fork/join events, and per-level BC3 scratch
-    // (every region's tallies zero when chain_counts_zero).
+    // Concurrent compression of several images (mip chains, image sets):
+    // the side streams, fork/join events, and per-image BC3 scratch (every
+    // region's tallies zero when chain_counts_zero).
     static constexpr int kSideStreams = 2;
     hipStream_t side[kSideStreams] = {};
     std::vector<hipEvent_t> sync_events;
     ntx::DevBuf chain;
     bool chain_counts_zero = false;
-    std::tuple<int, int, int> chain_layout{ 0, 0, 0 };  // (w, h, levels) the regions were laid out for
+    std::vector<int> chain_layout;  // the image sizes the regions were laid out for
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;

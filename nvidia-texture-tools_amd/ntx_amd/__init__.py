@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional, Sequence
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -41,7 +41,7 @@ EXPORTS = (
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
-    "ntx_hip_compress_chain_device",
+    "ntx_hip_compress_chain_device", "ntx_hip_compress_images_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
         "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
+        "ntx_hip_compress_images_device": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_downsample_device": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_downsample_chain_device": (_I, [_P, _I, _F, _P, _I, _I, _I, _I, _I, _P]),
@@ -282,6 +283,22 @@ class Context:
                                                     _ptr(pp) if pp is not None else None, wrap, w, h, len(d_levels),
                                                     ctypes.cast(lv, _P), ctypes.cast(out, _P))
         self._check(st, "ntx_hip_compress_chain_device")
+
+    def compress_images_device(self, fmt: int, d_rgba: Sequence[DeviceBuffer], sizes: Sequence[Tuple[int, int]],
+                               d_out: Sequence[DeviceBuffer], quality: int = QUALITY_NORMAL,
+                               alpha_mode: int = ALPHA_NONE, color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        """Independent images compressed together (ntx_hip_compress_images_device)."""
+        n = len(d_rgba)
+        if len(sizes) != n or len(d_out) != n:
+            raise ValueError("one size and one output buffer per image")
+        cw = _f32c(color_weights)
+        ins = (_P * n)(*[d.ptr for d in d_rgba])
+        outs = (_P * n)(*[d.ptr for d in d_out])
+        ws = np.array([a for a, _ in sizes], np.int32)
+        hs = np.array([b for _, b in sizes], np.int32)
+        st = self.lib.ntx_hip_compress_images_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), n,
+                                                     ctypes.cast(ins, _P), _ptr(ws), _ptr(hs), ctypes.cast(outs, _P))
+        self._check(st, "ntx_hip_compress_images_device")
 
     def compress_bc1_stats_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
                                   d_nm: DeviceBuffer, alpha_mode: int = ALPHA_NONE,

@@ -114,3 +114,26 @@ def test_downsample_chain_matches_per_level(gpu, filt, wrap, size):
             got = d.download(np.float32)
             assert np.array_equal(got.view(np.uint32), ref[k].view(np.uint32)), (k, sizes[k], alpha_mode)
             d.free()
+
+
+@pytest.mark.parametrize("fmt", [N.FORMAT_BC1, N.FORMAT_BC3, N.FORMAT_BC4, N.FORMAT_BC5])
+def test_images_match_per_image_calls(gpu, fmt):
+    """ntx_hip_compress_images_device: unordered images of mixed sizes
+    (largest scheduled first, the rest on the side streams) give the
+    per-image bytes; run twice to reuse the per-image scratch regions."""
+    sizes = [(37, 23), (256, 192), (1, 1), (64, 64), (130, 66), (8, 4), (512, 128)]
+    imgs = [_image(fmt, w, h, seed=3 * i + fmt) for i, (w, h) in enumerate(sizes)]
+    d_in = [gpu.alloc(x.nbytes).upload(x) for x in imgs]
+    ref = []
+    for d, (w, h) in zip(d_in, sizes):
+        out = gpu.alloc(N.compressed_size(fmt, w, h))
+        gpu.compress_device(fmt, d, w, h, out)
+        ref.append(out.download(np.uint8))
+        out.free()
+    d_out = [gpu.alloc(N.compressed_size(fmt, w, h)) for w, h in sizes]
+    for rep in range(2):
+        gpu.compress_images_device(fmt, d_in, sizes, d_out)
+        for k, d in enumerate(d_out):
+            assert np.array_equal(d.download(np.uint8), ref[k]), (rep, sizes[k])
+    for d in d_in + d_out:
+        d.free()
