@@ -91,11 +91,13 @@ __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 // 8-bit texels are stored for the search as one 64-byte line per block,
 // R | G << 8 | B << 16 | A << 24 (A = 255 unless alpha weights the fit).  A
 // finished block is written as one 16-byte store when the output is 16-byte
-// aligned.
+// aligned; inner blocks read their rows as 16-byte loads when the rows are
+// aligned (rows16).
 __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                         const ntx::Tables *__restrict__ tab, uint8_t *__restrict__ out,
                                                         int wide, int *__restrict__ counts, int *__restrict__ lists, int cap,
-                                                        int weight_by_alpha, uint4 *__restrict__ packed, int urgent)
+                                                        int weight_by_alpha, uint4 *__restrict__ packed, int urgent,
+                                                        int rows16)
 {
     if (urgent) __builtin_amdgcn_s_setprio(3);
     __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
@@ -108,15 +110,34 @@ __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict_
         const int bx = blk % bw, by = blk / bw;
         const size_t plane = size_t(w) * size_t(h);
         uint32_t av[4];
-        gather_channel(src + 3 * plane, w, h, bx, by, av);
+        uint32_t rgb[16];
+        if (rows16 && 4 * bx + 4 <= w && 4 * by + 4 <= h) {
+            // Inner block, 16-byte aligned rows: each row of each plane is one
+            // 16-byte load (the same texels ColorBlock::init gathers).
+            const float *p0 = src + size_t(4 * by) * size_t(w) + size_t(4 * bx);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float *p = p0 + size_t(r) * size_t(w);
+                const float4 vr = *reinterpret_cast<const float4 *>(p);
+                const float4 vg = *reinterpret_cast<const float4 *>(p + plane);
+                const float4 vb = *reinterpret_cast<const float4 *>(p + 2 * plane);
+                const float4 va = *reinterpret_cast<const float4 *>(p + 3 * plane);
+                rgb[4 * r + 0] = to_u8(vr.x) | to_u8(vg.x) << 8 | to_u8(vb.x) << 16;
+                rgb[4 * r + 1] = to_u8(vr.y) | to_u8(vg.y) << 8 | to_u8(vb.y) << 16;
+                rgb[4 * r + 2] = to_u8(vr.z) | to_u8(vg.z) << 8 | to_u8(vb.z) << 16;
+                rgb[4 * r + 3] = to_u8(vr.w) | to_u8(vg.w) << 8 | to_u8(vb.w) << 16;
+                av[r] = to_u8(va.x) | to_u8(va.y) << 8 | to_u8(va.z) << 16 | to_u8(va.w) << 24;
+            }
+        } else {
+            gather_channel(src + 3 * plane, w, h, bx, by, av);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const size_t o = block_texel(w, h, bx, by, i);
+                rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
+            }
+        }
         const uint64_t alpha = compress_channel(av);
         uint8_t *dst = out + size_t(blk) * 16;
-        uint32_t rgb[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const size_t o = block_texel(w, h, bx, by, i);
-            rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
-        }
         bool single = true;
 #pragma unroll
         for (int i = 1; i < 16; i++) single = single && rgb[i] == rgb[0];
@@ -619,9 +640,11 @@ hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_ta
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
     const int wide = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    // Every row of every plane starts 16-byte aligned: float4 row loads.
+    const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
     hipLaunchKernelGGL(bc3_front_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
                        reinterpret_cast<uint8_t *>(d_out), wide, d_counts, d_lists, cap, weight_by_alpha,
-                       reinterpret_cast<uint4 *>(d_packed), urgent ? 1 : 0);
+                       reinterpret_cast<uint4 *>(d_packed), urgent ? 1 : 0, rows16);
     return hipGetLastError();
 }
 

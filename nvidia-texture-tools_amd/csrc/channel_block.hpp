@@ -134,10 +134,15 @@ __device__ __forceinline__ uint64_t compress_channel(const uint32_t vq[4])
         mx = v > mx ? v : mx;
         mn = v < mn ? v : mn;
     }
+    // One value: both endpoints are it, every texel takes palette entry 0
+    // (first on ties) with error 0, and the refinement below stops at its
+    // first step (no error is below 0): the same block, without the work.
+    if (mx == mn) return uint64_t(mx) | uint64_t(mx) << 8;
     const uint32_t d = (mx - mn) / 34;
     uint64_t blk = uint64_t((mx - d) & 0xff) | uint64_t((mn + d) & 0xff) << 8;
     uint32_t besterr = alpha_indices(vq, blk);
     uint64_t best = blk;
+    if (besterr == 0) return best;  // likewise: the first refinement step stops
 #pragma unroll 1
     for (int it = 0; it < 8; it++) {
         optimize_alpha8(vq, blk);
