@@ -522,9 +522,11 @@ def bench_learned(ctx, args):
     psnr_plain = ctx.bc1_psnr_device(d, S, S, d_plain)
     runs = []
     for cfg in (dict(greedy_rounds=1), dict(greedy_rounds=1, refine_iters=1), dict(greedy_rounds=1, refine_iters=0),
+                dict(greedy_rounds=0, refine_iters=1), dict(greedy_rounds=0, refine_iters=0),
                 dict(greedy_rounds=1, tau_gap=0.5), dict(greedy_rounds=2),
                 dict(greedy_rounds=1, box_candidates=True), dict(greedy_rounds=1, refine_iters=1, box_candidates=True),
-                dict(greedy_rounds=1, refine_iters=0, box_candidates=True), dict(greedy_rounds=2, box_candidates=True)):
+                dict(greedy_rounds=1, refine_iters=0, box_candidates=True), dict(greedy_rounds=0, refine_iters=0,
+                box_candidates=True), dict(greedy_rounds=2, box_candidates=True)):
         kw = dict(tau_pred=1e9, tau_gap=-1.0, refine_iters=2)
         kw.update(cfg)
         fb = net.compress_bc1_device(d, S, S, d_mlp, **kw)

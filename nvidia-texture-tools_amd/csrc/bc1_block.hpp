@@ -13,6 +13,7 @@
 #include "ntx_common.h"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace ntx_bc1 {
 
@@ -20,7 +21,10 @@ constexpr int kGroup = 16;
 constexpr int kBlocksPerWG = 16;
 
 struct Bc1Shared {
-    float4 sat[kBlocksPerWG][16];    // (r, g, b, w) prefix sums in projection order
+    // Row 0 is zero; row i + 1 holds the (r, g, b, w) prefix sums over the
+    // first i + 1 clusters in projection order, so a cluster-table entry
+    // (which stores cluster index + 1, 0 for none) indexes it directly.
+    float4 sat[kBlocksPerWG][17];
     float4 col[kBlocksPerWG][16];    // scratch: sorted products for the SAT scan
     float dps[kBlocksPerWG][16];
     int order[kBlocksPerWG][16];
@@ -40,6 +44,36 @@ __device__ __forceinline__ float round6(float x) { return rintf(x * 63.0f) * (1.
 __device__ __forceinline__ float gshfl(float v, int src) { return __shfl(v, src, kGroup); }
 __device__ __forceinline__ int gshfl(int v, int src) { return __shfl(v, src, kGroup); }
 
+// Lane I of this group's 16-lane row, read by every lane of the row (DPP
+// row_newbcast, one VALU op).  Every branch around these reads is uniform
+// over a group, so the source lane is always active.
+template <int I>
+__device__ __forceinline__ float gbcast(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + I, 0xf, 0xf, false));
+}
+// The value of the lane R places further on in the row, cyclically (DPP row_ror).
+template <int R>
+__device__ __forceinline__ int gror(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x120 + R, 0xf, 0xf, false);
+}
+template <int R>
+__device__ __forceinline__ float gror(float v)
+{
+    return __int_as_float(gror<R>(__float_as_int(v)));
+}
+
+// f(std::integral_constant<int, i>) for i = I .. N-1, in order.
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
 // Mask of the lanes of this group for which pred holds, bit i = lane i.
 __device__ __forceinline__ uint32_t gballot(bool pred)
 {
@@ -53,21 +87,18 @@ struct Cand {
 };
 
 // One least-squares candidate, icbc.h:2501-2538 (four colours) or
-// :2201-2236 (three colours).  FLT_MAX SAT entries past the colour count are
-// read exactly as the SSE build reads them.
+// :2201-2236 (three colours).  sat is the zero-led table: entry 0 gives the
+// zeros icbc uses for an absent cluster, and the FLT_MAX entries past the
+// colour count are read exactly as the SSE build reads them.
 template <bool FOUR>
 __device__ __forceinline__ float eval_candidate(const float4 *sat, uint32_t packed, float4 sum, float mx, float my,
                                                 float mz, Cand *out)
 {
-    const int i0 = int(packed & 0xff) - 1;
-    const int i1 = int((packed >> 8) & 0xff) - 1;
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 x0 = i0 >= 0 ? sat[i0] : z;
-    float4 x1 = i1 >= 0 ? sat[i1] : z;
+    const float4 x0 = sat[packed & 0xff];
+    const float4 x1 = sat[(packed >> 8) & 0xff];
     float alpha2, beta2, ab, axr, axg, axb;
     if (FOUR) {
-        const int i2 = int((packed >> 16) & 0xff) - 1;
-        float4 x2 = i2 >= 0 ? sat[i2] : z;
+        const float4 x2 = sat[(packed >> 16) & 0xff];
         const float w3 = sum.w - x2.w;
         const float x2r = x2.x - x1.x, x2g = x2.y - x1.y, x2b = x2.z - x1.z;
         const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
@@ -137,11 +168,14 @@ __device__ void cluster_fit(const float4 *sat, const uint32_t *table, int total,
         if (e < be) { be = e; bp = p; }
     }
     // Each lane saw p & 3 == lane & 3 in increasing order; combine lanes.
-    for (int off = 8; off >= 1; off >>= 1) {
-        const float oe = gshfl(be, lane ^ off);
-        const int op = gshfl(bp, lane ^ off);
+    // be is never NaN, so the order is total and any combining order gives
+    // the same winner.
+    static_for<0, 4>([&](auto k) {
+        constexpr int r = 8 >> decltype(k)::value;
+        const float oe = gror<r>(be);
+        const int op = gror<r>(bp);
         if (key_less(oe, op, be, bp)) { be = oe; bp = op; }
-    }
+    });
     if (!(be < FLT_MAX)) {
         // No lane improved on FLT_MAX: the SSE build returns lane 0's initial zeros.
         best->ax = best->ay = best->az = best->bx = best->by = best->bz = 0.0f;
@@ -235,29 +269,35 @@ __device__ float output_block(const Cand &c, float tr, float tg, float tb, float
     const float eb = ((p[idx][2] - tb) * cwz) * 255.0f;
     const float term = tw * (er * er + eg * eg + eb * eb);
     float err = 0.0f;
-    for (int i = 0; i < 16; i++) err += gshfl(term, i);
-    uint32_t bits = idx << (2 * lane);
-    for (int off = 8; off >= 1; off >>= 1) bits |= uint32_t(gshfl(int(bits), lane ^ off));
+    static_for<0, 16>([&](auto i) { err += gbcast<decltype(i)::value>(term); });
+    int bits = int(idx << (2 * lane));
+    bits |= gror<8>(bits);
+    bits |= gror<4>(bits);
+    bits |= gror<2>(bits);
+    bits |= gror<1>(bits);
     blk->x = c0 | (c1 << 16);
-    blk->y = bits;
+    blk->y = uint32_t(bits);
     return err;
 }
 
 // icbc.h:1710-1882: PCA (power method) of the n clusters held one per lane,
 // stable sort by projection, and the weighted prefix-sum table in LDS.
 // Returns false only for an empty set.
-__device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, float kg, float kb, float kw)
+__device__ __forceinline__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, float kg, float kb, float kw)
 {
     const bool live = lane < n;
     // Centroid: sequential sums over clusters 0..n-1 (computeCentroid).
     const float pr = kr * kw, pg = kg * kw, pb = kb * kw;
     float total = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    for (int j = 0; j < n; j++) {
-        total += gshfl(kw, j);
-        cr += gshfl(pr, j);
-        cg += gshfl(pg, j);
-        cb += gshfl(pb, j);
-    }
+    static_for<0, 16>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j < n) {
+            total += gbcast<j>(kw);
+            cr += gbcast<j>(pr);
+            cg += gbcast<j>(pg);
+            cb += gbcast<j>(pb);
+        }
+    });
     const float inv = 1.0f / total;
     cr *= inv; cg *= inv; cb *= inv;
     // Covariance (computeCovariance).
@@ -265,10 +305,13 @@ __device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, flo
     const float br = ar * kw, bg = ag * kw, bb = ab * kw;
     const float q0 = ar * br, q1 = ar * bg, q2 = ar * bb, q3 = ag * bg, q4 = ag * bb, q5 = ab * bb;
     float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f, m5 = 0.0f;
-    for (int j = 0; j < n; j++) {
-        m0 += gshfl(q0, j); m1 += gshfl(q1, j); m2 += gshfl(q2, j);
-        m3 += gshfl(q3, j); m4 += gshfl(q4, j); m5 += gshfl(q5, j);
-    }
+    static_for<0, 16>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j < n) {
+            m0 += gbcast<j>(q0); m1 += gbcast<j>(q1); m2 += gbcast<j>(q2);
+            m3 += gbcast<j>(q3); m4 += gbcast<j>(q4); m5 += gbcast<j>(q5);
+        }
+    });
     // firstEigenVector_PowerMethod (icbc.h:1767-1791).
     float vx = 0.0f, vy = 0.0f, vz = 0.0f;
     if (!(m0 == 0 && m3 == 0 && m5 == 0)) {
@@ -294,10 +337,13 @@ __device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, flo
     const bool has_nan = gballot(live && dp != dp) != 0;
     int rank = 0;
     if (!has_nan) {
-        for (int j = 0; j < n; j++) {
-            const float o = gshfl(dp, j);
-            rank += (o < dp || (o == dp && j < lane)) ? 1 : 0;
-        }
+        static_for<0, 16>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j < n) {
+                const float o = gbcast<j>(dp);
+                rank += (o < dp || (o == dp && j < lane)) ? 1 : 0;
+            }
+        });
         if (live) sh.order[g][rank] = lane;
     } else {
         if (live) { sh.dps[g][lane] = dp; sh.order[g][lane] = lane; }
@@ -322,7 +368,8 @@ __device__ void compute_sat(Bc1Shared &sh, int g, int lane, int n, float kr, flo
         const float4 v = sh.col[g][i];
         acc.x = acc.x + v.x; acc.y = acc.y + v.y; acc.z = acc.z + v.z; acc.w = acc.w + v.w;
     }
-    sh.sat[g][lane] = live ? acc : make_float4(FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX);
+    sh.sat[g][lane + 1] = live ? acc : make_float4(FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX);
+    if (lane == 0) sh.sat[g][0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ntx_wave_sync();
 }
 
@@ -371,8 +418,9 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
     float kr = 0.0f, kg = 0.0f, kb = 0.0f, kw = 0.0f;
     int n = 0;
     bool any_black = false;
-    for (int i = 0; i < 16; i++) {
-        const float ir = gshfl(tr, i), ig = gshfl(tg, i), ib = gshfl(tb, i), iw = gshfl(tw, i);
+    static_for<0, 16>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const float ir = gbcast<i>(tr), ig = gbcast<i>(tg), ib = gbcast<i>(tb), iw = gbcast<i>(tw);
         if (iw > 0) {
             const bool match = lane < n && fabsf(kr - ir) < thr && fabsf(kg - ig) < thr && fabsf(kb - ib) < thr;
             const uint32_t m = gballot(match);
@@ -389,7 +437,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
             }
             if (ir < 1.0f / 8 && ig < 1.0f / 8 && ib < 1.0f / 8) any_black = true;
         }
-    }
+    });
 
     const float cwsq_x = cwx * cwx, cwsq_y = cwy * cwy, cwsq_z = cwz * cwz;
     const float *mid5 = tab->mid5;
@@ -403,7 +451,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
     } else {
         // compress_dxt1_cluster_fit (icbc.h:3292-3326).
         compute_sat(sh, g, lane, n, kr, kg, kb, kw);
-        float4 sum = sh.sat[g][n - 1];
+        float4 sum = sh.sat[g][n];
         Cand c;
         cluster_fit<true>(sh.sat[g], sh.four, tab->four_total[n - 1], lane, sum, cwsq_x, cwsq_y, cwsq_z, &c);
         uint2 b4;
@@ -429,7 +477,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
                 }
                 const float nr = gshfl(kr, srcl), ng = gshfl(kg, srcl), nb = gshfl(kb, srcl), nw = gshfl(kw, srcl);
                 compute_sat(sh, g, lane, m, nr, ng, nb, nw);
-                sum = sh.sat[g][m - 1];
+                sum = sh.sat[g][m];
             }
         }
         if (run3) {

@@ -53,10 +53,15 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdg
 // (2 x 4), then the transposed W3^T (4 x 2) and W2^T (4 x 4) for backward.
 constexpr int kF1 = 0, kF2 = 12, kF3 = 28, kF3T = 36, kF2T = 44, kFrags = 60;
 
-struct Net {
-    v4s frag[kFrags][64];
+// The workgroup's copy of the network: F fragments (all kFrags for
+// training, the forward ones, kF3T, for inference) and the biases.
+template <int F>
+struct NetT {
+    v4s frag[F][64];
     float b1[kHidden], b2[kHidden], b3[kOutPad];
 };
+using Net = NetT<kFrags>;
+using ForwardNet = NetT<kF3T>;
 
 // A fragment (tm, ts) of a rows x cols row-major matrix M (or of its
 // transpose): lane l holds M[16 tm + l % 16][16 ts + 4 (l / 16) + j].
@@ -76,9 +81,10 @@ __device__ __forceinline__ v4s load_frag(const float *M, int rows, int cols, boo
 }
 
 // Builds the workgroup's copy of the network from the FP32 parameters.
-__device__ void load_net(Net &net, const float *__restrict__ p, int t, int nthreads)
+template <int F>
+__device__ void load_net(NetT<F> &net, const float *__restrict__ p, int t, int nthreads)
 {
-    for (int i = t; i < kFrags * 64; i += nthreads) {
+    for (int i = t; i < F * 64; i += nthreads) {
         const int f = i / 64, lane = i % 64;
         v4s v;
         if (f < kF2) v = load_frag(p + kW1, kHidden, kIn, false, f / 3, f % 3, lane);
@@ -100,7 +106,8 @@ __device__ void load_net(Net &net, const float *__restrict__ p, int t, int nthre
 // Returns the post-ReLU hidden activations as FP32 values of their BF16
 // roundings (what the next layer and the weight gradients consume) and the
 // 32 output rows (25 used).
-__device__ __forceinline__ void forward(const Net &net, const v4s bx[3], int lane, v4f h1[4], v4f h2[4], v4f y[2])
+template <int F>
+__device__ __forceinline__ void forward(const NetT<F> &net, const v4s bx[3], int lane, v4f h1[4], v4f h2[4], v4f y[2])
 {
     const int g = lane >> 4;
     const v4f zero = { 0.0f, 0.0f, 0.0f, 0.0f };
@@ -611,9 +618,10 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(float *__restrict__ param
 // ------------------------------------------------------------ compression --
 
 // Least-squares refinement of one endpoint pair for the block's texels
-// (4-colour index by projection onto the segment, then the 2x2 normal
-// equations), `iters` times; endpoints stay in [0, 1].
-__device__ __forceinline__ void refine(const float *tb, float a[3], float b[3], int iters)
+// t4[i] = (r, g, b, weight), weight 0 outside the image (4-colour index by
+// projection onto the segment, then the 2x2 normal equations), `iters`
+// times; endpoints stay in [0, 1].
+__device__ __forceinline__ void refine(const float4 *t4, float a[3], float b[3], int iters)
 {
     for (int it = 0; it < iters; it++) {
         const float d[3] = { b[0] - a[0], b[1] - a[1], b[2] - a[2] };
@@ -621,14 +629,14 @@ __device__ __forceinline__ void refine(const float *tb, float a[3], float b[3], 
         const float il = len2 > 0.0f ? 1.0f / len2 : 0.0f;
         float saa = 0.0f, sbb = 0.0f, sab = 0.0f, sac[3] = { 0, 0, 0 }, sbc[3] = { 0, 0, 0 };
         for (int t = 0; t < 16; t++) {
-            const float t0 = tb[3 * t], t1 = tb[3 * t + 1], t2 = tb[3 * t + 2];
-            if (t0 < 0.0f) continue;
-            const float u = ((t0 - a[0]) * d[0] + (t1 - a[1]) * d[1] + (t2 - a[2]) * d[2]) * il;
+            const float4 v = t4[t];
+            if (v.w == 0.0f) continue;
+            const float u = ((v.x - a[0]) * d[0] + (v.y - a[1]) * d[1] + (v.z - a[2]) * d[2]) * il;
             const float q = fminf(fmaxf(rintf(3.0f * u), 0.0f), 3.0f);
             const float be = q * (1.0f / 3.0f), al = 1.0f - be;
             saa += al * al; sbb += be * be; sab += al * be;
-            sac[0] += al * t0; sac[1] += al * t1; sac[2] += al * t2;
-            sbc[0] += be * t0; sbc[1] += be * t1; sbc[2] += be * t2;
+            sac[0] += al * v.x; sac[1] += al * v.y; sac[2] += al * v.z;
+            sbc[0] += be * v.x; sbc[1] += be * v.y; sbc[2] += be * v.z;
         }
         const float det = saa * sbb - sab * sab;
         if (!(det > 1e-8f)) break;
@@ -641,12 +649,15 @@ __device__ __forceinline__ void refine(const float *tb, float a[3], float b[3], 
     }
 }
 
-// Error (squared, 8-bit units, evaluate_mse's measure) and index bits of the
-// block with 565 endpoints c0, c1: four-colour order when c0 > c1 (the
-// caller orders them), each texel's nearest palette entry.
-__device__ __forceinline__ float eval_pair(const float *tb, uint32_t c0, uint32_t c1, uint32_t *bits_out)
+// Error (squared, 8-bit units, evaluate_mse's measure up to rounding) and
+// index bits of the block with 565 endpoints c0, c1: four-colour order when
+// c0 > c1 (the caller orders them), each texel's nearest palette entry.  The
+// distances are taken in [0, 1] units with fused multiply-adds (this search
+// is not the bit-exact path) and scaled once at the end; a texel outside the
+// image (weight 0) adds nothing.
+__device__ __forceinline__ float eval_pair(const float4 *t4, uint32_t c0, uint32_t c1, uint32_t *bits_out)
 {
-    // Palette in 8-bit units (the D3D10 integer palette), texels scaled once.
+    // The D3D10 integer palette, as [0, 1] floats.
     int q[4][3];
     ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
     ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
@@ -661,39 +672,38 @@ __device__ __forceinline__ float eval_pair(const float *tb, uint32_t c0, uint32_
             q[3][k] = 0;
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) p[i][k] = float(q[i][k]);
+        for (int i = 0; i < 4; i++) p[i][k] = float(q[i][k]) * (1.0f / 255.0f);
     }
     float err = 0.0f;
     uint32_t bits = 0;
+#pragma unroll 4
     for (int t = 0; t < 16; t++) {
-        const float r = tb[3 * t];
-        if (r < 0.0f) continue;
-        const float t0 = r * 255.0f, t1 = tb[3 * t + 1] * 255.0f, t2 = tb[3 * t + 2] * 255.0f;
-        float best = FLT_MAX;
+        const float4 v = t4[t];
+        float best = 0.0f;
         uint32_t bi = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float dr = p[i][0] - t0, dg = p[i][1] - t1, db = p[i][2] - t2;
-            const float e = dr * dr + dg * dg + db * db;
-            if (e < best) { best = e; bi = uint32_t(i); }
+            const float dr = p[i][0] - v.x, dg = p[i][1] - v.y, db = p[i][2] - v.z;
+            const float e = fmaf(dr, dr, fmaf(dg, dg, db * db));
+            if (i == 0 || e < best) { best = e; bi = uint32_t(i); }
         }
-        err += best;
+        err = fmaf(best, v.w, err);
         bits |= bi << (2 * t);
     }
     if (bits_out) *bits_out = bits;
-    return err;
+    return err * 65025.0f;
 }
 
 // Quantises (a, b) as icbc does (vector3_to_color16), orders the endpoints
 // for four colours and returns the block and its error.
-__device__ __forceinline__ float encode_pair(const float *tb, const float a[3], const float b[3], const float *mid5,
+__device__ __forceinline__ float encode_pair(const float4 *t4, const float a[3], const float b[3], const float *mid5,
                                              const float *mid6, uint2 *blk)
 {
     uint32_t c0 = ntx_bc1::to565(a[0], a[1], a[2], mid5, mid6);
     uint32_t c1 = ntx_bc1::to565(b[0], b[1], b[2], mid5, mid6);
     if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; }
     uint32_t bits;
-    const float err = eval_pair(tb, c0, c1, &bits);
+    const float err = eval_pair(t4, c0, c1, &bits);
     blk->x = c0 | (c1 << 16);
     blk->y = bits;
     return err;
@@ -716,8 +726,11 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
                                                       uint2 *__restrict__ out, int *__restrict__ fb_count,
                                                       int *__restrict__ fb_list, float *__restrict__ err_out)
 {
-    __shared__ Net net;
-    __shared__ float s_tex[4][16][49];  // 16 texels x RGB per block, padded row
+    __shared__ ForwardNet net;
+    // Each wave's 16 blocks: texels (r, g, b, weight 1 inside the image, 0
+    // outside; a padded row per block), network outputs and (mean, scale).
+    // Only the wave that owns them touches them after the network is loaded.
+    __shared__ float4 s_tex[4][16][17];
     __shared__ float s_y[4][16][kOutPad + 1];
     __shared__ float4 s_aux[4][16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, col = lane & 15;
@@ -725,8 +738,7 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
     __syncthreads();
     const size_t plane = size_t(w) * size_t(h);
     const int tiles = (n + 15) / 16;
-    // Trip count uniform over the workgroup (the loop body synchronises it);
-    // a wave past the last tile computes on no valid block.
+    // A wave past the last tile computes on no valid block.
     for (int group = blockIdx.x; group * 4 < tiles; group += gridDim.x) {
         const int tile = group * 4 + wave;
         const int b = tile * 16 + col;
@@ -776,25 +788,29 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         forward(net, bx3, lane, h1, h2, yv);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) s_tex[wave][col][3 * (4 * g + j) + k] = inr[j] ? c[k][j] : -1.0f;
+            s_tex[wave][col][4 * g + j] = inr[j] ? make_float4(c[0][j], c[1][j], c[2][j], 1.0f)
+                                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) s_y[wave][col][16 * mm + 4 * g + j] = yv[mm][j];
         }
         if (g == 0) s_aux[wave][col] = make_float4(m[0], m[1], m[2], s);
-        __syncthreads();
+        ntx_wave_sync();
         // Lane (g, col): candidate g of block col, texels read from LDS.
-        const float *tb = s_tex[wave][col];
+        const float4 *tb = s_tex[wave][col];
         const float4 ax = s_aux[wave][col];
         const float is = 1.0f / ax.w;
         // Candidate g: diagonal g of the block's colour bounding box, plus
         // the network's offsets (in the block frame, hence divided by the
         // scale) unless box_candidates.
         float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
-        for (int i = 0; i < 16; i++)
-            if (tb[3 * i] >= 0.0f)
-#pragma unroll
-                for (int k = 0; k < 3; k++) { lo[k] = fminf(lo[k], tb[3 * i + k]); hi[k] = fmaxf(hi[k], tb[3 * i + k]); }
+        for (int i = 0; i < 16; i++) {
+            const float4 v = tb[i];
+            if (v.w != 0.0f) {
+                lo[0] = fminf(lo[0], v.x); hi[0] = fmaxf(hi[0], v.x);
+                lo[1] = fminf(lo[1], v.y); hi[1] = fmaxf(hi[1], v.y);
+                lo[2] = fminf(lo[2], v.z); hi[2] = fmaxf(hi[2], v.z);
+            }
+        }
         float a[3], bb[3];
         box_diagonal(g, lo, hi, a, bb);
         float pred = fmaxf(s_y[wave][col][24], 0.0f) * is * 255.0f;  // predicted RMSE, 8-bit units
@@ -835,18 +851,21 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         // cluster fit gives it: reduce_colors' running weighted mean of the
         // equal colours (icbc.h:1638-1683, weights 1) and the single-colour
         // tables.  Texel 0 is always inside the image.
+        const float4 t0 = tb[0];
         bool uniform = valid;
-        for (int i = 1; i < 16; i++)
-            if (tb[3 * i] >= 0.0f && (tb[3 * i] != tb[0] || tb[3 * i + 1] != tb[1] || tb[3 * i + 2] != tb[2]))
-                uniform = false;
+        for (int i = 1; i < 16; i++) {
+            const float4 v = tb[i];
+            if (v.w != 0.0f && (v.x != t0.x || v.y != t0.y || v.z != t0.z)) uniform = false;
+        }
         if (uniform) {
-            float kr = tb[0], kg = tb[1], kb = tb[2], kw = 1.0f;
+            float kr = t0.x, kg = t0.y, kb = t0.z, kw = 1.0f;
             for (int i = 1; i < 16; i++) {
-                if (tb[3 * i] < 0.0f) continue;
+                const float4 v = tb[i];
+                if (v.w == 0.0f) continue;
                 const float s1 = kw + 1.0f;
-                kr = (kr * kw + tb[3 * i]) / s1;
-                kg = (kg * kw + tb[3 * i + 1]) / s1;
-                kb = (kb * kw + tb[3 * i + 2]) / s1;
+                kr = (kr * kw + v.x) / s1;
+                kg = (kg * kw + v.y) / s1;
+                kb = (kb * kw + v.z) / s1;
                 kw = s1;
             }
             blk = ntx_bc1::single_color_block(tab, kr, kg, kb);
@@ -906,14 +925,14 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         }
         if (g == 0 && valid) {
             int cntb = 0;
-            for (int i = 0; i < 16; i++) cntb += tb[3 * i] >= 0.0f ? 1 : 0;
+            for (int i = 0; i < 16; i++) cntb += tb[i].w != 0.0f ? 1 : 0;
             const float rmse = sqrtf(err / float(3 * cntb));
             const bool fallback = !uniform && (pred > tau_pred || (tau_gap >= 0.0f && rmse > pred + tau_gap));
             out[b] = blk;
             if (err_out) err_out[b] = rmse;
             if (fallback) fb_list[atomicAdd(fb_count, 1)] = b;
         }
-        __syncthreads();
+        ntx_wave_sync();
     }
 }
 
