@@ -108,7 +108,9 @@ enum {
     NTX_KERNEL_BC1_LIST = 12,    /* BC1 cluster fit on the blocks the proposal pass queued */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
     NTX_KERNEL_BC3_SEARCH_SPLIT = 14, /* BC3 colour search of a small image: 2..64 lanes per block */
-    NTX_KERNEL_COUNT = 15
+    NTX_KERNEL_GAMMA = 15,       /* gamma transforms of R, G, B (one launch per up to 16 levels) */
+    NTX_KERNEL_NORMALS = 16,     /* normal-map renormalisation of a generated level */
+    NTX_KERNEL_COUNT = 17
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -206,6 +208,39 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_device(ntx_hip_context *ctx, const uint8
  * over the texels; the same values as the two calls above. */
 NTX_API ntx_status ntx_hip_unpack_bgra8_srgb_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
                                                     float *d_rgba);
+
+/* FloatImage::toLinear(0, 3, gamma) (FloatImage.cpp:258-270) when
+ * to_gamma == 0, FloatImage::toGamma(0, 3, gamma) (FloatImage.cpp:272-284)
+ * when to_gamma == 1, on the R, G, B planes of a w x h planar FP32 RGBA image;
+ * the A plane is copied.  gamma == 2.2f takes the reference's table-and-
+ * polynomial powf(x, 11/5) / powf(x, 5/11) (Gamma.cpp:311-441), bit-exact;
+ * any other gamma takes powf(max(0, x), gamma) or powf(max(0, x), 1 / gamma)
+ * (FloatImage.cpp:286-297) correctly rounded (exp2 and log2 in double), which
+ * differs by one unit in the last place from the host C library's powf where
+ * that is not correctly rounded (about 5 in 10^4 inputs).  d_dst may equal d_src.  Unlike
+ * Surface::toLinear / toGamma (Surface.cpp:1470-1487) there is no shortcut for
+ * a gamma of 1. */
+NTX_API ntx_status ntx_hip_gamma_device(ntx_hip_context *ctx, int to_gamma, float gamma, const float *d_src, int w,
+                                        int h, float *d_dst);
+/* The same over n images (the levels of a chain) in one launch per 16. */
+NTX_API ntx_status ntx_hip_gamma_levels_device(ntx_hip_context *ctx, int to_gamma, float gamma, int n,
+                                               const float *const *d_src, const int *w, const int *h,
+                                               float *const *d_dst);
+/* Surface::setImage(InputFormat_BGRA_8UB, ...) followed by
+ * Surface::toLinear(gamma) (Surface.cpp:745-761, 1470-1477) in one pass: each
+ * 8-bit R, G, B value goes through a 256-entry table built on the host with
+ * the reference's operations (the table form for 2.2f, the C library's powf
+ * otherwise), so the result is bit-identical to the two calls; a gamma within
+ * nv::equal of 1 leaves the values as setImage gives them. */
+NTX_API ntx_status ntx_hip_unpack_bgra8_gamma_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
+                                                     float gamma, float *d_rgba);
+
+/* What Compressor::process does to each generated level of a normal map
+ * with normalizeMipmaps set (Context.cpp:329-336): Surface::expandNormals()
+ * (scale 2, bias -2 * 127 / 255; Surface.cpp:2959-2963), normalizeNormalMap()
+ * (FloatImage.cpp:201-219: v / |v|, zero for a zero vector) and packNormals()
+ * (scale 0.5, bias 0.5; Surface.cpp:2952-2956), in place on R, G, B. */
+NTX_API ntx_status ntx_hip_normalize_normals_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
 
 /* Error of a BC1 level against its FP32 source: D3D10 decode
  * (BlockDXT.cpp:43-98) and the sum over texels of the squared R, G, B

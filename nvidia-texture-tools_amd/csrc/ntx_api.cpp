@@ -11,6 +11,7 @@
 #include <tuple>
 #include <vector>
 
+#include "gamma.hpp"
 #include "ntx_context.h"
 
 namespace {
@@ -226,6 +227,7 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         if (st) (void)hipStreamDestroy(st);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     if (ctx->d_srgb) (void)hipFree(ctx->d_srgb);
+    if (ctx->d_gamma_lut) (void)hipFree(ctx->d_gamma_lut);
     for (auto &m : ctx->marks) { (void)hipEventDestroy(m.start); (void)hipEventDestroy(m.stop); }
     for (hipEvent_t e : ctx->spare) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -329,7 +331,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "srgb_kernel",   "unpack_kernel",    "poly_fused_kernel",
                                                    "bc3_front_kernel", "batch_kernel", "bc1_mlp_kernel",
                                                    "bc1_list_kernel", "mlp_train_kernel",
-                                                   "bc3_search_split_kernel" };
+                                                   "bc3_search_split_kernel", "gamma_kernel",
+                                                   "normal_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -777,6 +780,86 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_srgb_device(ntx_hip_context *ctx, const 
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
     hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] {
         return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, ctx->d_srgb, ctx->stream);
+    });
+    if (e != hipSuccess) return fail(ctx, e, "unpack launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_gamma_levels_device(ntx_hip_context *ctx, int to_gamma, float gamma, int n,
+                                               const float *const *d_src, const int *w, const int *h,
+                                               float *const *d_dst)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (n < 0 || (n > 0 && (!d_src || !w || !h || !d_dst)) || !(gamma > 0.0f)) return NTX_ERR_INVALID_ARGUMENT;
+    for (int l = 0; l < n; l++)
+        if (w[l] < 0 || h[l] < 0 || (size_t(w[l]) * size_t(h[l]) > 0 && (!d_src[l] || !d_dst[l])))
+            return NTX_ERR_INVALID_ARGUMENT;
+    // FloatImage.cpp:258-284: gamma 2.2f takes the table forms, anything
+    // else powf(max(0, x), gamma) to linear or powf(max(0, x), 1 / gamma) back.
+    const int mode = gamma == 2.2f ? (to_gamma ? 1 : 0) : 2;
+    const float power = to_gamma ? 1.0f / gamma : gamma;
+    for (int first = 0; first < n; first += ntx::kMaxGammaLevels) {
+        ntx::GammaJob job{};
+        job.levels = std::min(ntx::kMaxGammaLevels, n - first);
+        for (int l = 0; l < job.levels; l++) {
+            job.src[l] = d_src[first + l];
+            job.dst[l] = d_dst[first + l];
+            job.texels[l] = (unsigned long long)w[first + l] * (unsigned long long)h[first + l];
+        }
+        hipError_t e = timed(ctx, NTX_KERNEL_GAMMA, [&] { return ntx::launch_gamma(job, mode, power, ctx->d_tab, ctx->stream); });
+        if (e != hipSuccess) return fail(ctx, e, "gamma launch");
+    }
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_gamma_device(ntx_hip_context *ctx, int to_gamma, float gamma, const float *d_src, int w, int h,
+                                        float *d_dst)
+{
+    return ntx_hip_gamma_levels_device(ctx, to_gamma, gamma, 1, &d_src, &w, &h, &d_dst);
+}
+
+NTX_API ntx_status ntx_hip_normalize_normals_device(ntx_hip_context *ctx, float *d_rgba, int w, int h)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && !d_rgba)) return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e = timed(ctx, NTX_KERNEL_NORMALS, [&] { return ntx::launch_normals(d_rgba, size_t(w) * size_t(h), ctx->stream); });
+    if (e != hipSuccess) return fail(ctx, e, "normals launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_unpack_bgra8_gamma_device(ntx_hip_context *ctx, const uint8_t *d_bgra, int w, int h,
+                                                     float gamma, float *d_rgba)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_bgra || !d_rgba)) || !(gamma > 0.0f))
+        return NTX_ERR_INVALID_ARGUMENT;
+    const float *lut = nullptr;
+    if (!ntx_gamma::is_unit_gamma(gamma)) {
+        uint32_t key;
+        std::memcpy(&key, &gamma, 4);
+        if (!ctx->d_gamma_lut || ctx->gamma_lut_key != key) {
+            // Surface::toLinear(gamma) of every 8-bit value v / 255
+            // (Surface.cpp:1470-1477, FloatImage.cpp:258-270, 286-297), with
+            // the same operations as the reference: the table form for 2.2,
+            // the C library's powf otherwise.
+            float table[512], h_lut[256];
+            for (int k = 0; k < 512; k++) table[k] = ntx_gamma::table_entry(k, 11.0, 5.0);
+            for (int k = 0; k < 256; k++) {
+                const float v = float(k) / 255.0f;
+                h_lut[k] = gamma == 2.2f ? ntx_gamma::pow_11_5(v, table) : ::powf(ntx_gamma::max0(v), gamma);
+            }
+            if (!ctx->d_gamma_lut) NTX_HIP(ctx, hipMalloc(&ctx->d_gamma_lut, sizeof(h_lut)));
+            NTX_HIP(ctx, hipMemcpyAsync(ctx->d_gamma_lut, h_lut, sizeof(h_lut), hipMemcpyHostToDevice, ctx->stream));
+            NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            ctx->gamma_lut_key = key;
+        }
+        lut = ctx->d_gamma_lut;
+    }
+    hipError_t e = timed(ctx, NTX_KERNEL_UNPACK, [&] {
+        return ntx::launch_unpack(d_bgra, size_t(w) * size_t(h), d_rgba, lut, ctx->stream);
     });
     if (e != hipSuccess) return fail(ctx, e, "unpack launch");
     return NTX_OK;

@@ -43,6 +43,11 @@ struct Tables {
     // weights are point multiplicities, 16 in all): entry
     // (w0 * 17 + w1) * 17 + w2 for w0 + w1 + w2 <= 16, others zero.
     float sqw[17 * 17 * 17][4];
+    // Exponent tables of the gamma-2.2 transforms (Gamma.cpp:33-300): entry k
+    // (a float's sign-and-exponent field) is 2^((k - 127) * 5/11), resp.
+    // 2^((k - 127) * 11/5), as a float; 0 for k == 0 and k >= 256, +inf for 255.
+    float pow_5_11[512];
+    float pow_11_5[512];
 };
 
 void build_tables(Tables *t);

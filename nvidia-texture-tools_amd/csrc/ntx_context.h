@@ -39,6 +39,21 @@ int poly_tile_y();
 hipError_t launch_srgb(float *d, size_t n, const float *d_lut, hipStream_t stream);
 // d_srgb_lut: NULL, or the 256-entry sRGB->linear table to convert R, G, B as well.
 hipError_t launch_unpack(const uint8_t *d_bgra, size_t n, float *d_dst, const float *d_srgb_lut, hipStream_t stream);
+// Levels for one gamma launch: texels[l] = w * h of level l; start[] is
+// filled by launch_gamma.
+constexpr int kMaxGammaLevels = 16;
+struct GammaJob {
+    const float *src[kMaxGammaLevels];
+    float *dst[kMaxGammaLevels];
+    unsigned long long texels[kMaxGammaLevels];
+    unsigned long long start[kMaxGammaLevels + 1];
+    int levels;
+};
+// mode 0: gamma 2.2 -> linear (table form), 1: linear -> gamma 2.2 (table
+// form), 2: powf(max(0, x), power).
+hipError_t launch_gamma(const GammaJob &job, int mode, float power, const Tables *d_tab, hipStream_t stream);
+// expandNormals + normalizeNormalMap + packNormals on the R, G, B planes of n texels.
+hipError_t launch_normals(float *d_rgba, size_t n, hipStream_t stream);
 hipError_t launch_bc1_sse(const float *d_src, int w, int h, const void *d_blocks, double *d_partial, double *d_out,
                           hipStream_t stream);
 int bc1_sse_partials(int w, int h);
@@ -66,6 +81,10 @@ struct ntx_hip_context {
     hipStream_t stream = nullptr;
     ntx::Tables *d_tab = nullptr;
     float *d_srgb = nullptr;
+    // 8-bit -> linear table of ntx_hip_unpack_bgra8_gamma_device for the
+    // gamma whose float bits are gamma_lut_key (valid when d_gamma_lut set).
+    float *d_gamma_lut = nullptr;
+    uint32_t gamma_lut_key = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ntx::DevBuf in, out, tmp, dst, bucket, metric;
     // The BC3 tallies at the start of `bucket` are zero: set after a complete

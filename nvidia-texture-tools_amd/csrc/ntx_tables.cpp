@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "gamma.hpp"
 #include "ntx_common.h"
 
 namespace ntx {
@@ -122,6 +123,12 @@ void build_tables(Tables *t)
                     t->sqw[(a * 17 + b) * 17 + c][j] = v[j];
                 }
             }
+
+    // Gamma-2.2 exponent tables (Gamma.cpp:30-300) from their closed form.
+    for (int k = 0; k < 512; k++) {
+        t->pow_5_11[k] = ntx_gamma::table_entry(k, 5.0, 11.0);
+        t->pow_11_5[k] = ntx_gamma::table_entry(k, 11.0, 5.0);
+    }
 }
 
 // ---------------------------------------------------------------- filters --

@@ -30,7 +30,8 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 # ntx_hip.h kernel ids for ntx_hip_profile_read.
 KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
            "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_front_kernel", "batch_kernel",
-           "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel")
+           "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel", "gamma_kernel",
+           "normal_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
@@ -44,6 +45,8 @@ EXPORTS = (
     "ntx_hip_compress_chain_device", "ntx_hip_compress_images_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
+    "ntx_hip_gamma_device", "ntx_hip_gamma_levels_device", "ntx_hip_unpack_bgra8_gamma_device",
+    "ntx_hip_normalize_normals_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
     "ntx_hip_batch_autotune",
@@ -126,6 +129,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_srgb_to_linear_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_unpack_bgra8_device": (_I, [_P, _P, _I, _I, _P]),
         "ntx_hip_unpack_bgra8_srgb_device": (_I, [_P, _P, _I, _I, _P]),
+        "ntx_hip_gamma_device": (_I, [_P, _I, _F, _P, _I, _I, _P]),
+        "ntx_hip_gamma_levels_device": (_I, [_P, _I, _F, _I, _P, _P, _P, _P]),
+        "ntx_hip_unpack_bgra8_gamma_device": (_I, [_P, _P, _I, _I, _F, _P]),
+        "ntx_hip_normalize_normals_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_P)]),
         "ntx_hip_batch_destroy": (_I, [_P]),
         "ntx_hip_batch_submit": (_I, [_P]),
@@ -341,6 +348,36 @@ class Context:
     def unpack_bgra8_srgb_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
         """unpack_bgra8_device followed by srgb_to_linear_device, in one pass."""
         self._check(self.lib.ntx_hip_unpack_bgra8_srgb_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack_srgb")
+
+    def gamma_device(self, to_gamma: bool, gamma: float, d_src: DeviceBuffer, w: int, h: int,
+                     d_dst: Optional[DeviceBuffer] = None) -> None:
+        """FloatImage::toLinear / toGamma on R, G, B (ntx_hip_gamma_device); in place without d_dst."""
+        dst = d_src if d_dst is None else d_dst
+        self._check(self.lib.ntx_hip_gamma_device(self.handle, int(bool(to_gamma)), gamma, d_src.ptr, w, h, dst.ptr),
+                    "ntx_hip_gamma_device")
+
+    def gamma_levels_device(self, to_gamma: bool, gamma: float, d_src: Sequence[DeviceBuffer],
+                            sizes: Sequence[Tuple[int, int]], d_dst: Sequence[DeviceBuffer]) -> None:
+        """gamma_device over several images in one launch (ntx_hip_gamma_levels_device)."""
+        n = len(d_src)
+        src = (_P * n)(*[d.ptr for d in d_src])
+        dst = (_P * n)(*[d.ptr for d in d_dst])
+        ws = (_I * n)(*[int(w) for w, _ in sizes])
+        hs = (_I * n)(*[int(h) for _, h in sizes])
+        st = self.lib.ntx_hip_gamma_levels_device(self.handle, int(bool(to_gamma)), gamma, n, ctypes.cast(src, _P),
+                                                  ctypes.cast(ws, _P), ctypes.cast(hs, _P), ctypes.cast(dst, _P))
+        self._check(st, "ntx_hip_gamma_levels_device")
+
+    def unpack_bgra8_gamma_device(self, d_bgra: DeviceBuffer, w: int, h: int, gamma: float,
+                                  d_rgba: DeviceBuffer) -> None:
+        """unpack_bgra8_device followed by Surface::toLinear(gamma), in one pass."""
+        self._check(self.lib.ntx_hip_unpack_bgra8_gamma_device(self.handle, d_bgra.ptr, w, h, gamma, d_rgba.ptr),
+                    "ntx_hip_unpack_bgra8_gamma_device")
+
+    def normalize_normals_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
+        """expandNormals + normalizeNormalMap + packNormals in place (ntx_hip_normalize_normals_device)."""
+        self._check(self.lib.ntx_hip_normalize_normals_device(self.handle, d_rgba.ptr, w, h),
+                    "ntx_hip_normalize_normals_device")
 
     def bc1_psnr_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
         """PSNR (dB) of a BC1 level against its FP32 source, computed on the device."""

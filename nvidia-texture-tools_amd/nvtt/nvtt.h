@@ -13,8 +13,14 @@
 //    (Context.cpp:494) therefore does not apply.
 //  - Surface keeps its FP32 planes in device memory; data()/channel() copy
 //    them back on demand.
-//  - Container headers (outputHeader), InputOptions/process, file IO and the
-//    Surface transforms outside this path are not provided.
+//  - InputOptions / Compressor::process run the reference's input pipeline
+//    (Context.cpp:217-344) on the device for 2D and cube textures: 8-bit and
+//    float input, gamma to linear and back, generated or supplied mip levels,
+//    normal-map renormalisation, DDS and DDS10 headers.  Resizing
+//    (setMaxExtents / setRoundMode changing the extents), height-to-normal
+//    conversion, dithering and the KTX container report
+//    Error_UnsupportedFeature / Error_UnsupportedOutputFormat instead.
+//  - Surface transforms outside this path are not provided.
 #ifndef NVTT_AMD_H
 #define NVTT_AMD_H
 
@@ -73,6 +79,20 @@ enum InputFormat { InputFormat_BGRA_8UB, InputFormat_RGBA_16F, InputFormat_RGBA_
 enum MipmapFilter { MipmapFilter_Box, MipmapFilter_Triangle, MipmapFilter_Kaiser };
 enum AlphaMode { AlphaMode_None, AlphaMode_Transparency, AlphaMode_Premultiplied };
 
+// nvtt.h:253-262.
+enum RoundMode {
+    RoundMode_None,
+    RoundMode_ToNextPowerOfTwo,
+    RoundMode_ToNearestPowerOfTwo,
+    RoundMode_ToPreviousPowerOfTwo,
+    RoundMode_ToNextMultipleOfFour,
+    RoundMode_ToNearestMultipleOfFour,
+    RoundMode_ToPreviousMultipleOfFour,
+};
+
+// nvtt.h:367-373.
+enum Container { Container_DDS, Container_DDS10, Container_KTX };
+
 // nvtt.h:345-355.
 enum Error {
     Error_Unknown,
@@ -118,14 +138,52 @@ struct ErrorHandler {
     virtual void error(Error e) = 0;
 };
 
-// nvtt.h:378-399 (handler plumbing only).
+// nvtt.h:281-326 (all setters are accepted; process() reports the
+// settings it does not implement, see above).
+struct InputOptions {
+    struct Private;
+    NVTT_API InputOptions();
+    NVTT_API ~InputOptions();
+    NVTT_API void reset();
+    NVTT_API void setTextureLayout(TextureType type, int w, int h, int d = 1, int arraySize = 1);
+    NVTT_API void resetTextureLayout();
+    // Copies the data.
+    NVTT_API bool setMipmapData(const void *data, int w, int h, int d = 1, int face = 0, int mipmap = 0);
+    NVTT_API void setFormat(InputFormat format);
+    NVTT_API void setAlphaMode(AlphaMode alphaMode);
+    NVTT_API void setGamma(float inputGamma, float outputGamma);
+    NVTT_API void setWrapMode(WrapMode mode);
+    NVTT_API void setMipmapFilter(MipmapFilter filter);
+    NVTT_API void setMipmapGeneration(bool enabled, int maxLevel = -1);
+    NVTT_API void setKaiserParameters(float width, float alpha, float stretch);
+    NVTT_API void setNormalMap(bool b);
+    NVTT_API void setConvertToNormalMap(bool convert);
+    NVTT_API void setHeightEvaluation(float redScale, float greenScale, float blueScale, float alphaScale);
+    NVTT_API void setNormalFilter(float sm, float medium, float big, float large);
+    NVTT_API void setNormalizeMipmaps(bool b);
+    NVTT_API void setMaxExtents(int d);
+    NVTT_API void setRoundMode(RoundMode mode);
+    Private &m;
+
+private:
+    InputOptions(const InputOptions &);
+    void operator=(const InputOptions &);
+};
+
+// nvtt.h:378-399.
 struct OutputOptions {
     struct Private;
     NVTT_API OutputOptions();
     NVTT_API ~OutputOptions();
     NVTT_API void reset();
+    // Writes the output to a file (a handler of the library's own).
+    NVTT_API void setFileName(const char *fileName);
     NVTT_API void setOutputHandler(OutputHandler *outputHandler);
     NVTT_API void setErrorHandler(ErrorHandler *errorHandler);
+    NVTT_API void setOutputHeader(bool outputHeader);
+    NVTT_API void setContainer(Container container);
+    NVTT_API void setUserVersion(int version);
+    NVTT_API void setSrgbFlag(bool b);
     Private &m;
 
 private:
@@ -175,6 +233,13 @@ struct Compressor {
     // Accepted for source compatibility; the device path does not use it.
     NVTT_API void setTaskDispatcher(TaskDispatcher *disp);
 
+    // InputOptions API (nvtt.h:426-428): the whole texture, header first.
+    NVTT_API bool process(const InputOptions &inputOptions, const CompressionOptions &compressionOptions,
+                          const OutputOptions &outputOptions) const;
+    NVTT_API int estimateSize(const InputOptions &inputOptions, const CompressionOptions &compressionOptions) const;
+
+    NVTT_API bool outputHeader(const Surface &img, int mipmapCount, const CompressionOptions &compressionOptions,
+                               const OutputOptions &outputOptions) const;
     NVTT_API bool compress(const Surface &img, int face, int mipmap, const CompressionOptions &compressionOptions,
                            const OutputOptions &outputOptions) const;
     NVTT_API int estimateSize(const Surface &img, int mipmapCount, const CompressionOptions &compressionOptions) const;
@@ -229,6 +294,9 @@ struct Surface {
     NVTT_API bool canMakeNextMipmap(int min_size = 1);
 
     NVTT_API void toLinearFromSrgb();
+    // Surface.cpp:1470-1487 (R, G, B; nothing for a gamma of 1).
+    NVTT_API void toLinear(float gamma);
+    NVTT_API void toGamma(float gamma);
 
     // The device copy of the planes (4 planes of width*height floats), for
     // callers that keep work on the device.  Null for a null surface.

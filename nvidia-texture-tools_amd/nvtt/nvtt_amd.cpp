@@ -6,7 +6,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <cmath>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "nvtt.h"
@@ -106,9 +109,45 @@ void CompressionOptions::setColorWeights(float r, float g, float b, float a)
 }
 Format CompressionOptions::format() const { return m.format; }
 
+namespace {
+
+// The handler setFileName installs (OutputOptions.cpp:60-90): every byte to
+// one file, in order.
+struct FileOutputHandler : OutputHandler {
+    std::FILE *f = nullptr;
+    explicit FileOutputHandler(const char *name) : f(std::fopen(name, "wb")) {}
+    ~FileOutputHandler() override
+    {
+        if (f) std::fclose(f);
+    }
+    void beginImage(int, int, int, int, int, int) override {}
+    bool writeData(const void *data, int size) override
+    {
+        return f && (size == 0 || std::fwrite(data, size_t(size), 1, f) == 1);
+    }
+    void endImage() override {}
+};
+
+}  // namespace
+
 struct OutputOptions::Private {
     OutputHandler *outputHandler = nullptr;
     ErrorHandler *errorHandler = nullptr;
+    std::string fileName;
+    FileOutputHandler *fileHandler = nullptr;  // owned; the outputHandler when a file name is set
+    bool outputHeader = true;                  // OutputOptions.cpp:44-57 defaults
+    Container container = Container_DDS;
+    int version = 0;
+    bool srgb = false;
+
+    void dropFile()
+    {
+        delete fileHandler;
+        fileHandler = nullptr;
+        fileName.clear();
+    }
+    // OutputOptions.cpp:149-157: with a file name the file must have opened.
+    bool hasValidOutputHandler() const { return fileName.empty() || outputHandler != nullptr; }
     // OutputOptions.cpp:159-177.
     void beginImage(int size, int w, int h, int d, int face, int mip) const
     {
@@ -126,10 +165,204 @@ struct OutputOptions::Private {
 };
 
 OutputOptions::OutputOptions() : m(*new Private) {}
-OutputOptions::~OutputOptions() { delete &m; }
-void OutputOptions::reset() { m.outputHandler = nullptr; m.errorHandler = nullptr; }
-void OutputOptions::setOutputHandler(OutputHandler *h) { m.outputHandler = h; }
+OutputOptions::~OutputOptions()
+{
+    m.dropFile();
+    delete &m;
+}
+void OutputOptions::reset()
+{
+    m.dropFile();
+    m.outputHandler = nullptr;
+    m.errorHandler = nullptr;
+    m.outputHeader = true;
+    m.container = Container_DDS;
+    m.version = 0;
+    m.srgb = false;
+}
+void OutputOptions::setFileName(const char *fileName)
+{
+    m.dropFile();
+    m.outputHandler = nullptr;
+    if (!fileName) return;
+    m.fileName = fileName;
+    m.fileHandler = new FileOutputHandler(fileName);
+    if (m.fileHandler->f) {
+        m.outputHandler = m.fileHandler;
+    } else {
+        delete m.fileHandler;
+        m.fileHandler = nullptr;  // process() then reports Error_FileOpen
+    }
+}
+void OutputOptions::setOutputHandler(OutputHandler *h)
+{
+    m.dropFile();
+    m.outputHandler = h;
+}
 void OutputOptions::setErrorHandler(ErrorHandler *h) { m.errorHandler = h; }
+void OutputOptions::setOutputHeader(bool b) { m.outputHeader = b; }
+void OutputOptions::setContainer(Container c) { m.container = c; }
+void OutputOptions::setUserVersion(int version) { m.version = version; }
+void OutputOptions::setSrgbFlag(bool b) { m.srgb = b; }
+
+// ---------------------------------------------------------- input options --
+
+struct InputOptions::Private {
+    // InputOptions.cpp:80-110 defaults.
+    WrapMode wrapMode;
+    TextureType textureType;
+    InputFormat inputFormat;
+    AlphaMode alphaMode;
+    float inputGamma, outputGamma;
+    bool generateMipmaps;
+    int maxLevel;
+    MipmapFilter mipmapFilter;
+    float kaiserWidth, kaiserAlpha, kaiserStretch;
+    bool isNormalMap, normalizeMipmaps, convertToNormalMap;
+    float heightFactors[4], bumpFrequencyScale[4];
+    int maxExtent;
+    RoundMode roundMode;
+    // Layout (InputOptions.cpp:113-185): faceCount faces x mipmapCount levels,
+    // image (mip, face) at mip * faceCount + face; empty = not supplied.
+    int width = 0, height = 0, depth = 0, faceCount = 0, mipmapCount = 0;
+    std::vector<std::vector<uint8_t>> images;
+};
+
+InputOptions::InputOptions() : m(*new Private) { reset(); }
+InputOptions::~InputOptions() { delete &m; }
+
+void InputOptions::reset()
+{
+    m.wrapMode = WrapMode_Mirror;
+    m.textureType = TextureType_2D;
+    m.inputFormat = InputFormat_BGRA_8UB;
+    m.alphaMode = AlphaMode_None;
+    m.inputGamma = 2.2f;
+    m.outputGamma = 2.2f;
+    m.generateMipmaps = true;
+    m.maxLevel = -1;
+    m.mipmapFilter = MipmapFilter_Box;
+    m.kaiserWidth = 3.0f;
+    m.kaiserAlpha = 4.0f;
+    m.kaiserStretch = 1.0f;
+    m.isNormalMap = false;
+    m.normalizeMipmaps = true;
+    m.convertToNormalMap = false;
+    const float hf[4] = { 0.0f, 0.0f, 0.0f, 1.0f };
+    const float bs = 1.0f + 0.5f + 0.25f + 0.125f;
+    const float bf[4] = { 1.0f / bs, 0.5f / bs, 0.25f / bs, 0.125f / bs };
+    std::memcpy(m.heightFactors, hf, sizeof hf);
+    std::memcpy(m.bumpFrequencyScale, bf, sizeof bf);
+    m.maxExtent = 0;
+    m.roundMode = RoundMode_None;
+}
+
+namespace {
+
+int count_mipmaps3(int w, int h, int d)
+{
+    int n = 0;
+    while (w != 1 || h != 1 || d != 1) {
+        w = w / 2 > 1 ? w / 2 : 1;
+        h = h / 2 > 1 ? h / 2 : 1;
+        d = d / 2 > 1 ? d / 2 : 1;
+        n++;
+    }
+    return n + 1;
+}
+
+size_t input_texel_bytes(InputFormat f)
+{
+    switch (f) {
+    case InputFormat_BGRA_8UB: return 4;
+    case InputFormat_RGBA_16F: return 8;
+    case InputFormat_RGBA_32F: return 16;
+    case InputFormat_R_32F: return 4;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// InputOptions.cpp:113-162.
+void InputOptions::setTextureLayout(TextureType type, int w, int h, int d, int arraySize)
+{
+    if (w <= 0) w = 1;
+    if (h <= 0) h = 1;
+    if (d <= 0) d = 1;
+    if (arraySize <= 0) arraySize = 1;
+    resetTextureLayout();
+    m.textureType = type;
+    m.width = w;
+    m.height = h;
+    m.depth = d;
+    m.faceCount = type == TextureType_Cube ? 6 : type == TextureType_Array ? arraySize : 1;
+    m.mipmapCount = count_mipmaps3(w, h, d);
+    m.images.assign(size_t(m.faceCount) * size_t(m.mipmapCount), std::vector<uint8_t>());
+}
+
+void InputOptions::resetTextureLayout()
+{
+    m.images.clear();
+    m.faceCount = 0;
+    m.mipmapCount = 0;
+}
+
+// InputOptions.cpp:188-245: the extents must be those of that level.
+bool InputOptions::setMipmapData(const void *data, int w, int h, int d, int face, int mipmap)
+{
+    if (face < 0 || face >= m.faceCount || mipmap < 0 || mipmap >= m.mipmapCount || !data) return false;
+    int ew = m.width, eh = m.height, ed = m.depth;
+    for (int i = 0; i < mipmap; i++) {
+        ew = ew / 2 > 1 ? ew / 2 : 1;
+        eh = eh / 2 > 1 ? eh / 2 : 1;
+        ed = ed / 2 > 1 ? ed / 2 : 1;
+    }
+    if (ew != w || eh != h || ed != d) return false;
+    const size_t bytes = size_t(w) * size_t(h) * size_t(d) * input_texel_bytes(m.inputFormat);
+    if (bytes == 0) return false;
+    std::vector<uint8_t> &img = m.images[size_t(mipmap) * size_t(m.faceCount) + size_t(face)];
+    img.assign(static_cast<const uint8_t *>(data), static_cast<const uint8_t *>(data) + bytes);
+    return true;
+}
+
+void InputOptions::setFormat(InputFormat format) { m.inputFormat = format; }
+void InputOptions::setAlphaMode(AlphaMode alphaMode) { m.alphaMode = alphaMode; }
+void InputOptions::setGamma(float inputGamma, float outputGamma)
+{
+    m.inputGamma = inputGamma;
+    m.outputGamma = outputGamma;
+}
+void InputOptions::setWrapMode(WrapMode mode) { m.wrapMode = mode; }
+void InputOptions::setMipmapFilter(MipmapFilter filter) { m.mipmapFilter = filter; }
+void InputOptions::setMipmapGeneration(bool enabled, int maxLevel)
+{
+    m.generateMipmaps = enabled;
+    m.maxLevel = maxLevel;
+}
+void InputOptions::setKaiserParameters(float width, float alpha, float stretch)
+{
+    m.kaiserWidth = width;
+    m.kaiserAlpha = alpha;
+    m.kaiserStretch = stretch;
+}
+void InputOptions::setNormalMap(bool b) { m.isNormalMap = b; }
+void InputOptions::setConvertToNormalMap(bool convert) { m.convertToNormalMap = convert; }
+void InputOptions::setHeightEvaluation(float r, float g, float b, float a)
+{
+    // InputOptions.cpp:313-318: kept as given.
+    m.heightFactors[0] = r; m.heightFactors[1] = g; m.heightFactors[2] = b; m.heightFactors[3] = a;
+}
+// InputOptions.cpp:321-325: normalised by their sum.
+void InputOptions::setNormalFilter(float sm, float medium, float big, float large)
+{
+    const float total = sm + medium + big + large;
+    m.bumpFrequencyScale[0] = sm / total; m.bumpFrequencyScale[1] = medium / total;
+    m.bumpFrequencyScale[2] = big / total; m.bumpFrequencyScale[3] = large / total;
+}
+void InputOptions::setNormalizeMipmaps(bool b) { m.normalizeMipmaps = b; }
+void InputOptions::setMaxExtents(int d) { m.maxExtent = d; }
+void InputOptions::setRoundMode(RoundMode mode) { m.roundMode = mode; }
 
 // ---------------------------------------------------------------- surface --
 
@@ -362,6 +595,39 @@ void Surface::toLinearFromSrgb()
     m->hostValid = false;
 }
 
+namespace {
+
+// nv::equal(gamma, 1.0f) (nvmath/nvmath.h:52, 139-143).
+bool unit_gamma(float gamma)
+{
+    const float ag = std::fabs(gamma);
+    return std::fabs(gamma - 1.0f) <= 0.0001f * (ag > 1.0f ? ag : 1.0f);
+}
+
+}  // namespace
+
+void Surface::toLinear(float gamma)
+{
+    if (isNull() || unit_gamma(gamma) || !detach(m)) return;
+    std::lock_guard<std::mutex> g(backend_lock());
+    ntx_hip_context *ctx = backend();
+    ntx_status s = ntx_hip_gamma_device(ctx, 0, gamma, m->dev, m->w, m->h, m->dev);
+    if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+    if (s != NTX_OK) report("toLinear", s);
+    m->hostValid = false;
+}
+
+void Surface::toGamma(float gamma)
+{
+    if (isNull() || unit_gamma(gamma) || !detach(m)) return;
+    std::lock_guard<std::mutex> g(backend_lock());
+    ntx_hip_context *ctx = backend();
+    ntx_status s = ntx_hip_gamma_device(ctx, 1, gamma, m->dev, m->w, m->h, m->dev);
+    if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+    if (s != NTX_OK) report("toGamma", s);
+    m->hostValid = false;
+}
+
 // ------------------------------------------------------------- compressor --
 
 // The backend seam: same virtual as nv::CompressorInterface (Compressor.h:34-38),
@@ -453,7 +719,447 @@ struct Compressor::Private {
         oo.endImage();
         return true;
     }
+
+    bool outputHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount, bool isNormalMap,
+                      const CompressionOptions::Private &co, const OutputOptions::Private &oo) const;
+    bool process(const InputOptions::Private &io, const CompressionOptions::Private &co,
+                 const OutputOptions::Private &oo) const;
 };
+
+namespace {
+
+// DDS header constants (DirectDrawSurface.h/.cpp, the DDS_HEADER layout of
+// the DirectDraw Surface format).
+constexpr uint32_t kDDSD_CAPS = 0x1, kDDSD_HEIGHT = 0x2, kDDSD_WIDTH = 0x4, kDDSD_PITCH = 0x8,
+                   kDDSD_PIXELFORMAT = 0x1000, kDDSD_MIPMAPCOUNT = 0x20000, kDDSD_LINEARSIZE = 0x80000,
+                   kDDSD_DEPTH = 0x800000;
+constexpr uint32_t kDDSCAPS_COMPLEX = 0x8, kDDSCAPS_TEXTURE = 0x1000, kDDSCAPS_MIPMAP = 0x400000;
+constexpr uint32_t kDDSCAPS2_CUBEMAP = 0x200, kDDSCAPS2_CUBEMAP_ALL_FACES = 0xFC00, kDDSCAPS2_VOLUME = 0x200000;
+constexpr uint32_t kDDPF_ALPHAPIXELS = 0x1, kDDPF_FOURCC = 0x4, kDDPF_NORMAL = 0x80000000u, kDDPF_SRGB = 0x40000000u;
+constexpr uint32_t kDimTexture2D = 3, kDimTexture3D = 4, kMiscTextureCube = 0x4;
+constexpr uint32_t kDXGI_BC1 = 71, kDXGI_BC1_SRGB = 72, kDXGI_BC2 = 74, kDXGI_BC2_SRGB = 75, kDXGI_BC3 = 77,
+                   kDXGI_BC3_SRGB = 78, kDXGI_BC4 = 80, kDXGI_BC5 = 83;
+
+constexpr uint32_t fourcc(char a, char b, char c, char d)
+{
+    return uint32_t(uint8_t(a)) | uint32_t(uint8_t(b)) << 8 | uint32_t(uint8_t(c)) << 16 | uint32_t(uint8_t(d)) << 24;
+}
+
+// The 148-byte header (128 + the DX10 extension), as 37 little-endian words.
+struct DdsHeader {
+    uint32_t magic = fourcc('D', 'D', 'S', ' ');
+    uint32_t size = 124;
+    uint32_t flags = kDDSD_CAPS | kDDSD_PIXELFORMAT;
+    uint32_t height = 0, width = 0, pitch = 0, depth = 0, mipmapcount = 0;
+    uint32_t reserved[11] = {};
+    uint32_t pf_size = 32, pf_flags = 0, pf_fourcc = 0, pf_bitcount = 0, pf_rmask = 0, pf_gmask = 0, pf_bmask = 0,
+             pf_amask = 0;
+    uint32_t caps1 = kDDSCAPS_TEXTURE, caps2 = 0, caps3 = 0, caps4 = 0;
+    uint32_t notused = 0;
+    uint32_t dxgiFormat = 0, resourceDimension = 0, miscFlag = 0, arraySize = 0, reserved10 = 0;
+
+    // DirectDrawSurface.cpp:574-609: the writer's version in reserved[9..10].
+    DdsHeader()
+    {
+        reserved[9] = fourcc('N', 'V', 'T', 'T');
+        reserved[10] = (2u << 16) | (1u << 8) | 2u;
+    }
+    // DirectDrawSurface.cpp:629-650.
+    void setMipmapCount(uint32_t count)
+    {
+        if (count == 0 || count == 1) {
+            flags &= ~kDDSD_MIPMAPCOUNT;
+            mipmapcount = 1;
+            caps1 = caps2 == 0 ? kDDSCAPS_TEXTURE : (kDDSCAPS_TEXTURE | kDDSCAPS_COMPLEX);
+        } else {
+            flags |= kDDSD_MIPMAPCOUNT;
+            mipmapcount = count;
+            caps1 |= kDDSCAPS_COMPLEX | kDDSCAPS_MIPMAP;
+        }
+    }
+    void setFourCC(uint32_t code)
+    {
+        pf_flags = kDDPF_FOURCC;
+        pf_fourcc = code;
+        pf_bitcount = pf_rmask = pf_gmask = pf_bmask = pf_amask = 0;
+    }
+    void setDX10Format(uint32_t format)
+    {
+        pf_flags = kDDPF_FOURCC;
+        pf_fourcc = fourcc('D', 'X', '1', '0');
+        dxgiFormat = format;
+    }
+    bool hasDX10() const { return pf_fourcc == fourcc('D', 'X', '1', '0'); }
+};
+static_assert(sizeof(DdsHeader) == 148, "DDS header layout");
+
+// nv::getTargetExtent (Surface.cpp:220-305), restated with its helpers
+// (Surface.cpp:60-100; nvcore/Utils.h nextPowerOfTwo).
+uint32_t next_pow2(uint32_t v)
+{
+    v--;
+    v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+    return v + 1;
+}
+uint32_t prev_pow2(uint32_t v) { return next_pow2(v + 1) / 2; }
+uint32_t nearest_pow2(uint32_t v)
+{
+    const uint32_t np2 = next_pow2(v), pp2 = prev_pow2(v);
+    return (np2 - v <= v - pp2) ? np2 : pp2;
+}
+uint32_t prev_mul4(uint32_t v) { return v & ~3u; }
+uint32_t next_mul4(uint32_t v) { return (v + 3) & ~3u; }
+
+void target_extent(int *width, int *height, int *depth, int maxExtent, RoundMode roundMode, TextureType type)
+{
+    int w = *width, h = *height, d = *depth;
+    if (roundMode != RoundMode_None && maxExtent > 0) maxExtent = int(prev_pow2(uint32_t(maxExtent)));
+    const int mx = std::max(std::max(w, h), d);
+    if (maxExtent > 0 && mx > maxExtent) {
+        w = std::max((w * maxExtent) / mx, 1);
+        h = std::max((h * maxExtent) / mx, 1);
+        d = std::max((d * maxExtent) / mx, 1);
+    }
+    if (type == TextureType_2D) {
+        d = 1;
+    } else if (type == TextureType_Cube) {
+        w = h = (w + h) / 2;
+        d = 1;
+    }
+    switch (roundMode) {
+    case RoundMode_ToNextPowerOfTwo: w = int(next_pow2(w)); h = int(next_pow2(h)); d = int(next_pow2(d)); break;
+    case RoundMode_ToNearestPowerOfTwo: w = int(nearest_pow2(w)); h = int(nearest_pow2(h)); d = int(nearest_pow2(d)); break;
+    case RoundMode_ToPreviousPowerOfTwo: w = int(prev_pow2(w)); h = int(prev_pow2(h)); d = int(prev_pow2(d)); break;
+    case RoundMode_ToNextMultipleOfFour: w = int(next_mul4(w)); h = int(next_mul4(h)); d = int(next_mul4(d)); break;
+    // RoundMode_ToNearestMultipleOfFour: the reference tests ToNextMultipleOfFour
+    // a second time in its place (Surface.cpp:273-284), so it leaves the extents.
+    case RoundMode_ToPreviousMultipleOfFour: w = int(prev_mul4(w)); h = int(prev_mul4(h)); d = int(prev_mul4(d)); break;
+    default: break;
+    }
+    *width = w; *height = h; *depth = d;
+}
+
+// Context.cpp:181-213 computeImageSize for the block formats.
+int block_bytes(Format f)
+{
+    switch (f) {
+    case Format_DXT1: case Format_DXT1a: case Format_DXT1n: case Format_BC4: case Format_CTX1: return 8;
+    case Format_DXT3: case Format_DXT5: case Format_DXT5n: case Format_BC5: case Format_BC3_RGBM: return 16;
+    default: return 0;
+    }
+}
+
+// Filter width and parameters of a mip step of process() (Context.cpp:297-304).
+void process_filter(const InputOptions::Private &io, float *width, float params[2])
+{
+    if (io.mipmapFilter == MipmapFilter_Kaiser) {
+        *width = io.kaiserWidth;
+        params[0] = io.kaiserAlpha;
+        params[1] = io.kaiserStretch;
+    } else {
+        *width = io.mipmapFilter == MipmapFilter_Triangle ? 1.0f : 0.5f;
+        params[0] = params[1] = 0.0f;
+    }
+}
+
+}  // namespace
+
+// Context.cpp:604-868 for the DDS containers.
+bool Compressor::Private::outputHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount,
+                                       bool isNormalMap, const CompressionOptions::Private &co,
+                                       const OutputOptions::Private &oo) const
+{
+    if (w <= 0 || h <= 0 || d <= 0 || arraySize <= 0 || mipmapCount <= 0) {
+        oo.error(Error_InvalidInput);
+        return false;
+    }
+    if (!oo.outputHeader) return true;
+    if (oo.container != Container_DDS && oo.container != Container_DDS10) {
+        oo.error(Error_UnsupportedOutputFormat);  // KTX is not provided
+        return false;
+    }
+    DdsHeader hd;
+    hd.reserved[7] = fourcc('U', 'V', 'E', 'R');  // setUserVersion (DirectDrawSurface.cpp:809-813)
+    hd.reserved[8] = uint32_t(oo.version);
+    // setTexture2D / setTextureCube / setTexture3D / setTextureArray (DirectDrawSurface.cpp:652-682).
+    if (type == TextureType_2D) {
+        hd.resourceDimension = kDimTexture2D; hd.miscFlag = 0; hd.arraySize = 1;
+    } else if (type == TextureType_Cube) {
+        hd.caps1 |= kDDSCAPS_COMPLEX;
+        hd.caps2 = kDDSCAPS2_CUBEMAP | kDDSCAPS2_CUBEMAP_ALL_FACES;
+        hd.resourceDimension = kDimTexture2D; hd.miscFlag = kMiscTextureCube; hd.arraySize = 1;
+    } else if (type == TextureType_3D) {
+        hd.caps2 = kDDSCAPS2_VOLUME;
+        hd.resourceDimension = kDimTexture3D; hd.miscFlag = 0; hd.arraySize = 1;
+        hd.flags |= kDDSD_DEPTH;
+        hd.depth = uint32_t(d);
+    } else {
+        hd.resourceDimension = kDimTexture2D; hd.arraySize = uint32_t(arraySize);
+    }
+    hd.flags |= kDDSD_WIDTH;
+    hd.width = uint32_t(w);
+    hd.flags |= kDDSD_HEIGHT;
+    hd.height = uint32_t(h);
+    hd.setMipmapCount(uint32_t(mipmapCount));
+
+    bool supported = true;
+    const Format f = co.format;
+    if (oo.container == Container_DDS10) {
+        if (f == Format_DXT1 || f == Format_DXT1a || f == Format_DXT1n) {
+            hd.setDX10Format(oo.srgb ? kDXGI_BC1_SRGB : kDXGI_BC1);
+            if (f == Format_DXT1a) hd.pf_flags |= kDDPF_ALPHAPIXELS;
+            if (isNormalMap) hd.pf_flags |= kDDPF_NORMAL;
+        } else if (f == Format_DXT3) {
+            hd.setDX10Format(oo.srgb ? kDXGI_BC2_SRGB : kDXGI_BC2);
+        } else if (f == Format_DXT5 || f == Format_BC3_RGBM) {
+            hd.setDX10Format(oo.srgb ? kDXGI_BC3_SRGB : kDXGI_BC3);
+        } else if (f == Format_DXT5n) {
+            hd.setDX10Format(kDXGI_BC3);
+            if (isNormalMap) hd.pf_flags |= kDDPF_NORMAL;
+        } else if (f == Format_BC4) {
+            hd.setDX10Format(kDXGI_BC4);
+        } else if (f == Format_BC5) {
+            hd.setDX10Format(kDXGI_BC5);
+            if (isNormalMap) hd.pf_flags |= kDDPF_NORMAL;
+        } else {
+            supported = false;
+        }
+    } else {
+        const int bb = block_bytes(f);
+        // setLinearSize (DirectDrawSurface.cpp:684-689): the top level's bytes.
+        hd.flags &= ~kDDSD_PITCH;
+        hd.flags |= kDDSD_LINEARSIZE;
+        hd.pitch = uint32_t(((w + 3) / 4) * ((h + 3) / 4) * bb * d);
+        if (f == Format_DXT1 || f == Format_DXT1a || f == Format_DXT1n) {
+            hd.setFourCC(fourcc('D', 'X', 'T', '1'));
+            if (isNormalMap) hd.pf_flags |= kDDPF_NORMAL;
+        } else if (f == Format_DXT3) {
+            hd.setFourCC(fourcc('D', 'X', 'T', '3'));
+        } else if (f == Format_DXT5 || f == Format_BC3_RGBM) {
+            hd.setFourCC(fourcc('D', 'X', 'T', '5'));
+        } else if (f == Format_DXT5n) {
+            hd.setFourCC(fourcc('D', 'X', 'T', '5'));
+            if (isNormalMap) {
+                hd.pf_flags |= kDDPF_NORMAL;
+                hd.pf_bitcount = fourcc('A', '2', 'D', '5');  // setSwizzleCode (DirectDrawSurface.cpp:724-728)
+            }
+        } else if (f == Format_BC4) {
+            hd.setFourCC(fourcc('A', 'T', 'I', '1'));
+        } else if (f == Format_BC5) {
+            hd.setFourCC(fourcc('A', 'T', 'I', '2'));
+            if (isNormalMap) {
+                hd.pf_flags |= kDDPF_NORMAL;
+                hd.pf_bitcount = fourcc('A', '2', 'X', 'Y');
+            }
+        } else if (f == Format_CTX1) {
+            hd.setFourCC(fourcc('C', 'T', 'X', '1'));
+            if (isNormalMap) hd.pf_flags |= kDDPF_NORMAL;
+        } else {
+            supported = false;  // uncompressed, BC6, BC7 and the rest: not on this path
+        }
+        if (oo.srgb) hd.pf_flags |= kDDPF_SRGB;
+    }
+    if (!supported) {
+        oo.error(Error_UnsupportedOutputFormat);
+        return false;
+    }
+    const int size = hd.hasDX10() ? 148 : 128;
+    const bool ok = oo.writeData(&hd, size);
+    if (!ok) oo.error(Error_FileWrite);
+    return ok;
+}
+
+namespace {
+
+// Surface::setImage(format, w, h, 1, data) into the device planes dst
+// (Surface.cpp:728-815), followed by Surface::toLinear(gamma) when linearize
+// (Surface.cpp:1470-1477).  staging collects device buffers to free later.
+ntx_status load_level(ntx_hip_context *ctx, InputFormat format, int w, int h, const std::vector<uint8_t> &data,
+                      bool linearize, float gamma, float *dst, std::vector<void *> &staging)
+{
+    const size_t n = size_t(w) * size_t(h);
+    const float g = linearize ? gamma : 1.0f;
+    if (format == InputFormat_BGRA_8UB) {
+        void *p = nullptr;
+        ntx_status s = ntx_hip_malloc(ctx, 4 * n, &p);
+        if (s != NTX_OK) return s;
+        staging.push_back(p);
+        s = ntx_hip_upload(ctx, p, data.data(), 4 * n);
+        if (s == NTX_OK) s = ntx_hip_unpack_bgra8_gamma_device(ctx, static_cast<const uint8_t *>(p), w, h, g, dst);
+        return s;
+    }
+    std::vector<float> planar(4 * n, 0.0f);
+    if (format == InputFormat_RGBA_16F) {
+        const uint16_t *src = reinterpret_cast<const uint16_t *>(data.data());
+        for (size_t i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) planar[c * n + i] = half_to_float(src[4 * i + c]);
+    } else if (format == InputFormat_RGBA_32F) {
+        const float *src = reinterpret_cast<const float *>(data.data());
+        for (size_t i = 0; i < n; i++)
+            for (int c = 0; c < 4; c++) planar[c * n + i] = src[4 * i + c];
+    } else {  // InputFormat_R_32F
+        const float *src = reinterpret_cast<const float *>(data.data());
+        for (size_t i = 0; i < n; i++) planar[i] = src[i];
+    }
+    ntx_status s = ntx_hip_upload(ctx, dst, planar.data(), planar.size() * sizeof(float));
+    if (s == NTX_OK && !unit_gamma(g)) s = ntx_hip_gamma_device(ctx, 0, g, dst, w, h, dst);
+    return s;
+}
+
+}  // namespace
+
+// Context.cpp:217-344 (the DDS branch).  Per face, level 0 comes from the
+// input (to linear unless a normal map) and every further level is either the
+// supplied image (to linear) or buildNextMipmap of the one before, a normal
+// map's renormalised; each level is compressed from its toGamma copy.  Here
+// every face's levels are built on the device first, the gamma copies of all
+// of them made in one launch, and all images compressed together by the
+// image scheduler (ntx_hip_compress_images_device); they are then delivered
+// face by face, level by level, with the reference's begin/write/end order.
+bool Compressor::Private::process(const InputOptions::Private &io, const CompressionOptions::Private &co,
+                                  const OutputOptions::Private &oo) const
+{
+    if (!oo.hasValidOutputHandler()) {
+        oo.error(Error_FileOpen);
+        return false;
+    }
+    const int faceCount = io.faceCount;
+    if (faceCount <= 0) {
+        oo.error(Error_InvalidInput);
+        return false;
+    }
+    int width = io.width, height = io.height, depth = io.depth;
+    const int arraySize = io.textureType == TextureType_Array ? faceCount : 1;
+    target_extent(&width, &height, &depth, io.maxExtent, io.roundMode, io.textureType);
+    const bool canUseSourceImages = io.width == width && io.height == height && io.depth == depth;
+    int mipmapCount = 1;
+    if (io.generateMipmaps) {
+        mipmapCount = count_mipmaps3(width, height, depth);
+        if (io.maxLevel > 0) mipmapCount = std::min(mipmapCount, io.maxLevel);
+    }
+    // Not on this path: resizing, volume textures, height-map conversion.
+    if (!canUseSourceImages || io.textureType == TextureType_3D || depth != 1 || io.convertToNormalMap) {
+        oo.error(Error_UnsupportedFeature);
+        return false;
+    }
+    for (int f = 0; f < faceCount; f++)
+        if (io.images[size_t(f)].empty()) {
+            oo.error(Error_InvalidInput);
+            return false;
+        }
+    if (!outputHeader(io.textureType, width, height, depth, arraySize, mipmapCount, io.isNormalMap, co, oo)) return false;
+
+    // The images in delivery order: face-major, then level.
+    struct Image { int face, mip, w, h; };
+    std::vector<Image> images;
+    for (int f = 0; f < faceCount; f++) {
+        int w = width, h = height;
+        for (int m = 0; m < mipmapCount; m++) {
+            images.push_back({ f, m, w, h });
+            w = w / 2 > 1 ? w / 2 : 1;
+            h = h / 2 > 1 ? h / 2 : 1;
+        }
+    }
+    const int fmt = int(co.format);
+    auto deliver_error = [&](Error e) {
+        for (const Image &im : images) {
+            oo.beginImage(int(ntx_hip_compressed_size(fmt, im.w, im.h)), im.w, im.h, 1, im.face, im.mip);
+            oo.error(e);
+            oo.endImage();
+        }
+        return true;
+    };
+    if (!hipEnabled || !backend()) return deliver_error(Error_CudaError);
+    CompressorInterface *probe = chooseCompressor(co, true);
+    if (!probe) return deliver_error(Error_UnsupportedFeature);
+    delete probe;
+
+    const size_t n = images.size();
+    const bool normal = io.isNormalMap;
+    const bool regamma = !normal && !unit_gamma(io.outputGamma);
+    const AlphaMode alphaMode = io.alphaMode;
+    float fw, fp[2];
+    process_filter(io, &fw, fp);
+    std::vector<float *> lin(n, nullptr), gam(n, nullptr);
+    std::vector<void *> out(n, nullptr), staging;
+    std::vector<int> ws(n), hs(n);
+    std::vector<std::vector<uint8_t>> bytes(n);
+    ntx_status s = NTX_OK;
+    {
+        std::lock_guard<std::mutex> g(backend_lock());
+        ntx_hip_context *ctx = backend();
+        for (size_t i = 0; s == NTX_OK && i < n; i++) {
+            ws[i] = images[i].w;
+            hs[i] = images[i].h;
+            const size_t fbytes = 4 * size_t(ws[i]) * size_t(hs[i]) * sizeof(float);
+            void *p = nullptr;
+            s = ntx_hip_malloc(ctx, fbytes, &p);
+            lin[i] = static_cast<float *>(p);
+            if (s == NTX_OK && regamma) {
+                s = ntx_hip_malloc(ctx, fbytes, &p);
+                gam[i] = static_cast<float *>(p);
+            } else {
+                gam[i] = lin[i];
+            }
+            if (s == NTX_OK) s = ntx_hip_malloc(ctx, ntx_hip_compressed_size(fmt, ws[i], hs[i]), &out[i]);
+        }
+        size_t i = 0;
+        for (int f = 0; s == NTX_OK && f < faceCount; f++) {
+            bool canUseFace = canUseSourceImages;
+            for (int m = 0; s == NTX_OK && m < mipmapCount; m++, i++) {
+                if (m == 0) {
+                    s = load_level(ctx, io.inputFormat, ws[i], hs[i], io.images[size_t(f)], !normal, io.inputGamma, lin[i],
+                                   staging);
+                    continue;
+                }
+                const std::vector<uint8_t> &src = io.images[size_t(m) * size_t(faceCount) + size_t(f)];
+                bool useSource = false;
+                if (canUseFace) {
+                    if (src.empty()) canUseFace = false;  // a missing level: generate this and the rest
+                    else useSource = true;
+                }
+                if (useSource)
+                    s = load_level(ctx, io.inputFormat, ws[i], hs[i], src, !normal, io.inputGamma, lin[i], staging);
+                else
+                    s = ntx_hip_downsample_device(ctx, int(io.mipmapFilter), fw, fp, int(io.wrapMode), int(alphaMode),
+                                                  lin[i - 1], ws[i - 1], hs[i - 1], lin[i]);
+                if (s == NTX_OK && normal && io.normalizeMipmaps)
+                    s = ntx_hip_normalize_normals_device(ctx, lin[i], ws[i], hs[i]);
+            }
+        }
+        if (s == NTX_OK && regamma)
+            s = ntx_hip_gamma_levels_device(ctx, 1, io.outputGamma, int(n), const_cast<const float *const *>(lin.data()),
+                                            ws.data(), hs.data(), gam.data());
+        if (s == NTX_OK)
+            s = ntx_hip_compress_images_device(ctx, fmt, int(co.quality), int(alphaMode), co.colorWeight, int(n),
+                                               const_cast<const float *const *>(gam.data()), ws.data(), hs.data(),
+                                               out.data());
+        if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+        for (size_t j = 0; s == NTX_OK && j < n; j++) {
+            bytes[j].resize(ntx_hip_compressed_size(fmt, ws[j], hs[j]));
+            s = ntx_hip_download(ctx, bytes[j].data(), out[j], bytes[j].size());
+        }
+        (void)ntx_hip_synchronize(ctx);
+        for (size_t j = 0; j < n; j++) {
+            if (gam[j] && gam[j] != lin[j]) ntx_hip_free(ctx, gam[j]);
+            if (lin[j]) ntx_hip_free(ctx, lin[j]);
+            if (out[j]) ntx_hip_free(ctx, out[j]);
+        }
+        for (void *p : staging) ntx_hip_free(ctx, p);
+    }
+    if (s != NTX_OK) {
+        report("process", s);
+        return deliver_error(s == NTX_ERR_INVALID_ARGUMENT ? Error_InvalidInput
+                             : s == NTX_ERR_UNSUPPORTED   ? Error_UnsupportedFeature
+                                                          : Error_CudaError);
+    }
+    for (size_t j = 0; j < n; j++) {
+        oo.beginImage(int(bytes[j].size()), ws[j], hs[j], 1, images[j].face, images[j].mip);
+        oo.writeData(bytes[j].data(), int(bytes[j].size()));
+        oo.endImage();
+    }
+    return true;
+}
 
 struct BatchList::Private {
     struct Entry {
@@ -521,6 +1227,33 @@ bool Compressor::compress(const Surface &img, int face, int mipmap, const Compre
 {
     return m.compress(img.alphaMode(), img.width(), img.height(), img.depth(), face, mipmap, img.deviceData(), true, co.m,
                       oo.m);
+}
+
+bool Compressor::process(const InputOptions &io, const CompressionOptions &co, const OutputOptions &oo) const
+{
+    return m.process(io.m, co.m, oo.m);
+}
+
+// Context.cpp:122-137.
+int Compressor::estimateSize(const InputOptions &io, const CompressionOptions &co) const
+{
+    int w = io.m.width, h = io.m.height, d = io.m.depth;
+    if (w <= 0 || h <= 0 || d <= 0) return 0;
+    target_extent(&w, &h, &d, io.m.maxExtent, io.m.roundMode, io.m.textureType);
+    int mipmapCount = 1;
+    if (io.m.generateMipmaps) {
+        mipmapCount = count_mipmaps3(w, h, d);
+        if (io.m.maxLevel > 0) mipmapCount = std::min(mipmapCount, io.m.maxLevel);
+    }
+    return io.m.faceCount * estimateSize(w, h, d, mipmapCount, co);
+}
+
+// Context.cpp:141-144.
+bool Compressor::outputHeader(const Surface &img, int mipmapCount, const CompressionOptions &co,
+                              const OutputOptions &oo) const
+{
+    return m.outputHeader(img.type(), img.width(), img.height(), img.depth(), 1, mipmapCount, img.isNormalMap(), co.m,
+                          oo.m);
 }
 
 bool Compressor::compress(int w, int h, int d, int face, int mipmap, const float *rgba, const CompressionOptions &co,

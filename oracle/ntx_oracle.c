@@ -1222,3 +1222,102 @@ void ntxo_srgb_to_linear(size_t n, float *d)
         d[i] = f;
     }
 }
+
+/* ---------------------------------------------------------------- gamma --
+ * Gamma.cpp:30-300: exponent tables.  Entry k is 2^((k - 127) p) for the
+ * sign-and-exponent field k of the input, rounded to float; 0 for zero and
+ * subnormal inputs (k = 0) and for negative ones (k >= 256); +inf for inf and
+ * NaN (k = 255).  Rebuilt here from that closed form. */
+static float g_pow_5_11[512], g_pow_11_5[512];
+static int g_gamma_ready;
+
+static void gamma_tables(void)
+{
+    if (g_gamma_ready) return;
+    for (int k = 0; k < 512; k++) {
+        if (k == 0 || k >= 256) { g_pow_5_11[k] = 0.0f; g_pow_11_5[k] = 0.0f; }
+        else if (k == 255) { g_pow_5_11[k] = INFINITY; g_pow_11_5[k] = INFINITY; }
+        else {
+            g_pow_5_11[k] = (float)pow(2.0, (double)(k - 127) * 5.0 / 11.0);
+            g_pow_11_5[k] = (float)pow(2.0, (double)(k - 127) * 11.0 / 5.0);
+        }
+    }
+    g_gamma_ready = 1;
+}
+
+void ntxo_gamma_table(int which, float *out512)
+{
+    gamma_tables();
+    memcpy(out512, which ? g_pow_11_5 : g_pow_5_11, sizeof g_pow_5_11);
+}
+
+/* Mantissa of x with the exponent of 1.0: a float in [1, 2). */
+static float unit_mantissa(float x, unsigned *k)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    *k = u >> 23;
+    u = (u & ((1u << 23) - 1u)) | (127u << 23);
+    float m;
+    memcpy(&m, &u, 4);
+    return m;
+}
+
+/* Gamma.cpp:311-330 _powf_5_11: table(exponent) * quartic(mantissa). */
+float ntxo_powf_5_11(float x)
+{
+    gamma_tables();
+    unsigned k;
+    const float m = unit_mantissa(x, &k);
+    const float pe = g_pow_5_11[k];
+    const float pm = (((-0.0110083047f * m + 0.0905038750f) * m - 0.324697506f) * m + 0.876040946f) * m + 0.369160989f;
+    return pe * pm;
+}
+
+/* Gamma.cpp:338-357 _powf_11_5. */
+float ntxo_powf_11_5(float x)
+{
+    gamma_tables();
+    unsigned k;
+    const float m = unit_mantissa(x, &k);
+    const float pe = g_pow_11_5[k];
+    const float pm = (((-0.00916587552f * m + 0.119315466f) * m + 1.01847068f) * m - 0.158338739f) * m + 0.0297184721f;
+    return pe * pm;
+}
+
+/* FloatImage.cpp:286-297 exponentiate: powf(nv::max(0, x), power). */
+static void exponentiate(size_t n, float *d, float power)
+{
+    for (size_t i = 0; i < n; i++) {
+        const float x = d[i] < 0.0f ? 0.0f : d[i];
+        d[i] = powf(x, power);
+    }
+}
+
+/* FloatImage.cpp:258-270 toLinear. */
+void ntxo_to_linear(size_t n, float *d, float gamma)
+{
+    if (gamma == 2.2f) { for (size_t i = 0; i < n; i++) d[i] = ntxo_powf_11_5(d[i]); }
+    else exponentiate(n, d, gamma);
+}
+
+/* FloatImage.cpp:272-284 toGamma. */
+void ntxo_to_gamma(size_t n, float *d, float gamma)
+{
+    if (gamma == 2.2f) { for (size_t i = 0; i < n; i++) d[i] = ntxo_powf_5_11(d[i]); }
+    else exponentiate(n, d, 1.0f / gamma);
+}
+
+/* Context.cpp:329-336: scaleBias(2, -2*127/255) (FloatImage.cpp:231-242),
+ * normalizeSafe(v, 0, 0) (Vector.inl:549-595), scaleBias(0.5, 0.5). */
+void ntxo_normalize_normals(size_t n, float *p)
+{
+    const float bias = -2.0f * 127.0f / 255.0f;
+    for (size_t i = 0; i < n; i++) {
+        float x = 2.0f * p[i] + bias, y = 2.0f * p[i + n] + bias, z = 2.0f * p[i + 2 * n] + bias;
+        const float l = sqrtf(x * x + y * y + z * z);
+        if (fabsf(l) <= 0.0f) { x = y = z = 0.0f; }
+        else { const float s = 1.0f / l; x = x * s; y = y * s; z = z * s; }
+        p[i] = 0.5f * x + 0.5f; p[i + n] = 0.5f * y + 0.5f; p[i + 2 * n] = 0.5f * z + 0.5f;
+    }
+}

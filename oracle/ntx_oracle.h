@@ -80,6 +80,19 @@ void ntxo_build_next_mipmap(int filter, float width, const float *params, int wr
                             int W, int H, const float *src, float *dst);
 void ntxo_srgb_to_linear(size_t n, float *data);
 
+/* ---- gamma (FloatImage.cpp:258-297, Gamma.cpp:30-441) ----
+ * The exponent tables of the gamma-2.2 forms (which: 0 = 5/11, 1 = 11/5),
+ * and FloatImage::toLinear / toGamma on n floats (the R, G, B planes). */
+void ntxo_gamma_table(int which, float *out512);
+float ntxo_powf_5_11(float x);
+float ntxo_powf_11_5(float x);
+void ntxo_to_linear(size_t n, float *data, float gamma);
+void ntxo_to_gamma(size_t n, float *data, float gamma);
+/* Context.cpp:329-336 on a generated normal-map level: expandNormals()
+ * (Surface.cpp:2959-2963), normalizeNormalMap() (FloatImage.cpp:201-219),
+ * packNormals() (Surface.cpp:2952-2956); planar RGBA of n texels, in place. */
+void ntxo_normalize_normals(size_t n, float *planar);
+
 #ifdef __cplusplus
 }
 #endif

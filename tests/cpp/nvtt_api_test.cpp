@@ -7,6 +7,12 @@
 //       in.raw:  int32 w, int32 h, then w*h BGRA8 texels
 //       out.bin: per level int32 w, h, size, then size bytes; then int32 event count
 //                and the handler events (int32 code, six int32 arguments)
+//   nvtt_api_test process <in.raw> <out.bin> <format> <filter> <cube> <alphaMode> <wrapMode> <normalMap>
+//                         <inGamma> <outGamma> <container> <header> <mipmaps> <maxLevel> <normalize>
+//       Compressor::process through InputOptions.  in.raw: int32 w, h, faces,
+//       supplied levels L, then for each face its levels 0..L-1 as BGRA8 texels.
+//       out.bin: int32 byte count, the bytes of every writeData in order, then
+//                the handler events as above, then int32 error count and the errors
 //   nvtt_api_test api      checks error reporting, handler order and Surface semantics
 //   nvtt_api_test batch    Compressor::compress(BatchList) against per-image compress
 #include <cmath>
@@ -107,6 +113,70 @@ int run_chain(int argc, char **argv)
     int32_t n = int32_t(cap.events.size() / 7);
     std::fwrite(&n, 4, 1, o);
     std::fwrite(cap.events.data(), 4, cap.events.size(), o);
+    std::fclose(o);
+    return 0;
+}
+
+int run_process(int argc, char **argv)
+{
+    if (argc < 17) return fail("usage");
+    FILE *f = std::fopen(argv[2], "rb");
+    if (!f) return fail("open input");
+    int32_t hd[4];
+    if (std::fread(hd, 4, 4, f) != 4) return fail("read header");
+    const int w = hd[0], h = hd[1], faces = hd[2], supplied = hd[3];
+    nvtt::InputOptions io;
+    io.setTextureLayout(std::atoi(argv[6]) ? nvtt::TextureType_Cube : nvtt::TextureType_2D, w, h);
+    for (int face = 0; face < faces; face++) {
+        int lw = w, lh = h;
+        for (int m = 0; m < supplied; m++) {
+            std::vector<uint8_t> bgra(size_t(lw) * lh * 4);
+            if (std::fread(bgra.data(), 1, bgra.size(), f) != bgra.size()) return fail("read texels");
+            CHECK(io.setMipmapData(bgra.data(), lw, lh, 1, face, m));
+            lw = lw / 2 > 1 ? lw / 2 : 1;
+            lh = lh / 2 > 1 ? lh / 2 : 1;
+        }
+    }
+    std::fclose(f);
+    io.setMipmapFilter(nvtt::MipmapFilter(std::atoi(argv[5])));
+    io.setAlphaMode(nvtt::AlphaMode(std::atoi(argv[7])));
+    io.setWrapMode(nvtt::WrapMode(std::atoi(argv[8])));
+    io.setNormalMap(std::atoi(argv[9]) != 0);
+    io.setGamma(float(std::atof(argv[10])), float(std::atof(argv[11])));
+    io.setMipmapGeneration(std::atoi(argv[14]) != 0, std::atoi(argv[15]));
+    io.setNormalizeMipmaps(std::atoi(argv[16]) != 0);
+
+    nvtt::CompressionOptions co;
+    co.setFormat(nvtt::Format(std::atoi(argv[4])));
+    nvtt::OutputOptions oo;
+    Capture cap;
+    Errors err;
+    oo.setOutputHandler(&cap);
+    oo.setErrorHandler(&err);
+    oo.setContainer(nvtt::Container(std::atoi(argv[12])));
+    oo.setOutputHeader(std::atoi(argv[13]) != 0);
+    oo.setUserVersion(7);
+    nvtt::Compressor compressor;
+    CHECK(compressor.isCudaAccelerationEnabled());
+    const bool ok = compressor.process(io, co, oo);
+    const int est = compressor.estimateSize(io, co);
+
+    FILE *o = std::fopen(argv[3], "wb");
+    if (!o) return fail("open output");
+    int32_t nb = int32_t(cap.bytes.size());
+    std::fwrite(&nb, 4, 1, o);
+    std::fwrite(cap.bytes.data(), 1, cap.bytes.size(), o);
+    int32_t n = int32_t(cap.events.size() / 7);
+    std::fwrite(&n, 4, 1, o);
+    std::fwrite(cap.events.data(), 4, cap.events.size(), o);
+    int32_t ne = int32_t(err.seen.size());
+    std::fwrite(&ne, 4, 1, o);
+    for (nvtt::Error e : err.seen) {
+        int32_t v = int32_t(e);
+        std::fwrite(&v, 4, 1, o);
+    }
+    int32_t tail[2] = { ok ? 1 : 0, est };
+    std::fwrite(tail, 4, 2, o);
     std::fclose(o);
     return 0;
 }
@@ -287,6 +357,7 @@ int run_batch()
 int main(int argc, char **argv)
 {
     if (argc >= 2 && std::strcmp(argv[1], "chain") == 0) return run_chain(argc, argv);
+    if (argc >= 2 && std::strcmp(argv[1], "process") == 0) return run_process(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "api") == 0) return run_api();
     if (argc >= 2 && std::strcmp(argv[1], "batch") == 0) return run_batch();
     return fail("usage: nvtt_api_test chain|api|batch ...");

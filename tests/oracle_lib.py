@@ -33,6 +33,14 @@ class Oracle:
         L.ntxo_downsample.argtypes = [ctypes.c_int, ctypes.c_float, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
         L.ntxo_build_next_mipmap.argtypes = L.ntxo_downsample.argtypes
         L.ntxo_srgb_to_linear.argtypes = [ctypes.c_size_t, _P]
+        L.ntxo_gamma_table.argtypes = [ctypes.c_int, _P]
+        L.ntxo_powf_5_11.argtypes = [ctypes.c_float]
+        L.ntxo_powf_5_11.restype = ctypes.c_float
+        L.ntxo_powf_11_5.argtypes = [ctypes.c_float]
+        L.ntxo_powf_11_5.restype = ctypes.c_float
+        L.ntxo_to_linear.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
+        L.ntxo_to_gamma.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
+        L.ntxo_normalize_normals.argtypes = [ctypes.c_size_t, _P]
         L.ntxo_init()
 
     # ---- tables
@@ -121,6 +129,28 @@ class Oracle:
 
     def srgb_to_linear(self, x):
         a = f32(x).copy(); self.lib.ntxo_srgb_to_linear(a.size, _p(a)); return a
+
+    def gamma_table(self, which):
+        """Exponent table of powf(x, 5/11) (which=0) or powf(x, 11/5) (which=1)."""
+        t = np.zeros(512, np.float32); self.lib.ntxo_gamma_table(int(which), _p(t)); return t
+
+    def to_linear(self, x, gamma=2.2):
+        a = f32(x).copy(); self.lib.ntxo_to_linear(a.size, _p(a), float(gamma)); return a
+
+    def to_gamma(self, x, gamma=2.2):
+        a = f32(x).copy(); self.lib.ntxo_to_gamma(a.size, _p(a), float(gamma)); return a
+
+    def planar_to_linear(self, planar, gamma=2.2):
+        """FloatImage::toLinear(0, 3, gamma) of planar RGBA: R, G, B transformed, A kept."""
+        a = f32(planar).reshape(-1).copy(); n = a.size // 4
+        a[:3 * n] = self.to_linear(a[:3 * n], gamma); return a
+
+    def normalize_normals(self, planar):
+        a = f32(planar).reshape(-1).copy(); self.lib.ntxo_normalize_normals(a.size // 4, _p(a)); return a
+
+    def planar_to_gamma(self, planar, gamma=2.2):
+        a = f32(planar).reshape(-1).copy(); n = a.size // 4
+        a[:3 * n] = self.to_gamma(a[:3 * n], gamma); return a
 
 
 class Reference:
