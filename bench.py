@@ -297,6 +297,7 @@ def main():
     }
 
     result["c1_box_chain_bc1"] = bench_c1(ctx, args, not args.no_cpu)
+    result["input_pipeline"] = bench_process(args, not args.no_cpu)
     if not args.no_batch:
         result["batch"] = bench_batch(ctx, args)
     if not args.no_learned:
@@ -586,6 +587,63 @@ def _cube_face(face: int, S: int, seed: int) -> np.ndarray:
         img[:, :, ch] = np.floor(np.clip(0.6 * grad + 0.4 * noise, 0, 1) * 255 + 0.5).astype(np.uint8)
     img[:, :, 3] = 255
     return img
+
+
+def bench_process(args, cpu: bool) -> dict:
+    """nvtt::Compressor::process (the InputOptions pipeline) through the nvtt.h
+    front end, timed by the C++ driver around whole calls: BGRA8 host input
+    uploaded, to linear (gamma 2.2), mips in linear space, the gamma-2.2
+    copies, compression, the DDS header and every level delivered to an
+    OutputHandler on the host.  Host-to-device and device-to-host copies are
+    inside the time.  With cpu, the C1 case through the oracle's restatement
+    of the same steps on the host."""
+    import struct
+    import subprocess
+    import tempfile
+
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    driver = os.path.join(N.LIB_DIR, "nvtt_api_test")
+    out = {}
+    cases = [("c1_bc1_box_1024", 1024, N.FORMAT_BC1, N.FILTER_BOX, 20),
+             ("c2_bc3_kaiser_4096", 4096, N.FORMAT_BC3, N.FILTER_KAISER, 8)]
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, S, fmt, filt, reps in cases:
+            img = synth.noise_rect_rgba(S, S, seed=args.seed, transparent=(fmt == N.FORMAT_BC3))
+            path = os.path.join(tmp, name + ".raw")
+            with open(path, "wb") as f:
+                f.write(struct.pack("<iiii", S, S, 1, 1))
+                f.write(synth.to_bgra(img).tobytes())
+            log(f"input pipeline: {name}")
+            r = subprocess.run([driver, "process-time", path, str(fmt), str(filt), str(reps)], capture_output=True,
+                               text=True, timeout=300)
+            if r.returncode != 0:
+                out[name] = {"error": r.stderr.strip()[-200:]}
+                continue
+            fields = r.stdout.split()
+            ms = float(fields[fields.index("process_ms") + 1])
+            nbytes = int(fields[fields.index("bytes") + 1])
+            texels = sum(w * h for w, h in N.mip_sizes(S, S))
+            out[name] = {"workload": f"{S}x{S} BGRA8 noise+rectangles -> InputOptions defaults (gamma 2.2 in and "
+                                     f"out, {'Box' if filt == N.FILTER_BOX else 'Kaiser(3; 4, 1)'} mips to 1x1) -> "
+                                     f"{'BC1' if fmt == N.FORMAT_BC1 else 'BC3'} -> DDS",
+                         "ms_per_call": round(ms, 3), "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1),
+                         "bytes_delivered": nbytes}
+            if cpu and fmt == N.FORMAT_BC1:
+                sys.path.insert(0, os.path.join(ROOT, "tests"))
+                import oracle_lib
+                orc = oracle_lib.Oracle()
+                threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+                t0 = time.perf_counter()
+                lin = orc.planar_to_linear(synth.to_planar(img).reshape(-1))
+                for k, (w, h) in enumerate(N.mip_sizes(S, S)):
+                    orc.compress(fmt, orc.planar_to_gamma(lin), w, h, threads=threads)
+                    if (w, h) != (1, 1):
+                        lin = orc.build_next_mipmap(filt, 0.5, None, N.WRAP_MIRROR, lin, w, h)
+                cpu_s = time.perf_counter() - t0
+                out[name]["cpu_port"] = {"mtexels_s": round(texels / cpu_s / 1e6, 3), "cores": threads}
+    return out
 
 
 def bench_c1(ctx, args, cpu: bool) -> dict:

@@ -677,6 +677,18 @@ struct HipCompressorBCn : CompressorInterface {
 struct Compressor::Private {
     bool hipEnabled = false;
     TaskDispatcher *dispatcher = nullptr;
+    // Device memory process() keeps between calls (level planes, their gamma
+    // copies, staging, compressed output), grown on demand, freed with the
+    // Compressor.
+    mutable void *arena = nullptr;
+    mutable size_t arenaBytes = 0;
+    ~Private()
+    {
+        if (arena) {
+            std::lock_guard<std::mutex> g(backend_lock());
+            ntx_hip_free(backend(), arena);
+        }
+    }
 
     // Context.cpp:1166-1224 chooseGpuCompressor: the formats and qualities the
     // device implements exactly; anything else gets no compressor.
@@ -973,18 +985,16 @@ namespace {
 
 // Surface::setImage(format, w, h, 1, data) into the device planes dst
 // (Surface.cpp:728-815), followed by Surface::toLinear(gamma) when linearize
-// (Surface.cpp:1470-1477).  staging collects device buffers to free later.
+// (Surface.cpp:1470-1477).  staging: 4 w h bytes of device memory for 8-bit
+// input.
 ntx_status load_level(ntx_hip_context *ctx, InputFormat format, int w, int h, const std::vector<uint8_t> &data,
-                      bool linearize, float gamma, float *dst, std::vector<void *> &staging)
+                      bool linearize, float gamma, float *dst, void *staging)
 {
     const size_t n = size_t(w) * size_t(h);
     const float g = linearize ? gamma : 1.0f;
     if (format == InputFormat_BGRA_8UB) {
-        void *p = nullptr;
-        ntx_status s = ntx_hip_malloc(ctx, 4 * n, &p);
-        if (s != NTX_OK) return s;
-        staging.push_back(p);
-        s = ntx_hip_upload(ctx, p, data.data(), 4 * n);
+        void *p = staging;  // 4 n bytes
+        ntx_status s = ntx_hip_upload(ctx, p, data.data(), 4 * n);
         if (s == NTX_OK) s = ntx_hip_unpack_bgra8_gamma_device(ctx, static_cast<const uint8_t *>(p), w, h, g, dst);
         return s;
     }
@@ -1080,29 +1090,53 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
     const AlphaMode alphaMode = io.alphaMode;
     float fw, fp[2];
     process_filter(io, &fw, fp);
+    // One device arena: level planes, gamma copies, the largest 8-bit staging
+    // image, compressed output (one block for a single download); offsets
+    // rounded to 256 bytes.
     std::vector<float *> lin(n, nullptr), gam(n, nullptr);
-    std::vector<void *> out(n, nullptr), staging;
+    std::vector<void *> out(n, nullptr);
     std::vector<int> ws(n), hs(n);
-    std::vector<std::vector<uint8_t>> bytes(n);
+    std::vector<size_t> outOff(n);
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t planeBytes = 0, outBytes = 0, stageBytes = 0;
+    for (size_t i = 0; i < n; i++) {
+        ws[i] = images[i].w;
+        hs[i] = images[i].h;
+        planeBytes += up(4 * size_t(ws[i]) * size_t(hs[i]) * sizeof(float));
+        outOff[i] = outBytes;
+        outBytes += ntx_hip_compressed_size(fmt, ws[i], hs[i]);
+        stageBytes = std::max(stageBytes, 4 * size_t(ws[i]) * size_t(hs[i]));
+    }
+    const size_t total = planeBytes * (regamma ? 2 : 1) + up(stageBytes) + up(outBytes);
+    std::vector<uint8_t> bytes(outBytes);
     ntx_status s = NTX_OK;
     {
         std::lock_guard<std::mutex> g(backend_lock());
         ntx_hip_context *ctx = backend();
+        if (arenaBytes < total) {
+            if (arena) ntx_hip_free(ctx, arena);
+            arena = nullptr;
+            arenaBytes = 0;
+            s = ntx_hip_malloc(ctx, total, &arena);
+            if (s == NTX_OK) arenaBytes = total;
+        }
+        uint8_t *base = static_cast<uint8_t *>(arena);
+        size_t off = 0;
         for (size_t i = 0; s == NTX_OK && i < n; i++) {
-            ws[i] = images[i].w;
-            hs[i] = images[i].h;
-            const size_t fbytes = 4 * size_t(ws[i]) * size_t(hs[i]) * sizeof(float);
-            void *p = nullptr;
-            s = ntx_hip_malloc(ctx, fbytes, &p);
-            lin[i] = static_cast<float *>(p);
-            if (s == NTX_OK && regamma) {
-                s = ntx_hip_malloc(ctx, fbytes, &p);
-                gam[i] = static_cast<float *>(p);
+            lin[i] = reinterpret_cast<float *>(base + off);
+            off += up(4 * size_t(ws[i]) * size_t(hs[i]) * sizeof(float));
+        }
+        for (size_t i = 0; s == NTX_OK && i < n; i++) {
+            if (regamma) {
+                gam[i] = reinterpret_cast<float *>(base + off);
+                off += up(4 * size_t(ws[i]) * size_t(hs[i]) * sizeof(float));
             } else {
                 gam[i] = lin[i];
             }
-            if (s == NTX_OK) s = ntx_hip_malloc(ctx, ntx_hip_compressed_size(fmt, ws[i], hs[i]), &out[i]);
         }
+        void *staging = base + off;
+        off += up(stageBytes);
+        for (size_t i = 0; i < n; i++) out[i] = base + off + outOff[i];
         size_t i = 0;
         for (int f = 0; s == NTX_OK && f < faceCount; f++) {
             bool canUseFace = canUseSourceImages;
@@ -1134,18 +1168,8 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
             s = ntx_hip_compress_images_device(ctx, fmt, int(co.quality), int(alphaMode), co.colorWeight, int(n),
                                                const_cast<const float *const *>(gam.data()), ws.data(), hs.data(),
                                                out.data());
-        if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
-        for (size_t j = 0; s == NTX_OK && j < n; j++) {
-            bytes[j].resize(ntx_hip_compressed_size(fmt, ws[j], hs[j]));
-            s = ntx_hip_download(ctx, bytes[j].data(), out[j], bytes[j].size());
-        }
+        if (s == NTX_OK) s = ntx_hip_download(ctx, bytes.data(), out[0], outBytes);
         (void)ntx_hip_synchronize(ctx);
-        for (size_t j = 0; j < n; j++) {
-            if (gam[j] && gam[j] != lin[j]) ntx_hip_free(ctx, gam[j]);
-            if (lin[j]) ntx_hip_free(ctx, lin[j]);
-            if (out[j]) ntx_hip_free(ctx, out[j]);
-        }
-        for (void *p : staging) ntx_hip_free(ctx, p);
     }
     if (s != NTX_OK) {
         report("process", s);
@@ -1154,8 +1178,9 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
                                                           : Error_CudaError);
     }
     for (size_t j = 0; j < n; j++) {
-        oo.beginImage(int(bytes[j].size()), ws[j], hs[j], 1, images[j].face, images[j].mip);
-        oo.writeData(bytes[j].data(), int(bytes[j].size()));
+        const int size = int(ntx_hip_compressed_size(fmt, ws[j], hs[j]));
+        oo.beginImage(size, ws[j], hs[j], 1, images[j].face, images[j].mip);
+        oo.writeData(bytes.data() + outOff[j], size);
         oo.endImage();
     }
     return true;

@@ -13,8 +13,12 @@
 //       supplied levels L, then for each face its levels 0..L-1 as BGRA8 texels.
 //       out.bin: int32 byte count, the bytes of every writeData in order, then
 //                the handler events as above, then int32 error count and the errors
+//   nvtt_api_test process-time <in.raw> <format> <filter> <reps>
+//       the same call (2D, defaults otherwise) repeated: prints the best wall
+//       time of one process() call in ms and the bytes it delivered
 //   nvtt_api_test api      checks error reporting, handler order and Surface semantics
 //   nvtt_api_test batch    Compressor::compress(BatchList) against per-image compress
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -178,6 +182,55 @@ int run_process(int argc, char **argv)
     int32_t tail[2] = { ok ? 1 : 0, est };
     std::fwrite(tail, 4, 2, o);
     std::fclose(o);
+    return 0;
+}
+
+struct Sink : nvtt::OutputHandler {
+    size_t bytes = 0;
+    void beginImage(int, int, int, int, int, int) override {}
+    bool writeData(const void *, int size) override
+    {
+        bytes += size_t(size);
+        return true;
+    }
+    void endImage() override {}
+};
+
+int run_process_time(int argc, char **argv)
+{
+    if (argc < 6) return fail("usage");
+    FILE *f = std::fopen(argv[2], "rb");
+    if (!f) return fail("open input");
+    int32_t hd[4];
+    if (std::fread(hd, 4, 4, f) != 4) return fail("read header");
+    std::vector<uint8_t> bgra(size_t(hd[0]) * hd[1] * 4);
+    if (std::fread(bgra.data(), 1, bgra.size(), f) != bgra.size()) return fail("read texels");
+    std::fclose(f);
+    nvtt::InputOptions io;
+    io.setTextureLayout(nvtt::TextureType_2D, hd[0], hd[1]);
+    CHECK(io.setMipmapData(bgra.data(), hd[0], hd[1]));
+    io.setMipmapFilter(nvtt::MipmapFilter(std::atoi(argv[4])));
+    nvtt::CompressionOptions co;
+    co.setFormat(nvtt::Format(std::atoi(argv[3])));
+    nvtt::OutputOptions oo;
+    Sink sink;
+    Errors err;
+    oo.setOutputHandler(&sink);
+    oo.setErrorHandler(&err);
+    nvtt::Compressor compressor;
+    CHECK(compressor.isCudaAccelerationEnabled());
+    const int reps = std::atoi(argv[5]);
+    double best = 1e30;
+    for (int r = 0; r <= reps; r++) {  // the first call warms up
+        sink.bytes = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        CHECK(compressor.process(io, co, oo));
+        const auto t1 = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (r > 0 && ms < best) best = ms;
+    }
+    CHECK(err.seen.empty());
+    std::printf("process_ms %.4f bytes %zu\n", best, sink.bytes);
     return 0;
 }
 
@@ -358,6 +411,7 @@ int main(int argc, char **argv)
 {
     if (argc >= 2 && std::strcmp(argv[1], "chain") == 0) return run_chain(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "process") == 0) return run_process(argc, argv);
+    if (argc >= 2 && std::strcmp(argv[1], "process-time") == 0) return run_process_time(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "api") == 0) return run_api();
     if (argc >= 2 && std::strcmp(argv[1], "batch") == 0) return run_batch();
     return fail("usage: nvtt_api_test chain|api|batch ...");
