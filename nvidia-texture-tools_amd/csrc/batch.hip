@@ -104,7 +104,7 @@ __device__ void bc1_item(const BatchArgs &a, const BatchItem &it, const BatchLev
         const int blk = it.first + k;
         int n, m;
         const uint2 r = ntx_bc1::encode_block(sh, data, lv.w, lv.h, blk % bw, blk / bw, lv.alpha_mode, lv.cw[0],
-                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m);
+                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m, lv.refine != 0);
         if (lane == 0) out[blk] = r;
     }
 }
@@ -139,8 +139,7 @@ void batch_kernel(BatchArgs a)
     __shared__ ntx_bc1::Bc1Shared sh;
     __shared__ int s_item;
     const int t = threadIdx.x;
-    for (int i = t; i < 976; i += 256) sh.four[i] = a.tab->four[i];
-    for (int i = t; i < 160; i += 256) sh.three[i] = a.tab->three[i];
+    ntx_bc1::load_shared_tables(sh, a.tab, t, 256);
     for (;;) {
         __syncthreads();  // the previous item is finished by every thread
         if (t == 0) {

@@ -1,7 +1,8 @@
 // bc1_block.hpp -- device code of the BC1 encoder shared by the per-image
 // kernel (bc1.hip) and the batch kernel (batch.hip): icbc v1.05
-// compress_dxt1 at Quality_Level8, the SSE4.1 build (no FMA, round-half-even,
-// 4-lane tie-break), icbc.h:3485-3559.
+// compress_dxt1 at Quality_Level8 (and Level9: the same plus refine_endpoints),
+// the SSE4.1 build (no FMA, round-half-even, 4-lane tie-break),
+// icbc.h:3485-3559.
 //
 // One 4x4 block per 16-lane group.  Lane l of a group owns texel l (gather,
 // index selection, error terms) and, after colour reduction, cluster l.  The
@@ -30,7 +31,16 @@ struct Bc1Shared {
     int order[kBlocksPerWG][16];
     uint32_t four[976];
     uint32_t three[160];
+    float div255[256];
 };
+
+// The workgroup's copy of the cluster tables and the k / 255 table.
+__device__ __forceinline__ void load_shared_tables(Bc1Shared &sh, const ntx::Tables *tab, int t, int nthreads)
+{
+    for (int i = t; i < 976; i += nthreads) sh.four[i] = tab->four[i];
+    for (int i = t; i < 160; i += nthreads) sh.three[i] = tab->three[i];
+    for (int i = t; i < 256; i += nthreads) sh.div255[i] = tab->div255[i];
+}
 
 __device__ __forceinline__ float sse_sat(float a)
 {
@@ -206,8 +216,9 @@ __device__ __forceinline__ void expand565(uint32_t c, int &r, int &g, int &b)
     b = (b5 << 3) | (b5 >> 2);
 }
 
-// D3D10 palette (icbc.h:2557-2585) as floats in [0,1] (color_to_vector3).
-__device__ __forceinline__ void palette(uint32_t c0, uint32_t c1, float p[4][3])
+// D3D10 palette (icbc.h:2557-2585) as floats in [0,1] (color_to_vector3,
+// through the k / 255.0f table div255).
+__device__ __forceinline__ void palette(uint32_t c0, uint32_t c1, float p[4][3], const float *div255)
 {
     int q[4][3];
     expand565(c0, q[0][0], q[0][1], q[0][2]);
@@ -225,20 +236,20 @@ __device__ __forceinline__ void palette(uint32_t c0, uint32_t c1, float p[4][3])
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) p[i][k] = float(q[i][k]) / 255.0f;
+        for (int k = 0; k < 3; k++) p[i][k] = div255[q[i][k]];
 }
 
 // output_block4 / output_block3 (icbc.h:2944-2980) for the group: lane l
 // classifies texel l; the weighted MSE is summed in texel order.
 template <bool FOUR>
 __device__ float output_block(const Cand &c, float tr, float tg, float tb, float tw, float cwx, float cwy, float cwz,
-                              const float *mid5, const float *mid6, int lane, uint2 *blk)
+                              const float *mid5, const float *mid6, const float *div255, int lane, uint2 *blk)
 {
     uint32_t c0 = to565(c.ax, c.ay, c.az, mid5, mid6);
     uint32_t c1 = to565(c.bx, c.by, c.bz, mid5, mid6);
     if (FOUR ? (c0 < c1) : (c0 > c1)) { const uint32_t t = c0; c0 = c1; c1 = t; }
     float p[4][3];
-    palette(c0, c1, p);
+    palette(c0, c1, p, div255);
     const float vr = tr * cwx, vg = tg * cwy, vb = tb * cwz;
     float d[4];
 #pragma unroll
@@ -373,6 +384,93 @@ __device__ __forceinline__ void compute_sat(Bc1Shared &sh, int g, int lane, int 
     ntx_wave_sync();
 }
 
+// One channel of entry `idx` of the D3D10 palette of (c0, c1) as an 8-bit
+// integer: shift/width select the 565 field, e0/e1 its bit expansion.
+__device__ __forceinline__ int palette_entry(int e0, int e1, int idx, bool four)
+{
+    if (idx == 0) return e0;
+    if (idx == 1) return e1;
+    if (four) return idx == 2 ? (2 * e0 + e1) / 3 : (2 * e1 + e0) / 3;
+    return idx == 2 ? (e0 + e1) / 2 : 0;
+}
+
+// compute_indices (icbc.h:2885-2930, the SSE form) for this lane's texel
+// against the palette of (c0, c1); the group's 32 index bits.
+__device__ __forceinline__ int group_indices(uint32_t c0, uint32_t c1, float tr, float tg, float tb, float cwx, float cwy,
+                                             float cwz, const float *div255, int lane)
+{
+    float p[4][3];
+    palette(c0, c1, p, div255);
+    const float cr = tr * cwx, cg = tg * cwy, cb = tb * cwz;
+    float d[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float dr = cr - p[k][0] * cwx, dg = cg - p[k][1] * cwy, db = cb - p[k][2] * cwz;
+        d[k] = dr * dr + dg * dg + db * db;
+    }
+    const bool i1 = (d[1] <= d[0]) && (d[1] < d[2]) && (d[1] < d[3]);
+    const bool i2 = (d[2] <= d[0]) && (d[2] <= d[1]) && (d[2] < d[3]);
+    const bool i3 = (d[3] <= d[0]) && (d[3] <= d[1]) && (d[3] <= d[2]);
+    int bits = int(((i1 || i3) ? 1u : 0u) | ((i2 || i3) ? 2u : 0u)) << (2 * lane);
+    bits |= gror<8>(bits);
+    bits |= gror<4>(bits);
+    bits |= gror<2>(bits);
+    bits |= gror<1>(bits);
+    return bits;
+}
+
+// refine_endpoints (icbc.h:3329-3406) with three_color_mode set, as
+// compress_dxt1 calls it at Quality_Level9 (CompressorDXT1::compressBlock
+// passes three_color_mode = true, BlockCompressor.cpp:211-222): up to 256
+// single steps of one endpoint's 5:6:5 fields (the fields wrap, as the
+// reference's bit-fields do), endpoint c1 for steps 0-15, c0 for 16-31, ...;
+// each candidate takes the indices of the current block's palette and is
+// kept when its evaluate_mse is lower; stops 33 steps after the last gain.
+// Lane l holds texel l; the error is summed in texel order.
+__device__ __forceinline__ uint2 refine_endpoints(uint2 blk, float best, float tr, float tg, float tb, float tw,
+                                                  float cwx, float cwy, float cwz, const float *div255, int lane)
+{
+    // The 16 steps (dr, dg, db), each component + 1 in two bits.
+    constexpr uint32_t kSteps[16] = { 0x16, 0x19, 0x25, 0x14, 0x11, 0x05, 0x1a, 0x26, 0x29,
+                                      0x10, 0x04, 0x01, 0x18, 0x12, 0x21, 0x09 };
+    uint32_t c0 = blk.x & 0xffffu, c1 = blk.x >> 16;
+    int ind = int(blk.y);
+    int idx = group_indices(c0, c1, tr, tg, tb, cwx, cwy, cwz, div255, lane);
+    int mine = (idx >> (2 * lane)) & 3;
+    int last = 0;
+    for (int i = 0; i < 256; i++) {
+        const uint32_t st = kSteps[i & 15];
+        const int dr = int(st & 3) - 1, dg = int((st >> 2) & 3) - 1, db = int((st >> 4) & 3) - 1;
+        uint32_t r0 = c0, r1 = c1;
+        uint32_t &t = ((i >> 4) & 1) ? r0 : r1;
+        t = (uint32_t((int(t >> 11) + dr) & 31) << 11) | (uint32_t((int((t >> 5) & 63) + dg) & 63) << 5) |
+            uint32_t((int(t & 31) + db) & 31);
+        // evaluate_mse (icbc.h:2738-2749) of (r0, r1, idx): this texel's term.
+        int er0, eg0, eb0, er1, eg1, eb1;
+        expand565(r0, er0, eg0, eb0);
+        expand565(r1, er1, eg1, eb1);
+        const bool four = r0 > r1;
+        const float pr = div255[palette_entry(er0, er1, mine, four)];
+        const float pg = div255[palette_entry(eg0, eg1, mine, four)];
+        const float pb = div255[palette_entry(eb0, eb1, mine, four)];
+        const float xr = ((pr - tr) * cwx) * 255.0f, xg = ((pg - tg) * cwy) * 255.0f, xb = ((pb - tb) * cwz) * 255.0f;
+        const float term = tw * (xr * xr + xg * xg + xb * xb);
+        float err = 0.0f;
+        static_for<0, 16>([&](auto k) { err += gbcast<decltype(k)::value>(term); });
+        if (err < best) {
+            best = err;
+            c0 = r0;
+            c1 = r1;
+            ind = idx;
+            last = i;
+            idx = group_indices(c0, c1, tr, tg, tb, cwx, cwy, cwz, div255, lane);
+            mine = (idx >> (2 * lane)) & 3;
+        }
+        if (i - last > 32) break;
+    }
+    return make_uint2(c0 | (c1 << 16), uint32_t(ind));
+}
+
 // compress_dxt1_single_color_optimal (icbc.h:3274-3289) on the rounded
 // colour (vector3_to_color32).
 __device__ __forceinline__ uint2 single_color_block(const ntx::Tables *tab, float kr, float kg, float kb)
@@ -393,7 +491,8 @@ __device__ __forceinline__ uint2 single_color_block(const ntx::Tables *tab, floa
 // it did not run).  Uses only wave-level synchronisation.
 __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, int w, int h, int bx, int by,
                                               int alpha_mode, float cwx, float cwy, float cwz,
-                                              const ntx::Tables *tab, int g, int lane, int *n_out, int *m_out)
+                                              const ntx::Tables *tab, int g, int lane, int *n_out, int *m_out,
+                                              bool refine = false)
 {
     // Gather (BlockCompressor.cpp:116-166): texel l; outside the image -> colour 0, weight 0.
     const int x = bx * 4 + (lane & 3), y = by * 4 + (lane >> 2);
@@ -455,7 +554,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
         Cand c;
         cluster_fit<true>(sh.sat[g], sh.four, tab->four_total[n - 1], lane, sum, cwsq_x, cwsq_y, cwsq_z, &c);
         uint2 b4;
-        const float e4 = output_block<true>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, lane, &b4);
+        const float e4 = output_block<true>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, sh.div255, lane, &b4);
         float best = e4;
         result = b4;
         bool run3 = true;
@@ -484,10 +583,12 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
             m_count = m;
             cluster_fit<false>(sh.sat[g], sh.three, tab->three_total[m - 1], lane, sum, cwsq_x, cwsq_y, cwsq_z, &c);
             uint2 b3;
-            const float e3 = output_block<false>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, lane, &b3);
+            const float e3 = output_block<false>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, sh.div255, lane, &b3);
             if (e3 < best) { best = e3; result = b3; }
         }
         if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
+        // Quality_Level9 (icbc.h:3551-3553): the cluster fit's block refined.
+        else if (refine) result = refine_endpoints(result, best, tr, tg, tb, tw, cwx, cwy, cwz, sh.div255, lane);
     }
     *n_out = n;
     *m_out = m_count;

@@ -943,8 +943,7 @@ __global__ __launch_bounds__(256) void bc1_list_kernel(const float *__restrict__
                                                        const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out)
 {
     __shared__ ntx_bc1::Bc1Shared sh;
-    for (int i = threadIdx.x; i < 976; i += blockDim.x) sh.four[i] = tab->four[i];
-    for (int i = threadIdx.x; i < 160; i += blockDim.x) sh.three[i] = tab->three[i];
+    ntx_bc1::load_shared_tables(sh, tab, threadIdx.x, blockDim.x);
     __syncthreads();
     const int total = *count;
     const int g = threadIdx.x >> 4, lane = threadIdx.x & 15;

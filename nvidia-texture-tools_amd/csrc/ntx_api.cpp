@@ -91,7 +91,10 @@ ntx_status bind(ntx_hip_context *ctx)
 bool supported(int format, int quality)
 {
     switch (format) {
-    case NTX_FORMAT_BC1: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_HIGHEST;
+    // BlockCompressor.cpp:211-217: Normal and Highest are icbc Level8,
+    // Production Level9 (Level8 + endpoint refinement).
+    case NTX_FORMAT_BC1:
+        return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_HIGHEST || quality == NTX_QUALITY_PRODUCTION;
     case NTX_FORMAT_BC3: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
     case NTX_FORMAT_BC4:
     case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
@@ -382,14 +385,15 @@ namespace ntx {
 // the flag is left true only after a complete front + search sequence.
 // lanes_log2: the BC3 search's lane target (bc3_lanes_shift); urgent: its
 // small launches raise their wave priority.
-static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int format, int alpha_mode, const float *cw,
-                              const float *d_rgba, int w, int h, void *d_out, int *bucket, bool *counts_zero,
-                              int lanes_log2 = 18, bool urgent = false)
+static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int format, int quality, int alpha_mode,
+                              const float *cw, const float *d_rgba, int w, int h, void *d_out, int *bucket,
+                              bool *counts_zero, int lanes_log2 = 18, bool urgent = false)
 {
     hipError_t e = hipSuccess;
     if (format == NTX_FORMAT_BC1) {
+        const bool refine = quality == NTX_QUALITY_PRODUCTION;
         e = timed_on(ctx, stream, NTX_KERNEL_BC1, [&] {
-            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent);
+            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent, refine);
         });
     } else if (format == NTX_FORMAT_BC3) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
@@ -504,7 +508,7 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
         if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
         bucket = static_cast<int *>(ctx->bucket.ptr);
     }
-    hipError_t e = ntx::compress_on(ctx, ctx->stream, format, alpha_mode, cw, d_rgba, w, h, d_out, bucket,
+    hipError_t e = ntx::compress_on(ctx, ctx->stream, format, quality, alpha_mode, cw, d_rgba, w, h, d_out, bucket,
                                     &ctx->bc3_counts_zero);
     if (e != hipSuccess) return fail(ctx, e, "compress launch");
     return NTX_OK;
@@ -526,7 +530,7 @@ namespace ntx {
 // one of which the null stream takes, more streams would share queues and
 // run one after another.  BC3 scratch: one region per image in ctx->chain,
 // laid out by the image sizes.
-static ntx_status compress_many(ntx_hip_context *ctx, int format, int alpha_mode, const float *cw, int n,
+static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, int alpha_mode, const float *cw, int n,
                                 const float *const *d_in, const int *w, const int *h, void *const *d_out)
 {
     ntx_status s = NTX_OK;
@@ -559,7 +563,7 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int alpha_mode
     auto compress_one = [&](hipStream_t st, int k) -> ntx_status {
         int *bucket = format == NTX_FORMAT_BC3 ? static_cast<int *>(ctx->chain.ptr) + region[k] : nullptr;
         bool z = zero;
-        hipError_t e = compress_on(ctx, st, format, alpha_mode, cw, d_in[k], w[k], h[k], d_out[k], bucket, &z,
+        hipError_t e = compress_on(ctx, st, format, quality, alpha_mode, cw, d_in[k], w[k], h[k], d_out[k], bucket, &z,
                                    k ? 16 : 18, st == small);
         if (e != hipSuccess) return fail(ctx, e, "compress launch");
         all_zero = all_zero && z;
@@ -621,7 +625,7 @@ NTX_API ntx_status ntx_hip_compress_chain_device(ntx_hip_context *ctx, int forma
     if ((s = ntx::downsample_chain_on(ctx, ctx->stream, filter, filter_width, filter_params, wrap, alpha_mode,
                                       lw.data(), lh.data(), levels, d_levels)))
         return s;
-    return ntx::compress_many(ctx, format, alpha_mode, cw, levels, d_levels, lw.data(), lh.data(), d_out);
+    return ntx::compress_many(ctx, format, quality, alpha_mode, cw, levels, d_levels, lw.data(), lh.data(), d_out);
 }
 
 NTX_API ntx_status ntx_hip_compress_images_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
@@ -651,7 +655,7 @@ NTX_API ntx_status ntx_hip_compress_images_device(ntx_hip_context *ctx, int form
         sh[k] = h[order[k]];
         out[k] = d_out[order[k]];
     }
-    return ntx::compress_many(ctx, format, alpha_mode, cw, n, in.data(), sw.data(), sh.data(), out.data());
+    return ntx::compress_many(ctx, format, quality, alpha_mode, cw, n, in.data(), sw.data(), sh.data(), out.data());
 }
 
 NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_rgba, int w, int h, const void *d_blocks,

@@ -48,6 +48,8 @@ struct Tables {
     // 2^((k - 127) * 11/5), as a float; 0 for k == 0 and k >= 256, +inf for 255.
     float pow_5_11[512];
     float pow_11_5[512];
+    // k / 255.0f for k = 0..255 (icbc's color_to_vector3, icbc.h:1611-1613).
+    float div255[256];
 };
 
 void build_tables(Tables *t);

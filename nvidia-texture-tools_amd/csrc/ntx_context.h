@@ -12,7 +12,7 @@
 
 namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
-                      int16_t *d_nm, hipStream_t stream, bool urgent = false);
+                      int16_t *d_nm, hipStream_t stream, bool urgent = false, bool refine = false);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
                            bool urgent = false);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,

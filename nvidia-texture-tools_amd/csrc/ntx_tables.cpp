@@ -124,6 +124,8 @@ void build_tables(Tables *t)
                 }
             }
 
+    for (int k = 0; k < 256; k++) t->div255[k] = float(k) / 255.0f;
+
     // Gamma-2.2 exponent tables (Gamma.cpp:30-300) from their closed form.
     for (int k = 0; k < 512; k++) {
         t->pow_5_11[k] = ntx_gamma::table_entry(k, 5.0, 11.0);

@@ -510,9 +510,63 @@ static void single_color(v3 c, uint8_t *out)
     put_block(out, c0, c1, idx);
 }
 
-/* icbc.h:3485-3559 compress_dxt1 at Quality_Level8 with three_color_mode and
- * three_color_black enabled (BlockCompressor.cpp:211-222). */
-float ntxo_bc1_block(const float *in, const float *inw, const float *cwp, uint8_t *out, int *n_out, int *m_out)
+/* icbc.h:2885-2930 compute_indices (SSE form): nearest of the four palette
+ * entries with the reference's comparison order. */
+static uint32_t indices_any(const float *in, v3 cw, const v3 pal[4])
+{
+    v3 p[4];
+    for (int k = 0; k < 4; k++) { p[k].x = pal[k].x * cw.x; p[k].y = pal[k].y * cw.y; p[k].z = pal[k].z * cw.z; }
+    uint32_t idx = 0;
+    for (int i = 0; i < 16; i++) {
+        v3 c = { in[4 * i] * cw.x, in[4 * i + 1] * cw.y, in[4 * i + 2] * cw.z };
+        float d0 = dist2(c, p[0]), d1 = dist2(c, p[1]), d2 = dist2(c, p[2]), d3 = dist2(c, p[3]);
+        int i1 = (d1 <= d0) && (d1 < d2) && (d1 < d3);
+        int i2 = (d2 <= d0) && (d2 <= d1) && (d2 < d3);
+        int i3 = (d3 <= d0) && (d3 <= d1) && (d3 <= d2);
+        idx |= (uint32_t)((i1 | i3) | ((i2 | i3) << 1)) << (2 * i);
+    }
+    return idx;
+}
+
+/* icbc.h:3329-3406 refine_endpoints with three_color_mode = true: step one
+ * endpoint's 5:6:5 bit-fields (wrapping) by deltas[i % 16], endpoint 1 for
+ * i / 16 even and endpoint 0 for odd; the candidate's indices are those of
+ * the current block's palette; keep it when its evaluate_mse is lower; stop
+ * 33 steps after the last improvement. */
+static float refine_endpoints(const float *in, const float *w, v3 cw, float input_error, uint8_t *out)
+{
+    static const int deltas[16][3] = {
+        { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 }, { -1, 0, 0 }, { 0, -1, 0 }, { 0, 0, -1 },
+        { 1, 1, 0 }, { 1, 0, 1 }, { 0, 1, 1 }, { -1, -1, 0 }, { -1, 0, -1 }, { 0, -1, -1 },
+        { -1, 1, 0 }, { 1, -1, 0 }, { 0, -1, 1 }, { 0, 1, -1 },
+    };
+    uint16_t c0 = (uint16_t)(out[0] | (out[1] << 8)), c1 = (uint16_t)(out[2] | (out[3] << 8));
+    uint32_t idx = (uint32_t)out[4] | ((uint32_t)out[5] << 8) | ((uint32_t)out[6] << 16) | ((uint32_t)out[7] << 24);
+    float best = input_error;
+    int last = 0;
+    for (int i = 0; i < 256; i++) {
+        const int *d = deltas[i % 16];
+        uint16_t r0 = c0, r1 = c1;
+        uint16_t *t = ((i / 16) & 1) ? &r0 : &r1;
+        int r = ((*t >> 11) + d[0]) & 31, g = (((*t >> 5) & 63) + d[1]) & 63, b = ((*t & 31) + d[2]) & 31;
+        *t = (uint16_t)((r << 11) | (g << 5) | b);
+        v3 pal[4], rpal[4];
+        palette_float(c0, c1, pal);
+        uint32_t ridx = indices_any(in, cw, pal);
+        palette_float(r0, r1, rpal);
+        float e = block_mse(in, w, cw, rpal, ridx);
+        if (e < best) { best = e; c0 = r0; c1 = r1; idx = ridx; last = i; }
+        if (i - last > 32) break;
+    }
+    put_block(out, c0, c1, idx);
+    return best;
+}
+
+/* icbc.h:3485-3559 compress_dxt1 with three_color_mode and three_color_black
+ * enabled (BlockCompressor.cpp:211-222) at Quality_Level8 (refine = 0, nvtt's
+ * Normal and Highest) or Quality_Level9 (refine = 1, Production). */
+static float bc1_block_level(const float *in, const float *inw, const float *cwp, int refine, uint8_t *out, int *n_out,
+                             int *m_out)
 {
     ntxo_init();
     v3 cw = { cwp[0], cwp[1], cwp[2] };
@@ -553,8 +607,19 @@ float ntxo_bc1_block(const float *in, const float *inw, const float *cwp, uint8_
         if (e3 < best) { best = e3; memcpy(best_blk, blk3, 8); }
     }
     if (best < FLT_MAX) memcpy(out, best_blk, 8);
-    else memset(out, 0, 8); /* the reference leaves the block unwritten here */
+    else { memset(out, 0, 8); return best; } /* the reference leaves the block unwritten here */
+    if (refine) best = refine_endpoints(in, inw, cw, best, out);
     return best;
+}
+
+float ntxo_bc1_block(const float *in, const float *inw, const float *cwp, uint8_t *out, int *n_out, int *m_out)
+{
+    return bc1_block_level(in, inw, cwp, 0, out, n_out, m_out);
+}
+
+float ntxo_bc1_block_q(const float *in, const float *inw, const float *cwp, int quality, uint8_t *out)
+{
+    return bc1_block_level(in, inw, cwp, quality == NTXO_QUALITY_PRODUCTION, out, NULL, NULL);
 }
 
 void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba)
@@ -920,7 +985,7 @@ static void gather_f32(int w, int h, const float *d, int bx, int by, int alpha_m
 }
 
 typedef struct {
-    int format, alpha_mode, w, h, bw;
+    int format, quality, alpha_mode, w, h, bw;
     const float *cw;
     const float *data;
     uint8_t *out;
@@ -936,7 +1001,7 @@ static void do_block(job_t *j, int i)
         float col[64], wt[16];
         int n, m;
         gather_f32(j->w, j->h, j->data, bx, by, j->alpha_mode, col, wt);
-        ntxo_bc1_block(col, wt, j->cw, j->out + (size_t)i * 8, &n, &m);
+        bc1_block_level(col, wt, j->cw, j->quality == NTXO_QUALITY_PRODUCTION, j->out + (size_t)i * 8, &n, &m);
         if (j->nm) { j->nm[2 * i] = (int16_t)n; j->nm[2 * i + 1] = (int16_t)m; }
         return;
     }
@@ -980,10 +1045,20 @@ static void run_job(job_t *j, int threads)
     for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
 }
 
+void ntxo_compress_image_q(int format, int quality, int alpha_mode, const float *cw4, int w, int h, const float *planar,
+                           uint8_t *out, int threads)
+{
+    job_t j;
+    j.format = format; j.quality = quality; j.alpha_mode = alpha_mode; j.w = w; j.h = h; j.bw = (w + 3) / 4;
+    j.cw = cw4; j.data = planar; j.out = out; j.nm = NULL;
+    j.count = j.bw * ((h + 3) / 4);
+    run_job(&j, threads);
+}
+
 void ntxo_compress_image(int format, int alpha_mode, const float *cw4, int w, int h, const float *planar, uint8_t *out, int threads)
 {
     job_t j;
-    j.format = format; j.alpha_mode = alpha_mode; j.w = w; j.h = h; j.bw = (w + 3) / 4;
+    j.format = format; j.quality = NTXO_QUALITY_NORMAL; j.alpha_mode = alpha_mode; j.w = w; j.h = h; j.bw = (w + 3) / 4;
     j.cw = cw4; j.data = planar; j.out = out; j.nm = NULL;
     j.count = j.bw * ((h + 3) / 4);
     run_job(&j, threads);
@@ -992,7 +1067,8 @@ void ntxo_compress_image(int format, int alpha_mode, const float *cw4, int w, in
 void ntxo_compress_bc1_counts(int alpha_mode, const float *cw4, int w, int h, const float *planar, uint8_t *out, int16_t *nm, int threads)
 {
     job_t j;
-    j.format = NTXO_FMT_BC1; j.alpha_mode = alpha_mode; j.w = w; j.h = h; j.bw = (w + 3) / 4;
+    j.format = NTXO_FMT_BC1; j.quality = NTXO_QUALITY_NORMAL; j.alpha_mode = alpha_mode; j.w = w; j.h = h;
+    j.bw = (w + 3) / 4;
     j.cw = cw4; j.data = planar; j.out = out; j.nm = nm;
     j.count = j.bw * ((h + 3) / 4);
     run_job(&j, threads);

@@ -32,6 +32,8 @@ enum { NTXO_FMT_BC1 = 1, NTXO_FMT_BC3 = 4, NTXO_FMT_BC4 = 6, NTXO_FMT_BC5 = 7 };
 enum { NTXO_FILTER_BOX = 0, NTXO_FILTER_TRIANGLE = 1, NTXO_FILTER_KAISER = 2 };
 enum { NTXO_WRAP_CLAMP = 0, NTXO_WRAP_REPEAT = 1, NTXO_WRAP_MIRROR = 2 };
 enum { NTXO_ALPHA_NONE = 0, NTXO_ALPHA_TRANSPARENCY = 1 };
+/* nvtt::Quality values (nvtt.h:148-154). */
+enum { NTXO_QUALITY_FASTEST = 0, NTXO_QUALITY_NORMAL = 1, NTXO_QUALITY_PRODUCTION = 2, NTXO_QUALITY_HIGHEST = 3 };
 
 /* Builds every lookup table (icbc init_dxt1 + initSingleColorLookup). Idempotent. */
 void ntxo_init(void);
@@ -49,6 +51,9 @@ void ntxo_midpoints(float *mid5 /* 32 */, float *mid6 /* 64 */);
  * *m_out = count seen by the three-colour fit (or -1 if it did not run). */
 float ntxo_bc1_block(const float *colors, const float *weights, const float *cw,
                      uint8_t *out, int *n_out, int *m_out);
+/* The same at a quality: Production is icbc Quality_Level9 (Level8 plus
+ * refine_endpoints, icbc.h:3329-3406); every other quality Level8. */
+float ntxo_bc1_block_q(const float *colors, const float *weights, const float *cw, int quality, uint8_t *out);
 /* D3D10 decode of one BC1 block into 16 RGBA8 texels. */
 void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba);
 
@@ -62,6 +67,8 @@ void ntxo_bc3_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, u
 /* ---- whole-image compressors (planar RGBA float input, block-raster output) ---- */
 size_t ntxo_output_size(int format, int w, int h);
 void ntxo_gather_u8(int w, int h, const float *planar, int bx, int by, uint8_t *bgra);
+void ntxo_compress_image_q(int format, int quality, int alpha_mode, const float *cw4, int w, int h,
+                           const float *planar, uint8_t *out, int threads);
 void ntxo_compress_image(int format, int alpha_mode, const float *cw4, int w, int h,
                          const float *planar, uint8_t *out, int threads);
 /* BC1 image compression that also reports n,m per block (flop accounting). */

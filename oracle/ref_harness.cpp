@@ -25,6 +25,15 @@ float ntxref_bc1_block(const float *colors, const float *weights, const float *c
     return icbc::compress_dxt1(icbc::Quality_Level8, colors, weights, cw, true, true, out);
 }
 
+// BlockCompressor.cpp:211-222 -- CompressorDXT1::compressBlock at any nvtt
+// quality (qualityLevel: Fastest -> Quality_Fast, Production ->
+// Quality_Max, otherwise Quality_Default).
+float ntxref_bc1_block_q(const float *colors, const float *weights, const float *cw, int quality, uint8_t *out)
+{
+    const icbc::Quality q = quality == 0 ? icbc::Quality_Fast : quality == 2 ? icbc::Quality_Max : icbc::Quality_Default;
+    return icbc::compress_dxt1(q, colors, weights, cw, true, true, out);
+}
+
 void ntxref_bc1_decode(const uint8_t *blk, uint8_t *rgba)
 {
     icbc::decode_dxt1(blk, rgba, icbc::Decoder_D3D10);

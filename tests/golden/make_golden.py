@@ -12,6 +12,11 @@ chain_c1_64.npz       the 64x64 noise+rectangles texture (seed 7), and icbc's
                       BC1 bytes for every level of its Box chain; the levels
                       are built by the oracle's fastDownSample (FloatImage is
                       not buildable from the reference without its cmake step).
+bc1_level9_blocks.npz adversarial BC1 blocks (another seed) and icbc's output at
+                      Quality_Level9 (nvtt's Quality_Production: the cluster
+                      fit plus refine_endpoints) + returned error.
+
+`python tests/golden/make_golden.py level9` writes only the last file.
 """
 import os
 import sys
@@ -46,6 +51,20 @@ def gather_bc1(planar, w, h):
                         cols[i, 4 * y + x] = p[:, Y, X]
                         wts[i, 4 * y + x] = 1.0
     return cols, wts
+
+
+def level9(ref):
+    cols, wts, kinds = blocks.bc1_set(seed=2026, count=2048)
+    cws = np.ones((len(cols), 3), np.float32)
+    cws[2::9] = np.array([0.2126, 0.7152, 0.0722], np.float32)
+    out = np.zeros((len(cols), 8), np.uint8)
+    err = np.zeros(len(cols), np.float32)
+    for i in range(len(cols)):
+        out[i], err[i] = ref.bc1_block_q(cols[i], wts[i], 2, cws[i])
+    path = os.path.join(HERE, "bc1_level9_blocks.npz")
+    np.savez_compressed(path, colors=cols, weights=wts, color_weights=cws, kinds=kinds, expected=out, error=err,
+                        meta=META + "; icbc::compress_dxt1(Quality_Max)")
+    print("bc1_level9_blocks.npz", os.path.getsize(path), "bytes")
 
 
 def main():
@@ -92,4 +111,11 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["level9"]:
+        r = oracle_lib.Reference.load()
+        if r is None:
+            raise SystemExit("oracle/_ref/libntxref.so missing")
+        level9(r)
+    else:
+        main()
+        level9(oracle_lib.Reference.load())

@@ -33,6 +33,9 @@ class Oracle:
         L.ntxo_downsample.argtypes = [ctypes.c_int, ctypes.c_float, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]
         L.ntxo_build_next_mipmap.argtypes = L.ntxo_downsample.argtypes
         L.ntxo_srgb_to_linear.argtypes = [ctypes.c_size_t, _P]
+        L.ntxo_bc1_block_q.restype = ctypes.c_float
+        L.ntxo_compress_image_q.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P,
+                                            _P, ctypes.c_int]
         L.ntxo_gamma_table.argtypes = [ctypes.c_int, _P]
         L.ntxo_powf_5_11.argtypes = [ctypes.c_float]
         L.ntxo_powf_5_11.restype = ctypes.c_float
@@ -70,6 +73,19 @@ class Oracle:
         out = np.zeros(8, np.uint8); n = ctypes.c_int(); m = ctypes.c_int()
         e = self.lib.ntxo_bc1_block(_p(c), _p(w), _p(cwv), _p(out), ctypes.byref(n), ctypes.byref(m))
         return out, float(e), n.value, m.value
+
+    def bc1_block_q(self, colors, weights, quality, cw=(1, 1, 1)):
+        c = f32(colors).reshape(16, 4); w = f32(weights).reshape(16); cwv = f32(cw)
+        out = np.zeros(8, np.uint8)
+        e = self.lib.ntxo_bc1_block_q(_p(c), _p(w), _p(cwv), int(quality), _p(out))
+        return out, float(e)
+
+    def compress_q(self, fmt, quality, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1), threads=8):
+        src = f32(planar).reshape(-1); cwv = f32(cw)
+        out = np.zeros(self.lib.ntxo_output_size(int(fmt), int(w), int(h)), np.uint8)
+        self.lib.ntxo_compress_image_q(int(fmt), int(quality), int(alpha_mode), _p(cwv), int(w), int(h), _p(src), _p(out),
+                                       int(threads))
+        return out
 
     def bc1_decode(self, blk):
         blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
@@ -165,7 +181,14 @@ class Reference:
     def __init__(self, path):
         L = self.lib = ctypes.CDLL(path)
         L.ntxref_bc1_block.restype = ctypes.c_float
+        L.ntxref_bc1_block_q.restype = ctypes.c_float
         L.ntxref_init()
+
+    def bc1_block_q(self, colors, weights, quality, cw=(1, 1, 1)):
+        c = f32(colors).reshape(16, 4); w = f32(weights).reshape(16); cwv = f32(cw)
+        out = np.zeros(8, np.uint8)
+        e = self.lib.ntxref_bc1_block_q(_p(c), _p(w), _p(cwv), int(quality), _p(out))
+        return out, float(e)
 
     def bc1_block(self, colors, weights, cw=(1, 1, 1)):
         c = f32(colors).reshape(16, 4); w = f32(weights).reshape(16); cwv = f32(cw)

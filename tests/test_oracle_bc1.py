@@ -33,6 +33,46 @@ def test_bc1_against_compiled_reference(oracle, ref):
         assert np.array_equal(a, b) and ea == eb, (i, a, b)
 
 
+def test_bc1_level9_golden_vectors(oracle):
+    """Quality_Production = icbc Quality_Level9 (cluster fit + refine_endpoints):
+    the golden vectors written by the reference's icbc."""
+    g = np.load(os.path.join(GOLD, "bc1_level9_blocks.npz"))
+    bad = []
+    for i in range(len(g["expected"])):
+        out, err = oracle.bc1_block_q(g["colors"][i], g["weights"][i], 2, g["color_weights"][i])
+        if not np.array_equal(out, g["expected"][i]) or np.float32(err) != g["error"][i]:
+            bad.append(i)
+    assert not bad, f"{len(bad)} golden Level9 blocks differ, first {bad[:5]}"
+
+
+def test_bc1_level9_against_compiled_reference(oracle, ref):
+    cols, wts, _ = blocks.bc1_set(seed=77, count=2000)
+    for i in range(len(cols)):
+        cw = (1, 1, 1) if i % 4 else (0.3, 0.59, 0.11)
+        a, ea = oracle.bc1_block_q(cols[i], wts[i], 2, cw)
+        b, eb = ref.bc1_block_q(cols[i], wts[i], 2, cw)
+        assert np.array_equal(a, b) and ea == eb, (i, a, b)
+
+
+def test_bc1_level9_refines_level8(oracle):
+    """refine_endpoints only accepts a lower evaluate_mse, so Level9's error never
+    exceeds Level8's, it returns the decoded MSE of its block, and it moves
+    a good share of the blocks of a real texture (a dropped step or a wrong
+    index rule would change these)."""
+    cols, wts, _ = blocks.bc1_set(seed=5, count=1500)
+    moved = 0
+    for i in range(len(cols)):
+        b8, e8, _, _ = oracle.bc1_block(cols[i], wts[i])
+        b9, e9 = oracle.bc1_block_q(cols[i], wts[i], 2)
+        assert e9 <= e8
+        moved += int(not np.array_equal(b8, b9))
+        dec = oracle.bc1_decode(b9).reshape(16, 4)[:, :3].astype(np.float64)
+        src = np.asarray(cols[i], np.float32).reshape(16, 4)[:, :3].astype(np.float64) * 255.0
+        mse = float(np.sum(np.asarray(wts[i], np.float64)[:, None] * (dec - src) ** 2))
+        assert mse == pytest.approx(e9, rel=1e-4, abs=1e-3)
+    assert moved > 100
+
+
 def _reference_table_procedure(n_levels, three):
     """The construction loop of init_cluster_tables (icbc.h:1906-1975): at each
     level, add every cumulative partition not present in the previous levels."""
