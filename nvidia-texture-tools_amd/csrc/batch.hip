@@ -104,7 +104,7 @@ __device__ void bc1_item(const BatchArgs &a, const BatchItem &it, const BatchLev
         const int blk = it.first + k;
         int n, m;
         const uint2 r = ntx_bc1::encode_block(sh, data, lv.w, lv.h, blk % bw, blk / bw, lv.alpha_mode, lv.cw[0],
-                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m, lv.refine != 0);
+                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m, lv.bc1_level);
         if (lane == 0) out[blk] = r;
     }
 }

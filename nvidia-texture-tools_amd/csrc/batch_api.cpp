@@ -295,7 +295,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             lv.h = h;
             lv.format = d.format;
             lv.alpha_mode = d.alpha_mode;
-            lv.refine = (d.format == NTX_FORMAT_BC1 && d.quality == NTX_QUALITY_PRODUCTION) ? 1 : 0;
+            lv.bc1_level = ntx::bc1_level(d.quality);
             const bool zero_w = d.color_weights[0] == 0 && d.color_weights[1] == 0 && d.color_weights[2] == 0 &&
                                 d.color_weights[3] == 0;
             for (int k = 0; k < 3; k++) lv.cw[k] = zero_w ? 1.0f : d.color_weights[k];

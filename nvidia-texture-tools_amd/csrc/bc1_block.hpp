@@ -1,8 +1,8 @@
 // bc1_block.hpp -- device code of the BC1 encoder shared by the per-image
 // kernel (bc1.hip) and the batch kernel (batch.hip): icbc v1.05
-// compress_dxt1 at Quality_Level8 (and Level9: the same plus refine_endpoints),
-// the SSE4.1 build (no FMA, round-half-even, 4-lane tie-break),
-// icbc.h:3485-3559.
+// compress_dxt1 at Quality_Level8 (and Level9: the same plus refine_endpoints;
+// Level1: box fit + least squares), the SSE4.1 build (no FMA,
+// round-half-even, 4-lane tie-break), icbc.h:3485-3559.
 //
 // One 4x4 block per 16-lane group.  Lane l of a group owns texel l (gather,
 // index selection, error terms) and, after colour reduction, cluster l.  The
@@ -485,6 +485,74 @@ __device__ __forceinline__ uint2 single_color_block(const ntx::Tables *tab, floa
     return make_uint2(c0 | (c1 << 16), idx);
 }
 
+// The icbc scalar helpers (icbc.h:183-193): max(a, b) = (b < a) ? a : b,
+// min(a, b) = (a < b) ? a : b, saturate = min(max(x, 0), 1).
+__device__ __forceinline__ float icbc_max(float a, float b) { return (b < a) ? a : b; }
+__device__ __forceinline__ float icbc_min(float a, float b) { return (a < b) ? a : b; }
+__device__ __forceinline__ float icbc_saturate(float x) { return icbc_min(icbc_max(x, 0.0f), 1.0f); }
+
+// Sum over the group's 16 lanes in lane order (the reference's sequential loops).
+__device__ __forceinline__ float group_sum(float v)
+{
+    float s = 0.0f;
+    static_for<0, 16>([&](auto k) { s += gbcast<decltype(k)::value>(v); });
+    return s;
+}
+
+// compress_dxt1 at Quality_Level1, nvtt's Quality_Fastest (icbc.h:3485-3540
+// with box_fit and least_squares_fit and no cluster fit): all 16 texels
+// (those outside the image as black), fit_colors_bbox + inset_bbox +
+// select_diagonal (icbc.h:3096-3153), output_block4, then
+// optimize_end_points4 on those indices (icbc.h:2984-3018) and output_block4
+// again, keeping the lower error.
+__device__ __forceinline__ uint2 encode_box_fit(float tr, float tg, float tb, float tw, float cwx, float cwy, float cwz,
+                                                const float *mid5, const float *mid6, const float *div255, int lane)
+{
+    // fit_colors_bbox: c0 = running max from 0, c1 = running min from 1, in texel order.
+    float x0 = 0.0f, y0 = 0.0f, z0 = 0.0f, x1 = 1.0f, y1 = 1.0f, z1 = 1.0f;
+    static_for<0, 16>([&](auto k) {
+        constexpr int i = decltype(k)::value;
+        const float r = gbcast<i>(tr), g = gbcast<i>(tg), b = gbcast<i>(tb);
+        x0 = icbc_max(x0, r); y0 = icbc_max(y0, g); z0 = icbc_max(z0, b);
+        x1 = icbc_min(x1, r); y1 = icbc_min(y1, g); z1 = icbc_min(z1, b);
+    });
+    // inset_bbox.
+    const float bias = (8.0f / 255.0f) / 16.0f;
+    const float ix = (x0 - x1) / 16.0f - bias, iy = (y0 - y1) / 16.0f - bias, iz = (z0 - z1) / 16.0f - bias;
+    x0 = icbc_saturate(x0 - ix); y0 = icbc_saturate(y0 - iy); z0 = icbc_saturate(z0 - iz);
+    x1 = icbc_saturate(x1 + ix); y1 = icbc_saturate(y1 + iy); z1 = icbc_saturate(z1 + iz);
+    // select_diagonal: signs of the x-z and y-z covariances about the box centre.
+    const float cx = (x0 + x1) * 0.5f, cy = (y0 + y1) * 0.5f, cz = (z0 + z1) * 0.5f;
+    const float tx = tr - cx, ty = tg - cy, tz = tb - cz;
+    const float cov_xz = group_sum(tx * tz), cov_yz = group_sum(ty * tz);
+    if (cov_xz < 0.0f) { const float t = x0; x0 = x1; x1 = t; }
+    if (cov_yz < 0.0f) { const float t = y0; y0 = y1; y1 = t; }
+    Cand c = { x0, y0, z0, x1, y1, z1 };
+    uint2 blk;
+    float err = output_block<true>(c, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, div255, lane, &blk);
+    // optimize_end_points4 (no weights): index bits -> beta in {0, 1, 1/3, 2/3}.
+    const uint32_t bits = (blk.y >> (2 * lane)) & 3u;
+    float beta = float(bits & 1u);
+    if (bits & 2u) beta = (1.0f + beta) / 3.0f;
+    const float alpha = 1.0f - beta;
+    const float a2 = group_sum(alpha * alpha), b2 = group_sum(beta * beta), ab = group_sum(alpha * beta);
+    const float axr = group_sum(alpha * tr), axg = group_sum(alpha * tg), axb = group_sum(alpha * tb);
+    const float bxr = group_sum(beta * tr), bxg = group_sum(beta * tg), bxb = group_sum(beta * tb);
+    const float denom = a2 * b2 - ab * ab;
+    if (fabsf(denom - 0.0f) < 0.0001f) return blk;  // icbc equal(denom, 0)
+    const float factor = 1.0f / denom;
+    Cand o;
+    o.ax = icbc_saturate((axr * b2 - bxr * ab) * factor);
+    o.ay = icbc_saturate((axg * b2 - bxg * ab) * factor);
+    o.az = icbc_saturate((axb * b2 - bxb * ab) * factor);
+    o.bx = icbc_saturate((bxr * a2 - axr * ab) * factor);
+    o.by = icbc_saturate((bxg * a2 - axg * ab) * factor);
+    o.bz = icbc_saturate((bxb * a2 - axb * ab) * factor);
+    uint2 blk2;
+    const float err2 = output_block<true>(o, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, div255, lane, &blk2);
+    return err2 < err ? blk2 : blk;
+}
+
 // Encodes block (bx, by) of the w x h planar RGBA image for this group;
 // every lane of the group returns the 8-byte block, the colour count n
 // after reduce_colors and the count m seen by the three-colour fit (-1 if
@@ -492,7 +560,7 @@ __device__ __forceinline__ uint2 single_color_block(const ntx::Tables *tab, floa
 __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, int w, int h, int bx, int by,
                                               int alpha_mode, float cwx, float cwy, float cwz,
                                               const ntx::Tables *tab, int g, int lane, int *n_out, int *m_out,
-                                              bool refine = false)
+                                              int level = 8)
 {
     // Gather (BlockCompressor.cpp:116-166): texel l; outside the image -> colour 0, weight 0.
     const int x = bx * 4 + (lane & 3), y = by * 4 + (lane >> 2);
@@ -510,6 +578,12 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
         } else {
             tw = 1.0f;
         }
+    }
+
+    if (level == 1) {  // group-uniform
+        *n_out = 16;
+        *m_out = -1;
+        return encode_box_fit(tr, tg, tb, tw, cwx, cwy, cwz, tab->mid5, tab->mid6, sh.div255, lane);
     }
 
     // reduce_colors (icbc.h:1638-1683): lane j holds cluster j.
@@ -588,7 +662,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
         }
         if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
         // Quality_Level9 (icbc.h:3551-3553): the cluster fit's block refined.
-        else if (refine) result = refine_endpoints(result, best, tr, tg, tb, tw, cwx, cwy, cwz, sh.div255, lane);
+        else if (level == 9) result = refine_endpoints(result, best, tr, tg, tb, tw, cwx, cwy, cwz, sh.div255, lane);
     }
     *n_out = n;
     *m_out = m_count;

@@ -88,13 +88,20 @@ ntx_status bind(ntx_hip_context *ctx)
 
 // Quality levels handled exactly by the kernels (qualityLevel in
 // BlockCompressor.cpp:202-209; chooseCpuCompressor, Context.cpp:1038-1163).
+// BlockCompressor.cpp:211-217 qualityLevel: Fastest -> icbc Level1 (box fit +
+// least squares), Production -> Level9 (cluster fit + refine_endpoints),
+// Normal and Highest -> Level8.
+int bc1_level(int quality)
+{
+    return quality == NTX_QUALITY_FASTEST ? 1 : quality == NTX_QUALITY_PRODUCTION ? 9 : 8;
+}
+
 bool supported(int format, int quality)
 {
     switch (format) {
-    // BlockCompressor.cpp:211-217: Normal and Highest are icbc Level8,
-    // Production Level9 (Level8 + endpoint refinement).
+    // BlockCompressor.cpp:211-217: every quality maps to an icbc level.
     case NTX_FORMAT_BC1:
-        return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_HIGHEST || quality == NTX_QUALITY_PRODUCTION;
+        return quality >= NTX_QUALITY_FASTEST && quality <= NTX_QUALITY_HIGHEST;
     case NTX_FORMAT_BC3: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
     case NTX_FORMAT_BC4:
     case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
@@ -391,9 +398,9 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
 {
     hipError_t e = hipSuccess;
     if (format == NTX_FORMAT_BC1) {
-        const bool refine = quality == NTX_QUALITY_PRODUCTION;
         e = timed_on(ctx, stream, NTX_KERNEL_BC1, [&] {
-            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent, refine);
+            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent,
+                              bc1_level(quality));
         });
     } else if (format == NTX_FORMAT_BC3) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);

@@ -27,7 +27,7 @@ struct BatchLevel {
     int format;         // NTX_FORMAT_BC1 / BC4 / BC5
     int alpha_mode;
     float cw[3];        // colour weights (BC1)
-    int refine;         // BC1 at Quality_Production: icbc's endpoint refinement
+    int bc1_level;      // BC1: icbc quality level (1, 8 or 9)
     int ready;          // counter that completes `data` (-1: complete before launch)
     int target;         // its value once every item writing `data` has finished
     int next;           // index of the next level, -1 for the last one

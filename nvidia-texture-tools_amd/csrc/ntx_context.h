@@ -12,7 +12,7 @@
 
 namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
-                      int16_t *d_nm, hipStream_t stream, bool urgent = false, bool refine = false);
+                      int16_t *d_nm, hipStream_t stream, bool urgent = false, int level = 8);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
                            bool urgent = false);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
@@ -117,6 +117,8 @@ ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes);
 hipEvent_t take_event(ntx_hip_context *ctx);
 // Quality levels handled exactly by the kernels.
 bool supported(int format, int quality);
+// icbc quality level of a BC1 nvtt quality (1, 8 or 9).
+int bc1_level(int quality);
 // Polyphase weights and first-tap indices for src -> dst samples, cached per context.
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out);
 

@@ -695,10 +695,8 @@ struct Compressor::Private {
     CompressorInterface *chooseCompressor(const CompressionOptions::Private &co, bool device_input) const
     {
         switch (co.format) {
-        case Format_BC1:  // BlockCompressor.cpp:211-217: Production is icbc Level9, Normal/Highest Level8
-            if (co.quality == Quality_Normal || co.quality == Quality_Highest || co.quality == Quality_Production)
-                return new HipCompressorBCn(device_input);
-            break;
+        case Format_BC1:  // BlockCompressor.cpp:211-217: icbc Level1 / 8 / 9 by quality
+            return new HipCompressorBCn(device_input);
         case Format_BC3:
             if (co.quality == Quality_Normal || co.quality == Quality_Production) return new HipCompressorBCn(device_input);
             break;

@@ -562,14 +562,80 @@ static float refine_endpoints(const float *in, const float *w, v3 cw, float inpu
     return best;
 }
 
+/* BlockCompressor.cpp:211-217 qualityLevel: Fastest -> Level1, Production ->
+ * Level9, otherwise Level8. */
+static int bc1_level(int quality)
+{
+    return quality == NTXO_QUALITY_FASTEST ? 1 : quality == NTXO_QUALITY_PRODUCTION ? 9 : 8;
+}
+
+/* icbc.h:191-193, 291-293 clamp / saturate on the scalar path. */
+static inline float isat(float x) { return imin(imax(x, 0.0f), 1.0f); }
+
+/* compress_dxt1 at Quality_Level1 (icbc.h:3485-3540, box_fit and
+ * least_squares_fit): fit_colors_bbox, inset_bbox, select_diagonal
+ * (icbc.h:3096-3153), output_block4, optimize_end_points4 (icbc.h:2984-3018)
+ * and output_block4 again, keeping the lower error.  All 16 input colours. */
+static float bc1_box_fit(const float *in, const float *inw, v3 cw, uint8_t *out)
+{
+    v3 c0 = { 0, 0, 0 }, c1 = { 1, 1, 1 };
+    for (int i = 0; i < 16; i++) {
+        c0.x = imax(c0.x, in[4 * i]); c0.y = imax(c0.y, in[4 * i + 1]); c0.z = imax(c0.z, in[4 * i + 2]);
+        c1.x = imin(c1.x, in[4 * i]); c1.y = imin(c1.y, in[4 * i + 1]); c1.z = imin(c1.z, in[4 * i + 2]);
+    }
+    const float bias = (8.0f / 255.0f) / 16.0f;
+    v3 ins = { (c0.x - c1.x) / 16.0f - bias, (c0.y - c1.y) / 16.0f - bias, (c0.z - c1.z) / 16.0f - bias };
+    c0.x = isat(c0.x - ins.x); c0.y = isat(c0.y - ins.y); c0.z = isat(c0.z - ins.z);
+    c1.x = isat(c1.x + ins.x); c1.y = isat(c1.y + ins.y); c1.z = isat(c1.z + ins.z);
+    v3 ctr = { (c0.x + c1.x) * 0.5f, (c0.y + c1.y) * 0.5f, (c0.z + c1.z) * 0.5f };
+    float cov_xz = 0.0f, cov_yz = 0.0f;
+    for (int i = 0; i < 16; i++) {
+        v3 t = { in[4 * i] - ctr.x, in[4 * i + 1] - ctr.y, in[4 * i + 2] - ctr.z };
+        cov_xz += t.x * t.z;
+        cov_yz += t.y * t.z;
+    }
+    if (cov_xz < 0) { float t = c0.x; c0.x = c1.x; c1.x = t; }
+    if (cov_yz < 0) { float t = c0.y; c0.y = c1.y; c1.y = t; }
+    float err = output_block(1, in, inw, cw, c0, c1, out);
+    uint32_t idx = (uint32_t)out[4] | ((uint32_t)out[5] << 8) | ((uint32_t)out[6] << 16) | ((uint32_t)out[7] << 24);
+    float a2 = 0.0f, b2 = 0.0f, ab = 0.0f;
+    v3 ax = { 0, 0, 0 }, bx = { 0, 0, 0 };
+    for (int i = 0; i < 16; i++) {
+        const uint32_t bits = idx >> (2 * i);
+        float beta = (float)(bits & 1);
+        if (bits & 2) beta = (1 + beta) / 3.0f;
+        float alpha = 1.0f - beta;
+        a2 += alpha * alpha; b2 += beta * beta; ab += alpha * beta;
+        ax.x += alpha * in[4 * i]; ax.y += alpha * in[4 * i + 1]; ax.z += alpha * in[4 * i + 2];
+        bx.x += beta * in[4 * i]; bx.y += beta * in[4 * i + 1]; bx.z += beta * in[4 * i + 2];
+    }
+    float denom = a2 * b2 - ab * ab;
+    if (fabsf(denom - 0.0f) < 0.0001f) return err; /* icbc equal(denom, 0) */
+    float factor = 1.0f / denom;
+    v3 a = { isat((ax.x * b2 - bx.x * ab) * factor), isat((ax.y * b2 - bx.y * ab) * factor),
+             isat((ax.z * b2 - bx.z * ab) * factor) };
+    v3 b = { isat((bx.x * a2 - ax.x * ab) * factor), isat((bx.y * a2 - ax.y * ab) * factor),
+             isat((bx.z * a2 - ax.z * ab) * factor) };
+    uint8_t blk2[8];
+    float err2 = output_block(1, in, inw, cw, a, b, blk2);
+    if (err2 < err) { err = err2; memcpy(out, blk2, 8); }
+    return err;
+}
+
 /* icbc.h:3485-3559 compress_dxt1 with three_color_mode and three_color_black
- * enabled (BlockCompressor.cpp:211-222) at Quality_Level8 (refine = 0, nvtt's
- * Normal and Highest) or Quality_Level9 (refine = 1, Production). */
-static float bc1_block_level(const float *in, const float *inw, const float *cwp, int refine, uint8_t *out, int *n_out,
+ * enabled (BlockCompressor.cpp:211-222) at Quality_Level8 (level 8, nvtt's
+ * Normal and Highest), Quality_Level9 (level 9, Production) or
+ * Quality_Level1 (level 1, Fastest). */
+static float bc1_block_level(const float *in, const float *inw, const float *cwp, int level, uint8_t *out, int *n_out,
                              int *m_out)
 {
     ntxo_init();
     v3 cw = { cwp[0], cwp[1], cwp[2] };
+    if (level == 1) {
+        if (n_out) *n_out = 16;
+        if (m_out) *m_out = -1;
+        return bc1_box_fit(in, inw, cw, out);
+    }
     v3 col[16];
     float w[16];
     int any_black = 0;
@@ -608,18 +674,18 @@ static float bc1_block_level(const float *in, const float *inw, const float *cwp
     }
     if (best < FLT_MAX) memcpy(out, best_blk, 8);
     else { memset(out, 0, 8); return best; } /* the reference leaves the block unwritten here */
-    if (refine) best = refine_endpoints(in, inw, cw, best, out);
+    if (level == 9) best = refine_endpoints(in, inw, cw, best, out);
     return best;
 }
 
 float ntxo_bc1_block(const float *in, const float *inw, const float *cwp, uint8_t *out, int *n_out, int *m_out)
 {
-    return bc1_block_level(in, inw, cwp, 0, out, n_out, m_out);
+    return bc1_block_level(in, inw, cwp, 8, out, n_out, m_out);
 }
 
 float ntxo_bc1_block_q(const float *in, const float *inw, const float *cwp, int quality, uint8_t *out)
 {
-    return bc1_block_level(in, inw, cwp, quality == NTXO_QUALITY_PRODUCTION, out, NULL, NULL);
+    return bc1_block_level(in, inw, cwp, bc1_level(quality), out, NULL, NULL);
 }
 
 void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba)
@@ -1001,7 +1067,7 @@ static void do_block(job_t *j, int i)
         float col[64], wt[16];
         int n, m;
         gather_f32(j->w, j->h, j->data, bx, by, j->alpha_mode, col, wt);
-        bc1_block_level(col, wt, j->cw, j->quality == NTXO_QUALITY_PRODUCTION, j->out + (size_t)i * 8, &n, &m);
+        bc1_block_level(col, wt, j->cw, bc1_level(j->quality), j->out + (size_t)i * 8, &n, &m);
         if (j->nm) { j->nm[2 * i] = (int16_t)n; j->nm[2 * i + 1] = (int16_t)m; }
         return;
     }

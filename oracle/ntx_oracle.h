@@ -52,7 +52,8 @@ void ntxo_midpoints(float *mid5 /* 32 */, float *mid6 /* 64 */);
 float ntxo_bc1_block(const float *colors, const float *weights, const float *cw,
                      uint8_t *out, int *n_out, int *m_out);
 /* The same at a quality: Production is icbc Quality_Level9 (Level8 plus
- * refine_endpoints, icbc.h:3329-3406); every other quality Level8. */
+ * refine_endpoints, icbc.h:3329-3406), Fastest Quality_Level1 (box fit +
+ * least squares), Normal and Highest Level8. */
 float ntxo_bc1_block_q(const float *colors, const float *weights, const float *cw, int quality, uint8_t *out);
 /* D3D10 decode of one BC1 block into 16 RGBA8 texels. */
 void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba);

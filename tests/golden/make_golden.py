@@ -15,8 +15,10 @@ chain_c1_64.npz       the 64x64 noise+rectangles texture (seed 7), and icbc's
 bc1_level9_blocks.npz adversarial BC1 blocks (another seed) and icbc's output at
                       Quality_Level9 (nvtt's Quality_Production: the cluster
                       fit plus refine_endpoints) + returned error.
+bc1_level1_blocks.npz the same for Quality_Level1 (nvtt's Quality_Fastest: box
+                      fit + least squares).
 
-`python tests/golden/make_golden.py level9` writes only the last file.
+`python tests/golden/make_golden.py levels` writes only the last two files.
 """
 import os
 import sys
@@ -53,18 +55,20 @@ def gather_bc1(planar, w, h):
     return cols, wts
 
 
-def level9(ref):
-    cols, wts, kinds = blocks.bc1_set(seed=2026, count=2048)
-    cws = np.ones((len(cols), 3), np.float32)
-    cws[2::9] = np.array([0.2126, 0.7152, 0.0722], np.float32)
-    out = np.zeros((len(cols), 8), np.uint8)
-    err = np.zeros(len(cols), np.float32)
-    for i in range(len(cols)):
-        out[i], err[i] = ref.bc1_block_q(cols[i], wts[i], 2, cws[i])
-    path = os.path.join(HERE, "bc1_level9_blocks.npz")
-    np.savez_compressed(path, colors=cols, weights=wts, color_weights=cws, kinds=kinds, expected=out, error=err,
-                        meta=META + "; icbc::compress_dxt1(Quality_Max)")
-    print("bc1_level9_blocks.npz", os.path.getsize(path), "bytes")
+def quality_levels(ref):
+    for name, quality, seed, level in (("bc1_level9_blocks.npz", 2, 2026, "Quality_Max"),
+                                       ("bc1_level1_blocks.npz", 0, 2027, "Quality_Fast")):
+        cols, wts, kinds = blocks.bc1_set(seed=seed, count=2048)
+        cws = np.ones((len(cols), 3), np.float32)
+        cws[2::9] = np.array([0.2126, 0.7152, 0.0722], np.float32)
+        out = np.zeros((len(cols), 8), np.uint8)
+        err = np.zeros(len(cols), np.float32)
+        for i in range(len(cols)):
+            out[i], err[i] = ref.bc1_block_q(cols[i], wts[i], quality, cws[i])
+        path = os.path.join(HERE, name)
+        np.savez_compressed(path, colors=cols, weights=wts, color_weights=cws, kinds=kinds, expected=out, error=err,
+                            meta=META + f"; icbc::compress_dxt1({level})")
+        print(name, os.path.getsize(path), "bytes")
 
 
 def main():
@@ -111,11 +115,11 @@ def main():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["level9"]:
+    if sys.argv[1:] == ["levels"]:
         r = oracle_lib.Reference.load()
         if r is None:
             raise SystemExit("oracle/_ref/libntxref.so missing")
-        level9(r)
+        quality_levels(r)
     else:
         main()
-        level9(oracle_lib.Reference.load())
+        quality_levels(oracle_lib.Reference.load())

@@ -151,7 +151,7 @@ def test_batch_rejects_bad_input(gpu):
         N.Batch(gpu, [dict(base, mip_levels=5)])  # 8x8 has 4 levels
     assert e.value.status == 1
     with pytest.raises(N.NtxError) as e:
-        N.Batch(gpu, [dict(base, quality=N.QUALITY_FASTEST)])
+        N.Batch(gpu, [dict(base, format=N.FORMAT_BC4, quality=N.QUALITY_PRODUCTION)])
     assert e.value.status == 2
     with N.Batch(gpu, [base]) as b:
         with pytest.raises(N.NtxError):

@@ -1,7 +1,8 @@
-"""BC1 at Quality_Production: icbc's Quality_Level9 (the Level8 cluster fit
-followed by refine_endpoints, icbc.h:3329-3406), on the device, against the
-golden vectors written by the reference's own icbc and against the oracle
-for images, adversarial blocks, chains, image sets and batches."""
+"""BC1 at Quality_Production (icbc's Quality_Level9: the Level8 cluster fit
+followed by refine_endpoints, icbc.h:3329-3406) and Quality_Fastest
+(Quality_Level1: box fit + least squares, icbc.h:2984-3153), on the device,
+against the golden vectors written by the reference's own icbc and against
+the oracle for images, adversarial blocks, chains, image sets and batches."""
 import os
 
 import numpy as np
@@ -13,7 +14,7 @@ from ntx_amd import synth
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-PROD = N.QUALITY_PRODUCTION
+PROD, FAST = N.QUALITY_PRODUCTION, N.QUALITY_FASTEST
 
 
 def _blocks_as_image(cols, wts):
@@ -29,21 +30,23 @@ def _blocks_as_image(cols, wts):
     return img.reshape(-1), w, h
 
 
-def test_level9_golden_blocks(gpu):
-    g = np.load(os.path.join(GOLD, "bc1_level9_blocks.npz"))
+@pytest.mark.parametrize("name,quality", [("bc1_level9_blocks.npz", PROD), ("bc1_level1_blocks.npz", FAST)])
+def test_quality_golden_blocks(gpu, name, quality):
+    g = np.load(os.path.join(GOLD, name))
     cws = g["color_weights"]
     bad = 0
     for key in {tuple(c) for c in cws}:
         sel = np.nonzero(np.all(cws == np.array(key, np.float32), axis=1))[0]
         img, w, h = _blocks_as_image(g["colors"][sel], g["weights"][sel])
-        out = gpu.compress(N.FORMAT_BC1, img, w, h, quality=PROD, alpha_mode=N.ALPHA_TRANSPARENCY,
+        out = gpu.compress(N.FORMAT_BC1, img, w, h, quality=quality, alpha_mode=N.ALPHA_TRANSPARENCY,
                            color_weights=(key[0], key[1], key[2], 1.0)).reshape(-1, 8)
         bad += int(np.sum(np.any(out != g["expected"][sel], axis=1)))
-    assert bad == 0, f"{bad} of {len(cws)} golden Level9 blocks differ"
+    assert bad == 0, f"{bad} of {len(cws)} golden blocks of {name} differ"
 
 
+@pytest.mark.parametrize("quality", [PROD, FAST])
 @pytest.mark.parametrize("kind", range(len(blocks.KINDS)))
-def test_level9_adversarial_blocks(gpu, oracle, kind):
+def test_quality_adversarial_blocks(gpu, oracle, kind, quality):
     rng = np.random.default_rng(300 + kind)
     cols = np.zeros((256, 16, 4), np.float32)
     wts = np.zeros((256, 16), np.float32)
@@ -51,26 +54,28 @@ def test_level9_adversarial_blocks(gpu, oracle, kind):
         cols[i], wts[i] = blocks.bc1_block(rng, blocks.KINDS[kind])
     img, w, h = _blocks_as_image(cols, wts)
     for cw in ((1, 1, 1, 1), (0.5, 1.0, 0.25, 1.0)):
-        a = gpu.compress(N.FORMAT_BC1, img, w, h, quality=PROD, alpha_mode=N.ALPHA_TRANSPARENCY, color_weights=cw)
-        b = oracle.compress_q(N.FORMAT_BC1, PROD, img, w, h, alpha_mode=N.ALPHA_TRANSPARENCY, cw=cw)
+        a = gpu.compress(N.FORMAT_BC1, img, w, h, quality=quality, alpha_mode=N.ALPHA_TRANSPARENCY, color_weights=cw)
+        b = oracle.compress_q(N.FORMAT_BC1, quality, img, w, h, alpha_mode=N.ALPHA_TRANSPARENCY, cw=cw)
         assert np.array_equal(a, b), (blocks.KINDS[kind], cw)
 
 
 @pytest.mark.parametrize("size", [(256, 192), (37, 23), (5, 3), (1, 1)])
-def test_level9_images_match_oracle(gpu, oracle, size):
+def test_quality_images_match_oracle(gpu, oracle, size):
     w, h = size
     planar = synth.to_planar(synth.noise_rect_rgba(w, h, seed=w + h, transparent=True)).reshape(-1)
-    for am in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
-        a = gpu.compress(N.FORMAT_BC1, planar, w, h, quality=PROD, alpha_mode=am)
-        b = oracle.compress_q(N.FORMAT_BC1, PROD, planar, w, h, alpha_mode=am)
-        assert np.array_equal(a, b), (size, am)
+    for q in (PROD, FAST):
+        for am in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
+            a = gpu.compress(N.FORMAT_BC1, planar, w, h, quality=q, alpha_mode=am)
+            b = oracle.compress_q(N.FORMAT_BC1, q, planar, w, h, alpha_mode=am)
+            assert np.array_equal(a, b), (size, q, am)
     # Normal and Highest stay Level8
     for q in (N.QUALITY_NORMAL, N.QUALITY_HIGHEST):
         assert np.array_equal(gpu.compress(N.FORMAT_BC1, planar, w, h, quality=q),
                               oracle.compress(N.FORMAT_BC1, planar, w, h))
 
 
-def test_level9_chain_images_and_batch(gpu, oracle):
+@pytest.mark.parametrize("quality", [PROD, FAST])
+def test_quality_chain_images_and_batch(gpu, oracle, quality):
     w, h = 96, 80
     img = synth.noise_rect_rgba(w, h, seed=12)
     planar = synth.to_planar(img).reshape(-1)
@@ -78,18 +83,18 @@ def test_level9_chain_images_and_batch(gpu, oracle):
     d_lv = [gpu.alloc(16 * a * b) for a, b in sizes]
     d_lv[0].upload(planar)
     d_out = [gpu.alloc(N.compressed_size(N.FORMAT_BC1, a, b)) for a, b in sizes]
-    gpu.compress_chain_device(N.FORMAT_BC1, N.FILTER_BOX, d_lv, w, h, d_out, quality=PROD)
+    gpu.compress_chain_device(N.FORMAT_BC1, N.FILTER_BOX, d_lv, w, h, d_out, quality=quality)
     levels = [d.download(np.float32) for d in d_lv]
     chain = [d.download(np.uint8) for d in d_out]
     for k, (a, b) in enumerate(sizes):
-        assert np.array_equal(chain[k], oracle.compress_q(N.FORMAT_BC1, PROD, levels[k], a, b)), (a, b)
+        assert np.array_equal(chain[k], oracle.compress_q(N.FORMAT_BC1, quality, levels[k], a, b)), (a, b)
     # the image scheduler
     d_out2 = [gpu.alloc(N.compressed_size(N.FORMAT_BC1, a, b)) for a, b in sizes]
-    gpu.compress_images_device(N.FORMAT_BC1, d_lv, sizes, d_out2, quality=PROD)
+    gpu.compress_images_device(N.FORMAT_BC1, d_lv, sizes, d_out2, quality=quality)
     for k in range(len(sizes)):
         assert np.array_equal(d_out2[k].download(np.uint8), chain[k])
     # a batch of one texture with its chain
-    tex = dict(d_rgba=d_lv[0], width=w, height=h, format=N.FORMAT_BC1, quality=PROD, filter=N.FILTER_BOX)
+    tex = dict(d_rgba=d_lv[0], width=w, height=h, format=N.FORMAT_BC1, quality=quality, filter=N.FILTER_BOX)
     with N.Batch(gpu, [tex]) as b:
         b.run()
         for k in range(len(sizes)):

@@ -33,25 +33,47 @@ def test_bc1_against_compiled_reference(oracle, ref):
         assert np.array_equal(a, b) and ea == eb, (i, a, b)
 
 
-def test_bc1_level9_golden_vectors(oracle):
-    """Quality_Production = icbc Quality_Level9 (cluster fit + refine_endpoints):
-    the golden vectors written by the reference's icbc."""
-    g = np.load(os.path.join(GOLD, "bc1_level9_blocks.npz"))
+@pytest.mark.parametrize("name,quality", [("bc1_level9_blocks.npz", 2), ("bc1_level1_blocks.npz", 0)])
+def test_bc1_quality_golden_vectors(oracle, name, quality):
+    """Quality_Production = icbc Quality_Level9 (cluster fit + refine_endpoints),
+    Quality_Fastest = Quality_Level1 (box fit + least squares): the golden
+    vectors written by the reference's icbc."""
+    g = np.load(os.path.join(GOLD, name))
     bad = []
     for i in range(len(g["expected"])):
-        out, err = oracle.bc1_block_q(g["colors"][i], g["weights"][i], 2, g["color_weights"][i])
+        out, err = oracle.bc1_block_q(g["colors"][i], g["weights"][i], quality, g["color_weights"][i])
         if not np.array_equal(out, g["expected"][i]) or np.float32(err) != g["error"][i]:
             bad.append(i)
-    assert not bad, f"{len(bad)} golden Level9 blocks differ, first {bad[:5]}"
+    assert not bad, f"{len(bad)} golden blocks of {name} differ, first {bad[:5]}"
 
 
-def test_bc1_level9_against_compiled_reference(oracle, ref):
-    cols, wts, _ = blocks.bc1_set(seed=77, count=2000)
+@pytest.mark.parametrize("quality", [0, 2, 3])
+def test_bc1_qualities_against_compiled_reference(oracle, ref, quality):
+    cols, wts, _ = blocks.bc1_set(seed=77 + quality, count=1500)
     for i in range(len(cols)):
         cw = (1, 1, 1) if i % 4 else (0.3, 0.59, 0.11)
-        a, ea = oracle.bc1_block_q(cols[i], wts[i], 2, cw)
-        b, eb = ref.bc1_block_q(cols[i], wts[i], 2, cw)
+        a, ea = oracle.bc1_block_q(cols[i], wts[i], quality, cw)
+        b, eb = ref.bc1_block_q(cols[i], wts[i], quality, cw)
         assert np.array_equal(a, b) and ea == eb, (i, a, b)
+
+
+def test_bc1_level1_is_a_box_fit(oracle):
+    """Quality_Fastest: always a four-colour block (c0 > c1 unless both
+    endpoints quantise to one colour), whose error is the decoded MSE, and
+    never better than the cluster fit by more than float noise on average."""
+    cols, wts, _ = blocks.bc1_set(seed=8, count=800)
+    worse = 0
+    for i in range(len(cols)):
+        b1, e1 = oracle.bc1_block_q(cols[i], wts[i], 0)
+        b8, e8 = oracle.bc1_block_q(cols[i], wts[i], 1)
+        c0, c1 = int(b1[0]) | int(b1[1]) << 8, int(b1[2]) | int(b1[3]) << 8
+        assert c0 >= c1
+        dec = oracle.bc1_decode(b1).reshape(16, 4)[:, :3].astype(np.float64)
+        src = np.asarray(cols[i], np.float32).reshape(16, 4)[:, :3].astype(np.float64) * 255.0
+        mse = float(np.sum(np.asarray(wts[i], np.float64)[:, None] * (dec - src) ** 2))
+        assert mse == pytest.approx(e1, rel=1e-4, abs=1e-3)
+        worse += int(e1 > e8 + 1e-3)
+    assert worse > 100  # the box fit is the cheap, worse method
 
 
 def test_bc1_level9_refines_level8(oracle):
