@@ -242,6 +242,29 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_gamma_device(ntx_hip_context *ctx, const
  * (scale 0.5, bias 0.5; Surface.cpp:2952-2956), in place on R, G, B. */
 NTX_API ntx_status ntx_hip_normalize_normals_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
 
+/* nvtt::Decoder values (nvtt.h:149-155). */
+enum { NTX_DECODER_D3D10 = 0, NTX_DECODER_D3D9 = 1, NTX_DECODER_NV5X = 2 };
+/* Surface::setImage2D(format, decoder, w, h, blocks) (Surface.cpp:908-1118)
+ * for BC1, BC3, BC4 and BC5 with the D3D10 or D3D9 decoder: every block
+ * decoded to 8-bit RGBA as nv::BlockDXT1 / BlockDXT5 / BlockATI1 / BlockATI2
+ * do (BlockDXT.cpp:43-98, 200-217, 336-432, 475-566; BC4 gives R = G = B,
+ * BC5 R, G and B = 0), then value / 255 into the four planes of d_rgba.
+ * NV5x: NTX_ERR_UNSUPPORTED. */
+NTX_API ntx_status ntx_hip_decode_device(ntx_hip_context *ctx, int format, int decoder, const void *d_blocks, int w, int h,
+                                         float *d_rgba);
+/* Error metrics of ErrorMetric.cpp, two planar RGBA images of w x h. */
+enum {
+    NTX_METRIC_RMS_COLOR = 0,          /* nvtt::rmsError, reference without Transparency (ErrorMetric.cpp:13-46) */
+    NTX_METRIC_RMS_COLOR_ALPHA_WEIGHTED = 1, /* nvtt::rmsError, reference with AlphaMode_Transparency */
+    NTX_METRIC_RMS_ALPHA = 2           /* nvtt::rmsAlphaError (ErrorMetric.cpp:50-69) */
+};
+/* The per-texel terms are the reference's float operations; they are summed
+ * in double as the reference does, but not one after another, so the
+ * double sum may differ in its last bits and the float result by one unit in
+ * the last place at most in practice.  Waits for the device. */
+NTX_API ntx_status ntx_hip_rms_error_device(ntx_hip_context *ctx, int metric, const float *d_reference,
+                                            const float *d_image, int w, int h, float *result);
+
 /* Error of a BC1 level against its FP32 source: D3D10 decode
  * (BlockDXT.cpp:43-98) and the sum over texels of the squared R, G, B
  * differences in 8-bit units (bc1enc.cpp:73-81); PSNR = 20 log10(255 /

@@ -680,6 +680,40 @@ NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_r
     return NTX_OK;
 }
 
+NTX_API ntx_status ntx_hip_decode_device(ntx_hip_context *ctx, int format, int decoder, const void *d_blocks, int w, int h,
+                                         float *d_rgba)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (format != NTX_FORMAT_BC1 && format != NTX_FORMAT_BC3 && format != NTX_FORMAT_BC4 && format != NTX_FORMAT_BC5)
+        return NTX_ERR_UNSUPPORTED;
+    if (decoder != NTX_DECODER_D3D10 && decoder != NTX_DECODER_D3D9) return NTX_ERR_UNSUPPORTED;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_blocks || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
+    NTX_HIP(ctx, ntx::launch_decode(format, decoder == NTX_DECODER_D3D9 ? 1 : 0, d_blocks, w, h, d_rgba,
+                                    ctx->d_tab->div255, ctx->stream));
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_rms_error_device(ntx_hip_context *ctx, int metric, const float *d_reference,
+                                            const float *d_image, int w, int h, float *result)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (metric < NTX_METRIC_RMS_COLOR || metric > NTX_METRIC_RMS_ALPHA || w < 1 || h < 1 || !d_reference || !d_image ||
+        !result)
+        return NTX_ERR_INVALID_ARGUMENT;
+    if ((s = ensure(ctx, ctx->metric, sizeof(double) * (size_t(ntx::rms_partials()) + 1)))) return s;
+    double *part = static_cast<double *>(ctx->metric.ptr);
+    double *total = part + ntx::rms_partials();
+    const size_t n = size_t(w) * size_t(h);
+    NTX_HIP(ctx, ntx::launch_rms(d_reference, d_image, n, metric, part, total, ctx->stream));
+    double mse = 0.0;
+    NTX_HIP(ctx, hipMemcpyAsync(&mse, total, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *result = float(std::sqrt(mse / double(uint32_t(n))));  // ErrorMetric.cpp:45
+    return NTX_OK;
+}
+
 NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int alpha_mode, const float *cw,
                                                      const float *d_rgba, int w, int h, void *d_out, int16_t *d_nm)
 {

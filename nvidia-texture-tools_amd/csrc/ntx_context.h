@@ -57,6 +57,11 @@ hipError_t launch_normals(float *d_rgba, size_t n, hipStream_t stream);
 hipError_t launch_bc1_sse(const float *d_src, int w, int h, const void *d_blocks, double *d_partial, double *d_out,
                           hipStream_t stream);
 int bc1_sse_partials(int w, int h);
+hipError_t launch_decode(int format, int d3d9, const void *d_blocks, int w, int h, float *d_dst, const float *d_div255,
+                         hipStream_t stream);
+int rms_partials();
+hipError_t launch_rms(const float *d_ref, const float *d_img, size_t n, int mode, double *d_partial, double *d_out,
+                      hipStream_t stream);
 
 struct DevBuf {
     void *ptr = nullptr;

@@ -20,6 +20,8 @@ NVTT_LIB_PATH = os.path.join(LIB_DIR, "libnvtt_amd.so")
 # nvtt::Format / Quality / AlphaMode / MipmapFilter / WrapMode values (nvtt.h).
 FORMAT_BC1, FORMAT_BC3, FORMAT_BC4, FORMAT_BC5 = 1, 4, 6, 7
 QUALITY_FASTEST, QUALITY_NORMAL, QUALITY_PRODUCTION, QUALITY_HIGHEST = 0, 1, 2, 3
+DECODER_D3D10, DECODER_D3D9, DECODER_NV5X = 0, 1, 2
+METRIC_RMS_COLOR, METRIC_RMS_COLOR_ALPHA_WEIGHTED, METRIC_RMS_ALPHA = 0, 1, 2
 ALPHA_NONE, ALPHA_TRANSPARENCY = 0, 1
 FILTER_BOX, FILTER_TRIANGLE, FILTER_KAISER = 0, 1, 2
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = 0, 1, 2
@@ -46,7 +48,7 @@ EXPORTS = (
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
     "ntx_hip_gamma_device", "ntx_hip_gamma_levels_device", "ntx_hip_unpack_bgra8_gamma_device",
-    "ntx_hip_normalize_normals_device",
+    "ntx_hip_normalize_normals_device", "ntx_hip_decode_device", "ntx_hip_rms_error_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
     "ntx_hip_batch_autotune",
@@ -133,6 +135,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_gamma_levels_device": (_I, [_P, _I, _F, _I, _P, _P, _P, _P]),
         "ntx_hip_unpack_bgra8_gamma_device": (_I, [_P, _P, _I, _I, _F, _P]),
         "ntx_hip_normalize_normals_device": (_I, [_P, _P, _I, _I]),
+        "ntx_hip_decode_device": (_I, [_P, _I, _I, _P, _I, _I, _P]),
+        "ntx_hip_rms_error_device": (_I, [_P, _I, _P, _P, _I, _I, ctypes.POINTER(_F)]),
         "ntx_hip_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_P)]),
         "ntx_hip_batch_destroy": (_I, [_P]),
         "ntx_hip_batch_submit": (_I, [_P]),
@@ -373,6 +377,19 @@ class Context:
         """unpack_bgra8_device followed by Surface::toLinear(gamma), in one pass."""
         self._check(self.lib.ntx_hip_unpack_bgra8_gamma_device(self.handle, d_bgra.ptr, w, h, gamma, d_rgba.ptr),
                     "ntx_hip_unpack_bgra8_gamma_device")
+
+    def decode_device(self, fmt: int, d_blocks: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer,
+                      decoder: int = DECODER_D3D10) -> None:
+        """Surface::setImage2D for BC1/BC3/BC4/BC5 (ntx_hip_decode_device)."""
+        self._check(self.lib.ntx_hip_decode_device(self.handle, fmt, decoder, d_blocks.ptr, w, h, d_rgba.ptr),
+                    "ntx_hip_decode_device")
+
+    def rms_error_device(self, metric: int, d_ref: DeviceBuffer, d_img: DeviceBuffer, w: int, h: int) -> float:
+        """nvtt::rmsError / rmsAlphaError (ntx_hip_rms_error_device)."""
+        r = _F()
+        self._check(self.lib.ntx_hip_rms_error_device(self.handle, metric, d_ref.ptr, d_img.ptr, w, h, ctypes.byref(r)),
+                    "ntx_hip_rms_error_device")
+        return float(r.value)
 
     def normalize_normals_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
         """expandNormals + normalizeNormalMap + packNormals in place (ntx_hip_normalize_normals_device)."""

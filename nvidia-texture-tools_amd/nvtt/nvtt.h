@@ -79,6 +79,9 @@ enum InputFormat { InputFormat_BGRA_8UB, InputFormat_RGBA_16F, InputFormat_RGBA_
 enum MipmapFilter { MipmapFilter_Box, MipmapFilter_Triangle, MipmapFilter_Kaiser };
 enum AlphaMode { AlphaMode_None, AlphaMode_Transparency, AlphaMode_Premultiplied };
 
+// nvtt.h:149-155.
+enum Decoder { Decoder_D3D10, Decoder_D3D9, Decoder_NV5x };
+
 // nvtt.h:253-262.
 enum RoundMode {
     RoundMode_None,
@@ -288,6 +291,9 @@ struct Surface {
 
     NVTT_API bool setImage(int w, int h, int d);
     NVTT_API bool setImage(InputFormat format, int w, int h, int d, const void *data);
+    // Surface.cpp:908-1118: decodes BC1/BC3/BC4/BC5 blocks (D3D10 or D3D9
+    // decoder) on the device; other formats and Decoder_NV5x return false.
+    NVTT_API bool setImage2D(Format format, Decoder decoder, int w, int h, const void *data);
 
     NVTT_API bool buildNextMipmap(MipmapFilter filter, int min_size = 1);
     NVTT_API bool buildNextMipmap(MipmapFilter filter, float filterWidth, const float *params = 0, int min_size = 1);
@@ -305,6 +311,11 @@ struct Surface {
     struct Private;
     Private *m;
 };
+
+// nvtt.h:699-700 (Surface.cpp:3270-3280, ErrorMetric.cpp:13-69), on the
+// device; FLT_MAX when the layouts differ.
+NVTT_API float rmsError(const Surface &reference, const Surface &img);
+NVTT_API float rmsAlphaError(const Surface &reference, const Surface &img);
 
 }  // namespace nvtt
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -541,6 +542,64 @@ bool Surface::setImage(InputFormat format, int w, int h, int d, const void *data
     m->hostValid = false;
     return true;
 }
+
+bool Surface::setImage2D(Format format, Decoder decoder, int w, int h, const void *data)
+{
+    if (w <= 0 || h <= 0 || !data) return false;
+    if (format != Format_BC1 && format != Format_BC3 && format != Format_BC4 && format != Format_BC5) return false;
+    if (decoder != Decoder_D3D10 && decoder != Decoder_D3D9) return false;
+    if (!detach(m)) return false;
+    std::lock_guard<std::mutex> g(backend_lock());
+    ntx_hip_context *ctx = backend();
+    if (!ctx) { report("setImage2D", NTX_ERR_NO_DEVICE); return false; }
+    const size_t bytes = ntx_hip_compressed_size(int(format), w, h);
+    void *p = nullptr, *blocks = nullptr;
+    ntx_status s = ntx_hip_malloc(ctx, 4 * size_t(w) * size_t(h) * sizeof(float), &p);
+    if (s == NTX_OK) s = ntx_hip_malloc(ctx, bytes, &blocks);
+    if (s == NTX_OK) s = ntx_hip_upload(ctx, blocks, data, bytes);
+    if (s == NTX_OK) s = ntx_hip_decode_device(ctx, int(format), int(decoder), blocks, w, h, static_cast<float *>(p));
+    if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+    if (blocks) ntx_hip_free(ctx, blocks);
+    if (s != NTX_OK) {
+        if (p) ntx_hip_free(ctx, p);
+        report("setImage2D", s);
+        return false;
+    }
+    if (m->dev) ntx_hip_free(ctx, m->dev);
+    m->dev = static_cast<float *>(p);
+    m->w = w; m->h = h; m->d = 1;
+    m->type = TextureType_2D;
+    m->hostValid = false;
+    return true;
+}
+
+namespace {
+
+float rms_metric(const Surface &reference, const Surface &img, int metric)
+{
+    if (reference.isNull() || img.isNull() || reference.width() != img.width() || reference.height() != img.height() ||
+        reference.depth() != img.depth())
+        return FLT_MAX;  // ErrorMetric.cpp:15-17 sameLayout
+    std::lock_guard<std::mutex> g(backend_lock());
+    float r = FLT_MAX;
+    ntx_status s = ntx_hip_rms_error_device(backend(), metric, reference.deviceData(), img.deviceData(),
+                                            reference.width(), reference.height(), &r);
+    if (s != NTX_OK) report("rmsError", s);
+    return s == NTX_OK ? r : FLT_MAX;
+}
+
+}  // namespace
+
+// Surface.cpp:3270-3274.
+float rmsError(const Surface &reference, const Surface &img)
+{
+    return rms_metric(reference, img,
+                      reference.alphaMode() == AlphaMode_Transparency ? NTX_METRIC_RMS_COLOR_ALPHA_WEIGHTED
+                                                                      : NTX_METRIC_RMS_COLOR);
+}
+
+// Surface.cpp:3276-3280.
+float rmsAlphaError(const Surface &reference, const Surface &img) { return rms_metric(reference, img, NTX_METRIC_RMS_ALPHA); }
 
 bool Surface::canMakeNextMipmap(int min_size)
 {

@@ -1463,3 +1463,95 @@ void ntxo_normalize_normals(size_t n, float *p)
         p[i] = 0.5f * x + 0.5f; p[i + n] = 0.5f * y + 0.5f; p[i + 2 * n] = 0.5f * z + 0.5f;
     }
 }
+
+/* ------------------------------------------------------ decode and errors --
+ * Surface::setImage2D (Surface.cpp:1022-1107) for BC1/BC3/BC4/BC5: the nvimage
+ * block decoders (BlockDXT.cpp:43-98, 200-217, 336-432, 475-566) to 8-bit
+ * RGBA, then float(c) * 1.0f / 255.0f per channel.  d3d9 only affects BC4/BC5
+ * (the BC1/BC3 colour decoder is called with d3d9 = false for both D3D
+ * decoders). */
+static void decode_alpha_palette(const uint8_t *b, int d3d9, int pal[8])
+{
+    const int a0 = b[0], a1 = b[1];
+    pal[0] = a0; pal[1] = a1;
+    if (a0 > a1) {
+        const int bias = d3d9 ? 3 : 0;
+        for (int i = 2; i < 8; i++) pal[i] = ((8 - i) * a0 + (i - 1) * a1 + bias) / 7;
+    } else {
+        const int bias = d3d9 ? 2 : 0;
+        for (int i = 2; i < 6; i++) pal[i] = ((6 - i) * a0 + (i - 1) * a1 + bias) / 5;
+        pal[6] = 0; pal[7] = 255;
+    }
+}
+
+static int alpha_index(const uint8_t *b, int t)
+{
+    uint64_t u = 0;
+    for (int k = 0; k < 8; k++) u |= (uint64_t)b[k] << (8 * k);
+    return (int)((u >> (16 + 3 * t)) & 7);
+}
+
+void ntxo_decode_image(int format, int d3d9, const uint8_t *blocks, int w, int h, float *planar)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4;
+    const size_t n = (size_t)w * h;
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            const uint8_t *b = blocks + (size_t)(by * bw + bx) * ((format == NTXO_FMT_BC1 || format == NTXO_FMT_BC4) ? 8 : 16);
+            uint8_t rgba[64];
+            if (format == NTXO_FMT_BC1) {
+                ntxo_bc1_decode(b, rgba);
+            } else if (format == NTXO_FMT_BC3) {
+                int pal[8];
+                ntxo_bc1_decode(b + 8, rgba);
+                decode_alpha_palette(b, 0, pal);
+                for (int t = 0; t < 16; t++) rgba[4 * t + 3] = (uint8_t)pal[alpha_index(b, t)];
+            } else if (format == NTXO_FMT_BC4) {
+                int pal[8];
+                decode_alpha_palette(b, d3d9, pal);
+                for (int t = 0; t < 16; t++) {
+                    const uint8_t v = (uint8_t)pal[alpha_index(b, t)];
+                    rgba[4 * t] = rgba[4 * t + 1] = rgba[4 * t + 2] = v;
+                    rgba[4 * t + 3] = 255;
+                }
+            } else {
+                int px[8], py[8];
+                decode_alpha_palette(b, d3d9, px);
+                decode_alpha_palette(b + 8, d3d9, py);
+                for (int t = 0; t < 16; t++) {
+                    rgba[4 * t] = (uint8_t)px[alpha_index(b, t)];
+                    rgba[4 * t + 1] = (uint8_t)py[alpha_index(b + 8, t)];
+                    rgba[4 * t + 2] = 0;
+                    rgba[4 * t + 3] = 255;
+                }
+            }
+            for (int t = 0; t < 16; t++) {
+                const int x = 4 * bx + (t & 3), y = 4 * by + (t >> 2);
+                if (x >= w || y >= h) continue;
+                const size_t o = (size_t)y * w + x;
+                for (int c = 0; c < 4; c++) planar[o + c * n] = (float)rgba[4 * t + c] * 1.0f / 255.0f;
+            }
+        }
+}
+
+/* ErrorMetric.cpp:13-46 rmsColorError and :50-69 rmsAlphaError, summed one
+ * texel after another in double.  metric 0: colour, 1: colour weighted by the
+ * reference's alpha squared, 2: alpha. */
+float ntxo_rms_error(int metric, const float *ref, const float *img, size_t n)
+{
+    double mse = 0;
+    for (size_t i = 0; i < n; i++) {
+        if (metric == 2) {
+            float a = img[i + 3 * n] - ref[i + 3 * n];
+            mse += a * a;
+        } else {
+            float r = ref[i] - img[i], g = ref[i + n] - img[i + n], b = ref[i + 2 * n] - img[i + 2 * n];
+            float a = 1;
+            if (metric == 1) a = ref[i + 3 * n] * ref[i + 3 * n];
+            mse += (r * r) * a;
+            mse += (g * g) * a;
+            mse += (b * b) * a;
+        }
+    }
+    return (float)sqrt(mse / (unsigned)n);
+}

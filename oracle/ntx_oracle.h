@@ -101,6 +101,10 @@ void ntxo_to_gamma(size_t n, float *data, float gamma);
  * packNormals() (Surface.cpp:2952-2956); planar RGBA of n texels, in place. */
 void ntxo_normalize_normals(size_t n, float *planar);
 
+/* ---- decode and error metrics (Surface.cpp:1022-1107, ErrorMetric.cpp:13-69) ---- */
+void ntxo_decode_image(int format, int d3d9, const uint8_t *blocks, int w, int h, float *planar);
+float ntxo_rms_error(int metric, const float *ref, const float *img, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@
 //       time of one process() call in ms and the bytes it delivered
 //   nvtt_api_test api      checks error reporting, handler order and Surface semantics
 //   nvtt_api_test batch    Compressor::compress(BatchList) against per-image compress
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -319,6 +320,40 @@ int run_api()
     // No handlers at all is allowed.
     nvtt::OutputOptions none;
     CHECK(comp.compress(16, 8, 1, 0, 0, img.data(), co, none));
+
+    // Decoding compressed output and measuring it (Surface::setImage2D,
+    // nvtt::rmsError / rmsAlphaError): a 16 x 8 ramp through BC3.
+    {
+        std::vector<uint8_t> px(16 * 8 * 4);
+        for (int i = 0; i < 16 * 8; i++) {
+            const int x = i % 16, y = i / 16;
+            px[4 * i] = uint8_t(x * 16);        // B: a ramp along x (colinear in each block)
+            px[4 * i + 1] = uint8_t(x * 8);     // G
+            px[4 * i + 2] = uint8_t(128);       // R
+            px[4 * i + 3] = uint8_t(y < 4 ? 255 : 128);
+        }
+        nvtt::Surface src;
+        CHECK(src.setImage(nvtt::InputFormat_BGRA_8UB, 16, 8, 1, px.data()));
+        Capture bc3;
+        nvtt::OutputOptions oo3;
+        oo3.setOutputHandler(&bc3);
+        nvtt::CompressionOptions co3;
+        co3.setFormat(nvtt::Format_BC3);
+        CHECK(comp.compress(src, 0, 0, co3, oo3));
+        CHECK(bc3.bytes.size() == 4 * 2 * 16);
+        nvtt::Surface dec;
+        CHECK(dec.setImage2D(nvtt::Format_BC3, nvtt::Decoder_D3D10, 16, 8, bc3.bytes.data()));
+        CHECK(dec.width() == 16 && dec.height() == 8);
+        const float e = nvtt::rmsError(src, dec), ea = nvtt::rmsAlphaError(src, dec);
+        if (!(e > 0.0f && e < 0.05f)) std::fprintf(stderr, "rmsError %g rmsAlphaError %g\n", e, ea);
+        CHECK(e > 0.0f && e < 0.05f);
+        CHECK(ea >= 0.0f && ea < 0.01f);
+        CHECK(nvtt::rmsError(src, src) == 0.0f);
+        nvtt::Surface small;
+        CHECK(small.setImage(8, 8, 1));
+        CHECK(nvtt::rmsError(src, small) == FLT_MAX);  // different layouts
+        CHECK(!dec.setImage2D(nvtt::Format_BC7, nvtt::Decoder_D3D10, 16, 8, bc3.bytes.data()));
+    }
     std::printf("api ok\n");
     return 0;
 }

@@ -44,6 +44,9 @@ class Oracle:
         L.ntxo_to_linear.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
         L.ntxo_to_gamma.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
         L.ntxo_normalize_normals.argtypes = [ctypes.c_size_t, _P]
+        L.ntxo_decode_image.argtypes = [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
+        L.ntxo_rms_error.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
+        L.ntxo_rms_error.restype = ctypes.c_float
         L.ntxo_init()
 
     # ---- tables
@@ -160,6 +163,14 @@ class Oracle:
         """FloatImage::toLinear(0, 3, gamma) of planar RGBA: R, G, B transformed, A kept."""
         a = f32(planar).reshape(-1).copy(); n = a.size // 4
         a[:3 * n] = self.to_linear(a[:3 * n], gamma); return a
+
+    def decode_image(self, fmt, blocks, w, h, d3d9=False):
+        b = np.ascontiguousarray(blocks, np.uint8); out = np.zeros(4 * w * h, np.float32)
+        self.lib.ntxo_decode_image(int(fmt), int(bool(d3d9)), _p(b), int(w), int(h), _p(out)); return out
+
+    def rms_error(self, metric, ref, img):
+        r = f32(ref).reshape(-1); i = f32(img).reshape(-1)
+        return float(self.lib.ntxo_rms_error(int(metric), _p(r), _p(i), r.size // 4))
 
     def normalize_normals(self, planar):
         a = f32(planar).reshape(-1).copy(); self.lib.ntxo_normalize_normals(a.size // 4, _p(a)); return a
