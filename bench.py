@@ -252,6 +252,25 @@ def main():
     ms_bc1i, _, _ = timed(step_bc1_images, args.steps, 1)
     bc1_mtex = texels * args.steps / (ms_bc1i / 1e3) / 1e6
     bc1_images_same = all(np.array_equal(a, d.download(np.uint8)) for a, d in zip(per_level_bc1, d_bc1))
+    # The other BC1 qualities over the same chain (icbc Level1 for Fastest,
+    # Level9 for Production), and the device decode + rmsError of level 0.
+    bc1_q = {}
+    for qname, q in (("fastest_level1", N.QUALITY_FASTEST), ("production_level9", N.QUALITY_PRODUCTION)):
+        ms_q, _, _ = timed(lambda: ctx.compress_images_device(N.FORMAT_BC1, d_lv, levels, d_bc1, quality=q), args.steps, 1)
+        bc1_q[qname] = {"mtexels_s": round(texels * args.steps / (ms_q / 1e3) / 1e6, 1),
+                        "level0_psnr_db": round(ctx.bc1_psnr_device(d_lv[0], S, S, d_bc1[0]), 3)}
+    d_dec = ctx.alloc(16 * S * S)
+    step_bc1_images()  # the Normal-quality blocks again, for the checks below
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.decode_device(N.FORMAT_BC1, d_bc1[0], S, S, d_dec)
+        rms = ctx.rms_error_device(N.METRIC_RMS_COLOR, d_lv[0], d_dec, S, S)
+    dec_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    d_dec.free()
+    bc1_q["normal_level8_psnr_db"] = round(ctx.bc1_psnr_device(d_lv[0], S, S, d_bc1[0]), 3)
+    bc1_q["decode_and_rms_level0"] = {"ms": round(dec_ms, 3), "rms_error": round(rms, 6),
+                                      "note": "setImage2D(BC1) + rmsError on the device, host-timed incl. the result copy"}
 
     gpu_bc3 = [d.download(np.uint8) for d in d_bc3]
     gpu_bc1 = [d.download(np.uint8) for d in d_bc1]
@@ -291,6 +310,7 @@ def main():
         "bc1_levels_together": {"entry_point": "ntx_hip_compress_images_device (the 13 levels scheduled together)",
                                 "per_level_calls_mtexels_s": round(bc1_mtex_per_level, 2),
                                 "blocks_identical_to_per_level": bool(bc1_images_same)},
+        "bc1_qualities": bc1_q,
         "bc1_roofline": {"kernel": "bc1_kernel", "bound": "valu", "achieved": round(bc1_tflops, 3),
                          "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(bc1_tflops / PEAK_FP32_TFLOPS, 4),
                          "algorithmic_flops_per_chain": flops_bc1, "kernel_ms_per_chain": round(bc1_kernel_ms, 4)},

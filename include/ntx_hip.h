@@ -241,6 +241,11 @@ NTX_API ntx_status ntx_hip_unpack_bgra8_gamma_device(ntx_hip_context *ctx, const
  * (FloatImage.cpp:201-219: v / |v|, zero for a zero vector) and packNormals()
  * (scale 0.5, bias 0.5; Surface.cpp:2952-2956), in place on R, G, B. */
 NTX_API ntx_status ntx_hip_normalize_normals_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
+/* Surface::normalizeNormalMap() alone (FloatImage::normalize(0),
+ * FloatImage.cpp:201-219), in place on R, G, B: what the KTX branch of
+ * Compressor::process applies to generated normal-map levels
+ * (Context.cpp:446-449). */
+NTX_API ntx_status ntx_hip_normalize_vectors_device(ntx_hip_context *ctx, float *d_rgba, int w, int h);
 
 /* nvtt::Decoder values (nvtt.h:149-155). */
 enum { NTX_DECODER_D3D10 = 0, NTX_DECODER_D3D9 = 1, NTX_DECODER_NV5X = 2 };

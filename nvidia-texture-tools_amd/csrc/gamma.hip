@@ -67,14 +67,19 @@ __global__ __launch_bounds__(256) void gamma_kernel(ntx::GammaJob job, int mode,
 // FloatImage.cpp:231-242), normalizeNormalMap() = FloatImage::normalize(0)
 // with normalizeSafe(v, 0, 0) (FloatImage.cpp:201-219, Vector.inl:549-595),
 // then packNormals() = scaleBias(0.5, 0.5) (Surface.cpp:2952-2956).
-__global__ __launch_bounds__(256) void normal_kernel(float *__restrict__ d, u64 n)
+// expand_pack = 0: normalizeNormalMap alone (the KTX branch of
+// Compressor::process, Context.cpp:446-449), on the values as they are.
+__global__ __launch_bounds__(256) void normal_kernel(float *__restrict__ d, u64 n, int expand_pack)
 {
     const float bias = -2.0f * 127.0f / 255.0f;
     const u64 stride = u64(gridDim.x) * blockDim.x;
     for (u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float x = 2.0f * d[i] + bias;
-        float y = 2.0f * d[i + n] + bias;
-        float z = 2.0f * d[i + 2 * n] + bias;
+        float x = d[i], y = d[i + n], z = d[i + 2 * n];
+        if (expand_pack) {
+            x = 2.0f * x + bias;
+            y = 2.0f * y + bias;
+            z = 2.0f * z + bias;
+        }
         const float l = sqrtf(x * x + y * y + z * z);
         if (fabsf(l) <= 0.0f) {
             x = y = z = 0.0f;
@@ -82,9 +87,14 @@ __global__ __launch_bounds__(256) void normal_kernel(float *__restrict__ d, u64 
             const float s = 1.0f / l;
             x = x * s; y = y * s; z = z * s;
         }
-        d[i] = 0.5f * x + 0.5f;
-        d[i + n] = 0.5f * y + 0.5f;
-        d[i + 2 * n] = 0.5f * z + 0.5f;
+        if (expand_pack) {
+            x = 0.5f * x + 0.5f;
+            y = 0.5f * y + 0.5f;
+            z = 0.5f * z + 0.5f;
+        }
+        d[i] = x;
+        d[i + n] = y;
+        d[i + 2 * n] = z;
     }
 }
 
@@ -92,12 +102,12 @@ __global__ __launch_bounds__(256) void normal_kernel(float *__restrict__ d, u64 
 
 namespace ntx {
 
-hipError_t launch_normals(float *d_rgba, size_t n, hipStream_t stream)
+hipError_t launch_normals(float *d_rgba, size_t n, bool expand_pack, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     size_t groups = (n + 255) / 256;
     if (groups > 16384) groups = 16384;
-    hipLaunchKernelGGL(normal_kernel, dim3(unsigned(groups)), dim3(256), 0, stream, d_rgba, u64(n));
+    hipLaunchKernelGGL(normal_kernel, dim3(unsigned(groups)), dim3(256), 0, stream, d_rgba, u64(n), expand_pack ? 1 : 0);
     return hipGetLastError();
 }
 

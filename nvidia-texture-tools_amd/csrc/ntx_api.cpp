@@ -869,7 +869,17 @@ NTX_API ntx_status ntx_hip_normalize_normals_device(ntx_hip_context *ctx, float 
     ntx_status s = bind(ctx);
     if (s) return s;
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && !d_rgba)) return NTX_ERR_INVALID_ARGUMENT;
-    hipError_t e = timed(ctx, NTX_KERNEL_NORMALS, [&] { return ntx::launch_normals(d_rgba, size_t(w) * size_t(h), ctx->stream); });
+    hipError_t e = timed(ctx, NTX_KERNEL_NORMALS, [&] { return ntx::launch_normals(d_rgba, size_t(w) * size_t(h), true, ctx->stream); });
+    if (e != hipSuccess) return fail(ctx, e, "normals launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_normalize_vectors_device(ntx_hip_context *ctx, float *d_rgba, int w, int h)
+{
+    ntx_status s = bind(ctx);
+    if (s) return s;
+    if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && !d_rgba)) return NTX_ERR_INVALID_ARGUMENT;
+    hipError_t e = timed(ctx, NTX_KERNEL_NORMALS, [&] { return ntx::launch_normals(d_rgba, size_t(w) * size_t(h), false, ctx->stream); });
     if (e != hipSuccess) return fail(ctx, e, "normals launch");
     return NTX_OK;
 }

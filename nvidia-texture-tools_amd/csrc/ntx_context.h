@@ -52,8 +52,8 @@ struct GammaJob {
 // mode 0: gamma 2.2 -> linear (table form), 1: linear -> gamma 2.2 (table
 // form), 2: powf(max(0, x), power).
 hipError_t launch_gamma(const GammaJob &job, int mode, float power, const Tables *d_tab, hipStream_t stream);
-// expandNormals + normalizeNormalMap + packNormals on the R, G, B planes of n texels.
-hipError_t launch_normals(float *d_rgba, size_t n, hipStream_t stream);
+// [expandNormals +] normalizeNormalMap [+ packNormals] on the R, G, B planes of n texels.
+hipError_t launch_normals(float *d_rgba, size_t n, bool expand_pack, hipStream_t stream);
 hipError_t launch_bc1_sse(const float *d_src, int w, int h, const void *d_blocks, double *d_partial, double *d_out,
                           hipStream_t stream);
 int bc1_sse_partials(int w, int h);

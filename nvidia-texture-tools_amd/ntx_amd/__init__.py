@@ -48,7 +48,8 @@ EXPORTS = (
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
     "ntx_hip_gamma_device", "ntx_hip_gamma_levels_device", "ntx_hip_unpack_bgra8_gamma_device",
-    "ntx_hip_normalize_normals_device", "ntx_hip_decode_device", "ntx_hip_rms_error_device",
+    "ntx_hip_normalize_normals_device", "ntx_hip_normalize_vectors_device", "ntx_hip_decode_device",
+    "ntx_hip_rms_error_device",
     "ntx_hip_batch_create", "ntx_hip_batch_destroy", "ntx_hip_batch_submit", "ntx_hip_batch_wait",
     "ntx_hip_batch_levels", "ntx_hip_batch_level", "ntx_hip_batch_get_tuning", "ntx_hip_batch_set_tuning",
     "ntx_hip_batch_autotune",
@@ -135,6 +136,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_gamma_levels_device": (_I, [_P, _I, _F, _I, _P, _P, _P, _P]),
         "ntx_hip_unpack_bgra8_gamma_device": (_I, [_P, _P, _I, _I, _F, _P]),
         "ntx_hip_normalize_normals_device": (_I, [_P, _P, _I, _I]),
+        "ntx_hip_normalize_vectors_device": (_I, [_P, _P, _I, _I]),
         "ntx_hip_decode_device": (_I, [_P, _I, _I, _P, _I, _I, _P]),
         "ntx_hip_rms_error_device": (_I, [_P, _I, _P, _P, _I, _I, ctypes.POINTER(_F)]),
         "ntx_hip_batch_create": (_I, [_P, _P, _I, ctypes.POINTER(_P)]),

@@ -16,10 +16,9 @@
 //  - InputOptions / Compressor::process run the reference's input pipeline
 //    (Context.cpp:217-344) on the device for 2D and cube textures: 8-bit and
 //    float input, gamma to linear and back, generated or supplied mip levels,
-//    normal-map renormalisation, DDS and DDS10 headers.  Resizing
+//    normal-map renormalisation, DDS, DDS10 and KTX output.  Resizing
 //    (setMaxExtents / setRoundMode changing the extents), height-to-normal
-//    conversion, dithering and the KTX container report
-//    Error_UnsupportedFeature / Error_UnsupportedOutputFormat instead.
+//    conversion and volume textures report Error_UnsupportedFeature.
 //  - Surface transforms outside this path are not provided.
 #ifndef NVTT_AMD_H
 #define NVTT_AMD_H

@@ -792,6 +792,8 @@ struct Compressor::Private {
 
     bool outputHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount, bool isNormalMap,
                       const CompressionOptions::Private &co, const OutputOptions::Private &oo) const;
+    bool outputKtxHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount,
+                         const CompressionOptions::Private &co, const OutputOptions::Private &oo) const;
     bool process(const InputOptions::Private &io, const CompressionOptions::Private &co,
                  const OutputOptions::Private &oo) const;
 };
@@ -919,13 +921,14 @@ int block_bytes(Format f)
     }
 }
 
-// Filter width and parameters of a mip step of process() (Context.cpp:297-304).
-void process_filter(const InputOptions::Private &io, float *width, float params[2])
+// Filter width and parameters of a mip step of process() (Context.cpp:297-304;
+// the KTX branch passes stretch and alpha the other way round, :427-429).
+void process_filter(const InputOptions::Private &io, bool ktx, float *width, float params[2])
 {
     if (io.mipmapFilter == MipmapFilter_Kaiser) {
         *width = io.kaiserWidth;
-        params[0] = io.kaiserAlpha;
-        params[1] = io.kaiserStretch;
+        params[0] = ktx ? io.kaiserStretch : io.kaiserAlpha;
+        params[1] = ktx ? io.kaiserAlpha : io.kaiserStretch;
     } else {
         *width = io.mipmapFilter == MipmapFilter_Triangle ? 1.0f : 0.5f;
         params[0] = params[1] = 0.0f;
@@ -933,6 +936,51 @@ void process_filter(const InputOptions::Private &io, float *width, float params[
 }
 
 }  // namespace
+
+// Context.cpp:870-1030: the 64-byte KTX 1.1 header (KtxFile.h:113-131,
+// KtxFile.cpp:17-34) for the block formats.
+bool Compressor::Private::outputKtxHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount,
+                                          const CompressionOptions::Private &co, const OutputOptions::Private &oo) const
+{
+    struct KtxHeader {
+        uint8_t identifier[12] = { 0xAB, 'K', 'T', 'X', ' ', '1', '1', 0xBB, '\r', '\n', 0x1A, '\n' };
+        uint32_t endianness = 0x04030201, glType = 0, glTypeSize = 1, glFormat = 0, glInternalFormat = 0,
+                 glBaseInternalFormat = 0, pixelWidth = 0, pixelHeight = 0, pixelDepth = 0, numberOfArrayElements = 0,
+                 numberOfFaces = 1, numberOfMipmapLevels = 0, bytesOfKeyValueData = 0;
+    } hd;
+    static_assert(sizeof(KtxHeader) == 64, "KTX header layout");
+    // GL enumerants of the compressed formats (KtxFile.h:58-109).
+    constexpr uint32_t kRGB_DXT1 = 0x83F0, kSRGB_DXT1 = 0x8C4C, kRGBA_DXT1 = 0x83F1, kSRGB_ALPHA_DXT1 = 0x8C4D,
+                       kRGBA_DXT3 = 0x83F2, kSRGB_ALPHA_DXT3 = 0x8C4E, kRGBA_DXT5 = 0x83F3, kSRGB_ALPHA_DXT5 = 0x8C4F,
+                       kRED_RGTC1 = 0x8DBB, kRG_RGTC2 = 0x8DBD, kBaseRED = 0x1903, kBaseRG = 0x8227, kBaseRGB = 0x1907,
+                       kBaseRGBA = 0x1908;
+    if (type == TextureType_Cube) hd.numberOfFaces = 6;
+    else if (type == TextureType_3D) hd.pixelDepth = uint32_t(d);
+    else if (type == TextureType_Array) hd.numberOfArrayElements = uint32_t(arraySize);
+    hd.pixelWidth = uint32_t(w);
+    hd.pixelHeight = uint32_t(h);
+    hd.numberOfMipmapLevels = uint32_t(mipmapCount);
+    const Format f = co.format;
+    if (f == Format_DXT1 || f == Format_DXT1n) {
+        hd.glInternalFormat = oo.srgb ? kSRGB_DXT1 : kRGB_DXT1; hd.glBaseInternalFormat = kBaseRGB;
+    } else if (f == Format_DXT1a) {
+        hd.glInternalFormat = oo.srgb ? kSRGB_ALPHA_DXT1 : kRGBA_DXT1; hd.glBaseInternalFormat = kBaseRGBA;
+    } else if (f == Format_DXT3) {
+        hd.glInternalFormat = oo.srgb ? kSRGB_ALPHA_DXT3 : kRGBA_DXT3; hd.glBaseInternalFormat = kBaseRGBA;
+    } else if (f == Format_DXT5 || f == Format_DXT5n || f == Format_BC3_RGBM) {
+        hd.glInternalFormat = oo.srgb ? kSRGB_ALPHA_DXT5 : kRGBA_DXT5; hd.glBaseInternalFormat = kBaseRGBA;
+    } else if (f == Format_BC4) {
+        hd.glInternalFormat = kRED_RGTC1; hd.glBaseInternalFormat = kBaseRED;
+    } else if (f == Format_BC5) {
+        hd.glInternalFormat = kRG_RGTC2; hd.glBaseInternalFormat = kBaseRG;
+    } else {
+        oo.error(Error_UnsupportedOutputFormat);  // uncompressed, BC6/BC7, ETC: not on this path
+        return false;
+    }
+    const bool ok = oo.writeData(&hd, 64);
+    if (!ok) oo.error(Error_FileWrite);
+    return ok;
+}
 
 // Context.cpp:604-868 for the DDS containers.
 bool Compressor::Private::outputHeader(TextureType type, int w, int h, int d, int arraySize, int mipmapCount,
@@ -944,8 +992,9 @@ bool Compressor::Private::outputHeader(TextureType type, int w, int h, int d, in
         return false;
     }
     if (!oo.outputHeader) return true;
+    if (oo.container == Container_KTX) return outputKtxHeader(type, w, h, d, arraySize, mipmapCount, co, oo);
     if (oo.container != Container_DDS && oo.container != Container_DDS10) {
-        oo.error(Error_UnsupportedOutputFormat);  // KTX is not provided
+        oo.error(Error_UnsupportedOutputFormat);
         return false;
     }
     DdsHeader hd;
@@ -1146,8 +1195,9 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
     const bool normal = io.isNormalMap;
     const bool regamma = !normal && !unit_gamma(io.outputGamma);
     const AlphaMode alphaMode = io.alphaMode;
+    const bool ktx = oo.container == Container_KTX;
     float fw, fp[2];
-    process_filter(io, &fw, fp);
+    process_filter(io, ktx, &fw, fp);
     // One device arena: level planes, gamma copies, the largest 8-bit staging
     // image, compressed output (one block for a single download); offsets
     // rounded to 256 bytes.
@@ -1206,7 +1256,11 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
                 }
                 const std::vector<uint8_t> &src = io.images[size_t(m) * size_t(faceCount) + size_t(f)];
                 bool useSource = false;
-                if (canUseFace) {
+                if (ktx) {
+                    // Context.cpp:411-418: the chain is never marked broken, so
+                    // every supplied level is used.
+                    useSource = !src.empty();
+                } else if (canUseFace) {
                     if (src.empty()) canUseFace = false;  // a missing level: generate this and the rest
                     else useSource = true;
                 }
@@ -1215,8 +1269,11 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
                 else
                     s = ntx_hip_downsample_device(ctx, int(io.mipmapFilter), fw, fp, int(io.wrapMode), int(alphaMode),
                                                   lin[i - 1], ws[i - 1], hs[i - 1], lin[i]);
+                // DDS: expand, normalise, pack (Context.cpp:329-336); KTX:
+                // normalise only (:446-449).
                 if (s == NTX_OK && normal && io.normalizeMipmaps)
-                    s = ntx_hip_normalize_normals_device(ctx, lin[i], ws[i], hs[i]);
+                    s = ktx ? ntx_hip_normalize_vectors_device(ctx, lin[i], ws[i], hs[i])
+                            : ntx_hip_normalize_normals_device(ctx, lin[i], ws[i], hs[i]);
             }
         }
         if (s == NTX_OK && regamma)
@@ -1235,11 +1292,29 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
                              : s == NTX_ERR_UNSUPPORTED   ? Error_UnsupportedFeature
                                                           : Error_CudaError);
     }
-    for (size_t j = 0; j < n; j++) {
+    auto deliver = [&](size_t j) {
         const int size = int(ntx_hip_compressed_size(fmt, ws[j], hs[j]));
         oo.beginImage(size, ws[j], hs[j], 1, images[j].face, images[j].mip);
         oo.writeData(bytes.data() + outOff[j], size);
         oo.endImage();
+    };
+    if (!ktx) {
+        for (size_t j = 0; j < n; j++) deliver(j);
+        return true;
+    }
+    // Context.cpp:345-474: per level, the byte count of all its faces
+    // (estimateSize x faceCount, as a uint32, outside any begin/endImage),
+    // then that level of every face; after each level past the first, the
+    // padding to a multiple of four bytes.
+    static const uint8_t padding[3] = { 0, 0, 0 };
+    for (int m = 0; m < mipmapCount; m++) {
+        const uint32_t imageSize = uint32_t(ntx_hip_compressed_size(fmt, ws[size_t(m)], hs[size_t(m)])) * uint32_t(faceCount);
+        oo.writeData(&imageSize, int(sizeof imageSize));
+        for (int f = 0; f < faceCount; f++) deliver(size_t(f) * size_t(mipmapCount) + size_t(m));
+        if (m > 0) {
+            const int mipPadding = 3 - int((imageSize + 3) % 4);
+            if (mipPadding != 0) oo.writeData(padding, mipPadding);
+        }
     }
     return true;
 }

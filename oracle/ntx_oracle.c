@@ -1555,3 +1555,15 @@ float ntxo_rms_error(int metric, const float *ref, const float *img, size_t n)
     }
     return (float)sqrt(mse / (unsigned)n);
 }
+
+/* FloatImage.cpp:201-219 normalize(0) alone (the KTX branch of process). */
+void ntxo_normalize_vectors(size_t n, float *p)
+{
+    for (size_t i = 0; i < n; i++) {
+        float x = p[i], y = p[i + n], z = p[i + 2 * n];
+        const float l = sqrtf(x * x + y * y + z * z);
+        if (fabsf(l) <= 0.0f) { x = y = z = 0.0f; }
+        else { const float s = 1.0f / l; x = x * s; y = y * s; z = z * s; }
+        p[i] = x; p[i + n] = y; p[i + 2 * n] = z;
+    }
+}

@@ -100,6 +100,8 @@ void ntxo_to_gamma(size_t n, float *data, float gamma);
  * (Surface.cpp:2959-2963), normalizeNormalMap() (FloatImage.cpp:201-219),
  * packNormals() (Surface.cpp:2952-2956); planar RGBA of n texels, in place. */
 void ntxo_normalize_normals(size_t n, float *planar);
+/* FloatImage.cpp:201-219 normalize(0) on the planes as they are (KTX branch). */
+void ntxo_normalize_vectors(size_t n, float *planar);
 
 /* ---- decode and error metrics (Surface.cpp:1022-1107, ErrorMetric.cpp:13-69) ---- */
 void ntxo_decode_image(int format, int d3d9, const uint8_t *blocks, int w, int h, float *planar);

@@ -44,6 +44,7 @@ class Oracle:
         L.ntxo_to_linear.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
         L.ntxo_to_gamma.argtypes = [ctypes.c_size_t, _P, ctypes.c_float]
         L.ntxo_normalize_normals.argtypes = [ctypes.c_size_t, _P]
+        L.ntxo_normalize_vectors.argtypes = [ctypes.c_size_t, _P]
         L.ntxo_decode_image.argtypes = [ctypes.c_int, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P]
         L.ntxo_rms_error.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
         L.ntxo_rms_error.restype = ctypes.c_float
@@ -171,6 +172,9 @@ class Oracle:
     def rms_error(self, metric, ref, img):
         r = f32(ref).reshape(-1); i = f32(img).reshape(-1)
         return float(self.lib.ntxo_rms_error(int(metric), _p(r), _p(i), r.size // 4))
+
+    def normalize_vectors(self, planar):
+        a = f32(planar).reshape(-1).copy(); self.lib.ntxo_normalize_vectors(a.size // 4, _p(a)); return a
 
     def normalize_normals(self, planar):
         a = f32(planar).reshape(-1).copy(); self.lib.ntxo_normalize_normals(a.size // 4, _p(a)); return a

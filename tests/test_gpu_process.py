@@ -102,8 +102,14 @@ def run_process(tmp_path, faces, fmt, filt, cube=False, alpha_mode=0, wrap=2, no
 
 
 def oracle_process(oracle, faces, fmt, filt, alpha_mode=0, wrap=2, normal=False, gammas=(2.2, 2.2), levels=1,
-                   normalize=True):
+                   normalize=True, ktx=False):
+    """The reference's steps; ktx: its KTX branch (Context.cpp:345-474), which
+    passes the Kaiser parameters as (stretch, alpha) and only normalises
+    normal-map levels.  Every supplied level is used in both cases here (the
+    tests supply leading levels only)."""
     width, params = N.DEFAULT_FILTER[filt]
+    if ktx and filt == N.FILTER_KAISER:
+        params = (params[1], params[0])
     out = []
     for lv in faces:
         h, w = lv[0].shape[:2]
@@ -119,7 +125,7 @@ def oracle_process(oracle, faces, fmt, filt, alpha_mode=0, wrap=2, normal=False,
                 else:
                     lin = oracle.build_next_mipmap(filt, width, params, wrap, lin, pw, ph)
             if m > 0 and normal and normalize:
-                lin = oracle.normalize_normals(lin)
+                lin = oracle.normalize_vectors(lin) if ktx else oracle.normalize_normals(lin)
             tmp = lin if (normal or unit(gammas[1])) else oracle.planar_to_gamma(lin, gammas[1])
             out.append((lw, lh, oracle.compress(fmt, tmp, lw, lh, alpha_mode=alpha_mode)))
     return out
@@ -216,7 +222,56 @@ def test_process_other_gamma(tmp_path, oracle):
     assert abs(p_got - p_exp) <= 0.01, (p_got, p_exp)
 
 
-def test_process_unsupported_settings(tmp_path):
+def test_process_unsupported_output_format(tmp_path):
     img = synth.noise_rect_rgba(16, 16, seed=1)
-    stream, events, errors, ok, est = run_process(tmp_path, [[img]], N.FORMAT_BC1, N.FILTER_BOX, container=KTX)
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], 11, N.FILTER_BOX)  # Format_BC7
     assert not ok and errors == [UNSUPPORTED_OUTPUT] and stream == b"" and len(events) == 0
+
+
+def ktx_header(fmt, w, h, mips, faces):
+    """KTX 1.1 header (the Khronos layout; KtxFile.h:113-131, Context.cpp:870-1030)."""
+    internal, base = {N.FORMAT_BC1: (0x83F0, 0x1907), N.FORMAT_BC3: (0x83F3, 0x1908), N.FORMAT_BC4: (0x8DBB, 0x1903),
+                      N.FORMAT_BC5: (0x8DBD, 0x8227)}[fmt]
+    ident = bytes([0xAB, 0x4B, 0x54, 0x58, 0x20, 0x31, 0x31, 0xBB, 0x0D, 0x0A, 0x1A, 0x0A])
+    return ident + struct.pack("<13I", 0x04030201, 0, 1, 0, internal, base, w, h, 0, 0, faces, mips, 0)
+
+
+KTX_CASES = [
+    ("bc1-kaiser-2d", N.FORMAT_BC1, N.FILTER_KAISER, False, False, (64, 48)),
+    ("bc3-box-cube", N.FORMAT_BC3, N.FILTER_BOX, True, False, (16, 16)),
+    ("bc5-normal-triangle", N.FORMAT_BC5, N.FILTER_TRIANGLE, False, True, (32, 32)),
+    ("bc4-kaiser", N.FORMAT_BC4, N.FILTER_KAISER, False, False, (40, 24)),
+]
+
+
+@pytest.mark.parametrize("case", KTX_CASES, ids=[c[0] for c in KTX_CASES])
+def test_process_ktx(tmp_path, oracle, case):
+    """The KTX branch: header, then per level a uint32 byte count of all faces,
+    that level of every face, padding after levels past the first."""
+    _, fmt, filt, cube, normal, (w, h) = case
+    nf = 6 if cube else 1
+    if fmt == N.FORMAT_BC5:
+        faces = [[synth.normal_map_rgba(w, h, seed=50 + f)] for f in range(nf)]
+    elif fmt == N.FORMAT_BC4:
+        faces = [[synth.single_channel_rgba(w, h, seed=50 + f)] for f in range(nf)]
+    else:
+        faces = [[synth.noise_rect_rgba(w, h, seed=50 + f, transparent=True)] for f in range(nf)]
+    stream, events, errors, ok, est = run_process(tmp_path, faces, fmt, filt, cube=cube, normal=normal, container=KTX)
+    assert ok and errors == []
+    levels = len(N.mip_sizes(w, h))
+    expect = oracle_process(oracle, faces, fmt, filt, normal=normal, levels=levels, ktx=True)
+    exp = ktx_header(fmt, w, h, levels, nf)
+    ev = [(2, 64)]
+    for m, (lw, lh) in enumerate(N.mip_sizes(w, h)):
+        size = N.compressed_size(fmt, lw, lh)
+        exp += struct.pack("<I", size * nf)
+        ev.append((2, 4))
+        for f in range(nf):
+            exp += expect[f * levels + m][2].tobytes()
+            ev += [(1, size), (2, size), (3, 0)]
+        if m > 0 and (3 - (size * nf + 3) % 4):
+            exp += bytes(3 - (size * nf + 3) % 4)
+            ev.append((2, 3 - (size * nf + 3) % 4))
+    assert stream == exp
+    assert [tuple(e[:2]) if e[0] != 3 else (3, 0) for e in events] == ev
+    assert est == nf * sum(N.compressed_size(fmt, a, b) for a, b in N.mip_sizes(w, h))
