@@ -118,6 +118,12 @@ struct CompressionOptions {
     NVTT_API void setFormat(Format format);
     NVTT_API void setQuality(Quality quality);
     NVTT_API void setColorWeights(float red, float green, float blue, float alpha = 1.0f);
+    // Kept for the uncompressed and dithered outputs, which this path does not
+    // produce: process() reports Error_UnsupportedFeature when dithering or
+    // binary alpha is asked for a block format (Context.cpp:519-541).
+    NVTT_API void setPixelFormat(unsigned int bitcount, unsigned int rmask, unsigned int gmask, unsigned int bmask,
+                                 unsigned int amask);
+    NVTT_API void setQuantization(bool colorDithering, bool alphaDithering, bool binaryAlpha, int alphaThreshold = 127);
     NVTT_API Format format() const;
     Private &m;
 
@@ -310,6 +316,10 @@ struct Surface {
     struct Private;
     Private *m;
 };
+
+// nvtt.h:451-455 (nvtt.cpp:31-59).
+NVTT_API const char *errorString(Error e);
+NVTT_API unsigned int version();
 
 // nvtt.h:699-700 (Surface.cpp:3270-3280, ErrorMetric.cpp:13-69), on the
 // device; FLT_MAX when the layouts differ.

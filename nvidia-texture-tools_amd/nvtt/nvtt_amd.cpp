@@ -90,6 +90,9 @@ struct CompressionOptions::Private {
     Format format;
     Quality quality;
     float colorWeight[4];
+    unsigned int bitcount, rmask, gmask, bmask, amask;
+    bool enableColorDithering, enableAlphaDithering, binaryAlpha;
+    int alphaThreshold;
 };
 
 CompressionOptions::CompressionOptions() : m(*new Private) { reset(); }
@@ -101,7 +104,40 @@ void CompressionOptions::reset()
     m.format = Format_DXT1;
     m.quality = Quality_Normal;
     m.colorWeight[0] = m.colorWeight[1] = m.colorWeight[2] = m.colorWeight[3] = 1.0f;
+    m.bitcount = 32;
+    m.rmask = 0x00FF0000u; m.gmask = 0x0000FF00u; m.bmask = 0x000000FFu; m.amask = 0xFF000000u;
+    m.enableColorDithering = m.enableAlphaDithering = m.binaryAlpha = false;
+    m.alphaThreshold = 127;
 }
+void CompressionOptions::setPixelFormat(unsigned int bitcount, unsigned int rmask, unsigned int gmask, unsigned int bmask,
+                                        unsigned int amask)
+{
+    m.bitcount = bitcount; m.rmask = rmask; m.gmask = gmask; m.bmask = bmask; m.amask = amask;
+}
+void CompressionOptions::setQuantization(bool colorDithering, bool alphaDithering, bool binaryAlpha, int alphaThreshold)
+{
+    m.enableColorDithering = colorDithering;
+    m.enableAlphaDithering = alphaDithering;
+    m.binaryAlpha = binaryAlpha;
+    m.alphaThreshold = alphaThreshold;
+}
+
+// nvtt.cpp:31-59 (worded here).
+const char *errorString(Error e)
+{
+    switch (e) {
+    case Error_Unknown: return "unknown error";
+    case Error_InvalidInput: return "invalid input";
+    case Error_UnsupportedFeature: return "unsupported feature";
+    case Error_CudaError: return "device (HIP) error";
+    case Error_FileOpen: return "cannot open the output file";
+    case Error_FileWrite: return "cannot write through the output handler";
+    case Error_UnsupportedOutputFormat: return "the container does not support the output format";
+    default: return "invalid error code";
+    }
+}
+
+unsigned int version() { return NVTT_VERSION; }
 void CompressionOptions::setFormat(Format format) { m.format = format; }
 void CompressionOptions::setQuality(Quality quality) { m.quality = quality; }
 void CompressionOptions::setColorWeights(float r, float g, float b, float a)
@@ -1154,8 +1190,12 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
         mipmapCount = count_mipmaps3(width, height, depth);
         if (io.maxLevel > 0) mipmapCount = std::min(mipmapCount, io.maxLevel);
     }
-    // Not on this path: resizing, volume textures, height-map conversion.
-    if (!canUseSourceImages || io.textureType == TextureType_3D || depth != 1 || io.convertToNormalMap) {
+    // Not on this path: resizing, volume textures, height-map conversion, and
+    // the quantize step (Context.cpp:519-541: colour dithering for BC1-BC3,
+    // binary alpha for any format).
+    const bool quantize = (co.enableColorDithering && co.format >= Format_BC1 && co.format <= Format_BC3) ||
+                          (!co.enableAlphaDithering && co.binaryAlpha);
+    if (!canUseSourceImages || io.textureType == TextureType_3D || depth != 1 || io.convertToNormalMap || quantize) {
         oo.error(Error_UnsupportedFeature);
         return false;
     }

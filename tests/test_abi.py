@@ -59,3 +59,21 @@ def test_front_end_exports_nvtt_api():
                 "nvtt::CompressionOptions::setFormat(nvtt::Format)"):
         assert sym in out, sym
     assert os.path.exists(os.path.join(N.LIB_DIR, "nvtt_api_test"))
+
+
+def test_c_interface_exports():
+    """Every function nvtt_wrapper.h declares is exported with C linkage, and
+    the C driver built from that header reports the version without a GPU."""
+    import re
+    import subprocess
+    hdr = open(os.path.join(N.PKG_DIR, "nvtt", "nvtt_wrapper.h")).read()
+    names = set(re.findall(r"NVTT_WRAPPER_API\s+[^;(]*?\b(nvtt\w+)\s*\(", hdr))
+    assert len(names) == 42
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(N.LIB_DIR, "libnvtt_amd.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert names <= exported, sorted(names - exported)
+    r = subprocess.run([os.path.join(N.LIB_DIR, "nvtt_wrapper_test"), "info"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0 and r.stdout.splitlines()[0] == "version 20102"
+    assert len(r.stdout.splitlines()) == 8

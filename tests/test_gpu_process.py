@@ -18,6 +18,7 @@ from ntx_amd import learned, synth
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(N.LIB_DIR, "nvtt_api_test")
+C_DRIVER = os.path.join(N.LIB_DIR, "nvtt_wrapper_test")
 DDS, DDS10, KTX = 0, 1, 2
 UNSUPPORTED_FEATURE, UNSUPPORTED_OUTPUT = 2, 6
 
@@ -73,8 +74,11 @@ def dds_header(fmt, w, h, mips, cube, normal, container, version=7):
 
 
 def run_process(tmp_path, faces, fmt, filt, cube=False, alpha_mode=0, wrap=2, normal=False, gammas=(2.2, 2.2),
-                container=DDS, header=True, mipmaps=True, max_level=-1, normalize=True):
-    """faces: per face, the supplied levels (level 0 first) as HxWx4 RGBA8."""
+                container=DDS, header=True, mipmaps=True, max_level=-1, normalize=True, driver=DRIVER, extra=()):
+    """faces: per face, the supplied levels (level 0 first) as HxWx4 RGBA8.
+    driver: the C++ front-end driver, or C_DRIVER for the C interface (which
+    has no user-version setter, so its DDS headers carry version 0); extra:
+    C_DRIVER's quality, colour-dithering and binary-alpha arguments."""
     h, w = faces[0][0].shape[:2]
     src, dst = tmp_path / "in.raw", tmp_path / "out.bin"
     with open(src, "wb") as f:
@@ -82,9 +86,9 @@ def run_process(tmp_path, faces, fmt, filt, cube=False, alpha_mode=0, wrap=2, no
         for lv in faces:
             for img in lv:
                 f.write(synth.to_bgra(img).tobytes())
-    args = [DRIVER, "process", str(src), str(dst), str(fmt), str(filt), str(int(cube)), str(alpha_mode), str(wrap),
+    args = [driver, "process", str(src), str(dst), str(fmt), str(filt), str(int(cube)), str(alpha_mode), str(wrap),
             str(int(normal)), repr(gammas[0]), repr(gammas[1]), str(container), str(int(header)), str(int(mipmaps)),
-            str(max_level), str(int(normalize))]
+            str(max_level), str(int(normalize))] + [str(x) for x in extra]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     data = dst.read_bytes()
@@ -102,7 +106,7 @@ def run_process(tmp_path, faces, fmt, filt, cube=False, alpha_mode=0, wrap=2, no
 
 
 def oracle_process(oracle, faces, fmt, filt, alpha_mode=0, wrap=2, normal=False, gammas=(2.2, 2.2), levels=1,
-                   normalize=True, ktx=False):
+                   normalize=True, ktx=False, quality=None):
     """The reference's steps; ktx: its KTX branch (Context.cpp:345-474), which
     passes the Kaiser parameters as (stretch, alpha) and only normalises
     normal-map levels.  Every supplied level is used in both cases here (the
@@ -127,7 +131,10 @@ def oracle_process(oracle, faces, fmt, filt, alpha_mode=0, wrap=2, normal=False,
             if m > 0 and normal and normalize:
                 lin = oracle.normalize_vectors(lin) if ktx else oracle.normalize_normals(lin)
             tmp = lin if (normal or unit(gammas[1])) else oracle.planar_to_gamma(lin, gammas[1])
-            out.append((lw, lh, oracle.compress(fmt, tmp, lw, lh, alpha_mode=alpha_mode)))
+            if quality is None:
+                out.append((lw, lh, oracle.compress(fmt, tmp, lw, lh, alpha_mode=alpha_mode)))
+            else:
+                out.append((lw, lh, oracle.compress_q(fmt, quality, tmp, lw, lh, alpha_mode=alpha_mode)))
     return out
 
 
@@ -275,3 +282,67 @@ def test_process_ktx(tmp_path, oracle, case):
     assert stream == exp
     assert [tuple(e[:2]) if e[0] != 3 else (3, 0) for e in events] == ev
     assert est == nf * sum(N.compressed_size(fmt, a, b) for a, b in N.mip_sizes(w, h))
+
+
+C_CASES = [c for c in CASES if c[0] in ("bc1-box-default", "bc3-kaiser-alpha", "bc5-normal-kaiser", "bc4-single-level")]
+
+
+@pytest.mark.parametrize("case", C_CASES, ids=[c[0] for c in C_CASES])
+def test_c_interface_process(tmp_path, oracle, case):
+    """nvttCompress (nvtt_wrapper.cpp:313-316) from a plain C caller: the same
+    stream, handler events and size estimate as Compressor::process."""
+    _, fmt, filt, (w, h), kw = case
+    if fmt == N.FORMAT_BC5:
+        img = synth.normal_map_rgba(w, h, seed=8)
+    elif fmt == N.FORMAT_BC4:
+        img = synth.single_channel_rgba(w, h, seed=9)
+    else:
+        img = synth.noise_rect_rgba(w, h, seed=10, transparent=True)
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], fmt, filt, driver=C_DRIVER, **kw)
+    levels = mip_count(w, h, kw.get("mipmaps", True), kw.get("max_level", -1))
+    okw = {k: v for k, v in kw.items() if k in ("alpha_mode", "wrap", "normal", "gammas", "normalize")}
+    expect = oracle_process(oracle, [[img]], fmt, filt, levels=levels, **okw)
+    hdr = dds_header(fmt, w, h, levels, False, kw.get("normal", False), kw.get("container", DDS), version=0)
+    check(stream, events, errors, ok, est, hdr, expect, 1, fmt)
+    cpp = run_process(tmp_path, [[img]], fmt, filt, **kw)
+    assert cpp[0][len(hdr):] == stream[len(hdr):]
+    assert np.array_equal(cpp[1], events) and cpp[4] == est
+
+
+def test_c_interface_ktx_production(tmp_path, oracle):
+    """The C interface with a quality set: BC1 Production (icbc Level9) into
+    KTX, every level against the oracle at that quality."""
+    w, h = 48, 32
+    img = synth.noise_rect_rgba(w, h, seed=60)
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], N.FORMAT_BC1, N.FILTER_BOX, driver=C_DRIVER,
+                                                  extra=(2, 0, 0), container=KTX)
+    assert ok and errors == []
+    sizes = N.mip_sizes(w, h)
+    expect = oracle_process(oracle, [[img]], N.FORMAT_BC1, N.FILTER_BOX, levels=len(sizes), ktx=True, quality=2)
+    exp = ktx_header(N.FORMAT_BC1, w, h, len(sizes), 1)
+    for m, (lw, lh, blocks) in enumerate(expect):
+        exp += struct.pack("<I", blocks.size) + blocks.tobytes()
+        if m > 0:
+            exp += bytes(3 - (blocks.size + 3) % 4)
+    assert stream == exp
+    assert est == sum(b.size for _, _, b in expect)
+
+
+@pytest.mark.parametrize("fmt,dither,binary", [(N.FORMAT_BC1, 1, 0), (N.FORMAT_BC3, 1, 0), (N.FORMAT_BC4, 0, 1)])
+def test_c_interface_quantize_unsupported(tmp_path, fmt, dither, binary):
+    """Colour dithering (BC1-BC3) and binary alpha ask for the quantize step
+    (Context.cpp:519-541), which this path does not run: UnsupportedFeature,
+    nothing delivered."""
+    img = synth.noise_rect_rgba(16, 16, seed=2)
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], fmt, N.FILTER_BOX, driver=C_DRIVER,
+                                                  extra=(1, dither, binary))
+    assert not ok and errors == [UNSUPPORTED_FEATURE] and stream == b"" and len(events) == 0
+
+
+def test_c_interface_dithering_ignored_for_bc4(tmp_path, oracle):
+    """Colour dithering applies to BC1-BC3 only (Context.cpp:521-526): BC4 with
+    it set compresses as without."""
+    img = synth.single_channel_rgba(32, 32, seed=3)
+    a = run_process(tmp_path, [[img]], N.FORMAT_BC4, N.FILTER_BOX, driver=C_DRIVER, extra=(1, 1, 0))
+    b = run_process(tmp_path, [[img]], N.FORMAT_BC4, N.FILTER_BOX, driver=C_DRIVER, extra=(1, 0, 0))
+    assert a[3] and a[2] == [] and a[0] == b[0]
