@@ -94,6 +94,7 @@ __device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLev
 }
 
 // BC1 blocks [first, first + count): block first + 16 i + g on group g.
+template <int LEVEL>
 __device__ void bc1_item(const BatchArgs &a, const BatchItem &it, const BatchLevel &lv, ntx_bc1::Bc1Shared &sh, int t)
 {
     const int bw = (lv.w + 3) / 4;
@@ -103,8 +104,8 @@ __device__ void bc1_item(const BatchArgs &a, const BatchItem &it, const BatchLev
     for (int k = g; k < it.count; k += ntx_bc1::kBlocksPerWG) {
         const int blk = it.first + k;
         int n, m;
-        const uint2 r = ntx_bc1::encode_block(sh, data, lv.w, lv.h, blk % bw, blk / bw, lv.alpha_mode, lv.cw[0],
-                                              lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m, lv.bc1_level);
+        const uint2 r = ntx_bc1::encode_block<LEVEL>(sh, data, lv.w, lv.h, blk % bw, blk / bw, lv.alpha_mode, lv.cw[0],
+                                                     lv.cw[1], lv.cw[2], a.tab, g, lane, &n, &m, lv.bc1_level == 9);
         if (lane == 0) out[blk] = r;
     }
 }
@@ -131,8 +132,10 @@ __device__ void channel_item(const BatchItem &it, const BatchLevel &lv, int t)
 }
 
 // kWaves: waves per SIMD the register allocation is held to (0: the
-// compiler's choice); the autotuner picks among the instances.
-template <int kWaves>
+// compiler's choice); the autotuner picks among the instances.  kTiers: the
+// batch holds BC1 levels at icbc levels other than 8 (Fastest, Production),
+// whose code would otherwise cost every batch registers.
+template <int kWaves, bool kTiers>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves > 0 ? kWaves : 1)))
 void batch_kernel(BatchArgs a)
 {
@@ -167,7 +170,16 @@ void batch_kernel(BatchArgs a)
                 __hip_atomic_fetch_add(a.counters + ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         } else if (it.kind == ntx::kItemBc1) {
-            bc1_item(a, it, lv, sh, t);
+            if constexpr (kTiers) {
+                // One cluster-fit instance for levels 8 and 9 (two would
+                // spill): level 9's refinement is switched per level.
+                if (lv.bc1_level == 1)
+                    bc1_item<1>(a, it, lv, sh, t);
+                else
+                    bc1_item<9>(a, it, lv, sh, t);
+            } else {
+                bc1_item<8>(a, it, lv, sh, t);
+            }
         } else {
             channel_item(it, lv, t);
         }
@@ -186,24 +198,28 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(int *p, int n)
 
 namespace ntx {
 
-const void *batch_kernel_variant(int variant)
+// variant 0-2: the tuned register budget (compiler's choice, 5 or 6 waves per
+// SIMD); tiers: the instance with the BC1 Fastest / Production paths.
+using BatchKernel = void (*)(BatchArgs);
+BatchKernel batch_kernel_of(int variant, bool tiers)
 {
     switch (variant) {
-    case 1: return reinterpret_cast<const void *>(&batch_kernel<5>);
-    case 2: return reinterpret_cast<const void *>(&batch_kernel<6>);
-    default: return reinterpret_cast<const void *>(&batch_kernel<0>);
+    case 1: return tiers ? batch_kernel<5, true> : batch_kernel<5, false>;
+    case 2: return tiers ? batch_kernel<6, true> : batch_kernel<6, false>;
+    default: return tiers ? batch_kernel<0, true> : batch_kernel<0, false>;
     }
+}
+
+const void *batch_kernel_variant(int variant, bool tiers)
+{
+    return reinterpret_cast<const void *>(batch_kernel_of(variant, tiers));
 }
 
 int batch_kernel_variants() { return 3; }
 
-hipError_t launch_batch(const BatchArgs &args, int variant, int workgroups, hipStream_t stream)
+hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream)
 {
-    switch (variant) {
-    case 1: hipLaunchKernelGGL(batch_kernel<5>, dim3(workgroups), dim3(256), 0, stream, args); break;
-    case 2: hipLaunchKernelGGL(batch_kernel<6>, dim3(workgroups), dim3(256), 0, stream, args); break;
-    default: hipLaunchKernelGGL(batch_kernel<0>, dim3(workgroups), dim3(256), 0, stream, args); break;
-    }
+    hipLaunchKernelGGL(batch_kernel_of(variant, tiers), dim3(workgroups), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
@@ -214,10 +230,10 @@ hipError_t launch_batch_reset(int *p, int n, hipStream_t stream)
 }
 
 // Workgroups of one batch_kernel instance that fit on the device at once.
-hipError_t batch_resident_workgroups(int device, int variant, int *count)
+hipError_t batch_resident_workgroups(int device, int variant, bool tiers, int *count)
 {
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_variant(variant), 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_variant(variant, tiers), 256, 0);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     *count = per_cu * cus;
     return e;

@@ -13,9 +13,9 @@
 #include "ntx_context.h"
 
 namespace ntx {
-hipError_t launch_batch(const BatchArgs &args, int variant, int workgroups, hipStream_t stream);
+hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream);
 hipError_t launch_batch_reset(int *p, int n, hipStream_t stream);
-hipError_t batch_resident_workgroups(int device, int variant, int *count);
+hipError_t batch_resident_workgroups(int device, int variant, bool tiers, int *count);
 int batch_kernel_variants();
 }  // namespace ntx
 
@@ -33,6 +33,7 @@ struct ntx_hip_batch {
     int *d_sync = nullptr;               // ready counters, queue head, status
     int ncounters = 0;
     int nitems = 0;
+    bool bc1_tiers = false;              // some BC1 level is not at icbc level 8
     int workgroups = 0;
     ntx_batch_tuning tuning{};
     hipGraph_t graph = nullptr;
@@ -156,7 +157,7 @@ ntx_status build_items(ntx_hip_batch *b)
         NTX_HIP(ctx, hipMemcpy(b->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
     NTX_HIP(ctx, hipMemcpy(b->d_levels, b->levels.data(), sizeof(BatchLevel) * b->levels.size(), hipMemcpyHostToDevice));
     int resident = 0;
-    NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, tu.kernel_variant, &resident));
+    NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, tu.kernel_variant, b->bc1_tiers, &resident));
     int wgs = tu.workgroups > 0 ? std::min(tu.workgroups, resident) : resident;
     wgs = std::max(1, std::min(wgs, std::max(b->nitems, 1)));
     b->workgroups = wgs;
@@ -183,7 +184,7 @@ hipError_t enqueue_run(ntx_hip_batch *b)
 {
     hipStream_t s = b->ctx->stream;
     hipError_t e = ntx::launch_batch_reset(b->d_sync, b->ncounters + 2, s);
-    if (e == hipSuccess && b->nitems > 0) e = ntx::launch_batch(make_args(b), b->tuning.kernel_variant, b->workgroups, s);
+    if (e == hipSuccess && b->nitems > 0) e = ntx::launch_batch(make_args(b), b->tuning.kernel_variant, b->bc1_tiers, b->workgroups, s);
     return e;
 }
 
@@ -296,6 +297,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             lv.format = d.format;
             lv.alpha_mode = d.alpha_mode;
             lv.bc1_level = ntx::bc1_level(d.quality);
+            if (d.format == NTX_FORMAT_BC1 && lv.bc1_level != 8) b->bc1_tiers = true;
             const bool zero_w = d.color_weights[0] == 0 && d.color_weights[1] == 0 && d.color_weights[2] == 0 &&
                                 d.color_weights[3] == 0;
             for (int k = 0; k < 3; k++) lv.cw[k] = zero_w ? 1.0f : d.color_weights[k];
@@ -459,7 +461,7 @@ NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *b, int reps, ntx_batch_
         case 3: cand = { 16, 32, 64, 128, 256 }; break;             // bc1_blocks_per_item
         case 4: cand = { 256, 512, 1024, 2048, 4096 }; break;       // channel_jobs_per_item
         default:                                                    // workgroups
-            NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, best.kernel_variant, &resident));
+            NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, best.kernel_variant, b->bc1_tiers, &resident));
             cand = { 0, resident * 3 / 4, resident / 2, resident / 4 };
             break;
         }

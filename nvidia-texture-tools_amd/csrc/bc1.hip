@@ -10,10 +10,11 @@ namespace {
 
 using namespace ntx_bc1;
 
+template <int LEVEL>
 __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                   int alpha_mode, float cwx, float cwy, float cwz,
                                                   const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
-                                                  short2 *__restrict__ nm, int urgent, int level)
+                                                  short2 *__restrict__ nm, int urgent)
 {
     if (urgent) __builtin_amdgcn_s_setprio(3);
     __shared__ Bc1Shared sh;
@@ -25,8 +26,8 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
     const int blk = blockIdx.x * kBlocksPerWG + g;
     if (blk >= nblocks) return;  // whole group leaves together
     int n, m;
-    const uint2 result = encode_block(sh, src, w, h, blk % bw, blk / bw, alpha_mode, cwx, cwy, cwz, tab, g, lane, &n, &m,
-                                      level);
+    const uint2 result =
+        encode_block<LEVEL>(sh, src, w, h, blk % bw, blk / bw, alpha_mode, cwx, cwy, cwz, tab, g, lane, &n, &m);
     if (lane == 0) {
         out[blk] = result;
         if (nm) nm[blk] = make_short2(short(n), short(m));
@@ -47,9 +48,10 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
     const int nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
     const int grid = (nblocks + kBlocksPerWG - 1) / kBlocksPerWG;
-    hipLaunchKernelGGL(bc1_kernel, dim3(grid), dim3(kBlocksPerWG * kGroup), 0, stream, d_src, w, h, bw, nblocks,
-                       alpha_mode, cw[0], cw[1], cw[2], d_tab, reinterpret_cast<uint2 *>(d_out),
-                       reinterpret_cast<short2 *>(d_nm), urgent ? 1 : 0, level);
+    auto kernel = level == 1 ? bc1_kernel<1> : level == 9 ? bc1_kernel<9> : bc1_kernel<8>;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlocksPerWG * kGroup), 0, stream, d_src, w, h, bw, nblocks, alpha_mode,
+                       cw[0], cw[1], cw[2], d_tab, reinterpret_cast<uint2 *>(d_out), reinterpret_cast<short2 *>(d_nm),
+                       urgent ? 1 : 0);
     return hipGetLastError();
 }
 

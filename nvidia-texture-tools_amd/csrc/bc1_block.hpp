@@ -491,12 +491,18 @@ __device__ __forceinline__ float icbc_max(float a, float b) { return (b < a) ? a
 __device__ __forceinline__ float icbc_min(float a, float b) { return (a < b) ? a : b; }
 __device__ __forceinline__ float icbc_saturate(float x) { return icbc_min(icbc_max(x, 0.0f), 1.0f); }
 
-// Sum over the group's 16 lanes in lane order (the reference's sequential loops).
-__device__ __forceinline__ float group_sum(float v)
+// Sums over the group's 16 lanes, each in lane order (the reference's
+// sequential loops); the N sums advance together, one lane at a time, so only
+// N broadcasts are live at once.
+template <int N>
+__device__ __forceinline__ void group_sums(const float (&v)[N], float (&s)[N])
 {
-    float s = 0.0f;
-    static_for<0, 16>([&](auto k) { s += gbcast<decltype(k)::value>(v); });
-    return s;
+    #pragma unroll
+    for (int j = 0; j < N; j++) s[j] = 0.0f;
+    static_for<0, 16>([&](auto k) {
+        #pragma unroll
+        for (int j = 0; j < N; j++) s[j] += gbcast<decltype(k)::value>(v[j]);
+    });
 }
 
 // compress_dxt1 at Quality_Level1, nvtt's Quality_Fastest (icbc.h:3485-3540
@@ -524,7 +530,9 @@ __device__ __forceinline__ uint2 encode_box_fit(float tr, float tg, float tb, fl
     // select_diagonal: signs of the x-z and y-z covariances about the box centre.
     const float cx = (x0 + x1) * 0.5f, cy = (y0 + y1) * 0.5f, cz = (z0 + z1) * 0.5f;
     const float tx = tr - cx, ty = tg - cy, tz = tb - cz;
-    const float cov_xz = group_sum(tx * tz), cov_yz = group_sum(ty * tz);
+    float cov[2];
+    group_sums<2>({ tx * tz, ty * tz }, cov);
+    const float cov_xz = cov[0], cov_yz = cov[1];
     if (cov_xz < 0.0f) { const float t = x0; x0 = x1; x1 = t; }
     if (cov_yz < 0.0f) { const float t = y0; y0 = y1; y1 = t; }
     Cand c = { x0, y0, z0, x1, y1, z1 };
@@ -535,11 +543,20 @@ __device__ __forceinline__ uint2 encode_box_fit(float tr, float tg, float tb, fl
     float beta = float(bits & 1u);
     if (bits & 2u) beta = (1.0f + beta) / 3.0f;
     const float alpha = 1.0f - beta;
-    const float a2 = group_sum(alpha * alpha), b2 = group_sum(beta * beta), ab = group_sum(alpha * beta);
-    const float axr = group_sum(alpha * tr), axg = group_sum(alpha * tg), axb = group_sum(alpha * tb);
-    const float bxr = group_sum(beta * tr), bxg = group_sum(beta * tg), bxb = group_sum(beta * tb);
+    float m[9];
+    group_sums<9>({ alpha * alpha, beta * beta, alpha * beta, alpha * tr, alpha * tg, alpha * tb, beta * tr, beta * tg,
+                    beta * tb },
+                  m);
+    const float a2 = m[0], b2 = m[1], ab = m[2];
+    const float axr = m[3], axg = m[4], axb = m[5];
+    const float bxr = m[6], bxg = m[7], bxb = m[8];
     const float denom = a2 * b2 - ab * ab;
-    if (fabsf(denom - 0.0f) < 0.0001f) return blk;  // icbc equal(denom, 0)
+    // icbc returns the first block when equal(denom, 0).  The test is a
+    // select at the end, not a branch: the lane broadcasts of the sums cannot
+    // move into a branch that the four groups of a wave may take differently,
+    // so a branch here keeps all of them live at once.  A degenerate denom
+    // gives non-finite factors, which saturate() maps to 0: harmless.
+    const bool degenerate = fabsf(denom - 0.0f) < 0.0001f;
     const float factor = 1.0f / denom;
     Cand o;
     o.ax = icbc_saturate((axr * b2 - bxr * ab) * factor);
@@ -550,18 +567,24 @@ __device__ __forceinline__ uint2 encode_box_fit(float tr, float tg, float tb, fl
     o.bz = icbc_saturate((bxb * a2 - axb * ab) * factor);
     uint2 blk2;
     const float err2 = output_block<true>(o, tr, tg, tb, tw, cwx, cwy, cwz, mid5, mid6, div255, lane, &blk2);
-    return err2 < err ? blk2 : blk;
+    return (!degenerate && err2 < err) ? blk2 : blk;
 }
 
 // Encodes block (bx, by) of the w x h planar RGBA image for this group;
 // every lane of the group returns the 8-byte block, the colour count n
 // after reduce_colors and the count m seen by the three-colour fit (-1 if
-// it did not run).  Uses only wave-level synchronisation.
+// it did not run).  Uses only wave-level synchronisation.  LEVEL: icbc
+// quality level 1, 8 or 9, a template argument so that each instance holds
+// only its own path's registers (the level-8 kernel keeps its occupancy).
+// refine (LEVEL 9 only, uniform across the workgroup): run refine_endpoints,
+// so that one instance serves levels 8 and 9 where both are needed.
+template <int LEVEL = 8>
 __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, int w, int h, int bx, int by,
                                               int alpha_mode, float cwx, float cwy, float cwz,
                                               const ntx::Tables *tab, int g, int lane, int *n_out, int *m_out,
-                                              int level = 8)
+                                              bool refine = true)
 {
+    static_assert(LEVEL == 1 || LEVEL == 8 || LEVEL == 9, "icbc levels 1, 8 and 9");
     // Gather (BlockCompressor.cpp:116-166): texel l; outside the image -> colour 0, weight 0.
     const int x = bx * 4 + (lane & 3), y = by * 4 + (lane >> 2);
     const size_t plane = size_t(w) * size_t(h);
@@ -580,7 +603,7 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
         }
     }
 
-    if (level == 1) {  // group-uniform
+    if constexpr (LEVEL == 1) {
         *n_out = 16;
         *m_out = -1;
         return encode_box_fit(tr, tg, tb, tw, cwx, cwy, cwz, tab->mid5, tab->mid6, sh.div255, lane);
@@ -662,7 +685,9 @@ __device__ __forceinline__ uint2 encode_block(Bc1Shared &sh, const float *src, i
         }
         if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
         // Quality_Level9 (icbc.h:3551-3553): the cluster fit's block refined.
-        else if (level == 9) result = refine_endpoints(result, best, tr, tg, tb, tw, cwx, cwy, cwz, sh.div255, lane);
+        else if constexpr (LEVEL == 9) {
+            if (refine) result = refine_endpoints(result, best, tr, tg, tb, tw, cwx, cwy, cwz, sh.div255, lane);
+        }
     }
     *n_out = n;
     *m_out = m_count;

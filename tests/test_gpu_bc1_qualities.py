@@ -101,3 +101,35 @@ def test_quality_chain_images_and_batch(gpu, oracle, quality):
             assert np.array_equal(b.blocks(0, k), chain[k]), k
     for d in d_lv + d_out + d_out2:
         d.free()
+
+
+def test_batch_mixing_qualities(gpu, oracle):
+    """One batch holding BC1 textures at Fastest, Normal and Production and a
+    BC4 texture (the batch kernel instance with every BC1 level, one shared
+    cluster-fit path for levels 8 and 9): every level equals the oracle at
+    its texture's quality."""
+    texs, bufs, plan = [], [], []
+    for i, (q, fmt, (w, h)) in enumerate([(FAST, N.FORMAT_BC1, (64, 48)), (N.QUALITY_NORMAL, N.FORMAT_BC1, (40, 72)),
+                                          (PROD, N.FORMAT_BC1, (96, 64)), (N.QUALITY_NORMAL, N.FORMAT_BC4, (32, 32))]):
+        img = synth.single_channel_rgba(w, h, seed=70 + i) if fmt == N.FORMAT_BC4 else \
+            synth.noise_rect_rgba(w, h, seed=70 + i)
+        planar = synth.to_planar(img).reshape(-1)
+        d = gpu.alloc(planar.nbytes)
+        d.upload(planar)
+        bufs.append(d)
+        texs.append(dict(d_rgba=d, width=w, height=h, format=fmt, quality=q, filter=N.FILTER_TRIANGLE))
+        plan.append((q, fmt, w, h, planar))
+    with N.Batch(gpu, texs) as b:
+        b.run()
+        for t, (q, fmt, w, h, planar) in enumerate(plan):
+            lv = planar
+            sizes = N.mip_sizes(w, h)
+            for k, (a, c) in enumerate(sizes):
+                if k > 0:
+                    pw, ph = sizes[k - 1]
+                    width, params = N.DEFAULT_FILTER[N.FILTER_TRIANGLE]
+                    lv = oracle.build_next_mipmap(N.FILTER_TRIANGLE, width, params, N.WRAP_MIRROR, lv, pw, ph)
+                exp = oracle.compress_q(fmt, q, lv, a, c) if fmt == N.FORMAT_BC1 else oracle.compress(fmt, lv, a, c)
+                assert np.array_equal(b.blocks(t, k), exp), (t, k)
+    for d in bufs:
+        d.free()
