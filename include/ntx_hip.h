@@ -40,8 +40,10 @@ typedef enum {
     NTX_ERR_OUT_OF_MEMORY = 5
 } ntx_status;
 
-/* nvtt::Format values (nvtt.h:77-97) accepted by the compressor. */
-enum { NTX_FORMAT_BC1 = 1, NTX_FORMAT_BC3 = 4, NTX_FORMAT_BC4 = 6, NTX_FORMAT_BC5 = 7 };
+/* nvtt::Format values (nvtt.h:77-97) accepted by the compressor.  BC3N is
+ * Format_BC3n (DXT5n, CompressorDX9.cpp:179-222): a normal map's X (red) in
+ * the alpha block and Y (green) in the colour block. */
+enum { NTX_FORMAT_BC1 = 1, NTX_FORMAT_BC3 = 4, NTX_FORMAT_BC3N = 5, NTX_FORMAT_BC4 = 6, NTX_FORMAT_BC5 = 7 };
 /* nvtt::Quality (nvtt.h:148-154). */
 enum { NTX_QUALITY_FASTEST = 0, NTX_QUALITY_NORMAL = 1, NTX_QUALITY_PRODUCTION = 2, NTX_QUALITY_HIGHEST = 3 };
 /* nvtt::AlphaMode (nvtt.h:274-279). */

@@ -93,6 +93,14 @@ __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 // finished block is written as one 16-byte store when the output is 16-byte
 // aligned; inner blocks read their rows as 16-byte loads when the rows are
 // aligned (rows16).
+// kN: Format_BC3n (CompressorDXT5n::compressBlock, CompressorDX9.cpp:179-222
+// at Normal/Production): the alpha block encodes the red channel
+// (swizzle(4, 1, 5, 0), then QuickCompress::compressDXT5A), the colour block
+// the texels (255, G, 0, A) (swizzle(4, 1, 5, 3)) with metric (0, 1, 0),
+// given by the host.  A single-green block takes compressDXT1G(g)
+// (OptimalCompressDXT.cpp:294-311), which is compressDXT1 of (255, g, 0):
+// the OMatch5 entries of 255 and 0 are (31, 31) and (0, 0).
+template <bool kN>
 __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict__ src, int w, int h, int bw, int nblocks,
                                                         const ntx::Tables *__restrict__ tab, uint8_t *__restrict__ out,
                                                         int wide, int *__restrict__ counts, int *__restrict__ lists, int cap,
@@ -136,7 +144,19 @@ __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict_
                 rgb[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16;
             }
         }
-        const uint64_t alpha = compress_channel(av);
+        uint64_t alpha;
+        if constexpr (kN) {
+            uint32_t xr[4];  // byte b of xr[q]: red of texel 4q + b
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                xr[q] = (rgb[4 * q] & 0xffu) | (rgb[4 * q + 1] & 0xffu) << 8 | (rgb[4 * q + 2] & 0xffu) << 16 |
+                        (rgb[4 * q + 3] & 0xffu) << 24;
+#pragma unroll
+            for (int i = 0; i < 16; i++) rgb[i] = 0xffu | (rgb[i] & 0xff00u);
+            alpha = compress_channel(xr);
+        } else {
+            alpha = compress_channel(av);
+        }
         uint8_t *dst = out + size_t(blk) * 16;
         bool single = true;
 #pragma unroll
@@ -635,16 +655,16 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
 // kCountSlots x cap ints.
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
                             int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
-                            hipStream_t stream)
+                            hipStream_t stream, bool normal)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     if (nblocks == 0) return hipSuccess;
     const int wide = (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
     // Every row of every plane starts 16-byte aligned: float4 row loads.
     const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
-    hipLaunchKernelGGL(bc3_front_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks, d_tab,
-                       reinterpret_cast<uint8_t *>(d_out), wide, d_counts, d_lists, cap, weight_by_alpha,
-                       reinterpret_cast<uint4 *>(d_packed), urgent ? 1 : 0, rows16);
+    hipLaunchKernelGGL(normal ? bc3_front_kernel<true> : bc3_front_kernel<false>, dim3((nblocks + 255) / 256), dim3(256),
+                       0, stream, d_src, w, h, bw, nblocks, d_tab, reinterpret_cast<uint8_t *>(d_out), wide, d_counts,
+                       d_lists, cap, weight_by_alpha, reinterpret_cast<uint4 *>(d_packed), urgent ? 1 : 0, rows16);
     return hipGetLastError();
 }
 

@@ -102,7 +102,10 @@ bool supported(int format, int quality)
     // BlockCompressor.cpp:211-217: every quality maps to an icbc level.
     case NTX_FORMAT_BC1:
         return quality >= NTX_QUALITY_FASTEST && quality <= NTX_QUALITY_HIGHEST;
-    case NTX_FORMAT_BC3: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
+    // CompressorDX9.cpp:145-222: Highest switches the alpha block to
+    // OptimalCompress, Fastest to other compressors (Context.cpp:1075-1097).
+    case NTX_FORMAT_BC3:
+    case NTX_FORMAT_BC3N: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
     case NTX_FORMAT_BC4:
     case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
     default: return false;
@@ -402,7 +405,11 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
             return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent,
                               bc1_level(quality));
         });
-    } else if (format == NTX_FORMAT_BC3) {
+    } else if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) {
+        // CompressorDXT5n: metric (0, 1, 0) whatever the options say.
+        const float green[3] = { 0.0f, 1.0f, 0.0f };
+        const bool normal = format == NTX_FORMAT_BC3N;
+        if (normal) cw = green;
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         int *counts = bucket, *lists = counts + bc3_lists_offset();
         uint32_t *packed = reinterpret_cast<uint32_t *>(counts + bc3_packed_offset(nblocks));
@@ -413,7 +420,7 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
         if (e == hipSuccess)
             e = timed_on(ctx, stream, NTX_KERNEL_BC3_CLASSIFY, [&] {
                 return launch_bc3_front(d_rgba, w, h, ctx->d_tab, d_out, counts, lists, nblocks, wba, packed, urgent,
-                                        stream);
+                                        stream, normal);
             });
         if (e == hipSuccess)
             e = timed_on(ctx, stream, shift ? NTX_KERNEL_BC3_SEARCH_SPLIT : NTX_KERNEL_BC3_SEARCH, [&] {
@@ -508,7 +515,7 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
     if (!cw) cw = ones;
     int *bucket = nullptr;
-    if (format == NTX_FORMAT_BC3) {
+    if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         const size_t cap_before = ctx->bucket.cap;
         if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
@@ -548,7 +555,7 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
         region[k] = total;
         layout.push_back(w[k]);
         layout.push_back(h[k]);
-        if (format == NTX_FORMAT_BC3)
+        if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
             total += (size_t(bc3_scratch_ints(((w[k] + 3) / 4) * ((h[k] + 3) / 4))) + 63) & ~size_t(63);
     }
     if (total) {
@@ -568,7 +575,9 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
     ctx->chain_counts_zero = false;
     bool all_zero = true;
     auto compress_one = [&](hipStream_t st, int k) -> ntx_status {
-        int *bucket = format == NTX_FORMAT_BC3 ? static_cast<int *>(ctx->chain.ptr) + region[k] : nullptr;
+        int *bucket = (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
+                          ? static_cast<int *>(ctx->chain.ptr) + region[k]
+                          : nullptr;
         bool z = zero;
         hipError_t e = compress_on(ctx, st, format, quality, alpha_mode, cw, d_in[k], w[k], h[k], d_out[k], bucket, &z,
                                    k ? 16 : 18, st == small);

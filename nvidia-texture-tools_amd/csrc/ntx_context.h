@@ -17,7 +17,7 @@ hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d
                            bool urgent = false);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
                             int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
-                            hipStream_t stream);
+                            hipStream_t stream, bool normal = false);
 hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream);
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
                              int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,

@@ -19,6 +19,7 @@ NVTT_LIB_PATH = os.path.join(LIB_DIR, "libnvtt_amd.so")
 
 # nvtt::Format / Quality / AlphaMode / MipmapFilter / WrapMode values (nvtt.h).
 FORMAT_BC1, FORMAT_BC3, FORMAT_BC4, FORMAT_BC5 = 1, 4, 6, 7
+FORMAT_BC3N = 5  # Format_BC3n (DXT5n): X in the alpha block, Y in the colour block
 QUALITY_FASTEST, QUALITY_NORMAL, QUALITY_PRODUCTION, QUALITY_HIGHEST = 0, 1, 2, 3
 DECODER_D3D10, DECODER_D3D9, DECODER_NV5X = 0, 1, 2
 METRIC_RMS_COLOR, METRIC_RMS_COLOR_ALPHA_WEIGHTED, METRIC_RMS_ALPHA = 0, 1, 2

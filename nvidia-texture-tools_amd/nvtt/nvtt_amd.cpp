@@ -793,6 +793,7 @@ struct Compressor::Private {
         case Format_BC1:  // BlockCompressor.cpp:211-217: icbc Level1 / 8 / 9 by quality
             return new HipCompressorBCn(device_input);
         case Format_BC3:
+        case Format_BC3n:  // CompressorDXT5n (CompressorDX9.cpp:179-222)
             if (co.quality == Quality_Normal || co.quality == Quality_Production) return new HipCompressorBCn(device_input);
             break;
         case Format_BC4:

@@ -1002,6 +1002,40 @@ void ntxo_bc3_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, u
     ntxo_bc3_color_block(bgra, weight_by_alpha, cw, out + 8);
 }
 
+/* CompressorDXT5n::compressBlock at Normal / Production
+ * (CompressorDX9.cpp:179-222).  Colour: a block whose texels share one green
+ * takes OptimalCompress::compressDXT1G(g) (OptimalCompressDXT.cpp:294-311:
+ * red 31, blue 0, green from OMatch6); any other goes through the weighted
+ * cluster fit on the swizzled texels (red 0xFF, G, blue 0, A) with metric
+ * (0, 1, 0).  Alpha: QuickCompress::compressDXT5A of the red channel
+ * (swizzle(4, 1, 5, 0) moves R into alpha). */
+void ntxo_bc3n_block(const uint8_t *bgra, int weight_by_alpha, uint8_t *out)
+{
+    ntxo_init();
+    uint8_t tile[64], x[16];
+    int single = 1;
+    for (int i = 0; i < 16; i++) {
+        tile[4 * i] = 0;
+        tile[4 * i + 1] = bgra[4 * i + 1];
+        tile[4 * i + 2] = 0xFF;
+        tile[4 * i + 3] = bgra[4 * i + 3];
+        x[i] = bgra[4 * i + 2];
+        single = single && bgra[4 * i + 1] == bgra[1];
+    }
+    if (single) {
+        const int g = bgra[1];
+        uint16_t c0 = (uint16_t)((31 << 11) | (g_omatch6[g][0] << 5));
+        uint16_t c1 = (uint16_t)((31 << 11) | (g_omatch6[g][1] << 5));
+        uint32_t idx = 0xaaaaaaaau;
+        if (c0 < c1) { uint16_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+        put_block(out + 8, c0, c1, idx);
+    } else {
+        const float green[4] = { 0.0f, 1.0f, 0.0f, 1.0f };
+        ntxo_bc3_color_block(tile, weight_by_alpha, green, out + 8);
+    }
+    ntxo_bc4_channel(x, out);
+}
+
 /* =========================================================================
  * Whole images
  * ========================================================================= */
@@ -1075,6 +1109,8 @@ static void do_block(job_t *j, int i)
     ntxo_gather_u8(j->w, j->h, j->data, bx, by, bgra);
     if (j->format == NTXO_FMT_BC3) {
         ntxo_bc3_block(bgra, j->alpha_mode == NTXO_ALPHA_TRANSPARENCY, j->cw, j->out + (size_t)i * 16);
+    } else if (j->format == NTXO_FMT_BC3N) {
+        ntxo_bc3n_block(bgra, j->alpha_mode == NTXO_ALPHA_TRANSPARENCY, j->out + (size_t)i * 16);
     } else if (j->format == NTXO_FMT_BC4) {
         uint8_t v[16];
         for (int k = 0; k < 16; k++) v[k] = bgra[4 * k + 2];

@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 /* nvtt::Format values (nvtt.h:77-97). */
-enum { NTXO_FMT_BC1 = 1, NTXO_FMT_BC3 = 4, NTXO_FMT_BC4 = 6, NTXO_FMT_BC5 = 7 };
+enum { NTXO_FMT_BC1 = 1, NTXO_FMT_BC3 = 4, NTXO_FMT_BC3N = 5, NTXO_FMT_BC4 = 6, NTXO_FMT_BC5 = 7 };
 /* nvtt::MipmapFilter / nvtt::WrapMode / nvtt::AlphaMode values (nvtt.h). */
 enum { NTXO_FILTER_BOX = 0, NTXO_FILTER_TRIANGLE = 1, NTXO_FILTER_KAISER = 2 };
 enum { NTXO_WRAP_CLAMP = 0, NTXO_WRAP_REPEAT = 1, NTXO_WRAP_MIRROR = 2 };
@@ -63,6 +63,7 @@ void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba);
 void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out8);
 void ntxo_bc4_channel(const uint8_t *v16, uint8_t *out8);
 void ntxo_bc4_decode(const uint8_t *blk, uint8_t *v16);
+void ntxo_bc3n_block(const uint8_t *bgra, int weight_by_alpha, uint8_t *out);
 void ntxo_bc3_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out16);
 
 /* ---- whole-image compressors (planar RGBA float input, block-raster output) ---- */

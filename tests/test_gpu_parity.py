@@ -295,3 +295,50 @@ def test_bc3_lane_split_widths(gpu, oracle, size):
         exp = oracle.compress(N.FORMAT_BC3, planar, size, size, alpha_mode=am)
         diff = np.nonzero(np.any(got.reshape(-1, 16) != exp.reshape(-1, 16), axis=1))[0]
         assert diff.size == 0, f"{diff.size} blocks differ (alpha mode {am}), first {diff[:5]}"
+
+
+@pytest.mark.parametrize("size", [(256, 256), (64, 128), (2048, 1024)] + ODD_SIZES)
+def test_bc3n_images_match_oracle(gpu, oracle, size):
+    """Format_BC3n (CompressorDXT5n): normal maps and noise (whose red, blue
+    and alpha must not leak into the colour block), both alpha modes, colour
+    weights that the format overrides with (0, 1, 0); 2048x1024 takes the
+    one-block-per-lane search, the others the lane-split one."""
+    w, h = size
+    for img in (synth.normal_map_rgba(w, h, seed=14), synth.noise_rect_rgba(w, h, seed=15, transparent=True)):
+        planar = synth.to_planar(img).reshape(-1)
+        for alpha_mode in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
+            exp = oracle.compress(N.FORMAT_BC3N, planar, w, h, alpha_mode=alpha_mode)
+            for cw in ((1, 1, 1, 1), (0.3, 0.59, 0.11, 1)):
+                got = gpu.compress(N.FORMAT_BC3N, planar, w, h, alpha_mode=alpha_mode, color_weights=cw)
+                diff = np.nonzero(np.any(got.reshape(-1, 16) != exp.reshape(-1, 16), axis=1))[0]
+                assert diff.size == 0, f"{size} am {alpha_mode} cw {cw}: {diff.size} blocks differ, first {diff[:5]}"
+
+
+def test_bc3n_single_green_blocks(gpu, oracle):
+    """Blocks whose texels share one green but differ in red, blue and alpha
+    take the single-colour path (compressDXT1G) on the device too."""
+    rng = np.random.default_rng(16)
+    w, h = 64, 64
+    img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+    for by in range(h // 4):
+        for bx in range(w // 4):
+            img[4 * by:4 * by + 4, 4 * bx:4 * bx + 4, 1] = (by * 16 + bx) % 256
+    planar = synth.to_planar(img).reshape(-1)
+    assert np.array_equal(gpu.compress(N.FORMAT_BC3N, planar, w, h), oracle.compress(N.FORMAT_BC3N, planar, w, h))
+
+
+def test_bc3n_chain_matches_per_level(gpu, oracle):
+    """A Kaiser chain of a normal map as BC3n through the chain scheduler."""
+    w, h = 128, 96
+    img = synth.normal_map_rgba(w, h, seed=17)
+    planar = synth.to_planar(img).reshape(-1)
+    sizes = N.mip_sizes(w, h)
+    d_lv = [gpu.alloc(16 * a * b) for a, b in sizes]
+    d_lv[0].upload(planar)
+    d_out = [gpu.alloc(N.compressed_size(N.FORMAT_BC3N, a, b)) for a, b in sizes]
+    gpu.compress_chain_device(N.FORMAT_BC3N, N.FILTER_KAISER, d_lv, w, h, d_out)
+    for k, (a, b) in enumerate(sizes):
+        lv = d_lv[k].download(np.float32)
+        assert np.array_equal(d_out[k].download(np.uint8), oracle.compress(N.FORMAT_BC3N, lv, a, b)), (a, b)
+    for d in d_lv + d_out:
+        d.free()

@@ -55,21 +55,26 @@ def dds_header(fmt, w, h, mips, cube, normal, container, version=7):
     pitch, pf_flags, pf_fourcc, pf_bits = 0, 0x4, 0, 0     # FOURCC
     if container == DDS10:
         pf_fourcc = fourcc("DX10")
-        if normal and fmt in (N.FORMAT_BC1, N.FORMAT_BC5):
+        if normal and fmt in (N.FORMAT_BC1, N.FORMAT_BC5, N.FORMAT_BC3N):
             pf_flags |= 0x80000000
     else:
         flags |= 0x80000                                   # LINEARSIZE
         pitch = N.compressed_size(fmt, w, h)
-        pf_fourcc = fourcc({N.FORMAT_BC1: "DXT1", N.FORMAT_BC3: "DXT5", N.FORMAT_BC4: "ATI1", N.FORMAT_BC5: "ATI2"}[fmt])
+        pf_fourcc = fourcc({N.FORMAT_BC1: "DXT1", N.FORMAT_BC3: "DXT5", N.FORMAT_BC3N: "DXT5", N.FORMAT_BC4: "ATI1",
+                            N.FORMAT_BC5: "ATI2"}[fmt])
         if normal and fmt == N.FORMAT_BC1:
             pf_flags |= 0x80000000
         if normal and fmt == N.FORMAT_BC5:
             pf_flags |= 0x80000000
             pf_bits = fourcc("A2XY")
+        if normal and fmt == N.FORMAT_BC3N:
+            pf_flags |= 0x80000000
+            pf_bits = fourcc("A2D5")
     words = [fourcc("DDS "), 124, flags, h, w, pitch, 0, count] + reserved
     words += [32, pf_flags, pf_fourcc, pf_bits, 0, 0, 0, 0, caps1, caps2, 0, 0, 0]
     if container == DDS10:
-        words += [{N.FORMAT_BC1: 71, N.FORMAT_BC3: 77, N.FORMAT_BC4: 80, N.FORMAT_BC5: 83}[fmt], 3, 4 if cube else 0, 1, 0]
+        words += [{N.FORMAT_BC1: 71, N.FORMAT_BC3: 77, N.FORMAT_BC3N: 77, N.FORMAT_BC4: 80, N.FORMAT_BC5: 83}[fmt], 3,
+                  4 if cube else 0, 1, 0]
     return struct.pack("<%dI" % len(words), *words)
 
 
@@ -169,6 +174,8 @@ CASES = [
     ("bc5-normal-kaiser", N.FORMAT_BC5, N.FILTER_KAISER, (64, 64), dict(normal=True)),
     ("bc5-normal-raw-mips", N.FORMAT_BC5, N.FILTER_BOX, (32, 48), dict(normal=True, normalize=False, container=DDS10)),
     ("bc4-single-level", N.FORMAT_BC4, N.FILTER_BOX, (96, 64), dict(mipmaps=False)),
+    ("bc3n-normal-kaiser", N.FORMAT_BC3N, N.FILTER_KAISER, (64, 48), dict(normal=True)),
+    ("bc3n-normal-dds10", N.FORMAT_BC3N, N.FILTER_TRIANGLE, (40, 40), dict(normal=True, container=DDS10)),
     ("bc1-max-level-no-header", N.FORMAT_BC1, N.FILTER_KAISER, (100, 60), dict(max_level=3, header=False)),
 ]
 
@@ -176,7 +183,7 @@ CASES = [
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_process_matches_oracle(tmp_path, oracle, case):
     _, fmt, filt, (w, h), kw = case
-    if fmt == N.FORMAT_BC5:
+    if fmt in (N.FORMAT_BC5, N.FORMAT_BC3N):
         img = synth.normal_map_rgba(w, h, seed=8)
     elif fmt == N.FORMAT_BC4:
         img = synth.single_channel_rgba(w, h, seed=9)

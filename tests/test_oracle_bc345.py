@@ -128,3 +128,43 @@ def test_gather_truncates_and_repeats_edges(oracle):
     assert np.array_equal(r[2], r[0]) and np.array_equal(r[3], r[1])  # y % 2
     assert np.all(px[:, :, 1] == 63)
     assert np.all(px[:, :, 0] == int(np.float32(255) * (np.float32(1) / np.float32(255))))
+
+
+def _bc3n_tile(px):
+    """swizzle(4, 1, 5, 3) of BGRA texels: red 0xFF, green kept, blue 0, alpha kept."""
+    t = np.array(px, np.uint8).reshape(16, 4).copy()
+    t[:, 0] = 0
+    t[:, 2] = 0xFF
+    return t
+
+
+def test_bc3n_single_green_is_the_single_colour_path(oracle):
+    """compressDXT1G(g) (OptimalCompressDXT.cpp:294-311: red 31, blue 0, green
+    from OMatch6) is compressDXT1 of (255, g, 0), since the brute-force OMatch5
+    maps 255 to (31, 31) and 0 to (0, 0): the device's BC3n front pass relies
+    on this.  Red and blue of the input do not matter."""
+    m5, _ = oracle.omatch_tables()
+    assert tuple(m5[255]) == (31, 31) and tuple(m5[0]) == (0, 0)
+    rng = np.random.default_rng(5)
+    for g in range(256):
+        px = rng.integers(0, 256, (16, 4), dtype=np.uint8)
+        px[:, 1] = g
+        got = oracle.bc3n_block(px)
+        assert np.array_equal(got[8:], oracle.bc3_color_block(_bc3n_tile(px))), g
+
+
+def test_bc3n_against_compiled_squish(oracle, ref):
+    """CompressorDXT5n at Normal (CompressorDX9.cpp:179-222): colour half = the
+    reference's own nvsquish on the swizzled texels with metric (0, 1, 0);
+    alpha half = the QuickCompress alpha block of the red channel."""
+    px = blocks.bgra_set(seed=78, count=2000)
+    checked = 0
+    for i in range(len(px)):
+        if np.all(px[i, :, 1] == px[i, 0, 1]):
+            continue  # single green: compressDXT1G, checked above
+        for wba in (0, 1):
+            got = oracle.bc3n_block(px[i], wba)
+            assert np.array_equal(got[8:], ref.bc3_color_squish(_bc3n_tile(px[i]), wba, (0, 1, 0))), i
+            assert np.array_equal(got[:8], oracle.bc4_channel(px[i, :, 2])), i
+        checked += 1
+    assert checked > 1500
