@@ -252,7 +252,8 @@ NTX_API ntx_status ntx_hip_normalize_vectors_device(ntx_hip_context *ctx, float 
 /* nvtt::Decoder values (nvtt.h:149-155). */
 enum { NTX_DECODER_D3D10 = 0, NTX_DECODER_D3D9 = 1, NTX_DECODER_NV5X = 2 };
 /* Surface::setImage2D(format, decoder, w, h, blocks) (Surface.cpp:908-1118)
- * for BC1, BC3, BC4 and BC5 with the D3D10 or D3D9 decoder: every block
+ * for BC1, BC3, BC3n (decoded as BC3, unswizzled), BC4 and BC5 with the
+ * D3D10 or D3D9 decoder: every block
  * decoded to 8-bit RGBA as nv::BlockDXT1 / BlockDXT5 / BlockATI1 / BlockATI2
  * do (BlockDXT.cpp:43-98, 200-217, 336-432, 475-566; BC4 gives R = G = B,
  * BC5 R, G and B = 0), then value / 255 into the four planes of d_rgba.

@@ -694,9 +694,12 @@ NTX_API ntx_status ntx_hip_decode_device(ntx_hip_context *ctx, int format, int d
 {
     ntx_status s = bind(ctx);
     if (s) return s;
-    if (format != NTX_FORMAT_BC1 && format != NTX_FORMAT_BC3 && format != NTX_FORMAT_BC4 && format != NTX_FORMAT_BC5)
+    if (format != NTX_FORMAT_BC1 && format != NTX_FORMAT_BC3 && format != NTX_FORMAT_BC3N && format != NTX_FORMAT_BC4 &&
+        format != NTX_FORMAT_BC5)
         return NTX_ERR_UNSUPPORTED;
     if (decoder != NTX_DECODER_D3D10 && decoder != NTX_DECODER_D3D9) return NTX_ERR_UNSUPPORTED;
+    // Surface.cpp:1058-1072: BC3n blocks decode as BC3 (BlockDXT5), unswizzled.
+    if (format == NTX_FORMAT_BC3N) format = NTX_FORMAT_BC3;
     if (w < 0 || h < 0 || (size_t(w) * size_t(h) > 0 && (!d_blocks || !d_rgba))) return NTX_ERR_INVALID_ARGUMENT;
     NTX_HIP(ctx, ntx::launch_decode(format, decoder == NTX_DECODER_D3D9 ? 1 : 0, d_blocks, w, h, d_rgba,
                                     ctx->d_tab->div255, ctx->stream));

@@ -582,7 +582,9 @@ bool Surface::setImage(InputFormat format, int w, int h, int d, const void *data
 bool Surface::setImage2D(Format format, Decoder decoder, int w, int h, const void *data)
 {
     if (w <= 0 || h <= 0 || !data) return false;
-    if (format != Format_BC1 && format != Format_BC3 && format != Format_BC4 && format != Format_BC5) return false;
+    if (format != Format_BC1 && format != Format_BC3 && format != Format_BC3n && format != Format_BC4 &&
+        format != Format_BC5)
+        return false;
     if (decoder != Decoder_D3D10 && decoder != Decoder_D3D9) return false;
     if (!detach(m)) return false;
     std::lock_guard<std::mutex> g(backend_lock());

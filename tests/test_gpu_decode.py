@@ -59,3 +59,25 @@ def test_rms_error_large(gpu, oracle, metric):
     assert got == pytest.approx(exp, rel=2e-7, abs=0)
     assert gpu.rms_error_device(metric, da, da, w, h) == 0.0
     da.free(); db.free()
+
+
+def test_bc3n_decodes_as_bc3(gpu, oracle):
+    """Surface::setImage2D(Format_BC3n) decodes the BlockDXT5 bytes as BC3,
+    with no swizzle (Surface.cpp:1058-1072): the normal's X comes back in
+    alpha, its Y in green, red 255 and blue 0 for the device's own BC3n."""
+    w, h = 64, 48
+    planar = synth.to_planar(synth.normal_map_rgba(w, h, seed=4)).reshape(-1)
+    d_src = gpu.alloc(planar.nbytes).upload(planar)
+    d_blk = gpu.alloc(N.compressed_size(N.FORMAT_BC3N, w, h))
+    gpu.compress_device(N.FORMAT_BC3N, d_src, w, h, d_blk)
+    raw = d_blk.download(np.uint8)
+    d_dec = gpu.alloc(16 * w * h)
+    gpu.decode_device(N.FORMAT_BC3N, d_blk, w, h, d_dec)
+    dec = d_dec.download(np.float32)
+    assert np.array_equal(dec, oracle.decode_image(N.FORMAT_BC3, raw, w, h))
+    n = w * h
+    assert np.all(dec[:n] == 1.0) and np.all(dec[2 * n:3 * n] == 0.0)
+    for got, src in ((dec[3 * n:], planar[:n]), (dec[n:2 * n], planar[n:2 * n])):
+        err = np.abs(got - src)
+        assert np.mean(err) < 0.02 and np.max(err) < 0.15, (np.mean(err), np.max(err))
+    d_src.free(); d_blk.free(); d_dec.free()
