@@ -5,7 +5,9 @@
  *
  *   nvtt_wrapper_test process <in.raw> <out.bin> <format> <filter> <cube> <alphaMode> <wrapMode> <normalMap>
  *                             <inGamma> <outGamma> <container> <header> <mipmaps> <maxLevel> <normalize>
- *                             [<quality> <colorDithering> <binaryAlpha>]
+ *                             [<quality> <colorDithering> <binaryAlpha> [<file>]]
+ *       <file>: nvttSetOutputOptionsFileName after the handlers, so the file
+ *       writer takes the stream (the callbacks then see nothing)
  *   nvtt_wrapper_test info    prints the version and every error string */
 #include <stdint.h>
 #include <stdio.h>
@@ -101,6 +103,7 @@ static int run_process(int argc, char **argv)
     nvttSetOutputOptionsErrorHandler(oo, on_error);
     nvttSetOutputOptionsContainer(oo, (NvttContainer)atoi(argv[12]));
     nvttSetOutputOptionsOutputHeader(oo, atoi(argv[13]) ? NVTT_True : NVTT_False);
+    if (argc >= 21 && strcmp(argv[20], "-") != 0) nvttSetOutputOptionsFileName(oo, argv[20]);
 
     NvttCompressor *c = nvttCreateCompressor();
     if (nvttIsCudaAccelerationEnabled(c) != NVTT_True) return fail("acceleration off");

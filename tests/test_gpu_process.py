@@ -244,8 +244,8 @@ def test_process_unsupported_output_format(tmp_path):
 
 def ktx_header(fmt, w, h, mips, faces):
     """KTX 1.1 header (the Khronos layout; KtxFile.h:113-131, Context.cpp:870-1030)."""
-    internal, base = {N.FORMAT_BC1: (0x83F0, 0x1907), N.FORMAT_BC3: (0x83F3, 0x1908), N.FORMAT_BC4: (0x8DBB, 0x1903),
-                      N.FORMAT_BC5: (0x8DBD, 0x8227)}[fmt]
+    internal, base = {N.FORMAT_BC1: (0x83F0, 0x1907), N.FORMAT_BC3: (0x83F3, 0x1908), N.FORMAT_BC3N: (0x83F3, 0x1908),
+                      N.FORMAT_BC4: (0x8DBB, 0x1903), N.FORMAT_BC5: (0x8DBD, 0x8227)}[fmt]
     ident = bytes([0xAB, 0x4B, 0x54, 0x58, 0x20, 0x31, 0x31, 0xBB, 0x0D, 0x0A, 0x1A, 0x0A])
     return ident + struct.pack("<13I", 0x04030201, 0, 1, 0, internal, base, w, h, 0, 0, faces, mips, 0)
 
@@ -255,6 +255,7 @@ KTX_CASES = [
     ("bc3-box-cube", N.FORMAT_BC3, N.FILTER_BOX, True, False, (16, 16)),
     ("bc5-normal-triangle", N.FORMAT_BC5, N.FILTER_TRIANGLE, False, True, (32, 32)),
     ("bc4-kaiser", N.FORMAT_BC4, N.FILTER_KAISER, False, False, (40, 24)),
+    ("bc3n-normal-box", N.FORMAT_BC3N, N.FILTER_BOX, False, True, (24, 40)),
 ]
 
 
@@ -264,7 +265,7 @@ def test_process_ktx(tmp_path, oracle, case):
     that level of every face, padding after levels past the first."""
     _, fmt, filt, cube, normal, (w, h) = case
     nf = 6 if cube else 1
-    if fmt == N.FORMAT_BC5:
+    if fmt in (N.FORMAT_BC5, N.FORMAT_BC3N):
         faces = [[synth.normal_map_rgba(w, h, seed=50 + f)] for f in range(nf)]
     elif fmt == N.FORMAT_BC4:
         faces = [[synth.single_channel_rgba(w, h, seed=50 + f)] for f in range(nf)]
@@ -353,3 +354,17 @@ def test_c_interface_dithering_ignored_for_bc4(tmp_path, oracle):
     a = run_process(tmp_path, [[img]], N.FORMAT_BC4, N.FILTER_BOX, driver=C_DRIVER, extra=(1, 1, 0))
     b = run_process(tmp_path, [[img]], N.FORMAT_BC4, N.FILTER_BOX, driver=C_DRIVER, extra=(1, 0, 0))
     assert a[3] and a[2] == [] and a[0] == b[0]
+
+
+def test_c_interface_file_output(tmp_path):
+    """nvttSetOutputOptionsFileName: the file holds exactly the stream the
+    C++ front end hands its OutputHandler (header and every level)."""
+    img = synth.noise_rect_rgba(72, 40, seed=61, transparent=True)
+    out = tmp_path / "c.dds"
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], N.FORMAT_BC3, N.FILTER_KAISER, driver=C_DRIVER,
+                                                  extra=(1, 0, 0, str(out)))
+    assert ok and errors == [] and stream == b"" and len(events) == 0
+    ref = run_process(tmp_path, [[img]], N.FORMAT_BC3, N.FILTER_KAISER)
+    data = out.read_bytes()
+    # the C++ driver sets user version 7 (reserved[8], byte 64); the C interface has no setter
+    assert data[:64] == ref[0][:64] and data[68:] == ref[0][68:] and data[64:68] == bytes(4)
