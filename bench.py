@@ -45,7 +45,7 @@ SQUISH_SETUP = 1500
 # rocprofv3 --pmc runs of `bench.py --profile-pass` summarised in
 # profiles/bc3_kaiser_chain_4096_hbm_traffic.md (the factors come from a
 # 256 MiB device copy in the same run).
-SEARCH_TRAFFIC_BYTES = {"bc3_search_kernel": round((71.60e6 + 18.57e6 + 16.25e6 + 3.76e6) / 2)}
+SEARCH_TRAFFIC_BYTES = {"bc3_search_kernel": round((71.69e6 + 18.56e6 + 16.26e6 + 3.75e6) / 2)}
 
 
 def log(*a):
