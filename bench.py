@@ -30,7 +30,7 @@ METRIC = "BC1 Mtexels/s at PSNR within 0.05 dB of ref; mip-chain HBM GB/s"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 
-# Algorithmic flops (SURVEY 8d-style accounting):
+# Algorithmic flops, counted from the reference loops each kernel restates:
 #  BC1 icbc Level8 block: 134*N4(n) + 117*N3(m) + 3500 (n >= 2), 200 otherwise,
 #  N4(n) = C(n+3,3)-1, N3(m) = C(m+2,2)-1.
 #  BC3 colour (squish Compress4): 137 flops per candidate (the loop body of
