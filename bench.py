@@ -101,7 +101,7 @@ def main():
                          "(2 steps each) and a 256 MiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build uses a single MI355X)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the kernels run on a single MI355X; sharding by texture or block band is a design note only)",
               file=sys.stderr)
         sys.exit(2)
 
