@@ -721,6 +721,8 @@ static int float_to_565(v3 c)
     return (float_to_int(31.0f * c.x, 31) << 11) | (float_to_int(63.0f * c.y, 63) << 5) | float_to_int(31.0f * c.z, 31);
 }
 
+static void squish_fit(const uint8_t *bgra, int weight_by_alpha, const float *cwp, int dxt1, uint8_t *out);
+
 void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float *cwp, uint8_t *out)
 {
     ntxo_init();
@@ -738,8 +740,18 @@ void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float 
         put_block(out, c0, c1, idx);
         return;
     }
+    squish_fit(bgra, weight_by_alpha, cwp, 0, out);
+}
+
+/* nvsquish WeightedClusterFit over ColourSet (colourfit.cpp:41-57): dxt1 set
+ * (kDxt1): texels of alpha 0 leave the set and take index 3, Compress3 runs,
+ * then Compress4 unless a texel was transparent; a later fit replaces the
+ * block only with a strictly lower error. */
+static void squish_fit(const uint8_t *bgra, int weight_by_alpha, const float *cwp, int dxt1, uint8_t *out)
+{
     v3 metric = { cwp[0], cwp[1], cwp[2] };
     v3 msq = { metric.x * metric.x, metric.y * metric.y, metric.z * metric.z };
+    int transparent = 0;
     /* colourset.cpp:35-101 with the header default createMinimalSet=true
      * (colourset.h:40): texels with identical RGB bytes share one point whose
      * weight accumulates in texel order; remap[] sends texels to points. */
@@ -749,9 +761,11 @@ void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float 
     int count = 0;
     for (int i = 0; i < 16; i++) {
         float wa = (float)(bgra[4 * i + 3] + 1) / 256.0f;
+        if (dxt1 && bgra[4 * i + 3] == 0) { remap[i] = -1; transparent = 1; continue; }
         int j;
         for (j = 0; j < i; j++)
-            if (bgra[4 * i] == bgra[4 * j] && bgra[4 * i + 1] == bgra[4 * j + 1] && bgra[4 * i + 2] == bgra[4 * j + 2]) break;
+            if (bgra[4 * i] == bgra[4 * j] && bgra[4 * i + 1] == bgra[4 * j + 1] && bgra[4 * i + 2] == bgra[4 * j + 2] &&
+                (bgra[4 * j + 3] != 0 || !dxt1)) break;
         if (j == i) {
             pts[count].x = (float)bgra[4 * i + 2] / 255.0f;
             pts[count].y = (float)bgra[4 * i + 1] / 255.0f;
@@ -819,6 +833,64 @@ void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float 
         wsum += ww[i];
     }
     for (int i = count; i < 17; i++) { wp[i].x = wp[i].y = wp[i].z = 0.0f; ww[i] = 0.0f; } /* read past the end by the reference loops; never used */
+    float m_best = FLT_MAX; /* ColourFit::m_besterror */
+    if (dxt1) {
+        /* weightedclusterfit.cpp:372-474 scalar Compress3: palette {a, b, (a+b)/2}. */
+        v3 s3 = { 0, 0, 0 }, e3 = { 0, 0, 0 };
+        float best3 = FLT_MAX;
+        int d0 = 0, d1 = 0;
+        v3 x0 = { 0, 0, 0 };
+        float w0 = 0.0f;
+        for (int c0 = 0; c0 <= count; c0++) {
+            v3 x1 = { 0, 0, 0 };
+            float w1 = 0.0f;
+            for (int c1 = 0; c1 <= count - c0; c1++) {
+                float w2 = (wsum - w0) - w1;
+                float alpha2 = w0 + w1 * 0.25f;
+                float beta2 = w2 + w1 * 0.25f;
+                float ab = w1 * 0.25f;
+                float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+                v3 ax = { x0.x + x1.x * 0.5f, x0.y + x1.y * 0.5f, x0.z + x1.z * 0.5f };
+                v3 bx = { xsum.x - ax.x, xsum.y - ax.y, xsum.z - ax.z };
+                v3 a = { (ax.x * beta2 - bx.x * ab) * factor, (ax.y * beta2 - bx.y * ab) * factor, (ax.z * beta2 - bx.z * ab) * factor };
+                v3 b = { (bx.x * alpha2 - ax.x * ab) * factor, (bx.y * alpha2 - ax.y * ab) * factor, (bx.z * alpha2 - ax.z * ab) * factor };
+                a.x = smin(1.0f, smax(0.0f, a.x)); a.y = smin(1.0f, smax(0.0f, a.y)); a.z = smin(1.0f, smax(0.0f, a.z));
+                b.x = smin(1.0f, smax(0.0f, b.x)); b.y = smin(1.0f, smax(0.0f, b.y)); b.z = smin(1.0f, smax(0.0f, b.z));
+                a.x = floorf(31.0f * a.x + 0.5f) * (1.0f / 31.0f);
+                a.y = floorf(63.0f * a.y + 0.5f) * (1.0f / 63.0f);
+                a.z = floorf(31.0f * a.z + 0.5f) * (1.0f / 31.0f);
+                b.x = floorf(31.0f * b.x + 0.5f) * (1.0f / 31.0f);
+                b.y = floorf(63.0f * b.y + 0.5f) * (1.0f / 63.0f);
+                b.z = floorf(31.0f * b.z + 0.5f) * (1.0f / 31.0f);
+                v3 e1 = {
+                    ((a.x * a.x) * alpha2 + (b.x * b.x) * beta2) + 2.0f * ((((a.x * b.x) * ab) - a.x * ax.x) - b.x * bx.x),
+                    ((a.y * a.y) * alpha2 + (b.y * b.y) * beta2) + 2.0f * ((((a.y * b.y) * ab) - a.y * ax.y) - b.y * bx.y),
+                    ((a.z * a.z) * alpha2 + (b.z * b.z) * beta2) + 2.0f * ((((a.z * b.z) * ab) - a.z * ax.z) - b.z * bx.z) };
+                float error = e1.x * msq.x + e1.y * msq.y + e1.z * msq.z;
+                if (error < best3) { best3 = error; s3 = a; e3 = b; d0 = c0; d1 = c1; }
+                int k = c0 + c1;
+                x1.x += wp[k].x; x1.y += wp[k].y; x1.z += wp[k].z; w1 += ww[k];
+            }
+            x0.x += wp[c0].x; x0.y += wp[c0].y; x0.z += wp[c0].z; w0 += ww[c0];
+        }
+        if (best3 < m_best) {
+            uint8_t sorted_idx[16], ordered[16], idx[16];
+            for (int i = 0; i < count; i++) sorted_idx[i] = (uint8_t)(i < d0 ? 0 : i < d0 + d1 ? 2 : 1);
+            for (int i = 0; i < count; i++) ordered[order[i]] = sorted_idx[i];
+            for (int i = 0; i < 16; i++) idx[i] = remap[i] < 0 ? 3 : ordered[remap[i]]; /* colourset.cpp:140-150 */
+            /* colourblock.cpp:75-106 WriteColourBlock3 (byte indices). */
+            int ca = float_to_565(s3), cb = float_to_565(e3);
+            if (ca > cb) {
+                int tt = ca; ca = cb; cb = tt;
+                for (int i = 0; i < 16; i++) idx[i] = (uint8_t)(idx[i] == 0 ? 1 : idx[i] == 1 ? 0 : idx[i]);
+            }
+            uint32_t packed = 0;
+            for (int i = 0; i < 16; i++) packed |= (uint32_t)idx[i] << (2 * i);
+            put_block(out, (uint16_t)ca, (uint16_t)cb, packed);
+            m_best = best3;
+        }
+        if (transparent) return;
+    }
     /* weightedclusterfit.cpp:476-589 scalar Compress4. */
     v3 beststart = { 0, 0, 0 }, bestend = { 0, 0, 0 };
     float besterror = FLT_MAX;
@@ -867,11 +939,14 @@ void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float 
         }
         x0.x += wp[c0].x; x0.y += wp[c0].y; x0.z += wp[c0].z; w0 += ww[c0];
     }
-    if (!(besterror < FLT_MAX)) { memset(out, 0, 8); return; } /* reference leaves the block unwritten */
+    if (!(besterror < m_best)) { /* no improvement: the block stays as written, or unwritten (zeros) */
+        if (!(m_best < FLT_MAX)) memset(out, 0, 8);
+        return;
+    }
     uint8_t sorted_idx[16], ordered[16], idx[16];
     for (int i = 0; i < count; i++) sorted_idx[i] = (uint8_t)(i < b0 ? 0 : i < b0 + b1 ? 2 : i < b0 + b1 + b2 ? 3 : 1);
     for (int i = 0; i < count; i++) ordered[order[i]] = sorted_idx[i];
-    for (int i = 0; i < 16; i++) idx[i] = ordered[remap[i]]; /* colourset.cpp:140-150 RemapIndices */
+    for (int i = 0; i < 16; i++) idx[i] = remap[i] < 0 ? 3 : ordered[remap[i]]; /* colourset.cpp:140-150 RemapIndices */
     /* colourblock.cpp:108-138 WriteColourBlock4. */
     int ca = float_to_565(beststart), cb = float_to_565(bestend);
     if (ca < cb) { int tt = ca; ca = cb; cb = tt; for (int i = 0; i < 16; i++) idx[i] = (uint8_t)((idx[i] ^ 1) & 3); }
@@ -1000,6 +1075,14 @@ void ntxo_bc3_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, u
     for (int i = 0; i < 16; i++) a[i] = bgra[4 * i + 3];
     ntxo_bc4_channel(a, out);
     ntxo_bc3_color_block(bgra, weight_by_alpha, cw, out + 8);
+}
+
+/* The squish branch of CompressorDXT1a::compressBlock (CompressorDX9.cpp:98-109):
+ * kDxt1 colour set, plus kWeightColourByAlpha when weight_by_alpha. */
+void ntxo_bc1a_squish(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out)
+{
+    ntxo_init();
+    squish_fit(bgra, weight_by_alpha, cw, 1, out);
 }
 
 /* CompressorDXT5n::compressBlock at Normal / Production

@@ -51,6 +51,18 @@ void ntxref_bc3_color_squish(const uint8_t *bgra, int weight_by_alpha, const flo
     fit.Compress(out8);
 }
 
+// CompressorDX9.cpp:98-109 -- the squish branch of CompressorDXT1a::compressBlock.
+void ntxref_bc1a_squish(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out8)
+{
+    nvsquish::WeightedClusterFit fit;
+    fit.SetMetric(cw[0], cw[1], cw[2]);
+    int flags = nvsquish::kDxt1 | (weight_by_alpha ? nvsquish::kWeightColourByAlpha : 0);
+    nvsquish::ColourSet colours(bgra, flags);
+    fit.SetColourSet(&colours, nvsquish::kDxt1);
+    for (int i = 0; i < 8; i++) out8[i] = 0;
+    fit.Compress(out8);
+}
+
 }
 
 // Whole-image BC1 with the reference encoder, for the CPU baseline.  The

@@ -95,6 +95,10 @@ class Oracle:
         blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
         self.lib.ntxo_bc1_decode(_p(blk), _p(out)); return out.reshape(16, 4)
 
+    def bc1a_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc1a_squish(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out
+
     def bc3n_block(self, bgra, weight_by_alpha=0):
         b = np.ascontiguousarray(bgra, np.uint8).reshape(64); out = np.zeros(16, np.uint8)
         self.lib.ntxo_bc3n_block(_p(b), int(weight_by_alpha), _p(out)); return out
@@ -218,6 +222,10 @@ class Reference:
     def bc1_decode(self, blk):
         blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
         self.lib.ntxref_bc1_decode(_p(blk), _p(out)); return out.reshape(16, 4)
+
+    def bc1a_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
+        self.lib.ntxref_bc1a_squish(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out
 
     def bc3_color_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
         b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)

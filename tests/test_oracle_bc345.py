@@ -168,3 +168,26 @@ def test_bc3n_against_compiled_squish(oracle, ref):
             assert np.array_equal(got[:8], oracle.bc4_channel(px[i, :, 2])), i
         checked += 1
     assert checked > 1500
+
+
+def test_bc1a_squish_against_compiled_squish(oracle, ref):
+    """The squish half of BC1a (CompressorDXT1a, kDxt1): alpha-0 texels leave
+    the colour set and take index 3, the three-colour fit, then the
+    four-colour fit for blocks without them.  Blocks with no, some and only
+    transparent texels, both alpha modes, two metrics; the reference's own
+    nvsquish decides every byte."""
+    px = blocks.bgra_set(seed=79, count=1800)
+    rng = np.random.default_rng(80)
+    kinds = 0
+    for i in range(len(px)):
+        b = px[i].copy()
+        mode = i % 3
+        if mode == 1:
+            b[rng.random(16) < 0.3, 3] = 0
+        elif mode == 2:
+            b[:, 3] = 0
+        kinds += int(np.any(b[:, 3] == 0))
+        for wba in (0, 1):
+            for cw in ((1, 1, 1), (0.3, 0.59, 0.11)):
+                assert np.array_equal(oracle.bc1a_squish(b, wba, cw), ref.bc1a_squish(b, wba, cw)), (i, wba, cw)
+    assert kinds > 1000
