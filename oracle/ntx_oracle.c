@@ -36,6 +36,7 @@ static cl3 g_three[152 + 8];
 static int g_three_total[16];
 static uint8_t g_match5[256][2], g_match6[256][2];   /* icbc s_match5/6 (D3D10 decoder) */
 static uint8_t g_omatch5[256][2], g_omatch6[256][2]; /* nvtt OMatch5/6 */
+static uint8_t g_omatcha5[256][2], g_omatcha6[256][2]; /* nvtt OMatchAlpha5/6 */
 static float g_mid5[32], g_mid6[64];
 static int g_inited = 0;
 
@@ -97,8 +98,9 @@ static void build_icbc_match(void)
     }
 }
 
-/* SingleColorLookup.cpp:34-89 PrepareOptTable (non-alpha): err = 100*|lerp13 - i| + 3*|max-min|. */
-static void build_omatch(uint8_t (*tab)[2], int bits)
+/* SingleColorLookup.cpp:34-89 PrepareOptTable: err = 100*|lerp - i| + 3*|max-min|,
+ * lerp = (2 max + min) / 3, or (max + min) / 2 for the alpha (three-colour) tables. */
+static void build_omatch_mode(uint8_t (*tab)[2], int bits, int alpha_mode)
 {
     int size = 1 << bits;
     for (int i = 0; i < 256; i++) {
@@ -107,7 +109,8 @@ static void build_omatch(uint8_t (*tab)[2], int bits)
             for (int mx = 0; mx < size; mx++) {
                 int mine = bits == 5 ? ((mn << 3) | (mn >> 2)) : ((mn << 2) | (mn >> 4));
                 int maxe = bits == 5 ? ((mx << 3) | (mx >> 2)) : ((mx << 2) | (mx >> 4));
-                int err = abs((maxe * 2 + mine) / 3 - i) * 100 + abs(mx - mn) * 3;
+                int lerp = alpha_mode ? (maxe + mine) / 2 : (maxe * 2 + mine) / 3;
+                int err = abs(lerp - i) * 100 + abs(mx - mn) * 3;
                 if (err < best) { tab[i][0] = (uint8_t)mx; tab[i][1] = (uint8_t)mn; best = err; }
             }
     }
@@ -139,8 +142,10 @@ void ntxo_init(void)
     if (g_inited) return;
     build_cluster_tables();
     build_icbc_match();
-    build_omatch(g_omatch5, 5);
-    build_omatch(g_omatch6, 6);
+    build_omatch_mode(g_omatch5, 5, 0);
+    build_omatch_mode(g_omatch6, 6, 0);
+    build_omatch_mode(g_omatcha5, 5, 1);
+    build_omatch_mode(g_omatcha6, 6, 1);
     build_midpoints();
     g_inited = 1;
 }
@@ -161,6 +166,7 @@ int ntxo_cluster_table3(uint8_t *out, int *totals)
 }
 void ntxo_icbc_match_tables(uint8_t *m5, uint8_t *m6) { ntxo_init(); memcpy(m5, g_match5, 512); memcpy(m6, g_match6, 512); }
 void ntxo_omatch_tables(uint8_t *m5, uint8_t *m6) { ntxo_init(); memcpy(m5, g_omatch5, 512); memcpy(m6, g_omatch6, 512); }
+void ntxo_omatch_alpha_tables(uint8_t *m5, uint8_t *m6) { ntxo_init(); memcpy(m5, g_omatcha5, 512); memcpy(m6, g_omatcha6, 512); }
 void ntxo_midpoints(float *m5, float *m6) { ntxo_init(); memcpy(m5, g_mid5, sizeof g_mid5); memcpy(m6, g_mid6, sizeof g_mid6); }
 
 /* =========================================================================
@@ -1083,6 +1089,36 @@ void ntxo_bc1a_squish(const uint8_t *bgra, int weight_by_alpha, const float *cw,
 {
     ntxo_init();
     squish_fit(bgra, weight_by_alpha, cw, 1, out);
+}
+
+/* CompressorDXT1a::compressBlock (CompressorDX9.cpp:83-111): the alpha-0
+ * mask; one RGB over all 16 texels -> OptimalCompress::compressDXT1a
+ * (OptimalCompressDXT.cpp:271-292: compressDXT1 without transparent texels,
+ * else the OMatchAlpha endpoints, col0 <= col1 by swapping, indices
+ * 0xaaaaaaaa | mask); otherwise the kDxt1 squish fit. */
+void ntxo_bc1a_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out)
+{
+    ntxo_init();
+    uint32_t mask = 0;
+    int single = 1;
+    for (int i = 0; i < 16; i++) {
+        if (bgra[4 * i + 3] == 0) mask |= 3u << (2 * i);
+        single = single && bgra[4 * i] == bgra[0] && bgra[4 * i + 1] == bgra[1] && bgra[4 * i + 2] == bgra[2];
+    }
+    if (!single) { squish_fit(bgra, weight_by_alpha, cw, 1, out); return; }
+    const int r = bgra[2], g = bgra[1], b = bgra[0];
+    if (mask == 0) {
+        uint16_t c0 = (uint16_t)((g_omatch5[r][0] << 11) | (g_omatch6[g][0] << 5) | g_omatch5[b][0]);
+        uint16_t c1 = (uint16_t)((g_omatch5[r][1] << 11) | (g_omatch6[g][1] << 5) | g_omatch5[b][1]);
+        uint32_t idx = 0xaaaaaaaau;
+        if (c0 < c1) { uint16_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+        put_block(out, c0, c1, idx);
+        return;
+    }
+    uint16_t c0 = (uint16_t)((g_omatcha5[r][0] << 11) | (g_omatcha6[g][0] << 5) | g_omatcha5[b][0]);
+    uint16_t c1 = (uint16_t)((g_omatcha5[r][1] << 11) | (g_omatcha6[g][1] << 5) | g_omatcha5[b][1]);
+    if (c0 > c1) { uint16_t t = c0; c0 = c1; c1 = t; }
+    put_block(out, c0, c1, 0xaaaaaaaau | mask);
 }
 
 /* CompressorDXT5n::compressBlock at Normal / Production

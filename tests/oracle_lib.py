@@ -95,6 +95,14 @@ class Oracle:
         blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
         self.lib.ntxo_bc1_decode(_p(blk), _p(out)); return out.reshape(16, 4)
 
+    def omatch_alpha_tables(self):
+        a = np.zeros(512, np.uint8); b = np.zeros(512, np.uint8)
+        self.lib.ntxo_omatch_alpha_tables(_p(a), _p(b)); return a.reshape(256, 2), b.reshape(256, 2)
+
+    def bc1a_block(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc1a_block(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out
+
     def bc1a_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
         b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
         self.lib.ntxo_bc1a_squish(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out

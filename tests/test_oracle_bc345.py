@@ -191,3 +191,47 @@ def test_bc1a_squish_against_compiled_squish(oracle, ref):
             for cw in ((1, 1, 1), (0.3, 0.59, 0.11)):
                 assert np.array_equal(oracle.bc1a_squish(b, wba, cw), ref.bc1a_squish(b, wba, cw)), (i, wba, cw)
     assert kinds > 1000
+
+
+def test_omatch_alpha_tables_brute_force(oracle):
+    """SingleColorLookup.cpp:34-89 with alpha_mode: lerp (max + min) / 2."""
+    m5, m6 = oracle.omatch_alpha_tables()
+    for bits, tab in ((5, m5), (6, m6)):
+        n = 1 << bits
+        ex = np.array([(i << 3) | (i >> 2) for i in range(n)] if bits == 5 else [(i << 2) | (i >> 4) for i in range(n)])
+        mn, mx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        lerp = (ex[mx] + ex[mn]) // 2
+        for v in range(256):
+            err = np.abs(lerp - v) * 100 + np.abs(mx - mn) * 3
+            k = int(np.argmin(err.reshape(-1)))
+            assert (tab[v, 0], tab[v, 1]) == (mx.reshape(-1)[k], mn.reshape(-1)[k])
+
+
+def test_bc1a_block_paths(oracle, ref):
+    """CompressorDXT1a: a block of one RGB with alpha-0 texels decodes to that
+    colour's three-colour midpoint with those texels transparent (index 3,
+    c0 <= c1); without them it is the four-colour single-colour block; any
+    other block is the kDxt1 squish fit."""
+    rng = np.random.default_rng(81)
+    for _ in range(300):
+        c = rng.integers(0, 256, 3)
+        b = np.zeros((16, 4), np.uint8)
+        b[:, :3] = c
+        b[:, 3] = rng.integers(0, 256, 16)
+        b[rng.random(16) < 0.4, 3] = 0
+        out = oracle.bc1a_block(b)
+        c0, c1 = int(out[0]) | int(out[1]) << 8, int(out[2]) | int(out[3]) << 8
+        idx = int(out[4]) | int(out[5]) << 8 | int(out[6]) << 16 | int(out[7]) << 24
+        if np.any(b[:, 3] == 0):
+            assert c0 <= c1
+            for i in range(16):
+                assert (idx >> (2 * i)) & 3 == (3 if b[i, 3] == 0 else 2)
+        dec = oracle.bc1_decode(out).reshape(16, 4)
+        opaque = b[:, 3] != 0
+        if opaque.any():
+            assert np.all(np.abs(dec[opaque, :3].astype(int) - c[::-1].astype(int)) <= 4)
+    px = blocks.bgra_set(seed=82, count=300)
+    for i in range(len(px)):
+        if np.all(px[i, :, :3] == px[i, 0, :3]):
+            continue
+        assert np.array_equal(oracle.bc1a_block(px[i]), ref.bc1a_squish(px[i], 0, (1, 1, 1))), i
