@@ -1066,6 +1066,11 @@ void ntxo_bc4_channel(const uint8_t *v, uint8_t *out)
     for (int i = 0; i < 8; i++) out[i] = (uint8_t)(best.u >> (8 * i));
 }
 
+void ntxo_bc4_channels(const uint8_t *v, size_t n, uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++) ntxo_bc4_channel(v + 16 * i, out + 8 * i);
+}
+
 void ntxo_bc4_decode(const uint8_t *blk, uint8_t *v)
 {
     uint64_t u = 0;

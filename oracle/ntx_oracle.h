@@ -62,6 +62,8 @@ void ntxo_bc1_decode(const uint8_t *blk, uint8_t *rgba);
  * bgra: 16 texels in nv::Color32 byte order (b,g,r,a). */
 void ntxo_bc3_color_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out8);
 void ntxo_bc4_channel(const uint8_t *v16, uint8_t *out8);
+/* n channel blocks of 16 values: ntxo_bc4_channel on each. */
+void ntxo_bc4_channels(const uint8_t *v, size_t n, uint8_t *out);
 void ntxo_bc4_decode(const uint8_t *blk, uint8_t *v16);
 void ntxo_bc1a_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out);
 void ntxo_omatch_alpha_tables(uint8_t *m5, uint8_t *m6);

@@ -95,3 +95,35 @@ def bgra_set(seed, count):
             px[:, 3] = np.where(rng.random(16) < 0.5, 0, 255)
         out[i] = px
     return out
+
+
+def channel_set(seed, count):
+    """16-value channel blocks for the BC4 / BC5 / BC3-alpha paths: uniform
+    noise, narrow clusters, two and three levels, 0/255 extremes with a middle
+    value, ramps, near-constant blocks and exact constants (kind = index % 8)."""
+    rng = np.random.default_rng(seed)
+    n = count
+    v = np.empty((n, 16), np.int32)
+    kind = np.arange(n) % 8
+    base = rng.integers(0, 256, (n, 1))
+    v[kind == 0] = rng.integers(0, 256, (np.sum(kind == 0), 16))
+    k = kind == 1
+    v[k] = base[k] + rng.integers(-12, 13, (np.sum(k), 16))
+    k = kind == 2
+    lv = rng.integers(0, 256, (np.sum(k), 2))
+    v[k] = np.take_along_axis(lv, rng.integers(0, 2, (np.sum(k), 16)), 1)
+    k = kind == 3
+    lv = rng.integers(0, 256, (np.sum(k), 3))
+    v[k] = np.take_along_axis(lv, rng.integers(0, 3, (np.sum(k), 16)), 1)
+    k = kind == 4
+    lv = np.concatenate([np.zeros((np.sum(k), 1), int), np.full((np.sum(k), 1), 255),
+                         rng.integers(0, 256, (np.sum(k), 1))], 1)
+    v[k] = np.take_along_axis(lv, rng.integers(0, 3, (np.sum(k), 16)), 1)
+    k = kind == 5
+    a, b = rng.integers(0, 256, (np.sum(k), 1)), rng.integers(0, 256, (np.sum(k), 1))
+    v[k] = a + ((b - a) * np.arange(16)[None] // 15)
+    k = kind == 6
+    v[k] = base[k] + rng.integers(0, 3, (np.sum(k), 16))
+    k = kind == 7
+    v[k] = base[k]
+    return np.clip(v, 0, 255).astype(np.uint8)

@@ -238,3 +238,83 @@ class Reference:
     def bc3_color_squish(self, bgra, weight_by_alpha=0, cw=(1, 1, 1)):
         b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(8, np.uint8)
         self.lib.ntxref_bc3_color_squish(_p(b), int(weight_by_alpha), _p(cwv), _p(out)); return out
+
+    # ---- the reference's nvimage / nvtt CPU code (oracle/ref_harness_nv.cpp)
+    KIND = {1: 0, 2: 1, 4: 2, 5: 3, 6: 4, 7: 5}  # nvtt::Format -> harness kind (BC1, BC1a, BC3, BC3n, BC4, BC5)
+
+    def colorblock_init(self, planar, w, h, x, y):
+        """ColorBlock::init(w, h, float*, x, y): 16 BGRA texels."""
+        src = f32(planar).reshape(-1); out = np.zeros(64, np.uint8)
+        self.lib.ntxref_colorblock_init(_p(src), int(w), int(h), int(x), int(y), _p(out)); return out
+
+    def bc4_channel(self, v):
+        """QuickCompress::compressDXT5A on one channel block."""
+        v = np.ascontiguousarray(v, np.uint8).reshape(16); out = np.zeros(8, np.uint8)
+        self.lib.ntxref_bc4_channel(_p(v), _p(out)); return out
+
+    def bc4_channel_optimal(self, v):
+        """OptimalCompress::compressDXT5A with its first error seed taken from (max, min)."""
+        v = np.ascontiguousarray(v, np.uint8).reshape(16); out = np.zeros(8, np.uint8)
+        self.lib.ntxref_bc4_channel_optimal(_p(v), _p(out)); return out
+
+    def block(self, fmt, quality, bgra, alpha_mode=0, cw=(1, 1, 1, 1)):
+        """One 8-bit block through the reference compressor Context.cpp picks for (fmt, quality)."""
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw)
+        out = np.zeros(8 if fmt in (1, 2, 6) else 16, np.uint8)
+        self.lib.ntxref_block(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(b), _p(out)); return out
+
+    def compress_image(self, fmt, quality, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1)):
+        """A whole image through the reference's own block loop and ParallelTaskDispatcher."""
+        src = f32(planar).reshape(-1); cwv = f32(cw)
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        out = np.zeros(nb * (8 if fmt in (1, 2, 6) else 16), np.uint8)
+        fn = self.lib.ntxref_compress_image
+        fn.restype = ctypes.c_size_t
+        n = fn(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(src), int(w), int(h), _p(out),
+               ctypes.c_size_t(out.size))
+        assert n == out.size, (n, out.size)
+        return out
+
+    def thread_count(self):
+        return int(self.lib.ntxref_thread_count())
+
+    def fast_downsample(self, planar, W, H):
+        src = f32(planar).reshape(-1); out = np.zeros(4 * max(1, W // 2) * max(1, H // 2), np.float32)
+        self.lib.ntxref_fast_downsample(int(W), int(H), _p(src), _p(out)); return out
+
+    def build_next_mipmap(self, filt, width, params, wrap, planar, W, H, alpha_mode=0):
+        src = f32(planar).reshape(-1); out = np.zeros(4 * max(1, W // 2) * max(1, H // 2), np.float32)
+        pp = f32(params) if params is not None else None
+        self.lib.ntxref_build_next_mipmap(int(filt), ctypes.c_float(width), _p(pp) if pp is not None else None,
+                                          int(wrap), int(alpha_mode), int(W), int(H), _p(src), _p(out))
+        return out
+
+    def polyphase(self, filt, width, params, src_len, dst_len):
+        pp = f32(params) if params is not None else None
+        fn = self.lib.ntxref_polyphase_kernel
+        kw = ctypes.c_float()
+        ws = fn(int(filt), ctypes.c_float(width), _p(pp) if pp is not None else None, int(src_len), int(dst_len), None,
+                ctypes.byref(kw))
+        out = np.zeros(ws * dst_len, np.float32)
+        fn(int(filt), ctypes.c_float(width), _p(pp) if pp is not None else None, int(src_len), int(dst_len), _p(out),
+           None)
+        return out.reshape(dst_len, ws), kw.value
+
+    def powf(self, which, x):
+        """nvmath powf_5_11 (which=0) / powf_11_5 (which=1) over an array."""
+        a = f32(x).copy(); self.lib.ntxref_powf_array(int(which), _p(a), int(a.size)); return a
+
+    def planar_gamma(self, planar, gamma, to_gamma):
+        """FloatImage::toLinear / toGamma(0, 3, gamma) of planar RGBA."""
+        a = f32(planar).reshape(-1).copy()
+        self.lib.ntxref_gamma(int(bool(to_gamma)), ctypes.c_size_t(a.size // 4), _p(a), ctypes.c_float(gamma))
+        return a
+
+    def normalize(self, planar):
+        """FloatImage::normalize(0) of planar RGBA."""
+        a = f32(planar).reshape(-1).copy(); self.lib.ntxref_normalize(ctypes.c_size_t(a.size // 4), _p(a)); return a
+
+    def decode_block(self, fmt, blk, d3d9=False):
+        """BlockDXT1/DXT5/ATI1/ATI2::decodeBlock: 16 BGRA texels."""
+        b = np.ascontiguousarray(blk, np.uint8); out = np.zeros(64, np.uint8)
+        self.lib.ntxref_decode_block(int(fmt), int(bool(d3d9)), _p(b), _p(out)); return out.reshape(16, 4)

@@ -1,7 +1,9 @@
 """Pins the BC3/BC4/BC5 parts of the oracle: squish colour against the
-reference's own nvsquish build and its golden vectors; QuickCompress (whose
-reference source needs the cmake-generated nvconfig.h and is not built here)
-through closed forms and invariants."""
+reference's own nvsquish build and its golden vectors; QuickCompress and the
+8-bit block compressors against golden vectors written by the reference's
+own compiled code (tests/golden/bc4_channels.npz, bc8bit_blocks.npz; the live
+comparisons are in test_oracle_vs_reference.py), plus closed forms and
+invariants."""
 import os
 
 import numpy as np
@@ -16,6 +18,34 @@ def test_bc3_color_golden_vectors(oracle):
     for i in range(len(g["expected"])):
         out = oracle.bc3_color_block(g["bgra"][i], g["weight_by_alpha"][i])
         assert np.array_equal(out, g["expected"][i]), i
+
+
+def test_bc4_channel_golden_vectors(oracle):
+    """QuickCompress::compressDXT5A as the reference's compiled code wrote it."""
+    g = np.load(os.path.join(GOLD, "bc4_channels.npz"))
+    for i in range(len(g["channels"])):
+        assert np.array_equal(oracle.bc4_channel(g["channels"][i]), g["quick"][i]), i
+
+
+def test_8bit_block_golden_vectors(oracle):
+    """CompressorDXT5 / CompressorDXT5n / CompressorDXT1a at Normal and
+    FastCompressorBC4 / BC5 (compressBlock of the reference's compiled code)
+    against the oracle's block functions."""
+    g = np.load(os.path.join(GOLD, "bc8bit_blocks.npz"))
+    px, am = g["bgra"], g["alpha_mode"]
+    ones = (1, 1, 1)
+    for i in range(len(px)):
+        a = int(am[i])
+        out16 = np.zeros(16, np.uint8)
+        oracle.lib.ntxo_bc3_block(px[i].ctypes.data_as(__import__("ctypes").c_void_p), a,
+                                  np.ones(3, np.float32).ctypes.data_as(__import__("ctypes").c_void_p),
+                                  out16.ctypes.data_as(__import__("ctypes").c_void_p))
+        assert np.array_equal(out16, g["bc3_normal"][i]), ("bc3", i)
+        assert np.array_equal(oracle.bc3n_block(px[i], a), g["bc3n_normal"][i]), ("bc3n", i)
+        assert np.array_equal(oracle.bc1a_block(px[i], a, ones), g["bc1a_normal"][i]), ("bc1a", i)
+        assert np.array_equal(oracle.bc4_channel(px[i, :, 2]), g["bc4_normal"][i]), ("bc4", i)
+        assert np.array_equal(np.concatenate([oracle.bc4_channel(px[i, :, 2]), oracle.bc4_channel(px[i, :, 1])]),
+                              g["bc5_normal"][i]), ("bc5", i)
 
 
 def test_bc3_color_against_compiled_squish(oracle, ref):

@@ -1,6 +1,9 @@
-"""Pins the oracle's mip filters (FloatImage / Filter, not buildable from the
-reference without its cmake step) by closed forms, library routines
-(numpy i0 / sinc) and exact-arithmetic inputs."""
+"""Pins the oracle's mip filters (FloatImage / Filter) by golden CRCs written by
+the reference's own compiled FloatImage (the live bit-for-bit comparisons are
+in test_oracle_vs_reference.py), closed forms, library routines (numpy i0 /
+sinc) and exact-arithmetic inputs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -141,3 +144,33 @@ def test_srgb_to_linear_closed_form(oracle):
     exp = np.where(f < 0.04045, f / 12.92, ((f + 0.055) / 1.055) ** 2.4)
     exp = np.where(v > 1, 1.0, exp)
     assert np.allclose(got, exp, rtol=3e-7, atol=1e-9)
+
+
+def test_mip_chain_golden_crcs(oracle):
+    """Every level of a 37x23 chain to 1x1, CRC-32 of the FP32 bytes as the
+    reference's FloatImage wrote them (tests/golden/mip_chains.npz): Box (fast
+    and polyphase), Triangle, Kaiser x three wrap modes x both alpha modes
+    (AlphaMode_Transparency always takes the polyphase resize)."""
+    import zlib
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mip_chains.npz"))
+    filters = {"f0_w0.5": (0, 0.5, None), "f0_w1.0": (0, 1.0, None), "f1_w1.0": (1, 1.0, None),
+               "f2_w3.0": (2, 3.0, (4.0, 1.0))}
+    seen = 0
+    for key in g.files:
+        if not key.startswith("f"):
+            continue
+        fw, wrap, am = key.rsplit("_", 2)
+        filt, width, params = filters[fw]
+        wrap, am = int(wrap[4:]), int(am[1:])
+        p, W, H = g["image"], int(g["width"]), int(g["height"])
+        crcs = [zlib.crc32(p.tobytes())]
+        while (W, H) != (1, 1):
+            if am:
+                p = oracle.downsample(filt, width, params, wrap, p, W, H)
+            else:
+                p = oracle.build_next_mipmap(filt, width, params, wrap, p, W, H)
+            W, H = max(1, W // 2), max(1, H // 2)
+            crcs.append(zlib.crc32(p.tobytes()))
+        assert np.array_equal(np.array(crcs, np.uint32), g[key]), key
+        seen += 1
+    assert seen == 24

@@ -1,0 +1,187 @@
+"""Pins the oracle against the reference's own CPU code, compiled from the
+reference sources where they lie into oracle/_ref/libntxref.so by
+oracle/Makefile (with the reference's project/vc2017/nvconfig.h and the pinned
+ICBC_SIMD=2, SQUISH_USE_SSE=0):
+
+* QuickCompress::compressDXT5A (QuickCompressDXT.cpp:505-592, 643-647,
+  772-821): the BC4 / BC5 / BC3-alpha refinement trajectory, byte for byte
+  on a million channel blocks;
+* ColorBlock::init(float) (ColorBlock.cpp:80-110): the truncating gather and
+  the edge repetition of partial blocks;
+* FloatImage::fastDownSample / downSample and PolyphaseKernel
+  (FloatImage.cpp:559-808, 877-925, 1115-1176; Filter.cpp:50-131, 157-175,
+  257-271, 563-608): FP32 bit patterns for every filter, wrap mode and odd or
+  one-dimensional size, and the weight tables;
+* the block compressors Context.cpp:1038-1115 picks (CompressorDX9.cpp,
+  CompressorDX10.cpp) through the reference's own image loop
+  (BlockCompressor.cpp:60-205) and ParallelTaskDispatcher;
+* CompressorDXT1a::compressBlock (CompressorDX9.cpp:83-111) with
+  OptimalCompress::compressDXT1a and the OMatchAlpha tables;
+* the BlockDXT decoders as Surface::setImage2D calls them (Surface.cpp:1022-1107);
+* nvmath powf_5_11 / powf_11_5, FloatImage::toLinear / toGamma / normalize.
+
+Skipped when oracle/_ref was not built (reference tree absent)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import blocks
+from ntx_amd import synth
+
+FMT_BC1, FMT_BC1A, FMT_BC3, FMT_BC3N, FMT_BC4, FMT_BC5 = 1, 2, 4, 5, 6, 7
+
+
+def test_bc4_quickcompress_million_channels(oracle, ref):
+    n = 1 << 20
+    v = blocks.channel_set(20261203, n)
+    got = np.zeros((n, 8), np.uint8)
+    exp = np.zeros((n, 8), np.uint8)
+    oracle.lib.ntxo_bc4_channels(ctypes.c_void_p(v.ctypes.data), ctypes.c_size_t(n), ctypes.c_void_p(got.ctypes.data))
+    ref.lib.ntxref_bc4_channels(ctypes.c_void_p(v.ctypes.data), ctypes.c_size_t(n), 0, ctypes.c_void_p(exp.ctypes.data))
+    bad = np.flatnonzero(np.any(got != exp, axis=1))
+    assert bad.size == 0, f"{bad.size} of {n} channel blocks differ, first {v[bad[0]].tolist()}"
+
+
+def test_colorblock_gather_partial_blocks(oracle, ref):
+    """Every block of every image size 1..9 x 1..9, with values below 0,
+    above 1, exactly on the 1/255 steps and non-finite."""
+    rng = np.random.default_rng(11)
+    for h in range(1, 10):
+        for w in range(1, 10):
+            x = (rng.random(4 * w * h) * 1.4 - 0.2).astype(np.float32)
+            x[::7] = np.round(x[::7] * 255) / np.float32(255)
+            x[5::97] = np.inf
+            x[9::101] = -np.inf
+            for by in range((h + 3) // 4):
+                for bx in range((w + 3) // 4):
+                    a = np.zeros(64, np.uint8)
+                    oracle.lib.ntxo_gather_u8(w, h, ctypes.c_void_p(x.ctypes.data), bx, by, ctypes.c_void_p(a.ctypes.data))
+                    assert np.array_equal(a, ref.colorblock_init(x, w, h, 4 * bx, 4 * by)), (w, h, bx, by)
+
+
+MIP_SIZES = [(2, 1), (1, 2), (3, 1), (1, 3), (2, 2), (3, 3), (5, 8), (8, 5), (7, 7), (37, 23), (64, 64), (63, 65),
+             (128, 1), (1, 129), (257, 130)]
+FILTERS = [(0, 0.5, None), (0, 1.0, None), (1, 1.0, None), (2, 3.0, (4.0, 1.0)), (2, 3.0, (2.0, 1.5)), (1, 2.0, None)]
+
+
+@pytest.mark.parametrize("filt,width,params", FILTERS)
+def test_mip_levels_bit_exact(oracle, ref, filt, width, params):
+    """Surface::buildNextMipmap with AlphaMode_None: fastDownSample for a Box
+    of width 0.5, FloatImage::downSample otherwise."""
+    rng = np.random.default_rng(3 + filt)
+    for W, H in MIP_SIZES:
+        x = rng.random(4 * W * H).astype(np.float32)
+        for wrap in range(3):
+            a = oracle.build_next_mipmap(filt, width, params, wrap, x, W, H)
+            b = ref.build_next_mipmap(filt, width, params, wrap, x, W, H)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (W, H, wrap)
+
+
+def test_transparency_mips_take_the_polyphase_path(oracle, ref):
+    """AlphaMode_Transparency: downSample(filter, wrap, 3), alpha first; a Box
+    of width 0.5 goes through the polyphase resize, not fastDownSample."""
+    rng = np.random.default_rng(9)
+    for W, H in [(5, 8), (37, 23), (64, 64), (3, 1)]:
+        x = rng.random(4 * W * H).astype(np.float32)
+        for filt, width, params in FILTERS[:4]:
+            for wrap in range(3):
+                a = oracle.downsample(filt, width, params, wrap, x, W, H)
+                b = ref.build_next_mipmap(filt, width, params, wrap, x, W, H, alpha_mode=1)
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (W, H, filt, wrap)
+
+
+def test_polyphase_weights_bit_exact(oracle, ref):
+    for filt, width, params in FILTERS:
+        for s in (1, 2, 3, 5, 7, 37, 64, 255, 4096):
+            d = max(1, s // 2)
+            a = oracle.polyphase(filt, width, params, s, d)
+            b, _ = ref.polyphase(filt, width, params, s, d)
+            assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32)), (filt, width, s)
+
+
+@pytest.mark.parametrize("fmt", [FMT_BC1, FMT_BC3, FMT_BC3N, FMT_BC4, FMT_BC5])
+def test_images_through_the_reference_image_loop(oracle, ref, fmt):
+    """Whole images: the oracle's image compressor against the reference's
+    ColorBlockCompressor / FloatColorCompressor loop on its thread pool, both
+    alpha modes, partial edge blocks."""
+    qualities = (0, 1, 2) if fmt == FMT_BC1 else (1,)
+    inputs = [(synth.noise_rect_rgba(64, 64, seed=3, transparent=True), 64, 64),
+              (synth.noise_rect_rgba(37, 23, seed=4, transparent=True), 37, 23),
+              (synth.normal_map_rgba(48, 20, seed=5), 48, 20), (synth.single_channel_rgba(5, 3, seed=6), 5, 3)]
+    for img, w, h in inputs:
+        pl = synth.to_planar(img).reshape(-1)
+        for am in (0, 1):
+            for q in qualities:
+                a = oracle.compress_q(fmt, q, pl, w, h, alpha_mode=am)
+                b = ref.compress_image(fmt, q, pl, w, h, alpha_mode=am)
+                assert np.array_equal(a, b), (fmt, w, h, am, q)
+
+
+def test_bc1a_blocks_against_the_reference_compressor(oracle, ref):
+    """CompressorDXT1a::compressBlock: squish with kDxt1 for several colours,
+    OptimalCompress::compressDXT1a with the alpha-0 mask for single colours
+    (with and without transparent texels), both alpha modes, two metrics."""
+    rng = np.random.default_rng(5)
+    px = blocks.bgra_set(seed=31, count=1500)
+    extra = []
+    for i in range(3000):
+        b = rng.integers(0, 256, (16, 4)).astype(np.uint8)
+        k = i % 4
+        if k == 0:
+            b[:, 3] = np.where(rng.random(16) < 0.3, 0, 255)
+        elif k == 1:
+            b[:] = b[0]
+            b[:, 3] = np.where(rng.random(16) < 0.4, 0, rng.integers(1, 256))
+        elif k == 2:
+            b[:, :3] = b[0, :3]
+            b[:, 3] = rng.choice([0, 128, 255], 16)
+        else:
+            b[:] = b[0]
+            b[:, 3] = 0
+        extra.append(b)
+    allpx = np.concatenate([px, np.stack(extra)])
+    for i in range(len(allpx)):
+        for am in (0, 1):
+            for cw in ((1, 1, 1, 1), (0.3, 0.59, 0.11, 1)):
+                assert np.array_equal(oracle.bc1a_block(allpx[i], am, cw[:3]), ref.block(FMT_BC1A, 1, allpx[i], am, cw)), i
+
+
+def test_decoders_as_setimage2d_calls_them(oracle, ref):
+    """Surface::setImage2D passes d3d9 = false to the BC1 and BC3 decoders
+    whatever the decoder, and decoder == Decoder_D3D9 to BC4 and BC5."""
+    rng = np.random.default_rng(12)
+    for fmt, size in ((FMT_BC1, 8), (FMT_BC3, 16), (FMT_BC4, 8), (FMT_BC5, 16)):
+        for d3d9 in (0, 1):
+            for _ in range(3000):
+                blk = rng.integers(0, 256, size).astype(np.uint8)
+                exp = ref.decode_block(fmt, blk, d3d9 if fmt in (FMT_BC4, FMT_BC5) else 0)[:, [2, 1, 0, 3]]
+                dec = oracle.decode_image(fmt, blk, 4, 4, d3d9).reshape(4, 16).T
+                got = np.round(dec * 255).astype(np.int32)
+                if fmt == FMT_BC4:  # setImage2D writes only red for BC4 (G = B = 0, A = 1)
+                    assert np.array_equal(got[:, 0], exp[:, 0].astype(np.int32))
+                elif fmt == FMT_BC5:
+                    assert np.array_equal(got[:, :2], exp[:, :2].astype(np.int32))
+                else:
+                    assert np.array_equal(got, exp.astype(np.int32)), (fmt, d3d9, blk.tolist())
+
+
+def test_gamma_transforms_bit_exact(oracle, ref):
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.random(100000).astype(np.float32), (10.0 ** rng.uniform(-15, 15, 40000)).astype(np.float32),
+                        np.array([0, -0.0, 1e-45, 1e-38, -1, np.inf, 1, 2, 0.5], np.float32)])
+    for which, fn in ((0, oracle.lib.ntxo_powf_5_11), (1, oracle.lib.ntxo_powf_11_5)):
+        exp = ref.powf(which, x)
+        got = np.array([fn(float(v)) for v in x], np.float32)
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), which
+    pl = (rng.random(4 * 30000) * 1.1 - 0.05).astype(np.float32)
+    for g in (2.2, 1.0, 1.8, 2.4):
+        assert np.array_equal(ref.planar_gamma(pl, g, False).view(np.uint32), oracle.planar_to_linear(pl, g).view(np.uint32)), g
+        assert np.array_equal(ref.planar_gamma(pl, g, True).view(np.uint32), oracle.planar_to_gamma(pl, g).view(np.uint32)), g
+
+
+def test_normalize_bit_exact(oracle, ref):
+    rng = np.random.default_rng(10)
+    v = (rng.random(4 * 20000) * 2 - 1).astype(np.float32)
+    v[:400] = 0
+    assert np.array_equal(ref.normalize(v).view(np.uint32), oracle.normalize_vectors(v).view(np.uint32))
