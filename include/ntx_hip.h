@@ -112,7 +112,9 @@ enum {
     NTX_KERNEL_BC3_SEARCH_SPLIT = 14, /* BC3 colour search of a small image: 2..64 lanes per block */
     NTX_KERNEL_GAMMA = 15,       /* gamma transforms of R, G, B (one launch per up to 16 levels) */
     NTX_KERNEL_NORMALS = 16,     /* normal-map renormalisation of a generated level */
-    NTX_KERNEL_COUNT = 17
+    NTX_KERNEL_BC1_CLASSIFY = 17, /* BC1 pass 1 (large images): colour counts, 0/1-colour blocks, bucketing */
+    NTX_KERNEL_BC1_SEARCH = 18,  /* BC1 pass 2: cluster fit, one block per lane, over the buckets */
+    NTX_KERNEL_COUNT = 19
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -140,6 +142,14 @@ NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int qualit
 NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
                                            const float *color_weights, const float *d_rgba, int w, int h,
                                            void *d_out);
+/* BC1 at icbc Level8 (Normal / Highest) has two encoders with identical
+ * output: one block per 16-lane group in one pass, and, for images of at
+ * least `min_blocks` 4x4 blocks, two passes with one block per lane (colour
+ * counts, then the cluster fit over blocks bucketed by count).  Sets the
+ * context's threshold; 0 restores the default (2^17 blocks).  No
+ * counterpart in the reference. */
+NTX_API ntx_status ntx_hip_set_bc1_lanes_min(ntx_hip_context *ctx, int min_blocks);
+
 /* BC1 only: also writes, per block, the colour count after reduce_colors (n)
  * and the count seen by the three-colour fit (m, -1 if not run) as int16
  * pairs; these fix the algorithmic flop count of each block. */

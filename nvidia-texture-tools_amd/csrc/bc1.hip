@@ -2,9 +2,19 @@
 // (icbc v1.05 compress_dxt1 at Quality_Level8; Level9 for Quality_Production,
 // Level1 for Quality_Fastest), icbc.h:3485-3559.
 //
-// Mapping: one 4x4 block per 16-lane group, four blocks per wave64, sixteen
-// per 256-thread workgroup; the per-block device code is in bc1_block.hpp.
-#include "bc1_block.hpp"
+// Two mappings:
+//  * bc1_kernel: one 4x4 block per 16-lane group, four blocks per wave64,
+//    sixteen per 256-thread workgroup (bc1_block.hpp); used for small
+//    images, the other quality levels and as the reference form.
+//  * bc1_classify_kernel + bc1_search_kernel (Level8, large images): one
+//    block per lane.  The classify pass reduces each block's colours,
+//    finishes blocks of 0 or 1 colours and appends the others to the list of
+//    their colour count n; the search pass takes whole wavefronts from one
+//    list, so every lane runs the same cluster-table loops (bc1_lane.hpp)
+//    with scalar loop control, no cross-lane traffic and, for unit-weight
+//    blocks, the candidates' weight terms from tables.
+#include "bc1_lane.hpp"
+#include "bucket.hpp"
 
 namespace {
 
@@ -34,6 +44,134 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
     }
 }
 
+// Level8 pass 1, one block per lane: FloatColorCompressor's gather and
+// reduce_colors; a block of no colour (every weight 0) is the zero block, a
+// block of one colour compress_dxt1_single_color_optimal (icbc.h:3515-3525),
+// every other block goes to list n.  The clusters live in LDS only while
+// they are counted.
+__global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                           int nblocks, int alpha_mode,
+                                                           const ntx::Tables *__restrict__ tab,
+                                                           uint2 *__restrict__ out, short2 *__restrict__ nm,
+                                                           int *__restrict__ counts, int *__restrict__ lists, int cap,
+                                                           int rows16, int urgent)
+{
+    using namespace ntx_bucket;
+    if (urgent) __builtin_amdgcn_s_setprio(3);
+    __shared__ float4 s_rows[4][16][64];
+    __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
+    const int t = threadIdx.x;
+    if (t < kCountSlots) s_cnt[t] = 0;
+    __syncthreads();
+    const int blk = blockIdx.x * blockDim.x + t;
+    int k = 0, local = -1;
+    if (blk < nblocks) {
+        float4 tx[16];
+        lane_gather(src, w, h, blk % bw, blk / bw, alpha_mode, rows16 != 0, tx);
+        bool any_black;
+        LaneRows rows = s_rows[t >> 6];
+        const int lane = t & 63;
+        const int n = lane_reduce(rows, lane, tx, any_black);
+        if (n <= 1) {
+            uint2 r = make_uint2(0u, 0u);
+            if (n == 1) {
+                const float4 c = rows[0][lane];
+                r = single_color_block(tab, c.x, c.y, c.z);
+            }
+            out[blk] = r;
+            if (nm) nm[blk] = make_short2(short(n), short(-1));
+        } else {
+            k = n;
+            local = atomicAdd(&s_cnt[n], 1);
+        }
+    }
+    append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
+}
+
+// Level8 pass 2, one block per lane (icbc.h:3292-3326 + :3543-3550).  Wave
+// b takes 64 consecutive entries of one list, the lists walked from n = 16
+// down; lanes past the end of their list repeat its first block and write
+// nothing.  The
+// block is gathered and reduced again (the same n), its prefix-sum table
+// built in LDS, the four-colour fit searched, then (after skip_blacks when
+// some cluster is black) the three-colour fit, and the better block kept.
+// The last workgroup clears the tallies.
+__global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                        int alpha_mode, float cwx, float cwy, float cwz,
+                                                        const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
+                                                        short2 *__restrict__ nm, int *__restrict__ counts,
+                                                        const int *__restrict__ lists, int cap, int rows16)
+{
+    using namespace ntx_bucket;
+    __shared__ float4 s_rows[17][64];
+    const int lane = threadIdx.x;
+    int wv = blockIdx.x, k = 0, first = 0, cnt = 0;
+    for (int b = 16; b >= 2; b--) {
+        const int c = counts[b];
+        const int waves = (c + 63) >> 6;
+        if (k == 0) {
+            if (wv < waves) { k = b; first = wv * 64; cnt = c; }
+            else wv -= waves;
+        }
+    }
+    if (k == 0) {
+        release_counts(counts);
+        return;
+    }
+    const int item = first + lane;
+    const bool live = item < cnt;
+    // Lanes past the end of the list encode the wave's first block again
+    // (same count, valid weights) and discard it.
+    const int blk = lists[size_t(k) * cap + (live ? item : first)];
+    float4 tx[16];
+    lane_gather(src, w, h, blk % bw, blk / bw, alpha_mode, rows16 != 0, tx);
+    bool unit = true;
+#pragma unroll
+    for (int i = 0; i < 16; i++) unit = unit && tx[i].w == 1.0f;
+    // Every live lane's texels weigh 1: cluster weights are integer counts.
+    const bool units = __builtin_amdgcn_ballot_w64(live && !unit) == 0;
+
+    bool any_black;
+    const int n = lane_reduce(s_rows, lane, tx, any_black);  // == k in live lanes
+    const bool rebuild = any_black && lane_has_black_cluster(s_rows, lane, n);
+    const float4 sum = lane_sat(s_rows, lane, n);
+    const float mx = cwx * cwx, my = cwy * cwy, mz = cwz * cwz;
+    const float *mid5 = tab->mid5, *mid6 = tab->mid6, *div255 = tab->div255;
+
+    // Four colours over the block's n clusters (n is k in every live lane).
+    const Best b4 = lane_search4(s_rows, lane, k, sum, mx, my, mz, tab, units);
+    uint2 result;
+    float best = lane_output<true>(lane_winner<true>(s_rows, lane, b4, sum, mx, my, mz, tab), tx, cwx, cwy, cwz, mid5,
+                                   mid6, div255, &result);
+
+    // Three colours: the same clusters, or the non-black ones.
+    int m = n;
+    float4 sum3 = sum;
+    if (rebuild) {
+        bool unused;
+        lane_reduce(s_rows, lane, tx, unused);
+        m = lane_skip_blacks(s_rows, lane, n);
+        if (m > 0) sum3 = lane_sat(s_rows, lane, m);
+    }
+    const bool run3 = m > 0;
+    const bool units3 = units && __builtin_amdgcn_ballot_w64(live && run3 && sum3.w != 16.0f) == 0;
+    const bool uniform3 = __builtin_amdgcn_ballot_w64(live && m != k) == 0;
+    if (run3) {
+        const Best b3 = uniform3 ? lane_search3(s_rows, lane, k, sum3, mx, my, mz, tab, units3)
+                                 : lane_search3(s_rows, lane, m, sum3, mx, my, mz, tab, units3);
+        uint2 r3;
+        const float e3 = lane_output<false>(lane_winner<false>(s_rows, lane, b3, sum3, mx, my, mz, tab), tx, cwx, cwy,
+                                            cwz, mid5, mid6, div255, &r3);
+        if (e3 < best) { best = e3; result = r3; }
+    }
+    if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
+    if (live) {
+        out[blk] = result;
+        if (nm) nm[blk] = make_short2(short(n), short(run3 ? m : -1));
+    }
+    release_counts(counts);
+}
+
 }  // namespace
 
 namespace ntx {
@@ -54,5 +192,36 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
                        urgent ? 1 : 0);
     return hipGetLastError();
 }
+
+// Level8 pass 1 over the block lists in `counts` / `lists` (zero tallies on
+// entry; lists: 17 x cap ints, cap >= the image's block count).
+hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
+                               int16_t *d_nm, int *d_counts, int *d_lists, int cap, hipStream_t stream, bool urgent)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
+    if (nblocks == 0) return hipSuccess;
+    const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
+    hipLaunchKernelGGL(bc1_classify_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
+                       alpha_mode, d_tab, reinterpret_cast<uint2 *>(d_out), reinterpret_cast<short2 *>(d_nm), d_counts,
+                       d_lists, cap, rows16, urgent ? 1 : 0);
+    return hipGetLastError();
+}
+
+// Level8 pass 2 over the lists pass 1 filled; leaves the tallies zero.
+hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
+                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
+    if (nblocks == 0) return hipSuccess;
+    const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
+    // Each list fills whole wavefronts: at most one partial wave per list.
+    const int waves = (nblocks + 63) / 64 + (ntx_bucket::kCountSlots - 2);
+    hipLaunchKernelGGL(bc1_search_kernel, dim3(waves), dim3(64), 0, stream, d_src, w, h, bw, alpha_mode, cw[0], cw[1],
+                       cw[2], d_tab, reinterpret_cast<uint2 *>(d_out), reinterpret_cast<short2 *>(d_nm), d_counts,
+                       d_lists, cap, rows16);
+    return hipGetLastError();
+}
+
+int bc1_scratch_ints(int nblocks) { return ntx_bucket::kListsOffset + ntx_bucket::kCountSlots * nblocks; }
 
 }  // namespace ntx

@@ -96,43 +96,55 @@ struct Cand {
     float ax, ay, az, bx, by, bz;
 };
 
-// One least-squares candidate, icbc.h:2501-2538 (four colours) or
-// :2201-2236 (three colours).  sat is the zero-led table: entry 0 gives the
-// zeros icbc uses for an absent cluster, and the FLT_MAX entries past the
-// colour count are read exactly as the SSE build reads them.
+// The cluster-weight terms of one least-squares candidate (alpha2, beta2,
+// alphabeta, 1 / (alpha2 beta2 - alphabeta^2)), icbc.h:2501-2511 (four
+// colours: x0, x1, x2 = prefix sums at c0 <= c1 <= c2) or :2201-2208 (three
+// colours: x0, x1 at c0 <= c1; x2 unused).  Only the .w fields are read.  The
+// host builds the same terms for integer weights (ntx_tables.cpp) with these
+// operations.
 template <bool FOUR>
-__device__ __forceinline__ float eval_candidate(const float4 *sat, uint32_t packed, float4 sum, float mx, float my,
-                                                float mz, Cand *out)
+__device__ __forceinline__ float4 cluster_terms(float4 x0, float4 x1, float4 x2, float sumw)
 {
-    const float4 x0 = sat[packed & 0xff];
-    const float4 x1 = sat[(packed >> 8) & 0xff];
-    float alpha2, beta2, ab, axr, axg, axb;
+    float alpha2, beta2, ab;
     if (FOUR) {
-        const float4 x2 = sat[(packed >> 16) & 0xff];
-        const float w3 = sum.w - x2.w;
-        const float x2r = x2.x - x1.x, x2g = x2.y - x1.y, x2b = x2.z - x1.z;
-        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
+        const float w3 = sumw - x2.w;
         const float w2 = x2.w - x1.w;
         const float w1 = x1.w - x0.w;
         const float w0 = x0.w;
         alpha2 = w2 * (1.0f / 9.0f) + (w1 * (4.0f / 9.0f) + w0);
         beta2 = w1 * (1.0f / 9.0f) + (w2 * (4.0f / 9.0f) + w3);
         ab = (w1 + w2) * (2.0f / 9.0f);
-        axr = x2r * (1.0f / 3.0f) + (x1r * (2.0f / 3.0f) + x0.x);
-        axg = x2g * (1.0f / 3.0f) + (x1g * (2.0f / 3.0f) + x0.y);
-        axb = x2b * (1.0f / 3.0f) + (x1b * (2.0f / 3.0f) + x0.z);
     } else {
-        const float w2 = sum.w - x1.w;
-        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
+        const float w2 = sumw - x1.w;
         const float w1 = x1.w - x0.w;
         ab = w1 * 0.25f;
         alpha2 = x0.w + ab;
         beta2 = w2 + ab;
+    }
+    return make_float4(alpha2, beta2, ab, 1.0f / (alpha2 * beta2 - ab * ab));
+}
+
+// The rest of the candidate (icbc.h:2513-2538 / :2210-2236): endpoints,
+// snap to 5:6:5, error under the squared metric (mx, my, mz).  t holds
+// cluster_terms of the same prefix sums.
+template <bool FOUR>
+__device__ __forceinline__ float eval_terms(float4 x0, float4 x1, float4 x2, float4 sum, float4 t, float mx, float my,
+                                            float mz, Cand *out)
+{
+    const float alpha2 = t.x, beta2 = t.y, ab = t.z, factor = t.w;
+    float axr, axg, axb;
+    if (FOUR) {
+        const float x2r = x2.x - x1.x, x2g = x2.y - x1.y, x2b = x2.z - x1.z;
+        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
+        axr = x2r * (1.0f / 3.0f) + (x1r * (2.0f / 3.0f) + x0.x);
+        axg = x2g * (1.0f / 3.0f) + (x1g * (2.0f / 3.0f) + x0.y);
+        axb = x2b * (1.0f / 3.0f) + (x1b * (2.0f / 3.0f) + x0.z);
+    } else {
+        const float x1r = x1.x - x0.x, x1g = x1.y - x0.y, x1b = x1.z - x0.z;
         axr = x0.x + x1r * 0.5f;
         axg = x0.y + x1g * 0.5f;
         axb = x0.z + x1b * 0.5f;
     }
-    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
     const float bxr = sum.x - axr, bxg = sum.y - axg, bxb = sum.z - axb;
     float ar = (axr * beta2 - bxr * ab) * factor;
     float ag = (axg * beta2 - bxg * ab) * factor;
@@ -148,9 +160,25 @@ __device__ __forceinline__ float eval_candidate(const float4 *sat, uint32_t pack
     const float e1r = (ar * ar) * alpha2 + ((br * br) * beta2 + e2r);
     const float e1g = (ag * ag) * alpha2 + ((bg * bg) * beta2 + e2g);
     const float e1b = (abl * abl) * alpha2 + ((bb * bb) * beta2 + e2b);
-    out->ax = ar; out->ay = ag; out->az = abl;
-    out->bx = br; out->by = bg; out->bz = bb;
+    if (out) {
+        out->ax = ar; out->ay = ag; out->az = abl;
+        out->bx = br; out->by = bg; out->bz = bb;
+    }
     return e1r * mx + e1g * my + e1b * mz;
+}
+
+// One least-squares candidate, icbc.h:2501-2538 (four colours) or
+// :2201-2236 (three colours).  sat is the zero-led table: entry 0 gives the
+// zeros icbc uses for an absent cluster, and the FLT_MAX entries past the
+// colour count are read exactly as the SSE build reads them.
+template <bool FOUR>
+__device__ __forceinline__ float eval_candidate(const float4 *sat, uint32_t packed, float4 sum, float mx, float my,
+                                                float mz, Cand *out)
+{
+    const float4 x0 = sat[packed & 0xff];
+    const float4 x1 = sat[(packed >> 8) & 0xff];
+    const float4 x2 = FOUR ? sat[(packed >> 16) & 0xff] : x1;
+    return eval_terms<FOUR>(x0, x1, x2, sum, cluster_terms<FOUR>(x0, x1, x2, sum.w), mx, my, mz, out);
 }
 
 // (error, p & 3, p >> 2) ordering: true if candidate (e, p) beats (be, bp).

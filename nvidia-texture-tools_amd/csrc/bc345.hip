@@ -5,6 +5,7 @@
 //     or OptimalCompress::compressDXT1 for single-colour blocks (CompressorDX9.cpp:158-176).
 // Input is gathered as ColorBlock::init does (ColorBlock.cpp:80-110):
 // truncating float->u8 conversion, edge blocks filled by repeating pixels.
+#include "bucket.hpp"
 #include "channel_block.hpp"
 
 #include <cfloat>
@@ -58,7 +59,7 @@ __device__ __forceinline__ int float_to_int(float a, int limit)
 // reference's nested (c0, c1, c2) loops with identical trip counts and the
 // reference's own running sums, with no cross-lane work.
 
-constexpr int kCountSlots = 17;
+using namespace ntx_bucket;
 
 __device__ __forceinline__ void bc3_single_colour(const ntx::Tables *__restrict__ tab, uint32_t r8, uint32_t g8, uint32_t b8,
                                                   uint2 *dst)
@@ -70,14 +71,6 @@ __device__ __forceinline__ void bc3_single_colour(const ntx::Tables *__restrict_
     if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
     *dst = make_uint2(c0 | c1 << 16, idx);
 }
-
-// Scratch layout (ints): counts[0, kCountSlots) per-count tallies;
-// counts[kTopSlot] and kTicketGroups ticket counters kTicketStride ints
-// apart from kTicketBase (own cache lines, so their atomics do not queue
-// behind one another); the lists start at kListsOffset.
-constexpr int kTopSlot = 31;
-constexpr int kTicketGroups = 16, kTicketBase = 32, kTicketStride = 32;
-constexpr int kListsOffset = kTicketBase + kTicketGroups * kTicketStride;
 
 __global__ __launch_bounds__(64) void zero_counts_kernel(int *counts)
 {
@@ -189,35 +182,7 @@ __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict_
             }
         }
     }
-    __syncthreads();
-    if (t < kCountSlots && s_cnt[t] > 0) s_base[t] = atomicAdd(&counts[t], s_cnt[t]);
-    __syncthreads();
-    if (local >= 0) lists[size_t(k) * cap + s_base[k] + local] = blk;
-}
-
-// End of a search workgroup (one wavefront): the last one to finish clears
-// the per-count tallies for the next image.  Every workgroup has used the
-// tallies it loaded before it takes its ticket, so once all tickets are
-// taken no one reads them again.  Two levels of tickets: workgroup b counts
-// into group b % kTicketGroups, the last of a group into the top counter,
-// which keeps the number of atomics on any one address small when thousands
-// of workgroups end together.  No fence: nothing is published through the
-// tickets, and the zeros reach the next kernel at the kernel boundary.
-__device__ __forceinline__ void bc3_release_counts(int *counts)
-{
-    if (threadIdx.x == 0) {
-        const int groups = min(int(gridDim.x), kTicketGroups);
-        const int g = int(blockIdx.x) % kTicketGroups;
-        const int members = (int(gridDim.x) - g + kTicketGroups - 1) / kTicketGroups;
-        int *ticket = counts + kTicketBase + g * kTicketStride;
-        if (atomicAdd(ticket, 1) == members - 1) {
-            *ticket = 0;
-            if (atomicAdd(&counts[kTopSlot], 1) == groups - 1) {
-                for (int i = 0; i < kCountSlots; i++) counts[i] = 0;
-                counts[kTopSlot] = 0;
-            }
-        }
-    }
+    append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
 }
 
 // The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums,
@@ -538,7 +503,7 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
     if (live)
         bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
                   reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));
-    bc3_release_counts(counts);
+    release_counts(counts);
 }
 
 // The same search with 2^lpb_shift lanes per block, for images too small to
@@ -622,7 +587,7 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
         const int ob = __shfl_xor(best, off);
         if (oe < besterror || (oe == besterror && (part & off))) { besterror = oe; best = ob; }
     }
-    bc3_release_counts(counts);
+    release_counts(counts);
     if (!live || part != 0) return;
     bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, best & 0xff, (best >> 8) & 0xff, best >> 16,
               reinterpret_cast<uint2 *>(out + size_t(blk) * stride + offset));

@@ -125,6 +125,20 @@ int bc3_lanes_shift(int nblocks, int target_log2)
     return s;
 }
 
+bool bc1_two_pass(const ntx_hip_context *ctx, int quality, int w, int h)
+{
+    const size_t nblocks = size_t((w + 3) / 4) * size_t((h + 3) / 4);
+    return bc1_level(quality) == 8 && nblocks >= size_t(ctx->bc1_lanes_min);
+}
+
+size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, int h)
+{
+    const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+    if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) return size_t(bc3_scratch_ints(nblocks));
+    if (format == NTX_FORMAT_BC1 && bc1_two_pass(ctx, quality, w, h)) return size_t(bc1_scratch_ints(nblocks));
+    return 0;
+}
+
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out)
 {
     const float alpha = (filter == NTX_FILTER_KAISER && params) ? params[0] : 4.0f;
@@ -345,7 +359,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "bc3_front_kernel", "batch_kernel", "bc1_mlp_kernel",
                                                    "bc1_list_kernel", "mlp_train_kernel",
                                                    "bc3_search_split_kernel", "gamma_kernel",
-                                                   "normal_kernel" };
+                                                   "normal_kernel", "bc1_classify_kernel",
+                                                   "bc1_search_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -397,12 +412,28 @@ namespace ntx {
 // small launches raise their wave priority.
 static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int format, int quality, int alpha_mode,
                               const float *cw, const float *d_rgba, int w, int h, void *d_out, int *bucket,
-                              bool *counts_zero, int lanes_log2 = 18, bool urgent = false)
+                              bool *counts_zero, int lanes_log2 = 18, bool urgent = false, int16_t *d_nm = nullptr)
 {
     hipError_t e = hipSuccess;
-    if (format == NTX_FORMAT_BC1) {
+    if (format == NTX_FORMAT_BC1 && bc1_two_pass(ctx, quality, w, h)) {
+        const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+        int *counts = bucket, *lists = counts + bc3_lists_offset();
+        if (!*counts_zero) e = launch_bc3_clear_counts(counts, stream);
+        *counts_zero = false;
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, NTX_KERNEL_BC1_CLASSIFY, [&] {
+                return launch_bc1_classify(d_rgba, w, h, alpha_mode, ctx->d_tab, d_out, d_nm, counts, lists, nblocks,
+                                           stream, urgent);
+            });
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, NTX_KERNEL_BC1_SEARCH, [&] {
+                return launch_bc1_search(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, d_nm, counts, lists, nblocks,
+                                         stream);
+            });
+        if (e == hipSuccess) *counts_zero = true;
+    } else if (format == NTX_FORMAT_BC1) {
         e = timed_on(ctx, stream, NTX_KERNEL_BC1, [&] {
-            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, nullptr, stream, urgent,
+            return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, d_nm, stream, urgent,
                               bc1_level(quality));
         });
     } else if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) {
@@ -500,6 +531,19 @@ static hipEvent_t sync_event(ntx_hip_context *ctx, size_t i)
     return ctx->sync_events[i];
 }
 
+// The context's bucket scratch for one image (BC3, two-pass BC1), or null.
+static ntx_status single_bucket(ntx_hip_context *ctx, int format, int quality, int w, int h, int **bucket)
+{
+    *bucket = nullptr;
+    const size_t ints = scratch_ints(ctx, format, quality, w, h);
+    if (!ints) return NTX_OK;
+    const size_t cap_before = ctx->bucket.cap;
+    if (ntx_status s = ensure(ctx, ctx->bucket, sizeof(int) * ints)) return s;
+    if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
+    *bucket = static_cast<int *>(ctx->bucket.ptr);
+    return NTX_OK;
+}
+
 }  // namespace ntx
 
 extern "C" {
@@ -515,16 +559,17 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
     if (!cw) cw = ones;
     int *bucket = nullptr;
-    if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) {
-        const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
-        const size_t cap_before = ctx->bucket.cap;
-        if ((s = ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc3_scratch_ints(nblocks))))) return s;
-        if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
-        bucket = static_cast<int *>(ctx->bucket.ptr);
-    }
+    if ((s = ntx::single_bucket(ctx, format, quality, w, h, &bucket))) return s;
     hipError_t e = ntx::compress_on(ctx, ctx->stream, format, quality, alpha_mode, cw, d_rgba, w, h, d_out, bucket,
                                     &ctx->bc3_counts_zero);
     if (e != hipSuccess) return fail(ctx, e, "compress launch");
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_set_bc1_lanes_min(ntx_hip_context *ctx, int min_blocks)
+{
+    if (!ctx || min_blocks < 0) return NTX_ERR_INVALID_ARGUMENT;
+    ctx->bc1_lanes_min = min_blocks ? min_blocks : ntx_hip_context::kBc1LanesMinDefault;
     return NTX_OK;
 }
 
@@ -555,8 +600,7 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
         region[k] = total;
         layout.push_back(w[k]);
         layout.push_back(h[k]);
-        if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
-            total += (size_t(bc3_scratch_ints(((w[k] + 3) / 4) * ((h[k] + 3) / 4))) + 63) & ~size_t(63);
+        total += (scratch_ints(ctx, format, quality, w[k], h[k]) + 63) & ~size_t(63);
     }
     if (total) {
         const size_t cap_before = ctx->chain.cap;
@@ -575,9 +619,8 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
     ctx->chain_counts_zero = false;
     bool all_zero = true;
     auto compress_one = [&](hipStream_t st, int k) -> ntx_status {
-        int *bucket = (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
-                          ? static_cast<int *>(ctx->chain.ptr) + region[k]
-                          : nullptr;
+        int *bucket = scratch_ints(ctx, format, quality, w[k], h[k]) ? static_cast<int *>(ctx->chain.ptr) + region[k]
+                                                                       : nullptr;
         bool z = zero;
         hipError_t e = compress_on(ctx, st, format, quality, alpha_mode, cw, d_in[k], w[k], h[k], d_out[k], bucket, &z,
                                    k ? 16 : 18, st == small);
@@ -732,8 +775,12 @@ NTX_API ntx_status ntx_hip_compress_bc1_stats_device(ntx_hip_context *ctx, int a
     ntx_status s = bind(ctx);
     if (s) return s;
     if (w < 0 || h < 0 || !d_nm || (size_t(w) * size_t(h) > 0 && (!d_rgba || !d_out))) return NTX_ERR_INVALID_ARGUMENT;
+    if (size_t(w) * size_t(h) == 0) return NTX_OK;
     const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
-    hipError_t e = timed(ctx, NTX_KERNEL_BC1, [&] { return ntx::launch_bc1(d_rgba, w, h, alpha_mode, cw ? cw : ones, ctx->d_tab, d_out, d_nm, ctx->stream); });
+    int *bucket = nullptr;
+    if ((s = ntx::single_bucket(ctx, NTX_FORMAT_BC1, NTX_QUALITY_NORMAL, w, h, &bucket))) return s;
+    hipError_t e = ntx::compress_on(ctx, ctx->stream, NTX_FORMAT_BC1, NTX_QUALITY_NORMAL, alpha_mode, cw ? cw : ones,
+                                    d_rgba, w, h, d_out, bucket, &ctx->bc3_counts_zero, 18, false, d_nm);
     if (e != hipSuccess) return fail(ctx, e, "bc1 launch");
     return NTX_OK;
 }

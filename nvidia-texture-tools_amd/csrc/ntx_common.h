@@ -43,6 +43,14 @@ struct Tables {
     // weights are point multiplicities, 16 in all): entry
     // (w0 * 17 + w1) * 17 + w2 for w0 + w1 + w2 <= 16, others zero.
     float sqw[17 * 17 * 17][4];
+    // icbc's cluster-weight terms (alpha2, beta2, alphabeta, 1 / (alpha2
+    // beta2 - alphabeta^2), icbc.h:2501-2511 and :2201-2208) of a block whose
+    // cluster weights are integers summing to 16 (every texel of weight 1):
+    // bc1w4 by the prefix counts (W0 * 17 + W1) * 17 + W2 at the candidate's
+    // c0 <= c1 <= c2, bc1w3 by W0 * 17 + W1; entries with W0 > W1 or W1 > W2
+    // are zero.  Computed with the kernel's own operations on exact values.
+    float bc1w4[17 * 17 * 17][4];
+    float bc1w3[17 * 17][4];
     // Exponent tables of the gamma-2.2 transforms (Gamma.cpp:33-300): entry k
     // (a float's sign-and-exponent field) is 2^((k - 127) * 5/11), resp.
     // 2^((k - 127) * 11/5), as a float; 0 for k == 0 and k >= 256, +inf for 255.

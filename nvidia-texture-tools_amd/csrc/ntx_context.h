@@ -13,6 +13,13 @@
 namespace ntx {
 hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab, void *d_out,
                       int16_t *d_nm, hipStream_t stream, bool urgent = false, int level = 8);
+// icbc Level8 in two passes, one block per lane (large images); d_counts:
+// bucket scratch of bc1_scratch_ints(blocks) ints with zero tallies.
+hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
+                               int16_t *d_nm, int *d_counts, int *d_lists, int cap, hipStream_t stream, bool urgent);
+hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
+                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, hipStream_t stream);
+int bc1_scratch_ints(int nblocks);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
                            bool urgent = false);
 hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int *d_counts,
@@ -92,6 +99,9 @@ struct ntx_hip_context {
     uint32_t gamma_lut_key = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ntx::DevBuf in, out, tmp, dst, bucket, metric;
+    // BC1 images of at least this many blocks take the two-pass encoder.
+    static constexpr int kBc1LanesMinDefault = 1 << 17;
+    int bc1_lanes_min = kBc1LanesMinDefault;
     // The BC3 tallies at the start of `bucket` are zero: set after a complete
     // front + search sequence (the search clears them), cleared before one.
     bool bc3_counts_zero = false;
@@ -124,6 +134,10 @@ hipEvent_t take_event(ntx_hip_context *ctx);
 bool supported(int format, int quality);
 // icbc quality level of a BC1 nvtt quality (1, 8 or 9).
 int bc1_level(int quality);
+// BC1 at this quality and size takes the two-pass (lane per block) encoder.
+bool bc1_two_pass(const ntx_hip_context *ctx, int quality, int w, int h);
+// Bucket scratch (ints) one image's compression needs: BC3, two-pass BC1.
+size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, int h);
 // Polyphase weights and first-tap indices for src -> dst samples, cached per context.
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out);
 

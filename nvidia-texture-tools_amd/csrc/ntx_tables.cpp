@@ -124,6 +124,37 @@ void build_tables(Tables *t)
                 }
             }
 
+    // icbc cluster-weight terms for integer weights summing to 16, by the
+    // prefix weights (W0, W1, W2) the kernel reads from its prefix-sum rows:
+    // the operations of ntx_bc1::cluster_terms (icbc.h:2501-2511, 2201-2208).
+    std::memset(t->bc1w4, 0, sizeof t->bc1w4);
+    std::memset(t->bc1w3, 0, sizeof t->bc1w3);
+    for (int a = 0; a <= 16; a++)
+        for (int b = a; b <= 16; b++) {
+            {
+                const float x0w = float(a), x1w = float(b), sumw = 16.0f;
+                const float w2 = sumw - x1w;
+                const float w1 = x1w - x0w;
+                const float ab = w1 * 0.25f;
+                const float alpha2 = x0w + ab;
+                const float beta2 = w2 + ab;
+                const float v[4] = { alpha2, beta2, ab, 1.0f / (alpha2 * beta2 - ab * ab) };
+                for (int j = 0; j < 4; j++) t->bc1w3[a * 17 + b][j] = v[j];
+            }
+            for (int c = b; c <= 16; c++) {
+                const float x0w = float(a), x1w = float(b), x2w = float(c), sumw = 16.0f;
+                const float w3 = sumw - x2w;
+                const float w2 = x2w - x1w;
+                const float w1 = x1w - x0w;
+                const float w0 = x0w;
+                const float alpha2 = w2 * (1.0f / 9.0f) + (w1 * (4.0f / 9.0f) + w0);
+                const float beta2 = w1 * (1.0f / 9.0f) + (w2 * (4.0f / 9.0f) + w3);
+                const float ab = (w1 + w2) * (2.0f / 9.0f);
+                const float v[4] = { alpha2, beta2, ab, 1.0f / (alpha2 * beta2 - ab * ab) };
+                for (int j = 0; j < 4; j++) t->bc1w4[(a * 17 + b) * 17 + c][j] = v[j];
+            }
+        }
+
     for (int k = 0; k < 256; k++) t->div255[k] = float(k) / 255.0f;
 
     // Gamma-2.2 exponent tables (Gamma.cpp:30-300) from their closed form.

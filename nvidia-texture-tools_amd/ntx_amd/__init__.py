@@ -34,7 +34,7 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
            "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_front_kernel", "batch_kernel",
            "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel", "gamma_kernel",
-           "normal_kernel")
+           "normal_kernel", "bc1_classify_kernel", "bc1_search_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
@@ -44,7 +44,7 @@ EXPORTS = (
     "ntx_hip_copy_device", "ntx_hip_memset_device",
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
-    "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device",
+    "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device", "ntx_hip_set_bc1_lanes_min",
     "ntx_hip_compress_chain_device", "ntx_hip_compress_images_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P, _S]),
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
+        "ntx_hip_set_bc1_lanes_min": (_I, [_P, _I]),
         "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
         "ntx_hip_compress_images_device": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
@@ -313,6 +314,11 @@ class Context:
         st = self.lib.ntx_hip_compress_images_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), n,
                                                      ctypes.cast(ins, _P), _ptr(ws), _ptr(hs), ctypes.cast(outs, _P))
         self._check(st, "ntx_hip_compress_images_device")
+
+    def set_bc1_lanes_min(self, min_blocks: int) -> None:
+        """BC1 Level8 images of at least min_blocks blocks take the two-pass
+        one-block-per-lane encoder (0: the default threshold)."""
+        self._check(self.lib.ntx_hip_set_bc1_lanes_min(self.handle, int(min_blocks)), "set_bc1_lanes_min")
 
     def compress_bc1_stats_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
                                   d_nm: DeviceBuffer, alpha_mode: int = ALPHA_NONE,
