@@ -107,7 +107,7 @@ enum {
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 pass 1: alpha halves, single-colour blocks, bucketing by colour count */
     NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
-    NTX_KERNEL_BC1_LIST = 12,    /* BC1 cluster fit on the blocks the proposal pass queued */
+    NTX_KERNEL_BC1_LIST = 12,    /* reserved (the fallback blocks now run through BC1_CLASSIFY / BC1_SEARCH) */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
     NTX_KERNEL_BC3_SEARCH_SPLIT = 14, /* BC3 colour search of a small image: 2..64 lanes per block */
     NTX_KERNEL_GAMMA = 15,       /* gamma transforms of R, G, B (one launch per up to 16 levels) */
@@ -128,14 +128,19 @@ NTX_API size_t ntx_hip_compressed_size(int format, int w, int h);
 
 /* Replaces CudaCompressor::compress (CudaCompressorDXT.cpp:128-206) behind
  * nv::CompressorInterface::compress (src/nvtt/Compressor.h:34-38), extended
- * from BC1 to BC1/BC3/BC4/BC5 at Quality_Normal:
- *   BC1: icbc compress_dxt1 Level8 per block (BlockCompressor.cpp:116-222),
- *   BC3: QuickCompress alpha + squish weighted cluster fit colour (CompressorDX9.cpp:145-176),
- *   BC4/BC5: QuickCompress::compressDXT5A on R / R,G (CompressorDX10.cpp:42-60).
+ * from BC1 to BC1/BC3/BC3n/BC4/BC5, each at the qualities whose CPU
+ * compressor it reproduces byte for byte:
+ *   BC1 (every quality; BlockCompressor.cpp:116-222): icbc compress_dxt1 per
+ *     block at Level1 (Fastest), Level8 (Normal, Highest) or Level9
+ *     (Production);
+ *   BC3 / BC3n (Normal, Production; CompressorDX9.cpp:145-222):
+ *     QuickCompress alpha + squish weighted cluster fit colour;
+ *   BC4 / BC5 (Fastest, Normal; CompressorDX10.cpp:42-60):
+ *     QuickCompress::compressDXT5A on R / R,G.
  * rgba: planar FP32, four planes R,G,B,A of w*h floats (Surface::data(),
  * Surface.cpp:492-495).  out: ntx_hip_compressed_size() bytes in block-raster
  * order.  color_weights: CompressionOptions::setColorWeights (r,g,b,a).
- * Quality_Fastest and other formats return NTX_ERR_UNSUPPORTED. */
+ * Other (format, quality) pairs return NTX_ERR_UNSUPPORTED. */
 NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
                                     const float *color_weights, const float *rgba, int w, int h,
                                     void *out, size_t out_bytes);
@@ -299,8 +304,10 @@ NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_r
  * kernel launch captured in a HIP graph.  Every output byte equals what the
  * per-image entry points above produce for the same level.
  *
- * Formats: BC1 (Normal/Highest), BC4 and BC5 (Fastest/Normal).  BC3 returns
- * NTX_ERR_UNSUPPORTED here; use ntx_hip_compress_device per level. */
+ * Formats: BC1 at every quality (icbc Level1 / Level8 / Level9, one
+ * block per 16-lane group), BC4 and BC5 (Fastest/Normal).  BC3 / BC3n return
+ * NTX_ERR_UNSUPPORTED here; use ntx_hip_compress_chain_device or
+ * ntx_hip_compress_images_device for them. */
 typedef struct {
     const float *d_rgba;   /* level 0: planar FP32 R,G,B,A planes of width*height, device memory */
     int width, height;

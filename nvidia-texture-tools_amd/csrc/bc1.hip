@@ -48,9 +48,11 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
 // reduce_colors; a block of no colour (every weight 0) is the zero block, a
 // block of one colour compress_dxt1_single_color_optimal (icbc.h:3515-3525),
 // every other block goes to list n.  The clusters live in LDS only while
-// they are counted.
+// they are counted.  Blocks 0..nblocks-1 of the image, or, when sel is not
+// null, the *sel_count blocks listed in sel.
 __global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                           int nblocks, int alpha_mode,
+                                                           int nblocks, const int *__restrict__ sel,
+                                                           const int *__restrict__ sel_count, int alpha_mode,
                                                            const ntx::Tables *__restrict__ tab,
                                                            uint2 *__restrict__ out, short2 *__restrict__ nm,
                                                            int *__restrict__ counts, int *__restrict__ lists, int cap,
@@ -63,9 +65,11 @@ __global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restri
     const int t = threadIdx.x;
     if (t < kCountSlots) s_cnt[t] = 0;
     __syncthreads();
-    const int blk = blockIdx.x * blockDim.x + t;
+    const int item = blockIdx.x * blockDim.x + t;
+    const bool has = sel ? item < *sel_count : item < nblocks;
+    const int blk = has ? (sel ? sel[item] : item) : 0;
     int k = 0, local = -1;
-    if (blk < nblocks) {
+    if (has) {
         float4 tx[16];
         lane_gather(src, w, h, blk % bw, blk / bw, alpha_mode, rows16 != 0, tx);
         bool any_black;
@@ -88,36 +92,13 @@ __global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restri
     append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
 }
 
-// Level8 pass 2, one block per lane (icbc.h:3292-3326 + :3543-3550).  Wave
-// b takes 64 consecutive entries of one list, the lists walked from n = 16
-// down; lanes past the end of their list repeat its first block and write
-// nothing.  The
-// block is gathered and reduced again (the same n), its prefix-sum table
-// built in LDS, the four-colour fit searched, then (after skip_blacks when
-// some cluster is black) the three-colour fit, and the better block kept.
-// The last workgroup clears the tallies.
-__global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                        int alpha_mode, float cwx, float cwy, float cwz,
-                                                        const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
-                                                        short2 *__restrict__ nm, int *__restrict__ counts,
-                                                        const int *__restrict__ lists, int cap, int rows16)
+// One wavefront of bc1_search_kernel: entries first .. first + 63 of list k.
+__device__ __forceinline__ void search_wave(const float *__restrict__ src, int w, int h, int bw, int alpha_mode,
+                                            float cwx, float cwy, float cwz, const ntx::Tables *__restrict__ tab,
+                                            uint2 *__restrict__ out, short2 *__restrict__ nm,
+                                            const int *__restrict__ lists, int cap, int rows16, LaneRows s_rows,
+                                            int lane, int k, int first, int cnt)
 {
-    using namespace ntx_bucket;
-    __shared__ float4 s_rows[17][64];
-    const int lane = threadIdx.x;
-    int wv = blockIdx.x, k = 0, first = 0, cnt = 0;
-    for (int b = 16; b >= 2; b--) {
-        const int c = counts[b];
-        const int waves = (c + 63) >> 6;
-        if (k == 0) {
-            if (wv < waves) { k = b; first = wv * 64; cnt = c; }
-            else wv -= waves;
-        }
-    }
-    if (k == 0) {
-        release_counts(counts);
-        return;
-    }
     const int item = first + lane;
     const bool live = item < cnt;
     // Lanes past the end of the list encode the wave's first block again
@@ -169,6 +150,40 @@ __global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict_
         out[blk] = result;
         if (nm) nm[blk] = make_short2(short(n), short(run3 ? m : -1));
     }
+}
+
+// Level8 pass 2, one block per lane (icbc.h:3292-3326 + :3543-3550).  The
+// lists are cut into waves of 64 consecutive entries, walked from n = 16
+// down; workgroup b takes waves b, b + grid, ...; lanes past the end of
+// their list repeat its first block and write nothing.  The
+// block is gathered and reduced again (the same n), its prefix-sum table
+// built in LDS, the four-colour fit searched, then (after skip_blacks when
+// some cluster is black) the three-colour fit, and the better block kept.
+// The last workgroup clears the tallies.
+__global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                        int alpha_mode, float cwx, float cwy, float cwz,
+                                                        const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
+                                                        short2 *__restrict__ nm, int *__restrict__ counts,
+                                                        const int *__restrict__ lists, int cap, int rows16)
+{
+    using namespace ntx_bucket;
+    __shared__ float4 s_rows[17][64];
+    const int lane = threadIdx.x;
+    int total = 0;
+    for (int b = 2; b <= 16; b++) total += (counts[b] + 63) >> 6;
+    for (int wave = blockIdx.x; wave < total; wave += gridDim.x) {
+        int wv = wave, k = 0, first = 0, cnt = 0;
+        for (int b = 16; b >= 2; b--) {
+            const int c = counts[b];
+            const int waves = (c + 63) >> 6;
+            if (k == 0) {
+                if (wv < waves) { k = b; first = wv * 64; cnt = c; }
+                else wv -= waves;
+            }
+        }
+        search_wave(src, w, h, bw, alpha_mode, cwx, cwy, cwz, tab, out, nm, lists, cap, rows16, s_rows, lane, k,
+                    first, cnt);
+    }
     release_counts(counts);
 }
 
@@ -193,30 +208,37 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
     return hipGetLastError();
 }
 
-// Level8 pass 1 over the block lists in `counts` / `lists` (zero tallies on
-// entry; lists: 17 x cap ints, cap >= the image's block count).
+// Level8 pass 1 into the block lists in `counts` / `lists` (zero tallies on
+// entry; lists: 17 x cap ints, cap >= the number of blocks classified): the
+// image's blocks, or the *d_sel_count blocks listed in d_sel (at most cap).
 hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
-                               int16_t *d_nm, int *d_counts, int *d_lists, int cap, hipStream_t stream, bool urgent)
+                               int16_t *d_nm, const int *d_sel, const int *d_sel_count, int *d_counts, int *d_lists,
+                               int cap, hipStream_t stream, bool urgent)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
-    if (nblocks == 0) return hipSuccess;
+    const int items = d_sel ? cap : nblocks;
+    if (items == 0) return hipSuccess;
     const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
-    hipLaunchKernelGGL(bc1_classify_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
-                       alpha_mode, d_tab, reinterpret_cast<uint2 *>(d_out), reinterpret_cast<short2 *>(d_nm), d_counts,
-                       d_lists, cap, rows16, urgent ? 1 : 0);
+    hipLaunchKernelGGL(bc1_classify_kernel, dim3((items + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
+                       d_sel, d_sel_count, alpha_mode, d_tab, reinterpret_cast<uint2 *>(d_out),
+                       reinterpret_cast<short2 *>(d_nm), d_counts, d_lists, cap, rows16, urgent ? 1 : 0);
     return hipGetLastError();
 }
 
-// Level8 pass 2 over the lists pass 1 filled; leaves the tallies zero.
+// Level8 pass 2 over the lists pass 1 filled (at most `cap` blocks in all);
+// leaves the tallies zero.  max_groups bounds the grid (the workgroups loop
+// over the wavefronts).
 hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
-                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, hipStream_t stream)
+                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, int max_groups,
+                             hipStream_t stream)
 {
-    const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
-    if (nblocks == 0) return hipSuccess;
+    const int bw = (w + 3) / 4;
+    if (cap == 0) return hipSuccess;
     const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
     // Each list fills whole wavefronts: at most one partial wave per list.
-    const int waves = (nblocks + 63) / 64 + (ntx_bucket::kCountSlots - 2);
-    hipLaunchKernelGGL(bc1_search_kernel, dim3(waves), dim3(64), 0, stream, d_src, w, h, bw, alpha_mode, cw[0], cw[1],
+    const int waves = (cap + 63) / 64 + (ntx_bucket::kCountSlots - 2);
+    const int grid = max_groups > 0 && max_groups < waves ? max_groups : waves;
+    hipLaunchKernelGGL(bc1_search_kernel, dim3(grid), dim3(64), 0, stream, d_src, w, h, bw, alpha_mode, cw[0], cw[1],
                        cw[2], d_tab, reinterpret_cast<uint2 *>(d_out), reinterpret_cast<short2 *>(d_nm), d_counts,
                        d_lists, cap, rows16);
     return hipGetLastError();

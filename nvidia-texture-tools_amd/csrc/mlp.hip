@@ -936,26 +936,6 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
     }
 }
 
-// The full cluster fit on a list of blocks (the fallback), 16 blocks per
-// workgroup pass; count is read on the device.
-__global__ __launch_bounds__(256) void bc1_list_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                       const int *__restrict__ count, const int *__restrict__ list,
-                                                       const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out)
-{
-    __shared__ ntx_bc1::Bc1Shared sh;
-    ntx_bc1::load_shared_tables(sh, tab, threadIdx.x, blockDim.x);
-    __syncthreads();
-    const int total = *count;
-    const int g = threadIdx.x >> 4, lane = threadIdx.x & 15;
-    for (int i = blockIdx.x * 16 + g; i < total; i += gridDim.x * 16) {
-        const int b = list[i];
-        int nn, mm;
-        const uint2 r = ntx_bc1::encode_block(sh, src, w, h, b % bw, b / bw, NTX_ALPHA_NONE, 1.0f, 1.0f, 1.0f, tab, g,
-                                              lane, &nn, &mm);
-        if (lane == 0) out[b] = r;
-    }
-}
-
 }  // namespace
 
 namespace ntx {
@@ -1020,17 +1000,6 @@ hipError_t launch_bc1_mlp(const float *d_params, const float *d_src, int w, int 
     hipLaunchKernelGGL(bc1_mlp_kernel, dim3(grid), dim3(256), 0, stream, d_params, d_src, w, h, bw, n, d_tab, tau_pred,
                        tau_gap, refine_iters, greedy_rounds, box_candidates, static_cast<uint2 *>(d_out), d_fb_count,
                        d_fb_list, d_err);
-    return hipGetLastError();
-}
-
-hipError_t launch_bc1_list(const float *d_src, int w, int h, const int *d_count, const int *d_list, int max_blocks,
-                           const Tables *d_tab, void *d_out, int workgroups, hipStream_t stream)
-{
-    const int bw = (w + 3) / 4;
-    if (max_blocks <= 0) return hipSuccess;
-    const int grid = std::max(1, std::min(workgroups, (max_blocks + 15) / 16));
-    hipLaunchKernelGGL(bc1_list_kernel, dim3(grid), dim3(256), 0, stream, d_src, w, h, bw, d_count, d_list, d_tab,
-                       static_cast<uint2 *>(d_out));
     return hipGetLastError();
 }
 

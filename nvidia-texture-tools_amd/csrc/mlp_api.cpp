@@ -22,8 +22,6 @@ hipError_t launch_mlp_adam(float *d_params, float *d_m1, float *d_m2, const floa
 hipError_t launch_bc1_mlp(const float *d_params, const float *d_src, int w, int h, const Tables *d_tab, float tau_pred,
                           float tau_gap, int refine_iters, int greedy_rounds, int box_candidates, void *d_out,
                           int *d_fb_count, int *d_fb_list, float *d_err, int workgroups, hipStream_t stream);
-hipError_t launch_bc1_list(const float *d_src, int w, int h, const int *d_count, const int *d_list, int max_blocks,
-                           const Tables *d_tab, void *d_out, int workgroups, hipStream_t stream);
 }  // namespace ntx
 
 static_assert(ntx_mlp::kParams == NTX_MLP_PARAMS, "parameter count");
@@ -234,10 +232,30 @@ NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *m, const float *
                                    (flags & NTX_MLP_BOX_CANDIDATES) ? 1 : 0, d_out, d_count, d_list, nullptr, 2 * cus,
                                    ctx->stream);
     });
+    // The fallback blocks through the two-pass cluster fit (colour counts of
+    // the listed blocks, then the search over their buckets).
+    int *counts = nullptr;
+    if (e == hipSuccess) {
+        const size_t cap_before = ctx->bucket.cap;
+        if ((s = ntx::ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc1_scratch_ints(n))))) return s;
+        if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
+        counts = static_cast<int *>(ctx->bucket.ptr);
+        if (!ctx->bc3_counts_zero) e = ntx::launch_bc3_clear_counts(counts, ctx->stream);
+        ctx->bc3_counts_zero = false;
+    }
+    const float ones[3] = { 1.0f, 1.0f, 1.0f };
     if (e == hipSuccess)
-        e = ntx::timed(ctx, NTX_KERNEL_BC1_LIST, [&] {
-            return ntx::launch_bc1_list(d_rgba, w, h, d_count, d_list, n, ctx->d_tab, d_out, 8 * cus, ctx->stream);
+        e = ntx::timed(ctx, NTX_KERNEL_BC1_CLASSIFY, [&] {
+            return ntx::launch_bc1_classify(d_rgba, w, h, NTX_ALPHA_NONE, ctx->d_tab, d_out, nullptr, d_list, d_count,
+                                            counts, counts + ntx::bc3_lists_offset(), n, ctx->stream, false);
         });
+    if (e == hipSuccess)
+        e = ntx::timed(ctx, NTX_KERNEL_BC1_SEARCH, [&] {
+            return ntx::launch_bc1_search(d_rgba, w, h, NTX_ALPHA_NONE, ones, ctx->d_tab, d_out, nullptr, counts,
+                                          counts + ntx::bc3_lists_offset(), n, ntx::kBc1SearchGroupsPerCU * ctx->cus,
+                                          ctx->stream);
+        });
+    if (e == hipSuccess) ctx->bc3_counts_zero = true;
     if (e != hipSuccess) return ntx::fail(ctx, e, "bc1 proposal launch");
     if (fallback_blocks) {
         NTX_HIP(ctx, hipMemcpyAsync(fallback_blocks, d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

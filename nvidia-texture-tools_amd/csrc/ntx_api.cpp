@@ -220,6 +220,9 @@ NTX_API ntx_status ntx_hip_create(int device, ntx_hip_context **out)
     ctx->device = device;
     auto cleanup = [&](ntx_status s) { ntx_hip_destroy(ctx); return s; };
     if (hipSetDevice(device) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->cus = cus;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return cleanup(NTX_ERR_DEVICE);
     ntx::Tables *tab = new ntx::Tables;
@@ -422,13 +425,13 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
         *counts_zero = false;
         if (e == hipSuccess)
             e = timed_on(ctx, stream, NTX_KERNEL_BC1_CLASSIFY, [&] {
-                return launch_bc1_classify(d_rgba, w, h, alpha_mode, ctx->d_tab, d_out, d_nm, counts, lists, nblocks,
-                                           stream, urgent);
+                return launch_bc1_classify(d_rgba, w, h, alpha_mode, ctx->d_tab, d_out, d_nm, nullptr, nullptr, counts,
+                                           lists, nblocks, stream, urgent);
             });
         if (e == hipSuccess)
             e = timed_on(ctx, stream, NTX_KERNEL_BC1_SEARCH, [&] {
                 return launch_bc1_search(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, d_nm, counts, lists, nblocks,
-                                         stream);
+                                         kBc1SearchGroupsPerCU * ctx->cus, stream);
             });
         if (e == hipSuccess) *counts_zero = true;
     } else if (format == NTX_FORMAT_BC1) {

@@ -16,9 +16,13 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
 // icbc Level8 in two passes, one block per lane (large images); d_counts:
 // bucket scratch of bc1_scratch_ints(blocks) ints with zero tallies.
 hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
-                               int16_t *d_nm, int *d_counts, int *d_lists, int cap, hipStream_t stream, bool urgent);
+                               int16_t *d_nm, const int *d_sel, const int *d_sel_count, int *d_counts, int *d_lists,
+                               int cap, hipStream_t stream, bool urgent);
 hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
-                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, hipStream_t stream);
+                             void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, int max_groups,
+                             hipStream_t stream);
+// Workgroups of bc1_search_kernel resident at once per compute unit.
+constexpr int kBc1SearchGroupsPerCU = 8;
 int bc1_scratch_ints(int nblocks);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
                            bool urgent = false);
@@ -90,6 +94,7 @@ using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
 
 struct ntx_hip_context {
     int device = 0;
+    int cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     ntx::Tables *d_tab = nullptr;
     float *d_srgb = nullptr;

@@ -183,6 +183,25 @@ def _f32c(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _need(buf: "DeviceBuffer", nbytes: int, what: str) -> None:
+    """Raises ValueError unless buf is a live allocation of at least nbytes:
+    every device entry point reads or writes exactly the extent its sizes
+    imply, so a short buffer would be an out-of-bounds device access."""
+    if not isinstance(buf, DeviceBuffer) or not buf.ptr:
+        raise ValueError(f"{what}: not a live DeviceBuffer")
+    if int(nbytes) > buf.nbytes:
+        raise ValueError(f"{what}: the call needs {int(nbytes)} bytes, the buffer holds {buf.nbytes}")
+
+
+def _dims(w: int, h: int, what: str) -> None:
+    if int(w) < 0 or int(h) < 0:
+        raise ValueError(f"{what}: negative extent {w}x{h}")
+
+
+def _planar(w: int, h: int) -> int:
+    return 16 * int(w) * int(h)
+
+
 class DeviceBuffer:
     """A device allocation owned by a Context."""
 
@@ -195,13 +214,16 @@ class DeviceBuffer:
 
     def upload(self, host: np.ndarray) -> "DeviceBuffer":
         host = np.ascontiguousarray(host)
-        assert host.nbytes <= self.nbytes
+        _need(self, host.nbytes, "upload")
         self.ctx._check(self.ctx.lib.ntx_hip_upload(self.ctx.handle, self.ptr, _ptr(host), host.nbytes), "upload")
         return self
 
     def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
         dtype = np.dtype(dtype)
-        count = self.nbytes // dtype.itemsize if count is None else count
+        count = self.nbytes // dtype.itemsize if count is None else int(count)
+        if count < 0:
+            raise ValueError("download: negative count")
+        _need(self, count * dtype.itemsize, "download")
         out = np.empty(count, dtype)
         self.ctx._check(self.ctx.lib.ntx_hip_download(self.ctx.handle, _ptr(out), self.ptr, out.nbytes), "download")
         return out
@@ -242,8 +264,10 @@ class Context:
     # -- host-pointer entry points ------------------------------------------
     def compress(self, fmt: int, rgba_planar: np.ndarray, w: int, h: int, quality: int = QUALITY_NORMAL,
                  alpha_mode: int = ALPHA_NONE, color_weights: Sequence[float] = (1, 1, 1, 1)) -> np.ndarray:
+        _dims(w, h, "compress")
         src = _f32c(rgba_planar).reshape(-1)
-        assert src.size == 4 * w * h
+        if src.size != 4 * w * h:
+            raise ValueError(f"compress: {src.size} floats for a {w}x{h} planar RGBA image")
         out = np.zeros(compressed_size(fmt, w, h), np.uint8)
         cw = _f32c(color_weights)
         st = self.lib.ntx_hip_compress(self.handle, fmt, quality, alpha_mode, _ptr(cw), _ptr(src), w, h,
@@ -258,6 +282,8 @@ class Context:
         width = dw if width is None else width
         params = dp if params is None else params
         src = _f32c(rgba_planar).reshape(-1)
+        if src.size != 4 * W * H:
+            raise ValueError(f"downsample: {src.size} floats for a {W}x{H} planar RGBA image")
         w, h = max(1, W // 2), max(1, H // 2)
         out = np.empty(4 * w * h, np.float32)
         pp = _f32c(params) if params is not None else None
@@ -273,6 +299,9 @@ class Context:
     def compress_device(self, fmt: int, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
                         quality: int = QUALITY_NORMAL, alpha_mode: int = ALPHA_NONE,
                         color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        _dims(w, h, "compress_device")
+        _need(d_rgba, _planar(w, h), "compress_device input")
+        _need(d_out, compressed_size(fmt, w, h), "compress_device output")
         cw = _f32c(color_weights)
         st = self.lib.ntx_hip_compress_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), d_rgba.ptr, w, h,
                                               d_out.ptr)
@@ -287,6 +316,10 @@ class Context:
         every level's blocks to d_out[k] (ntx_hip_compress_chain_device)."""
         if len(d_levels) != len(d_out):
             raise ValueError("one output buffer per level")
+        _dims(w, h, "compress_chain_device")
+        for k, (a, b) in enumerate(mip_sizes(w, h)[:len(d_levels)]):
+            _need(d_levels[k], _planar(a, b), f"compress_chain_device level {k}")
+            _need(d_out[k], compressed_size(fmt, a, b), f"compress_chain_device output {k}")
         dw, dp = DEFAULT_FILTER[filt]
         width = dw if width is None else width
         params = dp if params is None else params
@@ -306,6 +339,10 @@ class Context:
         n = len(d_rgba)
         if len(sizes) != n or len(d_out) != n:
             raise ValueError("one size and one output buffer per image")
+        for k, (a, b) in enumerate(sizes):
+            _dims(a, b, "compress_images_device")
+            _need(d_rgba[k], _planar(a, b), f"compress_images_device input {k}")
+            _need(d_out[k], compressed_size(fmt, a, b), f"compress_images_device output {k}")
         cw = _f32c(color_weights)
         ins = (_P * n)(*[d.ptr for d in d_rgba])
         outs = (_P * n)(*[d.ptr for d in d_out])
@@ -323,6 +360,11 @@ class Context:
     def compress_bc1_stats_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
                                   d_nm: DeviceBuffer, alpha_mode: int = ALPHA_NONE,
                                   color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        _dims(w, h, "compress_bc1_stats_device")
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        _need(d_rgba, _planar(w, h), "compress_bc1_stats_device input")
+        _need(d_out, 8 * nb, "compress_bc1_stats_device output")
+        _need(d_nm, 4 * nb, "compress_bc1_stats_device counts")
         cw = _f32c(color_weights)
         st = self.lib.ntx_hip_compress_bc1_stats_device(self.handle, alpha_mode, _ptr(cw), d_rgba.ptr, w, h,
                                                         d_out.ptr, d_nm.ptr)
@@ -334,6 +376,9 @@ class Context:
         dw, dp = DEFAULT_FILTER[filt]
         width = dw if width is None else width
         params = dp if params is None else params
+        _dims(W, H, "downsample_device")
+        _need(d_src, _planar(W, H), "downsample_device input")
+        _need(d_dst, _planar(max(1, W // 2), max(1, H // 2)), "downsample_device output")
         pp = _f32c(params) if params is not None else None
         st = self.lib.ntx_hip_downsample_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
                                                 wrap, alpha_mode, d_src.ptr, W, H, d_dst.ptr)
@@ -346,6 +391,9 @@ class Context:
         dw, dp = DEFAULT_FILTER[filt]
         width = dw if width is None else width
         params = dp if params is None else params
+        _dims(w, h, "downsample_chain_device")
+        for k, (a, b) in enumerate(mip_sizes(w, h)[:len(d_levels)]):
+            _need(d_levels[k], _planar(a, b), f"downsample_chain_device level {k}")
         pp = _f32c(params) if params is not None else None
         lv = (_P * len(d_levels))(*[d.ptr for d in d_levels])
         st = self.lib.ntx_hip_downsample_chain_device(self.handle, filt, width, _ptr(pp) if pp is not None else None,
@@ -353,19 +401,26 @@ class Context:
         self._check(st, "ntx_hip_downsample_chain_device")
 
     def srgb_to_linear_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
+        _dims(w, h, "srgb_to_linear_device")
+        _need(d_rgba, _planar(w, h), "srgb_to_linear_device")
         self._check(self.lib.ntx_hip_srgb_to_linear_device(self.handle, d_rgba.ptr, w, h), "srgb_to_linear")
 
     def unpack_bgra8_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
+        self._need_unpack(d_bgra, w, h, d_rgba, "unpack_bgra8_device")
         self._check(self.lib.ntx_hip_unpack_bgra8_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack")
 
     def unpack_bgra8_srgb_device(self, d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer) -> None:
         """unpack_bgra8_device followed by srgb_to_linear_device, in one pass."""
+        self._need_unpack(d_bgra, w, h, d_rgba, "unpack_bgra8_srgb_device")
         self._check(self.lib.ntx_hip_unpack_bgra8_srgb_device(self.handle, d_bgra.ptr, w, h, d_rgba.ptr), "unpack_srgb")
 
     def gamma_device(self, to_gamma: bool, gamma: float, d_src: DeviceBuffer, w: int, h: int,
                      d_dst: Optional[DeviceBuffer] = None) -> None:
         """FloatImage::toLinear / toGamma on R, G, B (ntx_hip_gamma_device); in place without d_dst."""
         dst = d_src if d_dst is None else d_dst
+        _dims(w, h, "gamma_device")
+        _need(d_src, _planar(w, h), "gamma_device input")
+        _need(dst, _planar(w, h), "gamma_device output")
         self._check(self.lib.ntx_hip_gamma_device(self.handle, int(bool(to_gamma)), gamma, d_src.ptr, w, h, dst.ptr),
                     "ntx_hip_gamma_device")
 
@@ -373,6 +428,12 @@ class Context:
                             sizes: Sequence[Tuple[int, int]], d_dst: Sequence[DeviceBuffer]) -> None:
         """gamma_device over several images in one launch (ntx_hip_gamma_levels_device)."""
         n = len(d_src)
+        if len(sizes) != n or len(d_dst) != n:
+            raise ValueError("gamma_levels_device: one size and one output per level")
+        for k, (a, b) in enumerate(sizes):
+            _dims(a, b, "gamma_levels_device")
+            _need(d_src[k], _planar(a, b), f"gamma_levels_device input {k}")
+            _need(d_dst[k], _planar(a, b), f"gamma_levels_device output {k}")
         src = (_P * n)(*[d.ptr for d in d_src])
         dst = (_P * n)(*[d.ptr for d in d_dst])
         ws = (_I * n)(*[int(w) for w, _ in sizes])
@@ -384,17 +445,24 @@ class Context:
     def unpack_bgra8_gamma_device(self, d_bgra: DeviceBuffer, w: int, h: int, gamma: float,
                                   d_rgba: DeviceBuffer) -> None:
         """unpack_bgra8_device followed by Surface::toLinear(gamma), in one pass."""
+        self._need_unpack(d_bgra, w, h, d_rgba, "unpack_bgra8_gamma_device")
         self._check(self.lib.ntx_hip_unpack_bgra8_gamma_device(self.handle, d_bgra.ptr, w, h, gamma, d_rgba.ptr),
                     "ntx_hip_unpack_bgra8_gamma_device")
 
     def decode_device(self, fmt: int, d_blocks: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer,
                       decoder: int = DECODER_D3D10) -> None:
         """Surface::setImage2D for BC1/BC3/BC4/BC5 (ntx_hip_decode_device)."""
+        _dims(w, h, "decode_device")
+        _need(d_blocks, compressed_size(fmt, w, h), "decode_device blocks")
+        _need(d_rgba, _planar(w, h), "decode_device output")
         self._check(self.lib.ntx_hip_decode_device(self.handle, fmt, decoder, d_blocks.ptr, w, h, d_rgba.ptr),
                     "ntx_hip_decode_device")
 
     def rms_error_device(self, metric: int, d_ref: DeviceBuffer, d_img: DeviceBuffer, w: int, h: int) -> float:
         """nvtt::rmsError / rmsAlphaError (ntx_hip_rms_error_device)."""
+        _dims(w, h, "rms_error_device")
+        _need(d_ref, _planar(w, h), "rms_error_device reference")
+        _need(d_img, _planar(w, h), "rms_error_device image")
         r = _F()
         self._check(self.lib.ntx_hip_rms_error_device(self.handle, metric, d_ref.ptr, d_img.ptr, w, h, ctypes.byref(r)),
                     "ntx_hip_rms_error_device")
@@ -402,16 +470,27 @@ class Context:
 
     def normalize_normals_device(self, d_rgba: DeviceBuffer, w: int, h: int) -> None:
         """expandNormals + normalizeNormalMap + packNormals in place (ntx_hip_normalize_normals_device)."""
+        _dims(w, h, "normalize_normals_device")
+        _need(d_rgba, _planar(w, h), "normalize_normals_device")
         self._check(self.lib.ntx_hip_normalize_normals_device(self.handle, d_rgba.ptr, w, h),
                     "ntx_hip_normalize_normals_device")
 
     def bc1_psnr_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
         """PSNR (dB) of a BC1 level against its FP32 source, computed on the device."""
+        _dims(w, h, "bc1_psnr_device")
+        _need(d_rgba, _planar(w, h), "bc1_psnr_device image")
+        _need(d_blocks, compressed_size(FORMAT_BC1, w, h), "bc1_psnr_device blocks")
         sse = ctypes.c_double()
         self._check(self.lib.ntx_hip_bc1_sse_device(self.handle, d_rgba.ptr, w, h, d_blocks.ptr, ctypes.byref(sse)),
                     "bc1_sse")
         mse = sse.value / (w * h)
         return float("inf") if mse == 0 else float(20.0 * np.log10(255.0 / np.sqrt(mse)))
+
+    @staticmethod
+    def _need_unpack(d_bgra: DeviceBuffer, w: int, h: int, d_rgba: DeviceBuffer, what: str) -> None:
+        _dims(w, h, what)
+        _need(d_bgra, 4 * int(w) * int(h), what + " input")
+        _need(d_rgba, _planar(w, h), what + " output")
 
     def synchronize(self) -> None:
         self._check(self.lib.ntx_hip_synchronize(self.handle), "synchronize")
@@ -541,8 +620,8 @@ class Mlp:
         self.ctx = ctx
         h = _P()
         pp = None if params is None else _f32c(params)
-        if pp is not None:
-            assert pp.size == MLP_PARAMS
+        if pp is not None and pp.size != MLP_PARAMS:
+            raise ValueError(f"Mlp: {pp.size} parameters, the network has {MLP_PARAMS}")
         ctx._check(ctx.lib.ntx_hip_mlp_create(ctx.handle, _ptr(pp) if pp is not None else None, seed, ctypes.byref(h)),
                    "ntx_hip_mlp_create")
         self.handle = h.value
@@ -565,7 +644,8 @@ class Mlp:
 
     def set_params(self, params: np.ndarray) -> None:
         pp = _f32c(params)
-        assert pp.size == MLP_PARAMS
+        if pp.size != MLP_PARAMS:
+            raise ValueError(f"set_params: {pp.size} parameters, the network has {MLP_PARAMS}")
         self.ctx._check(self.ctx.lib.ntx_hip_mlp_set_params(self.handle, _ptr(pp)), "mlp_set_params")
 
     def forward(self, d_features: int, n: int) -> np.ndarray:
@@ -588,6 +668,9 @@ class Mlp:
     def compress_bc1_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer, tau_pred: float = 4.0,
                             tau_gap: float = -1.0, refine_iters: int = 2, greedy_rounds: int = 2,
                             box_candidates: bool = False, count_fallback: bool = True) -> Optional[int]:
+        _dims(w, h, "compress_bc1_device")
+        _need(d_rgba, _planar(w, h), "compress_bc1_device input")
+        _need(d_out, compressed_size(FORMAT_BC1, w, h), "compress_bc1_device output")
         fb = _I()
         st = self.ctx.lib.ntx_hip_compress_bc1_mlp_device(self.handle, d_rgba.ptr, w, h, float(tau_pred), float(tau_gap),
                                                           int(refine_iters), int(greedy_rounds),
@@ -599,12 +682,23 @@ class Mlp:
 
 def mlp_features_device(ctx: Context, d_rgba: DeviceBuffer, w: int, h: int, d_features: DeviceBuffer,
                         d_aux: DeviceBuffer) -> None:
+    nb = ((w + 3) // 4) * ((h + 3) // 4)
+    _dims(w, h, "mlp_features_device")
+    _need(d_rgba, _planar(w, h), "mlp_features_device input")
+    _need(d_features, 2 * 48 * nb, "mlp_features_device features")
+    _need(d_aux, 16 * nb, "mlp_features_device aux")
     ctx._check(ctx.lib.ntx_hip_mlp_features_device(ctx.handle, d_rgba.ptr, w, h, d_features.ptr, d_aux.ptr),
                "mlp_features")
 
 
 def mlp_labels_device(ctx: Context, d_rgba: DeviceBuffer, w: int, h: int, d_bc1: DeviceBuffer, d_aux: DeviceBuffer,
                       d_labels: DeviceBuffer) -> None:
+    nb = ((w + 3) // 4) * ((h + 3) // 4)
+    _dims(w, h, "mlp_labels_device")
+    _need(d_rgba, _planar(w, h), "mlp_labels_device input")
+    _need(d_bc1, 8 * nb, "mlp_labels_device blocks")
+    _need(d_aux, 16 * nb, "mlp_labels_device aux")
+    _need(d_labels, 4 * 8 * nb, "mlp_labels_device labels")
     ctx._check(ctx.lib.ntx_hip_mlp_labels_device(ctx.handle, d_rgba.ptr, w, h, d_bc1.ptr, d_aux.ptr, d_labels.ptr),
                "mlp_labels")
 

@@ -70,7 +70,16 @@ typedef enum {
 
 typedef enum { NVTT_Quality_Fastest, NVTT_Quality_Normal, NVTT_Quality_Production, NVTT_Quality_Highest } NvttQuality;
 typedef enum { NVTT_WrapMode_Clamp, NVTT_WrapMode_Repeat, NVTT_WrapMode_Mirror } NvttWrapMode;
-typedef enum { NVTT_TextureType_2D, NVTT_TextureType_Cube, NVTT_TextureType_3D, NVTT_TextureType_Array } NvttTextureType;
+/* The reference spells the last two without the NVTT_ prefix
+ * (nvtt_wrapper.h:131-135); both spellings name the same values. */
+typedef enum {
+    NVTT_TextureType_2D,
+    NVTT_TextureType_Cube,
+    TextureType_3D,
+    TextureType_Array,
+    NVTT_TextureType_3D = TextureType_3D,
+    NVTT_TextureType_Array = TextureType_Array
+} NvttTextureType;
 typedef enum {
     NVTT_InputFormat_BGRA_8UB,
     NVTT_InputFormat_RGBA_16F,

@@ -135,6 +135,10 @@ static int run_process(int argc, char **argv)
 
 static int run_info(void)
 {
+    /* The reference header's spellings of the texture types. */
+    const NvttTextureType t3 = TextureType_3D, ta = TextureType_Array;
+    if (t3 != NVTT_TextureType_3D || ta != NVTT_TextureType_Array || (int)t3 != 2 || (int)ta != 3)
+        return fail("texture type values");
     printf("version %u\n", nvttVersion());
     for (int e = NVTT_Error_Unknown; e <= NVTT_Error_UnsupportedOutputFormat; e++)
         printf("error %d %s\n", e, nvttErrorString((NvttError)e));
