@@ -147,6 +147,20 @@ NTX_API ntx_status ntx_hip_compress(ntx_hip_context *ctx, int format, int qualit
 NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
                                            const float *color_weights, const float *d_rgba, int w, int h,
                                            void *d_out);
+/* ntx_hip_compress_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, hosts that forbid
+ * hipMalloc); no counterpart in the reference.  Same arguments, same bytes;
+ * calls no allocating routine.  The scratch of the encoders that need one
+ * (BC3 / BC3n, and BC1 at Level8 when the two-pass encoder applies) is a
+ * device array of the library, 8.7 MB per GPU, held until the process ends:
+ * such images may have at most 65,536 4x4 blocks (1024 x 1024 texels), else
+ * NTX_ERR_INVALID_ARGUMENT (BC1 above that uses its one-pass encoder, which
+ * needs none).  Every call on a GPU shares its array: calls that overlap on
+ * one GPU (several threads or contexts) race. */
+NTX_API ntx_status ntx_hip_compress_device_noalloc(ntx_hip_context *ctx, int format, int quality, int alpha_mode,
+                                                   const float *color_weights, const float *d_rgba, int w, int h,
+                                                   void *d_out);
+
 /* BC1 at icbc Level8 (Normal / Highest) has two encoders with identical
  * output: one block per 16-lane group in one pass, and, for images of at
  * least `min_blocks` 4x4 blocks, two passes with one block per lane (colour

@@ -235,6 +235,7 @@ NTX_API ntx_status ntx_hip_create(int device, ntx_hip_context **out)
     if (e == hipSuccess) e = hipMemcpy(ctx->d_srgb, lut, sizeof(lut), hipMemcpyHostToDevice);
     delete tab;
     if (e != hipSuccess) return cleanup(e == hipErrorOutOfMemory ? NTX_ERR_OUT_OF_MEMORY : NTX_ERR_DEVICE);
+    if (ntx_status s = ntx::noalloc_init(ctx)) return cleanup(s);
     *out = ctx;
     return NTX_OK;
 }
@@ -418,7 +419,7 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
                               bool *counts_zero, int lanes_log2 = 18, bool urgent = false, int16_t *d_nm = nullptr)
 {
     hipError_t e = hipSuccess;
-    if (format == NTX_FORMAT_BC1 && bc1_two_pass(ctx, quality, w, h)) {
+    if (format == NTX_FORMAT_BC1 && bucket && bc1_two_pass(ctx, quality, w, h)) {
         const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
         int *counts = bucket, *lists = counts + bc3_lists_offset();
         if (!*counts_zero) e = launch_bc3_clear_counts(counts, stream);
@@ -467,6 +468,13 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
                      [&] { return launch_channels(d_rgba, w, h, format, d_out, stream, urgent); });
     }
     return e;
+}
+
+hipError_t compress_scratch(ntx_hip_context *ctx, int format, int quality, int alpha_mode, const float *cw,
+                            const float *d_rgba, int w, int h, void *d_out, int *bucket)
+{
+    bool counts_zero = false;  // cleared by a fill kernel first
+    return compress_on(ctx, ctx->stream, format, quality, alpha_mode, cw, d_rgba, w, h, d_out, bucket, &counts_zero);
 }
 
 // Surface::buildNextMipmap of one level enqueued on `stream` (the fallback

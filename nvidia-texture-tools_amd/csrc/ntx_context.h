@@ -143,6 +143,14 @@ int bc1_level(int quality);
 bool bc1_two_pass(const ntx_hip_context *ctx, int quality, int w, int h);
 // Bucket scratch (ints) one image's compression needs: BC3, two-pass BC1.
 size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, int h);
+// One image's compression on the context's stream with the given bucket
+// scratch (null: BC1 takes its one-pass encoder); the tallies are cleared by
+// a fill kernel first.
+hipError_t compress_scratch(ntx_hip_context *ctx, int format, int quality, int alpha_mode, const float *cw,
+                            const float *d_rgba, int w, int h, void *d_out, int *bucket);
+// Looks up the device address of the no-allocation entry point's scratch
+// array on the context's (current) device; once per device.
+ntx_status noalloc_init(ntx_hip_context *ctx);
 // Polyphase weights and first-tap indices for src -> dst samples, cached per context.
 ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const float *params, int src, int dst, PolyDev **out);
 

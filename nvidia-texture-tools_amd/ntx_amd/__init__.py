@@ -45,6 +45,7 @@ EXPORTS = (
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device", "ntx_hip_set_bc1_lanes_min",
+    "ntx_hip_compress_device_noalloc",
     "ntx_hip_compress_chain_device", "ntx_hip_compress_images_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
@@ -126,6 +127,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress_device": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
         "ntx_hip_set_bc1_lanes_min": (_I, [_P, _I]),
+        "ntx_hip_compress_device_noalloc": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
         "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
         "ntx_hip_compress_images_device": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),
@@ -351,6 +353,19 @@ class Context:
         st = self.lib.ntx_hip_compress_images_device(self.handle, fmt, quality, alpha_mode, _ptr(cw), n,
                                                      ctypes.cast(ins, _P), _ptr(ws), _ptr(hs), ctypes.cast(outs, _P))
         self._check(st, "ntx_hip_compress_images_device")
+
+    def compress_device_noalloc(self, fmt: int, d_rgba: DeviceBuffer, w: int, h: int, d_out: DeviceBuffer,
+                                quality: int = QUALITY_NORMAL, alpha_mode: int = ALPHA_NONE,
+                                color_weights: Sequence[float] = (1, 1, 1, 1)) -> None:
+        """compress_device without any device allocation (the library's own
+        scratch array; images needing it are limited to 65,536 blocks)."""
+        _dims(w, h, "compress_device_noalloc")
+        _need(d_rgba, _planar(w, h), "compress_device_noalloc input")
+        _need(d_out, compressed_size(fmt, w, h), "compress_device_noalloc output")
+        cw = _f32c(color_weights)
+        st = self.lib.ntx_hip_compress_device_noalloc(self.handle, fmt, quality, alpha_mode, _ptr(cw), d_rgba.ptr, w, h,
+                                                      d_out.ptr)
+        self._check(st, "ntx_hip_compress_device_noalloc")
 
     def set_bc1_lanes_min(self, min_blocks: int) -> None:
         """BC1 Level8 images of at least min_blocks blocks take the two-pass
