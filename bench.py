@@ -1,18 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the NVTT hot path on one MI355X.
 
-Workload (one step): a 4096x4096 RGBA8 texture from the seeded
-noise+rectangles generator, alpha thresholded into 30% transparent texels,
-already resident in HBM -> Surface::setImage(BGRA_8UB) unpack ->
-toLinearFromSrgb -> the full 13-level Kaiser(width 3, alpha 4, stretch 1)
-mip chain in FP32 -> BC3 (QuickCompress alpha + squish cluster-fit colour)
-for every level.  value = texels compressed per second over the chain.
+Headline (`value`): BC1 Mtexels/s on the 16384^2 workload (16,777,216 4x4
+blocks of RGBA8 noise+rectangles, already resident in HBM as planar FP32) at
+PSNR within 0.05 dB of the reference's cluster fit.  Two forms are timed over
+the same `--steps`: the plain cluster fit (icbc Level8, byte-identical to the
+reference) and the learned form (network proposals from the project's
+BF16-MFMA network trained on the device beforehand, with the full cluster fit
+as the fallback for blocks whose predicted error exceeds tau_pred, at the
+shipped settings ntx_amd.learned.SHIPPED).  `value` is the learned form's
+rate when its PSNR is within 0.05 dB of the plain fit's, else the plain fit's.
+The CPU baseline is the reference's own BC1 compressor (compiled from the
+reference sources into oracle/_ref) on the host's cores over the same image.
 
-Also measured on the same chain: BC1 (icbc Level8) Mtexels/s and the Kaiser
-mip chain's algorithmic HBM GB/s.  The CPU baseline is the oracle (CPU
-restatement, threaded like nv::ParallelFor, mips single-threaded like the
-reference) on the same full workload, plus the reference's own icbc (compiled
-from the reference sources into oracle/_ref) on the BC1 chain.
+Secondary legs: the 4096^2 sRGB->Kaiser->BC3 chain (with the mip chain's HBM
+GB/s, the metric's second part), the 1024^2 Box chain (C1), the InputOptions
+pipeline, the 256-texture batch (C3) and the cube map (C5).
 """
 import argparse
 import json
@@ -28,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "nvidia-texture-tools_amd"))
 
 METRIC = "BC1 Mtexels/s at PSNR within 0.05 dB of ref; mip-chain HBM GB/s"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 
 # Algorithmic flops, counted from the reference loops each kernel restates:
@@ -39,6 +43,17 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 #  single-colour blocks take the table path (0).
 SQUISH_PER_CANDIDATE = 137
 SQUISH_SETUP = 1500
+# The proposal form's kernel (bc1_mlp_kernel, no least-squares pass, no
+# greedy round), per block: the network forward on the matrix cores,
+# 2 * (48*64 + 64*64 + 64*25) = 17,536 flops; on the vector units, the
+# features (block mean, largest deviation, scaling: 250), then for each of
+# the 4 candidate lanes the colour bounding box (16 texels x 6 min/max = 96),
+# the diagonal and the network's offsets (20) and two encodings (5:6:5
+# endpoints, D3D10 palette, and for 16 texels 4 squared distances of 8 flops,
+# the nearest of 4 and the weighted error: 2 x 650), and the uniform-block
+# test (48): 250 + 4 * (96 + 20 + 1300) + 48 = 5,962 flops.
+MLP_FORWARD_FLOPS = 2 * (48 * 64 + 64 * 64 + 64 * 25)
+MLP_VALU_FLOPS = 250 + 4 * (96 + 20 + 2 * 650) + 48
 
 # HBM bytes per launch of bc3_search_kernel on the 4096^2 chain (mean of its
 # two launches, levels 0 and 1): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from the
@@ -83,28 +98,344 @@ def psnr(a: np.ndarray, b: np.ndarray) -> float:
     return float("inf") if mse == 0 else float(20 * np.log10(255.0 / np.sqrt(mse)))
 
 
+def host_threads(cap: int) -> int:
+    """Worker threads for the CPU baselines: the CPUs this process may run
+    on, at most `cap` (the box's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(cap, n))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=4096, help="edge of the BC3 chain leg's texture")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-secondary", action="store_true", help="only the headline (C4) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-workload leg")
     ap.add_argument("--batch-textures", type=int, default=256)
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal BC1 and cube-map legs")
-    ap.add_argument("--learned-size", type=int, default=16384)
+    ap.add_argument("--learned-size", type=int, default=16384, help="edge of the headline texture")
     ap.add_argument("--profile-pass", action="store_true",
-                    help="short run for rocprofv3 counter passes: the BC3 chain, the Kaiser chain, the BC1 chain "
-                         "(2 steps each) and a 256 MiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
+                    help="short run for rocprofv3 counter passes: the headline's two forms, the BC3 chain, the "
+                         "Kaiser chain, the BC1 chain (2 steps each) and a 256 MiB device copy used to calibrate "
+                         "FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the kernels run on a single MI355X; sharding by texture or block band is a design note only)",
               file=sys.stderr)
         sys.exit(2)
 
+    import ntx_amd as N
+
+    ctx = N.Context(0)
+    net, training = train_network(ctx, args)
+    result = bench_c4(ctx, args, net, training)
+    if args.profile_pass:
+        profile_pass(ctx, args)
+        net.close()
+        ctx.close()
+        return
+    if not args.no_secondary:
+        c2 = bench_c2(ctx, args)
+        result["mip_chain_gbs"] = c2.pop("mip_chain_gbs")
+        result["mip_roofline"] = c2.pop("mip_roofline")
+        result["c2_bc3_chain"] = c2
+        result["c1_box_chain_bc1"] = bench_c1(ctx, args, not args.no_cpu)
+        result["input_pipeline"] = bench_process(args, not args.no_cpu)
+        if not args.no_batch:
+            result["batch"] = bench_batch(ctx, args)
+        result["cube_map"] = bench_cube(ctx, args, net)
+    net.close()
+    print(json.dumps(result), flush=True)
+    ctx.close()
+
+
+def train_network(ctx, args):
+    """The 48-64-64-25 network (BF16, FP32 accumulation) trained on the
+    device with the project's own kernels on the blocks of 8 seeded 2048^2
+    noise+rectangle textures, each in 4 rotations (8,388,608 blocks cut; the
+    uniform ones dropped), labelled by the cluster fit.  Setup, not timed."""
+    import ntx_amd as N
+    from ntx_amd import learned, synth
+
+    t0 = time.perf_counter()
+    texs = []
+    for i in range(8):
+        img = synth.noise_rect_rgba(2048, 2048, seed=args.seed + 2000 + i)
+        for r in range(4):
+            texs.append((synth.to_planar(np.ascontiguousarray(np.rot90(img, r))).reshape(-1), 2048, 2048))
+    cut = sum(((w + 3) // 4) * ((h + 3) // 4) for _, w, h in texs)
+    data = learned.make_dataset(ctx, texs, seed=args.seed)
+    del texs
+    data_s = time.perf_counter() - t0
+    net = N.Mlp(ctx, seed=args.seed)
+    log(f"training on {data.n} non-uniform blocks of {cut} cut")
+    first, _ = net.train_step(data.feat(0), data.label(0), min(data.n, 65536), 0.0)
+    t0 = time.perf_counter()
+    epochs, batch = 3, 16384
+    losses = learned.train(net, data, epochs=epochs, batch=batch, lr=3e-3)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    steps = epochs * ((data.n + batch - 1) // batch)
+    n = data.n
+    data.free()
+    return net, {"textures": "8 seeded 2048^2 noise+rectangle textures (seeds %d-%d) x 4 rotations"
+                             % (args.seed + 2000, args.seed + 2007),
+                 "blocks_cut": cut, "blocks_used": n, "epochs": epochs, "batch": batch, "adam_steps": steps,
+                 "seconds": round(train_s, 3), "data_seconds": round(data_s, 2), "loss_before": round(first, 5),
+                 "loss_per_epoch": [round(x, 5) for x in losses]}
+
+
+def c4_input(ctx, S: int, seed: int):
+    """16 tiles of 4 seeded (S/4)^2 noise+rectangle textures in 4 rotations,
+    uploaded as BGRA8 and unpacked on the device to planar FP32."""
+    import ntx_amd as N
+    from ntx_amd import synth
+
+    T = S // 4
+    base = [synth.noise_rect_rgba(T, T, seed=seed + 3000 + i) for i in range(4)]
+    big = np.empty((S, S, 4), np.uint8)
+    for k in range(16):
+        ty, tx = divmod(k, 4)
+        big[ty * T:(ty + 1) * T, tx * T:(tx + 1) * T] = np.rot90(base[k % 4], k // 4)
+    del base
+    bgra = synth.to_bgra(big)
+    del big
+    d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
+    del bgra
+    d = ctx.alloc(16 * S * S)
+    ctx.unpack_bgra8_device(d_bgra, S, S, d)
+    d_bgra.free()
+    return d
+
+
+def bench_c4(ctx, args, net, training) -> dict:
+    """The headline: BC1 of the S^2 texture by the plain cluster fit and by
+    the learned form, each timed over args.steps (after args.warmup), PSNR of
+    both on the device, the per-kernel breakdown from a separate profiled pass
+    of the same length, the rooflines and the CPU reference."""
+    import ntx_amd as N
+    from ntx_amd import learned
+
+    S = args.learned_size
+    texels = S * S
+    nb = (S // 4) * (S // 4)
+    log(f"headline: {S}x{S} input")
+    d = c4_input(ctx, S, args.seed)
+    d_plain = ctx.alloc(8 * nb)
+    d_mlp = ctx.alloc(8 * nb)
+    kw = dict(learned.SHIPPED)
+
+    def plain():
+        ctx.compress_device(N.FORMAT_BC1, d, S, S, d_plain)
+
+    def proposals():
+        net.compress_bc1_device(d, S, S, d_mlp, count_fallback=False, **kw)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.timer_begin()
+        for _ in range(args.steps):
+            fn()
+        ms = ctx.timer_end()
+        wall = time.perf_counter() - t0
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        for _ in range(args.steps):
+            fn()
+        prof = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        return ms / args.steps, wall * 1e3 / args.steps, {k: (v[0] / args.steps, v[1] / args.steps)
+                                                          for k, v in prof.items()}
+
+    ms_plain, wall_plain, prof_plain = timed(plain)
+    ms_mlp, wall_mlp, prof_mlp = timed(proposals)
+    psnr_plain = ctx.bc1_psnr_device(d, S, S, d_plain)
+    psnr_mlp = ctx.bc1_psnr_device(d, S, S, d_mlp)
+    fallback = net.compress_bc1_device(d, S, S, d_mlp, **kw)
+    delta = psnr_mlp - psnr_plain
+    within = delta >= -0.05
+    rate_plain = texels / (ms_plain / 1e3) / 1e6
+    rate_mlp = texels / (ms_mlp / 1e3) / 1e6
+    log(f"  plain {ms_plain:.3f} ms ({rate_plain:.0f} Mtexels/s, {psnr_plain:.3f} dB); learned {ms_mlp:.3f} ms "
+        f"({rate_mlp:.0f} Mtexels/s, {delta:+.3f} dB, fallback {fallback / nb:.4f})")
+
+    # Counted flops of the plain fit from its own per-block colour counts.
+    d_nm = ctx.alloc(4 * nb)
+    ctx.compress_bc1_stats_device(d, S, S, d_plain, d_nm)
+    nm = d_nm.download(np.int16).reshape(-1, 2)
+    d_nm.free()
+    flops_plain = bc1_flops(nm)
+    hist = np.bincount(nm[:, 0].clip(0, 16), minlength=17)
+    search_ms = prof_plain.get("bc1_search_kernel", (0.0, 0))[0]
+    search_flops = bc1_flops(nm[nm[:, 0] >= 2])
+    plain_roofline = {"kernel": "bc1_search_kernel", "bound": "valu",
+                      "achieved": round(search_flops / (search_ms / 1e3) / 1e12, 3) if search_ms else None,
+                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                      "frac": round(search_flops / (search_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4) if search_ms else None,
+                      "traffic": None, "avg_launch_ms": round(search_ms, 4),
+                      "algorithmic_flops_per_launch": search_flops,
+                      "note": "icbc Level8 flops counted per block from its colour counts (n, m), blocks of two or "
+                              "more colours; the classify pass finishes the others"}
+    # The learned step's dominant kernel.
+    mlp_ms = prof_mlp.get("bc1_mlp_kernel", (0.0, 0))[0]
+    valu = MLP_VALU_FLOPS * nb / (mlp_ms / 1e3) / 1e12
+    mfma = MLP_FORWARD_FLOPS * nb / (mlp_ms / 1e3) / 1e12
+    mlp_roofline = {"kernel": "bc1_mlp_kernel", "bound": "valu", "achieved": round(valu, 3), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(valu / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "avg_launch_ms": round(mlp_ms, 4), "algorithmic_flops_per_launch": MLP_VALU_FLOPS * nb,
+                    "mfma_share": {"achieved": round(mfma, 3), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": round(mfma / PEAK_BF16_TFLOPS, 5),
+                                   "algorithmic_flops_per_launch": MLP_FORWARD_FLOPS * nb}}
+    head_is_learned = within
+    dominant = max(prof_mlp if head_is_learned else prof_plain, key=lambda k: (prof_mlp if head_is_learned else
+                                                                                prof_plain)[k][0])
+    roofline = mlp_roofline if head_is_learned else plain_roofline
+    roofline = dict(roofline, dominant_kernel_of_step=dominant)
+
+    result = {
+        "metric": METRIC,
+        "value": round(rate_mlp if head_is_learned else rate_plain, 2),
+        "unit": "Mtexels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mlp if head_is_learned else ms_plain, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{S}x{S} RGBA8 noise+rectangles (16 tiles: 4 seeded {S // 4}^2 textures x 4 rotations), "
+                        f"planar FP32 in HBM -> BC1 ({'learned proposals + cluster-fit fallback' if head_is_learned else 'plain cluster fit'})",
+            "texels_per_step": texels,
+            "blocks_per_step": nb,
+            "form": "learned" if head_is_learned else "plain",
+            "input_resident_in_hbm": True,
+        },
+        "roofline": roofline,
+        "psnr_gate": {"plain_db": round(psnr_plain, 4), "learned_db": round(psnr_mlp, 4), "delta_db": round(delta, 4),
+                      "limit_db": -0.05, "within": bool(within)},
+        "plain_cluster_fit": {"ms_per_step": round(ms_plain, 4), "wall_ms_per_step": round(wall_plain, 4),
+                              "mtexels_s": round(rate_plain, 2), "psnr_db": round(psnr_plain, 4),
+                              "kernels_ms_per_step": {k: round(v[0], 4) for k, v in prof_plain.items()},
+                              "roofline": plain_roofline,
+                              "algorithmic_flops_per_step": flops_plain,
+                              "colour_count_histogram": hist.tolist()},
+        "learned": {"ms_per_step": round(ms_mlp, 4), "wall_ms_per_step": round(wall_mlp, 4),
+                    "mtexels_s": round(rate_mlp, 2), "psnr_db": round(psnr_mlp, 4),
+                    "speedup_vs_plain": round(ms_plain / ms_mlp, 3), "settings": kw,
+                    "fallback_blocks": int(fallback), "fallback_fraction": round(fallback / nb, 5),
+                    "kernels_ms_per_step": {k: round(v[0], 4) for k, v in prof_mlp.items()},
+                    "roofline": mlp_roofline,
+                    "target": "learned >= 3x plain at PSNR within 0.05 dB",
+                    "meets_target": bool(ms_plain / ms_mlp >= 3.0 and within),
+                    "training": training},
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_reference_bc1(ctx, args, d, S, d_plain, texels)
+    for b in (d, d_plain, d_mlp):
+        b.free()
+    return result
+
+
+def cpu_reference_bc1(ctx, args, d, S, d_plain, texels) -> dict:
+    """The reference's own BC1 compressor (CompressorDXT1 -> icbc Level8,
+    compiled from the reference sources into oracle/_ref) over the same image
+    on the host, with the reference's task loop (one block per task from an
+    atomic counter) on host_threads() workers; its blocks against the
+    device's plain fit, and the oracle port's rate beside it on a 2048^2
+    crop (the port/reference calibration)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    ref = oracle_lib.Reference.load()
+    threads = host_threads(args.cpu_threads)
+    planar = d.download(np.float32)
+    gpu = d_plain.download(np.uint8)
+    info = {"unit": "Mtexels/s", "cores": threads, "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    if ref is None:
+        return dict(info, value=None, kind="reference", sample="oracle/_ref not built")
+    log(f"CPU reference: BC1 of {S}x{S} on {threads} threads")
+    t0 = time.perf_counter()
+    out = ref.compress_image(1, 1, planar, S, S, threads=threads)
+    sec = time.perf_counter() - t0
+    same = float(np.mean(np.all(out.reshape(-1, 8) == gpu.reshape(-1, 8), axis=1)))
+    # Port / reference calibration on a 2048^2 crop of the same image.
+    C = min(2048, S)
+    crop = np.ascontiguousarray(planar.reshape(4, S, S)[:, :C, :C]).reshape(-1)
+    orc = oracle_lib.Oracle()
+    t0 = time.perf_counter()
+    a = ref.compress_image(1, 1, crop, C, C, threads=threads)
+    t_ref = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = orc.compress(1, crop, C, C, threads=threads)
+    t_port = time.perf_counter() - t0
+    return dict(info, value=round(texels / sec / 1e6, 3), kind="reference",
+                sample=f"the whole headline image ({texels} texels) through the reference's CompressorDXT1 "
+                       f"(icbc Level8, ICBC_SIMD=2 build of the reference sources)",
+                seconds=round(sec, 3), blocks_identical_to_device=same,
+                port_calibration={"crop": f"{C}x{C}", "reference_mtexels_s": round(C * C / t_ref / 1e6, 3),
+                                  "port_mtexels_s": round(C * C / t_port / 1e6, 3),
+                                  "port_over_reference_time": round(t_port / t_ref, 3),
+                                  "blocks_identical": bool(np.array_equal(a, b))})
+
+
+def profile_pass(ctx, args):
+    """A short run for rocprofv3 counter passes (see --profile-pass)."""
+    import ntx_amd as N
+
+    S = args.size
+    levels = N.mip_sizes(S, S)
+    from ntx_amd import synth
+    img = synth.noise_rect_rgba(S, S, seed=args.seed, transparent=True)
+    bgra = synth.to_bgra(img)
+    d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
+    d_lv = [ctx.alloc(16 * w * h) for w, h in levels]
+    d_bc3 = [ctx.alloc(N.compressed_size(N.FORMAT_BC3, w, h)) for w, h in levels]
+    d_bc1 = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels]
+    kw, kp = 3.0, (4.0, 1.0)
+    for _ in range(2):
+        ctx.unpack_bgra8_srgb_device(d_bgra, S, S, d_lv[0])
+        ctx.compress_chain_device(N.FORMAT_BC3, N.FILTER_KAISER, d_lv, S, S, d_bc3, width=kw, params=kp)
+    for _ in range(2):
+        ctx.downsample_chain_device(N.FILTER_KAISER, d_lv, S, S, kw, kp)
+    for _ in range(2):
+        for k, (w, h) in enumerate(levels):
+            ctx.compress_device(N.FORMAT_BC1, d_lv[k], w, h, d_bc1[k])
+    ctx.synchronize()
+    nbytes = 256 << 20
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    for _ in range(3):
+        ctx.lib.ntx_hip_copy_device(ctx.handle, b.ptr, a.ptr, nbytes)
+    ctx.synchronize()
+    log(f"profile pass done: headline forms, BC3 chain/Kaiser/BC1 x2, 3 copies of {nbytes} bytes")
+
+
+def bench_c2(ctx, args) -> dict:
+    """The 4096^2 sRGB -> Kaiser -> BC3 chain, the Kaiser chain alone (the
+    mip-chain HBM GB/s), BC1 over the same levels, and the CPU port."""
     import ntx_amd as N
     from ntx_amd import synth
 
@@ -117,7 +448,6 @@ def main():
     img = synth.noise_rect_rgba(S, S, seed=args.seed, transparent=True)
     bgra = synth.to_bgra(img)
 
-    ctx = N.Context(0)
     d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
     d_lv = [ctx.alloc(16 * w * h) for w, h in levels]
     d_bc3 = [ctx.alloc(N.compressed_size(N.FORMAT_BC3, w, h)) for w, h in levels]
@@ -170,20 +500,6 @@ def main():
         prof = ctx.profile_read(reset=True)
         ctx.profile(False)
         return ms, wall, prof
-
-    if args.profile_pass:
-        for fn in (step_bc3_per_level, step_bc3, mips, step_bc1):
-            for _ in range(2):
-                fn()
-            ctx.synchronize()
-        nbytes = 256 << 20
-        a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
-        for _ in range(3):
-            ctx.lib.ntx_hip_copy_device(ctx.handle, b.ptr, a.ptr, nbytes)
-        ctx.synchronize()
-        log(f"profile pass done: BC3 per level/BC3 chain/Kaiser/BC1 x2, 3 copies of {nbytes} bytes")
-        ctx.close()
-        return
 
     log(f"BC3 pipeline: {args.warmup} warm-up + {args.steps} timed steps")
     ms, wall, prof = timed(step_bc3, args.steps, args.warmup)
@@ -246,7 +562,8 @@ def main():
     flops_bc1 = sum(bc1_flops(x.download(np.int16).reshape(-1, 2)) for x in nm)
     ms_bc1, _, prof_bc1 = timed(step_bc1, args.steps, 1)
     bc1_mtex_per_level = texels * args.steps / (ms_bc1 / 1e3) / 1e6
-    bc1_kernel_ms = prof_bc1["bc1_kernel"][0] / args.steps
+    bc1_kernel_ms = sum(prof_bc1.get(k, (0.0, 0))[0] for k in ("bc1_kernel", "bc1_classify_kernel",
+                                                                 "bc1_search_kernel")) / args.steps
     bc1_tflops = flops_bc1 / (bc1_kernel_ms / 1e3) / 1e12
     per_level_bc1 = [d.download(np.uint8) for d in d_bc1]
     ms_bc1i, _, _ = timed(step_bc1_images, args.steps, 1)
@@ -275,26 +592,14 @@ def main():
     gpu_bc3 = [d.download(np.uint8) for d in d_bc3]
     gpu_bc1 = [d.download(np.uint8) for d in d_bc1]
 
-    result = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "Mtexels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+
+    c2 = {
+        "workload": f"{S}x{S} RGBA8 noise+rectangles, 30% transparent alpha (seed {args.seed}) -> "
+                    f"sRGB->linear FP32 -> {len(levels)}-level Kaiser(3; 4, 1) mip chain -> BC3 per level",
+        "texels_per_step": texels,
+        "mtexels_s": round(value, 2),
         "ms_per_step": round(ms / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{S}x{S} RGBA8 noise+rectangles, 30% transparent alpha (seed {args.seed}) -> "
-                        f"sRGB->linear FP32 -> {len(levels)}-level Kaiser(3; 4, 1) mip chain -> BC3 per level",
-            "texels_per_step": texels,
-            "levels": len(levels),
-            "input_resident_in_hbm": True,
-        },
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "roofline": roofline,
         "split_search_roofline": split_roofline,
         "per_level_calls": {"ms_per_step": round(ms_pl / args.steps, 4),
@@ -311,24 +616,17 @@ def main():
                                 "per_level_calls_mtexels_s": round(bc1_mtex_per_level, 2),
                                 "blocks_identical_to_per_level": bool(bc1_images_same)},
         "bc1_qualities": bc1_q,
-        "bc1_roofline": {"kernel": "bc1_kernel", "bound": "valu", "achieved": round(bc1_tflops, 3),
-                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(bc1_tflops / PEAK_FP32_TFLOPS, 4),
-                         "algorithmic_flops_per_chain": flops_bc1, "kernel_ms_per_chain": round(bc1_kernel_ms, 4)},
+        "bc1_roofline": {"kernels": "bc1_kernel + bc1_classify_kernel + bc1_search_kernel", "bound": "valu",
+                         "achieved": round(bc1_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(bc1_tflops / PEAK_FP32_TFLOPS, 4), "algorithmic_flops_per_chain": flops_bc1,
+                         "kernel_ms_per_chain": round(bc1_kernel_ms, 4),
+                         "kernels_ms_per_chain": {k: round(v[0] / args.steps, 4) for k, v in prof_bc1.items()}},
     }
-
-    result["c1_box_chain_bc1"] = bench_c1(ctx, args, not args.no_cpu)
-    result["input_pipeline"] = bench_process(args, not args.no_cpu)
-    if not args.no_batch:
-        result["batch"] = bench_batch(ctx, args)
-    if not args.no_learned:
-        result["learned_bc1"], net = bench_learned(ctx, args)
-        result["cube_map"] = bench_cube(ctx, args, net)
-        net.close()
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads = host_threads(args.cpu_threads)
         orc = oracle_lib.Oracle()
         log(f"CPU baseline: oracle, {threads} threads, full workload")
         t0 = time.perf_counter()
@@ -348,14 +646,14 @@ def main():
         ident = sum(int(np.sum(np.all(a.reshape(-1, 16) == b.reshape(-1, 16), axis=1))) for a, b in zip(gpu_bc3, cpu_bc3))
         nblk = sum(len(a) // 16 for a in cpu_bc3)
         mips_equal = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(lv_host, cpu_lv))
-        result["cpu_baseline"] = {
+        c2["cpu_port"] = {
             "value": round(texels / cpu_s / 1e6, 3), "unit": "Mtexels/s", "cores": threads, "kind": "port",
             "sample": f"the full workload ({texels} texels, {len(levels)} levels): oracle BC3 with {threads} threads "
                       f"(one block per task, atomic counter), Kaiser mips single-threaded as in the reference "
                       f"({t_mip:.2f} s of {cpu_s:.2f} s)",
             "seconds": round(cpu_s, 3),
         }
-        result["parity"] = {"bc3_identical_blocks": ident / nblk, "bc3_levels_identical": all(same),
+        c2["parity"] = {"bc3_identical_blocks": ident / nblk, "bc3_levels_identical": all(same),
                             "mip_levels_bit_identical": bool(mips_equal)}
         ref = oracle_lib.Reference.load()
         if ref is not None:
@@ -378,13 +676,15 @@ def main():
             src8 = np.floor(np.clip(lvl0, 0, 1) * 255 + 0.5).astype(np.uint8)
             dec = lambda blk: _decode_bc1(blk, S, S)
             p_gpu, p_ref = psnr(dec(gpu_bc1[0]), src8), psnr(dec(ref_bc1[0]), src8)
-            result["cpu_reference_bc1"] = {
+            c2["cpu_reference_bc1"] = {
                 "value": round(texels / ref_s / 1e6, 3), "unit": "Mtexels/s", "cores": threads, "kind": "reference",
                 "sample": "icbc Level8 from the reference sources over the same 13 FP32 levels",
                 "bc1_identical_blocks": ident1 / nb1, "psnr_level0_gpu_db": round(p_gpu, 4),
                 "psnr_level0_ref_db": round(p_ref, 4)}
-    print(json.dumps(result), flush=True)
-    ctx.close()
+
+    for buf in [d_bgra] + d_lv + d_bc3 + d_bc1 + nm:
+        buf.free()
+    return c2
 
 
 def bench_batch(ctx, args) -> dict:
@@ -478,115 +778,6 @@ def bench_batch(ctx, args) -> dict:
     }
 
 
-def bench_learned(ctx, args):
-    """Learned endpoint proposals for BC1: the 48-64-64-25 network (BF16, FP32
-    accumulation) trained on the device on ~8M blocks cut from seeded
-    noise+rectangle textures (8 x 2048^2, each in 4 rotations) labelled by the
-    cluster fit; then a 16384^2 input compressed by the plain cluster fit and
-    by the proposal form, PSNR of both measured on the device."""
-    import ntx_amd as N
-    from ntx_amd import learned, synth
-
-    t0 = time.perf_counter()
-    texs = []
-    for i in range(8):
-        img = synth.noise_rect_rgba(2048, 2048, seed=args.seed + 2000 + i)
-        for r in range(4):
-            texs.append((synth.to_planar(np.ascontiguousarray(np.rot90(img, r))).reshape(-1), 2048, 2048))
-    cut = sum(((w + 3) // 4) * ((h + 3) // 4) for _, w, h in texs)
-    data = learned.make_dataset(ctx, texs, seed=args.seed)
-    del texs
-    data_s = time.perf_counter() - t0
-    net = N.Mlp(ctx, seed=args.seed)
-    log(f"learned: training on {data.n} non-uniform blocks of {cut} cut")
-    first, _ = net.train_step(data.feat(0), data.label(0), min(data.n, 65536), 0.0)
-    t0 = time.perf_counter()
-    epochs, batch = 3, 16384
-    losses = learned.train(net, data, epochs=epochs, batch=batch, lr=3e-3)
-    ctx.synchronize()
-    train_s = time.perf_counter() - t0
-    steps = epochs * ((data.n + batch - 1) // batch)
-    data.free()
-
-    S = args.learned_size
-    T = S // 4
-    log(f"learned: {S}x{S} input from 4 generated {T}x{T} textures in 4 rotations")
-    base = [synth.noise_rect_rgba(T, T, seed=args.seed + 3000 + i) for i in range(4)]
-    big = np.empty((S, S, 4), np.uint8)
-    for k in range(16):
-        ty, tx = divmod(k, 4)
-        big[ty * T:(ty + 1) * T, tx * T:(tx + 1) * T] = np.rot90(base[k % 4], k // 4)
-    del base
-    bgra = synth.to_bgra(big)
-    del big
-    d_bgra = ctx.alloc(bgra.nbytes).upload(bgra)
-    del bgra
-    d = ctx.alloc(16 * S * S)
-    ctx.unpack_bgra8_device(d_bgra, S, S, d)
-    d_bgra.free()
-    d_plain = ctx.alloc(N.compressed_size(N.FORMAT_BC1, S, S))
-    d_mlp = ctx.alloc(N.compressed_size(N.FORMAT_BC1, S, S))
-    texels = S * S
-
-    def timed(fn, reps=3):
-        fn()
-        ctx.synchronize()
-        best = None
-        for _ in range(reps):
-            ctx.timer_begin()
-            fn()
-            ms = ctx.timer_end()
-            best = ms if best is None else min(best, ms)
-        return best
-
-    ms_plain = timed(lambda: ctx.compress_device(N.FORMAT_BC1, d, S, S, d_plain))
-    psnr_plain = ctx.bc1_psnr_device(d, S, S, d_plain)
-    runs = []
-    for cfg in (dict(greedy_rounds=1), dict(greedy_rounds=1, refine_iters=1), dict(greedy_rounds=1, refine_iters=0),
-                dict(greedy_rounds=0, refine_iters=1), dict(greedy_rounds=0, refine_iters=0),
-                dict(greedy_rounds=1, tau_gap=0.5), dict(greedy_rounds=2),
-                dict(greedy_rounds=1, box_candidates=True), dict(greedy_rounds=1, refine_iters=1, box_candidates=True),
-                dict(greedy_rounds=1, refine_iters=0, box_candidates=True), dict(greedy_rounds=0, refine_iters=0,
-                box_candidates=True), dict(greedy_rounds=2, box_candidates=True)):
-        kw = dict(tau_pred=1e9, tau_gap=-1.0, refine_iters=2)
-        kw.update(cfg)
-        fb = net.compress_bc1_device(d, S, S, d_mlp, **kw)
-        ms = timed(lambda: net.compress_bc1_device(d, S, S, d_mlp, count_fallback=False, **kw))
-        p = ctx.bc1_psnr_device(d, S, S, d_mlp)
-        runs.append({"proposals": "box diagonals (no network)" if kw.get("box_candidates") else "network",
-                     "refine_iters": kw["refine_iters"], "greedy_rounds": kw["greedy_rounds"], "tau_gap": kw["tau_gap"],
-                     "ms": round(ms, 3),
-                     "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1), "speedup": round(ms_plain / ms, 3),
-                     "psnr_db": round(p, 4), "delta_db": round(p - psnr_plain, 4),
-                     "fallback_fraction": round(fb / (texels / 16), 4)})
-        log(f"  {runs[-1]}")
-    d.free()
-    d_plain.free()
-    d_mlp.free()
-    # Headline: the fastest network setting within 0.05 dB of the plain fit
-    # (else the best PSNR); the same for the box method without the network.
-    def pick(rs):
-        ok = [r for r in rs if r["delta_db"] >= -0.05]
-        return max(ok, key=lambda r: r["speedup"]) if ok else max(rs, key=lambda r: r["psnr_db"])
-    head = pick([r for r in runs if r["proposals"] == "network"])
-    nonet = pick([r for r in runs if r["proposals"] != "network"])
-    return {
-        "workload": f"{S}x{S} RGBA8 noise+rectangles (16 tiles: 4 seeded {T}^2 textures x 4 rotations) -> BC1; "
-                    f"network 48-64-64-25 BF16/FP32 trained on the device on {cut} blocks cut from 32 seeded 2048^2 "
-                    f"textures ({data.n} non-uniform used)",
-        "plain_cluster_fit": {"ms": round(ms_plain, 3), "mtexels_s": round(texels / (ms_plain / 1e3) / 1e6, 1),
-                              "psnr_db": round(psnr_plain, 4)},
-        "learned": head,
-        "without_network": nonet,
-        "target": "learned >= 3x plain at PSNR within 0.05 dB",
-        "meets_target": bool(head["speedup"] >= 3.0 and head["delta_db"] >= -0.05),
-        "settings": runs,
-        "training": {"blocks": data.n, "epochs": epochs, "batch": batch, "steps": steps, "seconds": round(train_s, 3),
-                     "data_seconds": round(data_s, 2), "loss_before": round(first, 5),
-                     "loss_per_epoch": [round(x, 5) for x in losses]},
-    }, net
-
-
 def _cube_face(face: int, S: int, seed: int) -> np.ndarray:
     """One 4096^2-style cube face: seeded value noise plus smooth gradients of
     the unit direction, which are continuous across face edges."""
@@ -654,7 +845,7 @@ def bench_process(args, cpu: bool) -> dict:
                 sys.path.insert(0, os.path.join(ROOT, "tests"))
                 import oracle_lib
                 orc = oracle_lib.Oracle()
-                threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+                threads = host_threads(args.cpu_threads)
                 t0 = time.perf_counter()
                 lin = orc.planar_to_linear(synth.to_planar(img).reshape(-1))
                 for k, (w, h) in enumerate(N.mip_sizes(S, S)):
@@ -702,7 +893,7 @@ def bench_c1(ctx, args, cpu: bool) -> dict:
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads = host_threads(args.cpu_threads)
         orc = oracle_lib.Oracle()
         t0 = time.perf_counter()
         x = planar
@@ -743,7 +934,7 @@ def bench_cube(ctx, args, net):
     every level compressed to BC1 by the plain cluster fit and by the learned
     proposal form; per-level PSNR (mean over faces) and Mtexels/s of both."""
     import ntx_amd as N
-    from ntx_amd import synth
+    from ntx_amd import learned, synth
 
     S = 4096
     levels = N.mip_sizes(S, S)
@@ -770,8 +961,7 @@ def bench_cube(ctx, args, net):
         for f in range(6):
             for k, (w, h) in enumerate(levels):
                 if learned_form:
-                    net.compress_bc1_device(lv[f][k], w, h, outs[f][k], tau_pred=1e9, greedy_rounds=1,
-                                            count_fallback=False)
+                    net.compress_bc1_device(lv[f][k], w, h, outs[f][k], count_fallback=False, **learned.SHIPPED)
                 else:
                     ctx.compress_device(N.FORMAT_BC1, lv[f][k], w, h, outs[f][k])
 
@@ -792,7 +982,7 @@ def bench_cube(ctx, args, net):
             b.free()
     return {"workload": "6 faces x 4096^2 RGBA8 (seeded noise + direction gradients continuous across faces), "
                         "13-level Kaiser(3; 4, 1) chains per face, BC1 per level: plain cluster fit and learned "
-                        "proposals (1 greedy round, no fallback)",
+                        "proposals at the shipped settings (cluster-fit fallback above tau_pred)",
             "texels": texels, "mip_chains_ms": round(ms_mips, 3), **res,
             "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3)}
 

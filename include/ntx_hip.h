@@ -161,6 +161,14 @@ NTX_API ntx_status ntx_hip_compress_device_noalloc(ntx_hip_context *ctx, int for
                                                    const float *color_weights, const float *d_rgba, int w, int h,
                                                    void *d_out);
 
+/* Caps the device memory the context allocates for its own scratch and
+ * staging (host-pointer staging, bucket lists, chain regions, metric
+ * partials); a call that would exceed it allocates nothing and returns
+ * NTX_ERR_OUT_OF_MEMORY (Error_CudaError in the nvtt front end, which then
+ * goes on with the next image, as CudaCompressorDXT.cpp:183-189 does).
+ * 0 = no cap (the default).  No counterpart in the reference. */
+NTX_API ntx_status ntx_hip_set_scratch_limit(ntx_hip_context *ctx, size_t bytes);
+
 /* BC1 at icbc Level8 (Normal / Highest) has two encoders with identical
  * output: one block per 16-lane group in one pass, and, for images of at
  * least `min_blocks` 4x4 blocks, two passes with one block per lane (colour

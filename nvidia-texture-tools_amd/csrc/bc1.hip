@@ -44,30 +44,21 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
     }
 }
 
-// Level8 pass 1, one block per lane: FloatColorCompressor's gather and
-// reduce_colors; a block of no colour (every weight 0) is the zero block, a
-// block of one colour compress_dxt1_single_color_optimal (icbc.h:3515-3525),
-// every other block goes to list n.  The clusters live in LDS only while
-// they are counted.  Blocks 0..nblocks-1 of the image, or, when sel is not
-// null, the *sel_count blocks listed in sel.
-__global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restrict__ src, int w, int h, int bw,
-                                                           int nblocks, const int *__restrict__ sel,
-                                                           const int *__restrict__ sel_count, int alpha_mode,
-                                                           const ntx::Tables *__restrict__ tab,
-                                                           uint2 *__restrict__ out, short2 *__restrict__ nm,
-                                                           int *__restrict__ counts, int *__restrict__ lists, int cap,
-                                                           int rows16, int urgent)
+// One item of bc1_classify_kernel per thread; the whole workgroup calls it.
+__device__ __forceinline__ void classify_chunk(const float *__restrict__ src, int w, int h, int bw,
+                                               const int *__restrict__ sel, int items, int alpha_mode,
+                                               const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
+                                               short2 *__restrict__ nm, int *__restrict__ counts,
+                                               int *__restrict__ lists, int cap, int rows16, float4 (*s_rows)[16][64],
+                                               int *s_cnt, int *s_base, int item)
 {
     using namespace ntx_bucket;
-    if (urgent) __builtin_amdgcn_s_setprio(3);
-    __shared__ float4 s_rows[4][16][64];
-    __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
     const int t = threadIdx.x;
     if (t < kCountSlots) s_cnt[t] = 0;
     __syncthreads();
-    const int item = blockIdx.x * blockDim.x + t;
-    const bool has = sel ? item < *sel_count : item < nblocks;
-    const int blk = has ? (sel ? sel[item] : item) : 0;
+    bool has = item < items;
+    int blk = has ? (sel ? sel[item] : item) : 0;
+    if (unsigned(blk) >= unsigned(cap)) has = false, blk = 0;  // listed indices are below cap
     int k = 0, local = -1;
     if (has) {
         float4 tx[16];
@@ -92,6 +83,35 @@ __global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restri
     append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
 }
 
+// Level8 pass 1, one block per lane: FloatColorCompressor's gather and
+// reduce_colors; a block of no colour (every weight 0) is the zero block, a
+// block of one colour compress_dxt1_single_color_optimal (icbc.h:3515-3525),
+// every other block goes to list n.  The clusters live in LDS only while
+// they are counted.  Blocks 0..nblocks-1 of the image, or, when sel is not
+// null, the *sel_count blocks listed in sel.
+__global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                           int nblocks, const int *__restrict__ sel,
+                                                           const int *__restrict__ sel_count, int alpha_mode,
+                                                           const ntx::Tables *__restrict__ tab,
+                                                           uint2 *__restrict__ out, short2 *__restrict__ nm,
+                                                           int *__restrict__ counts, int *__restrict__ lists, int cap,
+                                                           int rows16, int urgent)
+{
+    using namespace ntx_bucket;
+    if (urgent) __builtin_amdgcn_s_setprio(3);
+    __shared__ float4 s_rows[4][16][64];
+    __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
+    const int t = threadIdx.x;
+    const int items = sel ? min(*sel_count, cap) : nblocks;
+    // Workgroups stride over the items (the grid of a list is sized for its
+    // capacity, not its count, which only the device knows).
+    for (int base = blockIdx.x * blockDim.x; base < items; base += gridDim.x * blockDim.x) {
+        classify_chunk(src, w, h, bw, sel, items, alpha_mode, tab, out, nm, counts, lists, cap, rows16, s_rows, s_cnt,
+                       s_base, base + t);
+    }
+}
+
+
 // One wavefront of bc1_search_kernel: entries first .. first + 63 of list k.
 __device__ __forceinline__ void search_wave(const float *__restrict__ src, int w, int h, int bw, int alpha_mode,
                                             float cwx, float cwy, float cwz, const ntx::Tables *__restrict__ tab,
@@ -100,10 +120,15 @@ __device__ __forceinline__ void search_wave(const float *__restrict__ src, int w
                                             int lane, int k, int first, int cnt)
 {
     const int item = first + lane;
-    const bool live = item < cnt;
+    bool live = item < cnt;
     // Lanes past the end of the list encode the wave's first block again
-    // (same count, valid weights) and discard it.
-    const int blk = lists[size_t(k) * cap + (live ? item : first)];
+    // (same count, valid weights) and discard it.  Every caller's lists hold
+    // block indices below cap; anything else is written nowhere.
+    int blk = lists[size_t(k) * cap + (live ? item : first)];
+    if (unsigned(blk) >= unsigned(cap)) {
+        blk = 0;
+        live = false;
+    }
     float4 tx[16];
     lane_gather(src, w, h, blk % bw, blk / bw, alpha_mode, rows16 != 0, tx);
     bool unit = true;
@@ -169,12 +194,13 @@ __global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict_
     using namespace ntx_bucket;
     __shared__ float4 s_rows[17][64];
     const int lane = threadIdx.x;
+    // A list never holds more than cap entries.
     int total = 0;
-    for (int b = 2; b <= 16; b++) total += (counts[b] + 63) >> 6;
+    for (int b = 2; b <= 16; b++) total += (min(counts[b], cap) + 63) >> 6;
     for (int wave = blockIdx.x; wave < total; wave += gridDim.x) {
         int wv = wave, k = 0, first = 0, cnt = 0;
         for (int b = 16; b >= 2; b--) {
-            const int c = counts[b];
+            const int c = min(counts[b], cap);
             const int waves = (c + 63) >> 6;
             if (k == 0) {
                 if (wv < waves) { k = b; first = wv * 64; cnt = c; }
@@ -210,16 +236,19 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
 
 // Level8 pass 1 into the block lists in `counts` / `lists` (zero tallies on
 // entry; lists: 17 x cap ints, cap >= the number of blocks classified): the
-// image's blocks, or the *d_sel_count blocks listed in d_sel (at most cap).
+// image's blocks, or the *d_sel_count blocks listed in d_sel (at most cap;
+// then at most max_groups workgroups, striding over the list).
 hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
                                int16_t *d_nm, const int *d_sel, const int *d_sel_count, int *d_counts, int *d_lists,
-                               int cap, hipStream_t stream, bool urgent)
+                               int cap, int max_groups, hipStream_t stream, bool urgent)
 {
     const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
     const int items = d_sel ? cap : nblocks;
     if (items == 0) return hipSuccess;
     const int rows16 = (reinterpret_cast<uintptr_t>(d_src) & 15) == 0 && (w & 3) == 0;
-    hipLaunchKernelGGL(bc1_classify_kernel, dim3((items + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
+    int grid = (items + 255) / 256;
+    if (d_sel && grid > max_groups) grid = max_groups;
+    hipLaunchKernelGGL(bc1_classify_kernel, dim3(grid), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
                        d_sel, d_sel_count, alpha_mode, d_tab, reinterpret_cast<uint2 *>(d_out),
                        reinterpret_cast<short2 *>(d_nm), d_counts, d_lists, cap, rows16, urgent ? 1 : 0);
     return hipGetLastError();

@@ -413,23 +413,24 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
     __shared__ float4 s_p[17][64];
     const int lane = threadIdx.x;
 
-    // Work item -> (colour count k, position in list k), k = 16 first.
+    // Work item -> (colour count k, position in list k), k = 16 first.  A
+    // list never holds more than cap entries, nor indices at or above cap.
     int item = blockIdx.x * blockDim.x + lane;
     int k = 0, pos_in = 0;
     for (int b = 16; b >= 2; b--) {
-        const int c = counts[b];
+        const int c = min(counts[b], cap);
         if (k == 0) {
             if (item < c) { k = b; pos_in = item; }
             else item -= c;
         }
     }
-    const bool live = k != 0;
-    int blk = 0, count = 0;
+    int blk = k != 0 ? lists[size_t(k) * cap + pos_in] : 0;
+    const bool live = k != 0 && unsigned(blk) < unsigned(cap);
+    int count = 0;
     int op[16], pidx[16];
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 xsum = zero;
     if (live) {
-        blk = lists[size_t(k) * cap + pos_in];
         count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
     }
 
@@ -527,13 +528,14 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
     int item = (blockIdx.x * blockDim.x + lane) >> lpb_shift;
     int k = 0, pos_in = 0;
     for (int b = 16; b >= 2; b--) {
-        const int c = counts[b];
+        const int c = min(counts[b], cap);
         if (k == 0) {
             if (item < c) { k = b; pos_in = item; }
             else item -= c;
         }
     }
-    const bool live = k != 0;
+    const int listed = k != 0 ? lists[size_t(k) * cap + pos_in] : 0;
+    const bool live = k != 0 && unsigned(listed) < unsigned(cap);
     const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float besterror = FLT_MAX;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
     int op[16], pidx[16];
     float4 xsum = zero;
     if (live) {
-        blk = lists[size_t(k) * cap + pos_in];
+        blk = listed;
         const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op,
                                       pidx);
         const int first = tab->sq_off[count], total = tab->sq_off[count + 1] - first;

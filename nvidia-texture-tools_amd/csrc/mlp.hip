@@ -709,6 +709,73 @@ __device__ __forceinline__ float encode_pair(const float4 *t4, const float a[3],
     return err;
 }
 
+// The same evaluation on the block's texels held as 8-bit RGB
+// (R | G << 8 | B << 16, each channel round(255 * saturate(v))), `inside` bit
+// t set for a texel inside the image: nearest palette entry by
+// |p|^2 - 2 p.t (one v_dot4_u32_u8 per entry; the first entry on ties) and
+// the exact integer squared error in 8-bit units.  For 8-bit sources (every
+// value k / 255) this is the error above without rounding.
+__device__ __forceinline__ float eval_pair8(const uint32_t t8[16], uint32_t inside, uint32_t c0, uint32_t c1,
+                                            uint32_t *bits_out)
+{
+    int q[4][3];
+    ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
+    ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (c0 > c1) {
+            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
+            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
+        } else {
+            q[2][k] = (q[0][k] + q[1][k]) / 2;
+            q[3][k] = 0;
+        }
+    }
+    uint32_t pk[4];
+    int n2[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        pk[i] = uint32_t(q[i][0]) | uint32_t(q[i][1]) << 8 | uint32_t(q[i][2]) << 16;
+        n2[i] = q[i][0] * q[i][0] + q[i][1] * q[i][1] + q[i][2] * q[i][2];
+    }
+    int err = 0;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const uint32_t v = t8[t];
+        int best = n2[0] - 2 * int(__builtin_amdgcn_udot4(pk[0], v, 0u, false));
+        uint32_t bi = 0;
+#pragma unroll
+        for (int i = 1; i < 4; i++) {
+            const int d = n2[i] - 2 * int(__builtin_amdgcn_udot4(pk[i], v, 0u, false));
+            if (d < best) { best = d; bi = uint32_t(i); }
+        }
+        if ((inside >> t) & 1u) err += best + int(__builtin_amdgcn_udot4(v, v, 0u, false));
+        bits |= bi << (2 * t);
+    }
+    if (bits_out) *bits_out = bits;
+    return float(err);
+}
+
+__device__ __forceinline__ float encode_pair8(const uint32_t t8[16], uint32_t inside, const float a[3], const float b[3],
+                                              const float *mid5, const float *mid6, uint2 *blk)
+{
+    uint32_t c0 = ntx_bc1::to565(a[0], a[1], a[2], mid5, mid6);
+    uint32_t c1 = ntx_bc1::to565(b[0], b[1], b[2], mid5, mid6);
+    if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; }
+    uint32_t bits;
+    const float err = eval_pair8(t8, inside, c0, c1, &bits);
+    blk->x = c0 | (c1 << 16);
+    blk->y = bits;
+    return err;
+}
+
+__device__ __forceinline__ uint32_t to_u8(float v)
+{
+    const float s = fminf(fmaxf(v, 0.0f), 1.0f);
+    return uint32_t(rintf(s * 255.0f));
+}
+
 // One wave per 16 blocks at a time, 4 waves per workgroup.  Lane (g, n) of a
 // tile gathers row g of block n, the network runs on the matrix cores, and
 // then lane (g, n) refines candidate g of block n; the four candidates of a
@@ -719,7 +786,7 @@ __device__ __forceinline__ float encode_pair(const float4 *t4, const float a[3],
 // and moved by the network's offsets, keeping the better; box_candidates
 // drops the moved candidates (the same method without the network; no
 // prediction, so only tau_pred < 0 sends blocks to the fallback).
-__global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
+__global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
                                                       int w, int h, int bw, int n, const ntx::Tables *__restrict__ tab,
                                                       float tau_pred, float tau_gap, int refine_iters, int greedy_rounds,
                                                       int box_candidates,
@@ -728,11 +795,12 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
 {
     __shared__ ForwardNet net;
     // Each wave's 16 blocks: texels (r, g, b, weight 1 inside the image, 0
-    // outside; a padded row per block), network outputs and (mean, scale).
-    // Only the wave that owns them touches them after the network is loaded.
+    // outside; a padded row per block; only for least-squares passes), the
+    // texels as 8-bit RGB, and the network outputs.  Only the wave that owns
+    // them touches them after the network is loaded.
     __shared__ float4 s_tex[4][16][17];
+    __shared__ __align__(16) uint32_t s_t8[4][16][16];
     __shared__ float s_y[4][16][kOutPad + 1];
-    __shared__ float4 s_aux[4][16];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, col = lane & 15;
     load_net(net, params, t, 256);
     __syncthreads();
@@ -784,40 +852,72 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         for (int k = 0; k < 3; k++)
             bx3[k] = pack4(inr[0] ? (c[k][0] - m[k]) * s : 0.0f, inr[1] ? (c[k][1] - m[k]) * s : 0.0f,
                            inr[2] ? (c[k][2] - m[k]) * s : 0.0f, inr[3] ? (c[k][3] - m[k]) * s : 0.0f);
-        v4f h1[4], h2[4], yv[2];
-        forward(net, bx3, lane, h1, h2, yv);
+        // Block-level facts from the four lanes that hold its rows: the
+        // colour bounding box of the texels inside the image, whether they
+        // all equal texel 0 (row 0's lane, lane col), which texels are
+        // inside, and the texels as 8-bit RGB for the candidate errors.
+        float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
+        const float t0[3] = { __shfl(c[0][0], col), __shfl(c[1][0], col), __shfl(c[2][0], col) };
+        int same = 1;
+        uint32_t inside = 0;
+        uint32_t p8[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            s_tex[wave][col][4 * g + j] = inr[j] ? make_float4(c[0][j], c[1][j], c[2][j], 1.0f)
-                                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (inr[j]) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = fminf(lo[k], c[k][j]);
+                    hi[k] = fmaxf(hi[k], c[k][j]);
+                }
+                if ((g | j) != 0 && (c[0][j] != t0[0] || c[1][j] != t0[1] || c[2][j] != t0[2])) same = 0;
+                inside |= 1u << (4 * g + j);
+            }
+            p8[j] = to_u8(c[0][j]) | to_u8(c[1][j]) << 8 | to_u8(c[2][j]) << 16;
+        }
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+                hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+            }
+            same &= __shfl_xor(same, off);
+            inside |= uint32_t(__shfl_xor(int(inside), off));
+        }
+        const bool uniform = valid && same;
+
+        v4f h1[4], h2[4], yv[2];
+        forward(net, bx3, lane, h1, h2, yv);
+        *reinterpret_cast<uint4 *>(&s_t8[wave][col][4 * g]) = make_uint4(p8[0], p8[1], p8[2], p8[3]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (refine_iters > 0)
+                s_tex[wave][col][4 * g + j] = inr[j] ? make_float4(c[0][j], c[1][j], c[2][j], 1.0f)
+                                                     : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) s_y[wave][col][16 * mm + 4 * g + j] = yv[mm][j];
         }
-        if (g == 0) s_aux[wave][col] = make_float4(m[0], m[1], m[2], s);
         ntx_wave_sync();
-        // Lane (g, col): candidate g of block col, texels read from LDS.
+        // Lane (g, col): candidate g of block col, on the block's 16 texels
+        // in registers.
+        uint32_t t8[16];
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(&s_t8[wave][col][4 * q4]);
+            t8[4 * q4] = v.x; t8[4 * q4 + 1] = v.y; t8[4 * q4 + 2] = v.z; t8[4 * q4 + 3] = v.w;
+        }
         const float4 *tb = s_tex[wave][col];
-        const float4 ax = s_aux[wave][col];
-        const float is = 1.0f / ax.w;
+        const float is = 1.0f / s;
         // Candidate g: diagonal g of the block's colour bounding box, plus
         // the network's offsets (in the block frame, hence divided by the
         // scale) unless box_candidates.
-        float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
-        for (int i = 0; i < 16; i++) {
-            const float4 v = tb[i];
-            if (v.w != 0.0f) {
-                lo[0] = fminf(lo[0], v.x); hi[0] = fmaxf(hi[0], v.x);
-                lo[1] = fminf(lo[1], v.y); hi[1] = fmaxf(hi[1], v.y);
-                lo[2] = fminf(lo[2], v.z); hi[2] = fmaxf(hi[2], v.z);
-            }
-        }
         float a[3], bb[3];
         box_diagonal(g, lo, hi, a, bb);
         float pred = fmaxf(s_y[wave][col][24], 0.0f) * is * 255.0f;  // predicted RMSE, 8-bit units
         // The diagonal as it is, refined and encoded ...
-        refine(tb, a, bb, refine_iters);
+        if (refine_iters > 0) refine(tb, a, bb, refine_iters);
         uint2 blk;
-        float err = valid ? encode_pair(tb, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
+        float err = valid ? encode_pair8(t8, inside, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
         if (box_candidates) {
             pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
         } else {
@@ -830,9 +930,9 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
                 na[k] = fminf(fmaxf(na[k] + s_y[wave][col][6 * g + k] * is, 0.0f), 1.0f);
                 nb[k] = fminf(fmaxf(nb[k] + s_y[wave][col][6 * g + 3 + k] * is, 0.0f), 1.0f);
             }
-            refine(tb, na, nb, refine_iters);
+            if (refine_iters > 0) refine(tb, na, nb, refine_iters);
             uint2 nblk;
-            const float nerr = valid ? encode_pair(tb, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
+            const float nerr = valid ? encode_pair8(t8, inside, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
             if (nerr < err) { err = nerr; blk = nblk; }
         }
         // Best of the four candidates (lower candidate on equal error).
@@ -845,27 +945,15 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         }
         // A block whose texels inside the image are all equal gets what the
         // cluster fit gives it: reduce_colors' running weighted mean of the
-        // equal colours (icbc.h:1638-1683, weights 1) and the single-colour
-        // tables.
-        // A block whose texels inside the image are all equal gets what the
-        // cluster fit gives it: reduce_colors' running weighted mean of the
-        // equal colours (icbc.h:1638-1683, weights 1) and the single-colour
-        // tables.  Texel 0 is always inside the image.
-        const float4 t0 = tb[0];
-        bool uniform = valid;
-        for (int i = 1; i < 16; i++) {
-            const float4 v = tb[i];
-            if (v.w != 0.0f && (v.x != t0.x || v.y != t0.y || v.z != t0.z)) uniform = false;
-        }
+        // equal colours (icbc.h:1638-1683, weights 1, texel order) and the
+        // single-colour tables.  Texel 0 is always inside the image.
         if (uniform) {
-            float kr = t0.x, kg = t0.y, kb = t0.z, kw = 1.0f;
-            for (int i = 1; i < 16; i++) {
-                const float4 v = tb[i];
-                if (v.w == 0.0f) continue;
+            float kr = t0[0], kg = t0[1], kb = t0[2], kw = 1.0f;
+            for (int i = 1; i < cnt; i++) {
                 const float s1 = kw + 1.0f;
-                kr = (kr * kw + v.x) / s1;
-                kg = (kg * kw + v.y) / s1;
-                kb = (kb * kw + v.z) / s1;
+                kr = (kr * kw + t0[0]) / s1;
+                kg = (kg * kw + t0[1]) / s1;
+                kb = (kb * kw + t0[2]) / s1;
                 kw = s1;
             }
             blk = ntx_bc1::single_color_block(tab, kr, kg, kb);
@@ -885,7 +973,7 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
         for (int round = 0; round < rounds; round++) {
             float be = err;
             int bm = 64;
-#pragma unroll
+#pragma unroll 1
             for (int i = 0; i < 3; i++) {
                 const int mv = 3 * g + i, slot = mv >> 1, step = (mv & 1) ? 1 : -1;
                 int t2[6];
@@ -899,7 +987,7 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
                 uint32_t c0 = uint32_t(t2[0]) << 11 | uint32_t(t2[1]) << 5 | uint32_t(t2[2]);
                 uint32_t c1 = uint32_t(t2[3]) << 11 | uint32_t(t2[4]) << 5 | uint32_t(t2[5]);
                 if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
-                const float e = valid ? eval_pair(tb, c0, c1, nullptr) : FLT_MAX;
+                const float e = valid ? eval_pair8(t8, inside, c0, c1, nullptr) : FLT_MAX;
                 if (e < be) { be = e; bm = mv; }
             }
 #pragma unroll
@@ -919,14 +1007,12 @@ __global__ __launch_bounds__(256) void bc1_mlp_kernel(const float *__restrict__ 
             uint32_t c1 = uint32_t(q[3]) << 11 | uint32_t(q[4]) << 5 | uint32_t(q[5]);
             if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
             uint32_t bits = 0;
-            if (valid) err = eval_pair(tb, c0, c1, &bits);
+            if (valid) err = eval_pair8(t8, inside, c0, c1, &bits);
             blk.x = c0 | (c1 << 16);
             blk.y = bits;
         }
         if (g == 0 && valid) {
-            int cntb = 0;
-            for (int i = 0; i < 16; i++) cntb += tb[i].w != 0.0f ? 1 : 0;
-            const float rmse = sqrtf(err / float(3 * cntb));
+            const float rmse = sqrtf(err / float(3 * cnt));
             const bool fallback = !uniform && (pred > tau_pred || (tau_gap >= 0.0f && rmse > pred + tau_gap));
             out[b] = blk;
             if (err_out) err_out[b] = rmse;
@@ -991,12 +1077,18 @@ hipError_t launch_mlp_adam(float *d_params, float *d_m1, float *d_m2, const floa
 
 hipError_t launch_bc1_mlp(const float *d_params, const float *d_src, int w, int h, const Tables *d_tab, float tau_pred,
                           float tau_gap, int refine_iters, int greedy_rounds, int box_candidates, void *d_out,
-                          int *d_fb_count, int *d_fb_list, float *d_err, int workgroups, hipStream_t stream)
+                          int *d_fb_count, int *d_fb_list, float *d_err, int cus, hipStream_t stream)
 {
     const int bw = (w + 3) / 4, n = bw * ((h + 3) / 4);
     if (n == 0) return hipSuccess;
     const int tiles = (n + 15) / 16;
-    const int grid = std::max(1, std::min(workgroups, (tiles + 3) / 4));
+    // The persistent grid: as many workgroups as stay resident on each
+    // compute unit (the kernel's LDS and registers decide), times the CUs.
+    static int per_cu = 0;
+    if (per_cu <= 0 &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bc1_mlp_kernel, 256, 0) != hipSuccess || per_cu <= 0))
+        per_cu = 2;
+    const int grid = std::max(1, std::min(per_cu * cus, (tiles + 3) / 4));
     hipLaunchKernelGGL(bc1_mlp_kernel, dim3(grid), dim3(256), 0, stream, d_params, d_src, w, h, bw, n, d_tab, tau_pred,
                        tau_gap, refine_iters, greedy_rounds, box_candidates, static_cast<uint2 *>(d_out), d_fb_count,
                        d_fb_list, d_err);

@@ -21,7 +21,7 @@ hipError_t launch_mlp_adam(float *d_params, float *d_m1, float *d_m2, const floa
                            int step, float *d_grad, hipStream_t stream);
 hipError_t launch_bc1_mlp(const float *d_params, const float *d_src, int w, int h, const Tables *d_tab, float tau_pred,
                           float tau_gap, int refine_iters, int greedy_rounds, int box_candidates, void *d_out,
-                          int *d_fb_count, int *d_fb_list, float *d_err, int workgroups, hipStream_t stream);
+                          int *d_fb_count, int *d_fb_list, float *d_err, int cus, hipStream_t stream);
 }  // namespace ntx
 
 static_assert(ntx_mlp::kParams == NTX_MLP_PARAMS, "parameter count");
@@ -115,9 +115,9 @@ NTX_API ntx_status ntx_hip_mlp_destroy(ntx_hip_mlp *m)
     if (!m) return NTX_OK;
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);
-    for (void *p : { static_cast<void *>(m->d_params), static_cast<void *>(m->d_m1), static_cast<void *>(m->d_m2),
-                     m->partials.ptr, m->loss.ptr, m->grad.ptr, m->fb.ptr })
+    for (void *p : { static_cast<void *>(m->d_params), static_cast<void *>(m->d_m1), static_cast<void *>(m->d_m2) })
         if (p) (void)hipFree(p);
+    for (ntx::DevBuf *b : { &m->partials, &m->loss, &m->grad, &m->fb }) ntx::release(m->ctx, *b);
     delete m;
     return NTX_OK;
 }
@@ -229,7 +229,7 @@ NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *m, const float *
     const int cus = resident_cus(ctx);
     hipError_t e = ntx::timed(ctx, NTX_KERNEL_BC1_MLP, [&] {
         return ntx::launch_bc1_mlp(m->d_params, d_rgba, w, h, ctx->d_tab, tau_pred, tau_gap, refine_iters, greedy_rounds,
-                                   (flags & NTX_MLP_BOX_CANDIDATES) ? 1 : 0, d_out, d_count, d_list, nullptr, 2 * cus,
+                                   (flags & NTX_MLP_BOX_CANDIDATES) ? 1 : 0, d_out, d_count, d_list, nullptr, cus,
                                    ctx->stream);
     });
     // The fallback blocks through the two-pass cluster fit (colour counts of
@@ -247,7 +247,8 @@ NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *m, const float *
     if (e == hipSuccess)
         e = ntx::timed(ctx, NTX_KERNEL_BC1_CLASSIFY, [&] {
             return ntx::launch_bc1_classify(d_rgba, w, h, NTX_ALPHA_NONE, ctx->d_tab, d_out, nullptr, d_list, d_count,
-                                            counts, counts + ntx::bc3_lists_offset(), n, ctx->stream, false);
+                                            counts, counts + ntx::bc3_lists_offset(), n, 4 * ctx->cus, ctx->stream,
+                                            false);
         });
     if (e == hipSuccess)
         e = ntx::timed(ctx, NTX_KERNEL_BC1_SEARCH, [&] {

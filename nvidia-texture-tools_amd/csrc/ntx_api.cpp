@@ -71,12 +71,28 @@ hipEvent_t take_event(ntx_hip_context *ctx)
 ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes)
 {
     if (b.cap >= bytes) return NTX_OK;
+    // The context's scratch stays within its limit (ntx_hip_set_scratch_limit).
+    if (ctx->scratch_limit && ctx->scratch_bytes - b.cap + bytes > ctx->scratch_limit) {
+        ctx->err = "scratch of " + std::to_string(ctx->scratch_bytes - b.cap + bytes) + " bytes exceeds the limit of " +
+                   std::to_string(ctx->scratch_limit);
+        return NTX_ERR_OUT_OF_MEMORY;
+    }
     if (b.ptr) NTX_HIP(ctx, hipFree(b.ptr));
+    ctx->scratch_bytes -= b.cap;
     b.ptr = nullptr;
     b.cap = 0;
     NTX_HIP(ctx, hipMalloc(&b.ptr, bytes));
     b.cap = bytes;
+    ctx->scratch_bytes += bytes;
     return NTX_OK;
+}
+
+void release(ntx_hip_context *ctx, DevBuf &b)
+{
+    if (b.ptr) (void)hipFree(b.ptr);
+    ctx->scratch_bytes -= b.cap;
+    b.ptr = nullptr;
+    b.cap = 0;
 }
 
 ntx_status bind(ntx_hip_context *ctx)
@@ -427,7 +443,7 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
         if (e == hipSuccess)
             e = timed_on(ctx, stream, NTX_KERNEL_BC1_CLASSIFY, [&] {
                 return launch_bc1_classify(d_rgba, w, h, alpha_mode, ctx->d_tab, d_out, d_nm, nullptr, nullptr, counts,
-                                           lists, nblocks, stream, urgent);
+                                           lists, nblocks, 0, stream, urgent);
             });
         if (e == hipSuccess)
             e = timed_on(ctx, stream, NTX_KERNEL_BC1_SEARCH, [&] {
@@ -577,6 +593,13 @@ NTX_API ntx_status ntx_hip_compress_device(ntx_hip_context *ctx, int format, int
     return NTX_OK;
 }
 
+NTX_API ntx_status ntx_hip_set_scratch_limit(ntx_hip_context *ctx, size_t bytes)
+{
+    if (!ctx) return NTX_ERR_INVALID_ARGUMENT;
+    ctx->scratch_limit = bytes;
+    return NTX_OK;
+}
+
 NTX_API ntx_status ntx_hip_set_bc1_lanes_min(ntx_hip_context *ctx, int min_blocks)
 {
     if (!ctx || min_blocks < 0) return NTX_ERR_INVALID_ARGUMENT;
@@ -605,19 +628,21 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
 {
     ntx_status s = NTX_OK;
     std::vector<size_t> region(n, 0);
-    std::vector<int> layout;
+    // The offsets of the regions that hold tallies: they depend on the
+    // format and quality as well as the sizes (scratch_ints).
+    std::vector<size_t> layout;
     size_t total = 0;
     for (int k = 0; k < n; k++) {
         region[k] = total;
-        layout.push_back(w[k]);
-        layout.push_back(h[k]);
-        total += (scratch_ints(ctx, format, quality, w[k], h[k]) + 63) & ~size_t(63);
+        const size_t ints = scratch_ints(ctx, format, quality, w[k], h[k]);
+        if (ints) layout.push_back(total);
+        total += (ints + 63) & ~size_t(63);
     }
     if (total) {
         const size_t cap_before = ctx->chain.cap;
         if ((s = ensure(ctx, ctx->chain, sizeof(int) * total))) return s;
-        // The tallies are known to be zero only in the same memory with the
-        // same region layout as the last complete run.
+        // The tallies are known to be zero only in the same memory at the
+        // same region offsets as the last complete run.
         if (ctx->chain.cap != cap_before || ctx->chain_layout != layout) ctx->chain_counts_zero = false;
         ctx->chain_layout = layout;
     }

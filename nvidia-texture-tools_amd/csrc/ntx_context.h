@@ -17,7 +17,7 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
 // bucket scratch of bc1_scratch_ints(blocks) ints with zero tallies.
 hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
                                int16_t *d_nm, const int *d_sel, const int *d_sel_count, int *d_counts, int *d_lists,
-                               int cap, hipStream_t stream, bool urgent);
+                               int cap, int max_groups, hipStream_t stream, bool urgent);
 hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
                              void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, int max_groups,
                              hipStream_t stream);
@@ -104,6 +104,9 @@ struct ntx_hip_context {
     uint32_t gamma_lut_key = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ntx::DevBuf in, out, tmp, dst, bucket, metric;
+    // Bytes of the DevBuf scratch above (and `chain`) now allocated, and the
+    // cap set by ntx_hip_set_scratch_limit (0: none).
+    size_t scratch_bytes = 0, scratch_limit = 0;
     // BC1 images of at least this many blocks take the two-pass encoder.
     static constexpr int kBc1LanesMinDefault = 1 << 17;
     int bc1_lanes_min = kBc1LanesMinDefault;
@@ -118,7 +121,7 @@ struct ntx_hip_context {
     std::vector<hipEvent_t> sync_events;
     ntx::DevBuf chain;
     bool chain_counts_zero = false;
-    std::vector<int> chain_layout;  // the image sizes the regions were laid out for
+    std::vector<size_t> chain_layout;  // offsets (ints) of the regions with tallies in the last complete run
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;
@@ -134,6 +137,8 @@ namespace ntx {
 ntx_status fail(ntx_hip_context *ctx, hipError_t e, const char *what);
 ntx_status bind(ntx_hip_context *ctx);
 ntx_status ensure(ntx_hip_context *ctx, DevBuf &b, size_t bytes);
+// Frees a buffer ensure() allocated (and takes it off the scratch count).
+void release(ntx_hip_context *ctx, DevBuf &b);
 hipEvent_t take_event(ntx_hip_context *ctx);
 // Quality levels handled exactly by the kernels.
 bool supported(int format, int quality);

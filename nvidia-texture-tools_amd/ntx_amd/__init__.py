@@ -45,7 +45,7 @@ EXPORTS = (
     "ntx_hip_timer_begin", "ntx_hip_timer_end", "ntx_hip_kernel_name", "ntx_hip_profile_enable",
     "ntx_hip_profile_read", "ntx_hip_compressed_size",
     "ntx_hip_compress", "ntx_hip_compress_device", "ntx_hip_compress_bc1_stats_device", "ntx_hip_set_bc1_lanes_min",
-    "ntx_hip_compress_device_noalloc",
+    "ntx_hip_compress_device_noalloc", "ntx_hip_set_scratch_limit",
     "ntx_hip_compress_chain_device", "ntx_hip_compress_images_device",
     "ntx_hip_downsample", "ntx_hip_downsample_device", "ntx_hip_downsample_chain_device",
     "ntx_hip_srgb_to_linear_device", "ntx_hip_unpack_bgra8_device", "ntx_hip_unpack_bgra8_srgb_device",
@@ -128,6 +128,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_compress_bc1_stats_device": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
         "ntx_hip_set_bc1_lanes_min": (_I, [_P, _I]),
         "ntx_hip_compress_device_noalloc": (_I, [_P, _I, _I, _I, _P, _P, _I, _I, _P]),
+        "ntx_hip_set_scratch_limit": (_I, [_P, _S]),
         "ntx_hip_compress_chain_device": (_I, [_P, _I, _I, _I, _P, _I, _F, _P, _I, _I, _I, _I, _P, _P]),
         "ntx_hip_compress_images_device": (_I, [_P, _I, _I, _I, _P, _I, _P, _P, _P, _P]),
         "ntx_hip_downsample": (_I, [_P, _I, _F, _P, _I, _I, _P, _I, _I, _P]),

@@ -5,6 +5,7 @@
 // step on the device through include/ntx_hip.h.
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <cfloat>
@@ -37,6 +38,13 @@ ntx_hip_context *backend()
     if (!tried) {
         tried = true;
         if (ntx_hip_is_hardware_present() && ntx_hip_create(0, &ctx) != NTX_OK) ctx = nullptr;
+        // NTX_SCRATCH_LIMIT_BYTES: cap on the backend's device scratch
+        // (ntx_hip_set_scratch_limit); images that need more report
+        // Error_CudaError.
+        if (ctx) {
+            if (const char *lim = std::getenv("NTX_SCRATCH_LIMIT_BYTES"))
+                ntx_hip_set_scratch_limit(ctx, size_t(std::strtoull(lim, nullptr, 10)));
+        }
     }
     return ctx;
 }

@@ -1308,6 +1308,12 @@ void ntxo_compress_bc1_counts(int alpha_mode, const float *cw4, int w, int h, co
 void ntxo_fast_downsample(int W, int H, const float *src_all, float *dst_all)
 {
     const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    if (W == 1 && H == 1) {
+        /* No next level exists (canMakeNextMipmap, Surface.cpp:155-167); the
+         * 3-tap 1-D rule below would read past the single texel. */
+        for (int c = 0; c < 4; c++) dst_all[c] = src_all[c];
+        return;
+    }
     for (int c = 0; c < 4; c++) {
         const float *src = src_all + (size_t)c * W * H;
         float *dst = dst_all + (size_t)c * w * h;

@@ -12,6 +12,10 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
+#include <thread>
+#include <vector>
+
 #include "nvtt/BlockCompressor.h"
 #include "nvtt/CompressorDX9.h"
 #include "nvtt/CompressorDX10.h"
@@ -249,6 +253,50 @@ size_t ntxref_compress_image(int kind, int quality, int alpha_mode, const float 
 
 // Worker threads nv::ParallelFor uses (nvthread's processor count).
 int ntxref_thread_count(void) { return int(nv::processorCount()); }
+
+// ntxref_compress_image with the reference's task loop on exactly `threads`
+// host threads: nv::ParallelFor's scheme (ParallelFor.cpp:14-60: every
+// worker takes the next task index from one atomic counter, one block per
+// task) on a pool of the given size, instead of nv::ThreadPool's
+// processorCount() workers (sysconf, which ignores the CPU share a process
+// is given).  The compressor and its per-block code are the reference's.
+size_t ntxref_compress_image_threads(int kind, int quality, int alpha_mode, const float *cw, const float *planar,
+                                     int w, int h, uint8_t *out, size_t out_bytes, int threads)
+{
+    struct Dispatcher : public nvtt::TaskDispatcher {
+        int threads = 1;
+        void dispatch(nvtt::Task *task, void *context, int count) override
+        {
+            std::atomic<int> next(0);
+            auto work = [&]() {
+                for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1)) task(context, i);
+            };
+            std::vector<std::thread> pool;
+            for (int t = 1; t < threads; t++) pool.emplace_back(work);
+            work();
+            for (auto &t : pool) t.join();
+        }
+    } dispatcher;
+    dispatcher.threads = threads < 1 ? 1 : threads;
+    nvtt::CompressionOptions::Private o;
+    options(o, kind_format(kind), quality, cw);
+    Sink sink;
+    sink.dst = out; sink.cap = out_bytes;
+    nvtt::OutputOptions::Private oo;
+    oo.fileHandle = nullptr;
+    oo.outputHandler = &sink;
+    oo.errorHandler = nullptr;
+    oo.outputHeader = false;
+    oo.container = nvtt::Container_DDS;
+    oo.version = 0;
+    oo.srgb = false;
+    oo.deleteOutputHandler = false;
+    oo.wrapperProxy = nullptr;
+    CompressorInterface *c = make_compressor(kind, quality);
+    c->compress(nvtt::AlphaMode(alpha_mode), uint(w), uint(h), 1, planar, &dispatcher, o, oo);
+    delete c;
+    return sink.used;
+}
 
 // FloatImage::fastDownSample (FloatImage.cpp:559-737) on planar RGBA.
 void ntxref_fast_downsample(int W, int H, const float *src, float *dst)

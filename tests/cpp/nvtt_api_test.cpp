@@ -18,6 +18,9 @@
 //       time of one process() call in ms and the bytes it delivered
 //   nvtt_api_test api      checks error reporting, handler order and Surface semantics
 //   nvtt_api_test batch    Compressor::compress(BatchList) against per-image compress
+//   nvtt_api_test device-error   run with NTX_SCRATCH_LIMIT_BYTES=1048576: an
+//       image whose device staging exceeds the limit reports Error_CudaError
+//       (begin, error, end, no data) and the next, smaller image compresses
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -443,8 +446,43 @@ int run_batch()
 
 }  // namespace
 
+// CudaCompressorDXT.cpp:183-189: a device error is reported through the
+// ErrorHandler and processing continues.  The backend's scratch limit makes
+// the 512 x 512 image's upload staging (4 MiB) fail to allocate.
+int run_device_error()
+{
+    nvtt::Compressor comp;
+    nvtt::CompressionOptions co;
+    co.setFormat(nvtt::Format_BC3);
+    Capture cap;
+    Errors err;
+    nvtt::OutputOptions oo;
+    oo.setOutputHandler(&cap);
+    oo.setErrorHandler(&err);
+    std::vector<float> big(size_t(4) * 512 * 512, 0.5f), small(size_t(4) * 64 * 64);
+    for (size_t i = 0; i < small.size(); i++) small[i] = float((i * 37) % 251) / 250.0f;
+    for (int round = 0; round < 2; round++) {
+        cap.events.clear();
+        cap.bytes.clear();
+        err.seen.clear();
+        CHECK(comp.compress(512, 512, 1, 0, 0, big.data(), co, oo));
+        CHECK(err.seen.size() == 1 && err.seen[0] == nvtt::Error_CudaError);
+        CHECK(cap.bytes.empty());
+        CHECK(cap.events.size() == 14 && cap.events[0] == 1 && cap.events[1] == 512 * 512 && cap.events[7] == 3);
+        cap.events.clear();
+        err.seen.clear();
+        CHECK(comp.compress(64, 64, 1, 0, 0, small.data(), co, oo));
+        CHECK(err.seen.empty());
+        CHECK(cap.bytes.size() == 16 * 16 * 16);
+        CHECK(cap.events.size() == 21 && cap.events[7] == 2 && cap.events[14] == 3);
+    }
+    std::printf("device-error ok\n");
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc >= 2 && std::strcmp(argv[1], "device-error") == 0) return run_device_error();
     if (argc >= 2 && std::strcmp(argv[1], "chain") == 0) return run_chain(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "process") == 0) return run_process(argc, argv);
     if (argc >= 2 && std::strcmp(argv[1], "process-time") == 0) return run_process_time(argc, argv);

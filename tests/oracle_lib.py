@@ -263,15 +263,23 @@ class Reference:
         out = np.zeros(8 if fmt in (1, 2, 6) else 16, np.uint8)
         self.lib.ntxref_block(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(b), _p(out)); return out
 
-    def compress_image(self, fmt, quality, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1)):
-        """A whole image through the reference's own block loop and ParallelTaskDispatcher."""
+    def compress_image(self, fmt, quality, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1), threads=None):
+        """A whole image through the reference's own block loop: its
+        ParallelTaskDispatcher (nvthread's processor count), or, with
+        `threads`, the same task loop on that many host threads."""
         src = f32(planar).reshape(-1); cwv = f32(cw)
         nb = ((w + 3) // 4) * ((h + 3) // 4)
         out = np.zeros(nb * (8 if fmt in (1, 2, 6) else 16), np.uint8)
-        fn = self.lib.ntxref_compress_image
-        fn.restype = ctypes.c_size_t
-        n = fn(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(src), int(w), int(h), _p(out),
-               ctypes.c_size_t(out.size))
+        if threads is None:
+            fn = self.lib.ntxref_compress_image
+            fn.restype = ctypes.c_size_t
+            n = fn(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(src), int(w), int(h), _p(out),
+                   ctypes.c_size_t(out.size))
+        else:
+            fn = self.lib.ntxref_compress_image_threads
+            fn.restype = ctypes.c_size_t
+            n = fn(self.KIND[fmt], int(quality), int(alpha_mode), _p(cwv), _p(src), int(w), int(h), _p(out),
+                   ctypes.c_size_t(out.size), int(threads))
         assert n == out.size, (n, out.size)
         return out
 

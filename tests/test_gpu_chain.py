@@ -137,3 +137,33 @@ def test_images_match_per_image_calls(gpu, fmt):
             assert np.array_equal(d.download(np.uint8), ref[k]), (rep, sizes[k])
     for d in d_in + d_out:
         d.free()
+
+
+def test_scheduler_scratch_reused_across_formats():
+    """One context compresses the same set of level sizes as BC3 (regions
+    laid out for BC3's scratch), then as BC1 with the two-pass encoder
+    (regions of other sizes at other offsets), then as BC3 again: each run
+    must find its own tallies zero, and every level must equal the per-level
+    calls."""
+    ctx = N.Context(0)
+    ctx.set_bc1_lanes_min(1)
+    try:
+        w = h = 128
+        sizes = N.mip_sizes(w, h)
+        planar = _image(N.FORMAT_BC3, w, h, seed=77)
+        d_lv = [ctx.alloc(16 * a * b) for a, b in sizes]
+        d_lv[0].upload(planar)
+        ctx.downsample_chain_device(N.FILTER_BOX, d_lv, w, h)
+        lv = [d.download(np.float32) for d in d_lv]
+        for fmt in (N.FORMAT_BC3, N.FORMAT_BC1, N.FORMAT_BC3, N.FORMAT_BC1):
+            d_out = [ctx.alloc(N.compressed_size(fmt, a, b)) for a, b in sizes]
+            ctx.compress_images_device(fmt, d_lv, sizes, d_out)
+            for k, (a, b) in enumerate(sizes):
+                exp = ctx.compress(fmt, lv[k], a, b)
+                assert np.array_equal(d_out[k].download(np.uint8), exp), (fmt, a, b)
+            for d in d_out:
+                d.free()
+        for d in d_lv:
+            d.free()
+    finally:
+        ctx.close()

@@ -200,3 +200,54 @@ def test_device_psnr_matches_host(gpu):
     blocks = gpu.compress(N.FORMAT_BC1, planar, w, h)
     db = gpu.alloc(blocks.nbytes).upload(blocks)
     assert gpu.bc1_psnr_device(d, w, h, db) == pytest.approx(learned.bc1_psnr(blocks, planar, w, h), abs=1e-9)
+
+
+def test_shipped_settings_contract(gpu):
+    """The learned form at its shipped settings (learned.SHIPPED: fallback
+    above tau_pred, no least-squares pass, no greedy round) on a held-out
+    texture, after training on other seeds: PSNR within 0.05 dB of the
+    cluster fit; the trained network at least MARGIN_DB above the box
+    diagonals it moves (same candidates, no network); and the fallback is the
+    network's own prediction: every block whose predicted cluster-fit RMSE
+    is above tau_pred gets exactly the cluster fit's bytes, and the number of
+    fallback blocks is the number of such predictions."""
+    MARGIN_DB = 0.5
+    data = learned.make_dataset(gpu, [(_texture(512, 512, s), 512, 512) for s in (51, 52, 53, 54)])
+    w = h = 512
+    planar = _texture(w, h, 60)
+    n = (w // 4) * (h // 4)
+    d = gpu.alloc(planar.nbytes).upload(planar)
+    out = gpu.alloc(N.compressed_size(N.FORMAT_BC1, w, h))
+    plain = gpu.compress(N.FORMAT_BC1, planar, w, h)
+    kw = dict(learned.SHIPPED)
+    with N.Mlp(gpu, seed=7) as net:
+        learned.train(net, data, epochs=10, batch=4096, lr=3e-3)
+        fb = net.compress_bc1_device(d, w, h, out, **kw)
+        shipped = out.download(np.uint8)
+        net.compress_bc1_device(d, w, h, out, tau_pred=1e9, refine_iters=0, greedy_rounds=0)
+        trained = out.download(np.uint8)
+        net.compress_bc1_device(d, w, h, out, tau_pred=1e9, refine_iters=0, greedy_rounds=0, box_candidates=True)
+        box = out.download(np.uint8)
+        # The network's predicted cluster-fit RMSE of every block (8-bit units).
+        d_f, d_a = gpu.alloc(learned.FEATURE_BYTES * n), gpu.alloc(16 * n)
+        N.mlp_features_device(gpu, d, w, h, d_f, d_a)
+        y = net.forward(d_f.ptr, n)
+        aux = d_a.download(np.float32).reshape(n, 4)
+        feats = d_f.download(np.uint16).reshape(n, 48)
+    pred = np.maximum(y[:, 24], 0.0) / aux[:, 3] * 255.0
+    uniform = ~np.any(feats != 0, axis=1)
+    p_plain = learned.bc1_psnr(plain, planar, w, h)
+    p_shipped = learned.bc1_psnr(shipped, planar, w, h)
+    p_trained = learned.bc1_psnr(trained, planar, w, h)
+    p_box = learned.bc1_psnr(box, planar, w, h)
+    tau = kw["tau_pred"]
+    above = (pred > tau * 1.001) & ~uniform
+    near = (np.abs(pred - tau) <= tau * 0.001) & ~uniform
+    print(f"psnr: plain {p_plain:.3f}, shipped {p_shipped:.3f} (fallback {fb} blocks, {int(above.sum())} predicted "
+          f"above tau), trained {p_trained:.3f}, box diagonals {p_box:.3f}")
+    assert p_shipped >= p_plain - 0.05
+    assert p_trained >= p_box + MARGIN_DB
+    assert above.sum() > 0
+    sb, pb = shipped.reshape(-1, 8), plain.reshape(-1, 8)
+    assert np.array_equal(sb[above], pb[above])
+    assert int(above.sum()) <= fb <= int(above.sum() + near.sum())

@@ -103,3 +103,13 @@ def test_front_end_batch_list():
     batch launch; BC3 through the chain scheduler)."""
     r = subprocess.run([DRIVER, "batch"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "batch ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_device_error_reported_and_next_image_compresses(oracle):
+    """A device failure (here an allocation over the backend's scratch limit)
+    reaches the ErrorHandler as Error_CudaError with no data written, and the
+    same Compressor then compresses the next image, twice over
+    (CudaCompressorDXT.cpp:183-189)."""
+    env = dict(os.environ, NTX_SCRATCH_LIMIT_BYTES=str(1 << 20))
+    r = subprocess.run([DRIVER, "device-error"], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0 and "device-error ok" in r.stdout, r.stdout + r.stderr
