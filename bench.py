@@ -62,6 +62,19 @@ MLP_VALU_FLOPS = 250 + 4 * (96 + 20 + 2 * 650) + 48
 # 256 MiB device copy in the same run).
 SEARCH_TRAFFIC_BYTES = {"bc3_search_kernel": round((71.69e6 + 18.56e6 + 16.26e6 + 3.75e6) / 2)}
 
+# The 16384^2 headline kernels, from the rocprofv3 passes of
+# `bench.py --profile-pass --no-cpu --steps 3 --warmup 1` summarised in
+# profiles/bc1_16384_learned_and_cluster_fit.md: HBM bytes per launch
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from three 256 MiB copies
+# in the same run) and VALU instructions issued per launch as a fraction of
+# the chip's issue rate (256 CUs x 4 SIMDs x one wave64 instruction per 4
+# cycles x 2.4 GHz), from SQ_INSTS_VALU.
+HEADLINE_TRAFFIC_BYTES = {"bc1_mlp_kernel": round(3221.7e6 + 136.2e6), "bc1_search_kernel": round(9327.3e6 + 582.4e6)}
+HEADLINE_VALU_ISSUE_FRAC = {"bc1_mlp_kernel": 0.810, "bc1_search_kernel": 0.988}
+# Algorithmic bytes per 4x4 block on the headline: the R, G and B planes of
+# 16 texels as FP32 (AlphaMode_None reads no alpha) in, one 8-byte block out.
+HEADLINE_BYTES_PER_BLOCK = 16 * 3 * 4 + 8
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -288,13 +301,24 @@ def bench_c4(ctx, args, net, training) -> dict:
     flops_plain = bc1_flops(nm)
     hist = np.bincount(nm[:, 0].clip(0, 16), minlength=17)
     search_ms = prof_plain.get("bc1_search_kernel", (0.0, 0))[0]
+
+    def hbm_share(ms):
+        """The launch's algorithmic bytes against the HBM peak."""
+        if not ms:
+            return None
+        gbs = HEADLINE_BYTES_PER_BLOCK * nb / (ms / 1e3) / 1e9
+        return {"achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                "algorithmic_bytes_per_launch": HEADLINE_BYTES_PER_BLOCK * nb}
+
     search_flops = bc1_flops(nm[nm[:, 0] >= 2])
     plain_roofline = {"kernel": "bc1_search_kernel", "bound": "valu",
                       "achieved": round(search_flops / (search_ms / 1e3) / 1e12, 3) if search_ms else None,
                       "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                       "frac": round(search_flops / (search_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 4) if search_ms else None,
-                      "traffic": None, "avg_launch_ms": round(search_ms, 4),
+                      "traffic": HEADLINE_TRAFFIC_BYTES["bc1_search_kernel"], "avg_launch_ms": round(search_ms, 4),
                       "algorithmic_flops_per_launch": search_flops,
+                      "valu_issue_frac_measured": HEADLINE_VALU_ISSUE_FRAC["bc1_search_kernel"],
+                      "hbm_share": hbm_share(search_ms),
                       "note": "icbc Level8 flops counted per block from its colour counts (n, m), blocks of two or "
                               "more colours; the classify pass finishes the others"}
     # The learned step's dominant kernel.
@@ -302,8 +326,11 @@ def bench_c4(ctx, args, net, training) -> dict:
     valu = MLP_VALU_FLOPS * nb / (mlp_ms / 1e3) / 1e12
     mfma = MLP_FORWARD_FLOPS * nb / (mlp_ms / 1e3) / 1e12
     mlp_roofline = {"kernel": "bc1_mlp_kernel", "bound": "valu", "achieved": round(valu, 3), "peak": PEAK_FP32_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(valu / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(valu / PEAK_FP32_TFLOPS, 4),
+                    "traffic": HEADLINE_TRAFFIC_BYTES["bc1_mlp_kernel"],
                     "avg_launch_ms": round(mlp_ms, 4), "algorithmic_flops_per_launch": MLP_VALU_FLOPS * nb,
+                    "valu_issue_frac_measured": HEADLINE_VALU_ISSUE_FRAC["bc1_mlp_kernel"],
+                    "hbm_share": hbm_share(mlp_ms),
                     "mfma_share": {"achieved": round(mfma, 3), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                                    "frac": round(mfma / PEAK_BF16_TFLOPS, 5),
                                    "algorithmic_flops_per_launch": MLP_FORWARD_FLOPS * nb}}

@@ -1161,12 +1161,338 @@ void ntxo_bc3n_block(const uint8_t *bgra, int weight_by_alpha, uint8_t *out)
 }
 
 /* =========================================================================
+ * Highest / Production channel blocks: OptimalCompress::compressDXT5A
+ * (OptimalCompressDXT.cpp:512-607, computeAlphaError :189-216,
+ * computeAlphaIndices :219-246, alphaDistance :52-57)
+ * ========================================================================= */
+
+/* computeAlphaError with unit weights: sum over texels of the squared
+ * distance to the nearest palette entry (integers, so the reference's float
+ * sum is exact).  Like the reference it stops once the sum exceeds `bound`:
+ * such a pair cannot win, so the outcome is the full search's. */
+static unsigned alpha_error(const uint8_t *v, unsigned a0, unsigned a1, unsigned bound)
+{
+    uint8_t pal[8];
+    alpha_palette(a0, a1, pal);
+    unsigned total = 0;
+    for (int i = 0; i < 16; i++) {
+        unsigned best = ~0u;
+        for (int p = 0; p < 8; p++) {
+            const int d = (int)pal[p] - (int)v[i];
+            if ((unsigned)(d * d) < best) best = (unsigned)(d * d);
+        }
+        total += best;
+        if (total > bound) break;
+    }
+    return total;
+}
+
+/* OptimalCompress::compressDXT5A.  The reference seeds its running best
+ * error from whatever endpoints the output block held before the call
+ * (line 546); here the seed is the error of (max, min), the endpoints the
+ * search keeps when no pair beats the seed. */
+void ntxo_bc4_channel_optimal(const uint8_t *v, uint8_t *out)
+{
+    unsigned mina = 255, maxa = 0, mina01 = 255, maxa01 = 0;
+    for (int i = 0; i < 16; i++) {
+        const unsigned a = v[i];
+        if (a < mina) mina = a;
+        if (a > maxa) maxa = a;
+        if (a != 0 && a != 255) {
+            if (a < mina01) mina01 = a;
+            if (a > maxa01) maxa01 = a;
+        }
+    }
+    unsigned e0, e1;
+    if ((int)maxa - (int)mina < 8) {
+        e0 = maxa; e1 = mina;
+    } else if ((int)maxa01 - (int)mina01 < 6) {
+        e0 = mina01; e1 = maxa01;
+    } else {
+        unsigned best = alpha_error(v, maxa, mina, ~0u);
+        e0 = maxa; e1 = mina;
+        /* Eight-value palettes: a0 - a1 > 8 within the range widened by 8. */
+        const int lo = mina <= 8 ? 0 : (int)mina - 8, hi = maxa >= 247 ? 255 : (int)maxa + 8;
+        for (int a0 = lo + 9; a0 < hi; a0++)
+            for (int a1 = lo; a1 < a0 - 8; a1++) {
+                const unsigned e = alpha_error(v, (unsigned)a0, (unsigned)a1, best);
+                if (e < best) { best = e; e0 = (unsigned)a0; e1 = (unsigned)a1; }
+            }
+        /* Six-value palettes (0 and 255 implicit): endpoints swapped, the
+         * range of the texels other than 0 and 255 widened by 6. */
+        const int lo6 = mina01 <= 6 ? 0 : (int)mina01 - 6, hi6 = maxa01 >= 249 ? 255 : (int)maxa01 + 6;
+        for (int a0 = lo6 + 9; a0 < hi6; a0++)
+            for (int a1 = lo6; a1 < a0 - 8; a1++) {
+                const unsigned e = alpha_error(v, (unsigned)a1, (unsigned)a0, best);
+                if (e < best) { best = e; e0 = (unsigned)a1; e1 = (unsigned)a0; }
+            }
+    }
+    ablk block = { 0 };
+    ab_set_ends(&block, e0, e1);
+    alpha_indices(v, &block);
+    for (int i = 0; i < 8; i++) out[i] = (uint8_t)(block.u >> (8 * i));
+}
+
+void ntxo_bc4_channels_optimal(const uint8_t *v, size_t n, uint8_t *out)
+{
+    for (size_t i = 0; i < n; i++) ntxo_bc4_channel_optimal(v + 16 * i, out + 8 * i);
+}
+
+/* OptimalCompress::compressDXT1G(const ColorBlock &) (OptimalCompressDXT.cpp:315-390):
+ * green-only BC1 (red 31, blue 0) by exhaustive search of 6-bit green
+ * pairs, with computeGreenError (:71-110: four-colour palette of the 8-bit
+ * expansions, squared distance) and computeGreenIndices (:112-145).  The
+ * quantised green of 255 is (255 + 1) >> 2 = 64, which the 6-bit endpoint
+ * field stores as 0; the seed endpoints keep that wrap as the reference
+ * does. */
+static unsigned expand6(unsigned g) { g &= 63; return (g << 2) | (g >> 4); }
+static unsigned expand5(unsigned r) { r &= 31; return (r << 3) | (r >> 2); }
+
+static unsigned green_error(const uint8_t *bgra, unsigned g0, unsigned g1)
+{
+    int p[4];
+    p[0] = (int)expand6(g0); p[1] = (int)expand6(g1);
+    p[2] = (2 * p[0] + p[1]) / 3; p[3] = (2 * p[1] + p[0]) / 3;
+    unsigned total = 0;
+    for (int i = 0; i < 16; i++) {
+        const int g = bgra[4 * i + 1];
+        unsigned best = ~0u;
+        for (int k = 0; k < 4; k++) {
+            const unsigned e = (unsigned)((g - p[k]) * (g - p[k]));
+            if (e < best) best = e;
+        }
+        total += best;
+    }
+    return total;
+}
+
+void ntxo_bc1_green_optimal(const uint8_t *bgra, uint8_t *out)
+{
+    ntxo_init();
+    unsigned ming = 63, maxg = 0;
+    int single = 1;
+    for (int i = 0; i < 16; i++) {
+        const unsigned g = ((unsigned)bgra[4 * i + 1] + 1) >> 2;
+        if (g < ming) ming = g;
+        if (g > maxg) maxg = g;
+        single = single && bgra[4 * i + 1] == bgra[1];
+    }
+    if (single) {
+        const int g = bgra[1];
+        uint16_t c0 = (uint16_t)((31 << 11) | (g_omatch6[g][0] << 5));
+        uint16_t c1 = (uint16_t)((31 << 11) | (g_omatch6[g][1] << 5));
+        uint32_t idx = 0xaaaaaaaau;
+        if (c0 < c1) { uint16_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+        put_block(out, c0, c1, idx);
+        return;
+    }
+    unsigned best = green_error(bgra, maxg, ming), b0 = maxg, b1 = ming;
+    const unsigned lo = ming <= 4 ? 0 : ming - 4, hi = maxg >= 59 ? 63 : maxg + 4;
+    for (unsigned g0 = lo + 1; g0 <= hi; g0++)
+        for (unsigned g1 = lo; g1 < g0; g1++) {
+            const unsigned e = green_error(bgra, g0, g1);
+            if (e < best) { best = e; b0 = g0; b1 = g1; }
+        }
+    /* BlockDXT1::evaluatePalette (BlockDXT.cpp:43-99), green only. */
+    const uint16_t c0 = (uint16_t)((31 << 11) | ((b0 & 63) << 5)), c1 = (uint16_t)((31 << 11) | ((b1 & 63) << 5));
+    int p[4];
+    p[0] = (int)expand6(b0); p[1] = (int)expand6(b1);
+    if (c0 > c1) { p[2] = (2 * p[0] + p[1]) / 3; p[3] = (2 * p[1] + p[0]) / 3; }
+    else { p[2] = (p[0] + p[1]) / 2; p[3] = 0; }
+    uint32_t idx = 0;
+    for (int i = 0; i < 16; i++) {
+        const int g = bgra[4 * i + 1];
+        const unsigned d0 = (unsigned)((p[0] - g) * (p[0] - g)), d1 = (unsigned)((p[1] - g) * (p[1] - g));
+        const unsigned d2 = (unsigned)((p[2] - g) * (p[2] - g)), d3 = (unsigned)((p[3] - g) * (p[3] - g));
+        const unsigned x0 = (d1 > d2) & (d0 > d2), x1 = (d0 > d3) & (d1 > d3), x2 = (d0 > d3) & (d2 > d3);
+        idx |= (x2 | ((x0 | x1) << 1)) << (2 * i);
+    }
+    put_block(out, c0, c1, idx);
+}
+
+/* =========================================================================
+ * Fastest: QuickCompress::compressDXT1 / compressDXT5
+ * (QuickCompressDXT.cpp:45-452, 653-713, 823-827), in the reference's float
+ * operation order (nvmath Vector3: lerp a*s + t*b, dot x+y+z, division by a
+ * scalar per component).
+ * ========================================================================= */
+
+static v3 q_v3(float x, float y, float z) { v3 r = { x, y, z }; return r; }
+static float q_dist(v3 a, v3 b)
+{
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+static v3 q_lerp(v3 a, v3 b, float t)
+{
+    const float s = 1.0f - t;
+    return q_v3(a.x * s + t * b.x, a.y * s + t * b.y, a.z * s + t * b.z);
+}
+static float q_clamp(float x, float a, float b) { return nvmin(nvmax(x, a), b); }
+
+/* findMinMaxColorsBox (:72-82), selectDiagonal (:85-110), insetBBox (:112-117). */
+static void q_box(const v3 *blk, int num, v3 *mx, v3 *mn)
+{
+    *mx = q_v3(0, 0, 0);
+    *mn = q_v3(255, 255, 255);
+    for (int i = 0; i < num; i++) {
+        mx->x = nvmax(mx->x, blk[i].x); mx->y = nvmax(mx->y, blk[i].y); mx->z = nvmax(mx->z, blk[i].z);
+        mn->x = nvmin(mn->x, blk[i].x); mn->y = nvmin(mn->y, blk[i].y); mn->z = nvmin(mn->z, blk[i].z);
+    }
+    const v3 c = q_v3((mx->x + mn->x) * 0.5f, (mx->y + mn->y) * 0.5f, (mx->z + mn->z) * 0.5f);
+    float cx = 0.0f, cy = 0.0f;
+    for (int i = 0; i < num; i++) {
+        const float tx = blk[i].x - c.x, ty = blk[i].y - c.y, tz = blk[i].z - c.z;
+        cx += tx * tz;
+        cy += ty * tz;
+    }
+    float x0 = mx->x, y0 = mx->y, x1 = mn->x, y1 = mn->y;
+    if (cx < 0) { float t = x0; x0 = x1; x1 = t; }
+    if (cy < 0) { float t = y0; y0 = y1; y1 = t; }
+    mx->x = x0; mx->y = y0;
+    mn->x = x1; mn->y = y1;
+    const float k = (8.0f / 255.0f) / 16.0f;
+    const v3 in = q_v3((mx->x - mn->x) / 16.0f - k, (mx->y - mn->y) / 16.0f - k, (mx->z - mn->z) / 16.0f - k);
+    *mx = q_v3(q_clamp(mx->x - in.x, 0.0f, 255.0f), q_clamp(mx->y - in.y, 0.0f, 255.0f), q_clamp(mx->z - in.z, 0.0f, 255.0f));
+    *mn = q_v3(q_clamp(mn->x + in.x, 0.0f, 255.0f), q_clamp(mn->y + in.y, 0.0f, 255.0f), q_clamp(mn->z + in.z, 0.0f, 255.0f));
+}
+
+/* roundAndExpand (:122-150): floor to 5:6:5, step up when the next code's
+ * expansion is strictly nearer; v becomes the expanded colour. */
+static unsigned q_round_expand(v3 *v)
+{
+    unsigned r = (unsigned)floorf(q_clamp(v->x * (31.0f / 255.0f), 0.0f, 31.0f));
+    unsigned g = (unsigned)floorf(q_clamp(v->y * (63.0f / 255.0f), 0.0f, 63.0f));
+    unsigned b = (unsigned)floorf(q_clamp(v->z * (31.0f / 255.0f), 0.0f, 31.0f));
+    if (fabsf(v->x - (float)((((r + 1) << 3) | ((r + 1) >> 2)))) < fabsf(v->x - (float)(((r << 3) | (r >> 2)))))
+        r = r + 1 < 31 ? r + 1 : 31;
+    if (fabsf(v->y - (float)((((g + 1) << 2) | ((g + 1) >> 4)))) < fabsf(v->y - (float)(((g << 2) | (g >> 4)))))
+        g = g + 1 < 63 ? g + 1 : 63;
+    if (fabsf(v->z - (float)((((b + 1) << 3) | ((b + 1) >> 2)))) < fabsf(v->z - (float)(((b << 3) | (b >> 2)))))
+        b = b + 1 < 31 ? b + 1 : 31;
+    *v = q_v3((float)expand5(r), (float)expand6(g), (float)expand5(b));
+    return (r << 11) | (g << 5) | b;
+}
+
+/* computeIndices4 (:195-229). */
+static uint32_t q_indices4(const v3 *blk, v3 mx, v3 mn)
+{
+    const v3 p2 = q_lerp(mx, mn, 1.0f / 3.0f), p3 = q_lerp(mx, mn, 2.0f / 3.0f);
+    uint32_t idx = 0;
+    for (int i = 0; i < 16; i++) {
+        const float d0 = q_dist(mx, blk[i]), d1 = q_dist(mn, blk[i]), d2 = q_dist(p2, blk[i]), d3 = q_dist(p3, blk[i]);
+        const unsigned b0 = d0 > d3, b1 = d1 > d2, b2 = d0 > d2, b3 = d1 > d3, b4 = d2 > d3;
+        const unsigned x0 = b1 & b2, x1 = b0 & b3, x2 = b0 & b4;
+        idx |= (x2 | ((x0 | x1) << 1)) << (2 * i);
+    }
+    return idx;
+}
+
+/* optimizeEndPoints4 (:406-452): least squares on the current indices. */
+static void q_optimize4(const v3 *blk, uint16_t *c0, uint16_t *c1, uint32_t *idx)
+{
+    float a2 = 0.0f, b2 = 0.0f, ab = 0.0f;
+    v3 ax = q_v3(0, 0, 0), bx = q_v3(0, 0, 0);
+    for (int i = 0; i < 16; i++) {
+        const unsigned bits = *idx >> (2 * i);
+        float beta = (float)(bits & 1);
+        if (bits & 2) beta = (1 + beta) / 3.0f;
+        const float alpha = 1.0f - beta;
+        a2 += alpha * alpha;
+        b2 += beta * beta;
+        ab += alpha * beta;
+        ax.x += blk[i].x * alpha; ax.y += blk[i].y * alpha; ax.z += blk[i].z * alpha;
+        bx.x += blk[i].x * beta; bx.y += blk[i].y * beta; bx.z += blk[i].z * beta;
+    }
+    const float denom = a2 * b2 - ab * ab;
+    /* nv::equal(denom, 0) (nvmath.h:139-143): |denom| <= 1e-4 * max(1, |denom|). */
+    if (fabsf(denom) <= 0.0001f * nvmax(1.0f, nvmax(fabsf(denom), 0.0f))) return;
+    const float f = 1.0f / denom;
+    v3 a = q_v3((ax.x * b2 - bx.x * ab) * f, (ax.y * b2 - bx.y * ab) * f, (ax.z * b2 - bx.z * ab) * f);
+    v3 b = q_v3((bx.x * a2 - ax.x * ab) * f, (bx.y * a2 - ax.y * ab) * f, (bx.z * a2 - ax.z * ab) * f);
+    a = q_v3(q_clamp(a.x, 0.0f, 255.0f), q_clamp(a.y, 0.0f, 255.0f), q_clamp(a.z, 0.0f, 255.0f));
+    b = q_v3(q_clamp(b.x, 0.0f, 255.0f), q_clamp(b.y, 0.0f, 255.0f), q_clamp(b.z, 0.0f, 255.0f));
+    unsigned k0 = q_round_expand(&a), k1 = q_round_expand(&b);
+    if (k0 < k1) { v3 t = a; a = b; b = t; unsigned u = k0; k0 = k1; k1 = u; }
+    *c0 = (uint16_t)k0;
+    *c1 = (uint16_t)k1;
+    *idx = q_indices4(blk, a, b);
+}
+
+/* QuickCompress::compressDXT1 (:653-713) on BGRA texels. */
+static void q_dxt1(const uint8_t *bgra, uint8_t *out)
+{
+    int single = 1;
+    for (int i = 1; i < 16; i++)
+        single = single && bgra[4 * i] == bgra[0] && bgra[4 * i + 1] == bgra[1] && bgra[4 * i + 2] == bgra[2];
+    if (single) {
+        const int r = bgra[2], g = bgra[1], b = bgra[0];
+        uint16_t c0 = (uint16_t)((g_omatch5[r][0] << 11) | (g_omatch6[g][0] << 5) | g_omatch5[b][0]);
+        uint16_t c1 = (uint16_t)((g_omatch5[r][1] << 11) | (g_omatch6[g][1] << 5) | g_omatch5[b][1]);
+        uint32_t idx = 0xaaaaaaaau;
+        if (c0 < c1) { uint16_t t = c0; c0 = c1; c1 = t; idx ^= 0x55555555u; }
+        put_block(out, c0, c1, idx);
+        return;
+    }
+    v3 blk[16];
+    for (int i = 0; i < 16; i++) blk[i] = q_v3(bgra[4 * i + 2], bgra[4 * i + 1], bgra[4 * i]);
+    v3 mx, mn;
+    q_box(blk, 16, &mx, &mn);
+    unsigned k0 = q_round_expand(&mx), k1 = q_round_expand(&mn);
+    if (k0 < k1) { v3 t = mx; mx = mn; mn = t; unsigned u = k0; k0 = k1; k1 = u; }
+    uint16_t c0 = (uint16_t)k0, c1 = (uint16_t)k1;
+    uint32_t idx = q_indices4(blk, mx, mn);
+    q_optimize4(blk, &c0, &c1, &idx);
+    put_block(out, c0, c1, idx);
+}
+
+/* CompressorDXT5 / FastCompressorDXT5 / CompressorDXT5n / FastCompressorDXT5n
+ * at every quality (CompressorDX9.cpp:67-80, 145-222; chosen by
+ * Context.cpp:1080-1097).  Highest: alpha (BC3n: red) by
+ * OptimalCompress::compressDXT5A; BC3n colour by compressDXT1G.  Fastest:
+ * QuickCompress::compressDXT5 (:823-827), BC3n after swizzle(4, 1, 5, 0). */
+void ntxo_bc3_block_q(const uint8_t *bgra, int quality, int weight_by_alpha, const float *cw, uint8_t *out)
+{
+    ntxo_init();
+    uint8_t a[16];
+    for (int i = 0; i < 16; i++) a[i] = bgra[4 * i + 3];
+    if (quality == NTXO_QUALITY_FASTEST) {
+        q_dxt1(bgra, out + 8);
+        ntxo_bc4_channel(a, out);
+        return;
+    }
+    if (quality == NTXO_QUALITY_HIGHEST) ntxo_bc4_channel_optimal(a, out);
+    else ntxo_bc4_channel(a, out);
+    ntxo_bc3_color_block(bgra, weight_by_alpha, cw, out + 8);
+}
+
+void ntxo_bc3n_block_q(const uint8_t *bgra, int quality, int weight_by_alpha, uint8_t *out)
+{
+    ntxo_init();
+    uint8_t x[16];
+    for (int i = 0; i < 16; i++) x[i] = bgra[4 * i + 2];
+    if (quality == NTXO_QUALITY_FASTEST) {
+        uint8_t sw[64];
+        for (int i = 0; i < 16; i++) { sw[4 * i] = 0; sw[4 * i + 1] = bgra[4 * i + 1]; sw[4 * i + 2] = 0xFF; sw[4 * i + 3] = x[i]; }
+        q_dxt1(sw, out + 8);
+        ntxo_bc4_channel(x, out);
+        return;
+    }
+    if (quality == NTXO_QUALITY_HIGHEST) {
+        ntxo_bc1_green_optimal(bgra, out + 8);
+        ntxo_bc4_channel_optimal(x, out);
+        return;
+    }
+    ntxo_bc3n_block(bgra, weight_by_alpha, out);
+}
+
+/* =========================================================================
  * Whole images
  * ========================================================================= */
 
 size_t ntxo_output_size(int format, int w, int h)
 {
-    size_t bs = (format == NTXO_FMT_BC1 || format == NTXO_FMT_BC4) ? 8 : 16;
+    size_t bs = (format == NTXO_FMT_BC1 || format == NTXO_FMT_BC1A || format == NTXO_FMT_BC4) ? 8 : 16;
     return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * bs;
 }
 
@@ -1231,20 +1557,27 @@ static void do_block(job_t *j, int i)
     }
     uint8_t bgra[64];
     ntxo_gather_u8(j->w, j->h, j->data, bx, by, bgra);
-    if (j->format == NTXO_FMT_BC3) {
-        ntxo_bc3_block(bgra, j->alpha_mode == NTXO_ALPHA_TRANSPARENCY, j->cw, j->out + (size_t)i * 16);
+    const int wba = j->alpha_mode == NTXO_ALPHA_TRANSPARENCY;
+    /* Context.cpp:1098-1116: BC4/BC5 Production and Highest take
+     * ProductionCompressorBC4/BC5 (CompressorDX10.cpp:65-87). */
+    void (*channel)(const uint8_t *, uint8_t *) =
+        j->quality >= NTXO_QUALITY_PRODUCTION ? ntxo_bc4_channel_optimal : ntxo_bc4_channel;
+    if (j->format == NTXO_FMT_BC1A) {
+        ntxo_bc1a_block(bgra, wba, j->cw, j->out + (size_t)i * 8);
+    } else if (j->format == NTXO_FMT_BC3) {
+        ntxo_bc3_block_q(bgra, j->quality, wba, j->cw, j->out + (size_t)i * 16);
     } else if (j->format == NTXO_FMT_BC3N) {
-        ntxo_bc3n_block(bgra, j->alpha_mode == NTXO_ALPHA_TRANSPARENCY, j->out + (size_t)i * 16);
+        ntxo_bc3n_block_q(bgra, j->quality, wba, j->out + (size_t)i * 16);
     } else if (j->format == NTXO_FMT_BC4) {
         uint8_t v[16];
         for (int k = 0; k < 16; k++) v[k] = bgra[4 * k + 2];
-        ntxo_bc4_channel(v, j->out + (size_t)i * 8);
+        channel(v, j->out + (size_t)i * 8);
     } else {
         uint8_t v[16];
         for (int k = 0; k < 16; k++) v[k] = bgra[4 * k + 2];
-        ntxo_bc4_channel(v, j->out + (size_t)i * 16);
+        channel(v, j->out + (size_t)i * 16);
         for (int k = 0; k < 16; k++) v[k] = bgra[4 * k + 1];
-        ntxo_bc4_channel(v, j->out + (size_t)i * 16 + 8);
+        channel(v, j->out + (size_t)i * 16 + 8);
     }
 }
 

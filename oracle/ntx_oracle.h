@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 /* nvtt::Format values (nvtt.h:77-97). */
-enum { NTXO_FMT_BC1 = 1, NTXO_FMT_BC3 = 4, NTXO_FMT_BC3N = 5, NTXO_FMT_BC4 = 6, NTXO_FMT_BC5 = 7 };
+enum { NTXO_FMT_BC1 = 1, NTXO_FMT_BC1A = 2, NTXO_FMT_BC3 = 4, NTXO_FMT_BC3N = 5, NTXO_FMT_BC4 = 6, NTXO_FMT_BC5 = 7 };
 /* nvtt::MipmapFilter / nvtt::WrapMode / nvtt::AlphaMode values (nvtt.h). */
 enum { NTXO_FILTER_BOX = 0, NTXO_FILTER_TRIANGLE = 1, NTXO_FILTER_KAISER = 2 };
 enum { NTXO_WRAP_CLAMP = 0, NTXO_WRAP_REPEAT = 1, NTXO_WRAP_MIRROR = 2 };
@@ -70,6 +70,15 @@ void ntxo_omatch_alpha_tables(uint8_t *m5, uint8_t *m6);
 void ntxo_bc1a_squish(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out);
 void ntxo_bc3n_block(const uint8_t *bgra, int weight_by_alpha, uint8_t *out);
 void ntxo_bc3_block(const uint8_t *bgra, int weight_by_alpha, const float *cw, uint8_t *out16);
+/* OptimalCompress::compressDXT5A (OptimalCompressDXT.cpp:512-607) on one
+ * channel block, first error seed = the error of (max, min). */
+void ntxo_bc4_channel_optimal(const uint8_t *v16, uint8_t *out8);
+void ntxo_bc4_channels_optimal(const uint8_t *v, size_t n, uint8_t *out);
+/* OptimalCompress::compressDXT1G(ColorBlock) (OptimalCompressDXT.cpp:315-390). */
+void ntxo_bc1_green_optimal(const uint8_t *bgra, uint8_t *out8);
+/* CompressorDXT5(n) / FastCompressorDXT5(n) at any quality (CompressorDX9.cpp:67-80, 145-222). */
+void ntxo_bc3_block_q(const uint8_t *bgra, int quality, int weight_by_alpha, const float *cw, uint8_t *out16);
+void ntxo_bc3n_block_q(const uint8_t *bgra, int quality, int weight_by_alpha, uint8_t *out16);
 
 /* ---- whole-image compressors (planar RGBA float input, block-raster output) ---- */
 size_t ntxo_output_size(int format, int w, int h);

@@ -123,6 +123,26 @@ class Oracle:
         blk = np.ascontiguousarray(blk, np.uint8); out = np.zeros(16, np.uint8)
         self.lib.ntxo_bc4_decode(_p(blk), _p(out)); return out
 
+    def bc4_channel_optimal(self, v):
+        v = np.ascontiguousarray(v, np.uint8).reshape(16); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc4_channel_optimal(_p(v), _p(out)); return out
+
+    def bc4_channels_optimal(self, v):
+        v = np.ascontiguousarray(v, np.uint8); n = v.shape[0]; out = np.zeros((n, 8), np.uint8)
+        self.lib.ntxo_bc4_channels_optimal(_p(v), ctypes.c_size_t(n), _p(out)); return out
+
+    def bc1_green_optimal(self, bgra):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); out = np.zeros(8, np.uint8)
+        self.lib.ntxo_bc1_green_optimal(_p(b), _p(out)); return out
+
+    def bc3_block_q(self, bgra, quality, weight_by_alpha=0, cw=(1, 1, 1)):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); cwv = f32(cw); out = np.zeros(16, np.uint8)
+        self.lib.ntxo_bc3_block_q(_p(b), int(quality), int(weight_by_alpha), _p(cwv), _p(out)); return out
+
+    def bc3n_block_q(self, bgra, quality, weight_by_alpha=0):
+        b = np.ascontiguousarray(bgra, np.uint8).reshape(64); out = np.zeros(16, np.uint8)
+        self.lib.ntxo_bc3n_block_q(_p(b), int(quality), int(weight_by_alpha), _p(out)); return out
+
     # ---- images
     def compress(self, fmt, planar, w, h, alpha_mode=0, cw=(1, 1, 1, 1), threads=8):
         src = f32(planar).reshape(-1); assert src.size == 4 * w * h

@@ -17,6 +17,10 @@ ICBC_SIMD=2, SQUISH_USE_SSE=0):
   (BlockCompressor.cpp:60-205) and ParallelTaskDispatcher;
 * CompressorDXT1a::compressBlock (CompressorDX9.cpp:83-111) with
   OptimalCompress::compressDXT1a and the OMatchAlpha tables;
+* OptimalCompress::compressDXT5A and compressDXT1G (OptimalCompressDXT.cpp:
+  315-390, 512-607) and QuickCompress::compressDXT1 / compressDXT5
+  (QuickCompressDXT.cpp:653-713, 823-827): the Production / Highest /
+  Fastest variants of BC3, BC3n, BC4 and BC5;
 * the BlockDXT decoders as Surface::setImage2D calls them (Surface.cpp:1022-1107);
 * nvmath powf_5_11 / powf_11_5, FloatImage::toLinear / toGamma / normalize.
 
@@ -105,7 +109,7 @@ def test_images_through_the_reference_image_loop(oracle, ref, fmt):
     """Whole images: the oracle's image compressor against the reference's
     ColorBlockCompressor / FloatColorCompressor loop on its thread pool, both
     alpha modes, partial edge blocks."""
-    qualities = (0, 1, 2) if fmt == FMT_BC1 else (1,)
+    qualities = (0, 1, 2) if fmt in (FMT_BC1, FMT_BC3, FMT_BC3N) else (1,)
     inputs = [(synth.noise_rect_rgba(64, 64, seed=3, transparent=True), 64, 64),
               (synth.noise_rect_rgba(37, 23, seed=4, transparent=True), 37, 23),
               (synth.normal_map_rgba(48, 20, seed=5), 48, 20), (synth.single_channel_rgba(5, 3, seed=6), 5, 3)]
@@ -116,6 +120,44 @@ def test_images_through_the_reference_image_loop(oracle, ref, fmt):
                 a = oracle.compress_q(fmt, q, pl, w, h, alpha_mode=am)
                 b = ref.compress_image(fmt, q, pl, w, h, alpha_mode=am)
                 assert np.array_equal(a, b), (fmt, w, h, am, q)
+
+
+def test_optimal_dxt5a_channels(oracle, ref):
+    """OptimalCompress::compressDXT5A (OptimalCompressDXT.cpp:512-607) with
+    the first error seed taken from (max, min): every kind of channel block
+    (full-range noise, clusters, two and three levels, 0/255 with a middle
+    value, ramps, near-constant, constant)."""
+    v = blocks.channel_set(20261204, 4000)
+    got = oracle.bc4_channels_optimal(v)
+    exp = np.stack([ref.bc4_channel_optimal(x) for x in v])
+    bad = np.flatnonzero(np.any(got != exp, axis=1))
+    assert bad.size == 0, f"{bad.size} of {len(v)} differ, first {v[bad[0]].tolist()}"
+
+
+@pytest.mark.parametrize("fmt,quality", [(FMT_BC3, 0), (FMT_BC3, 3), (FMT_BC3N, 0), (FMT_BC3N, 3), (FMT_BC4, 2),
+                                         (FMT_BC4, 3), (FMT_BC5, 2), (FMT_BC5, 3)])
+def test_quality_variants_against_the_reference_compressors(oracle, ref, fmt, quality):
+    """The compressors Context.cpp:1075-1116 picks for Fastest (FastCompressorDXT5/5n:
+    QuickCompress::compressDXT5), Highest (CompressorDXT5/5n with
+    OptimalCompress::compressDXT5A, and compressDXT1G for BC3n colour) and
+    Production (ProductionCompressorBC4/BC5), block by block, both alpha
+    modes.  The reference's OptimalCompress seed is fixed the same way on
+    both sides."""
+    px = blocks.bgra_set(seed=41 + fmt + quality, count=1200)
+    # Greens at 255 quantise to 64 in compressDXT1G; near-flat greens take its seed.
+    px[::7, :, 1] = 255 - (np.arange(16) % 2)
+    px[3::11, :, 1] = 128 + (np.arange(16) % 3)
+    for i in range(len(px)):
+        for am in (0, 1):
+            if fmt == FMT_BC3:
+                a = oracle.bc3_block_q(px[i], quality, am)
+            elif fmt == FMT_BC3N:
+                a = oracle.bc3n_block_q(px[i], quality, am)
+            elif fmt == FMT_BC4:
+                a = oracle.bc4_channel_optimal(px[i][:, 2])
+            else:
+                a = np.concatenate([oracle.bc4_channel_optimal(px[i][:, 2]), oracle.bc4_channel_optimal(px[i][:, 1])])
+            assert np.array_equal(a, ref.block(fmt, quality, px[i], am)), (i, am)
 
 
 def test_bc1a_blocks_against_the_reference_compressor(oracle, ref):
