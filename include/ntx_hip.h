@@ -114,7 +114,10 @@ enum {
     NTX_KERNEL_NORMALS = 16,     /* normal-map renormalisation of a generated level */
     NTX_KERNEL_BC1_CLASSIFY = 17, /* BC1 pass 1 (large images): colour counts, 0/1-colour blocks, bucketing */
     NTX_KERNEL_BC1_SEARCH = 18,  /* BC1 pass 2: cluster fit, one block per lane, over the buckets */
-    NTX_KERNEL_COUNT = 19
+    NTX_KERNEL_OPTIMAL_CHANNEL = 19, /* OptimalCompress::compressDXT5A, one wavefront per channel block */
+    NTX_KERNEL_OPTIMAL_GREEN = 20,   /* OptimalCompress::compressDXT1G (BC3n Highest colour), one wavefront per block */
+    NTX_KERNEL_BC3_FAST = 21,        /* QuickCompress::compressDXT5 (BC3 / BC3n Fastest), one thread per block */
+    NTX_KERNEL_COUNT = 22
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -133,10 +136,20 @@ NTX_API size_t ntx_hip_compressed_size(int format, int w, int h);
  *   BC1 (every quality; BlockCompressor.cpp:116-222): icbc compress_dxt1 per
  *     block at Level1 (Fastest), Level8 (Normal, Highest) or Level9
  *     (Production);
- *   BC3 / BC3n (Normal, Production; CompressorDX9.cpp:145-222):
- *     QuickCompress alpha + squish weighted cluster fit colour;
- *   BC4 / BC5 (Fastest, Normal; CompressorDX10.cpp:42-60):
- *     QuickCompress::compressDXT5A on R / R,G.
+ *   BC3 / BC3n (every quality; Context.cpp:1080-1097 picks the compressor):
+ *     Fastest: QuickCompress::compressDXT5 (FastCompressorDXT5 / DXT5n,
+ *     CompressorDX9.cpp:67-80); Normal, Production: QuickCompress alpha +
+ *     squish weighted cluster fit colour (CompressorDX9.cpp:145-222);
+ *     Highest: OptimalCompress::compressDXT5A alpha (BC3n: red) and, for
+ *     BC3n, OptimalCompress::compressDXT1G colour;
+ *   BC4 / BC5 (every quality; Context.cpp:1098-1116): Fastest, Normal:
+ *     QuickCompress::compressDXT5A on R / R,G (CompressorDX10.cpp:42-60);
+ *     Production, Highest: OptimalCompress::compressDXT5A
+ *     (CompressorDX10.cpp:65-87).
+ * OptimalCompress::compressDXT5A reads its first error seed from the output
+ * block before writing it (OptimalCompressDXT.cpp:546), so the reference's
+ * result depends on what that memory held; here the seed is the error of
+ * the (max, min) endpoints it falls back to, a fixed choice.
  * rgba: planar FP32, four planes R,G,B,A of w*h floats (Surface::data(),
  * Surface.cpp:492-495).  out: ntx_hip_compressed_size() bytes in block-raster
  * order.  color_weights: CompressionOptions::setColorWeights (r,g,b,a).
@@ -327,9 +340,9 @@ NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_r
  * per-image entry points above produce for the same level.
  *
  * Formats: BC1 at every quality (icbc Level1 / Level8 / Level9, one
- * block per 16-lane group), BC4 and BC5 (Fastest/Normal).  BC3 / BC3n return
- * NTX_ERR_UNSUPPORTED here; use ntx_hip_compress_chain_device or
- * ntx_hip_compress_images_device for them. */
+ * block per 16-lane group), BC4 and BC5 (Fastest/Normal).  BC3 / BC3n, and
+ * BC4 / BC5 at Production / Highest, return NTX_ERR_UNSUPPORTED here; use
+ * ntx_hip_compress_chain_device or ntx_hip_compress_images_device for them. */
 typedef struct {
     const float *d_rgba;   /* level 0: planar FP32 R,G,B,A planes of width*height, device memory */
     int width, height;

@@ -276,6 +276,10 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
         if (d.mip_levels < 0 || d.mip_levels > count_mipmaps(d.width, d.height)) return NTX_ERR_INVALID_ARGUMENT;
         if (d.format != NTX_FORMAT_BC1 && d.format != NTX_FORMAT_BC4 && d.format != NTX_FORMAT_BC5) return NTX_ERR_UNSUPPORTED;
         if (!ntx::supported(d.format, d.quality)) return NTX_ERR_UNSUPPORTED;
+        // The batch kernel's channel encoder is QuickCompress::compressDXT5A;
+        // BC4 / BC5 at Production and Highest (OptimalCompress) go through
+        // the per-image and chain entry points.
+        if (d.format != NTX_FORMAT_BC1 && d.quality >= NTX_QUALITY_PRODUCTION) return NTX_ERR_UNSUPPORTED;
     }
     ntx_hip_batch *b = new ntx_hip_batch;
     b->ctx = ctx;

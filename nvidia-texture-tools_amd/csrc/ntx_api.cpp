@@ -118,12 +118,14 @@ bool supported(int format, int quality)
     // BlockCompressor.cpp:211-217: every quality maps to an icbc level.
     case NTX_FORMAT_BC1:
         return quality >= NTX_QUALITY_FASTEST && quality <= NTX_QUALITY_HIGHEST;
-    // CompressorDX9.cpp:145-222: Highest switches the alpha block to
-    // OptimalCompress, Fastest to other compressors (Context.cpp:1075-1097).
+    // Context.cpp:1080-1116: Fastest takes FastCompressorDXT5 / DXT5n,
+    // Highest OptimalCompress for the BC3 alpha and both BC3n halves
+    // (CompressorDX9.cpp:67-80, 145-222); BC4 / BC5 take the Production
+    // compressors at Production and Highest (CompressorDX10.cpp:42-87).
     case NTX_FORMAT_BC3:
-    case NTX_FORMAT_BC3N: return quality == NTX_QUALITY_NORMAL || quality == NTX_QUALITY_PRODUCTION;
+    case NTX_FORMAT_BC3N:
     case NTX_FORMAT_BC4:
-    case NTX_FORMAT_BC5: return quality == NTX_QUALITY_FASTEST || quality == NTX_QUALITY_NORMAL;
+    case NTX_FORMAT_BC5: return quality >= NTX_QUALITY_FASTEST && quality <= NTX_QUALITY_HIGHEST;
     default: return false;
     }
 }
@@ -141,6 +143,12 @@ int bc3_lanes_shift(int nblocks, int target_log2)
     return s;
 }
 
+// Whether a BC3 / BC3n level's colour half is the squish cluster fit.
+bool bc3_cluster_fit(int format, int quality)
+{
+    return quality != NTX_QUALITY_FASTEST && !(format == NTX_FORMAT_BC3N && quality == NTX_QUALITY_HIGHEST);
+}
+
 bool bc1_two_pass(const ntx_hip_context *ctx, int quality, int w, int h)
 {
     const size_t nblocks = size_t((w + 3) / 4) * size_t((h + 3) / 4);
@@ -150,7 +158,10 @@ bool bc1_two_pass(const ntx_hip_context *ctx, int quality, int w, int h)
 size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, int h)
 {
     const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
-    if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) return size_t(bc3_scratch_ints(nblocks));
+    // The weighted-cluster-fit colour of BC3 / BC3n buckets its blocks;
+    // Fastest and the BC3n Highest colour (compressDXT1G) need no scratch.
+    if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
+        return bc3_cluster_fit(format, quality) ? size_t(bc3_scratch_ints(nblocks)) : 0;
     if (format == NTX_FORMAT_BC1 && bc1_two_pass(ctx, quality, w, h)) return size_t(bc1_scratch_ints(nblocks));
     return 0;
 }
@@ -380,7 +391,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "bc1_list_kernel", "mlp_train_kernel",
                                                    "bc3_search_split_kernel", "gamma_kernel",
                                                    "normal_kernel", "bc1_classify_kernel",
-                                                   "bc1_search_kernel" };
+                                                   "bc1_search_kernel", "optimal_channel_kernel",
+                                                   "optimal_green_kernel", "bc3_fast_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -456,6 +468,23 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
             return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, d_nm, stream, urgent,
                               bc1_level(quality));
         });
+    } else if ((format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) && quality == NTX_QUALITY_FASTEST) {
+        e = timed_on(ctx, stream, NTX_KERNEL_BC3_FAST, [&] {
+            return launch_bc3_fast(d_rgba, w, h, format == NTX_FORMAT_BC3N, ctx->d_tab, d_out, stream);
+        });
+    } else if (format == NTX_FORMAT_BC3N && quality == NTX_QUALITY_HIGHEST) {
+        // CompressorDXT5n at Highest: compressDXT1G colour, OptimalCompress red.
+        e = timed_on(ctx, stream, NTX_KERNEL_OPTIMAL_GREEN, [&] {
+            return launch_optimal_green(d_rgba, w, h, ctx->d_tab, d_out, kOptimalGroupsPerCU * ctx->cus, stream);
+        });
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, NTX_KERNEL_OPTIMAL_CHANNEL, [&] {
+                return launch_optimal_channels(d_rgba, w, h, format, d_out, kOptimalGroupsPerCU * ctx->cus, stream);
+            });
+    } else if ((format == NTX_FORMAT_BC4 || format == NTX_FORMAT_BC5) && quality >= NTX_QUALITY_PRODUCTION) {
+        e = timed_on(ctx, stream, NTX_KERNEL_OPTIMAL_CHANNEL, [&] {
+            return launch_optimal_channels(d_rgba, w, h, format, d_out, kOptimalGroupsPerCU * ctx->cus, stream);
+        });
     } else if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) {
         // CompressorDXT5n: metric (0, 1, 0) whatever the options say.
         const float green[3] = { 0.0f, 1.0f, 0.0f };
@@ -479,6 +508,12 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
                                          stream);
             });
         if (e == hipSuccess) *counts_zero = true;
+        // CompressorDXT5 at Highest: the alpha half by OptimalCompress,
+        // over the QuickCompress half the front wrote.
+        if (e == hipSuccess && quality == NTX_QUALITY_HIGHEST)
+            e = timed_on(ctx, stream, NTX_KERNEL_OPTIMAL_CHANNEL, [&] {
+                return launch_optimal_channels(d_rgba, w, h, format, d_out, kOptimalGroupsPerCU * ctx->cus, stream);
+            });
     } else {
         e = timed_on(ctx, stream, NTX_KERNEL_CHANNEL,
                      [&] { return launch_channels(d_rgba, w, h, format, d_out, stream, urgent); });

@@ -23,6 +23,9 @@ hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, c
                              hipStream_t stream);
 // Workgroups of bc1_search_kernel resident at once per compute unit.
 constexpr int kBc1SearchGroupsPerCU = 8;
+// Workgroups (four wavefronts each) per compute unit for the wavefront-per-
+// block OptimalCompress searches: a grid-stride loop over the blocks.
+constexpr int kOptimalGroupsPerCU = 8;
 int bc1_scratch_ints(int nblocks);
 hipError_t launch_channels(const float *d_src, int w, int h, int format, void *d_out, hipStream_t stream,
                            bool urgent = false);
@@ -30,6 +33,12 @@ hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_ta
                             int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
                             hipStream_t stream, bool normal = false);
 hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream);
+hipError_t launch_optimal_channels(const float *d_src, int w, int h, int format, void *d_out, int groups,
+                                   hipStream_t stream);
+hipError_t launch_optimal_green(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int groups,
+                                hipStream_t stream);
+hipError_t launch_bc3_fast(const float *d_src, int w, int h, bool normal, const Tables *d_tab, void *d_out,
+                           hipStream_t stream);
 hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_by_alpha, const float *cw, void *d_out,
                              int *d_counts, const int *d_lists, int cap, const Tables *d_tab, int lpb_shift,
                              bool urgent, hipStream_t stream);

@@ -34,7 +34,8 @@ DEFAULT_FILTER = {FILTER_BOX: (0.5, None), FILTER_TRIANGLE: (1.0, None), FILTER_
 KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "poly_x_kernel", "poly_y_kernel",
            "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_front_kernel", "batch_kernel",
            "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel", "gamma_kernel",
-           "normal_kernel", "bc1_classify_kernel", "bc1_search_kernel")
+           "normal_kernel", "bc1_classify_kernel", "bc1_search_kernel", "optimal_channel_kernel",
+           "optimal_green_kernel", "bc3_fast_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (

@@ -802,14 +802,13 @@ struct Compressor::Private {
         switch (co.format) {
         case Format_BC1:  // BlockCompressor.cpp:211-217: icbc Level1 / 8 / 9 by quality
             return new HipCompressorBCn(device_input);
+        // Context.cpp:1080-1116: Fast*/Production*/Optimal variants by quality,
+        // every one reproduced by the device.
         case Format_BC3:
         case Format_BC3n:  // CompressorDXT5n (CompressorDX9.cpp:179-222)
-            if (co.quality == Quality_Normal || co.quality == Quality_Production) return new HipCompressorBCn(device_input);
-            break;
         case Format_BC4:
         case Format_BC5:
-            if (co.quality == Quality_Fastest || co.quality == Quality_Normal) return new HipCompressorBCn(device_input);
-            break;
+            return new HipCompressorBCn(device_input);
         default:
             break;
         }

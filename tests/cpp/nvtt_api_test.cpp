@@ -305,8 +305,7 @@ int run_api()
     CHECK(cap.events.size() == 14 && cap.events[0] == 1 && cap.events[7] == 3);  // begin, end, no data
 
     err.seen.clear();
-    co.setFormat(nvtt::Format_BC4);
-    co.setQuality(nvtt::Quality_Production);  // ProductionCompressorBC4: not on this path
+    co.setFormat(nvtt::Format_BC2);  // CompressorDXT3: not on this path
     CHECK(comp.compress(16, 8, 1, 0, 0, img.data(), co, oo));
     CHECK(err.seen.size() == 1 && err.seen[0] == nvtt::Error_UnsupportedFeature);
 

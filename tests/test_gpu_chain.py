@@ -88,7 +88,7 @@ def test_chain_rejects_bad_arguments(gpu):
         gpu.compress_chain_device(N.FORMAT_BC1, N.FILTER_BOX, [d] * 5, 8, 8, [out] * 5)
     assert e.value.status == 1
     with pytest.raises(N.NtxError) as e:
-        gpu.compress_chain_device(N.FORMAT_BC4, N.FILTER_BOX, [d], 8, 8, [out], quality=N.QUALITY_PRODUCTION)
+        gpu.compress_chain_device(N.FORMAT_BC4, N.FILTER_BOX, [d], 8, 8, [out], quality=4)  # no such quality
     assert e.value.status == 2
     d.free(); out.free()
 

@@ -245,10 +245,10 @@ def test_unpack_srgb_every_byte_value(gpu, oracle):
 def test_unsupported_and_invalid_calls_report_status(gpu):
     planar = np.zeros(4 * 16, np.float32)
     with pytest.raises(N.NtxError) as e:
-        gpu.compress(N.FORMAT_BC4, planar, 4, 4, quality=N.QUALITY_PRODUCTION)
+        gpu.compress(N.FORMAT_BC4, planar, 4, 4, quality=4)  # no such quality
     assert e.value.status == 2
     with pytest.raises(N.NtxError) as e:
-        gpu.compress(2, planar, 4, 4)  # Format_DXT1a: not on this path
+        gpu.compress(3, planar, 4, 4)  # Format_DXT3: not on this path
     assert e.value.status == 2
     with pytest.raises(N.NtxError) as e:
         gpu.downsample(N.FILTER_BOX, planar[:4], 1, 1)
