@@ -40,10 +40,13 @@ typedef enum {
     NTX_ERR_OUT_OF_MEMORY = 5
 } ntx_status;
 
-/* nvtt::Format values (nvtt.h:77-97) accepted by the compressor.  BC3N is
- * Format_BC3n (DXT5n, CompressorDX9.cpp:179-222): a normal map's X (red) in
- * the alpha block and Y (green) in the colour block. */
-enum { NTX_FORMAT_BC1 = 1, NTX_FORMAT_BC3 = 4, NTX_FORMAT_BC3N = 5, NTX_FORMAT_BC4 = 6, NTX_FORMAT_BC5 = 7 };
+/* nvtt::Format values (nvtt.h:77-97) accepted by the compressor.  BC1A is
+ * Format_BC1a (DXT1a, CompressorDX9.cpp:83-111): BC1 whose texels of alpha 0
+ * take the transparent index.  BC3N is Format_BC3n (DXT5n,
+ * CompressorDX9.cpp:179-222): a normal map's X (red) in the alpha block and
+ * Y (green) in the colour block. */
+enum { NTX_FORMAT_BC1 = 1, NTX_FORMAT_BC1A = 2, NTX_FORMAT_BC3 = 4, NTX_FORMAT_BC3N = 5, NTX_FORMAT_BC4 = 6,
+       NTX_FORMAT_BC5 = 7 };
 /* nvtt::Quality (nvtt.h:148-154). */
 enum { NTX_QUALITY_FASTEST = 0, NTX_QUALITY_NORMAL = 1, NTX_QUALITY_PRODUCTION = 2, NTX_QUALITY_HIGHEST = 3 };
 /* nvtt::AlphaMode (nvtt.h:274-279). */
@@ -117,7 +120,8 @@ enum {
     NTX_KERNEL_OPTIMAL_CHANNEL = 19, /* OptimalCompress::compressDXT5A, one wavefront per channel block */
     NTX_KERNEL_OPTIMAL_GREEN = 20,   /* OptimalCompress::compressDXT1G (BC3n Highest colour), one wavefront per block */
     NTX_KERNEL_BC3_FAST = 21,        /* QuickCompress::compressDXT5 (BC3 / BC3n Fastest), one thread per block */
-    NTX_KERNEL_COUNT = 22
+    NTX_KERNEL_BC1A = 22,            /* BC1a: bucketing front pass + kDxt1 cluster fit, one block per lane */
+    NTX_KERNEL_COUNT = 23
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);
@@ -136,6 +140,14 @@ NTX_API size_t ntx_hip_compressed_size(int format, int w, int h);
  *   BC1 (every quality; BlockCompressor.cpp:116-222): icbc compress_dxt1 per
  *     block at Level1 (Fastest), Level8 (Normal, Highest) or Level9
  *     (Production);
+ *   BC1a (Normal, Production, Highest; CompressorDX9.cpp:83-111):
+ *     OptimalCompress::compressDXT1a for blocks of one RGB, otherwise the
+ *     squish weighted cluster fit with kDxt1 (texels of alpha 0 take index
+ *     3; Compress3, then Compress4 unless a texel is transparent).  Fastest
+ *     (QuickCompress::compressDXT1a) returns NTX_ERR_UNSUPPORTED: its
+ *     three-colour index loop reads 16 entries of an array that holds only
+ *     the opaque texels (QuickCompressDXT.cpp:378-403, 740-760), so its
+ *     output depends on uninitialised memory;
  *   BC3 / BC3n (every quality; Context.cpp:1080-1097 picks the compressor):
  *     Fastest: QuickCompress::compressDXT5 (FastCompressorDXT5 / DXT5n,
  *     CompressorDX9.cpp:67-80); Normal, Production: QuickCompress alpha +

@@ -185,6 +185,80 @@ __global__ __launch_bounds__(256) void bc3_front_kernel(const float *__restrict_
     append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
 }
 
+// BC1a pass 1, one thread per block (CompressorDXT1a::compressBlock,
+// CompressorDX9.cpp:83-111): a block whose 16 texels share one RGB takes
+// OptimalCompress::compressDXT1a (OptimalCompressDXT.cpp:271-292: without
+// transparent texels compressDXT1's OMatch endpoints, otherwise the
+// OMatchAlpha endpoints with col0 <= col1 and indices 0xaaaaaaaa | the
+// alpha-0 mask); any other block is appended to the list of its number k of
+// distinct RGB values among the texels of non-zero alpha (the squish kDxt1
+// colour set) with its texels as R | G << 8 | B << 16 | A << 24.  A block
+// whose texels are all transparent but not of one RGB has an empty colour
+// set; the fit's single Compress3 candidate then has zero weights, its
+// endpoints clamp to 0 with error 0, and WriteColourBlock3 gives endpoints
+// (0, 0) with every index 3, which is written here directly.
+__global__ __launch_bounds__(256) void bc1a_front_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                         int nblocks, const ntx::Tables *__restrict__ tab,
+                                                         uint8_t *__restrict__ out, int *__restrict__ counts,
+                                                         int *__restrict__ lists, int cap, uint4 *__restrict__ packed)
+{
+    __shared__ int s_cnt[kCountSlots], s_base[kCountSlots];
+    const int t = threadIdx.x;
+    if (t < kCountSlots) s_cnt[t] = 0;
+    __syncthreads();
+    const int blk = blockIdx.x * blockDim.x + t;
+    int k = 0, local = -1;
+    if (blk < nblocks) {
+        const int bx = blk % bw, by = blk / bw;
+        const size_t plane = size_t(w) * size_t(h);
+        uint32_t px[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const size_t o = block_texel(w, h, bx, by, i);
+            px[i] = to_u8(src[o]) | to_u8(src[o + plane]) << 8 | to_u8(src[o + 2 * plane]) << 16 |
+                    to_u8(src[o + 3 * plane]) << 24;
+        }
+        uint32_t mask = 0;
+        bool single = true;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if ((px[i] >> 24) == 0) mask |= 3u << (2 * i);
+            single = single && ((px[i] ^ px[0]) & 0xffffffu) == 0;
+        }
+        uint2 *dst = reinterpret_cast<uint2 *>(out + size_t(blk) * 8);
+        if (single) {
+            const uint32_t r8 = px[0] & 0xff, g8 = (px[0] >> 8) & 0xff, b8 = (px[0] >> 16) & 0xff;
+            if (mask == 0) {
+                bc3_single_colour(tab, r8, g8, b8, dst);
+            } else {
+                uint32_t c0 = uint32_t(tab->omatcha5[r8] & 0xff) << 11 | uint32_t(tab->omatcha6[g8] & 0xff) << 5 |
+                              uint32_t(tab->omatcha5[b8] & 0xff);
+                uint32_t c1 = uint32_t(tab->omatcha5[r8] >> 8) << 11 | uint32_t(tab->omatcha6[g8] >> 8) << 5 |
+                              uint32_t(tab->omatcha5[b8] >> 8);
+                if (c0 > c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
+                *dst = make_uint2(c0 | c1 << 16, 0xaaaaaaaau | mask);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                bool d = (px[i] >> 24) != 0;
+#pragma unroll
+                for (int j = 0; j < i; j++) d = d && !((px[j] >> 24) != 0 && ((px[j] ^ px[i]) & 0xffffffu) == 0);
+                k += d ? 1 : 0;
+            }
+            if (k == 0) {
+                *dst = make_uint2(0u, 0xffffffffu);
+            } else {
+                local = atomicAdd(&s_cnt[k], 1);
+                uint4 *line = packed + size_t(blk) * 4;
+#pragma unroll
+                for (int q = 0; q < 4; q++) line[q] = make_uint4(px[4 * q], px[4 * q + 1], px[4 * q + 2], px[4 * q + 3]);
+            }
+        }
+    }
+    append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
+}
+
 // The Compress4 loop body (weightedclusterfit.cpp:507-541) on running sums,
 // in two parts: the cluster weights' terms (alpha2, beta2, alphabeta and
 // the inverse determinant), then the endpoints and the error.
@@ -236,6 +310,41 @@ __device__ __forceinline__ float sq_eval(float4 x0, float4 x1, float4 x2, float4
     return sq_eval_t(x0, x1, x2, xsum, sq_weights(x0.w, x1.w, x2.w, xsum.w), mx, my, mz, out);
 }
 
+// The scalar Compress3 loop body (weightedclusterfit.cpp:392-430): palette
+// {a, b, (a + b) / 2}, cluster weights w0 (a), w1 (the midpoint), w2 (b).
+__device__ __forceinline__ float sq_eval3(float4 x0, float4 x1, float4 xsum, float mx, float my, float mz, SqCand *out)
+{
+    const float w2 = (xsum.w - x0.w) - x1.w;
+    const float alpha2 = x0.w + x1.w * 0.25f;
+    const float beta2 = w2 + x1.w * 0.25f;
+    const float ab = x1.w * 0.25f;
+    const float factor = 1.0f / (alpha2 * beta2 - ab * ab);
+    const float axr = x0.x + x1.x * 0.5f, axg = x0.y + x1.y * 0.5f, axb = x0.z + x1.z * 0.5f;
+    const float bxr = xsum.x - axr, bxg = xsum.y - axg, bxb = xsum.z - axb;
+    float ar = (axr * beta2 - bxr * ab) * factor;
+    float ag = (axg * beta2 - bxg * ab) * factor;
+    float abl = (axb * beta2 - bxb * ab) * factor;
+    float br = (bxr * alpha2 - axr * ab) * factor;
+    float bg = (bxg * alpha2 - axg * ab) * factor;
+    float bb = (bxb * alpha2 - axb * ab) * factor;
+    ar = smin(1.0f, smax(0.0f, ar)); ag = smin(1.0f, smax(0.0f, ag)); abl = smin(1.0f, smax(0.0f, abl));
+    br = smin(1.0f, smax(0.0f, br)); bg = smin(1.0f, smax(0.0f, bg)); bb = smin(1.0f, smax(0.0f, bb));
+    ar = floorf(31.0f * ar + 0.5f) * (1.0f / 31.0f);
+    ag = floorf(63.0f * ag + 0.5f) * (1.0f / 63.0f);
+    abl = floorf(31.0f * abl + 0.5f) * (1.0f / 31.0f);
+    br = floorf(31.0f * br + 0.5f) * (1.0f / 31.0f);
+    bg = floorf(63.0f * bg + 0.5f) * (1.0f / 63.0f);
+    bb = floorf(31.0f * bb + 0.5f) * (1.0f / 31.0f);
+    const float e1r = ((ar * ar) * alpha2 + (br * br) * beta2) + 2.0f * ((((ar * br) * ab) - ar * axr) - br * bxr);
+    const float e1g = ((ag * ag) * alpha2 + (bg * bg) * beta2) + 2.0f * ((((ag * bg) * ab) - ag * axg) - bg * bxg);
+    const float e1b = ((abl * abl) * alpha2 + (bb * bb) * beta2) + 2.0f * ((((abl * bb) * ab) - abl * axb) - bb * bxb);
+    if (out) {
+        out->ax = ar; out->ay = ag; out->az = abl;
+        out->bx = br; out->by = bg; out->bz = bb;
+    }
+    return e1r * mx + e1g * my + e1b * mz;
+}
+
 __device__ __forceinline__ float4 add4(float4 a, float4 b)
 {
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
@@ -247,12 +356,15 @@ __device__ __forceinline__ float4 add4(float4 a, float4 b)
 // `texels`.  Leaves the weighted points in sorted order in column `lane` of
 // s_p (rows from `count` on are zero) and returns the point count, the sum
 // of the weighted points, the sorted order (op) and each texel's point (pidx).
+// dxt1 (the kDxt1 colour set, colourset.cpp:44-51): texels of alpha 0 stay
+// out of the set (pidx -1, index 3 when written) and set `transparent`.
 __device__ __forceinline__ int bc3_prepare(float4 (*s_p)[64], int lane, const uint4 *__restrict__ texels,
                                            int weight_by_alpha, float cwx, float cwy, float cwz, float4 &xsum_out,
-                                           int op[16], int pidx[16])
+                                           int op[16], int pidx[16], bool dxt1, bool &transparent)
 {
     uint32_t rgb[16];
     float wt[16];
+    bool clear[16];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const uint4 v = texels[q];
@@ -261,19 +373,27 @@ __device__ __forceinline__ int bc3_prepare(float4 (*s_p)[64], int lane, const ui
         for (int j = 0; j < 4; j++) {
             rgb[4 * q + j] = t4[j] & 0xffffffu;
             wt[4 * q + j] = weight_by_alpha ? float((t4[j] >> 24) + 1) / 256.0f : 1.0f;
+            clear[4 * q + j] = dxt1 && (t4[j] >> 24) == 0;
         }
     }
 
     // ColourSet minimal set (colourset.cpp:35-101): a texel whose RGB bytes
-    // repeat an earlier texel adds its weight to that point, in texel order.
+    // repeat an earlier texel of the set adds its weight to that point, in
+    // texel order.
     int count = 0;
+    transparent = false;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
+        if (clear[i]) {
+            pidx[i] = -1;
+            transparent = true;
+            continue;
+        }
         int p = count;
         bool d = true;
 #pragma unroll
         for (int j = i - 1; j >= 0; j--)
-            if (rgb[j] == rgb[i]) { d = false; p = pidx[j]; }
+            if (!clear[j] && rgb[j] == rgb[i]) { d = false; p = pidx[j]; }
         pidx[i] = p;
         if (d) {
             s_p[p][lane] = make_float4(float(rgb[i] & 0xff) / 255.0f, float((rgb[i] >> 8) & 0xff) / 255.0f,
@@ -375,6 +495,35 @@ __device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int
         *dst = make_uint2(0u, 0u);  // the reference leaves the block unwritten here
         return;
     }
+    if (b2 < 0) {
+        // Compress3's winner (c0, c1) = (b0, b1): sorted positions below b0
+        // take index 0, the next b1 index 2, the rest 1, texels outside the
+        // set 3; WriteColourBlock3 (colourblock.cpp:75-106) orders the
+        // endpoints a <= b, swapping indices 0 and 1.
+        float4 y0 = zero, y1 = zero;
+        for (int i = 0; i < b0; i++) y0 = add4(y0, s_p[i][lane]);
+        for (int i = b0; i < b0 + b1; i++) y1 = add4(y1, s_p[i][lane]);
+        SqCand c;
+        sq_eval3(y0, y1, xsum, msx, msy, msz, &c);
+        int ca = (float_to_int(31.0f * c.ax, 31) << 11) | (float_to_int(63.0f * c.ay, 63) << 5) | float_to_int(31.0f * c.az, 31);
+        int cb = (float_to_int(31.0f * c.bx, 31) << 11) | (float_to_int(63.0f * c.by, 63) << 5) | float_to_int(31.0f * c.bz, 31);
+        uint32_t bits = 0;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            uint32_t idx = 3u;
+            if (pidx[t] >= 0) {
+                int pos = 0;
+#pragma unroll
+                for (int i = 0; i < 16; i++) pos += (op[i] == pidx[t]) ? i : 0;
+                idx = pos < b0 ? 0u : pos < b0 + b1 ? 2u : 1u;
+                if (ca > cb) idx = idx < 2u ? idx ^ 1u : idx;
+            }
+            bits |= idx << (2 * t);
+        }
+        if (ca > cb) { const int t = ca; ca = cb; cb = t; }
+        *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
+        return;
+    }
     // The winner's endpoints from the same running sums.
     float4 y0 = zero, y1 = zero, y2 = zero;
     for (int i = 0; i < b0; i++) y0 = add4(y0, s_p[i][lane]);
@@ -402,11 +551,15 @@ __device__ __forceinline__ void bc3_write(float4 (*s_p)[64], int lane, const int
     *dst = make_uint2(uint32_t(ca) | uint32_t(cb) << 16, bits);
 }
 
+// dxt1 (BC1a, the kDxt1 fit, colourfit.cpp:41-57): Compress3 first
+// (weightedclusterfit.cpp:372-474), then Compress4 unless a texel is
+// transparent, which replaces it only with a strictly lower error; lists
+// from k = 1.
 __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict__ packed, int weight_by_alpha,
                                                         float cwx, float cwy, float cwz, int stride, int offset,
                                                         uint8_t *__restrict__ out, int *__restrict__ counts,
                                                         const int *__restrict__ lists, int cap,
-                                                        const ntx::Tables *__restrict__ tab)
+                                                        const ntx::Tables *__restrict__ tab, int dxt1)
 {
     // Column `lane` holds this lane's points; row 16 stays zero (the
     // reference's loops read one element past the end and discard it).
@@ -417,7 +570,8 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
     // list never holds more than cap entries, nor indices at or above cap.
     int item = blockIdx.x * blockDim.x + lane;
     int k = 0, pos_in = 0;
-    for (int b = 16; b >= 2; b--) {
+    const int kmin = dxt1 ? 1 : 2;
+    for (int b = 16; b >= kmin; b--) {
         const int c = min(counts[b], cap);
         if (k == 0) {
             if (item < c) { k = b; pos_in = item; }
@@ -428,22 +582,41 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
     const bool live = k != 0 && unsigned(blk) < unsigned(cap);
     int count = 0;
     int op[16], pidx[16];
+    bool transparent = false;
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float4 xsum = zero;
     if (live) {
-        count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx);
+        count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op, pidx,
+                            dxt1 != 0, transparent);
+    }
+    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+
+    // Compress3 (kDxt1 only): first strict minimum over (c0, c1).
+    float best3 = FLT_MAX;
+    int d0 = 0, d1 = 0;
+    if (live && dxt1) {
+        float4 x0 = zero;
+        for (int c0 = 0; c0 <= count; c0++) {
+            float4 x1 = zero;
+            for (int c1 = 0; c1 <= count - c0; c1++) {
+                const float e = sq_eval3(x0, x1, xsum, msx, msy, msz, nullptr);
+                if (e < best3) { best3 = e; d0 = c0; d1 = c1; }
+                x1 = add4(x1, s_p[c0 + c1][lane]);  // row `count` is zero
+            }
+            x0 = add4(x0, s_p[c0][lane]);
+        }
     }
 
     // Compress4 (weightedclusterfit.cpp:495-553): first strict minimum in loop order.
-    const float msx = cwx * cwx, msy = cwy * cwy, msz = cwz * cwz;
+    const bool live4 = live && !transparent;
     float besterror = FLT_MAX;
     int b0 = 0, b1 = 0, b2 = 0;
     // A wave whose live lanes all hold 16 points of weight 1 takes the
     // weight terms of each candidate from sq16 (scalar loads shared by the
     // wave); idle lanes run along on whatever their LDS column holds and
     // write nothing.
-    const uint64_t live_mask = __builtin_amdgcn_ballot_w64(live);
-    const uint64_t unit_mask = __builtin_amdgcn_ballot_w64(live && count == 16 && !weight_by_alpha);
+    const uint64_t live_mask = __builtin_amdgcn_ballot_w64(live4);
+    const uint64_t unit_mask = __builtin_amdgcn_ballot_w64(live4 && count == 16 && !weight_by_alpha);
     if (live_mask != 0 && unit_mask == live_mask) {
         const float4 *__restrict__ tw = reinterpret_cast<const float4 *>(tab->sq16);
         int idx = 0;
@@ -461,7 +634,7 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             }
             x0 = add4(x0, s_p[c0][lane]);
         }
-    } else if (live && !weight_by_alpha) {
+    } else if (live4 && !weight_by_alpha) {
         // Integer weights (point multiplicities): each candidate's terms come
         // from sqw by the exact cluster weights; the next candidate's entry
         // is loaded while this one is evaluated.
@@ -485,7 +658,7 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             }
             x0 = add4(x0, s_p[c0][lane]);
         }
-    } else if (live) {
+    } else if (live4) {
         float4 x0 = zero;
         for (int c0 = 0; c0 <= count; c0++) {
             float4 x1 = zero;
@@ -500,6 +673,17 @@ __global__ __launch_bounds__(64) void bc3_search_kernel(const uint4 *__restrict_
             }
             x0 = add4(x0, s_p[c0][lane]);
         }
+    }
+    // Lanes that skipped Compress4 but ran along in the wave-uniform loops
+    // hold no result of it.
+    if (!live4) besterror = FLT_MAX;
+    // Compress4 replaces Compress3's block only with a strictly lower error
+    // (ColourFit::m_besterror); b2 = -1 marks Compress3's winner.
+    if (dxt1 && !(besterror < best3)) {
+        besterror = best3;
+        b0 = d0;
+        b1 = d1;
+        b2 = -1;
     }
     if (live)
         bc3_write(s_p, lane, op, pidx, xsum, msx, msy, msz, besterror, b0, b1, b2,
@@ -545,8 +729,9 @@ __global__ __launch_bounds__(64) void bc3_search_split_kernel(const uint4 *__res
     float4 xsum = zero;
     if (live) {
         blk = listed;
+        bool transparent;
         const int count = bc3_prepare(s_p, lane, packed + size_t(blk) * 4, weight_by_alpha, cwx, cwy, cwz, xsum, op,
-                                      pidx);
+                                      pidx, false, transparent);
         const int first = tab->sq_off[count], total = tab->sq_off[count + 1] - first;
         const int begin = (part * total) >> lpb_shift, end = ((part + 1) * total) >> lpb_shift;
         if (begin < end) {
@@ -635,6 +820,24 @@ hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_ta
     return hipGetLastError();
 }
 
+// BC1a (CompressorDXT1a): pass 1 finishes single-colour and empty blocks and
+// buckets the others; pass 2 is the kDxt1 fit, one block per lane, into
+// 8-byte blocks.  Its last workgroup clears the tallies.
+hipError_t launch_bc1a(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
+                       void *d_out, int *d_counts, int *d_lists, int cap, uint32_t *d_packed, hipStream_t stream)
+{
+    const int bw = (w + 3) / 4, bh = (h + 3) / 4, nblocks = bw * bh;
+    if (nblocks == 0) return hipSuccess;
+    uint4 *packed = reinterpret_cast<uint4 *>(d_packed);
+    hipLaunchKernelGGL(bc1a_front_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, stream, d_src, w, h, bw, nblocks,
+                       d_tab, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap, packed);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, packed, weight_by_alpha,
+                       cw[0], cw[1], cw[2], 8, 0, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap, d_tab, 1);
+    return hipGetLastError();
+}
+
 // Zeroes the tallies and the ticket (a fill kernel, so the sequence stays
 // capturable in a graph); needed only for fresh scratch memory or after an
 // interrupted sequence.
@@ -657,7 +860,7 @@ hipError_t launch_bc3_search(const uint32_t *d_packed, int w, int h, int weight_
     if (lpb_shift <= 0) {
         hipLaunchKernelGGL(bc3_search_kernel, dim3((nblocks + 63) / 64), dim3(64), 0, stream, packed, weight_by_alpha,
                            cw[0], cw[1], cw[2], 16, 8, reinterpret_cast<uint8_t *>(d_out), d_counts, d_lists, cap,
-                           d_tab);
+                           d_tab, 0);
     } else {
         const size_t lanes = size_t(nblocks) << lpb_shift;
         hipLaunchKernelGGL(bc3_search_split_kernel, dim3(unsigned((lanes + 63) / 64)), dim3(64), 0, stream, packed,

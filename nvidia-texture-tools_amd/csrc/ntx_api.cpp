@@ -118,6 +118,9 @@ bool supported(int format, int quality)
     // BlockCompressor.cpp:211-217: every quality maps to an icbc level.
     case NTX_FORMAT_BC1:
         return quality >= NTX_QUALITY_FASTEST && quality <= NTX_QUALITY_HIGHEST;
+    // Context.cpp:1058-1066: CompressorDXT1a above Fastest.  FastCompressorDXT1a
+    // reads uninitialised memory (QuickCompressDXT.cpp:378-403, 740-760).
+    case NTX_FORMAT_BC1A: return quality >= NTX_QUALITY_NORMAL && quality <= NTX_QUALITY_HIGHEST;
     // Context.cpp:1080-1116: Fastest takes FastCompressorDXT5 / DXT5n,
     // Highest OptimalCompress for the BC3 alpha and both BC3n halves
     // (CompressorDX9.cpp:67-80, 145-222); BC4 / BC5 take the Production
@@ -162,6 +165,7 @@ size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, 
     // Fastest and the BC3n Highest colour (compressDXT1G) need no scratch.
     if (format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N)
         return bc3_cluster_fit(format, quality) ? size_t(bc3_scratch_ints(nblocks)) : 0;
+    if (format == NTX_FORMAT_BC1A) return size_t(bc3_scratch_ints(nblocks));
     if (format == NTX_FORMAT_BC1 && bc1_two_pass(ctx, quality, w, h)) return size_t(bc1_scratch_ints(nblocks));
     return 0;
 }
@@ -392,7 +396,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "bc3_search_split_kernel", "gamma_kernel",
                                                    "normal_kernel", "bc1_classify_kernel",
                                                    "bc1_search_kernel", "optimal_channel_kernel",
-                                                   "optimal_green_kernel", "bc3_fast_kernel" };
+                                                   "optimal_green_kernel", "bc3_fast_kernel",
+                                                   "bc1a_front_kernel+bc3_search_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -429,7 +434,7 @@ NTX_API ntx_status ntx_hip_profile_read(ntx_hip_context *ctx, float *total_ms, i
 NTX_API size_t ntx_hip_compressed_size(int format, int w, int h)
 {
     if (w <= 0 || h <= 0) return 0;
-    const size_t bs = (format == NTX_FORMAT_BC1 || format == NTX_FORMAT_BC4) ? 8 : 16;
+    const size_t bs = (format == NTX_FORMAT_BC1 || format == NTX_FORMAT_BC1A || format == NTX_FORMAT_BC4) ? 8 : 16;
     return size_t((w + 3) / 4) * size_t((h + 3) / 4) * bs;
 }
 
@@ -468,6 +473,18 @@ static hipError_t compress_on(ntx_hip_context *ctx, hipStream_t stream, int form
             return launch_bc1(d_rgba, w, h, alpha_mode, cw, ctx->d_tab, d_out, d_nm, stream, urgent,
                               bc1_level(quality));
         });
+    } else if (format == NTX_FORMAT_BC1A) {
+        const int nblocks = ((w + 3) / 4) * ((h + 3) / 4);
+        int *counts = bucket, *lists = counts + bc3_lists_offset();
+        uint32_t *packed = reinterpret_cast<uint32_t *>(counts + bc3_packed_offset(nblocks));
+        if (!*counts_zero) e = launch_bc3_clear_counts(counts, stream);
+        *counts_zero = false;
+        if (e == hipSuccess)
+            e = timed_on(ctx, stream, NTX_KERNEL_BC1A, [&] {
+                return launch_bc1a(d_rgba, w, h, alpha_mode == NTX_ALPHA_TRANSPARENCY ? 1 : 0, cw, ctx->d_tab, d_out,
+                                   counts, lists, nblocks, packed, stream);
+            });
+        if (e == hipSuccess) *counts_zero = true;
     } else if ((format == NTX_FORMAT_BC3 || format == NTX_FORMAT_BC3N) && quality == NTX_QUALITY_FASTEST) {
         e = timed_on(ctx, stream, NTX_KERNEL_BC3_FAST, [&] {
             return launch_bc3_fast(d_rgba, w, h, format == NTX_FORMAT_BC3N, ctx->d_tab, d_out, stream);

@@ -56,6 +56,11 @@ struct Tables {
     // 2^((k - 127) * 11/5), as a float; 0 for k == 0 and k >= 256, +inf for 255.
     float pow_5_11[512];
     float pow_11_5[512];
+    // nvtt OMatchAlpha5/6 (SingleColorLookup.cpp:34-89 with the three-colour
+    // midpoint (hi + lo) / 2): mx | mn<<8, for BC1a blocks of one colour with
+    // transparent texels (OptimalCompress::compressDXT1a).
+    uint16_t omatcha5[256];
+    uint16_t omatcha6[256];
     // k / 255.0f for k = 0..255 (icbc's color_to_vector3, icbc.h:1611-1613).
     float div255[256];
 };

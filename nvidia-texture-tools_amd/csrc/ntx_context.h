@@ -33,6 +33,8 @@ hipError_t launch_bc3_front(const float *d_src, int w, int h, const Tables *d_ta
                             int *d_lists, int cap, int weight_by_alpha, uint32_t *d_packed, bool urgent,
                             hipStream_t stream, bool normal = false);
 hipError_t launch_bc3_clear_counts(int *d_counts, hipStream_t stream);
+hipError_t launch_bc1a(const float *d_src, int w, int h, int weight_by_alpha, const float *cw, const Tables *d_tab,
+                       void *d_out, int *d_counts, int *d_lists, int cap, uint32_t *d_packed, hipStream_t stream);
 hipError_t launch_optimal_channels(const float *d_src, int w, int h, int format, void *d_out, int groups,
                                    hipStream_t stream);
 hipError_t launch_optimal_green(const float *d_src, int w, int h, const Tables *d_tab, void *d_out, int groups,

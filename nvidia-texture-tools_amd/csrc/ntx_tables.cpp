@@ -73,6 +73,20 @@ void build_tables(Tables *t)
                 if (e < b6) { b6 = e; t->omatch6[v] = uint16_t(hi | lo << 8); }
             }
     }
+    // The same search with the three-colour midpoint (OMatchAlpha5/6).
+    for (int v = 0; v < 256; v++) {
+        int b5 = 25600, b6 = 25600;
+        for (int lo = 0; lo < 32; lo++)
+            for (int hi = 0; hi < 32; hi++) {
+                int e = std::abs((expand5(hi) + expand5(lo)) / 2 - v) * 100 + std::abs(hi - lo) * 3;
+                if (e < b5) { b5 = e; t->omatcha5[v] = uint16_t(hi | lo << 8); }
+            }
+        for (int lo = 0; lo < 64; lo++)
+            for (int hi = 0; hi < 64; hi++) {
+                int e = std::abs((expand6(hi) + expand6(lo)) / 2 - v) * 100 + std::abs(hi - lo) * 3;
+                if (e < b6) { b6 = e; t->omatcha6[v] = uint16_t(hi | lo << 8); }
+            }
+    }
 
     // icbc.h:1516-1526: midpoints between consecutive bit-expanded levels,
     // printed to six decimals in the reference table.

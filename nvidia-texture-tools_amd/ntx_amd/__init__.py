@@ -20,6 +20,7 @@ NVTT_LIB_PATH = os.path.join(LIB_DIR, "libnvtt_amd.so")
 # nvtt::Format / Quality / AlphaMode / MipmapFilter / WrapMode values (nvtt.h).
 FORMAT_BC1, FORMAT_BC3, FORMAT_BC4, FORMAT_BC5 = 1, 4, 6, 7
 FORMAT_BC3N = 5  # Format_BC3n (DXT5n): X in the alpha block, Y in the colour block
+FORMAT_BC1A = 2  # Format_BC1a (DXT1a): BC1 with the transparent index for alpha-0 texels
 QUALITY_FASTEST, QUALITY_NORMAL, QUALITY_PRODUCTION, QUALITY_HIGHEST = 0, 1, 2, 3
 DECODER_D3D10, DECODER_D3D9, DECODER_NV5X = 0, 1, 2
 METRIC_RMS_COLOR, METRIC_RMS_COLOR_ALPHA_WEIGHTED, METRIC_RMS_ALPHA = 0, 1, 2
@@ -35,7 +36,7 @@ KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "p
            "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_front_kernel", "batch_kernel",
            "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel", "gamma_kernel",
            "normal_kernel", "bc1_classify_kernel", "bc1_search_kernel", "optimal_channel_kernel",
-           "optimal_green_kernel", "bc3_fast_kernel")
+           "optimal_green_kernel", "bc3_fast_kernel", "bc1a_front_kernel+bc3_search_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (
@@ -175,7 +176,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 
 def compressed_size(fmt: int, w: int, h: int) -> int:
-    bs = 8 if fmt in (FORMAT_BC1, FORMAT_BC4) else 16
+    bs = 8 if fmt in (FORMAT_BC1, FORMAT_BC1A, FORMAT_BC4) else 16
     return ((w + 3) // 4) * ((h + 3) // 4) * bs
 
 

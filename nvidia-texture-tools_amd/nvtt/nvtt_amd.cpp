@@ -802,6 +802,9 @@ struct Compressor::Private {
         switch (co.format) {
         case Format_BC1:  // BlockCompressor.cpp:211-217: icbc Level1 / 8 / 9 by quality
             return new HipCompressorBCn(device_input);
+        case Format_BC1a:  // CompressorDXT1a (CompressorDX9.cpp:83-111); not FastCompressorDXT1a
+            if (co.quality != Quality_Fastest) return new HipCompressorBCn(device_input);
+            break;
         // Context.cpp:1080-1116: Fast*/Production*/Optimal variants by quality,
         // every one reproduced by the device.
         case Format_BC3:

@@ -127,3 +127,91 @@ def test_variant_chain_and_batch_match_per_level(gpu, oracle, fmt, quality):
         assert np.array_equal(d_out2[k].download(np.uint8), d_out[k].download(np.uint8)), sizes[k]
     for d in d_lv + d_out + d_out2:
         d.free()
+
+
+# ------------------------------------------------------------------ BC1a --
+
+def _bc1a_blocks(seed, n):
+    """CompressorDXT1a's cases: several colours with some alpha-0 texels,
+    one RGB with transparent texels, one RGB with alphas 0 / 128 / 255, one
+    RGB all transparent, several RGB all transparent, and opaque noise."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((n, 16, 4), np.uint8)
+    for i in range(n):
+        b = rng.integers(0, 256, (16, 4)).astype(np.uint8)
+        k = i % 6
+        if k == 0:
+            b[:, 3] = np.where(rng.random(16) < 0.3, 0, 255)
+        elif k == 1:
+            b[:] = b[0]
+            b[:, 3] = np.where(rng.random(16) < 0.4, 0, rng.integers(1, 256))
+        elif k == 2:
+            b[:, :3] = b[0, :3]
+            b[:, 3] = rng.choice([0, 128, 255], 16)
+        elif k == 3:
+            b[:] = b[0]
+            b[:, 3] = 0
+        elif k == 4:
+            b[:, 3] = 0
+        else:
+            b[:, 3] = 255
+            b[:, :3] = np.clip(b[0, :3].astype(int) + rng.integers(-20, 21, (16, 3)), 0, 255)
+        out[i] = b
+    return out
+
+
+def _blocks_rgba_image(px):
+    """n blocks of 16 RGBA texels (R, G, B, A order) side by side in a 4 x 4n image."""
+    n = len(px)
+    img = np.zeros((4, 4 * n, 4), np.uint8)
+    for t in range(16):
+        y, x = divmod(t, 4)
+        img[y, x::4] = px[:, t]
+    return img
+
+
+@pytest.mark.parametrize("quality", [N.QUALITY_NORMAL, N.QUALITY_PRODUCTION, N.QUALITY_HIGHEST])
+@pytest.mark.parametrize("size", SIZES + [(1024, 512)])
+def test_bc1a_images_match_oracle(gpu, oracle, quality, size):
+    w, h = size
+    img = synth.noise_rect_rgba(w, h, seed=51, transparent=True)
+    planar = synth.to_planar(img).reshape(-1)
+    for am in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
+        for cw in ((1, 1, 1, 1), (0.3, 0.59, 0.11, 1)):
+            exp = oracle.compress_q(N.FORMAT_BC1A, quality, planar, w, h, alpha_mode=am, cw=cw)
+            got = gpu.compress(N.FORMAT_BC1A, planar, w, h, quality=quality, alpha_mode=am, color_weights=cw)
+            diff = np.nonzero(np.any(got.reshape(-1, 8) != exp.reshape(-1, 8), axis=1))[0]
+            assert diff.size == 0, f"am {am} cw {cw}: {diff.size} blocks differ, first {diff[:5]}"
+
+
+def test_bc1a_adversarial_blocks(gpu, oracle):
+    px = _bc1a_blocks(52, 3000)
+    img = _blocks_rgba_image(px)
+    h, w = img.shape[:2]
+    planar = synth.to_planar(img).reshape(-1)
+    for am in (N.ALPHA_NONE, N.ALPHA_TRANSPARENCY):
+        exp = oracle.compress_q(N.FORMAT_BC1A, N.QUALITY_NORMAL, planar, w, h, alpha_mode=am)
+        got = gpu.compress(N.FORMAT_BC1A, planar, w, h, alpha_mode=am)
+        diff = np.nonzero(np.any(got.reshape(-1, 8) != exp.reshape(-1, 8), axis=1))[0]
+        assert diff.size == 0, f"am {am}: {diff.size} blocks differ, first {diff[:5]} (kinds {diff[:5] % 6})"
+
+
+def test_bc1a_chain_and_fastest(gpu, oracle):
+    """A Kaiser chain of a transparent texture as BC1a through the chain
+    scheduler; Fastest (FastCompressorDXT1a) is refused."""
+    w, h = 96, 80
+    planar = synth.to_planar(synth.noise_rect_rgba(w, h, seed=53, transparent=True)).reshape(-1)
+    sizes = N.mip_sizes(w, h)
+    d_lv = [gpu.alloc(16 * a * b) for a, b in sizes]
+    d_lv[0].upload(planar)
+    d_out = [gpu.alloc(N.compressed_size(N.FORMAT_BC1A, a, b)) for a, b in sizes]
+    gpu.compress_chain_device(N.FORMAT_BC1A, N.FILTER_KAISER, d_lv, w, h, d_out, alpha_mode=N.ALPHA_TRANSPARENCY)
+    for k, (a, b) in enumerate(sizes):
+        lv = d_lv[k].download(np.float32)
+        exp = oracle.compress_q(N.FORMAT_BC1A, N.QUALITY_NORMAL, lv, a, b, alpha_mode=N.ALPHA_TRANSPARENCY)
+        assert np.array_equal(d_out[k].download(np.uint8), exp), (a, b)
+    for d in d_lv + d_out:
+        d.free()
+    with pytest.raises(N.NtxError) as e:
+        gpu.compress(N.FORMAT_BC1A, planar, w, h, quality=N.QUALITY_FASTEST)
+    assert e.value.status == 2

@@ -169,7 +169,7 @@ def test_bc1a_blocks_against_the_reference_compressor(oracle, ref):
     extra = []
     for i in range(3000):
         b = rng.integers(0, 256, (16, 4)).astype(np.uint8)
-        k = i % 4
+        k = i % 5
         if k == 0:
             b[:, 3] = np.where(rng.random(16) < 0.3, 0, 255)
         elif k == 1:
@@ -178,9 +178,11 @@ def test_bc1a_blocks_against_the_reference_compressor(oracle, ref):
         elif k == 2:
             b[:, :3] = b[0, :3]
             b[:, 3] = rng.choice([0, 128, 255], 16)
-        else:
+        elif k == 3:
             b[:] = b[0]
             b[:, 3] = 0
+        else:
+            b[:, 3] = 0  # every texel transparent, several RGB values: an empty kDxt1 colour set
         extra.append(b)
     allpx = np.concatenate([px, np.stack(extra)])
     for i in range(len(allpx)):
