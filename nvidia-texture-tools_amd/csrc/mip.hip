@@ -107,7 +107,9 @@ __device__ __forceinline__ void tile_coords(int tile, int tx, int &tX, int &tY)
 // neighbour's (downsampling by exactly 2), so lane xi's taps start at column
 // 2*xi of the staged row and are read two at a time with 8-byte LDS reads
 // (consecutive lanes on consecutive 8-byte words: no bank conflicts).
-template <bool kPair>
+// kWin > 0: both passes have exactly kWin taps, known when compiling, so the
+// tap loops unroll without a test per tap; 0: wx and wy taps as passed.
+template <bool kPair, int kWin>
 __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
                                                          float *__restrict__ dst, int w, int h, int tx,
                                                          const float *__restrict__ kx, const int *__restrict__ lx, int wx,
@@ -115,6 +117,12 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
                                                          int wrap)
 {
     static_assert(2 * (kTX - 1) + kMaxWin <= kSrcW, "paired X reads stay inside a staged row");
+    static_assert(kWin >= 0 && kWin <= kMaxWin, "window held in registers");
+    constexpr int kTaps = kWin > 0 ? kWin : kMaxWin;  // unrolled trip count
+    if (kWin > 0) {
+        wx = kWin;
+        wy = kWin;
+    }
     __shared__ __align__(16) float s_src[kSrcH * kSrcW];
     __shared__ float s_tmp[kSrcH * kTX];
     __shared__ float s_ky[kTY * kMaxWin];
@@ -210,18 +218,20 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
                     // guard; only the taps below wx are summed.
                     // 64-bit words, so each pair is one ds_read_b64.
                     const uint64_t *row2 = reinterpret_cast<const uint64_t *>(row);
-                    uint64_t q[kMaxWin / 2];
+                    constexpr int kPairs = (kTaps + 1) / 2;
+                    uint64_t q[kPairs];
 #pragma unroll
-                    for (int p = 0; p < kMaxWin / 2; ++p) q[p] = row2[p];
+                    for (int p = 0; p < kPairs; ++p) q[p] = row2[p];
 #pragma unroll
-                    for (int p = 0; p < kMaxWin / 2; ++p) {
-                        if (2 * p < wx) sum += kxr[2 * p] * __uint_as_float(uint32_t(q[p]));
-                        if (2 * p + 1 < wx) sum += kxr[2 * p + 1] * __uint_as_float(uint32_t(q[p] >> 32));
+                    for (int p = 0; p < kPairs; ++p) {
+                        if (kWin > 0 || 2 * p < wx) sum += kxr[2 * p] * __uint_as_float(uint32_t(q[p]));
+                        if ((kWin > 0 && 2 * p + 1 < kWin) || (kWin == 0 && 2 * p + 1 < wx))
+                            sum += kxr[2 * p + 1] * __uint_as_float(uint32_t(q[p] >> 32));
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < kMaxWin; ++j)
-                        if (j < wx) sum += kxr[j] * row[j];
+                    for (int j = 0; j < kTaps; ++j)
+                        if (kWin > 0 || j < wx) sum += kxr[j] * row[j];
                 }
                 s_tmp[r * kTX + xi] = sum;
             }
@@ -234,12 +244,172 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
                 const float *col = s_tmp + s_ly[o] * kTX + xi;
                 float sum = 0.0f;
 #pragma unroll
-                for (int j = 0; j < kMaxWin; ++j)
-                    if (j < wy) sum += k[j] * col[j * kTX];
+                for (int j = 0; j < kTaps; ++j)
+                    if (kWin > 0 || j < wy) sum += k[j] * col[j * kTX];
                 out[size_t(oy0 + o) * size_t(w) + ox0 + xi] = sum;
             }
         }
         __syncthreads();
+    }
+}
+
+// The last levels of a chain in one launch: one workgroup per plane holds
+// the level in LDS and produces every following level from it, writing each
+// to its place in HBM.  These levels are small enough that a launch per
+// level costs more than its work.  Each level is the same arithmetic as
+// above: box (FloatImage::fastDownSample) through box_texel, or the
+// polyphase X taps into a temporary and the Y taps from it, in tap order.
+// Source extents: at most kTailSrc texels per plane.
+// The tail kernel keeps, for every level whose tables fit (kTailTaps
+// entries over all levels), each tap as (weight, source offset) with the
+// wrap already applied: the X taps' offsets within a source row, the Y
+// taps' offsets of the temporary's rows.  Other levels read the tables from
+// HBM and wrap per tap.
+constexpr int kTailTaps = 4096;
+
+// A workgroup barrier that orders LDS only.  The tail kernel's global stores
+// are never read back by the kernel, so its barriers need not wait for them
+// (__syncthreads would wait for every outstanding store to reach memory).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One polyphase level from packed taps (see kTailTaps): tx holds w rows of
+// wx taps, ty h rows of wy taps.
+__device__ __forceinline__ void tail_packed_level(const float *cur, float *nxt, float *out, float *s_t, const float2 *tx,
+                                                  int wx, const float2 *ty, int wy, int W, int w, int h, int H, int t,
+                                                  int nt)
+{
+    for (int i = t; i < w * H; i += nt) {
+        const int y = i / w, x = i - y * w;
+        const float2 *kr = tx + x * wx;
+        const float *row = cur + y * W;
+        float sum = 0.0f;
+#pragma unroll 4
+        for (int j = 0; j < wx; ++j) {
+            const float2 q = kr[j];
+            sum += q.x * row[__float_as_int(q.y)];
+        }
+        s_t[i] = sum;
+    }
+    lds_barrier();
+    for (int i = t; i < w * h; i += nt) {
+        const int y = i / w, x = i - y * w;
+        const float2 *kr = ty + y * wy;
+        float sum = 0.0f;
+#pragma unroll 4
+        for (int j = 0; j < wy; ++j) {
+            const float2 q = kr[j];
+            sum += q.x * s_t[__float_as_int(q.y) + x];
+        }
+        nxt[i] = sum;
+        out[i] = sum;
+    }
+}
+
+// One polyphase level inside the tail kernel: the X taps of every row of
+// cur (W x H) into s_t (w x H), then the Y taps of s_t into nxt and out.
+// Called with the tap tables either in LDS or in HBM, so that each call
+// reads them through one kind of address.
+__device__ __forceinline__ void tail_poly_level(const float *cur, float *nxt, float *out, float *s_t,
+                                                const float *kx, const int *lx, int wx, const float *ky, const int *ly,
+                                                int wy, int W, int H, int w, int h, int wrap, int t, int nt)
+{
+    for (int i = t; i < w * H; i += nt) {
+        const int y = i / w, x = i - y * w;
+        const float *kr = kx + x * wx;
+        const float *row = cur + y * W;
+        const int l = lx[x];
+        float sum = 0.0f;
+        if (l >= 0 && l + wx <= W) {
+#pragma unroll 4
+            for (int j = 0; j < wx; ++j) sum += kr[j] * row[l + j];
+        } else {
+            for (int j = 0; j < wx; ++j) sum += kr[j] * row[wrap_index(l + j, W, wrap)];
+        }
+        s_t[i] = sum;
+    }
+    lds_barrier();
+    for (int i = t; i < w * h; i += nt) {
+        const int y = i / w, x = i - y * w;
+        const float *kr = ky + y * wy;
+        const int l = ly[y];
+        float sum = 0.0f;
+        if (l >= 0 && l + wy <= H) {
+#pragma unroll 4
+            for (int j = 0; j < wy; ++j) sum += kr[j] * s_t[(l + j) * w + x];
+        } else {
+            for (int j = 0; j < wy; ++j) sum += kr[j] * s_t[wrap_index(l + j, H, wrap) * w + x];
+        }
+        nxt[i] = sum;
+        out[i] = sum;
+    }
+}
+
+__global__ __launch_bounds__(512) void mip_tail_kernel(const float *__restrict__ src, ntx::MipTail tail, int wrap)
+{
+    __shared__ float s_a[ntx::kTailSrc];
+    __shared__ float s_b[ntx::kTailSrc / 2];
+    __shared__ float s_t[ntx::kTailSrc];
+    __shared__ float2 s_k[kTailTaps];
+    const int t = threadIdx.x, nt = blockDim.x, c = blockIdx.x;
+    {
+        const float *plane = src + size_t(c) * size_t(tail.W) * size_t(tail.H);
+        for (int i = t; i < tail.W * tail.H; i += nt) s_a[i] = plane[i];
+    }
+    // Every level's taps that fit, in level order, packed once.
+    if (!tail.box) {
+        int W = tail.W, H = tail.H, ko = 0;
+        for (int k = 0; k < tail.levels; k++) {
+            const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+            const int wx = tail.wx[k], wy = tail.wy[k];
+            if (ko + w * wx + h * wy <= kTailTaps) {
+                for (int i = t; i < w * wx; i += nt) {
+                    const int x = i / wx, j = i - x * wx;
+                    s_k[ko + i] = make_float2(tail.kx[k][i], __int_as_float(wrap_index(tail.lx[k][x] + j, W, wrap)));
+                }
+                for (int i = t; i < h * wy; i += nt) {
+                    const int y = i / wy, j = i - y * wy;
+                    s_k[ko + w * wx + i] =
+                        make_float2(tail.ky[k][i], __int_as_float(wrap_index(tail.ly[k][y] + j, H, wrap) * w));
+                }
+                ko += w * wx + h * wy;
+            }
+            W = w;
+            H = h;
+        }
+    }
+    lds_barrier();
+    int W = tail.W, H = tail.H, ko = 0;
+    for (int k = 0; k < tail.levels; k++) {
+        // Level k reads s_a and writes s_b when k is even, the reverse when odd.
+        float *cur = (k & 1) ? s_b : s_a, *nxt = (k & 1) ? s_a : s_b;
+        const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+        float *out = tail.dst[k] + size_t(c) * size_t(w) * size_t(h);
+        if (tail.box) {
+            // 1D sources: box_texel takes the output index along the longer axis.
+            for (int i = t; i < w * h; i += nt) {
+                const int y = (W == 1 || H == 1) ? 0 : i / w, x = (W == 1 || H == 1) ? i : i - y * w;
+                const float v = ntx_mip::box_texel(cur, W, H, x, y);
+                nxt[i] = v;
+                out[i] = v;
+            }
+        } else {
+            const int wx = tail.wx[k], wy = tail.wy[k];
+            if (ko + w * wx + h * wy <= kTailTaps) {
+                tail_packed_level(cur, nxt, out, s_t, s_k + ko, wx, s_k + ko + w * wx, wy, W, w, h, H, t, nt);
+                ko += w * wx + h * wy;
+            } else {
+                tail_poly_level(cur, nxt, out, s_t, tail.kx[k], tail.lx[k], wx, tail.ky[k], tail.ly[k], wy, W, H, w, h,
+                                wrap, t, nt);
+            }
+        }
+        lds_barrier();
+        W = w;
+        H = h;
     }
 }
 
@@ -361,6 +531,16 @@ bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy)
 }
 
 int poly_tile_x() { return kTX; }
+
+// The last tail.levels levels of a chain from the level at d_src (tail.W x
+// tail.H, at most kTailSrc texels): one workgroup per plane.
+hipError_t launch_mip_tail(const float *d_src, const MipTail &tail, int wrap, hipStream_t stream)
+{
+    if (tail.levels < 1 || tail.levels > kTailMaxLevels || size_t(tail.W) * size_t(tail.H) > size_t(kTailSrc))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mip_tail_kernel, dim3(4), dim3(512), 0, stream, d_src, tail, wrap);
+    return hipGetLastError();
+}
 int poly_tile_y() { return kTY; }
 
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
@@ -370,12 +550,14 @@ hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, con
     const int tx = (w + kTX - 1) / kTX, ty = (h + kTY - 1) / kTY;
     const int split = tx * ty < 1024 ? 4 : 1;  // workgroups per tile (planes split between them)
     const dim3 grid(unsigned(tx * ty), 1, split);
-    if (pair_x)
-        hipLaunchKernelGGL(poly_fused_kernel<true>, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx,
-                           d_ky, d_ly, wy, wrap);
-    else
-        hipLaunchKernelGGL(poly_fused_kernel<false>, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx,
-                           wx, d_ky, d_ly, wy, wrap);
+    // Exact halving with the windows of Kaiser(3), Triangle(2) and
+    // Box / Triangle(1): taps unrolled for that count.
+    auto kern = pair_x ? poly_fused_kernel<true, 0> : poly_fused_kernel<false, 0>;
+    if (pair_x && wx == wy && wx == 13) kern = poly_fused_kernel<true, 13>;
+    if (pair_x && wx == wy && wx == 9) kern = poly_fused_kernel<true, 9>;
+    if (pair_x && wx == wy && wx == 5) kern = poly_fused_kernel<true, 5>;
+    hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx, d_ky, d_ly,
+                       wy, wrap);
     return hipGetLastError();
 }
 

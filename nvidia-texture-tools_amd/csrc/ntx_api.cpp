@@ -397,7 +397,7 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "normal_kernel", "bc1_classify_kernel",
                                                    "bc1_search_kernel", "optimal_channel_kernel",
                                                    "optimal_green_kernel", "bc3_fast_kernel",
-                                                   "bc1a_front_kernel+bc3_search_kernel" };
+                                                   "bc1a_front_kernel+bc3_search_kernel", "mip_tail_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -581,12 +581,41 @@ static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int fi
     return NTX_OK;
 }
 
-// Levels 1 .. levels-1 of a chain on `stream`, one launch per level.
+// Levels 1 .. levels-1 of a chain on `stream`: a launch per level while the
+// source is large, then every remaining level in one launch (launch_mip_tail)
+// once the source fits a workgroup's LDS, when two or more levels remain.
 static ntx_status downsample_chain_on(ntx_hip_context *ctx, hipStream_t stream, int filter, float width,
                                       const float *params, int wrap, int alpha_mode, const int *lw, const int *lh,
                                       int levels, float *const *d_levels)
 {
+    const bool box = filter == NTX_FILTER_BOX && width == 0.5f && alpha_mode != NTX_ALPHA_TRANSPARENCY;
     for (int k = 1; k < levels; k++) {
+        const int rest = levels - k;
+        if (rest >= 2 && rest <= kTailMaxLevels && size_t(lw[k - 1]) * size_t(lh[k - 1]) <= size_t(kTailSrc)) {
+            MipTail tail{};
+            tail.W = lw[k - 1];
+            tail.H = lh[k - 1];
+            tail.levels = rest;
+            tail.box = box ? 1 : 0;
+            for (int j = 0; j < rest; j++) {
+                tail.dst[j] = d_levels[k + j];
+                if (box) continue;
+                PolyDev *kx = nullptr, *ky = nullptr;
+                ntx_status s = poly_tables(ctx, filter, width, params, lw[k - 1 + j], lw[k + j], &kx);
+                if (!s) s = poly_tables(ctx, filter, width, params, lh[k - 1 + j], lh[k + j], &ky);
+                if (s) return s;
+                tail.kx[j] = kx->weights;
+                tail.lx[j] = kx->left;
+                tail.wx[j] = kx->window;
+                tail.ky[j] = ky->weights;
+                tail.ly[j] = ky->left;
+                tail.wy[j] = ky->window;
+            }
+            const hipError_t e = timed_on(ctx, stream, NTX_KERNEL_MIP_TAIL,
+                                          [&] { return launch_mip_tail(d_levels[k - 1], tail, wrap, stream); });
+            if (e != hipSuccess) return fail(ctx, e, "mip tail launch");
+            return NTX_OK;
+        }
         const ntx_status s = downsample_on(ctx, stream, filter, width, params, wrap, alpha_mode, d_levels[k - 1],
                                            lw[k - 1], lh[k - 1], d_levels[k]);
         if (s) return s;

@@ -12,6 +12,20 @@
 
 namespace ntx {
 
+// The last levels of a mip chain built by one launch (launch_mip_tail): the
+// first source level (W x H, at most kTailSrc texels per plane), then per
+// produced level its destination and, for the polyphase filters, its X and
+// Y tap tables (as PolyDev holds them); box: FloatImage::fastDownSample.
+constexpr int kTailMaxLevels = 16;
+constexpr int kTailSrc = 8192;
+struct MipTail {
+    int W, H, levels, box;
+    float *dst[kTailMaxLevels];
+    const float *kx[kTailMaxLevels], *ky[kTailMaxLevels];
+    const int *lx[kTailMaxLevels], *ly[kTailMaxLevels];
+    int wx[kTailMaxLevels], wy[kTailMaxLevels];
+};
+
 // Tables shared by host set-up and kernels.  Layout is fixed here so the host
 // builders and device code agree.
 struct Tables {

@@ -52,6 +52,7 @@ hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const f
                          hipStream_t stream);
 hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
                          hipStream_t stream);
+hipError_t launch_mip_tail(const float *d_src, const MipTail &tail, int wrap, hipStream_t stream);
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
                              bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
 bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);

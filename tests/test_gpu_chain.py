@@ -95,11 +95,14 @@ def test_chain_rejects_bad_arguments(gpu):
 
 @pytest.mark.parametrize("filt", [N.FILTER_BOX, N.FILTER_TRIANGLE, N.FILTER_KAISER])
 @pytest.mark.parametrize("wrap", [N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR])
-@pytest.mark.parametrize("size", [(512, 256), (130, 66), (37, 23), (1, 300), (300, 1), (96, 1), (2, 2), (1, 1)])
+@pytest.mark.parametrize("size", [(512, 256), (130, 66), (37, 23), (1, 300), (300, 1), (96, 1), (2, 2), (1, 1),
+                                  (90, 90), (129, 63), (1, 8192), (8193, 1), (256, 33)])
 def test_downsample_chain_matches_per_level(gpu, filt, wrap, size):
     """ntx_hip_downsample_chain_device against per-level downsampling, FP32
     bit patterns, including 1-D images and the Transparency box (polyphase)
-    case."""
+    case.  Levels from a source of at most 8192 texels run in one launch
+    (the tail kernel); the sizes put that source at level 0, at an odd
+    extent, at exactly 8192 texels and just above."""
     w, h = size
     planar = np.random.default_rng(w * 7 + h).random(4 * w * h, dtype=np.float32)
     sizes = N.mip_sizes(w, h)
