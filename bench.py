@@ -173,8 +173,8 @@ def main():
         result["c1_box_chain_bc1"] = bench_c1(ctx, args, not args.no_cpu)
         result["input_pipeline"] = bench_process(args, not args.no_cpu)
         if not args.no_batch:
-            result["batch"] = bench_batch(ctx, args)
-        result["cube_map"] = bench_cube(ctx, args, net)
+            result["batch"] = bench_batch(ctx, args, not args.no_cpu)
+        result["cube_map"] = bench_cube(ctx, args, net, not args.no_cpu)
     net.close()
     print(json.dumps(result), flush=True)
     ctx.close()
@@ -183,17 +183,26 @@ def main():
 def train_network(ctx, args):
     """The 48-64-64-25 network (BF16, FP32 accumulation) trained on the
     device with the project's own kernels on the blocks of 8 seeded 2048^2
-    noise+rectangle textures, each in 4 rotations (8,388,608 blocks cut; the
-    uniform ones dropped), labelled by the cluster fit.  Setup, not timed."""
+    noise+rectangle textures, each in 4 rotations with its Kaiser levels 1-3
+    (11,141,120 blocks cut; the uniform ones dropped), labelled by the
+    cluster fit.  Setup, not timed."""
     import ntx_amd as N
     from ntx_amd import learned, synth
 
     t0 = time.perf_counter()
     texs = []
+    sizes = N.mip_sizes(2048, 2048)[:4]
     for i in range(8):
         img = synth.noise_rect_rgba(2048, 2048, seed=args.seed + 2000 + i)
         for r in range(4):
-            texs.append((synth.to_planar(np.ascontiguousarray(np.rot90(img, r))).reshape(-1), 2048, 2048))
+            # The texture and its first three Kaiser(3; 4, 1) levels: the
+            # network also sees the smoother blocks of downsampled levels.
+            d_lv = [ctx.alloc(16 * w * h) for w, h in sizes]
+            d_lv[0].upload(synth.to_planar(np.ascontiguousarray(np.rot90(img, r))).reshape(-1))
+            ctx.downsample_chain_device(N.FILTER_KAISER, d_lv, 2048, 2048, 3.0, (4.0, 1.0))
+            for d, (w, h) in zip(d_lv, sizes):
+                texs.append((d.download(np.float32), w, h))
+                d.free()
     cut = sum(((w + 3) // 4) * ((h + 3) // 4) for _, w, h in texs)
     data = learned.make_dataset(ctx, texs, seed=args.seed)
     del texs
@@ -209,8 +218,8 @@ def train_network(ctx, args):
     steps = epochs * ((data.n + batch - 1) // batch)
     n = data.n
     data.free()
-    return net, {"textures": "8 seeded 2048^2 noise+rectangle textures (seeds %d-%d) x 4 rotations"
-                             % (args.seed + 2000, args.seed + 2007),
+    return net, {"textures": "8 seeded 2048^2 noise+rectangle textures (seeds %d-%d) x 4 rotations, each with "
+                             "its Kaiser(3; 4, 1) levels 1-3" % (args.seed + 2000, args.seed + 2007),
                  "blocks_cut": cut, "blocks_used": n, "epochs": epochs, "batch": batch, "adam_steps": steps,
                  "seconds": round(train_s, 3), "data_seconds": round(data_s, 2), "loss_before": round(first, 5),
                  "loss_per_epoch": [round(x, 5) for x in losses]}
@@ -684,6 +693,34 @@ def bench_c2(ctx, args) -> dict:
                             "mip_levels_bit_identical": bool(mips_equal)}
         ref = oracle_lib.Reference.load()
         if ref is not None:
+            # The workload through the reference's own code: Kaiser levels by
+            # FloatImage::downSample (single-threaded, as the reference runs
+            # it) and BC3 by CompressorDXT5 on the reference's task loop;
+            # the sRGB->linear step (Surface.cpp:1562-1592, a per-texel
+            # table, not built into oracle/_ref) from the port.
+            log(f"CPU reference: BC3 chain (reference sources), {threads} threads")
+            t0 = time.perf_counter()
+            x = synth.to_planar(img).reshape(-1)
+            cur = np.concatenate([orc.srgb_to_linear(x[: 3 * S * S]), x[3 * S * S:]])
+            t_srgb = time.perf_counter() - t0
+            ref_same = 0
+            t_rmip = 0.0
+            for k, (w, h) in enumerate(levels):
+                blk = ref.compress_image(N.FORMAT_BC3, N.QUALITY_NORMAL, cur, w, h, threads=threads)
+                ref_same += int(np.sum(np.all(blk.reshape(-1, 16) == gpu_bc3[k].reshape(-1, 16), axis=1)))
+                if k + 1 < len(levels):
+                    tm = time.perf_counter()
+                    cur = ref.build_next_mipmap(N.FILTER_KAISER, kw, kp, N.WRAP_MIRROR, cur, w, h)
+                    t_rmip += time.perf_counter() - tm
+            ref_bc3_s = time.perf_counter() - t0
+            c2["cpu_reference"] = {
+                "value": round(texels / ref_bc3_s / 1e6, 3), "unit": "Mtexels/s", "cores": threads,
+                "kind": "reference", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                "sample": f"the full workload ({texels} texels): the reference's CompressorDXT5 on {threads} threads "
+                          f"and its FloatImage::downSample Kaiser levels single-threaded ({t_rmip:.2f} s), sRGB->linear "
+                          f"by the port ({t_srgb:.2f} s), {ref_bc3_s:.2f} s in all",
+                "seconds": round(ref_bc3_s, 3), "bc3_identical_blocks_to_device": round(ref_same / nblk, 6),
+                "port_over_reference_time": round(cpu_s / ref_bc3_s, 3)}
             import ctypes
             log(f"CPU reference: icbc (reference sources) BC1 on the chain, {threads} threads")
             fn = ref.lib.ntxref_compress_bc1_image
@@ -714,7 +751,7 @@ def bench_c2(ctx, args) -> dict:
     return c2
 
 
-def bench_batch(ctx, args) -> dict:
+def bench_batch(ctx, args, cpu: bool = False) -> dict:
     """Batched uneven workload: n textures with power-of-two sizes drawn
     uniformly from {64..2048} x {64..2048}; half RGBA noise+rectangles -> BC1,
     a quarter normal maps (packed XY) -> BC5, a quarter single-channel noise
@@ -802,7 +839,48 @@ def bench_batch(ctx, args) -> dict:
         "speedup_vs_per_image_path": round(ms_seq / ms, 3),
         "levels_identical_to_per_image_path": f"{same}/{nlev}",
         "kernels_ms_per_run": {k: round(v[0] / steps, 4) for k, v in prof.items()},
+        **({"cpu_reference": cpu_reference_batch(args, texs, texels)} if cpu else {}),
     }
+
+
+def cpu_reference_batch(args, texs, texels_total) -> dict:
+    """The reference's own CPU code over a bounded sample of the batch: the
+    first textures of each format (8 BC1, 4 BC5, 4 BC4) in full, each level
+    by Surface::buildNextMipmap's Triangle filter (FloatImage::downSample,
+    single-threaded as in the reference) and the compressor Context.cpp picks
+    (CompressorDXT1 / FastCompressorBC5 / FastCompressorBC4) on
+    host_threads() workers, all compiled from the reference sources."""
+    import ntx_amd as N
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    ref = oracle_lib.Reference.load()
+    threads = host_threads(args.cpu_threads)
+    info = {"unit": "Mtexels/s", "cores": threads, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "kind": "reference"}
+    if ref is None:
+        return dict(info, value=None, sample="oracle/_ref not built")
+    want = {N.FORMAT_BC1: 8, N.FORMAT_BC5: 4, N.FORMAT_BC4: 4}
+    sample = []
+    for t in texs:
+        if want.get(t["format"], 0) > 0:
+            want[t["format"]] -= 1
+            sample.append(t)
+    host = [t["d_rgba"].download(np.float32) for t in sample]
+    log(f"CPU reference: batch sample of {len(sample)} textures on {threads} threads")
+    t0 = time.perf_counter()
+    tex = 0
+    for t, planar in zip(sample, host):
+        sz = N.mip_sizes(t["width"], t["height"])
+        cur = planar
+        for k, (w, h) in enumerate(sz):
+            ref.compress_image(t["format"], N.QUALITY_NORMAL, cur, w, h, threads=threads)
+            tex += w * h
+            if k + 1 < len(sz):
+                cur = ref.build_next_mipmap(N.FILTER_TRIANGLE, 1.0, None, N.WRAP_MIRROR, cur, w, h)
+    sec = time.perf_counter() - t0
+    return dict(info, value=round(tex / sec / 1e6, 3), seconds=round(sec, 3),
+                sample=f"{len(sample)} of the {len(texs)} textures ({tex} of {texels_total} texels): 8 BC1, 4 BC5, "
+                       f"4 BC4, full Triangle chains, through the reference's FloatImage::downSample and compressors")
 
 
 def _cube_face(face: int, S: int, seed: int) -> np.ndarray:
@@ -956,10 +1034,16 @@ def bench_c1(ctx, args, cpu: bool) -> dict:
     return res
 
 
-def bench_cube(ctx, args, net):
-    """Cube map: 6 faces of 4096^2, full 13-level Kaiser chains per face,
-    every level compressed to BC1 by the plain cluster fit and by the learned
-    proposal form; per-level PSNR (mean over faces) and Mtexels/s of both."""
+def bench_cube(ctx, args, net, cpu: bool):
+    """Cube map: 6 faces of 4096^2, full 13-level Kaiser chains per face
+    (ntx_hip_downsample_chain_device), then all 78 levels compressed to BC1
+    in one call each way: the plain cluster fit (ntx_hip_compress_images_device)
+    and the learned proposals (ntx_hip_compress_bc1_mlp_images_device with
+    learned.level_settings: the shipped settings for level 0, one least-squares
+    pass, one greedy round and the tau_gap check for downsampled levels, and
+    the plain fit on a side stream for levels under 4096 blocks).  Per-level
+    and overall PSNR, Mtexels/s of both, and the reference's icbc on the host
+    over one face's chain."""
     import ntx_amd as N
     from ntx_amd import learned, synth
 
@@ -977,41 +1061,96 @@ def bench_cube(ctx, args, net):
     ctx.synchronize()
     ctx.timer_begin()
     for chain in lv:
-        for k in range(len(levels) - 1):
-            w, h = levels[k]
-            ctx.downsample_device(N.FILTER_KAISER, chain[k], w, h, chain[k + 1], 3.0, (4.0, 1.0))
+        ctx.downsample_chain_device(N.FILTER_KAISER, chain, S, S, 3.0, (4.0, 1.0))
     ms_mips = ctx.timer_end()
-    outs = [[ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in levels] for _ in range(6)]
-    texels = 6 * sum(w * h for w, h in levels)
-
-    def run(learned_form):
-        for f in range(6):
-            for k, (w, h) in enumerate(levels):
-                if learned_form:
-                    net.compress_bc1_device(lv[f][k], w, h, outs[f][k], count_fallback=False, **learned.SHIPPED)
-                else:
-                    ctx.compress_device(N.FORMAT_BC1, lv[f][k], w, h, outs[f][k])
-
+    # Every level of every face, largest first (the scheduling order the
+    # multi-image entry points take).
+    items = sorted(((f, k) for f in range(6) for k in range(len(levels))), key=lambda fk: (fk[1], fk[0]))
+    ins = [lv[f][k] for f, k in items]
+    sizes = [levels[k] for _, k in items]
+    texels = sum(w * h for w, h in sizes)
     res = {}
-    for name, flag in (("plain", False), ("learned", True)):
-        run(flag)
+    outs = {}
+    # "learned": learned.level_settings per level (every level within 0.05 dB
+    # of the plain fit on held-out chains); "learned_throughput": the shipped
+    # settings on every level of 4096 blocks or more.
+    fast = [dict(learned.SHIPPED) if ((w + 3) // 4) * ((h + 3) // 4) >= learned.MIN_BLOCKS else None
+            for w, h in sizes]
+    for name in ("plain", "learned", "learned_throughput"):
+        out = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in sizes]
+        outs[name] = out
+
+        def run():
+            if name == "plain":
+                ctx.compress_images_device(N.FORMAT_BC1, ins, sizes, out)
+            elif name == "learned":
+                net.compress_bc1_images_device(ins, sizes, out, levels=[k for _, k in items])
+            else:
+                net.compress_bc1_images_device(ins, sizes, out, settings=fast)
+
+        run()
         ctx.synchronize()
+        reps = 3
         ctx.timer_begin()
-        run(flag)
-        ms = ctx.timer_end()
-        psnr = [float(np.mean([ctx.bc1_psnr_device(lv[f][k], w, h, outs[f][k]) for f in range(6)]))
-                for k, (w, h) in enumerate(levels)]
+        for _ in range(reps):
+            run()
+        ms = ctx.timer_end() / reps
+        sse = [ctx.bc1_sse_device(ins[i], w, h, out[i]) for i, (w, h) in enumerate(sizes)]
+        per_level = []
+        for k, (w, h) in enumerate(levels):
+            e = sum(sse[i] for i, (_, kk) in enumerate(items) if kk == k) / (6 * w * h)
+            per_level.append(round(10 * np.log10(255.0 ** 2 / max(e, 1e-12)), 3))
+        overall = 10 * np.log10(255.0 ** 2 / max(sum(sse) / texels, 1e-12))
         res[name] = {"ms": round(ms, 3), "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1),
-                     "psnr_db_per_level": [round(p, 3) for p in psnr]}
-        log(f"  cube {name}: {res[name]['mtexels_s']} Mtexels/s")
-    for chain in lv + outs:
+                     "psnr_db": round(float(overall), 3), "psnr_db_per_level": per_level}
+        log(f"  cube {name}: {res[name]['ms']} ms, {res[name]['mtexels_s']} Mtexels/s, {overall:.3f} dB")
+    out = {"workload": "6 faces x 4096^2 RGBA8 (seeded noise + direction gradients continuous across faces), "
+                       "13-level Kaiser(3; 4, 1) chains per face, all 78 levels to BC1 in one call: plain cluster fit "
+                       "and learned proposals (learned.level_settings: shipped settings at level 0, refined at "
+                       "levels 1-4, the plain fit below 4096 blocks; cluster-fit fallback above tau_pred)",
+           "texels": texels, "mip_chains_ms": round(ms_mips, 3), **res,
+           "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3),
+           "psnr_delta_db": round(res["learned"]["psnr_db"] - res["plain"]["psnr_db"], 3),
+           "worst_level_delta_db": round(min(a - b for a, b in zip(res["learned"]["psnr_db_per_level"],
+                                                                   res["plain"]["psnr_db_per_level"])), 3),
+           "throughput_speedup": round(res["plain"]["ms"] / res["learned_throughput"]["ms"], 3),
+           "throughput_worst_level_delta_db": round(min(a - b for a, b in zip(
+               res["learned_throughput"]["psnr_db_per_level"], res["plain"]["psnr_db_per_level"])), 3)}
+    if cpu:
+        out["cpu_reference"] = cpu_reference_chain_bc1(args, [d.download(np.float32) for d in lv[0]], levels,
+                                                       [outs["plain"][items.index((0, k))].download(np.uint8)
+                                                        for k in range(len(levels))], texels)
+    for chain in lv + list(outs.values()):
         for b in chain:
             b.free()
-    return {"workload": "6 faces x 4096^2 RGBA8 (seeded noise + direction gradients continuous across faces), "
-                        "13-level Kaiser(3; 4, 1) chains per face, BC1 per level: plain cluster fit and learned "
-                        "proposals at the shipped settings (cluster-fit fallback above tau_pred)",
-            "texels": texels, "mip_chains_ms": round(ms_mips, 3), **res,
-            "speedup": round(res["plain"]["ms"] / res["learned"]["ms"], 3)}
+    return out
+
+
+def cpu_reference_chain_bc1(args, host_levels, levels, gpu_blocks, texels_total) -> dict:
+    """The reference's icbc Level8 (CompressorDXT1, compiled from the
+    reference sources) over the levels of one chain on host_threads()
+    workers, with its blocks against the device's plain fit."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    ref = oracle_lib.Reference.load()
+    threads = host_threads(args.cpu_threads)
+    info = {"unit": "Mtexels/s", "cores": threads, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "kind": "reference"}
+    if ref is None:
+        return dict(info, value=None, sample="oracle/_ref not built")
+    t0 = time.perf_counter()
+    same = 0
+    nb = 0
+    for k, (w, h) in enumerate(levels):
+        blk = ref.compress_image(1, 1, host_levels[k], w, h, threads=threads)
+        same += int(np.sum(np.all(blk.reshape(-1, 8) == gpu_blocks[k].reshape(-1, 8), axis=1)))
+        nb += blk.size // 8
+    sec = time.perf_counter() - t0
+    tex = sum(w * h for w, h in levels)
+    return dict(info, value=round(tex / sec / 1e6, 3), seconds=round(sec, 3),
+                sample=f"one face's {len(levels)}-level chain ({tex} texels of the workload's {texels_total}) through "
+                       f"the reference's CompressorDXT1 on the same FP32 levels",
+                blocks_identical_to_device=round(same / nb, 6))
 
 
 def _decode_bc1(blk: np.ndarray, w: int, h: int) -> np.ndarray:

@@ -110,7 +110,7 @@ enum {
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 pass 1: alpha halves, single-colour blocks, bucketing by colour count */
     NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
-    NTX_KERNEL_BC1_LIST = 12,    /* reserved (the fallback blocks now run through BC1_CLASSIFY / BC1_SEARCH) */
+    NTX_KERNEL_BC1_LIST = 12,    /* BC1 Level8 over a block list (the learned form's fallback), one block per 16-lane group */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
     NTX_KERNEL_BC3_SEARCH_SPLIT = 14, /* BC3 colour search of a small image: 2..64 lanes per block */
     NTX_KERNEL_GAMMA = 15,       /* gamma transforms of R, G, B (one launch per up to 16 levels) */
@@ -465,6 +465,21 @@ NTX_API ntx_status ntx_hip_mlp_train_step(ntx_hip_mlp *mlp, const void *d_featur
 NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *mlp, const float *d_rgba, int w, int h, float tau_pred,
                                                    float tau_gap, int refine_iters, int greedy_rounds, int flags,
                                                    void *d_out, int *fallback_blocks);
+
+/* The learned form over n images (the levels of mip chains, cube faces) in
+ * one call, each image with its own settings: images whose settings set
+ * `plain` take the plain cluster fit (the bytes of ntx_hip_compress_device
+ * at BC1 Normal) concurrently on a side stream; the others go through
+ * ntx_hip_compress_bc1_mlp_device with their settings one after another on
+ * the context's stream.  No counterpart in the reference. */
+typedef struct {
+    float tau_pred, tau_gap;
+    int refine_iters, greedy_rounds, flags;
+    int plain;
+} ntx_mlp_settings;
+NTX_API ntx_status ntx_hip_compress_bc1_mlp_images_device(ntx_hip_mlp *mlp, int n, const float *const *d_rgba,
+                                                          const int *w, const int *h,
+                                                          const ntx_mlp_settings *settings, void *const *d_out);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,34 @@ __global__ __launch_bounds__(256) void bc1_kernel(const float *__restrict__ src,
     }
 }
 
+// The blocks listed in list[0, min(*count, cap)) (indices below cap), one
+// per 16-lane group as bc1_kernel encodes them, the groups striding over the
+// list.  The encoder synchronises only within a wavefront, so groups may
+// finish their strides at different times.  For short lists (the learned
+// form's fallback blocks) this has a group's latency, where the one-block-
+// per-lane search pass has a whole block's search per lane.
+template <int LEVEL>
+__global__ __launch_bounds__(256) void bc1_list_kernel(const float *__restrict__ src, int w, int h, int bw,
+                                                       const int *__restrict__ list, const int *__restrict__ count,
+                                                       int cap, int alpha_mode, float cwx, float cwy, float cwz,
+                                                       const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out)
+{
+    __shared__ Bc1Shared sh;
+    load_shared_tables(sh, tab, threadIdx.x, blockDim.x);
+    __syncthreads();
+    const int g = threadIdx.x >> 4;
+    const int lane = threadIdx.x & 15;
+    const int items = min(*count, cap);
+    for (int item = blockIdx.x * kBlocksPerWG + g; item < items; item += gridDim.x * kBlocksPerWG) {
+        const int blk = list[item];
+        if (unsigned(blk) >= unsigned(cap)) continue;  // the whole group skips it
+        int n, m;
+        const uint2 result =
+            encode_block<LEVEL>(sh, src, w, h, blk % bw, blk / bw, alpha_mode, cwx, cwy, cwz, tab, g, lane, &n, &m);
+        if (lane == 0) out[blk] = result;
+    }
+}
+
 // One item of bc1_classify_kernel per thread; the whole workgroup calls it.
 __device__ __forceinline__ void classify_chunk(const float *__restrict__ src, int w, int h, int bw,
                                                const int *__restrict__ sel, int items, int alpha_mode,
@@ -274,5 +302,20 @@ hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, c
 }
 
 int bc1_scratch_ints(int nblocks) { return ntx_bucket::kListsOffset + ntx_bucket::kCountSlots * nblocks; }
+
+// icbc Level8 of the *d_count blocks listed in d_list (at most cap, each
+// below cap, cap = the image's block count), at most max_groups workgroups.
+hipError_t launch_bc1_list(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
+                           void *d_out, const int *d_list, const int *d_count, int cap, int max_groups,
+                           hipStream_t stream)
+{
+    if (cap == 0) return hipSuccess;
+    const int bw = (w + 3) / 4;
+    int grid = (cap + kBlocksPerWG - 1) / kBlocksPerWG;
+    if (max_groups > 0 && grid > max_groups) grid = max_groups;
+    hipLaunchKernelGGL(bc1_list_kernel<8>, dim3(grid), dim3(kBlocksPerWG * kGroup), 0, stream, d_src, w, h, bw, d_list,
+                       d_count, cap, alpha_mode, cw[0], cw[1], cw[2], d_tab, reinterpret_cast<uint2 *>(d_out));
+    return hipGetLastError();
+}
 
 }  // namespace ntx

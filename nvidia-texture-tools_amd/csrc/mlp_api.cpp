@@ -210,6 +210,43 @@ NTX_API ntx_status ntx_hip_mlp_train_step(ntx_hip_mlp *m, const void *d_features
     return NTX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// One image through the learned form on `stream`: the network's proposals,
+// then the blocks it sends to the fallback through the cluster fit.  d_count
+// (one int, then padding to 64) and d_list (the image's block count) are the
+// fallback count and list.
+ntx_status mlp_compress_on(ntx_hip_mlp *m, hipStream_t stream, int *d_count, int *d_list, const float *d_rgba, int w,
+                           int h, float tau_pred, float tau_gap, int refine_iters, int greedy_rounds, int flags,
+                           void *d_out)
+{
+    ntx_hip_context *ctx = m->ctx;
+    const int n = ((w + 3) / 4) * ((h + 3) / 4);
+    NTX_HIP(ctx, hipMemsetAsync(d_count, 0, sizeof(int), stream));
+    const int cus = resident_cus(ctx);
+    hipError_t e = ntx::timed_on(ctx, stream, NTX_KERNEL_BC1_MLP, [&] {
+        return ntx::launch_bc1_mlp(m->d_params, d_rgba, w, h, ctx->d_tab, tau_pred, tau_gap, refine_iters, greedy_rounds,
+                                   (flags & NTX_MLP_BOX_CANDIDATES) ? 1 : 0, d_out, d_count, d_list, nullptr, cus,
+                                   stream);
+    });
+    // The fallback blocks through the cluster fit, one block per 16-lane
+    // group over the list (the bytes of every other BC1 Level8 encoder).
+    const float ones[3] = { 1.0f, 1.0f, 1.0f };
+    if (e == hipSuccess)
+        e = ntx::timed_on(ctx, stream, NTX_KERNEL_BC1_LIST, [&] {
+            return ntx::launch_bc1_list(d_rgba, w, h, NTX_ALPHA_NONE, ones, ctx->d_tab, d_out, d_list, d_count, n,
+                                        8 * ctx->cus, stream);
+        });
+    if (e != hipSuccess) return ntx::fail(ctx, e, "bc1 proposal launch");
+    return NTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *m, const float *d_rgba, int w, int h, float tau_pred,
                                                    float tau_gap, int refine_iters, int greedy_rounds, int flags,
                                                    void *d_out, int *fallback_blocks)
@@ -224,44 +261,83 @@ NTX_API ntx_status ntx_hip_compress_bc1_mlp_device(ntx_hip_mlp *m, const float *
     if (!d_rgba || !d_out) return NTX_ERR_INVALID_ARGUMENT;
     // fallback count, then the list of up to n block indices
     if ((s = ntx::ensure(ctx, m->fb, sizeof(int) * (size_t(n) + 64)))) return s;
-    int *d_count = static_cast<int *>(m->fb.ptr), *d_list = d_count + 64;
-    NTX_HIP(ctx, hipMemsetAsync(d_count, 0, sizeof(int), ctx->stream));
-    const int cus = resident_cus(ctx);
-    hipError_t e = ntx::timed(ctx, NTX_KERNEL_BC1_MLP, [&] {
-        return ntx::launch_bc1_mlp(m->d_params, d_rgba, w, h, ctx->d_tab, tau_pred, tau_gap, refine_iters, greedy_rounds,
-                                   (flags & NTX_MLP_BOX_CANDIDATES) ? 1 : 0, d_out, d_count, d_list, nullptr, cus,
-                                   ctx->stream);
-    });
-    // The fallback blocks through the two-pass cluster fit (colour counts of
-    // the listed blocks, then the search over their buckets).
-    int *counts = nullptr;
-    if (e == hipSuccess) {
-        const size_t cap_before = ctx->bucket.cap;
-        if ((s = ntx::ensure(ctx, ctx->bucket, sizeof(int) * size_t(ntx::bc1_scratch_ints(n))))) return s;
-        if (ctx->bucket.cap != cap_before) ctx->bc3_counts_zero = false;  // fresh memory
-        counts = static_cast<int *>(ctx->bucket.ptr);
-        if (!ctx->bc3_counts_zero) e = ntx::launch_bc3_clear_counts(counts, ctx->stream);
-        ctx->bc3_counts_zero = false;
-    }
-    const float ones[3] = { 1.0f, 1.0f, 1.0f };
-    if (e == hipSuccess)
-        e = ntx::timed(ctx, NTX_KERNEL_BC1_CLASSIFY, [&] {
-            return ntx::launch_bc1_classify(d_rgba, w, h, NTX_ALPHA_NONE, ctx->d_tab, d_out, nullptr, d_list, d_count,
-                                            counts, counts + ntx::bc3_lists_offset(), n, 4 * ctx->cus, ctx->stream,
-                                            false);
-        });
-    if (e == hipSuccess)
-        e = ntx::timed(ctx, NTX_KERNEL_BC1_SEARCH, [&] {
-            return ntx::launch_bc1_search(d_rgba, w, h, NTX_ALPHA_NONE, ones, ctx->d_tab, d_out, nullptr, counts,
-                                          counts + ntx::bc3_lists_offset(), n, ntx::kBc1SearchGroupsPerCU * ctx->cus,
-                                          ctx->stream);
-        });
-    if (e == hipSuccess) ctx->bc3_counts_zero = true;
-    if (e != hipSuccess) return ntx::fail(ctx, e, "bc1 proposal launch");
+    int *d_count = static_cast<int *>(m->fb.ptr);
+    if ((s = mlp_compress_on(m, ctx->stream, d_count, d_count + 64, d_rgba, w, h, tau_pred, tau_gap, refine_iters,
+                             greedy_rounds, flags, d_out)))
+        return s;
     if (fallback_blocks) {
-        NTX_HIP(ctx, hipMemcpyAsync(fallback_blocks, d_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        NTX_HIP(ctx, hipMemcpyAsync(fallback_blocks, m->fb.ptr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    return NTX_OK;
+}
+
+NTX_API ntx_status ntx_hip_compress_bc1_mlp_images_device(ntx_hip_mlp *m, int n, const float *const *d_rgba,
+                                                          const int *w, const int *h,
+                                                          const ntx_mlp_settings *settings, void *const *d_out)
+{
+    if (!m || n < 0) return NTX_ERR_INVALID_ARGUMENT;
+    if (n > 0 && (!d_rgba || !w || !h || !settings || !d_out)) return NTX_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < n; k++) {
+        if (w[k] < 1 || h[k] < 1 || !d_rgba[k] || !d_out[k]) return NTX_ERR_INVALID_ARGUMENT;
+        if (!settings[k].plain && (settings[k].refine_iters < 0 || settings[k].greedy_rounds < 0))
+            return NTX_ERR_INVALID_ARGUMENT;
+    }
+    ntx_hip_context *ctx = m->ctx;
+    ntx_status s = ntx::bind(ctx);
+    if (s) return s;
+    std::vector<const float *> plain_in;
+    std::vector<void *> plain_out;
+    std::vector<int> plain_w, plain_h, learned;
+    for (int k = 0; k < n; k++) {
+        if (settings[k].plain) {
+            plain_in.push_back(d_rgba[k]);
+            plain_out.push_back(d_out[k]);
+            plain_w.push_back(w[k]);
+            plain_h.push_back(h[k]);
+        } else {
+            learned.push_back(k);
+        }
+    }
+    // Each learned image's fallback count and list in its own region of
+    // m->fb, so that the images can run on two streams at once.
+    std::vector<size_t> region(learned.size());
+    size_t total = 0;
+    for (size_t i = 0; i < learned.size(); i++) {
+        const int k = learned[i];
+        region[i] = total;
+        total += 64 + ((size_t((w[k] + 3) / 4) * size_t((h[k] + 3) / 4) + 63) & ~size_t(63));
+    }
+    if (total && (s = ntx::ensure(ctx, m->fb, sizeof(int) * total))) return s;
+    hipEvent_t join = nullptr, side_done = nullptr;
+    if ((s = ntx::side_plain_bc1(ctx, int(plain_in.size()), plain_in.data(), plain_w.data(), plain_h.data(),
+                                 plain_out.data(), &join)))
+        return s;
+    // Images of 2^17 blocks or more fill the device on the context's stream;
+    // the smaller learned ones go one after another on side stream 1 beside
+    // them, starting once the work already on the context's stream is done.
+    hipStream_t side = nullptr;
+    for (size_t i = 0; i < learned.size(); i++) {
+        const int k = learned[i];
+        const bool large = size_t((w[k] + 3) / 4) * size_t((h[k] + 3) / 4) >= (size_t(1) << 17);
+        hipStream_t st = ctx->stream;
+        if (!large) {
+            if (!side) {
+                if ((s = ntx::side_stream_after(ctx, 1, &side))) return s;
+            }
+            st = side;
+        }
+        int *d_count = static_cast<int *>(m->fb.ptr) + region[i];
+        const ntx_mlp_settings &c = settings[k];
+        if ((s = mlp_compress_on(m, st, d_count, d_count + 64, d_rgba[k], w[k], h[k], c.tau_pred, c.tau_gap,
+                                 c.refine_iters, c.greedy_rounds, c.flags, d_out[k])))
+            return s;
+    }
+    if (side) {
+        if ((s = ntx::side_stream_done(ctx, 1, &side_done))) return s;
+        NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, side_done, 0));
+    }
+    if (join) NTX_HIP(ctx, hipStreamWaitEvent(ctx->stream, join, 0));
     return NTX_OK;
 }
 

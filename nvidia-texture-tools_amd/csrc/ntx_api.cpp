@@ -768,6 +768,56 @@ static ntx_status compress_many(ntx_hip_context *ctx, int format, int quality, i
     return NTX_OK;
 }
 
+// Side stream i, made to wait for the work already on the context's stream.
+ntx_status side_stream_after(ntx_hip_context *ctx, int i, hipStream_t *out)
+{
+    for (int k = 0; k < ntx_hip_context::kSideStreams; k++)
+        if (!ctx->side[k]) NTX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+    hipEvent_t ready = sync_event(ctx, 5 + 2 * size_t(i));
+    if (!ready) return fail(ctx, hipErrorOutOfMemory, "schedule events");
+    NTX_HIP(ctx, hipEventRecord(ready, ctx->stream));
+    NTX_HIP(ctx, hipStreamWaitEvent(ctx->side[i], ready, 0));
+    *out = ctx->side[i];
+    return NTX_OK;
+}
+
+// An event recorded on side stream i after the work enqueued on it.
+ntx_status side_stream_done(ntx_hip_context *ctx, int i, hipEvent_t *out)
+{
+    hipEvent_t done = sync_event(ctx, 6 + 2 * size_t(i));
+    if (!done) return fail(ctx, hipErrorOutOfMemory, "schedule events");
+    NTX_HIP(ctx, hipEventRecord(done, ctx->side[i]));
+    *out = done;
+    return NTX_OK;
+}
+
+// BC1 images compressed by the one-pass cluster fit on side stream 0 with
+// raised wave priority, after the work already on the context's stream; the
+// context's stream goes on at once, and the caller makes it wait for *join
+// (null when n is 0).  Used for the small levels beside the learned form.
+ntx_status side_plain_bc1(ntx_hip_context *ctx, int n, const float *const *d_in, const int *w, const int *h,
+                          void *const *d_out, hipEvent_t *join)
+{
+    *join = nullptr;
+    if (n == 0) return NTX_OK;
+    for (int i = 0; i < ntx_hip_context::kSideStreams; i++)
+        if (!ctx->side[i]) NTX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side[i], hipStreamNonBlocking));
+    hipEvent_t ready = sync_event(ctx, 3), done = sync_event(ctx, 4);
+    if (!ready || !done) return fail(ctx, hipErrorOutOfMemory, "schedule events");
+    NTX_HIP(ctx, hipEventRecord(ready, ctx->stream));
+    NTX_HIP(ctx, hipStreamWaitEvent(ctx->side[0], ready, 0));
+    const float ones[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
+    for (int k = 0; k < n; k++) {
+        bool z = true;
+        const hipError_t e = compress_on(ctx, ctx->side[0], NTX_FORMAT_BC1, NTX_QUALITY_NORMAL, NTX_ALPHA_NONE, ones,
+                                         d_in[k], w[k], h[k], d_out[k], nullptr, &z, 16, true);
+        if (e != hipSuccess) return fail(ctx, e, "compress launch");
+    }
+    NTX_HIP(ctx, hipEventRecord(done, ctx->side[0]));
+    *join = done;
+    return NTX_OK;
+}
+
 }  // namespace ntx
 
 extern "C" {

@@ -18,6 +18,9 @@ hipError_t launch_bc1(const float *d_src, int w, int h, int alpha_mode, const fl
 hipError_t launch_bc1_classify(const float *d_src, int w, int h, int alpha_mode, const Tables *d_tab, void *d_out,
                                int16_t *d_nm, const int *d_sel, const int *d_sel_count, int *d_counts, int *d_lists,
                                int cap, int max_groups, hipStream_t stream, bool urgent);
+hipError_t launch_bc1_list(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
+                           void *d_out, const int *d_list, const int *d_count, int cap, int max_groups,
+                           hipStream_t stream);
 hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
                              void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, int max_groups,
                              hipStream_t stream);
@@ -163,6 +166,10 @@ size_t scratch_ints(const ntx_hip_context *ctx, int format, int quality, int w, 
 // One image's compression on the context's stream with the given bucket
 // scratch (null: BC1 takes its one-pass encoder); the tallies are cleared by
 // a fill kernel first.
+ntx_status side_stream_after(ntx_hip_context *ctx, int i, hipStream_t *out);
+ntx_status side_stream_done(ntx_hip_context *ctx, int i, hipEvent_t *out);
+ntx_status side_plain_bc1(ntx_hip_context *ctx, int n, const float *const *d_in, const int *w, const int *h,
+                          void *const *d_out, hipEvent_t *join);
 hipError_t compress_scratch(ntx_hip_context *ctx, int format, int quality, int alpha_mode, const float *cw,
                             const float *d_rgba, int w, int h, void *d_out, int *bucket);
 // Looks up the device address of the no-allocation entry point's scratch

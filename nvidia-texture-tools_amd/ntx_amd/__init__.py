@@ -59,11 +59,17 @@ EXPORTS = (
     "ntx_hip_batch_autotune",
     "ntx_hip_mlp_create", "ntx_hip_mlp_destroy", "ntx_hip_mlp_get_params", "ntx_hip_mlp_set_params",
     "ntx_hip_mlp_features_device", "ntx_hip_mlp_labels_device", "ntx_hip_mlp_forward_device", "ntx_hip_mlp_train_step",
-    "ntx_hip_compress_bc1_mlp_device", "ntx_hip_bc1_sse_device",
+    "ntx_hip_compress_bc1_mlp_device", "ntx_hip_compress_bc1_mlp_images_device", "ntx_hip_bc1_sse_device",
 )
 
 MLP_PARAMS = 8921          # NTX_MLP_PARAMS
 MLP_BOX_CANDIDATES = 1     # NTX_MLP_BOX_CANDIDATES
+
+
+class MlpSettings(ctypes.Structure):
+    """ntx_mlp_settings: one image's settings for ntx_hip_compress_bc1_mlp_images_device."""
+    _fields_ = [("tau_pred", ctypes.c_float), ("tau_gap", ctypes.c_float), ("refine_iters", ctypes.c_int),
+                ("greedy_rounds", ctypes.c_int), ("flags", ctypes.c_int), ("plain", ctypes.c_int)]
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -165,6 +171,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntx_hip_mlp_forward_device": (_I, [_P, _P, _I, _P]),
         "ntx_hip_mlp_train_step": (_I, [_P, _P, _P, _I, _F, ctypes.POINTER(_F), _P]),
         "ntx_hip_compress_bc1_mlp_device": (_I, [_P, _P, _I, _I, _F, _F, _I, _I, _I, _P, ctypes.POINTER(_I)]),
+        "ntx_hip_compress_bc1_mlp_images_device": (_I, [_P, _I, _P, _P, _P, _P, _P]),
         "ntx_hip_bc1_sse_device": (_I, [_P, _P, _I, _I, _P, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
@@ -493,6 +500,17 @@ class Context:
         self._check(self.lib.ntx_hip_normalize_normals_device(self.handle, d_rgba.ptr, w, h),
                     "ntx_hip_normalize_normals_device")
 
+    def bc1_sse_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
+        """Sum over texels of the squared R, G, B differences (8-bit units) of a
+        BC1 level against its FP32 source (ntx_hip_bc1_sse_device)."""
+        _dims(w, h, "bc1_sse_device")
+        _need(d_rgba, _planar(w, h), "bc1_sse_device image")
+        _need(d_blocks, compressed_size(FORMAT_BC1, w, h), "bc1_sse_device blocks")
+        sse = ctypes.c_double()
+        self._check(self.lib.ntx_hip_bc1_sse_device(self.handle, d_rgba.ptr, w, h, d_blocks.ptr, ctypes.byref(sse)),
+                    "bc1_sse")
+        return float(sse.value)
+
     def bc1_psnr_device(self, d_rgba: DeviceBuffer, w: int, h: int, d_blocks: DeviceBuffer) -> float:
         """PSNR (dB) of a BC1 level against its FP32 source, computed on the device."""
         _dims(w, h, "bc1_psnr_device")
@@ -696,6 +714,47 @@ class Mlp:
                                                           d_out.ptr, ctypes.byref(fb) if count_fallback else None)
         self.ctx._check(st, "compress_bc1_mlp")
         return fb.value if count_fallback else None
+
+    def compress_bc1_images_device(self, d_rgba: Sequence[DeviceBuffer], sizes: Sequence[Tuple[int, int]],
+                                   d_out: Sequence[DeviceBuffer], settings: Optional[Sequence[Optional[dict]]] = None,
+                                   levels: Optional[Sequence[int]] = None) -> None:
+        """The learned form over many images (ntx_hip_compress_bc1_mlp_images_device).
+        settings[k]: keyword settings of compress_bc1_device for image k, or
+        None for the plain cluster fit; by default learned.level_settings of
+        its size and its mip level (levels[k], 0 when not given)."""
+        from . import learned
+        n = len(d_rgba)
+        if len(sizes) != n or len(d_out) != n:
+            raise ValueError("one size and one output buffer per image")
+        if settings is None:
+            lv = levels if levels is not None else [0] * n
+            if len(lv) != n:
+                raise ValueError("one level per image")
+            settings = [learned.level_settings(a, b, int(k)) for (a, b), k in zip(sizes, lv)]
+        if len(settings) != n:
+            raise ValueError("one settings entry per image")
+        for k, (a, b) in enumerate(sizes):
+            _dims(a, b, "compress_bc1_images_device")
+            _need(d_rgba[k], _planar(a, b), "compress_bc1_images_device input")
+            _need(d_out[k], compressed_size(FORMAT_BC1, a, b), "compress_bc1_images_device output")
+        cs = (MlpSettings * max(n, 1))()
+        for k, c in enumerate(settings):
+            if c is None:
+                cs[k].plain = 1
+                continue
+            cs[k].tau_pred = float(c.get("tau_pred", 4.0))
+            cs[k].tau_gap = float(c.get("tau_gap", -1.0))
+            cs[k].refine_iters = int(c.get("refine_iters", 2))
+            cs[k].greedy_rounds = int(c.get("greedy_rounds", 2))
+            cs[k].flags = MLP_BOX_CANDIDATES if c.get("box_candidates", False) else 0
+            if cs[k].refine_iters < 0 or cs[k].greedy_rounds < 0:
+                raise ValueError("compress_bc1_images_device: negative refine_iters or greedy_rounds")
+        ins = (ctypes.c_void_p * max(n, 1))(*[d.ptr for d in d_rgba])
+        outs = (ctypes.c_void_p * max(n, 1))(*[d.ptr for d in d_out])
+        ws = (ctypes.c_int * max(n, 1))(*[a for a, _ in sizes])
+        hs = (ctypes.c_int * max(n, 1))(*[b for _, b in sizes])
+        st = self.ctx.lib.ntx_hip_compress_bc1_mlp_images_device(self.handle, n, ins, ws, hs, cs, outs)
+        self.ctx._check(st, "compress_bc1_mlp_images")
 
 
 def mlp_features_device(ctx: Context, d_rgba: DeviceBuffer, w: int, h: int, d_features: DeviceBuffer,

@@ -32,6 +32,29 @@ LABEL_BYTES = 8 * 4
 #  * no greedy round, tau_gap off.
 SHIPPED = dict(tau_pred=8.0, tau_gap=-1.0, refine_iters=0, greedy_rounds=0)
 
+# Downsampled mip levels are smoother than the textures the proposals do
+# best on: there the shipped form loses to the cluster fit (on a held-out
+# 4096^2 noise+rectangle chain, -0.4 dB at level 1 and -0.75 dB at level 2).
+# One least-squares pass, one greedy 5:6:5 round and the tau_gap check (a
+# block whose achieved RMSE exceeds the predicted cluster-fit RMSE by 1.0
+# goes to the fit) keep levels 1-5 of that chain and of a cube face's chain
+# within 0.04 dB of the fit at 0.4-0.45 of the plain fit's time (without
+# the gap check two levels fall 0.05-0.11 dB short; with a second
+# least-squares pass instead, the same quality, slower on the noise chain).
+# Below 4096 blocks (256 x 256 texels) a level's time is launch latency, and
+# the plain fit is used.
+REFINED = dict(tau_pred=8.0, tau_gap=1.0, refine_iters=1, greedy_rounds=1)
+MIN_BLOCKS = 1 << 12
+
+
+def level_settings(w: int, h: int, level: int = 0) -> Optional[dict]:
+    """The proposal settings for mip level `level` (w x h) of a chain:
+    SHIPPED for level 0, REFINED for downsampled levels, None (the plain
+    cluster fit) for images under MIN_BLOCKS blocks."""
+    if ((w + 3) // 4) * ((h + 3) // 4) < MIN_BLOCKS:
+        return None
+    return dict(SHIPPED) if level == 0 else dict(REFINED)
+
 
 def decode_bc1(blocks: np.ndarray, w: int, h: int) -> np.ndarray:
     """D3D10 BC1 decode (BlockDXT.cpp:43-98) of a w x h level into (3, h, w) uint8."""

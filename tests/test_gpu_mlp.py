@@ -251,3 +251,37 @@ def test_shipped_settings_contract(gpu):
     sb, pb = shipped.reshape(-1, 8), plain.reshape(-1, 8)
     assert np.array_equal(sb[above], pb[above])
     assert int(above.sum()) <= fb <= int(above.sum() + near.sum())
+
+
+def test_images_entry_matches_per_image_calls(gpu):
+    """ntx_hip_compress_bc1_mlp_images_device on a whole mip chain with the
+    per-level policy (learned.level_settings) and with explicit settings:
+    every level equals the single-image learned call with its settings, or
+    compress_device where the settings ask for the plain fit, byte for byte."""
+    w, h = 1024, 512
+    sizes = N.mip_sizes(w, h)
+    d_lv = [gpu.alloc(16 * a * b) for a, b in sizes]
+    d_lv[0].upload(_texture(w, h, 21))
+    gpu.downsample_chain_device(N.FILTER_KAISER, d_lv, w, h, 3.0, (4.0, 1.0))
+    policy = [learned.level_settings(a, b, k) for k, (a, b) in enumerate(sizes)]
+    assert policy[0] == learned.SHIPPED and policy[1] == learned.REFINED and policy[-1] is None
+    explicit = [dict(tau_pred=6.0, tau_gap=1.0, refine_iters=2, greedy_rounds=0), None,
+                dict(learned.SHIPPED), None] + [None] * (len(sizes) - 4)
+    with N.Mlp(gpu, params=_random_params(5)) as net:
+        for settings in (None, explicit):
+            outs = [gpu.alloc(N.compressed_size(N.FORMAT_BC1, a, b)) for a, b in sizes]
+            net.compress_bc1_images_device(d_lv, sizes, outs, settings=settings, levels=list(range(len(sizes))))
+            got = [o.download(np.uint8) for o in outs]
+            used = policy if settings is None else settings
+            for k, (a, b) in enumerate(sizes):
+                ref = gpu.alloc(N.compressed_size(N.FORMAT_BC1, a, b))
+                if used[k] is not None:
+                    net.compress_bc1_device(d_lv[k], a, b, ref, count_fallback=False, **used[k])
+                else:
+                    gpu.compress_device(N.FORMAT_BC1, d_lv[k], a, b, ref)
+                assert np.array_equal(got[k], ref.download(np.uint8)), (a, b, used[k])
+                ref.free()
+            for d in outs:
+                d.free()
+    for d in d_lv:
+        d.free()
