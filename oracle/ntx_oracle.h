@@ -12,7 +12,11 @@
  *   - BC1: icbc v1.05 at Quality_Level8 with ICBC_SIMD=2 (SSE4.1): no FMA,
  *     round-half-even, exact division, 4-lane tie-break.
  *   - BC3 colour: nvsquish WeightedClusterFit with SQUISH_USE_SSE=0 (scalar).
- *   - BC4/BC5 and BC3 alpha: QuickCompress::compressDXT5A.
+ *   - BC4/BC5 and BC3 alpha: QuickCompress::compressDXT5A (Fastest, Normal)
+ *     and OptimalCompress::compressDXT5A with its first error seed taken
+ *     from the (max, min) endpoints (Production, Highest).
+ *   - BC3/BC3n Fastest: QuickCompress::compressDXT5; BC3n Highest colour:
+ *     OptimalCompress::compressDXT1G; BC1a: CompressorDXT1a.
  *   - Mips: FloatImage::fastDownSample and the separable polyphase resize.
  * Compile with -ffp-contract=off and without -ffast-math.
  */
