@@ -26,25 +26,22 @@ using namespace ntx_mlp;
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ uint16_t bf16_bits(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ __forceinline__ float bf16_round(float x) { return __uint_as_float(uint32_t(bf16_bits(x)) << 16); }
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32, two values per
+// instruction).
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, __bf16(x)); }
+__device__ __forceinline__ float bf16_round(float x) { return float(__bf16(x)); }
 __device__ __forceinline__ float bf16_float(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
 __device__ __forceinline__ v4s pack4(float a, float b, float c, float d)
 {
-    v4s r;
-    r[0] = short(bf16_bits(a));
-    r[1] = short(bf16_bits(b));
-    r[2] = short(bf16_bits(c));
-    r[3] = short(bf16_bits(d));
-    return r;
+    const v2f lo = { a, b }, hi = { c, d };
+    const v2u r = { __builtin_bit_cast(unsigned, __builtin_convertvector(lo, v2bf)),
+                    __builtin_bit_cast(unsigned, __builtin_convertvector(hi, v2bf)) };
+    return __builtin_bit_cast(v4s, r);
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
@@ -120,9 +117,11 @@ __device__ __forceinline__ void forward(const NetT<F> &net, const v4s bx[3], int
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const float v = acc[j] + net.b1[16 * m + 4 * g + j];
-            h1[m][j] = bf16_round(v > 0.0f ? v : 0.0f);
+            acc[j] = v > 0.0f ? v : 0.0f;
         }
-        hb[m] = pack4(h1[m][0], h1[m][1], h1[m][2], h1[m][3]);
+        hb[m] = pack4(acc[0], acc[1], acc[2], acc[3]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) h1[m][j] = bf16_float(uint16_t(hb[m][j]));
     }
     v4s hb2[4];
 #pragma unroll
@@ -133,9 +132,11 @@ __device__ __forceinline__ void forward(const NetT<F> &net, const v4s bx[3], int
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const float v = acc[j] + net.b2[16 * m + 4 * g + j];
-            h2[m][j] = bf16_round(v > 0.0f ? v : 0.0f);
+            acc[j] = v > 0.0f ? v : 0.0f;
         }
-        hb2[m] = pack4(h2[m][0], h2[m][1], h2[m][2], h2[m][3]);
+        hb2[m] = pack4(acc[0], acc[1], acc[2], acc[3]);
+#pragma unroll
+        for (int j = 0; j < 4; j++) h2[m][j] = bf16_float(uint16_t(hb2[m][j]));
     }
 #pragma unroll
     for (int m = 0; m < 2; m++) {
@@ -649,122 +650,97 @@ __device__ __forceinline__ void refine(const float4 *t4, float a[3], float b[3],
     }
 }
 
-// Error (squared, 8-bit units, evaluate_mse's measure up to rounding) and
-// index bits of the block with 565 endpoints c0, c1: four-colour order when
-// c0 > c1 (the caller orders them), each texel's nearest palette entry.  The
-// distances are taken in [0, 1] units with fused multiply-adds (this search
-// is not the bit-exact path) and scaled once at the end; a texel outside the
-// image (weight 0) adds nothing.
-__device__ __forceinline__ float eval_pair(const float4 *t4, uint32_t c0, uint32_t c1, uint32_t *bits_out)
+// The block's four palette entries for 565 endpoints c0, c1 (four-colour
+// order when c0 > c1, the caller orders them; the D3D10 integer palette) as
+// 8-bit RGB pk[i] = R | G << 8 | B << 16, and the keys
+// nk[i] = 3 - i - 4 |p_i|^2 of eval_pair8.
+__device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, uint32_t pk[4], int nk[4])
 {
-    // The D3D10 integer palette, as [0, 1] floats.
-    int q[4][3];
-    ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
-    ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
-    float p[4][3];
+    int e[2][3];
+    ntx_bc1::expand565(c0, e[0][0], e[0][1], e[0][2]);
+    ntx_bc1::expand565(c1, e[1][0], e[1][1], e[1][2]);
+    uint32_t q[4][3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        if (c0 > c1) {
-            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
-            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
-        } else {
-            q[2][k] = (q[0][k] + q[1][k]) / 2;
-            q[3][k] = 0;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) p[i][k] = float(q[i][k]) * (1.0f / 255.0f);
+        const uint32_t a = uint32_t(e[0][k]), b = uint32_t(e[1][k]);
+        q[0][k] = a;
+        q[1][k] = b;
+        // x / 3 = (x * 43691) >> 17 for every x <= 765 (24-bit multiply)
+        q[2][k] = c0 > c1 ? __umul24(2 * a + b, 43691u) >> 17 : (a + b) >> 1;
+        q[3][k] = c0 > c1 ? __umul24(2 * b + a, 43691u) >> 17 : 0u;
     }
-    float err = 0.0f;
-    uint32_t bits = 0;
-#pragma unroll 4
-    for (int t = 0; t < 16; t++) {
-        const float4 v = t4[t];
-        float best = 0.0f;
-        uint32_t bi = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float dr = p[i][0] - v.x, dg = p[i][1] - v.y, db = p[i][2] - v.z;
-            const float e = fmaf(dr, dr, fmaf(dg, dg, db * db));
-            if (i == 0 || e < best) { best = e; bi = uint32_t(i); }
-        }
-        err = fmaf(best, v.w, err);
-        bits |= bi << (2 * t);
-    }
-    if (bits_out) *bits_out = bits;
-    return err * 65025.0f;
-}
-
-// Quantises (a, b) as icbc does (vector3_to_color16), orders the endpoints
-// for four colours and returns the block and its error.
-__device__ __forceinline__ float encode_pair(const float4 *t4, const float a[3], const float b[3], const float *mid5,
-                                             const float *mid6, uint2 *blk)
-{
-    uint32_t c0 = ntx_bc1::to565(a[0], a[1], a[2], mid5, mid6);
-    uint32_t c1 = ntx_bc1::to565(b[0], b[1], b[2], mid5, mid6);
-    if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; }
-    uint32_t bits;
-    const float err = eval_pair(t4, c0, c1, &bits);
-    blk->x = c0 | (c1 << 16);
-    blk->y = bits;
-    return err;
-}
-
-// The same evaluation on the block's texels held as 8-bit RGB
-// (R | G << 8 | B << 16, each channel round(255 * saturate(v))), `inside` bit
-// t set for a texel inside the image: nearest palette entry by
-// |p|^2 - 2 p.t (one v_dot4_u32_u8 per entry; the first entry on ties) and
-// the exact integer squared error in 8-bit units.  For 8-bit sources (every
-// value k / 255) this is the error above without rounding.
-__device__ __forceinline__ float eval_pair8(const uint32_t t8[16], uint32_t inside, uint32_t c0, uint32_t c1,
-                                            uint32_t *bits_out)
-{
-    int q[4][3];
-    ntx_bc1::expand565(c0, q[0][0], q[0][1], q[0][2]);
-    ntx_bc1::expand565(c1, q[1][0], q[1][1], q[1][2]);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        if (c0 > c1) {
-            q[2][k] = (2 * q[0][k] + q[1][k]) / 3;
-            q[3][k] = (2 * q[1][k] + q[0][k]) / 3;
-        } else {
-            q[2][k] = (q[0][k] + q[1][k]) / 2;
-            q[3][k] = 0;
-        }
-    }
-    uint32_t pk[4];
-    int n2[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        pk[i] = uint32_t(q[i][0]) | uint32_t(q[i][1]) << 8 | uint32_t(q[i][2]) << 16;
-        n2[i] = q[i][0] * q[i][0] + q[i][1] * q[i][1] + q[i][2] * q[i][2];
+        pk[i] = q[i][0] | q[i][1] << 8 | q[i][2] << 16;
+        nk[i] = 3 - i - 4 * int(__builtin_amdgcn_udot4(pk[i], pk[i], 0u, false));
     }
-    int err = 0;
-    uint32_t bits = 0;
+}
+
+// Sum of |t|^2 over the block's texels, t8[t] = R | G << 8 | B << 16 (zero
+// for a texel outside the image): the part of every candidate's error that
+// does not depend on the endpoints.
+__device__ __forceinline__ int texel_norms(const uint32_t t8[16])
+{
+    uint32_t v2 = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) v2 = __builtin_amdgcn_udot4(t8[t], t8[t], v2, false);
+    return int(v2);
+}
+
+// Nearest palette entry of each texel and the summed squared distance
+// |p|^2 - 2 p.t over the texels inside the image; the n_out texels outside
+// it hold t = 0.  Per texel and entry, e = 8 p.t + nk = -4 (|p|^2 - 2 p.t) +
+// (3 - i): one v_dot4_u32_u8 and one shift-add; the largest e is the
+// nearest entry and, among equally near ones, the lowest i (distinct
+// distances differ by 4 in e, 3 - i by at most 3).  Its low two bits, 3 - i,
+// are gathered into `inv`.  The sum runs over all 16 texels; a texel with
+// t = 0 takes e0 = max nk, which is taken back out n_out times.
+__device__ __forceinline__ int nearest8(const uint32_t t8[16], int n_out, const uint32_t pk[4], const int nk[4],
+                                        uint32_t &inv)
+{
+    int sum = 0;
+    inv = 0;
 #pragma unroll
     for (int t = 0; t < 16; t++) {
         const uint32_t v = t8[t];
-        int best = n2[0] - 2 * int(__builtin_amdgcn_udot4(pk[0], v, 0u, false));
-        uint32_t bi = 0;
+        int e = int(__builtin_amdgcn_udot4(pk[0], v, 0u, false) << 3) + nk[0];
 #pragma unroll
-        for (int i = 1; i < 4; i++) {
-            const int d = n2[i] - 2 * int(__builtin_amdgcn_udot4(pk[i], v, 0u, false));
-            if (d < best) { best = d; bi = uint32_t(i); }
-        }
-        if ((inside >> t) & 1u) err += best + int(__builtin_amdgcn_udot4(v, v, 0u, false));
-        bits |= bi << (2 * t);
+        for (int i = 1; i < 4; i++) e = max(e, int(__builtin_amdgcn_udot4(pk[i], v, 0u, false) << 3) + nk[i]);
+        inv |= uint32_t(e & 3) << (2 * t);
+        sum += e;
     }
-    if (bits_out) *bits_out = bits;
-    return float(err);
+    // sum = -4 D + (sum of the 3 - i fields) over all 16 texels
+    const int fields = __builtin_popcount(inv & 0x55555555u) + 2 * __builtin_popcount(inv & 0xaaaaaaaau);
+    const int e0 = max(max(nk[0], nk[1]), max(nk[2], nk[3]));
+    return (fields - sum + n_out * (e0 - (e0 & 3))) >> 2;
 }
 
-__device__ __forceinline__ float encode_pair8(const uint32_t t8[16], uint32_t inside, const float a[3], const float b[3],
-                                              const float *mid5, const float *mid6, uint2 *blk)
+// Error (exact squared error in 8-bit units) and index bits of the block
+// with 565 endpoints c0, c1, each texel taking its nearest palette entry
+// (the lowest entry on ties), on the block's texels held as 8-bit RGB
+// (each channel round(255 * saturate(v)); 0 for the n_out texels outside
+// the image) with v2 = texel_norms(t8).  For 8-bit sources (every value
+// k / 255) this is evaluate_mse's measure without rounding.
+__device__ __forceinline__ float eval_pair8(const uint32_t t8[16], int n_out, int v2, uint32_t c0, uint32_t c1,
+                                            uint32_t *bits_out)
+{
+    uint32_t pk[4];
+    int nk[4];
+    palette8(c0, c1, pk, nk);
+    uint32_t inv;
+    const int d = nearest8(t8, n_out, pk, nk, inv);
+    if (bits_out) *bits_out = ~inv;
+    return float(d + v2);
+}
+
+__device__ __forceinline__ float encode_pair8(const uint32_t t8[16], int n_out, int v2, const float a[3],
+                                              const float b[3], const float *mid5, const float *mid6, uint2 *blk)
 {
     uint32_t c0 = ntx_bc1::to565(a[0], a[1], a[2], mid5, mid6);
     uint32_t c1 = ntx_bc1::to565(b[0], b[1], b[2], mid5, mid6);
     if (c0 < c1) { const uint32_t t = c0; c0 = c1; c1 = t; }
     uint32_t bits;
-    const float err = eval_pair8(t8, inside, c0, c1, &bits);
+    const float err = eval_pair8(t8, n_out, v2, c0, c1, &bits);
     blk->x = c0 | (c1 << 16);
     blk->y = bits;
     return err;
@@ -854,12 +830,11 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                            inr[2] ? (c[k][2] - m[k]) * s : 0.0f, inr[3] ? (c[k][3] - m[k]) * s : 0.0f);
         // Block-level facts from the four lanes that hold its rows: the
         // colour bounding box of the texels inside the image, whether they
-        // all equal texel 0 (row 0's lane, lane col), which texels are
-        // inside, and the texels as 8-bit RGB for the candidate errors.
+        // all equal texel 0 (row 0's lane, lane col), and the texels as 8-bit
+        // RGB for the candidate errors.
         float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
         const float t0[3] = { __shfl(c[0][0], col), __shfl(c[1][0], col), __shfl(c[2][0], col) };
         int same = 1;
-        uint32_t inside = 0;
         uint32_t p8[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -870,9 +845,8 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                     hi[k] = fmaxf(hi[k], c[k][j]);
                 }
                 if ((g | j) != 0 && (c[0][j] != t0[0] || c[1][j] != t0[1] || c[2][j] != t0[2])) same = 0;
-                inside |= 1u << (4 * g + j);
             }
-            p8[j] = to_u8(c[0][j]) | to_u8(c[1][j]) << 8 | to_u8(c[2][j]) << 16;
+            p8[j] = to_u8(c[0][j]) | to_u8(c[1][j]) << 8 | to_u8(c[2][j]) << 16;  // 0 outside (c is 0 there)
         }
 #pragma unroll
         for (int off = 16; off < 64; off <<= 1) {
@@ -882,7 +856,6 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                 hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
             }
             same &= __shfl_xor(same, off);
-            inside |= uint32_t(__shfl_xor(int(inside), off));
         }
         const bool uniform = valid && same;
 
@@ -906,6 +879,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
             const uint4 v = *reinterpret_cast<const uint4 *>(&s_t8[wave][col][4 * q4]);
             t8[4 * q4] = v.x; t8[4 * q4 + 1] = v.y; t8[4 * q4 + 2] = v.z; t8[4 * q4 + 3] = v.w;
         }
+        const int v2 = texel_norms(t8), n_out = 16 - cnt;
         const float4 *tb = s_tex[wave][col];
         const float is = 1.0f / s;
         // Candidate g: diagonal g of the block's colour bounding box, plus
@@ -917,7 +891,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
         // The diagonal as it is, refined and encoded ...
         if (refine_iters > 0) refine(tb, a, bb, refine_iters);
         uint2 blk;
-        float err = valid ? encode_pair8(t8, inside, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
+        float err = valid ? encode_pair8(t8, n_out, v2, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
         if (box_candidates) {
             pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
         } else {
@@ -932,7 +906,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
             }
             if (refine_iters > 0) refine(tb, na, nb, refine_iters);
             uint2 nblk;
-            const float nerr = valid ? encode_pair8(t8, inside, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
+            const float nerr = valid ? encode_pair8(t8, n_out, v2, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
             if (nerr < err) { err = nerr; blk = nblk; }
         }
         // Best of the four candidates (lower candidate on equal error).
@@ -987,7 +961,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                 uint32_t c0 = uint32_t(t2[0]) << 11 | uint32_t(t2[1]) << 5 | uint32_t(t2[2]);
                 uint32_t c1 = uint32_t(t2[3]) << 11 | uint32_t(t2[4]) << 5 | uint32_t(t2[5]);
                 if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
-                const float e = valid ? eval_pair8(t8, inside, c0, c1, nullptr) : FLT_MAX;
+                const float e = valid ? eval_pair8(t8, n_out, v2, c0, c1, nullptr) : FLT_MAX;
                 if (e < be) { be = e; bm = mv; }
             }
 #pragma unroll
@@ -1007,7 +981,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
             uint32_t c1 = uint32_t(q[3]) << 11 | uint32_t(q[4]) << 5 | uint32_t(q[5]);
             if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
             uint32_t bits = 0;
-            if (valid) err = eval_pair8(t8, inside, c0, c1, &bits);
+            if (valid) err = eval_pair8(t8, n_out, v2, c0, c1, &bits);
             blk.x = c0 | (c1 << 16);
             blk.y = bits;
         }
