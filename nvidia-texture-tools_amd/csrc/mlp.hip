@@ -51,9 +51,10 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdg
 constexpr int kF1 = 0, kF2 = 12, kF3 = 28, kF3T = 36, kF2T = 44, kFrags = 60;
 
 // The workgroup's copy of the network: F fragments (all kFrags for
-// training, the forward ones, kF3T, for inference) and the biases.
+// training, the forward ones, kF3T, for inference) and the biases (16-byte
+// aligned: a lane reads four consecutive biases at once).
 template <int F>
-struct NetT {
+struct alignas(16) NetT {
     v4s frag[F][64];
     float b1[kHidden], b2[kHidden], b3[kOutPad];
 };
@@ -98,53 +99,51 @@ __device__ void load_net(NetT<F> &net, const float *__restrict__ p, int t, int n
     for (int i = t; i < kOutPad; i += nthreads) net.b3[i] = i < kOut ? p[kB3 + i] : 0.0f;
 }
 
+// ReLU of four BF16 values: as 16-bit integers a negative BF16 value (sign
+// bit set) is negative and a positive one keeps its order, so the packed
+// integer max with 0 is the ReLU (v_pk_max_i16, two values per instruction).
+__device__ __forceinline__ v4s relu_bf16(v4s x)
+{
+    const v4s zero = { 0, 0, 0, 0 };
+    return __builtin_elementwise_max(x, zero);
+}
+
 // Forward pass for the 16 blocks whose input fragments are bx[0..2] (k-step
 // s = colour channel s; lane l holds texels 4(l/16)..+3 of block l % 16).
-// Returns the post-ReLU hidden activations as FP32 values of their BF16
-// roundings (what the next layer and the weight gradients consume) and the
-// 32 output rows (25 used).
+// Each layer accumulates onto its bias (the matrix cores' C operand), and
+// the ReLU acts on the BF16 rounding.  Returns the post-ReLU hidden
+// activations as FP32 values of their BF16 roundings (what the next layer
+// and the weight gradients consume) and the 32 output rows (25 used).
 template <int F>
 __device__ __forceinline__ void forward(const NetT<F> &net, const v4s bx[3], int lane, v4f h1[4], v4f h2[4], v4f y[2])
 {
     const int g = lane >> 4;
-    const v4f zero = { 0.0f, 0.0f, 0.0f, 0.0f };
     v4s hb[4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        v4f acc = zero;
+        v4f acc = *reinterpret_cast<const v4f *>(&net.b1[16 * m + 4 * g]);
 #pragma unroll
         for (int s = 0; s < 3; s++) acc = mfma(net.frag[kF1 + 3 * m + s][lane], bx[s], acc);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float v = acc[j] + net.b1[16 * m + 4 * g + j];
-            acc[j] = v > 0.0f ? v : 0.0f;
-        }
-        hb[m] = pack4(acc[0], acc[1], acc[2], acc[3]);
+        hb[m] = relu_bf16(pack4(acc[0], acc[1], acc[2], acc[3]));
 #pragma unroll
         for (int j = 0; j < 4; j++) h1[m][j] = bf16_float(uint16_t(hb[m][j]));
     }
     v4s hb2[4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        v4f acc = zero;
+        v4f acc = *reinterpret_cast<const v4f *>(&net.b2[16 * m + 4 * g]);
 #pragma unroll
         for (int s = 0; s < 4; s++) acc = mfma(net.frag[kF2 + 4 * m + s][lane], hb[s], acc);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float v = acc[j] + net.b2[16 * m + 4 * g + j];
-            acc[j] = v > 0.0f ? v : 0.0f;
-        }
-        hb2[m] = pack4(acc[0], acc[1], acc[2], acc[3]);
+        hb2[m] = relu_bf16(pack4(acc[0], acc[1], acc[2], acc[3]));
 #pragma unroll
         for (int j = 0; j < 4; j++) h2[m][j] = bf16_float(uint16_t(hb2[m][j]));
     }
 #pragma unroll
     for (int m = 0; m < 2; m++) {
-        v4f acc = zero;
+        v4f acc = *reinterpret_cast<const v4f *>(&net.b3[16 * m + 4 * g]);
 #pragma unroll
         for (int s = 0; s < 4; s++) acc = mfma(net.frag[kF3 + 4 * m + s][lane], hb2[s], acc);
-#pragma unroll
-        for (int j = 0; j < 4; j++) y[m][j] = acc[j] + net.b3[16 * m + 4 * g + j];
+        y[m] = acc;
     }
 }
 
@@ -618,26 +617,32 @@ __global__ __launch_bounds__(256) void mlp_adam_kernel(float *__restrict__ param
 
 // ------------------------------------------------------------ compression --
 
-// Least-squares refinement of one endpoint pair for the block's texels
-// t4[i] = (r, g, b, weight), weight 0 outside the image (4-colour index by
-// projection onto the segment, then the 2x2 normal equations), `iters`
-// times; endpoints stay in [0, 1].
-__device__ __forceinline__ void refine(const float4 *t4, float a[3], float b[3], int iters)
+// Least-squares refinement of one endpoint pair for the block's texels held
+// as 8-bit RGB in LDS, t8[t] = R | G << 8 | B << 16 (4-colour index by projection
+// onto the segment, then the 2x2 normal equations), `iters` times; texel t
+// takes part when it lies inside the image, (t & 3) < xe and (t >> 2) < ye.
+// Endpoints stay in [0, 1].
+__device__ __forceinline__ void refine8(const uint32_t *t8, int xe, int ye, float a[3], float b[3], int iters)
 {
     for (int it = 0; it < iters; it++) {
         const float d[3] = { b[0] - a[0], b[1] - a[1], b[2] - a[2] };
         const float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         const float il = len2 > 0.0f ? 1.0f / len2 : 0.0f;
         float saa = 0.0f, sbb = 0.0f, sab = 0.0f, sac[3] = { 0, 0, 0 }, sbc[3] = { 0, 0, 0 };
+#pragma unroll 4
         for (int t = 0; t < 16; t++) {
-            const float4 v = t4[t];
-            if (v.w == 0.0f) continue;
-            const float u = ((v.x - a[0]) * d[0] + (v.y - a[1]) * d[1] + (v.z - a[2]) * d[2]) * il;
+            const float v[3] = { float(t8[t] & 255u) * (1.0f / 255.0f), float((t8[t] >> 8) & 255u) * (1.0f / 255.0f),
+                                 float(t8[t] >> 16) * (1.0f / 255.0f) };
+            const bool in = (t & 3) < xe && (t >> 2) < ye;
+            const float u = ((v[0] - a[0]) * d[0] + (v[1] - a[1]) * d[1] + (v[2] - a[2]) * d[2]) * il;
             const float q = fminf(fmaxf(rintf(3.0f * u), 0.0f), 3.0f);
-            const float be = q * (1.0f / 3.0f), al = 1.0f - be;
+            const float be = in ? q * (1.0f / 3.0f) : 0.0f, al = in ? 1.0f - q * (1.0f / 3.0f) : 0.0f;
             saa += al * al; sbb += be * be; sab += al * be;
-            sac[0] += al * v.x; sac[1] += al * v.y; sac[2] += al * v.z;
-            sbc[0] += be * v.x; sbc[1] += be * v.y; sbc[2] += be * v.z;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sac[k] += al * v[k];
+                sbc[k] += be * v[k];
+            }
         }
         const float det = saa * sbb - sab * sab;
         if (!(det > 1e-8f)) break;
@@ -706,7 +711,7 @@ __device__ __forceinline__ int nearest8(const uint32_t t8[16], int n_out, const 
         int e = int(__builtin_amdgcn_udot4(pk[0], v, 0u, false) << 3) + nk[0];
 #pragma unroll
         for (int i = 1; i < 4; i++) e = max(e, int(__builtin_amdgcn_udot4(pk[i], v, 0u, false) << 3) + nk[i]);
-        inv |= uint32_t(e & 3) << (2 * t);
+        inv = __builtin_amdgcn_alignbit(uint32_t(e), inv, 2);  // (inv >> 2) | e << 30: texel 0 ends in bits 0-1
         sum += e;
     }
     // sum = -4 D + (sum of the 3 - i fields) over all 16 texels
@@ -752,6 +757,49 @@ __device__ __forceinline__ uint32_t to_u8(float v)
     return uint32_t(rintf(s * 255.0f));
 }
 
+// Exchanges across the four 16-lane rows of a wavefront without LDS
+// (v_permlane16_swap_b32 and v_permlane32_swap_b32 with both operands x):
+// rows16(x) gives each lane x of the even row (lo) and of the odd row (hi)
+// of its row pair {0, 1} or {2, 3}, same column; halves32(x) gives x of the
+// lower (lo) and upper (hi) lane of its pair l, l ^ 32.
+struct LanePair {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ LanePair rows16(uint32_t x)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return { uint32_t(r[0]), uint32_t(r[1]) };
+}
+__device__ __forceinline__ LanePair halves32(uint32_t x)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return { uint32_t(r[0]), uint32_t(r[1]) };
+}
+
+// op over the four lanes l % 16 + 16 r (r = 0..3) of a column, in each of
+// them (op commutative: the result is the same in all four).
+template <typename Op>
+__device__ __forceinline__ float column_reduce(float x, Op op)
+{
+    LanePair p = rows16(__float_as_uint(x));
+    x = op(__uint_as_float(p.lo), __uint_as_float(p.hi));
+    p = halves32(__float_as_uint(x));
+    return op(__uint_as_float(p.lo), __uint_as_float(p.hi));
+}
+template <typename Op>
+__device__ __forceinline__ int column_reduce_int(int x, Op op)
+{
+    LanePair p = rows16(uint32_t(x));
+    x = op(int(p.lo), int(p.hi));
+    p = halves32(uint32_t(x));
+    return op(int(p.lo), int(p.hi));
+}
+// x of row 0 of the lane's column.
+__device__ __forceinline__ float column_row0(float x)
+{
+    return __uint_as_float(halves32(rows16(__float_as_uint(x)).lo).lo);
+}
+
 // One wave per 16 blocks at a time, 4 waves per workgroup.  Lane (g, n) of a
 // tile gathers row g of block n, the network runs on the matrix cores, and
 // then lane (g, n) refines candidate g of block n; the four candidates of a
@@ -762,7 +810,7 @@ __device__ __forceinline__ uint32_t to_u8(float v)
 // and moved by the network's offsets, keeping the better; box_candidates
 // drops the moved candidates (the same method without the network; no
 // prediction, so only tau_pred < 0 sends blocks to the fallback).
-__global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
+__global__ __launch_bounds__(256, 4) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
                                                       int w, int h, int bw, int n, const ntx::Tables *__restrict__ tab,
                                                       float tau_pred, float tau_gap, int refine_iters, int greedy_rounds,
                                                       int box_candidates,
@@ -770,17 +818,17 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                                                       int *__restrict__ fb_list, float *__restrict__ err_out)
 {
     __shared__ ForwardNet net;
-    // Each wave's 16 blocks: texels (r, g, b, weight 1 inside the image, 0
-    // outside; a padded row per block; only for least-squares passes), the
-    // texels as 8-bit RGB, and the network outputs.  Only the wave that owns
-    // them touches them after the network is loaded.
-    __shared__ float4 s_tex[4][16][17];
+    // Each wave's 16 blocks: the texels as 8-bit RGB and the network
+    // outputs.  Only the wave that owns them touches them after the network
+    // is loaded.
     __shared__ __align__(16) uint32_t s_t8[4][16][16];
     __shared__ float s_y[4][16][kOutPad + 1];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, col = lane & 15;
     load_net(net, params, t, 256);
     __syncthreads();
     const size_t plane = size_t(w) * size_t(h);
+    // Rows of 4 texels start on 16-byte boundaries in every plane.
+    const bool aligned = (w & 3) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0;
     const int tiles = (n + 15) / 16;
     // A wave past the last tile computes on no valid block.
     for (int group = blockIdx.x; group * 4 < tiles; group += gridDim.x) {
@@ -791,37 +839,55 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
         // Row g of block b, and the block mean / scale over the texels inside.
         float c[3][4];
         bool inr[4];
+        const int y = 4 * by + g;
+        if (aligned && valid && 4 * bx + 3 < w && y < h) {
+            // The whole row inside the image: one 16-byte load per plane.
+            const float4 *row = reinterpret_cast<const float4 *>(src + size_t(y) * size_t(w) + size_t(4 * bx));
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float4 v = row[k * (plane >> 2)];
+                c[k][0] = v.x; c[k][1] = v.y; c[k][2] = v.z; c[k][3] = v.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) inr[j] = true;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int x = 4 * bx + j;
+                inr[j] = valid && x < w && y < h;
+                const size_t o = inr[j] ? size_t(y) * size_t(w) + size_t(x) : 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) c[k][j] = inr[j] ? src[o + k * plane] : 0.0f;
+            }
+        }
         float sum[3] = { 0, 0, 0 };
         int cnt = 0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int x = 4 * bx + j, y = 4 * by + g;
-            inr[j] = valid && x < w && y < h;
-            const size_t o = inr[j] ? size_t(y) * size_t(w) + size_t(x) : 0;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                c[k][j] = inr[j] ? src[o + k * plane] : 0.0f;
-                sum[k] += c[k][j];
-            }
+            for (int k = 0; k < 3; k++) sum[k] += c[k][j];
             cnt += inr[j] ? 1 : 0;
         }
+        const auto add = [](auto a, auto b) { return a + b; };
 #pragma unroll
-        for (int off = 16; off < 64; off <<= 1) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) sum[k] += __shfl_xor(sum[k], off);
-            cnt += __shfl_xor(cnt, off);
-        }
+        for (int k = 0; k < 3; k++) sum[k] = column_reduce(sum[k], add);
+        cnt = column_reduce_int(cnt, add);
         float m[3];
+        if (cnt == 16) {
+            // sum / 16, exactly
 #pragma unroll
-        for (int k = 0; k < 3; k++) m[k] = cnt > 0 ? sum[k] / float(cnt) : 0.0f;
+            for (int k = 0; k < 3; k++) m[k] = sum[k] * 0.0625f;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++) m[k] = cnt > 0 ? sum[k] / float(cnt) : 0.0f;
+        }
         float d = 0.0f;
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int k = 0; k < 3; k++)
                 if (inr[j]) d = fmaxf(d, fabsf(c[k][j] - m[k]));
-#pragma unroll
-        for (int off = 16; off < 64; off <<= 1) d = fmaxf(d, __shfl_xor(d, off));
+        d = column_reduce(d, [](float a, float b) { return fmaxf(a, b); });
         const float s = 1.0f / fmaxf(d, kMinSpread);
         v4s bx3[3];
 #pragma unroll
@@ -833,7 +899,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
         // all equal texel 0 (row 0's lane, lane col), and the texels as 8-bit
         // RGB for the candidate errors.
         float lo[3] = { 1.0f, 1.0f, 1.0f }, hi[3] = { 0.0f, 0.0f, 0.0f };
-        const float t0[3] = { __shfl(c[0][0], col), __shfl(c[1][0], col), __shfl(c[2][0], col) };
+        const float t0[3] = { column_row0(c[0][0]), column_row0(c[1][0]), column_row0(c[2][0]) };
         int same = 1;
         uint32_t p8[4];
 #pragma unroll
@@ -849,14 +915,11 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
             p8[j] = to_u8(c[0][j]) | to_u8(c[1][j]) << 8 | to_u8(c[2][j]) << 16;  // 0 outside (c is 0 there)
         }
 #pragma unroll
-        for (int off = 16; off < 64; off <<= 1) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
-                hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
-            }
-            same &= __shfl_xor(same, off);
+        for (int k = 0; k < 3; k++) {
+            lo[k] = column_reduce(lo[k], [](float a, float b) { return fminf(a, b); });
+            hi[k] = column_reduce(hi[k], [](float a, float b) { return fmaxf(a, b); });
         }
+        same = column_reduce_int(same, [](int a, int b) { return a & b; });
         const bool uniform = valid && same;
 
         v4f h1[4], h2[4], yv[2];
@@ -864,9 +927,6 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
         *reinterpret_cast<uint4 *>(&s_t8[wave][col][4 * g]) = make_uint4(p8[0], p8[1], p8[2], p8[3]);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            if (refine_iters > 0)
-                s_tex[wave][col][4 * g + j] = inr[j] ? make_float4(c[0][j], c[1][j], c[2][j], 1.0f)
-                                                     : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) s_y[wave][col][16 * mm + 4 * g + j] = yv[mm][j];
         }
@@ -880,7 +940,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
             t8[4 * q4] = v.x; t8[4 * q4 + 1] = v.y; t8[4 * q4 + 2] = v.z; t8[4 * q4 + 3] = v.w;
         }
         const int v2 = texel_norms(t8), n_out = 16 - cnt;
-        const float4 *tb = s_tex[wave][col];
+        const int xe = min(4, w - 4 * bx), ye = min(4, h - 4 * by);
         const float is = 1.0f / s;
         // Candidate g: diagonal g of the block's colour bounding box, plus
         // the network's offsets (in the block frame, hence divided by the
@@ -889,7 +949,7 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
         box_diagonal(g, lo, hi, a, bb);
         float pred = fmaxf(s_y[wave][col][24], 0.0f) * is * 255.0f;  // predicted RMSE, 8-bit units
         // The diagonal as it is, refined and encoded ...
-        if (refine_iters > 0) refine(tb, a, bb, refine_iters);
+        if (refine_iters > 0) refine8(s_t8[wave][col], xe, ye, a, bb, refine_iters);
         uint2 blk;
         float err = valid ? encode_pair8(t8, n_out, v2, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
         if (box_candidates) {
@@ -904,18 +964,22 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                 na[k] = fminf(fmaxf(na[k] + s_y[wave][col][6 * g + k] * is, 0.0f), 1.0f);
                 nb[k] = fminf(fmaxf(nb[k] + s_y[wave][col][6 * g + 3 + k] * is, 0.0f), 1.0f);
             }
-            if (refine_iters > 0) refine(tb, na, nb, refine_iters);
+            if (refine_iters > 0) refine8(s_t8[wave][col], xe, ye, na, nb, refine_iters);
             uint2 nblk;
             const float nerr = valid ? encode_pair8(t8, n_out, v2, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
             if (nerr < err) { err = nerr; blk = nblk; }
         }
-        // Best of the four candidates (lower candidate on equal error).
-#pragma unroll
-        for (int off = 16; off < 64; off <<= 1) {
-            const float oe = __shfl_xor(err, off);
-            const uint32_t ox = __shfl_xor(blk.x, off), oy = __shfl_xor(blk.y, off);
-            const bool lower = ((lane ^ off) >> 4) < g;
-            if (oe < err || (oe == err && lower)) { err = oe; blk.x = ox; blk.y = oy; }
+        // Best of the four candidates (lower candidate on equal error): of
+        // each exchanged pair, lo holds the lower candidates.
+        {
+            LanePair e = rows16(__float_as_uint(err)), px = rows16(blk.x), py = rows16(blk.y);
+            bool up = __uint_as_float(e.hi) < __uint_as_float(e.lo);
+            err = __uint_as_float(up ? e.hi : e.lo);
+            blk = make_uint2(up ? px.hi : px.lo, up ? py.hi : py.lo);
+            e = halves32(__float_as_uint(err)); px = halves32(blk.x); py = halves32(blk.y);
+            up = __uint_as_float(e.hi) < __uint_as_float(e.lo);
+            err = __uint_as_float(up ? e.hi : e.lo);
+            blk = make_uint2(up ? px.hi : px.lo, up ? py.hi : py.lo);
         }
         // A block whose texels inside the image are all equal gets what the
         // cluster fit gives it: reduce_colors' running weighted mean of the
@@ -964,11 +1028,18 @@ __global__ __launch_bounds__(256, 3) void bc1_mlp_kernel(const float *__restrict
                 const float e = valid ? eval_pair8(t8, n_out, v2, c0, c1, nullptr) : FLT_MAX;
                 if (e < be) { be = e; bm = mv; }
             }
-#pragma unroll
-            for (int off = 16; off < 64; off <<= 1) {
-                const float oe = __shfl_xor(be, off);
-                const int om = __shfl_xor(bm, off);
-                if (oe < be || (oe == be && om < bm)) { be = oe; bm = om; }
+            // The lowest (error, move) over the block's four lanes.
+            {
+                LanePair e = rows16(__float_as_uint(be)), m = rows16(uint32_t(bm));
+                float el = __uint_as_float(e.lo), eh = __uint_as_float(e.hi);
+                bool up = eh < el || (eh == el && int(m.hi) < int(m.lo));
+                be = up ? eh : el;
+                bm = int(up ? m.hi : m.lo);
+                e = halves32(__float_as_uint(be)); m = halves32(uint32_t(bm));
+                el = __uint_as_float(e.lo); eh = __uint_as_float(e.hi);
+                up = eh < el || (eh == el && int(m.hi) < int(m.lo));
+                be = up ? eh : el;
+                bm = int(up ? m.hi : m.lo);
             }
             if (bm < 64) {
 #pragma unroll
