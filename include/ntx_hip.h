@@ -122,7 +122,8 @@ enum {
     NTX_KERNEL_BC3_FAST = 21,        /* QuickCompress::compressDXT5 (BC3 / BC3n Fastest), one thread per block */
     NTX_KERNEL_BC1A = 22,            /* BC1a: bucketing front pass + kDxt1 cluster fit, one block per lane */
     NTX_KERNEL_MIP_TAIL = 23,        /* the small last levels of a mip chain, one launch, one workgroup per plane */
-    NTX_KERNEL_COUNT = 24
+    NTX_KERNEL_POLY_SLIDE = 24,      /* exact halving, same taps for every output: one wavefront per column strip */
+    NTX_KERNEL_COUNT = 25
 };
 NTX_API const char *ntx_hip_kernel_name(int kernel);
 NTX_API ntx_status ntx_hip_profile_enable(ntx_hip_context *ctx, int enable);

@@ -253,6 +253,118 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
     }
 }
 
+// Sliding-window form of the fused resize for exact halving with the same
+// taps for every output (the polyphase tables of Box, Triangle and Kaiser at
+// 2:1 on even extents): each wavefront owns one plane's strip of 64 output
+// columns by `rows` output rows and walks down the source rows the strip
+// needs, in order.  A source row is staged in the wavefront's own LDS row,
+// its X taps give the lane's temporary value (applyKernelX), and that value
+// is added into the (kWin + 1) / 2 outputs whose Y windows contain the row,
+// whose running sums stay in registers.  Each output therefore sums its Y
+// taps in tap order from the same temporaries as applyKernelY: bit-identical
+// to the two-pass form, with each staged row read by X taps once (no
+// temporary in LDS, no workgroup barriers).  Source rows are fetched four
+// row pairs ahead of their use.
+struct SlideTaps {
+    float kx[kMaxWin], ky[kMaxWin];
+};
+
+template <int kWin>
+__global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict__ src, int W, int H,
+                                                         float *__restrict__ dst, int w, int h, int lx0, int ly0,
+                                                         int rows, int ctiles, int strips, int wrap, SlideTaps taps)
+{
+    static_assert(kWin & 1, "odd window: (kWin + 1) / 2 outputs per source row");
+    constexpr int K = (kWin + 1) / 2;
+    constexpr int kSeg = 2 * 63 + kWin;  // staged source columns of 64 outputs
+    constexpr int kSlots = (kSeg + 63) / 64;
+    __shared__ __align__(16) float s_row[4][2][kSlots * 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int job = blockIdx.x * 4 + wv;
+    if (job >= 4 * strips * ctiles) return;  // whole wavefront
+    const int ct = job % ctiles, rest = job / ctiles, st = rest % strips, c = rest / strips;
+    const int ox0 = ct * 64, oy0 = st * rows;
+    const int nrows = min(rows, h - oy0);
+    const int x = ox0 + lane;
+    const bool xlive = x < w;
+    // Wrapped source columns of the lane's staging slots (always a valid
+    // column, also past the segment, whose slots no lane reads).
+    int gx[kSlots];
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) gx[k] = wrap_index(lx0 + 2 * ox0 + lane + 64 * k, W, wrap);
+    const float *plane = src + size_t(c) * size_t(W) * size_t(H);
+    float *out = dst + size_t(c) * size_t(w) * size_t(h) + size_t(oy0) * size_t(w) + x;
+    const int r0 = ly0 + 2 * oy0;              // first source row, before wrapping
+    const int last = 2 * (nrows - 1) + kWin - 1;  // index of the strip's last source row (even)
+    float *const buf0 = s_row[wv][0], *const buf1 = s_row[wv][1];
+
+    // Register pipeline: rows 2p, 2p + 1 of pair p in r[p % kDepth].
+    constexpr int kDepth = 4;
+    float r[kDepth][2][kSlots];
+    auto fetch = [&](float (&rr)[2][kSlots], int q) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int qq = min(q + i, last);
+            const float *row = plane + size_t(wrap_index(r0 + qq, H, wrap)) * size_t(W);
+#pragma unroll
+            for (int k = 0; k < kSlots; k++) rr[i][k] = row[gx[k]];
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < kDepth; d++)
+        if (2 * d <= last) fetch(r[d], 2 * d);
+
+    float acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.0f;
+    // X taps of the staged row at `b` for the lane's output column.
+    auto xtaps = [&](const float *b) {
+        const uint64_t *t = reinterpret_cast<const uint64_t *>(b + 2 * lane);
+        uint64_t q[(kWin + 1) / 2];
+#pragma unroll
+        for (int p = 0; p < (kWin + 1) / 2; p++) q[p] = t[p];
+        float sum = 0.0f;
+#pragma unroll
+        for (int p = 0; p < (kWin + 1) / 2; p++) {
+            sum += taps.kx[2 * p] * __uint_as_float(uint32_t(q[p]));
+            if (2 * p + 1 < kWin) sum += taps.kx[2 * p + 1] * __uint_as_float(uint32_t(q[p] >> 32));
+        }
+        return sum;
+    };
+    // One pair of source rows 2p (even: the last Y tap of acc[0], which is
+    // then complete) and 2p + 1, from the registers rr; then rr is refilled
+    // with pair p + kDepth.  False after the strip's last row.
+    auto pair = [&](float (&rr)[2][kSlots], int p) -> bool {
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) {
+            buf0[lane + 64 * k] = rr[0][k];
+            buf1[lane + 64 * k] = rr[1][k];
+        }
+        ntx_wave_sync();
+        if (2 * (p + kDepth) <= last) fetch(rr, 2 * (p + kDepth));
+        const float xe = xtaps(buf0);
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += taps.ky[2 * (K - 1 - k)] * xe;
+        const int s = p - (K - 1);  // output row completed by row 2p
+        if (s >= 0 && xlive) out[size_t(s) * size_t(w)] = acc[0];
+        if (2 * p == last) return false;
+#pragma unroll
+        for (int k = 0; k + 1 < K; k++) acc[k] = acc[k + 1];
+        acc[K - 1] = 0.0f;
+        const float xo = xtaps(buf1);
+#pragma unroll
+        for (int k = 0; k + 1 < K; k++) acc[k] += taps.ky[2 * K - 3 - 2 * k] * xo;
+        ntx_wave_sync();  // this pair's LDS reads before the next pair's writes
+        return true;
+    };
+    bool more = true;
+    for (int p = 0; more; p += kDepth) {
+#pragma unroll
+        for (int d = 0; d < kDepth; d++)
+            if (more) more = pair(r[d], p + d);
+    }
+}
+
 // The last levels of a chain in one launch: one workgroup per plane holds
 // the level in LDS and produces every following level from it, writing each
 // to its place in HBM.  These levels are small enough that a launch per
@@ -542,6 +654,49 @@ hipError_t launch_mip_tail(const float *d_src, const MipTail &tail, int wrap, hi
     return hipGetLastError();
 }
 int poly_tile_y() { return kTY; }
+
+// Strip height of the sliding-window kernel for a W x H source: as tall as
+// keeps eight wavefronts per compute unit busy (each strip re-reads the
+// window's overlap with its neighbour), 0 when that takes strips under 32
+// rows: a wavefront then walks too few rows to hide its row fetches, and
+// the tiled kernel, which issues a whole tile's loads at once, is faster.
+int poly_slide_rows(int W, int H, int cus)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    const size_t ctiles = size_t((w + 63) / 64);
+    for (int rows = 64; rows >= 32; rows /= 2)
+        if (size_t(4) * ctiles * size_t((h + rows - 1) / rows) >= size_t(8) * size_t(cus)) return rows;
+    return 0;
+}
+
+hipError_t launch_poly_slide(const float *d_src, int W, int H, float *d_dst, const float *kx, int lx0, const float *ky,
+                             int ly0, int win, int wrap, int rows, hipStream_t stream)
+{
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    if (win < 3 || win > 15 || (win & 1) == 0 || rows < 1) return hipErrorInvalidValue;
+    SlideTaps taps{};
+    for (int j = 0; j < win; j++) {
+        taps.kx[j] = kx[j];
+        taps.ky[j] = ky[j];
+    }
+    const int ctiles = (w + 63) / 64;
+    const int strips = (h + rows - 1) / rows;
+    const size_t jobs = size_t(4) * size_t(ctiles) * size_t(strips);
+    const dim3 grid(unsigned((jobs + 3) / 4));
+    auto kern = poly_slide_kernel<13>;
+    switch (win) {
+    case 3: kern = poly_slide_kernel<3>; break;
+    case 5: kern = poly_slide_kernel<5>; break;
+    case 7: kern = poly_slide_kernel<7>; break;
+    case 9: kern = poly_slide_kernel<9>; break;
+    case 11: kern = poly_slide_kernel<11>; break;
+    case 15: kern = poly_slide_kernel<15>; break;
+    default: break;
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, stream, d_src, W, H, d_dst, w, h, lx0, ly0, rows, ctiles, strips, wrap,
+                       taps);
+    return hipGetLastError();
+}
 
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
                              bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream)

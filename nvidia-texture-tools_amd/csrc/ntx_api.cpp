@@ -189,6 +189,11 @@ ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const floa
     d.span_y = max_span(left, k.window, ntx::poly_tile_y());
     d.stride2 = true;
     for (int i = 0; i < dst; i++) d.stride2 = d.stride2 && left[i] == left[0] + 2 * i;
+    d.uniform = d.stride2 && k.window <= 16;
+    for (int i = 1; d.uniform && i < dst; i++)
+        d.uniform = std::memcmp(k.weights + size_t(i) * k.window, k.weights, sizeof(float) * size_t(k.window)) == 0;
+    d.left0 = left[0];
+    if (d.uniform) std::memcpy(d.w0, k.weights, sizeof(float) * size_t(k.window));
     hipError_t e = hipMalloc(&d.weights, sizeof(float) * size_t(k.window) * dst);
     if (e == hipSuccess) e = hipMalloc(&d.left, sizeof(int) * dst);
     if (e == hipSuccess) e = hipMemcpy(d.weights, k.weights, sizeof(float) * size_t(k.window) * dst, hipMemcpyHostToDevice);
@@ -397,7 +402,8 @@ NTX_API const char *ntx_hip_kernel_name(int kernel)
                                                    "normal_kernel", "bc1_classify_kernel",
                                                    "bc1_search_kernel", "optimal_channel_kernel",
                                                    "optimal_green_kernel", "bc3_fast_kernel",
-                                                   "bc1a_front_kernel+bc3_search_kernel", "mip_tail_kernel" };
+                                                   "bc1a_front_kernel+bc3_search_kernel", "mip_tail_kernel",
+                                                   "poly_slide_kernel" };
     return (kernel >= 0 && kernel < NTX_KERNEL_COUNT) ? names[kernel] : "";
 }
 
@@ -559,7 +565,14 @@ static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int fi
         PolyDev *kx = nullptr, *ky = nullptr;
         if ((s = poly_tables(ctx, filter, width, params, W, w, &kx))) return s;
         if ((s = poly_tables(ctx, filter, width, params, H, h, &ky))) return s;
-        if (poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
+        const int win = kx->window;
+        const int rows = poly_slide_rows(W, H, ctx->cus);
+        if (kx->uniform && ky->uniform && ky->window == win && (win & 1) && win >= 3 && win <= 15 && rows > 0) {
+            e = timed_on(ctx, stream, NTX_KERNEL_POLY_SLIDE, [&] {
+                return launch_poly_slide(d_rgba, W, H, d_out, kx->w0, kx->left0, ky->w0, ky->left0, win, wrap, rows,
+                                         stream);
+            });
+        } else if (poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
             e = timed_on(ctx, stream, NTX_KERNEL_POLY_FUSED, [&] {
                 return launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, kx->stride2, ky->weights,
                                          ky->left, ky->window, wrap, stream);

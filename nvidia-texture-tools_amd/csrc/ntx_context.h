@@ -56,6 +56,13 @@ hipError_t launch_poly_x(const float *d_src, int W, int H, float *d_tmp, const f
 hipError_t launch_poly_y(const float *d_tmp, int W, int H, float *d_dst, const float *d_ky, const int *d_ly, int wy, int wrap,
                          hipStream_t stream);
 hipError_t launch_mip_tail(const float *d_src, const MipTail &tail, int wrap, hipStream_t stream);
+// Exact halving with the same taps for every output (PolyDev::uniform) in
+// both directions, an odd window of 3 to 15 taps: host weights kx, ky,
+// first taps lx0, ly0 of output 0, strips of `rows` output rows
+// (poly_slide_rows: 0 when the tiled kernel suits the size better).
+int poly_slide_rows(int W, int H, int cus);
+hipError_t launch_poly_slide(const float *d_src, int W, int H, float *d_dst, const float *kx, int lx0, const float *ky,
+                             int ly0, int win, int wrap, int rows, hipStream_t stream);
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
                              bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
 bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
@@ -101,6 +108,12 @@ struct PolyDev {
     int span_x = 0;  // largest source span of a run of poly_tile_x() outputs
     int span_y = 0;  // same for poly_tile_y() outputs
     bool stride2 = false;  // left[i] == left[0] + 2 i for every output i
+    // stride2 and every output's weights equal output 0's (bit for bit):
+    // exact halving, whose taps left0, w0[0 .. window) the sliding-window
+    // kernel takes as arguments.
+    bool uniform = false;
+    int left0 = 0;
+    float w0[16] = {};
 };
 
 using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;

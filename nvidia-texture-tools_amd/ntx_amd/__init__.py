@@ -36,7 +36,8 @@ KERNELS = ("bc1_kernel", "bc3_search_kernel", "channel_kernel", "box_kernel", "p
            "srgb_kernel", "unpack_kernel", "poly_fused_kernel", "bc3_front_kernel", "batch_kernel",
            "bc1_mlp_kernel", "bc1_list_kernel", "mlp_train_kernel", "bc3_search_split_kernel", "gamma_kernel",
            "normal_kernel", "bc1_classify_kernel", "bc1_search_kernel", "optimal_channel_kernel",
-           "optimal_green_kernel", "bc3_fast_kernel", "bc1a_front_kernel+bc3_search_kernel", "mip_tail_kernel")
+           "optimal_green_kernel", "bc3_fast_kernel", "bc1a_front_kernel+bc3_search_kernel", "mip_tail_kernel",
+           "poly_slide_kernel")
 
 # Every symbol include/ntx_hip.h declares.
 EXPORTS = (

@@ -267,6 +267,24 @@ def test_polyphase_large_and_odd(gpu, oracle, size, filt):
         assert np.array_equal(_bits(a), _bits(b)), (size, wrap)
 
 
+@pytest.mark.parametrize("filt,width,params,alpha_mode", [
+    (N.FILTER_BOX, 0.5, None, N.ALPHA_TRANSPARENCY), (N.FILTER_BOX, 1.0, None, N.ALPHA_NONE),
+    (N.FILTER_TRIANGLE, 1.0, None, N.ALPHA_NONE), (N.FILTER_TRIANGLE, 2.0, None, N.ALPHA_NONE),
+    (N.FILTER_KAISER, 2.0, (4.0, 1.0), N.ALPHA_NONE), (N.FILTER_KAISER, 3.0, (4.0, 1.0), N.ALPHA_NONE)])
+@pytest.mark.parametrize("size", [(2048, 2048), (2570, 1640), (1024, 4096), (256, 130), (2, 258), (514, 2)])
+def test_halving_windows_match_oracle(gpu, oracle, filt, width, params, alpha_mode, size):
+    """Exact halving (even extents) with windows of 3, 5, 9 and 13 taps, the
+    same taps for every output, in every wrap mode: on sources of 2048^2 and
+    up the sliding-window kernel (partial column tiles and a partial last
+    strip at 2570 x 1640), on the smaller ones the tiled kernel."""
+    W, H = size
+    planar = np.random.default_rng(W * 3 + H).random(4 * W * H, dtype=np.float32)
+    for wrap in (N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR):
+        a = gpu.downsample(filt, planar, W, H, width=width, params=params, wrap=wrap, alpha_mode=alpha_mode)
+        b = oracle.downsample(filt, width, params, wrap, planar, W, H)
+        assert np.array_equal(_bits(a), _bits(b)), (size, wrap)
+
+
 @pytest.mark.parametrize("size", [(64, 64), (37, 23), (5, 4)])
 def test_box_with_transparency_uses_polyphase_box(gpu, oracle, size):
     """Surface.cpp:1356-1362: with AlphaMode_Transparency a Box mip goes through
