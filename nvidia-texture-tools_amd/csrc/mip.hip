@@ -280,7 +280,14 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
     constexpr int kSlots = (kSeg + 63) / 64;
     __shared__ __align__(16) float s_row[4][2][kSlots * 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int job = blockIdx.x * 4 + wv;
+    // Workgroups are dispatched round-robin over the eight XCDs, each with
+    // its own L2: when the grid divides by eight, XCD k takes the k-th
+    // contiguous eighth of the jobs (column tile fastest, then strip), so
+    // the halo columns and rows that neighbouring strips share are fetched
+    // into the same L2 at about the same time.
+    const int nwg = gridDim.x;
+    const int wg = (nwg & 7) == 0 ? (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+    const int job = wg * 4 + wv;
     if (job >= 4 * strips * ctiles) return;  // whole wavefront
     const int ct = job % ctiles, rest = job / ctiles, st = rest % strips, c = rest / strips;
     const int ox0 = ct * 64, oy0 = st * rows;

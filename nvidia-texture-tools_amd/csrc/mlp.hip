@@ -985,16 +985,20 @@ __global__ __launch_bounds__(256, 4) void bc1_mlp_kernel(const float *__restrict
         // cluster fit gives it: reduce_colors' running weighted mean of the
         // equal colours (icbc.h:1638-1683, weights 1, texel order) and the
         // single-colour tables.  Texel 0 is always inside the image.
+        // The block's lane g runs the mean of channel g (lane 3 repeats
+        // channel 2): one division per step instead of three.
         if (uniform) {
-            float kr = t0[0], kg = t0[1], kb = t0[2], kw = 1.0f;
+            const float tc = g == 0 ? t0[0] : (g == 1 ? t0[1] : t0[2]);
+            float kc = tc, kw = 1.0f;
             for (int i = 1; i < cnt; i++) {
                 const float s1 = kw + 1.0f;
-                kr = (kr * kw + t0[0]) / s1;
-                kg = (kg * kw + t0[1]) / s1;
-                kb = (kb * kw + t0[2]) / s1;
+                kc = (kc * kw + tc) / s1;
                 kw = s1;
             }
-            blk = ntx_bc1::single_color_block(tab, kr, kg, kb);
+            const LanePair a = rows16(__float_as_uint(kc));  // lo: channel 0 / 2, hi: channel 1 / 2
+            const LanePair rb = halves32(a.lo), gg = halves32(a.hi);
+            blk = ntx_bc1::single_color_block(tab, __uint_as_float(rb.lo), __uint_as_float(gg.lo),
+                                              __uint_as_float(rb.hi));
         }
         // Greedy search on the quantised endpoints (+-1 in one 565 channel of
         // one endpoint; the 12 moves spread 3 per lane over the block's lanes),
