@@ -986,14 +986,21 @@ __global__ __launch_bounds__(256, 4) void bc1_mlp_kernel(const float *__restrict
         // equal colours (icbc.h:1638-1683, weights 1, texel order) and the
         // single-colour tables.  Texel 0 is always inside the image.
         // The block's lane g runs the mean of channel g (lane 3 repeats
-        // channel 2): one division per step instead of three.
+        // channel 2): one division per step instead of three; a whole block
+        // of an 8-bit value takes the tabulated mean.
         if (uniform) {
             const float tc = g == 0 ? t0[0] : (g == 1 ? t0[1] : t0[2]);
-            float kc = tc, kw = 1.0f;
-            for (int i = 1; i < cnt; i++) {
-                const float s1 = kw + 1.0f;
-                kc = (kc * kw + tc) / s1;
-                kw = s1;
+            const int k8 = int(rintf(fminf(fmaxf(tc, 0.0f), 1.0f) * 255.0f));
+            float kc = tc;
+            if (cnt == 16 && tab->div255[k8] == tc) {
+                kc = tab->mean16[k8];
+            } else {
+                float kw = 1.0f;
+                for (int i = 1; i < cnt; i++) {
+                    const float s1 = kw + 1.0f;
+                    kc = (kc * kw + tc) / s1;
+                    kw = s1;
+                }
             }
             const LanePair a = rows16(__float_as_uint(kc));  // lo: channel 0 / 2, hi: channel 1 / 2
             const LanePair rb = halves32(a.lo), gg = halves32(a.hi);

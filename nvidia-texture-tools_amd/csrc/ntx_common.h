@@ -77,6 +77,10 @@ struct Tables {
     uint16_t omatcha6[256];
     // k / 255.0f for k = 0..255 (icbc's color_to_vector3, icbc.h:1611-1613).
     float div255[256];
+    // reduce_colors' running weighted mean (icbc.h:1638-1683) of 16 equal
+    // colour channels of value div255[k], weights 1, in texel order: what a
+    // uniform 4x4 block of an 8-bit source averages to.
+    float mean16[256];
 };
 
 void build_tables(Tables *t);

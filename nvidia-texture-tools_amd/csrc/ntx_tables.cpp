@@ -170,6 +170,16 @@ void build_tables(Tables *t)
         }
 
     for (int k = 0; k < 256; k++) t->div255[k] = float(k) / 255.0f;
+    for (int k = 0; k < 256; k++) {
+        const float c = t->div255[k];
+        float m = c, w = 1.0f;
+        for (int i = 1; i < 16; i++) {
+            const float s1 = w + 1.0f;
+            m = (m * w + c) / s1;
+            w = s1;
+        }
+        t->mean16[k] = m;
+    }
 
     // Gamma-2.2 exponent tables (Gamma.cpp:30-300) from their closed form.
     for (int k = 0; k < 512; k++) {
