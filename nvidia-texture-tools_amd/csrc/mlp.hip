@@ -810,7 +810,7 @@ __device__ __forceinline__ float column_row0(float x)
 // and moved by the network's offsets, keeping the better; box_candidates
 // drops the moved candidates (the same method without the network; no
 // prediction, so only tau_pred < 0 sends blocks to the fallback).
-__global__ __launch_bounds__(256, 4) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
+__global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict__ params, const float *__restrict__ src,
                                                       int w, int h, int bw, int n, const ntx::Tables *__restrict__ tab,
                                                       float tau_pred, float tau_gap, int refine_iters, int greedy_rounds,
                                                       int box_candidates,
