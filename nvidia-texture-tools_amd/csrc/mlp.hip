@@ -751,10 +751,13 @@ __device__ __forceinline__ float encode_pair8(const uint32_t t8[16], int n_out, 
     return err;
 }
 
-__device__ __forceinline__ uint32_t to_u8(float v)
+// R | G << 8 | B << 16 with each channel round(255 * saturate(v)) (round to
+// nearest even): v_cvt_pk_u8_f32 rounds and saturates 255 v into its byte.
+__device__ __forceinline__ uint32_t rgb_u8(float r, float g, float b)
 {
-    const float s = fminf(fmaxf(v, 0.0f), 1.0f);
-    return uint32_t(rintf(s * 255.0f));
+    uint32_t p = __builtin_amdgcn_cvt_pk_u8_f32(r * 255.0f, 0, 0u);
+    p = __builtin_amdgcn_cvt_pk_u8_f32(g * 255.0f, 1, p);
+    return __builtin_amdgcn_cvt_pk_u8_f32(b * 255.0f, 2, p);
 }
 
 // Exchanges across the four 16-lane rows of a wavefront without LDS
@@ -912,7 +915,7 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
                 }
                 if ((g | j) != 0 && (c[0][j] != t0[0] || c[1][j] != t0[1] || c[2][j] != t0[2])) same = 0;
             }
-            p8[j] = to_u8(c[0][j]) | to_u8(c[1][j]) << 8 | to_u8(c[2][j]) << 16;  // 0 outside (c is 0 there)
+            p8[j] = rgb_u8(c[0][j], c[1][j], c[2][j]);  // 0 outside (c is 0 there)
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
