@@ -263,7 +263,7 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
 // whose running sums stay in registers.  Each output therefore sums its Y
 // taps in tap order from the same temporaries as applyKernelY: bit-identical
 // to the two-pass form, with each staged row read by X taps once (no
-// temporary in LDS, no workgroup barriers).  Source rows are fetched four
+// temporary in LDS, no workgroup barriers).  Source rows are fetched eight
 // row pairs ahead of their use.
 struct SlideTaps {
     float kx[kMaxWin], ky[kMaxWin];
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
     float *const buf0 = s_row[wv][0], *const buf1 = s_row[wv][1];
 
     // Register pipeline: rows 2p, 2p + 1 of pair p in r[p % kDepth].
-    constexpr int kDepth = 4;
+    constexpr int kDepth = 8;
     float r[kDepth][2][kSlots];
     auto fetch = [&](float (&rr)[2][kSlots], int q) {
 #pragma unroll

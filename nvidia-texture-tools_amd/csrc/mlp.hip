@@ -1021,10 +1021,14 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
             q[0] = int(e0 >> 11); q[1] = int((e0 >> 5) & 63u); q[2] = int(e0 & 31u);
             q[3] = int(e1 >> 11); q[4] = int((e1 >> 5) & 63u); q[5] = int(e1 & 31u);
         }
+        // Each move's evaluation keeps its index bits, which travel with the
+        // winning (error, move), so the final block needs no re-evaluation.
         const int rounds = uniform ? 0 : greedy_rounds;  // uniform across the block's lanes
+        uint32_t bits = blk.y;
         for (int round = 0; round < rounds; round++) {
             float be = err;
             int bm = 64;
+            uint32_t bb = 0;
 #pragma unroll 1
             for (int i = 0; i < 3; i++) {
                 const int mv = 3 * g + i, slot = mv >> 1, step = (mv & 1) ? 1 : -1;
@@ -1039,34 +1043,36 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
                 uint32_t c0 = uint32_t(t2[0]) << 11 | uint32_t(t2[1]) << 5 | uint32_t(t2[2]);
                 uint32_t c1 = uint32_t(t2[3]) << 11 | uint32_t(t2[4]) << 5 | uint32_t(t2[5]);
                 if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
-                const float e = valid ? eval_pair8(t8, n_out, v2, c0, c1, nullptr) : FLT_MAX;
-                if (e < be) { be = e; bm = mv; }
+                uint32_t mbits = 0;
+                const float e = valid ? eval_pair8(t8, n_out, v2, c0, c1, &mbits) : FLT_MAX;
+                if (e < be) { be = e; bm = mv; bb = mbits; }
             }
-            // The lowest (error, move) over the block's four lanes.
+            // The lowest (error, move) over the block's four lanes, with its bits.
             {
-                LanePair e = rows16(__float_as_uint(be)), m = rows16(uint32_t(bm));
+                LanePair e = rows16(__float_as_uint(be)), m = rows16(uint32_t(bm)), pb = rows16(bb);
                 float el = __uint_as_float(e.lo), eh = __uint_as_float(e.hi);
                 bool up = eh < el || (eh == el && int(m.hi) < int(m.lo));
                 be = up ? eh : el;
                 bm = int(up ? m.hi : m.lo);
-                e = halves32(__float_as_uint(be)); m = halves32(uint32_t(bm));
+                bb = up ? pb.hi : pb.lo;
+                e = halves32(__float_as_uint(be)); m = halves32(uint32_t(bm)); pb = halves32(bb);
                 el = __uint_as_float(e.lo); eh = __uint_as_float(e.hi);
                 up = eh < el || (eh == el && int(m.hi) < int(m.lo));
                 be = up ? eh : el;
                 bm = int(up ? m.hi : m.lo);
+                bb = up ? pb.hi : pb.lo;
             }
             if (bm < 64) {
 #pragma unroll
                 for (int k = 0; k < 6; k++) q[k] += k == (bm >> 1) ? ((bm & 1) ? 1 : -1) : 0;
                 err = be;
+                bits = bb;
             }
         }
         if (rounds > 0) {
             uint32_t c0 = uint32_t(q[0]) << 11 | uint32_t(q[1]) << 5 | uint32_t(q[2]);
             uint32_t c1 = uint32_t(q[3]) << 11 | uint32_t(q[4]) << 5 | uint32_t(q[5]);
             if (c0 < c1) { const uint32_t tt = c0; c0 = c1; c1 = tt; }
-            uint32_t bits = 0;
-            if (valid) err = eval_pair8(t8, n_out, v2, c0, c1, &bits);
             blk.x = c0 | (c1 << 16);
             blk.y = bits;
         }
