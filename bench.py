@@ -52,6 +52,9 @@ SQUISH_SETUP = 1500
 # endpoints, D3D10 palette, and for 16 texels 4 squared distances of 8 flops,
 # the nearest of 4 and the weighted error: 2 x 650), and the uniform-block
 # test (48): 250 + 4 * (96 + 20 + 1300) + 48 = 5,962 flops.
+# BC4 / BC5 Normal channel block (QuickCompress::compressDXT5A with its
+# refinement, QuickCompressDXT.cpp:505-592): about 3,500 operations.
+CHANNEL_BLOCK_OPS = 3500
 MLP_FORWARD_FLOPS = 2 * (48 * 64 + 64 * 64 + 64 * 25)
 MLP_VALU_FLOPS = 250 + 4 * (96 + 20 + 2 * 650) + 48
 
@@ -87,6 +90,35 @@ def bc1_flops(nm: np.ndarray) -> float:
     n3 = np.where(m >= 1, (m + 2) * (m + 1) // 2 - 1, 0)
     f = np.where(n >= 2, 134 * n4 + 117 * n3 + 3500, 200)
     return float(f.sum())
+
+
+def bc1_levels_flops(ctx, d_levels, sizes) -> float:
+    """Counted flops of BC1 Level8 over the given device images, from each
+    block's colour counts (ntx_hip_compress_bc1_stats_device)."""
+    import ntx_amd as N
+    total = 0.0
+    for d, (w, h) in zip(d_levels, sizes):
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        d_nm, d_blk = ctx.alloc(4 * nb), ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h))
+        ctx.compress_bc1_stats_device(d, w, h, d_blk, d_nm)
+        total += bc1_flops(d_nm.download(np.int16).reshape(-1, 2))
+        d_nm.free()
+        d_blk.free()
+    return total
+
+
+def kernels_roofline(prof: dict, kernels, work: float, runs: int, bound: str = "valu") -> dict:
+    """Counted work (flops) over the summed durations of `kernels` in a
+    profiled pass of `runs` runs.  Where the kernels of one run overlap on
+    several streams, their summed time exceeds the wall time, so the rate is
+    a lower bound of what the kernels sustain."""
+    ks = [k for k in kernels if k in prof]
+    k_ms = sum(prof[k][0] for k in ks) / runs
+    achieved = work / (k_ms / 1e3) / 1e12 if k_ms > 0 else 0.0
+    return {"kernels": " + ".join(ks), "bound": bound, "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "algorithmic_flops_per_run": work,
+            "kernel_ms_per_run": round(k_ms, 4),
+            "kernels_ms_per_run": {k: round(v[0] / runs, 4) for k, v in prof.items()}}
 
 
 def bc3_color_flops(planar: np.ndarray, w: int, h: int) -> float:
@@ -825,6 +857,25 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
             nlev += 1
             same += int(np.array_equal(b.blocks(i, k), out[k].download(np.uint8)))
     b.close()
+    # Roofline of the persistent kernel: the counted compression work (BC1 by
+    # each block's colour counts, BC4 / BC5 per channel block) over its time;
+    # the mips (16 B per source and output texel) run inside the same launch.
+    work = 0.0
+    mip_bytes = 0
+    for t, (lv, out) in zip(texs, chains):
+        sz = N.mip_sizes(t["width"], t["height"])
+        nblk = sum(((a + 3) // 4) * ((c + 3) // 4) for a, c in sz)
+        if t["format"] == N.FORMAT_BC1:
+            work += bc1_levels_flops(ctx, lv, sz)
+        else:
+            work += CHANNEL_BLOCK_OPS * nblk * (2 if t["format"] == N.FORMAT_BC5 else 1)
+        mip_bytes += sum(16 * (a * c + sz[k + 1][0] * sz[k + 1][1]) for k, (a, c) in enumerate(sz[:-1]))
+    roof = kernels_roofline(prof, ("batch_kernel",), work, steps)
+    roof["mip_bytes_per_run"] = mip_bytes
+    roof["note"] = "compression work only in `achieved`; the same launch also writes every mip level"
+    for lv, out in chains:
+        for d in lv[1:] + out:
+            d.free()
     return {
         "workload": f"{n} textures, sizes uniform over {{64..2048}}^2 (powers of two), {n // 2} BC1 noise+rect, "
                     f"{n // 4} BC5 normal maps, {n - n // 2 - n // 4} BC4 single-channel; full Triangle mip chains; "
@@ -839,6 +890,7 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
         "speedup_vs_per_image_path": round(ms_seq / ms, 3),
         "levels_identical_to_per_image_path": f"{same}/{nlev}",
         "kernels_ms_per_run": {k: round(v[0] / steps, 4) for k, v in prof.items()},
+        "roofline": roof,
         **({"cpu_reference": cpu_reference_batch(args, texs, texels)} if cpu else {}),
     }
 
@@ -995,6 +1047,27 @@ def bench_c1(ctx, args, cpu: bool) -> dict:
     res = {"workload": f"{S}x{S} RGBA8 noise+rectangles (seed {args.seed}), {len(levels)}-level Box chain, BC1 Normal "
                        f"per level (ntx_hip_compress_chain_device)",
            "texels_per_step": texels, "ms_per_step": round(ms, 4), "mtexels_s": round(texels / (ms / 1e3) / 1e6, 1)}
+    # Rooflines: BC1's counted flops over its kernels' time, the Box mips'
+    # 16 B per source and output texel over box_kernel's.
+    flops = bc1_levels_flops(ctx, d_lv, levels)
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    res["roofline"] = kernels_roofline(prof, ("bc1_kernel", "bc1_classify_kernel", "bc1_search_kernel"), flops,
+                                       args.steps)
+    box_ms = prof.get("box_kernel", (0.0, 0))[0] / args.steps
+    box_bytes = sum(16 * (w * h + levels[k + 1][0] * levels[k + 1][1]) for k, (w, h) in enumerate(levels[:-1]))
+    if box_ms > 0:
+        res["mip_roofline"] = {"kernel": "box_kernel", "bound": "hbm", "achieved": round(box_bytes / (box_ms / 1e3) / 1e9, 1),
+                               "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": round(box_bytes / (box_ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
+                               "algorithmic_bytes_per_chain": box_bytes, "kernel_ms_per_chain": round(box_ms, 4)}
+    res["note"] = ("1.4 M texels per chain: the levels are launch- and latency-bound (the device is far from "
+                   "full below 512^2); the kernel times overlap on three streams")
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
@@ -1076,6 +1149,7 @@ def bench_cube(ctx, args, net, cpu: bool):
     # settings on every level of 4096 blocks or more.
     fast = [dict(learned.SHIPPED) if ((w + 3) // 4) * ((h + 3) // 4) >= learned.MIN_BLOCKS else None
             for w, h in sizes]
+    profs = {}
     for name in ("plain", "learned", "learned_throughput"):
         out = [ctx.alloc(N.compressed_size(N.FORMAT_BC1, w, h)) for w, h in sizes]
         outs[name] = out
@@ -1095,6 +1169,13 @@ def bench_cube(ctx, args, net, cpu: bool):
         for _ in range(reps):
             run()
         ms = ctx.timer_end() / reps
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        for _ in range(reps):
+            run()
+        ctx.synchronize()
+        profs[name] = ctx.profile_read(reset=True)
+        ctx.profile(False)
         sse = [ctx.bc1_sse_device(ins[i], w, h, out[i]) for i, (w, h) in enumerate(sizes)]
         per_level = []
         for k, (w, h) in enumerate(levels):
@@ -1116,6 +1197,17 @@ def bench_cube(ctx, args, net, cpu: bool):
            "throughput_speedup": round(res["plain"]["ms"] / res["learned_throughput"]["ms"], 3),
            "throughput_worst_level_delta_db": round(min(a - b for a, b in zip(
                res["learned_throughput"]["psnr_db_per_level"], res["plain"]["psnr_db_per_level"])), 3)}
+    # Rooflines: the plain fit's counted flops (every block's colour counts)
+    # over its kernels' time, and the network kernel's counted flops per
+    # block (the shipped settings' work; the refined levels do more) over
+    # bc1_mlp_kernel's time.
+    flops_plain = bc1_levels_flops(ctx, ins, sizes)
+    out["plain"]["roofline"] = kernels_roofline(profs["plain"], ("bc1_kernel", "bc1_classify_kernel",
+                                                                  "bc1_search_kernel"), flops_plain, 3)
+    for name, st in (("learned", [learned.level_settings(w, h, k) for (w, h), (_, k) in zip(sizes, items)]),
+                     ("learned_throughput", fast)):
+        nb = sum(((w + 3) // 4) * ((h + 3) // 4) for (w, h), c in zip(sizes, st) if c is not None)
+        out[name]["roofline"] = kernels_roofline(profs[name], ("bc1_mlp_kernel",), MLP_VALU_FLOPS * nb, 3)
     if cpu:
         out["cpu_reference"] = cpu_reference_chain_bc1(args, [d.download(np.float32) for d in lv[0]], levels,
                                                        [outs["plain"][items.index((0, k))].download(np.uint8)
