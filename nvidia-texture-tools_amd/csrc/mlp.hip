@@ -826,8 +826,12 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
     // is loaded.
     __shared__ __align__(16) uint32_t s_t8[4][16][16];
     __shared__ float s_y[4][16][kOutPad + 1];
+    // The 5:6:5 quantisation midpoints, read per endpoint channel.
+    __shared__ float s_mid5[32], s_mid6[64];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63, g = lane >> 4, col = lane & 15;
     load_net(net, params, t, 256);
+    if (t < 32) s_mid5[t] = tab->mid5[t];
+    if (t < 64) s_mid6[t] = tab->mid6[t];
     __syncthreads();
     const size_t plane = size_t(w) * size_t(h);
     // Rows of 4 texels start on 16-byte boundaries in every plane.
@@ -954,7 +958,7 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
         // The diagonal as it is, refined and encoded ...
         if (refine_iters > 0) refine8(s_t8[wave][col], xe, ye, a, bb, refine_iters);
         uint2 blk;
-        float err = valid ? encode_pair8(t8, n_out, v2, a, bb, tab->mid5, tab->mid6, &blk) : FLT_MAX;
+        float err = valid ? encode_pair8(t8, n_out, v2, a, bb, s_mid5, s_mid6, &blk) : FLT_MAX;
         if (box_candidates) {
             pred = tau_pred < 0.0f ? FLT_MAX : 0.0f;
         } else {
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
             }
             if (refine_iters > 0) refine8(s_t8[wave][col], xe, ye, na, nb, refine_iters);
             uint2 nblk;
-            const float nerr = valid ? encode_pair8(t8, n_out, v2, na, nb, tab->mid5, tab->mid6, &nblk) : FLT_MAX;
+            const float nerr = valid ? encode_pair8(t8, n_out, v2, na, nb, s_mid5, s_mid6, &nblk) : FLT_MAX;
             if (nerr < err) { err = nerr; blk = nblk; }
         }
         // Best of the four candidates (lower candidate on equal error): of
