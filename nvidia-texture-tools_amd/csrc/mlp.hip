@@ -1081,7 +1081,10 @@ __global__ __launch_bounds__(256, 5) void bc1_mlp_kernel(const float *__restrict
             blk.y = bits;
         }
         if (g == 0 && valid) {
-            const float rmse = sqrtf(err / float(3 * cnt));
+            // The block's RMSE only where the gap test or err_out needs it
+            // (both launch arguments: the branch is taken by whole waves).
+            const bool need_rmse = tau_gap >= 0.0f || err_out != nullptr;
+            const float rmse = need_rmse ? sqrtf(err / float(3 * cnt)) : 0.0f;
             const bool fallback = !uniform && (pred > tau_pred || (tau_gap >= 0.0f && rmse > pred + tau_gap));
             out[b] = blk;
             if (err_out) err_out[b] = rmse;
