@@ -124,6 +124,16 @@ struct CompressionOptions {
     NVTT_API void setPixelFormat(unsigned int bitcount, unsigned int rmask, unsigned int gmask, unsigned int bmask,
                                  unsigned int amask);
     NVTT_API void setQuantization(bool colorDithering, bool alphaDithering, bool binaryAlpha, int alphaThreshold = 127);
+    // Extension, no counterpart in the reference: BC1 at Quality_Normal (no
+    // transparency weighting, unit colour weights) by the learned endpoint
+    // proposals of ntx_hip_compress_bc1_mlp_images_device instead of the
+    // cluster fit, with the network parameters `params` (count must be 8921,
+    // NTX_MLP_PARAMS of ntx_hip.h; copied), in Compressor::process and Compressor::compress
+    // (a batch list keeps the cluster fit).  Per image: the shipped settings
+    // at mip level 0, the refined ones below it, the plain cluster fit under
+    // 4096 blocks.  The output is then not the reference's bytes.  NULL (or
+    // count 0) switches it off; returns false for a wrong count.
+    NVTT_API bool setLearnedBC1(const float *params, int count);
     NVTT_API Format format() const;
     Private &m;
 

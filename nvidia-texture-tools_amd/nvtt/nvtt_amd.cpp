@@ -101,6 +101,7 @@ struct CompressionOptions::Private {
     unsigned int bitcount, rmask, gmask, bmask, amask;
     bool enableColorDithering, enableAlphaDithering, binaryAlpha;
     int alphaThreshold;
+    std::vector<float> learned;  // NTX_MLP_PARAMS floats, or empty
 };
 
 CompressionOptions::CompressionOptions() : m(*new Private) { reset(); }
@@ -116,6 +117,7 @@ void CompressionOptions::reset()
     m.rmask = 0x00FF0000u; m.gmask = 0x0000FF00u; m.bmask = 0x000000FFu; m.amask = 0xFF000000u;
     m.enableColorDithering = m.enableAlphaDithering = m.binaryAlpha = false;
     m.alphaThreshold = 127;
+    m.learned.clear();
 }
 void CompressionOptions::setPixelFormat(unsigned int bitcount, unsigned int rmask, unsigned int gmask, unsigned int bmask,
                                         unsigned int amask)
@@ -128,6 +130,17 @@ void CompressionOptions::setQuantization(bool colorDithering, bool alphaDitherin
     m.enableAlphaDithering = alphaDithering;
     m.binaryAlpha = binaryAlpha;
     m.alphaThreshold = alphaThreshold;
+}
+
+bool CompressionOptions::setLearnedBC1(const float *params, int count)
+{
+    if (!params || count == 0) {
+        m.learned.clear();
+        return true;
+    }
+    if (count != NTX_MLP_PARAMS) return false;
+    m.learned.assign(params, params + count);
+    return true;
 }
 
 // nvtt.cpp:31-59 (worded here).
@@ -735,6 +748,53 @@ void Surface::toGamma(float gamma)
 
 // ------------------------------------------------------------- compressor --
 
+namespace {
+
+// Whether the learned BC1 form applies (CompressionOptions::setLearnedBC1):
+// BC1 at Normal quality without transparency weighting or colour weights.
+bool learned_bc1(const CompressionOptions::Private &co, AlphaMode alphaMode)
+{
+    return !co.learned.empty() && co.format == Format_BC1 && co.quality == Quality_Normal &&
+           alphaMode != AlphaMode_Transparency && co.colorWeight[0] == 1.0f && co.colorWeight[1] == 1.0f &&
+           co.colorWeight[2] == 1.0f;
+}
+
+// The learned form's settings for a w x h image at mip level `level`: the
+// plain cluster fit under 4096 blocks; otherwise tau_pred 8 with no
+// least-squares pass, greedy round or gap test at level 0, and one
+// least-squares pass, one greedy round and a gap of 1.0 below it (the
+// SHIPPED and REFINED settings of ntx_amd/learned.py).
+ntx_mlp_settings learned_settings(int w, int h, int level)
+{
+    ntx_mlp_settings st{};
+    if (long((w + 3) / 4) * long((h + 3) / 4) < 4096) {
+        st.plain = 1;
+        return st;
+    }
+    st.tau_pred = 8.0f;
+    st.tau_gap = level == 0 ? -1.0f : 1.0f;
+    st.refine_iters = level == 0 ? 0 : 1;
+    st.greedy_rounds = level == 0 ? 0 : 1;
+    return st;
+}
+
+// n device images (level[i]: the image's mip level) through the learned form
+// with the network in co; waits for the device before the network is freed.
+ntx_status compress_learned(ntx_hip_context *ctx, const CompressionOptions::Private &co, int n,
+                            const float *const *d_in, const int *w, const int *h, const int *level, void *const *d_out)
+{
+    ntx_hip_mlp *mlp = nullptr;
+    ntx_status s = ntx_hip_mlp_create(ctx, co.learned.data(), 0, &mlp);
+    std::vector<ntx_mlp_settings> st(size_t(n > 0 ? n : 0));
+    for (int i = 0; i < n; i++) st[size_t(i)] = learned_settings(w[i], h[i], level[i]);
+    if (s == NTX_OK) s = ntx_hip_compress_bc1_mlp_images_device(mlp, n, d_in, w, h, st.data(), d_out);
+    if (s == NTX_OK) s = ntx_hip_synchronize(ctx);
+    if (mlp) ntx_hip_mlp_destroy(mlp);
+    return s;
+}
+
+}  // namespace
+
 // The backend seam: same virtual as nv::CompressorInterface (Compressor.h:34-38),
 // with the input known to be host or device memory.
 struct CompressorInterface {
@@ -747,8 +807,9 @@ struct CompressorInterface {
 // on the device and hands the whole level to writeData in one call, on the
 // calling thread, in block-raster order.
 struct HipCompressorBCn : CompressorInterface {
-    explicit HipCompressorBCn(bool device_input) : device_input(device_input) {}
+    HipCompressorBCn(bool device_input, int level) : device_input(device_input), level(level) {}
     bool device_input;
+    int level;  // the image's mip level (the learned form's settings depend on it)
     void compress(AlphaMode alphaMode, unsigned w, unsigned h, unsigned d, const float *rgba, TaskDispatcher *,
                   const CompressionOptions::Private &co, const OutputOptions::Private &oo) override
     {
@@ -759,7 +820,22 @@ struct HipCompressorBCn : CompressorInterface {
         {
             std::lock_guard<std::mutex> g(backend_lock());
             ntx_hip_context *ctx = backend();
-            if (!device_input) {
+            if (learned_bc1(co, alphaMode)) {
+                void *din = nullptr, *dout = nullptr;
+                const float *src = rgba;
+                s = NTX_OK;
+                if (!device_input) {
+                    s = ntx_hip_malloc(ctx, 16 * size_t(w) * size_t(h), &din);
+                    if (s == NTX_OK) s = ntx_hip_upload(ctx, din, rgba, 16 * size_t(w) * size_t(h));
+                    src = static_cast<const float *>(din);
+                }
+                if (s == NTX_OK) s = ntx_hip_malloc(ctx, size, &dout);
+                const int wi = int(w), hi = int(h);
+                if (s == NTX_OK) s = compress_learned(ctx, co, 1, &src, &wi, &hi, &level, &dout);
+                if (s == NTX_OK) s = ntx_hip_download(ctx, out.data(), dout, size);
+                if (dout) ntx_hip_free(ctx, dout);
+                if (din) ntx_hip_free(ctx, din);
+            } else if (!device_input) {
                 s = ntx_hip_compress(ctx, int(co.format), int(co.quality), int(alphaMode), co.colorWeight, rgba, int(w),
                                      int(h), out.data(), size);
             } else {
@@ -797,13 +873,13 @@ struct Compressor::Private {
 
     // Context.cpp:1166-1224 chooseGpuCompressor: the formats and qualities the
     // device implements exactly; anything else gets no compressor.
-    CompressorInterface *chooseCompressor(const CompressionOptions::Private &co, bool device_input) const
+    CompressorInterface *chooseCompressor(const CompressionOptions::Private &co, bool device_input, int level = 0) const
     {
         switch (co.format) {
         case Format_BC1:  // BlockCompressor.cpp:211-217: icbc Level1 / 8 / 9 by quality
-            return new HipCompressorBCn(device_input);
+            return new HipCompressorBCn(device_input, level);
         case Format_BC1a:  // CompressorDXT1a (CompressorDX9.cpp:83-111); not FastCompressorDXT1a
-            if (co.quality != Quality_Fastest) return new HipCompressorBCn(device_input);
+            if (co.quality != Quality_Fastest) return new HipCompressorBCn(device_input, level);
             break;
         // Context.cpp:1080-1116: Fast*/Production*/Optimal variants by quality,
         // every one reproduced by the device.
@@ -811,7 +887,7 @@ struct Compressor::Private {
         case Format_BC3n:  // CompressorDXT5n (CompressorDX9.cpp:179-222)
         case Format_BC4:
         case Format_BC5:
-            return new HipCompressorBCn(device_input);
+            return new HipCompressorBCn(device_input, level);
         default:
             break;
         }
@@ -827,7 +903,7 @@ struct Compressor::Private {
         if (!hipEnabled || !backend()) {
             oo.error(Error_CudaError);
         } else {
-            CompressorInterface *c = chooseCompressor(co, device_input);
+            CompressorInterface *c = chooseCompressor(co, device_input, mipmap);
             if (!c) {
                 oo.error(Error_UnsupportedFeature);
             } else {
@@ -1332,10 +1408,16 @@ bool Compressor::Private::process(const InputOptions::Private &io, const Compres
         if (s == NTX_OK && regamma)
             s = ntx_hip_gamma_levels_device(ctx, 1, io.outputGamma, int(n), const_cast<const float *const *>(lin.data()),
                                             ws.data(), hs.data(), gam.data());
-        if (s == NTX_OK)
+        if (s == NTX_OK && learned_bc1(co, alphaMode)) {
+            std::vector<int> lv(n);
+            for (size_t k = 0; k < n; k++) lv[k] = int(k % size_t(mipmapCount));  // images are face-major
+            s = compress_learned(ctx, co, int(n), const_cast<const float *const *>(gam.data()), ws.data(), hs.data(),
+                                 lv.data(), out.data());
+        } else if (s == NTX_OK) {
             s = ntx_hip_compress_images_device(ctx, fmt, int(co.quality), int(alphaMode), co.colorWeight, int(n),
                                                const_cast<const float *const *>(gam.data()), ws.data(), hs.data(),
                                                out.data());
+        }
         if (s == NTX_OK) s = ntx_hip_download(ctx, bytes.data(), out[0], outBytes);
         (void)ntx_hip_synchronize(ctx);
     }

@@ -156,6 +156,17 @@ int run_process(int argc, char **argv)
 
     nvtt::CompressionOptions co;
     co.setFormat(nvtt::Format(std::atoi(argv[4])));
+    if (argc > 17) {
+        // The learned BC1 form with the network parameters in file argv[17].
+        FILE *pf = std::fopen(argv[17], "rb");
+        if (!pf) return fail("open parameters");
+        std::vector<float> params(8921);
+        const size_t got = std::fread(params.data(), sizeof(float), params.size(), pf);
+        std::fclose(pf);
+        if (got != params.size()) return fail("read parameters");
+        CHECK(!co.setLearnedBC1(params.data(), 17));
+        CHECK(co.setLearnedBC1(params.data(), int(params.size())));
+    }
     nvtt::OutputOptions oo;
     Capture cap;
     Errors err;

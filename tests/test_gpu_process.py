@@ -242,6 +242,54 @@ def test_process_unsupported_output_format(tmp_path):
     assert not ok and errors == [UNSUPPORTED_OUTPUT] and stream == b"" and len(events) == 0
 
 
+def test_process_learned_bc1(tmp_path, oracle, gpu):
+    """CompressionOptions::setLearnedBC1 (an extension of the front end):
+    process() compresses the levels by the learned form with the per-level
+    settings of learned.level_settings (shipped at level 0, refined at level
+    1, the plain cluster fit from 4096 blocks down).  The stream equals the
+    multi-image learned entry point's output for the same network on the
+    pipeline's levels (the oracle's, which the other tests pin to the
+    device's), and level 0 differs from the plain fit (the learned path ran)."""
+    w = h = 512
+    img = synth.noise_rect_rgba(w, h, seed=21)
+    rng = np.random.default_rng(4)
+    with N.Mlp(gpu, seed=5) as net:
+        params = net.params()
+    # A network whose output layer moves the candidates (the seeded start has
+    # a zero output layer, which is exactly the box-diagonal method).
+    w3 = 3072 + 64 + 4096 + 64
+    params[w3:w3 + 25 * 64] = rng.normal(0.0, 0.05, 25 * 64).astype(np.float32)
+    pfile = tmp_path / "params.bin"
+    params.astype(np.float32).tofile(pfile)
+    levels = len(N.mip_sizes(w, h))
+    stream, events, errors, ok, est = run_process(tmp_path, [[img]], N.FORMAT_BC1, N.FILTER_BOX, header=False,
+                                                  extra=(str(pfile),))
+    assert ok and errors == []
+    # The pipeline's levels: to linear, Box mips, back to gamma 2.2.
+    lin = oracle.planar_to_linear(synth.to_planar(img).reshape(-1), 2.2)
+    sizes = N.mip_sizes(w, h)
+    gam = []
+    for m, (lw, lh) in enumerate(sizes):
+        if m:
+            lin = oracle.build_next_mipmap(N.FILTER_BOX, 0.5, None, N.WRAP_MIRROR, lin, *sizes[m - 1])
+        gam.append(oracle.planar_to_gamma(lin, 2.2))
+    d_in = [gpu.alloc(x.nbytes).upload(x) for x in gam]
+    d_out = [gpu.alloc(N.compressed_size(N.FORMAT_BC1, a, b)) for a, b in sizes]
+    with N.Mlp(gpu, params) as net:
+        net.compress_bc1_images_device(d_in, sizes, d_out, levels=list(range(levels)))
+    expect = [d.download(np.uint8) for d in d_out]
+    assert [learned.level_settings(a, b, m) is None for m, (a, b) in enumerate(sizes)][:3] == [False, False, True]
+    off = 0
+    for m, blocks in enumerate(expect):
+        got = np.frombuffer(stream, np.uint8, blocks.size, off)
+        assert np.array_equal(got, blocks), m
+        off += blocks.size
+    assert off == len(stream)
+    assert not np.array_equal(expect[0], oracle.compress(N.FORMAT_BC1, gam[0], w, h))
+    for d in d_in + d_out:
+        d.free()
+
+
 def ktx_header(fmt, w, h, mips, faces):
     """KTX 1.1 header (the Khronos layout; KtxFile.h:113-131, Context.cpp:870-1030)."""
     internal, base = {N.FORMAT_BC1: (0x83F0, 0x1907), N.FORMAT_BC3: (0x83F3, 0x1908), N.FORMAT_BC3N: (0x83F3, 0x1908),
