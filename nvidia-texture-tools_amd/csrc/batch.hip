@@ -62,9 +62,21 @@ __device__ __forceinline__ void wait_ready(int *counter, int target, int spin_li
     }
 }
 
+// LDS for a polyphase tile in the separable form: the tile's source region
+// (wrap applied while staging) and the X taps' temporary of its rows.
+constexpr int kMipSrc = 2048, kMipTmp = 1024, kMipTaps = 768;
+
 // Tiles of the next level: thread t of a (1 << shift) x (256 >> shift) tile
-// computes one output texel on all four planes.
-__device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLevel &src, int t)
+// computes one output texel on all four planes.  A polyphase tile whose
+// source region fits kMipSrc (and its temporary kMipTmp) runs separably: the
+// region is staged in LDS, the X taps of every source row the tile's Y taps
+// read fill the temporary, and the Y taps read the temporary.  Every X value
+// is the reference's applyKernelX value of that (wrapped) row and every
+// output sums its Y taps in tap order, exactly as poly_texel does per texel,
+// so both forms give the same bits; poly_texel handles the tiles that do not
+// fit.
+__device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLevel &src, int t, float *s_src,
+                         float *s_tmp, float *s_taps, int *s_first)
 {
     const BatchLevel dst = uniform_copy(a.levels + src.next);
     const int W = src.w, H = src.h, w = dst.w, h = dst.h;
@@ -75,20 +87,84 @@ __device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLev
     float *d = global_ptr(dst.data);
     for (int k = 0; k < it.count; k++) {
         const int tile = it.first + k;
-        const int x = (tile % tiles_x) * tw + (t & (tw - 1));
-        const int y = (tile / tiles_x) * th + (t >> a.tile_shift);
-        if (x >= w || y >= h) continue;
+        const int ox0 = (tile % tiles_x) * tw, oy0 = (tile / tiles_x) * th;
+        const int x = ox0 + (t & (tw - 1));
+        const int y = oy0 + (t >> a.tile_shift);
+        const bool live = x < w && y < h;
         const size_t o = size_t(y) * size_t(w) + size_t(x);
         if (src.mip_kind == 0) {
+            if (!live) continue;
             const bool line = W == 1 || H == 1;
             const int bx = line ? x + y : x, by = line ? 0 : y;  // one of x, y is 0 for a line
 #pragma unroll
             for (int c = 0; c < 4; c++) d[c * dplane + o] = ntx_mip::box_texel(s + c * splane, W, H, bx, by);
-        } else {
+            continue;
+        }
+        // The tile's source region: columns lx[ox0] .. lx[ox0 + nx - 1] + wx,
+        // rows likewise (the first taps grow with the output index).
+        const int nx = min(tw, w - ox0), ny = min(th, h - oy0);
+        const int cx0 = src.lx[ox0], ry0 = src.ly[oy0];
+        const int sw = src.lx[ox0 + nx - 1] - cx0 + src.wx, sh = src.ly[oy0 + ny - 1] - ry0 + src.wy;
+        if (sw * sh > kMipSrc || sh * nx > kMipTmp || nx * src.wx + ny * src.wy > kMipTaps || splane > 0xffffffffu) {
+            if (!live) continue;
 #pragma unroll 1
             for (int c = 0; c < 4; c++)
-                d[c * dplane + o] = ntx_mip::poly_texel(s + c * splane, W, H, x, y, src.kx, src.lx, src.wx, src.ky, src.ly,
-                                                        src.wy, src.wrap);
+                d[c * dplane + o] = ntx_mip::poly_texel(s + c * splane, W, H, x, y, src.kx, src.lx, src.wx, src.ky,
+                                                        src.ly, src.wy, src.wrap);
+            continue;
+        }
+        // The tile's taps and first taps, shared by its four planes: the X
+        // weights of outputs ox0.., then the Y weights of rows oy0.. (both
+        // contiguous in the tables), the first taps relative to the region.
+        float *s_kx = s_taps, *s_ky = s_taps + nx * src.wx;
+        for (int i = t; i < nx * src.wx; i += 256) s_kx[i] = src.kx[size_t(ox0) * src.wx + i];
+        for (int i = t; i < ny * src.wy; i += 256) s_ky[i] = src.ky[size_t(oy0) * src.wy + i];
+        if (t < nx) s_first[t] = src.lx[ox0 + t] - cx0;
+        if (t < ny) s_first[256 + t] = src.ly[oy0 + t] - ry0;
+        // Thread t stages region elements t + 256 k; their (wrapped) source
+        // offsets are the same in every plane, and plane c + 1's values are
+        // fetched into registers while plane c is filtered.
+        constexpr int kSlots = kMipSrc / 256;
+        const int nsrc = sw * sh;
+        uint32_t off[kSlots];
+        float v[kSlots];
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) {
+            const int i = min(t + 256 * k, nsrc - 1);
+            const int r = i / sw, col = i - r * sw;
+            off[k] = uint32_t(ntx_mip::wrap_index(ry0 + r, H, src.wrap)) * uint32_t(W) +
+                     uint32_t(ntx_mip::wrap_index(cx0 + col, W, src.wrap));
+            v[k] = s[off[k]];
+        }
+#pragma unroll 1
+        for (int c = 0; c < 4; c++) {
+#pragma unroll
+            for (int k = 0; k < kSlots; k++)
+                if (t + 256 * k < nsrc) s_src[t + 256 * k] = v[k];
+            __syncthreads();
+            if (c + 1 < 4) {
+                const float *pl = s + size_t(c + 1) * splane;
+#pragma unroll
+                for (int k = 0; k < kSlots; k++) v[k] = pl[off[k]];
+            }
+            for (int i = t; i < sh * nx; i += 256) {
+                const int r = i / nx, xi = i - r * nx;
+                const float *kr = s_kx + xi * src.wx;
+                const float *row = s_src + r * sw + s_first[xi];
+                float sum = 0.0f;
+                for (int j = 0; j < src.wx; ++j) sum += kr[j] * row[j];
+                s_tmp[i] = sum;
+            }
+            __syncthreads();
+            if (live) {
+                const int xi = x - ox0, yi = y - oy0;
+                const float *kr = s_ky + yi * src.wy;
+                const float *col = s_tmp + s_first[256 + yi] * nx + xi;
+                float sum = 0.0f;
+                for (int j = 0; j < src.wy; ++j) sum += kr[j] * col[j * nx];
+                d[c * dplane + o] = sum;
+            }
+            __syncthreads();  // the next plane's staging overwrites both arrays (and the next tile the taps)
         }
     }
 }
@@ -140,6 +216,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves > 0 
 void batch_kernel(BatchArgs a)
 {
     __shared__ ntx_bc1::Bc1Shared sh;
+    __shared__ float s_src[kMipSrc], s_tmp[kMipTmp], s_taps[kMipTaps];
+    __shared__ int s_first[512];
     __shared__ int s_item;
     const int t = threadIdx.x;
     ntx_bc1::load_shared_tables(sh, a.tab, t, 256);
@@ -160,7 +238,7 @@ void batch_kernel(BatchArgs a)
         const BatchLevel lv = uniform_copy(a.levels + it.level);
         if (lv.ready >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (it.kind == ntx::kItemMip) {
-            mip_item(a, it, lv, t);
+            mip_item(a, it, lv, t, s_src, s_tmp, s_taps, s_first);
             // Every thread makes its stores visible device-wide before the
             // workgroup counts the item as done.
             __threadfence();
