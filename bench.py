@@ -857,9 +857,10 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
             nlev += 1
             same += int(np.array_equal(b.blocks(i, k), out[k].download(np.uint8)))
     b.close()
-    # Roofline of the persistent kernel: the counted compression work (BC1 by
-    # each block's colour counts, BC4 / BC5 per channel block) over its time;
-    # the mips (16 B per source and output texel) run inside the same launch.
+    # Roofline of the batch run (the persistent kernel and the two-pass BC1
+    # launches of one graph): the counted compression work (BC1 by each
+    # block's colour counts, BC4 / BC5 per channel block) over its time; the
+    # mips (16 B per source and output texel) run inside the persistent kernel.
     work = 0.0
     mip_bytes = 0
     for t, (lv, out) in zip(texs, chains):
@@ -872,14 +873,14 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
         mip_bytes += sum(16 * (a * c + sz[k + 1][0] * sz[k + 1][1]) for k, (a, c) in enumerate(sz[:-1]))
     roof = kernels_roofline(prof, ("batch_kernel",), work, steps)
     roof["mip_bytes_per_run"] = mip_bytes
-    roof["note"] = "compression work only in `achieved`; the same launch also writes every mip level"
+    roof["note"] = "compression work only in `achieved`; the same graph also writes every mip level"
     for lv, out in chains:
         for d in lv[1:] + out:
             d.free()
     return {
         "workload": f"{n} textures, sizes uniform over {{64..2048}}^2 (powers of two), {n // 2} BC1 noise+rect, "
                     f"{n // 4} BC5 normal maps, {n - n // 2 - n // 4} BC4 single-channel; full Triangle mip chains; "
-                    f"one persistent launch per run from a HIP graph",
+                    f"one HIP graph per run (persistent kernel, then the two-pass BC1 encoder over every BC1 level)",
         "texels_per_run": texels,
         "mtexels_s": round(texels / (ms / 1e3) / 1e6, 2),
         "ms_per_run": round(ms, 4),

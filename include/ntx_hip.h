@@ -108,7 +108,7 @@ enum {
     NTX_KERNEL_UNPACK = 7,
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 pass 1: alpha halves, single-colour blocks, bucketing by colour count */
-    NTX_KERNEL_BATCH = 10,       /* one batch launch: counter reset + persistent kernel */
+    NTX_KERNEL_BATCH = 10,       /* one batch run: counter reset + persistent kernel + two-pass BC1 of its Level8 levels */
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
     NTX_KERNEL_BC1_LIST = 12,    /* BC1 Level8 over a block list (the learned form's fallback), one block per 16-lane group */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */

@@ -39,6 +39,16 @@ struct ntx_hip_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string signature;
+    // BC1 levels at icbc Level8 go to the multi-image two-pass encoder
+    // (classify, then search) after the persistent launch rather than to
+    // its one-block-per-16-lanes items: the flag per level, the device image
+    // table and 256-block chunk map, and the bucket scratch (tallies zero
+    // between runs: the search clears them).
+    std::vector<char> bc1_multi;
+    ntx::Bc1Image *d_bc1_imgs = nullptr;
+    void *d_bc1_map = nullptr;
+    int *d_bc1_bucket = nullptr;
+    int bc1_imgs = 0, bc1_chunks = 0, bc1_cap = 0;
 };
 
 namespace {
@@ -134,7 +144,7 @@ ntx_status build_items(ntx_hip_batch *b)
                 const int li = b->first[t] + s;
                 const BatchLevel &lv = b->levels[li];
                 const int blocks = ((lv.w + 3) / 4) * ((lv.h + 3) / 4);
-                if (pass == 0 && lv.format == NTX_FORMAT_BC1) {
+                if (pass == 0 && lv.format == NTX_FORMAT_BC1 && !b->bc1_multi[size_t(li)]) {
                     for (int k = 0; k < blocks; k += tu.bc1_blocks_per_item)
                         items.push_back({ ntx::kItemBc1, li, k, std::min(tu.bc1_blocks_per_item, blocks - k) });
                 } else if (pass == 1 && lv.format != NTX_FORMAT_BC1) {
@@ -185,6 +195,14 @@ hipError_t enqueue_run(ntx_hip_batch *b)
     hipStream_t s = b->ctx->stream;
     hipError_t e = ntx::launch_batch_reset(b->d_sync, b->ncounters + 2, s);
     if (e == hipSuccess && b->nitems > 0) e = ntx::launch_batch(make_args(b), b->tuning.kernel_variant, b->bc1_tiers, b->workgroups, s);
+    if (e == hipSuccess && b->bc1_imgs > 0) {
+        int *counts = b->d_bc1_bucket, *lists = counts + ntx::bc3_lists_offset();
+        e = ntx::launch_bc1_classify_multi(b->d_bc1_imgs, b->d_bc1_map, b->bc1_chunks, b->ctx->d_tab, counts, lists,
+                                           b->bc1_cap, s);
+        if (e == hipSuccess)
+            e = ntx::launch_bc1_search_multi(b->d_bc1_imgs, b->bc1_imgs, b->ctx->d_tab, counts, lists, b->bc1_cap,
+                                             ntx::kBc1SearchGroupsPerCU * b->ctx->cus, s);
+    }
     return e;
 }
 
@@ -359,6 +377,52 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             }
         }
     }
+    // The Level8 BC1 levels for the multi-image encoder (within its image
+    // and block limits; any others stay persistent-kernel items).
+    {
+        std::vector<ntx::Bc1Image> imgs;
+        std::vector<int> map;
+        long long total = 0;
+        b->bc1_multi.assign(b->levels.size(), 0);
+        for (size_t li = 0; li < b->levels.size(); li++) {
+            const BatchLevel &lv = b->levels[li];
+            if (lv.format != NTX_FORMAT_BC1 || lv.bc1_level != 8) continue;
+            const int blocks = ((lv.w + 3) / 4) * ((lv.h + 3) / 4);
+            if (int(imgs.size()) >= ntx::bc1_multi_max_images() || blocks >= ntx::bc1_multi_max_blocks() ||
+                total + blocks >= (1ll << 28))
+                continue;
+            b->bc1_multi[li] = 1;
+            ntx::Bc1Image im{};
+            im.src = lv.data;
+            im.out = lv.out;
+            im.w = lv.w;
+            im.h = lv.h;
+            im.alpha_mode = lv.alpha_mode;
+            im.rows16 = (reinterpret_cast<uintptr_t>(lv.data) & 15) == 0 && (lv.w & 3) == 0;
+            for (int k = 0; k < 3; k++) im.cw[k] = lv.cw[k];
+            for (int k = 0; k < blocks; k += 256) {
+                map.push_back(int(imgs.size()));
+                map.push_back(k);
+            }
+            imgs.push_back(im);
+            total += blocks;
+        }
+        if (!imgs.empty()) {
+            b->bc1_imgs = int(imgs.size());
+            b->bc1_chunks = int(map.size() / 2);
+            b->bc1_cap = int(total);
+            const size_t bucket = sizeof(int) * size_t(ntx::bc1_scratch_ints(b->bc1_cap));
+            e = hipMalloc(&b->d_bc1_imgs, sizeof(ntx::Bc1Image) * imgs.size());
+            if (e == hipSuccess) e = hipMalloc(&b->d_bc1_map, sizeof(int) * map.size());
+            if (e == hipSuccess) e = hipMalloc(&b->d_bc1_bucket, bucket);
+            if (e == hipSuccess)
+                e = hipMemcpy(b->d_bc1_imgs, imgs.data(), sizeof(ntx::Bc1Image) * imgs.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(b->d_bc1_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice);
+            // Tallies and tickets zero before the first run (the lists need no clearing).
+            if (e == hipSuccess) e = hipMemset(b->d_bc1_bucket, 0, sizeof(int) * size_t(ntx::bc3_lists_offset()));
+            if (e != hipSuccess) return cleanup(ntx::fail(ctx, e, "batch BC1 tables"));
+        }
+    }
     b->signature = signature_of(b->tex);
     auto it = ctx->batch_tuning.find(b->signature);
     if ((s = apply_tuning(b, it != ctx->batch_tuning.end() ? it->second : default_tuning()))) return cleanup(s);
@@ -373,7 +437,8 @@ NTX_API ntx_status ntx_hip_batch_destroy(ntx_hip_batch *b)
     (void)hipStreamSynchronize(b->ctx->stream);
     release_graph(b);
     for (void *p : { b->d_images, b->d_out, static_cast<void *>(b->d_levels), static_cast<void *>(b->d_items),
-                     static_cast<void *>(b->d_sync) })
+                     static_cast<void *>(b->d_sync), static_cast<void *>(b->d_bc1_imgs), b->d_bc1_map,
+                     static_cast<void *>(b->d_bc1_bucket) })
         if (p) (void)hipFree(p);
     delete b;
     return NTX_OK;

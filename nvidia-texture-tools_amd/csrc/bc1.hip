@@ -78,7 +78,7 @@ __device__ __forceinline__ void classify_chunk(const float *__restrict__ src, in
                                                const ntx::Tables *__restrict__ tab, uint2 *__restrict__ out,
                                                short2 *__restrict__ nm, int *__restrict__ counts,
                                                int *__restrict__ lists, int cap, int rows16, float4 (*s_rows)[16][64],
-                                               int *s_cnt, int *s_base, int item)
+                                               int *s_cnt, int *s_base, int item, int tag = 0)
 {
     using namespace ntx_bucket;
     const int t = threadIdx.x;
@@ -108,7 +108,7 @@ __device__ __forceinline__ void classify_chunk(const float *__restrict__ src, in
             local = atomicAdd(&s_cnt[n], 1);
         }
     }
-    append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, blk);
+    append_to_lists(s_cnt, s_base, counts, lists, cap, k, local, tag | blk);
 }
 
 // Level8 pass 1, one block per lane: FloatColorCompressor's gather and
@@ -140,25 +140,26 @@ __global__ __launch_bounds__(256) void bc1_classify_kernel(const float *__restri
 }
 
 
-// One wavefront of bc1_search_kernel: entries first .. first + 63 of list k.
-__device__ __forceinline__ void search_wave(const float *__restrict__ src, int w, int h, int bw, int alpha_mode,
-                                            float cwx, float cwy, float cwz, const ntx::Tables *__restrict__ tab,
-                                            uint2 *__restrict__ out, short2 *__restrict__ nm,
-                                            const int *__restrict__ lists, int cap, int rows16, LaneRows s_rows,
-                                            int lane, int k, int first, int cnt)
+// The image a search lane encodes a block of: the single image of
+// bc1_search_kernel (the same in every lane) or, in the multi-image form,
+// the image its list entry names.
+struct LaneImage {
+    const float *src;
+    uint2 *out;
+    int w, h, bw, alpha_mode, rows16;
+    float cwx, cwy, cwz;
+};
+
+// One block per lane of a wavefront that holds blocks of count k: block blk
+// of image im, written when live (lanes past the end of their list encode
+// the wave's first block again, with the same count and valid weights, and
+// discard it).
+__device__ __forceinline__ void search_lane(const LaneImage &im, int blk, bool live,
+                                            const ntx::Tables *__restrict__ tab, short2 *__restrict__ nm,
+                                            LaneRows s_rows, int lane, int k)
 {
-    const int item = first + lane;
-    bool live = item < cnt;
-    // Lanes past the end of the list encode the wave's first block again
-    // (same count, valid weights) and discard it.  Every caller's lists hold
-    // block indices below cap; anything else is written nowhere.
-    int blk = lists[size_t(k) * cap + (live ? item : first)];
-    if (unsigned(blk) >= unsigned(cap)) {
-        blk = 0;
-        live = false;
-    }
     float4 tx[16];
-    lane_gather(src, w, h, blk % bw, blk / bw, alpha_mode, rows16 != 0, tx);
+    lane_gather(im.src, im.w, im.h, blk % im.bw, blk / im.bw, im.alpha_mode, im.rows16 != 0, tx);
     bool unit = true;
 #pragma unroll
     for (int i = 0; i < 16; i++) unit = unit && tx[i].w == 1.0f;
@@ -169,14 +170,14 @@ __device__ __forceinline__ void search_wave(const float *__restrict__ src, int w
     const int n = lane_reduce(s_rows, lane, tx, any_black);  // == k in live lanes
     const bool rebuild = any_black && lane_has_black_cluster(s_rows, lane, n);
     const float4 sum = lane_sat(s_rows, lane, n);
-    const float mx = cwx * cwx, my = cwy * cwy, mz = cwz * cwz;
+    const float mx = im.cwx * im.cwx, my = im.cwy * im.cwy, mz = im.cwz * im.cwz;
     const float *mid5 = tab->mid5, *mid6 = tab->mid6, *div255 = tab->div255;
 
     // Four colours over the block's n clusters (n is k in every live lane).
     const Best b4 = lane_search4(s_rows, lane, k, sum, mx, my, mz, tab, units);
     uint2 result;
-    float best = lane_output<true>(lane_winner<true>(s_rows, lane, b4, sum, mx, my, mz, tab), tx, cwx, cwy, cwz, mid5,
-                                   mid6, div255, &result);
+    float best = lane_output<true>(lane_winner<true>(s_rows, lane, b4, sum, mx, my, mz, tab), tx, im.cwx, im.cwy,
+                                   im.cwz, mid5, mid6, div255, &result);
 
     // Three colours: the same clusters, or the non-black ones.
     int m = n;
@@ -194,15 +195,34 @@ __device__ __forceinline__ void search_wave(const float *__restrict__ src, int w
         const Best b3 = uniform3 ? lane_search3(s_rows, lane, k, sum3, mx, my, mz, tab, units3)
                                  : lane_search3(s_rows, lane, m, sum3, mx, my, mz, tab, units3);
         uint2 r3;
-        const float e3 = lane_output<false>(lane_winner<false>(s_rows, lane, b3, sum3, mx, my, mz, tab), tx, cwx, cwy,
-                                            cwz, mid5, mid6, div255, &r3);
+        const float e3 = lane_output<false>(lane_winner<false>(s_rows, lane, b3, sum3, mx, my, mz, tab), tx, im.cwx,
+                                            im.cwy, im.cwz, mid5, mid6, div255, &r3);
         if (e3 < best) { best = e3; result = r3; }
     }
     if (!(best < FLT_MAX)) result = make_uint2(0u, 0u);
     if (live) {
-        out[blk] = result;
+        im.out[blk] = result;
         if (nm) nm[blk] = make_short2(short(n), short(run3 ? m : -1));
     }
+}
+
+// The wave of bc1_search_kernel's search: entries first .. first + 63 of
+// list k, k and first found from the wave's index among the lists' waves
+// (walked from count 16 down).  False past the last wave.
+__device__ __forceinline__ bool list_wave(const int *__restrict__ counts, int cap, int wave, int &k, int &first,
+                                          int &cnt)
+{
+    int wv = wave;
+    k = 0;
+    for (int b = 16; b >= 2; b--) {
+        const int c = min(counts[b], cap);
+        const int waves = (c + 63) >> 6;
+        if (k == 0) {
+            if (wv < waves) { k = b; first = wv * 64; cnt = c; }
+            else wv -= waves;
+        }
+    }
+    return k != 0;
 }
 
 // Level8 pass 2, one block per lane (icbc.h:3292-3326 + :3543-3550).  The
@@ -222,21 +242,80 @@ __global__ __launch_bounds__(64) void bc1_search_kernel(const float *__restrict_
     using namespace ntx_bucket;
     __shared__ float4 s_rows[17][64];
     const int lane = threadIdx.x;
+    const LaneImage im{ src, out, w, h, bw, alpha_mode, rows16, cwx, cwy, cwz };
     // A list never holds more than cap entries.
     int total = 0;
     for (int b = 2; b <= 16; b++) total += (min(counts[b], cap) + 63) >> 6;
     for (int wave = blockIdx.x; wave < total; wave += gridDim.x) {
-        int wv = wave, k = 0, first = 0, cnt = 0;
-        for (int b = 16; b >= 2; b--) {
-            const int c = min(counts[b], cap);
-            const int waves = (c + 63) >> 6;
-            if (k == 0) {
-                if (wv < waves) { k = b; first = wv * 64; cnt = c; }
-                else wv -= waves;
-            }
+        int k = 0, first = 0, cnt = 0;
+        if (!list_wave(counts, cap, wave, k, first, cnt)) break;
+        const int item = first + lane;
+        bool live = item < cnt;
+        // Every caller's lists hold block indices below cap; anything else
+        // is written nowhere.
+        int blk = lists[size_t(k) * cap + (live ? item : first)];
+        if (unsigned(blk) >= unsigned(cap)) {
+            blk = 0;
+            live = false;
         }
-        search_wave(src, w, h, bw, alpha_mode, cwx, cwy, cwz, tab, out, nm, lists, cap, rows16, s_rows, lane, k,
-                    first, cnt);
+        search_lane(im, blk, live, tab, nm, s_rows, lane, k);
+    }
+    release_counts(counts);
+}
+
+// The multi-image form (many images' blocks in one pair of launches, e.g.
+// every BC1 level of a batch): an image table, list entries
+// image << kImageShift | block, and for pass 1 one workgroup per chunk of
+// 256 blocks of one image (map[g] = (image, first block)).
+constexpr int kImageShift = 20;
+
+__global__ __launch_bounds__(256) void bc1_classify_multi_kernel(const ntx::Bc1Image *__restrict__ imgs,
+                                                                 const int2 *__restrict__ map,
+                                                                 const ntx::Tables *__restrict__ tab,
+                                                                 int *__restrict__ counts, int *__restrict__ lists,
+                                                                 int cap)
+{
+    __shared__ float4 s_rows[4][16][64];
+    __shared__ int s_cnt[ntx_bucket::kCountSlots], s_base[ntx_bucket::kCountSlots];
+    const int2 m = map[blockIdx.x];
+    const ntx::Bc1Image im = imgs[m.x];
+    const int bw = (im.w + 3) / 4;
+    const int blocks = bw * ((im.h + 3) / 4);
+    classify_chunk(im.src, im.w, im.h, bw, nullptr, blocks, im.alpha_mode, tab, reinterpret_cast<uint2 *>(im.out),
+                   nullptr, counts, lists, cap, im.rows16, s_rows, s_cnt, s_base, m.y + int(threadIdx.x),
+                   m.x << kImageShift);
+}
+
+__global__ __launch_bounds__(64) void bc1_search_multi_kernel(const ntx::Bc1Image *__restrict__ imgs,
+                                                              const ntx::Tables *__restrict__ tab,
+                                                              int *__restrict__ counts, const int *__restrict__ lists,
+                                                              int cap, int nimgs)
+{
+    using namespace ntx_bucket;
+    __shared__ float4 s_rows[17][64];
+    const int lane = threadIdx.x;
+    int total = 0;
+    for (int b = 2; b <= 16; b++) total += (min(counts[b], cap) + 63) >> 6;
+    for (int wave = blockIdx.x; wave < total; wave += gridDim.x) {
+        int k = 0, first = 0, cnt = 0;
+        if (!list_wave(counts, cap, wave, k, first, cnt)) break;
+        const int item = first + lane;
+        bool live = item < cnt;
+        const int e = lists[size_t(k) * cap + (live ? item : first)];
+        int img = int(unsigned(e) >> kImageShift), blk = e & ((1 << kImageShift) - 1);
+        if (img >= nimgs) {
+            img = 0;
+            blk = 0;
+            live = false;
+        }
+        const ntx::Bc1Image d = imgs[img];
+        const LaneImage im{ d.src, reinterpret_cast<uint2 *>(d.out), d.w, d.h, (d.w + 3) / 4, d.alpha_mode, d.rows16,
+                            d.cw[0], d.cw[1], d.cw[2] };
+        if (blk >= im.bw * ((im.h + 3) / 4)) {
+            blk = 0;
+            live = false;
+        }
+        search_lane(im, blk, live, tab, nullptr, s_rows, lane, k);
     }
     release_counts(counts);
 }
@@ -302,6 +381,34 @@ hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, c
 }
 
 int bc1_scratch_ints(int nblocks) { return ntx_bucket::kListsOffset + ntx_bucket::kCountSlots * nblocks; }
+
+int bc1_multi_max_images() { return 1 << (31 - kImageShift); }
+int bc1_multi_max_blocks() { return 1 << kImageShift; }
+
+// Pass 1 of the multi-image form: d_map holds nchunks (image, first block)
+// pairs, one per workgroup of 256 blocks; the lists (zero tallies on entry)
+// take up to cap entries each, cap >= all the images' blocks.
+hipError_t launch_bc1_classify_multi(const Bc1Image *d_imgs, const void *d_map, int nchunks, const Tables *d_tab,
+                                     int *d_counts, int *d_lists, int cap, hipStream_t stream)
+{
+    if (nchunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(bc1_classify_multi_kernel, dim3(nchunks), dim3(256), 0, stream, d_imgs,
+                       static_cast<const int2 *>(d_map), d_tab, d_counts, d_lists, cap);
+    return hipGetLastError();
+}
+
+// Pass 2 of the multi-image form over the lists pass 1 filled; leaves the
+// tallies zero.
+hipError_t launch_bc1_search_multi(const Bc1Image *d_imgs, int nimgs, const Tables *d_tab, int *d_counts,
+                                   const int *d_lists, int cap, int max_groups, hipStream_t stream)
+{
+    if (cap == 0) return hipSuccess;
+    const int waves = (cap + 63) / 64 + (ntx_bucket::kCountSlots - 2);
+    const int grid = max_groups > 0 && max_groups < waves ? max_groups : waves;
+    hipLaunchKernelGGL(bc1_search_multi_kernel, dim3(grid), dim3(64), 0, stream, d_imgs, d_tab, d_counts, d_lists, cap,
+                       nimgs);
+    return hipGetLastError();
+}
 
 // icbc Level8 of the *d_count blocks listed in d_list (at most cap, each
 // below cap, cap = the image's block count), at most max_groups workgroups.

@@ -16,6 +16,18 @@ namespace ntx {
 // first source level (W x H, at most kTailSrc texels per plane), then per
 // produced level its destination and, for the polyphase filters, its X and
 // Y tap tables (as PolyDev holds them); box: FloatImage::fastDownSample.
+// One image of the multi-image two-pass BC1 encoder (Level8): its planar
+// FP32 RGBA texels, its blocks (8 bytes each), and how they are weighted.
+struct Bc1Image {
+    const float *src;
+    void *out;
+    int w, h;
+    int alpha_mode;
+    int rows16;  // src 16-byte aligned and w % 4 == 0: rows read as float4
+    float cw[3];
+    int pad;
+};
+
 constexpr int kTailMaxLevels = 16;
 constexpr int kTailSrc = 8192;
 struct MipTail {

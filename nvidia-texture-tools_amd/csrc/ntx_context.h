@@ -24,6 +24,15 @@ hipError_t launch_bc1_list(const float *d_src, int w, int h, int alpha_mode, con
 hipError_t launch_bc1_search(const float *d_src, int w, int h, int alpha_mode, const float *cw, const Tables *d_tab,
                              void *d_out, int16_t *d_nm, int *d_counts, const int *d_lists, int cap, int max_groups,
                              hipStream_t stream);
+// The multi-image two-pass BC1 encoder (Level8) over a table of images:
+// at most bc1_multi_max_images() images of at most bc1_multi_max_blocks()
+// blocks; d_map: (image, first block) per 256-block chunk.
+int bc1_multi_max_images();
+int bc1_multi_max_blocks();
+hipError_t launch_bc1_classify_multi(const Bc1Image *d_imgs, const void *d_map, int nchunks, const Tables *d_tab,
+                                     int *d_counts, int *d_lists, int cap, hipStream_t stream);
+hipError_t launch_bc1_search_multi(const Bc1Image *d_imgs, int nimgs, const Tables *d_tab, int *d_counts,
+                                   const int *d_lists, int cap, int max_groups, hipStream_t stream);
 // Workgroups of bc1_search_kernel resident at once per compute unit.
 constexpr int kBc1SearchGroupsPerCU = 8;
 // Workgroups (four wavefronts each) per compute unit for the wavefront-per-
