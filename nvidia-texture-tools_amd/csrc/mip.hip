@@ -103,21 +103,31 @@ __device__ __forceinline__ void tile_coords(int tile, int tx, int &tX, int &tY)
     }
 }
 
+// The taps of exact halving with the same weights for every output, passed
+// by value (they arrive in scalar registers): weights kx, ky of output 0.
+struct SlideTaps {
+    float kx[kMaxWin], ky[kMaxWin];
+};
+
 // kPair: every output's first tap is two source columns after its left
 // neighbour's (downsampling by exactly 2), so lane xi's taps start at column
 // 2*xi of the staged row and are read two at a time with 8-byte LDS reads
 // (consecutive lanes on consecutive 8-byte words: no bank conflicts).
 // kWin > 0: both passes have exactly kWin taps, known when compiling, so the
 // tap loops unroll without a test per tap; 0: wx and wy taps as passed.
-template <bool kPair, int kWin>
+// kUni: exact halving with the same taps for every output (kPair, kWin > 0):
+// the weights come from `taps` and output i's first tap is l0 + 2 i, so no
+// table is read before the staging loads (small levels are latency-bound).
+template <bool kPair, int kWin, bool kUni = false>
 __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__restrict__ src, int W, int H,
                                                          float *__restrict__ dst, int w, int h, int tx,
                                                          const float *__restrict__ kx, const int *__restrict__ lx, int wx,
                                                          const float *__restrict__ ky, const int *__restrict__ ly, int wy,
-                                                         int wrap)
+                                                         int wrap, SlideTaps taps, int lx0, int ly0)
 {
     static_assert(2 * (kTX - 1) + kMaxWin <= kSrcW, "paired X reads stay inside a staged row");
     static_assert(kWin >= 0 && kWin <= kMaxWin, "window held in registers");
+    static_assert(!kUni || (kPair && kWin > 0), "uniform taps: exact halving of a fixed window");
     constexpr int kTaps = kWin > 0 ? kWin : kMaxWin;  // unrolled trip count
     if (kWin > 0) {
         wx = kWin;
@@ -132,17 +142,19 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
     tile_coords(blockIdx.x, tx, tX, tY);
     const int ox0 = tX * kTX, oy0 = tY * kTY;
     const int nx = min(kTX, w - ox0), ny = min(kTY, h - oy0);
-    const int cx0 = lx[ox0], ry0 = ly[oy0];
-    const int sw = lx[ox0 + nx - 1] - cx0 + wx;  // staged columns
-    const int sh = ly[oy0 + ny - 1] - ry0 + wy;  // staged rows
+    const int cx0 = kUni ? lx0 + 2 * ox0 : lx[ox0], ry0 = kUni ? ly0 + 2 * oy0 : ly[oy0];
+    const int sw = kUni ? 2 * (nx - 1) + wx : lx[ox0 + nx - 1] - cx0 + wx;  // staged columns
+    const int sh = kUni ? 2 * (ny - 1) + wy : ly[oy0 + ny - 1] - ry0 + wy;  // staged rows
 
     // The tile's Y weights and first rows, shared by every plane.
     static_assert(kTY * kMaxWin <= 64 * kWaves, "one thread per staged Y weight");
-    if (t < ny * kMaxWin) {
-        const int o = t / kMaxWin, j = t % kMaxWin;
-        s_ky[t] = j < wy ? ky[size_t(oy0 + o) * wy + j] : 0.0f;
+    if (!kUni) {
+        if (t < ny * kMaxWin) {
+            const int o = t / kMaxWin, j = t % kMaxWin;
+            s_ky[t] = j < wy ? ky[size_t(oy0 + o) * wy + j] : 0.0f;
+        }
+        if (t < ny) s_ly[t] = ly[oy0 + t] - ry0;
     }
-    if (t < ny) s_ly[t] = ly[oy0 + t] - ry0;
 
     // Per-thread X-pass column and its weights, reused for every row and plane.
     const int xi = t & (kTX - 1);
@@ -152,7 +164,8 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
     if (xlive) {
         xoff = kPair ? 2 * xi : lx[ox0 + xi] - cx0;
 #pragma unroll
-        for (int j = 0; j < kMaxWin; j++) kxr[j] = j < wx ? kx[size_t(ox0 + xi) * wx + j] : 0.0f;
+        for (int j = 0; j < kMaxWin; j++)
+            kxr[j] = j < wx ? (kUni ? taps.kx[j] : kx[size_t(ox0 + xi) * wx + j]) : 0.0f;
     }
 
     // Staging: lane (t & 63) owns source columns (t & 63) + 64k of the tile,
@@ -241,11 +254,11 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
             float *out = dst + size_t(c) * size_t(w) * size_t(h);
             for (int o = wave; o < ny; o += kWaves) {
                 const float *k = s_ky + o * kMaxWin;
-                const float *col = s_tmp + s_ly[o] * kTX + xi;
+                const float *col = s_tmp + (kUni ? 2 * o : s_ly[o]) * kTX + xi;
                 float sum = 0.0f;
 #pragma unroll
                 for (int j = 0; j < kTaps; ++j)
-                    if (kWin > 0 || j < wy) sum += k[j] * col[j * kTX];
+                    if (kWin > 0 || j < wy) sum += (kUni ? taps.ky[j] : k[j]) * col[j * kTX];
                 out[size_t(oy0 + o) * size_t(w) + ox0 + xi] = sum;
             }
         }
@@ -265,10 +278,6 @@ __global__ __launch_bounds__(64 * kWaves) void poly_fused_kernel(const float *__
 // to the two-pass form, with each staged row read by X taps once (no
 // temporary in LDS, no workgroup barriers).  Source rows are fetched eight
 // row pairs ahead of their use.
-struct SlideTaps {
-    float kx[kMaxWin], ky[kMaxWin];
-};
-
 template <int kWin>
 __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict__ src, int W, int H,
                                                          float *__restrict__ dst, int w, int h, int lx0, int ly0,
@@ -279,7 +288,9 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
     constexpr int kSeg = 2 * 63 + kWin;  // staged source columns of 64 outputs
     constexpr int kSlots = (kSeg + 63) / 64;
     __shared__ __align__(16) float s_row[4][2][kSlots * 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // The wavefront index as a scalar: every job coordinate and source row
+    // below is then wave-uniform (scalar registers, scalar wrap of rows).
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Workgroups are dispatched round-robin over the eight XCDs, each with
     // its own L2: when the grid divides by eight, XCD k takes the k-th
     // contiguous eighth of the jobs (column tile fastest, then strip), so
@@ -317,9 +328,11 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
             for (int k = 0; k < kSlots; k++) rr[i][k] = row[gx[k]];
         }
     };
+    // Fetches are unconditional (rows past the strip's last read the last
+    // row again, from cache): the same number of loads on every path keeps
+    // each pair's wait at the loads of its own registers instead of all.
 #pragma unroll
-    for (int d = 0; d < kDepth; d++)
-        if (2 * d <= last) fetch(r[d], 2 * d);
+    for (int d = 0; d < kDepth; d++) fetch(r[d], 2 * d);
 
     float acc[K];
 #pragma unroll
@@ -348,7 +361,7 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
             buf1[lane + 64 * k] = rr[1][k];
         }
         ntx_wave_sync();
-        if (2 * (p + kDepth) <= last) fetch(rr, 2 * (p + kDepth));
+        fetch(rr, 2 * (p + kDepth));
         const float xe = xtaps(buf0);
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] += taps.ky[2 * (K - 1 - k)] * xe;
@@ -382,9 +395,10 @@ __global__ __launch_bounds__(256) void poly_slide_kernel(const float *__restrict
 // The tail kernel keeps, for every level whose tables fit (kTailTaps
 // entries over all levels), each tap as (weight, source offset) with the
 // wrap already applied: the X taps' offsets within a source row, the Y
-// taps' offsets of the temporary's rows.  Other levels read the tables from
-// HBM and wrap per tap.
-constexpr int kTailTaps = 4096;
+// taps' offsets of the temporary's rows (packed on the host, MipTail::packed).
+// Other levels read the tables from HBM and wrap per tap.
+using ntx::kTailTaps;
+constexpr int kTailThreads = 512;
 
 // A workgroup barrier that orders LDS only.  The tail kernel's global stores
 // are never read back by the kernel, so its barriers need not wait for them
@@ -468,38 +482,34 @@ __device__ __forceinline__ void tail_poly_level(const float *cur, float *nxt, fl
     }
 }
 
-__global__ __launch_bounds__(512) void mip_tail_kernel(const float *__restrict__ src, ntx::MipTail tail, int wrap)
+__global__ __launch_bounds__(kTailThreads) void mip_tail_kernel(const float *__restrict__ src, ntx::MipTail tail,
+                                                                int wrap)
 {
     __shared__ float s_a[ntx::kTailSrc];
     __shared__ float s_b[ntx::kTailSrc / 2];
     __shared__ float s_t[ntx::kTailSrc];
     __shared__ float2 s_k[kTailTaps];
-    const int t = threadIdx.x, nt = blockDim.x, c = blockIdx.x;
+    const int t = threadIdx.x, nt = kTailThreads, c = blockIdx.x;
+    // The source plane and the packed taps into LDS: every load issued
+    // before the first store, so the kernel waits for one memory latency.
     {
+        constexpr int kSrcPer = ntx::kTailSrc / kTailThreads, kTapPer = kTailTaps / kTailThreads;
         const float *plane = src + size_t(c) * size_t(tail.W) * size_t(tail.H);
-        for (int i = t; i < tail.W * tail.H; i += nt) s_a[i] = plane[i];
-    }
-    // Every level's taps that fit, in level order, packed once.
-    if (!tail.box) {
-        int W = tail.W, H = tail.H, ko = 0;
-        for (int k = 0; k < tail.levels; k++) {
-            const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
-            const int wx = tail.wx[k], wy = tail.wy[k];
-            if (ko + w * wx + h * wy <= kTailTaps) {
-                for (int i = t; i < w * wx; i += nt) {
-                    const int x = i / wx, j = i - x * wx;
-                    s_k[ko + i] = make_float2(tail.kx[k][i], __int_as_float(wrap_index(tail.lx[k][x] + j, W, wrap)));
-                }
-                for (int i = t; i < h * wy; i += nt) {
-                    const int y = i / wy, j = i - y * wy;
-                    s_k[ko + w * wx + i] =
-                        make_float2(tail.ky[k][i], __int_as_float(wrap_index(tail.ly[k][y] + j, H, wrap) * w));
-                }
-                ko += w * wx + h * wy;
-            }
-            W = w;
-            H = h;
-        }
+        const int n = tail.W * tail.H, np = tail.box ? 0 : tail.npacked;
+        const float2 *packed = reinterpret_cast<const float2 *>(tail.packed);
+        float v[kSrcPer];
+        float2 q[kTapPer];
+#pragma unroll
+        for (int r = 0; r < kSrcPer; r++) v[r] = plane[min(t + r * nt, n - 1)];
+#pragma unroll
+        for (int r = 0; r < kTapPer; r++)
+            if (t + r * nt < np) q[r] = packed[t + r * nt];
+#pragma unroll
+        for (int r = 0; r < kSrcPer; r++)
+            if (t + r * nt < n) s_a[t + r * nt] = v[r];
+#pragma unroll
+        for (int r = 0; r < kTapPer; r++)
+            if (t + r * nt < np) s_k[t + r * nt] = q[r];
     }
     lds_barrier();
     int W = tail.W, H = tail.H, ko = 0;
@@ -657,22 +667,26 @@ hipError_t launch_mip_tail(const float *d_src, const MipTail &tail, int wrap, hi
 {
     if (tail.levels < 1 || tail.levels > kTailMaxLevels || size_t(tail.W) * size_t(tail.H) > size_t(kTailSrc))
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mip_tail_kernel, dim3(4), dim3(512), 0, stream, d_src, tail, wrap);
+    if (!tail.box && (tail.npacked < 0 || tail.npacked > kTailTaps || (tail.npacked > 0 && !tail.packed)))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mip_tail_kernel, dim3(4), dim3(kTailThreads), 0, stream, d_src, tail, wrap);
     return hipGetLastError();
 }
 int poly_tile_y() { return kTY; }
 
 // Strip height of the sliding-window kernel for a W x H source: as tall as
-// keeps eight wavefronts per compute unit busy (each strip re-reads the
-// window's overlap with its neighbour), 0 when that takes strips under 32
-// rows: a wavefront then walks too few rows to hide its row fetches, and
+// keeps sixteen wavefronts per compute unit busy (each strip re-reads the
+// window's overlap with its neighbour; with fewer wavefronts each one's
+// serial walk down its rows sets the time), 0 when that takes strips under
+// 16 rows: a wavefront then walks too few rows to hide its row fetches, and
 // the tiled kernel, which issues a whole tile's loads at once, is faster.
+// (Kaiser(3) at 4096^2 and 2048^2 on 256 compute units: 64 and 16 rows.)
 int poly_slide_rows(int W, int H, int cus)
 {
     const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
     const size_t ctiles = size_t((w + 63) / 64);
-    for (int rows = 64; rows >= 32; rows /= 2)
-        if (size_t(4) * ctiles * size_t((h + rows - 1) / rows) >= size_t(8) * size_t(cus)) return rows;
+    for (int rows = 64; rows >= 16; rows /= 2)
+        if (size_t(4) * ctiles * size_t((h + rows - 1) / rows) >= size_t(16) * size_t(cus)) return rows;
     return 0;
 }
 
@@ -719,7 +733,32 @@ hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, con
     if (pair_x && wx == wy && wx == 9) kern = poly_fused_kernel<true, 9>;
     if (pair_x && wx == wy && wx == 5) kern = poly_fused_kernel<true, 5>;
     hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, d_kx, d_lx, wx, d_ky, d_ly,
-                       wy, wrap);
+                       wy, wrap, SlideTaps{}, 0, 0);
+    return hipGetLastError();
+}
+
+bool poly_fused_uniform_window(int win) { return win == 13 || win == 9 || win == 5; }
+
+// The fused kernel with uniform taps (exact halving, the same `win` weights
+// kx / ky for every output, first taps lx0 + 2 i and ly0 + 2 i).
+hipError_t launch_poly_fused_uniform(const float *d_src, int W, int H, float *d_dst, const float *kx, int lx0,
+                                     const float *ky, int ly0, int win, int wrap, hipStream_t stream)
+{
+    if (!poly_fused_uniform_window(win)) return hipErrorInvalidValue;
+    const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+    const int tx = (w + kTX - 1) / kTX, ty = (h + kTY - 1) / kTY;
+    const int split = tx * ty < 1024 ? 4 : 1;
+    const dim3 grid(unsigned(tx * ty), 1, split);
+    SlideTaps taps{};
+    for (int j = 0; j < win; j++) {
+        taps.kx[j] = kx[j];
+        taps.ky[j] = ky[j];
+    }
+    auto kern = poly_fused_kernel<true, 13, true>;
+    if (win == 9) kern = poly_fused_kernel<true, 9, true>;
+    if (win == 5) kern = poly_fused_kernel<true, 5, true>;
+    hipLaunchKernelGGL(kern, grid, dim3(64 * kWaves), 0, stream, d_src, W, H, d_dst, w, h, tx, nullptr, nullptr, win,
+                       nullptr, nullptr, win, wrap, taps, lx0, ly0);
     return hipGetLastError();
 }
 

@@ -194,6 +194,8 @@ ntx_status poly_tables(ntx_hip_context *ctx, int filter, float width, const floa
         d.uniform = std::memcmp(k.weights + size_t(i) * k.window, k.weights, sizeof(float) * size_t(k.window)) == 0;
     d.left0 = left[0];
     if (d.uniform) std::memcpy(d.w0, k.weights, sizeof(float) * size_t(k.window));
+    d.host_weights.assign(k.weights, k.weights + size_t(k.window) * dst);
+    d.host_left = left;
     hipError_t e = hipMalloc(&d.weights, sizeof(float) * size_t(k.window) * dst);
     if (e == hipSuccess) e = hipMalloc(&d.left, sizeof(int) * dst);
     if (e == hipSuccess) e = hipMemcpy(d.weights, k.weights, sizeof(float) * size_t(k.window) * dst, hipMemcpyHostToDevice);
@@ -285,6 +287,7 @@ NTX_API ntx_status ntx_hip_destroy(ntx_hip_context *ctx)
         (void)hipFree(kv.second.weights);
         (void)hipFree(kv.second.left);
     }
+    for (auto &kv : ctx->tail_taps) (void)hipFree(kv.second.packed);
     for (hipStream_t st : ctx->side)
         if (st) (void)hipStreamSynchronize(st);
     for (DevBuf *b : { &ctx->in, &ctx->out, &ctx->tmp, &ctx->dst, &ctx->bucket, &ctx->metric, &ctx->chain })
@@ -572,6 +575,12 @@ static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int fi
                 return launch_poly_slide(d_rgba, W, H, d_out, kx->w0, kx->left0, ky->w0, ky->left0, win, wrap, rows,
                                          stream);
             });
+        } else if (kx->uniform && ky->uniform && ky->window == win && poly_fused_uniform_window(win) &&
+                   poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
+            e = timed_on(ctx, stream, NTX_KERNEL_POLY_FUSED, [&] {
+                return launch_poly_fused_uniform(d_rgba, W, H, d_out, kx->w0, kx->left0, ky->w0, ky->left0, win, wrap,
+                                                 stream);
+            });
         } else if (poly_fused_fits(kx->span_x, ky->span_y, kx->window, ky->window)) {
             e = timed_on(ctx, stream, NTX_KERNEL_POLY_FUSED, [&] {
                 return launch_poly_fused(d_rgba, W, H, d_out, kx->weights, kx->left, kx->window, kx->stride2, ky->weights,
@@ -591,6 +600,71 @@ static ntx_status downsample_on(ntx_hip_context *ctx, hipStream_t stream, int fi
         }
     }
     if (e != hipSuccess) return fail(ctx, e, "downsample launch");
+    return NTX_OK;
+}
+
+// FloatImage.h:298-330 wrap helpers, as the kernels apply them.
+static int host_wrap_index(int x, int n, int wm)
+{
+    if (x >= 0 && x < n) return x;
+    if (wm == NTX_WRAP_CLAMP) return x < 0 ? 0 : n - 1;
+    if (wm == NTX_WRAP_REPEAT) return x >= 0 ? x % n : (x + 1) % n + n - 1;
+    if (n == 1) return 0;
+    x = std::abs(x);
+    while (x >= n) x = std::abs(n + n - x - 2);
+    return x;
+}
+
+// The packed taps of a tail (MipTail::packed) from source W x H through
+// `levels` halvings, built once per filter, size and wrap and kept on the
+// device, so the tail kernel copies them into LDS with one coalesced pass
+// instead of resolving each tap from the tables.
+static ntx_status tail_taps(ntx_hip_context *ctx, int filter, float width, const float *params, int wrap, int W0, int H0,
+                            int levels, TailTaps **out)
+{
+    const float alpha = (filter == NTX_FILTER_KAISER && params) ? params[0] : 4.0f;
+    const float stretch = (filter == NTX_FILTER_KAISER && params) ? params[1] : 1.0f;
+    const TailKey key(filter, fbits(width), fbits(alpha), fbits(stretch), W0, H0, levels, wrap);
+    auto it = ctx->tail_taps.find(key);
+    if (it != ctx->tail_taps.end()) { *out = &it->second; return NTX_OK; }
+    std::vector<float> pk;
+    int W = W0, H = H0;
+    for (int k = 0; k < levels; k++) {
+        const int w = W / 2 > 1 ? W / 2 : 1, h = H / 2 > 1 ? H / 2 : 1;
+        PolyDev *kx = nullptr, *ky = nullptr;
+        ntx_status s = poly_tables(ctx, filter, width, params, W, w, &kx);
+        if (!s) s = poly_tables(ctx, filter, width, params, H, h, &ky);
+        if (s) return s;
+        const int wx = kx->window, wy = ky->window;
+        if (int(pk.size() / 2) + w * wx + h * wy <= kTailTaps) {
+            auto push = [&](float weight, int off) {
+                float bits;
+                std::memcpy(&bits, &off, sizeof bits);
+                pk.push_back(weight);
+                pk.push_back(bits);
+            };
+            for (int x = 0; x < w; x++)
+                for (int j = 0; j < wx; j++)
+                    push(kx->host_weights[size_t(x) * wx + j], host_wrap_index(kx->host_left[x] + j, W, wrap));
+            for (int y = 0; y < h; y++)
+                for (int j = 0; j < wy; j++)
+                    push(ky->host_weights[size_t(y) * wy + j], host_wrap_index(ky->host_left[y] + j, H, wrap) * w);
+        }
+        W = w;
+        H = h;
+    }
+    TailTaps t;
+    t.count = int(pk.size() / 2);
+    if (t.count > 0) {
+        hipError_t e = hipMalloc(&t.packed, sizeof(float) * pk.size());
+        if (e == hipSuccess) e = hipMemcpy(t.packed, pk.data(), sizeof(float) * pk.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (t.packed) (void)hipFree(t.packed);
+            return fail(ctx, e, "tail tap upload");
+        }
+    }
+    auto res = ctx->tail_taps.emplace(key, t);
+    *out = &res.first->second;
     return NTX_OK;
 }
 
@@ -623,6 +697,13 @@ static ntx_status downsample_chain_on(ntx_hip_context *ctx, hipStream_t stream, 
                 tail.ky[j] = ky->weights;
                 tail.ly[j] = ky->left;
                 tail.wy[j] = ky->window;
+            }
+            if (!box) {
+                TailTaps *tt = nullptr;
+                const ntx_status s = tail_taps(ctx, filter, width, params, wrap, lw[k - 1], lh[k - 1], rest, &tt);
+                if (s) return s;
+                tail.packed = tt->packed;
+                tail.npacked = tt->count;
             }
             const hipError_t e = timed_on(ctx, stream, NTX_KERNEL_MIP_TAIL,
                                           [&] { return launch_mip_tail(d_levels[k - 1], tail, wrap, stream); });

@@ -30,12 +30,19 @@ struct Bc1Image {
 
 constexpr int kTailMaxLevels = 16;
 constexpr int kTailSrc = 8192;
+// Packed taps of the tail's levels: (weight, wrapped source offset as int
+// bits) pairs, for each level in order whose X and Y taps still fit in
+// kTailTaps entries: the w * wx X taps (offset within a source row), then
+// the h * wy Y taps (offset of a temporary row, already times w).
+constexpr int kTailTaps = 4096;
 struct MipTail {
     int W, H, levels, box;
     float *dst[kTailMaxLevels];
     const float *kx[kTailMaxLevels], *ky[kTailMaxLevels];
     const int *lx[kTailMaxLevels], *ly[kTailMaxLevels];
     int wx[kTailMaxLevels], wy[kTailMaxLevels];
+    const float *packed;  // 2 * npacked floats, built on the host
+    int npacked;
 };
 
 // Tables shared by host set-up and kernels.  Layout is fixed here so the host

@@ -75,6 +75,11 @@ hipError_t launch_poly_slide(const float *d_src, int W, int H, float *d_dst, con
 hipError_t launch_poly_fused(const float *d_src, int W, int H, float *d_dst, const float *d_kx, const int *d_lx, int wx,
                              bool pair_x, const float *d_ky, const int *d_ly, int wy, int wrap, hipStream_t stream);
 bool poly_fused_fits(int max_sw, int max_sh, int wx, int wy);
+// The fused kernel without tables, for exact halving with uniform taps
+// (PolyDev::uniform) of a window poly_fused_uniform_window accepts.
+bool poly_fused_uniform_window(int win);
+hipError_t launch_poly_fused_uniform(const float *d_src, int W, int H, float *d_dst, const float *kx, int lx0,
+                                     const float *ky, int ly0, int win, int wrap, hipStream_t stream);
 
 int poly_tile_x();
 int poly_tile_y();
@@ -123,9 +128,18 @@ struct PolyDev {
     bool uniform = false;
     int left0 = 0;
     float w0[16] = {};
+    // Host copies of the weights and first taps (the tail's packed taps).
+    std::vector<float> host_weights;
+    std::vector<int> host_left;
 };
 
 using PolyKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int>;
+// Tail taps: filter, width, alpha, stretch bits, source W, H, levels, wrap.
+using TailKey = std::tuple<int, uint32_t, uint32_t, uint32_t, int, int, int, int>;
+struct TailTaps {
+    float *packed = nullptr;  // device, 2 floats per tap
+    int count = 0;
+};
 
 }  // namespace ntx
 
@@ -160,6 +174,7 @@ struct ntx_hip_context {
     bool chain_counts_zero = false;
     std::vector<size_t> chain_layout;  // offsets (ints) of the regions with tallies in the last complete run
     std::map<ntx::PolyKey, ntx::PolyDev> poly;
+    std::map<ntx::TailKey, ntx::TailTaps> tail_taps;
     std::map<std::string, ntx_batch_tuning> batch_tuning;  // autotuned, keyed by batch shape list
     std::string err;
     // per-kernel event timing (ntx_hip_profile_*)

@@ -274,19 +274,33 @@ def test_polyphase_large_and_odd(gpu, oracle, size, filt):
     (N.FILTER_KAISER, 1.5, (4.0, 1.0), N.ALPHA_NONE), (N.FILTER_KAISER, 2.5, (4.0, 1.0), N.ALPHA_NONE),
     (N.FILTER_KAISER, 3.5, (4.0, 1.0), N.ALPHA_NONE), (N.FILTER_KAISER, 3.75, (4.0, 1.0), N.ALPHA_NONE)])
 @pytest.mark.parametrize("size", [(2048, 2048), (2570, 1640), (1024, 4096), (131072, 2), (2, 131072), (256, 130),
-                                  (2, 258), (514, 2)])
+                                  (2, 258), (514, 2), (1000, 500)])
 def test_halving_windows_match_oracle(gpu, oracle, filt, width, params, alpha_mode, size):
     """Exact halving (even extents) with windows of 3, 5, 7, 9, 11, 13 and 15
     taps, the same taps for every output, in every wrap mode: on sources of
     2048^2 and up the sliding-window kernel (partial column tiles and a
     partial last strip at 2570 x 1640, rows and columns of two texels whose
-    taps all wrap), on the smaller ones and for the even 16-tap window the
-    tiled kernel."""
+    taps all wrap), on the smaller ones the tiled kernel (with the taps
+    passed by value for windows of 5, 9 and 13; partial tiles at 1000 x 500)
+    and for the even 16-tap window the tiled kernel with tables."""
     W, H = size
     planar = np.random.default_rng(W * 3 + H).random(4 * W * H, dtype=np.float32)
     for wrap in (N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR):
         a = gpu.downsample(filt, planar, W, H, width=width, params=params, wrap=wrap, alpha_mode=alpha_mode)
         b = oracle.downsample(filt, width, params, wrap, planar, W, H)
+        assert np.array_equal(_bits(a), _bits(b)), (size, wrap)
+
+
+@pytest.mark.parametrize("size", [(4096, 2048), (4096, 4096)])
+def test_halving_strip_heights_match_oracle(gpu, oracle, size):
+    """The sliding-window kernel at the strip heights the larger sources
+    take (32 output rows at 4096 x 2048, 64 at 4096^2), Kaiser(3) in every
+    wrap mode."""
+    W, H = size
+    planar = np.random.default_rng(W + H).random(4 * W * H, dtype=np.float32)
+    for wrap in (N.WRAP_CLAMP, N.WRAP_REPEAT, N.WRAP_MIRROR):
+        a = gpu.downsample(N.FILTER_KAISER, planar, W, H, width=3.0, params=(4.0, 1.0), wrap=wrap)
+        b = oracle.downsample(N.FILTER_KAISER, 3.0, (4.0, 1.0), wrap, planar, W, H)
         assert np.array_equal(_bits(a), _bits(b)), (size, wrap)
 
 
