@@ -108,7 +108,7 @@ enum {
     NTX_KERNEL_UNPACK = 7,
     NTX_KERNEL_POLY_FUSED = 8, /* X and Y polyphase taps in one pass through LDS */
     NTX_KERNEL_BC3_CLASSIFY = 9, /* BC3 pass 1: alpha halves, single-colour blocks, bucketing by colour count */
-    NTX_KERNEL_BATCH = 10,       /* one batch run: counter reset + persistent kernel + two-pass BC1 of its Level8 levels */
+    NTX_KERNEL_BATCH = 10,       /* one batch run: mip stages + persistent compression kernel + two-pass BC1 */
     NTX_KERNEL_BC1_MLP = 11,     /* BC1 from network proposals + least squares */
     NTX_KERNEL_BC1_LIST = 12,    /* BC1 Level8 over a block list (the learned form's fallback), one block per 16-lane group */
     NTX_KERNEL_MLP_TRAIN = 13,   /* network training step (forward, backward, Adam) */
@@ -349,12 +349,15 @@ NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_r
  * Compressor::compress call (Context.cpp:146-190, 486-516).  A batch is many
  * textures of any sizes, each with a full or partial mip chain
  * (Surface::buildNextMipmap semantics, Surface.cpp:1335-1406) and every level
- * block-compressed (Compressor::compress semantics), run as one persistent
- * kernel launch captured in a HIP graph.  Every output byte equals what the
- * per-image entry points above produce for the same level.
+ * block-compressed (Compressor::compress semantics), run as one HIP graph:
+ * a launch per mip stage (level s + 1 of every texture), one persistent
+ * launch over the compression items, and the two-pass BC1 encoder over every
+ * BC1 level at icbc Level8.  Every output byte equals what the per-image
+ * entry points above produce for the same level.
  *
- * Formats: BC1 at every quality (icbc Level1 / Level8 / Level9, one
- * block per 16-lane group), BC4 and BC5 (Fastest/Normal).  BC3 / BC3n, and
+ * Formats: BC1 at every quality (icbc Level8 one block per lane through the
+ * two-pass encoder, Level1 / Level9 one block per 16-lane group), BC4 and
+ * BC5 (Fastest/Normal).  BC3 / BC3n, and
  * BC4 / BC5 at Production / Highest, return NTX_ERR_UNSUPPORTED here; use
  * ntx_hip_compress_chain_device or ntx_hip_compress_images_device for them. */
 typedef struct {
@@ -371,8 +374,9 @@ typedef struct {
     int wrap;              /* NTX_WRAP_* */
 } ntx_texture_desc;
 
-/* Work decomposition of the persistent kernel; ntx_hip_batch_autotune
- * chooses it by timing candidates on the batch itself. */
+/* Work decomposition of the mip stages and the persistent compression
+ * kernel; ntx_hip_batch_autotune chooses it by timing candidates on the
+ * batch itself. */
 typedef struct {
     int bc1_blocks_per_item;   /* multiple of 16 */
     int channel_jobs_per_item; /* multiple of 256 (a BC5 block is two jobs) */
@@ -394,8 +398,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
 NTX_API ntx_status ntx_hip_batch_destroy(ntx_hip_batch *batch);
 /* Enqueues one run of the whole batch on the context's stream. */
 NTX_API ntx_status ntx_hip_batch_submit(ntx_hip_batch *batch);
-/* Waits for the submitted runs; NTX_ERR_DEVICE if the kernel reported a
- * dependency wait that did not complete. */
+/* Waits for the submitted runs. */
 NTX_API ntx_status ntx_hip_batch_wait(ntx_hip_batch *batch);
 /* Number of levels texture `tex` has in this batch. */
 NTX_API int ntx_hip_batch_levels(const ntx_hip_batch *batch, int tex);

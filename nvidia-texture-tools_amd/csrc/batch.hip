@@ -1,15 +1,13 @@
-// batch.hip -- one persistent kernel for a whole batch of textures: every
-// texture's mip chain (fastDownSample or polyphase resize) and the BC1 / BC4 /
-// BC5 compression of every level, taken from one work list.
+// batch.hip -- the kernels of a batch of textures: every texture's mip chain
+// (fastDownSample or polyphase resize) and the BC1 / BC4 / BC5 compression of
+// every level, from one work list.
 //
-// Workgroups take items from a queue in list order (one atomic per item).
-// Each item reads one level; levels produced inside the launch are guarded by
-// a ready counter that every mip item writing the level increments once its
-// outputs are globally visible.  The host orders the list so an item only
-// ever depends on items before it: the earliest unfinished item in the list
-// always has its inputs complete, so the queue drains whatever the number of
-// resident workgroups.  A wait that exceeds its poll limit sets a status bit
-// and continues, so every wave reaches the end of the kernel.
+// Mips run stage by stage: launch s makes level s + 1 of every texture from
+// level s (batch_mip_kernel, one workgroup per item), so each launch's
+// sources are complete when it starts and hundreds of textures' tiles fill
+// the device together.  The compression items of every level then run in one
+// persistent launch (batch_kernel): workgroups take items from a queue in
+// list order (one atomic per item), largest levels first.
 //
 // Per-item arithmetic is the per-image kernels' own device code (bc1_block.hpp,
 // channel_block.hpp, mip_taps.hpp), so results are bit-identical to them.
@@ -25,9 +23,7 @@ using ntx::BatchItem;
 using ntx::BatchLevel;
 
 // A descriptor every lane reads at the same address, copied into scalar
-// registers.  The loads themselves are vector loads: descriptors are read
-// after other workgroups' writes in the same launch, which the scalar cache
-// does not observe.
+// registers (vector loads, then the first lane's value).
 template <typename T>
 __device__ __forceinline__ T uniform_copy(const T *p)
 {
@@ -46,20 +42,6 @@ template <typename T>
 __device__ __forceinline__ T *global_ptr(T *p)
 {
     return (T *)(__attribute__((address_space(1))) T *)p;
-}
-
-__device__ __forceinline__ void wait_ready(int *counter, int target, int spin_limit, int *status)
-{
-    // Gives up after spin_limit polls, or at once when another wait already
-    // gave up (the launch is failing; later waits need not spin as well).
-    int polls = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++polls > spin_limit || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            atomicOr(status, 1);
-            return;
-        }
-        __builtin_amdgcn_s_sleep(4);
-    }
 }
 
 // LDS for a polyphase tile in the separable form: the tile's source region
@@ -207,6 +189,22 @@ __device__ void channel_item(const BatchItem &it, const BatchLevel &lv, int t)
     }
 }
 
+// The mip items of one stage (their sources complete before the launch):
+// workgroup b takes items b, b + grid, ...
+__global__ __launch_bounds__(256) void batch_mip_kernel(BatchArgs a)
+{
+    __shared__ float s_src[kMipSrc], s_tmp[kMipTmp], s_taps[kMipTaps];
+    __shared__ int s_first[512];
+    const int t = threadIdx.x;
+    for (int idx = blockIdx.x; idx < a.nitems; idx += gridDim.x) {
+        const BatchItem it = uniform_copy(a.items + idx);
+        const BatchLevel lv = uniform_copy(a.levels + it.level);
+        mip_item(a, it, lv, t, s_src, s_tmp, s_taps, s_first);
+        __syncthreads();  // the next item's taps overwrite this one's
+    }
+}
+
+// The compression items, every level already complete.
 // kWaves: waves per SIMD the register allocation is held to (0: the
 // compiler's choice); the autotuner picks among the instances.  kTiers: the
 // batch holds BC1 levels at icbc levels other than 8 (Fastest, Production),
@@ -216,38 +214,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves > 0 
 void batch_kernel(BatchArgs a)
 {
     __shared__ ntx_bc1::Bc1Shared sh;
-    __shared__ float s_src[kMipSrc], s_tmp[kMipTmp], s_taps[kMipTaps];
-    __shared__ int s_first[512];
     __shared__ int s_item;
     const int t = threadIdx.x;
     ntx_bc1::load_shared_tables(sh, a.tab, t, 256);
     for (;;) {
         __syncthreads();  // the previous item is finished by every thread
-        if (t == 0) {
-            const int it = atomicAdd(a.head, 1);
-            if (it < a.nitems) {
-                const BatchLevel &lv = a.levels[a.items[it].level];
-                if (lv.ready >= 0) wait_ready(a.counters + lv.ready, lv.target, a.spin_limit, a.status);
-            }
-            s_item = it;
-        }
+        if (t == 0) s_item = atomicAdd(a.head, 1);
         __syncthreads();
         const int idx = __builtin_amdgcn_readfirstlane(s_item);
         if (idx >= a.nitems) break;
         const BatchItem it = uniform_copy(a.items + idx);
         const BatchLevel lv = uniform_copy(a.levels + it.level);
-        if (lv.ready >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (it.kind == ntx::kItemMip) {
-            mip_item(a, it, lv, t, s_src, s_tmp, s_taps, s_first);
-            // Every thread makes its stores visible device-wide before the
-            // workgroup counts the item as done.
-            __threadfence();
-            __syncthreads();
-            if (t == 0) {
-                const int ready = __builtin_amdgcn_readfirstlane(a.levels[lv.next].ready);
-                __hip_atomic_fetch_add(a.counters + ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        } else if (it.kind == ntx::kItemBc1) {
+        if (it.kind == ntx::kItemBc1) {
             if constexpr (kTiers) {
                 // One cluster-fit instance for levels 8 and 9 (two would
                 // spill): level 9's refinement is switched per level.
@@ -264,8 +242,8 @@ void batch_kernel(BatchArgs a)
     }
 }
 
-// Clears the ready counters, the queue position and the status word (a
-// kernel rather than a memset node, so the pair can be captured in a graph).
+// Clears the queue position (a kernel rather than a memset node, so the run
+// can be captured in a graph).
 __global__ __launch_bounds__(256) void batch_reset_kernel(int *p, int n)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -298,6 +276,14 @@ int batch_kernel_variants() { return 3; }
 hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream)
 {
     hipLaunchKernelGGL(batch_kernel_of(variant, tiers), dim3(workgroups), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+// One stage of mips: args.items / args.nitems are the stage's items.
+hipError_t launch_batch_mips(const BatchArgs &args, hipStream_t stream)
+{
+    if (args.nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_mip_kernel, dim3(unsigned(args.nitems)), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // batch_api.cpp -- the batch entry points of include/ntx_hip.h: planning a
-// batch of textures into level buffers, a work list and a captured graph of
-// one persistent launch (batch.hip), and the autotuner that picks the work
+// batch of textures into level buffers, a work list and a captured graph
+// (the mip stages, the persistent compression launch and the multi-image
+// BC1 encoder; batch.hip, bc1.hip), and the autotuner that picks the work
 // decomposition by timing it.
 #include <algorithm>
 #include <cstdio>
@@ -15,6 +16,7 @@
 namespace ntx {
 hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream);
 hipError_t launch_batch_reset(int *p, int n, hipStream_t stream);
+hipError_t launch_batch_mips(const BatchArgs &args, hipStream_t stream);
 hipError_t batch_resident_workgroups(int device, int variant, bool tiers, int *count);
 int batch_kernel_variants();
 }  // namespace ntx
@@ -30,9 +32,11 @@ struct ntx_hip_batch {
     ntx::BatchLevel *d_levels = nullptr;
     ntx::BatchItem *d_items = nullptr;
     size_t items_cap = 0;
-    int *d_sync = nullptr;               // ready counters, queue head, status
-    int ncounters = 0;
+    int *d_sync = nullptr;               // queue head of the compression launch
     int nitems = 0;
+    // The work list: the mip items of stage s at [stage_first[s],
+    // stage_first[s + 1]), then the compression items from stage_first.back().
+    std::vector<int> stage_first;
     bool bc1_tiers = false;              // some BC1 level is not at icbc level 8
     int workgroups = 0;
     ntx_batch_tuning tuning{};
@@ -56,7 +60,6 @@ namespace {
 using ntx::BatchItem;
 using ntx::BatchLevel;
 
-constexpr int kSpinLimit = 1 << 22;  // polls (each a sleep plus an atomic read): seconds, far beyond any real wait
 
 int count_mipmaps(int w, int h)
 {
@@ -108,10 +111,10 @@ void release_graph(ntx_hip_batch *b)
     b->graph = nullptr;
 }
 
-// The work list for the current tuning.  Stage s holds the mip items that
-// make level s + 1 of every texture, then the compression items of level s
-// (BC1 before channel jobs), each group in decreasing texture size, so every
-// item depends only on items earlier in the list.
+// The work list for the current tuning: for each stage s the mip items that
+// make level s + 1 of every texture (one launch per stage), then the
+// compression items of every level (one launch), stage by stage with BC1
+// before channel jobs, each group in decreasing texture size.
 ntx_status build_items(ntx_hip_batch *b)
 {
     ntx_hip_context *ctx = b->ctx;
@@ -126,18 +129,20 @@ ntx_status build_items(ntx_hip_batch *b)
     int stages = 0;
     for (int t = 0; t < n; t++) stages = std::max(stages, b->first[t + 1] - b->first[t]);
     std::vector<BatchItem> items;
-    for (auto &lv : b->levels) lv.target = 0;
+    b->stage_first.clear();
     for (int s = 0; s < stages; s++) {
+        b->stage_first.push_back(int(items.size()));
         for (int t : order) {
             if (s + 1 >= b->first[t + 1] - b->first[t]) continue;
             const int li = b->first[t] + s;
             const BatchLevel &dst = b->levels[li + 1];
             const int tiles = ((dst.w + tw - 1) / tw) * ((dst.h + th - 1) / th);
-            for (int k = 0; k < tiles; k += tu.mip_tiles_per_item) {
+            for (int k = 0; k < tiles; k += tu.mip_tiles_per_item)
                 items.push_back({ ntx::kItemMip, li, k, std::min(tu.mip_tiles_per_item, tiles - k) });
-                b->levels[li + 1].target++;
-            }
         }
+    }
+    b->stage_first.push_back(int(items.size()));
+    for (int s = 0; s < stages; s++) {
         for (int pass = 0; pass < 2; pass++) {
             for (int t : order) {
                 if (s >= b->first[t + 1] - b->first[t]) continue;
@@ -169,7 +174,7 @@ ntx_status build_items(ntx_hip_batch *b)
     int resident = 0;
     NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, tu.kernel_variant, b->bc1_tiers, &resident));
     int wgs = tu.workgroups > 0 ? std::min(tu.workgroups, resident) : resident;
-    wgs = std::max(1, std::min(wgs, std::max(b->nitems, 1)));
+    wgs = std::max(1, std::min(wgs, std::max(b->nitems - b->stage_first.back(), 1)));
     b->workgroups = wgs;
     return NTX_OK;
 }
@@ -180,21 +185,33 @@ ntx::BatchArgs make_args(const ntx_hip_batch *b)
     a.items = b->d_items;
     a.levels = b->d_levels;
     a.tab = b->ctx->d_tab;
-    a.counters = b->d_sync;
-    a.head = b->d_sync + b->ncounters;
-    a.status = b->d_sync + b->ncounters + 1;
+    a.head = b->d_sync;
     a.nitems = b->nitems;
     a.tile_shift = log2i(b->tuning.mip_tile_width);
-    a.spin_limit = kSpinLimit;
     return a;
 }
 
-// The two launches of one run, enqueued on the context's stream.
+// The launches of one run, enqueued on the context's stream: the queue
+// reset, the mip stages, the compression items, the multi-image BC1 passes.
 hipError_t enqueue_run(ntx_hip_batch *b)
 {
     hipStream_t s = b->ctx->stream;
-    hipError_t e = ntx::launch_batch_reset(b->d_sync, b->ncounters + 2, s);
-    if (e == hipSuccess && b->nitems > 0) e = ntx::launch_batch(make_args(b), b->tuning.kernel_variant, b->bc1_tiers, b->workgroups, s);
+    hipError_t e = ntx::launch_batch_reset(b->d_sync, 1, s);
+    const ntx::BatchArgs all = make_args(b);
+    const int nstages = int(b->stage_first.size()) - 1;
+    for (int k = 0; e == hipSuccess && k < nstages; k++) {
+        ntx::BatchArgs a = all;
+        a.items = b->d_items + b->stage_first[k];
+        a.nitems = b->stage_first[k + 1] - b->stage_first[k];
+        e = ntx::launch_batch_mips(a, s);
+    }
+    const int c0 = b->stage_first.back();
+    if (e == hipSuccess && b->nitems > c0) {
+        ntx::BatchArgs a = all;
+        a.items = b->d_items + c0;
+        a.nitems = b->nitems - c0;
+        e = ntx::launch_batch(a, b->tuning.kernel_variant, b->bc1_tiers, b->workgroups, s);
+    }
     if (e == hipSuccess && b->bc1_imgs > 0) {
         int *counts = b->d_bc1_bucket, *lists = counts + ntx::bc3_lists_offset();
         e = ntx::launch_bc1_classify_multi(b->d_bc1_imgs, b->d_bc1_map, b->bc1_chunks, b->ctx->d_tab, counts, lists,
@@ -265,12 +282,6 @@ ntx_status time_runs(ntx_hip_batch *b, int reps, float *ms)
         NTX_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
         if (r == 0 || t < best) best = t;
     }
-    int status = 0;
-    NTX_HIP(ctx, hipMemcpy(&status, b->d_sync + b->ncounters + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (status) {
-        ctx->err = "batch: a dependency wait did not complete";
-        return NTX_ERR_DEVICE;
-    }
     *ms = best;
     return NTX_OK;
 }
@@ -323,7 +334,6 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             const bool zero_w = d.color_weights[0] == 0 && d.color_weights[1] == 0 && d.color_weights[2] == 0 &&
                                 d.color_weights[3] == 0;
             for (int k = 0; k < 3; k++) lv.cw[k] = zero_w ? 1.0f : d.color_weights[k];
-            lv.ready = l == 0 ? -1 : b->ncounters++;
             lv.next = l + 1 < L ? int(b->levels.size()) + 1 : -1;
             img_off.push_back(img_bytes);
             if (l > 0) img_bytes += align256(size_t(w) * size_t(h) * 4 * sizeof(float));
@@ -340,7 +350,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
     hipError_t e = hipMalloc(&b->d_images, std::max<size_t>(img_bytes, 256));
     if (e == hipSuccess) e = hipMalloc(&b->d_out, std::max<size_t>(out_total, 256));
     if (e == hipSuccess) e = hipMalloc(&b->d_levels, sizeof(BatchLevel) * b->levels.size());
-    if (e == hipSuccess) e = hipMalloc(&b->d_sync, sizeof(int) * size_t(b->ncounters + 2));
+    if (e == hipSuccess) e = hipMalloc(&b->d_sync, sizeof(int));
     if (e != hipSuccess) return cleanup(ntx::fail(ctx, e, "batch buffers"));
     for (int t = 0; t < n; t++) {
         const ntx_texture_desc &d = b->tex[t];
@@ -463,12 +473,6 @@ NTX_API ntx_status ntx_hip_batch_wait(ntx_hip_batch *b)
     ntx_status s = ntx::bind(ctx);
     if (s) return s;
     NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    int status = 0;
-    NTX_HIP(ctx, hipMemcpy(&status, b->d_sync + b->ncounters + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (status) {
-        ctx->err = "batch: a dependency wait did not complete";
-        return NTX_ERR_DEVICE;
-    }
     return NTX_OK;
 }
 
