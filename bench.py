@@ -787,8 +787,9 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
     """Batched uneven workload: n textures with power-of-two sizes drawn
     uniformly from {64..2048} x {64..2048}; half RGBA noise+rectangles -> BC1,
     a quarter normal maps (packed XY) -> BC5, a quarter single-channel noise
-    -> BC4; full Triangle mip chains.  One persistent launch per run, replayed
-    from a HIP graph, with the autotuned work decomposition.  Each texture is
+    -> BC4; full Triangle mip chains.  One HIP graph per run (a launch per mip
+    stage, the persistent compression launch, the two-pass BC1 encoder), with
+    the autotuned work decomposition.  Each texture is
     cut from one of three 2048x2048 generated images at a seeded offset."""
     import ntx_amd as N
     from ntx_amd import synth
@@ -857,10 +858,11 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
             nlev += 1
             same += int(np.array_equal(b.blocks(i, k), out[k].download(np.uint8)))
     b.close()
-    # Roofline of the batch run (the persistent kernel and the two-pass BC1
-    # launches of one graph): the counted compression work (BC1 by each
-    # block's colour counts, BC4 / BC5 per channel block) over its time; the
-    # mips (16 B per source and output texel) run inside the persistent kernel.
+    # Roofline of the batch run (the mip stages, the persistent compression
+    # kernel and the two-pass BC1 launches of one graph): the counted
+    # compression work (BC1 by each block's colour counts, BC4 / BC5 per
+    # channel block) over its time; the mips (16 B per source and output
+    # texel) are part of the same graph.
     work = 0.0
     mip_bytes = 0
     for t, (lv, out) in zip(texs, chains):
@@ -880,7 +882,7 @@ def bench_batch(ctx, args, cpu: bool = False) -> dict:
     return {
         "workload": f"{n} textures, sizes uniform over {{64..2048}}^2 (powers of two), {n // 2} BC1 noise+rect, "
                     f"{n // 4} BC5 normal maps, {n - n // 2 - n // 4} BC4 single-channel; full Triangle mip chains; "
-                    f"one HIP graph per run (persistent kernel, then the two-pass BC1 encoder over every BC1 level)",
+                    f"one HIP graph per run (mip stages, compression queue, two-pass BC1 over every BC1 level)",
         "texels_per_run": texels,
         "mtexels_s": round(texels / (ms / 1e3) / 1e6, 2),
         "ms_per_run": round(ms, 4),

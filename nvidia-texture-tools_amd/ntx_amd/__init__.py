@@ -553,7 +553,7 @@ class Context:
 
 class Batch:
     """Many textures, each with a mip chain and a block format, run as one
-    persistent launch (ntx_hip_batch_*).  `textures` is a sequence of dicts
+    HIP graph (ntx_hip_batch_*: mip stages, compression queue, two-pass BC1).  `textures` is a sequence of dicts
     with keys d_rgba (DeviceBuffer of planar FP32 level 0), width, height,
     format and optionally quality, alpha_mode, color_weights, mip_levels,
     filter, filter_width, filter_params, wrap."""

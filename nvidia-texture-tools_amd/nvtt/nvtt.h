@@ -267,9 +267,9 @@ struct Compressor {
     // Every image of the list (BatchList above), delivered to each entry's
     // OutputOptions in list order, level by level, with the same
     // beginImage / writeData / endImage / error sequence and the same bytes
-    // as compress(Surface) per image.  BC1, BC4 and BC5 run as one persistent
-    // device launch for the whole list (mip chains included); BC3 runs each
-    // entry's chain through the chain scheduler.
+    // as compress(Surface) per image.  BC1, BC4 and BC5 run as one batch
+    // graph for the whole list (mip chains included); BC3 runs each entry's
+    // chain through the chain scheduler.
     NVTT_API bool compress(const BatchList &list, const CompressionOptions &compressionOptions) const;
     Private &m;
 
