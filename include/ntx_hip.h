@@ -349,11 +349,14 @@ NTX_API ntx_status ntx_hip_bc1_sse_device(ntx_hip_context *ctx, const float *d_r
  * Compressor::compress call (Context.cpp:146-190, 486-516).  A batch is many
  * textures of any sizes, each with a full or partial mip chain
  * (Surface::buildNextMipmap semantics, Surface.cpp:1335-1406) and every level
- * block-compressed (Compressor::compress semantics), run as one HIP graph:
- * a launch per mip stage (level s + 1 of every texture), one persistent
- * launch over the compression items, and the two-pass BC1 encoder over every
- * BC1 level at icbc Level8.  Every output byte equals what the per-image
- * entry points above produce for the same level.
+ * block-compressed (Compressor::compress semantics), run as one HIP graph in
+ * one of two schedules (ntx_batch_tuning::schedule): a launch per mip stage
+ * (level s + 1 of every texture) then one persistent launch over the
+ * compression items, or one persistent launch taking mip and compression
+ * items from one queue, ordered by per-level ready counters; after either,
+ * the two-pass BC1 encoder over every BC1 level at icbc Level8.  Every output
+ * byte equals what the per-image entry points above produce for the same
+ * level.
  *
  * Formats: BC1 at every quality (icbc Level8 one block per lane through the
  * two-pass encoder, Level1 / Level9 one block per 16-lane group), BC4 and
@@ -384,6 +387,8 @@ typedef struct {
     int mip_tile_width;        /* 16, 32, 64, 128 or 256 outputs; tile height = 256 / width */
     int kernel_variant;        /* 0: 4 waves/SIMD, 1: 5, 2: 6 (register budget of the kernel) */
     int workgroups;            /* persistent workgroups, 0 = as many as are resident */
+    int schedule;              /* 0: a launch per mip stage, then the persistent compression launch;
+                                  1: mips and compression in one persistent launch */
 } ntx_batch_tuning;
 
 typedef struct ntx_hip_batch ntx_hip_batch;
@@ -398,7 +403,8 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
 NTX_API ntx_status ntx_hip_batch_destroy(ntx_hip_batch *batch);
 /* Enqueues one run of the whole batch on the context's stream. */
 NTX_API ntx_status ntx_hip_batch_submit(ntx_hip_batch *batch);
-/* Waits for the submitted runs. */
+/* Waits for the submitted runs; NTX_ERR_DEVICE if the one-launch schedule's
+ * kernel reported a dependency wait that did not complete. */
 NTX_API ntx_status ntx_hip_batch_wait(ntx_hip_batch *batch);
 /* Number of levels texture `tex` has in this batch. */
 NTX_API int ntx_hip_batch_levels(const ntx_hip_batch *batch, int tex);

@@ -1,13 +1,23 @@
 // batch.hip -- the kernels of a batch of textures: every texture's mip chain
 // (fastDownSample or polyphase resize) and the BC1 / BC4 / BC5 compression of
-// every level, from one work list.
+// every level, from one work list, in one of two schedules.
 //
-// Mips run stage by stage: launch s makes level s + 1 of every texture from
-// level s (batch_mip_kernel, one workgroup per item), so each launch's
-// sources are complete when it starts and hundreds of textures' tiles fill
-// the device together.  The compression items of every level then run in one
-// persistent launch (batch_kernel): workgroups take items from a queue in
-// list order (one atomic per item), largest levels first.
+// Staged: launch s makes level s + 1 of every texture from level s
+// (batch_mip_kernel, one workgroup per item), so each launch's sources are
+// complete when it starts and hundreds of textures' tiles fill the device
+// together; the compression items of every level then run in one persistent
+// launch (batch_kernel<..., false>) whose workgroups take items from a queue
+// in list order (one atomic per item), largest levels first.
+//
+// One launch: batch_kernel<..., true> takes mip and compression items from
+// the same queue.  Each item reads one level; levels produced inside the
+// launch are guarded by a ready counter that every mip item writing the
+// level increments once its outputs are globally visible.  The host orders
+// the list so an item only ever depends on items before it: the earliest
+// unfinished item in the list always has its inputs complete, so the queue
+// drains whatever the number of resident workgroups.  A wait that exceeds its
+// poll limit sets a status bit and continues, so every wave reaches the end
+// of the kernel.
 //
 // Per-item arithmetic is the per-image kernels' own device code (bc1_block.hpp,
 // channel_block.hpp, mip_taps.hpp), so results are bit-identical to them.
@@ -23,7 +33,9 @@ using ntx::BatchItem;
 using ntx::BatchLevel;
 
 // A descriptor every lane reads at the same address, copied into scalar
-// registers (vector loads, then the first lane's value).
+// registers.  The loads themselves are vector loads: in the one-launch
+// schedule descriptors are read after other workgroups' writes in the same
+// launch, which the scalar cache does not observe.
 template <typename T>
 __device__ __forceinline__ T uniform_copy(const T *p)
 {
@@ -42,6 +54,20 @@ template <typename T>
 __device__ __forceinline__ T *global_ptr(T *p)
 {
     return (T *)(__attribute__((address_space(1))) T *)p;
+}
+
+__device__ __forceinline__ void wait_ready(int *counter, int target, int spin_limit, int *status)
+{
+    // Gives up after spin_limit polls, or at once when another wait already
+    // gave up (the launch is failing; later waits need not spin as well).
+    int polls = 0;
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++polls > spin_limit || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            atomicOr(status, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
 }
 
 // LDS for a polyphase tile in the separable form: the tile's source region
@@ -204,28 +230,50 @@ __global__ __launch_bounds__(256) void batch_mip_kernel(BatchArgs a)
     }
 }
 
-// The compression items, every level already complete.
+// The persistent kernel.  kMips false: the compression items of the staged
+// schedule, every level already complete.  kMips true: the one-launch
+// schedule, mip items included and dependencies waited for.
 // kWaves: waves per SIMD the register allocation is held to (0: the
 // compiler's choice); the autotuner picks among the instances.  kTiers: the
 // batch holds BC1 levels at icbc levels other than 8 (Fastest, Production),
 // whose code would otherwise cost every batch registers.
-template <int kWaves, bool kTiers>
+template <int kWaves, bool kTiers, bool kMips>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves > 0 ? kWaves : 1)))
 void batch_kernel(BatchArgs a)
 {
     __shared__ ntx_bc1::Bc1Shared sh;
+    __shared__ float s_src[kMips ? kMipSrc : 1], s_tmp[kMips ? kMipTmp : 1], s_taps[kMips ? kMipTaps : 1];
+    __shared__ int s_first[kMips ? 512 : 1];
     __shared__ int s_item;
     const int t = threadIdx.x;
     ntx_bc1::load_shared_tables(sh, a.tab, t, 256);
     for (;;) {
         __syncthreads();  // the previous item is finished by every thread
-        if (t == 0) s_item = atomicAdd(a.head, 1);
+        if (t == 0) {
+            const int it = atomicAdd(a.head, 1);
+            if (kMips && it < a.nitems) {
+                const BatchLevel &lv = a.levels[a.items[it].level];
+                if (lv.ready >= 0) wait_ready(a.counters + lv.ready, lv.target, a.spin_limit, a.status);
+            }
+            s_item = it;
+        }
         __syncthreads();
         const int idx = __builtin_amdgcn_readfirstlane(s_item);
         if (idx >= a.nitems) break;
         const BatchItem it = uniform_copy(a.items + idx);
         const BatchLevel lv = uniform_copy(a.levels + it.level);
-        if (it.kind == ntx::kItemBc1) {
+        if (kMips && lv.ready >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (kMips && it.kind == ntx::kItemMip) {
+            mip_item(a, it, lv, t, s_src, s_tmp, s_taps, s_first);
+            // Every thread makes its stores visible device-wide before the
+            // workgroup counts the item as done.
+            __threadfence();
+            __syncthreads();
+            if (t == 0) {
+                const int ready = __builtin_amdgcn_readfirstlane(a.levels[lv.next].ready);
+                __hip_atomic_fetch_add(a.counters + ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (it.kind == ntx::kItemBc1) {
             if constexpr (kTiers) {
                 // One cluster-fit instance for levels 8 and 9 (two would
                 // spill): level 9's refinement is switched per level.
@@ -242,8 +290,8 @@ void batch_kernel(BatchArgs a)
     }
 }
 
-// Clears the queue position (a kernel rather than a memset node, so the run
-// can be captured in a graph).
+// Clears the ready counters, the queue position and the status word (a
+// kernel rather than a memset node, so the run can be captured in a graph).
 __global__ __launch_bounds__(256) void batch_reset_kernel(int *p, int n)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -255,27 +303,34 @@ __global__ __launch_bounds__(256) void batch_reset_kernel(int *p, int n)
 namespace ntx {
 
 // variant 0-2: the tuned register budget (compiler's choice, 5 or 6 waves per
-// SIMD); tiers: the instance with the BC1 Fastest / Production paths.
+// SIMD); tiers: the instance with the BC1 Fastest / Production paths; mips:
+// the one-launch schedule's instance.
 using BatchKernel = void (*)(BatchArgs);
-BatchKernel batch_kernel_of(int variant, bool tiers)
+template <bool kMips>
+BatchKernel batch_kernel_sched(int variant, bool tiers)
 {
     switch (variant) {
-    case 1: return tiers ? batch_kernel<5, true> : batch_kernel<5, false>;
-    case 2: return tiers ? batch_kernel<6, true> : batch_kernel<6, false>;
-    default: return tiers ? batch_kernel<0, true> : batch_kernel<0, false>;
+    case 1: return tiers ? batch_kernel<5, true, kMips> : batch_kernel<5, false, kMips>;
+    case 2: return tiers ? batch_kernel<6, true, kMips> : batch_kernel<6, false, kMips>;
+    default: return tiers ? batch_kernel<0, true, kMips> : batch_kernel<0, false, kMips>;
     }
 }
 
-const void *batch_kernel_variant(int variant, bool tiers)
+BatchKernel batch_kernel_of(int variant, bool tiers, bool mips)
 {
-    return reinterpret_cast<const void *>(batch_kernel_of(variant, tiers));
+    return mips ? batch_kernel_sched<true>(variant, tiers) : batch_kernel_sched<false>(variant, tiers);
+}
+
+const void *batch_kernel_variant(int variant, bool tiers, bool mips)
+{
+    return reinterpret_cast<const void *>(batch_kernel_of(variant, tiers, mips));
 }
 
 int batch_kernel_variants() { return 3; }
 
-hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream)
+hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, bool mips, int workgroups, hipStream_t stream)
 {
-    hipLaunchKernelGGL(batch_kernel_of(variant, tiers), dim3(workgroups), dim3(256), 0, stream, args);
+    hipLaunchKernelGGL(batch_kernel_of(variant, tiers, mips), dim3(workgroups), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
@@ -294,10 +349,11 @@ hipError_t launch_batch_reset(int *p, int n, hipStream_t stream)
 }
 
 // Workgroups of one batch_kernel instance that fit on the device at once.
-hipError_t batch_resident_workgroups(int device, int variant, bool tiers, int *count)
+hipError_t batch_resident_workgroups(int device, int variant, bool tiers, bool mips, int *count)
 {
     int per_cu = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_variant(variant, tiers), 256, 0);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_variant(variant, tiers, mips), 256, 0);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     *count = per_cu * cus;
     return e;

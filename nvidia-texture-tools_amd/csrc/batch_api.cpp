@@ -14,10 +14,10 @@
 #include "ntx_context.h"
 
 namespace ntx {
-hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, int workgroups, hipStream_t stream);
+hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, bool mips, int workgroups, hipStream_t stream);
 hipError_t launch_batch_reset(int *p, int n, hipStream_t stream);
 hipError_t launch_batch_mips(const BatchArgs &args, hipStream_t stream);
-hipError_t batch_resident_workgroups(int device, int variant, bool tiers, int *count);
+hipError_t batch_resident_workgroups(int device, int variant, bool tiers, bool mips, int *count);
 int batch_kernel_variants();
 }  // namespace ntx
 
@@ -32,10 +32,13 @@ struct ntx_hip_batch {
     ntx::BatchLevel *d_levels = nullptr;
     ntx::BatchItem *d_items = nullptr;
     size_t items_cap = 0;
-    int *d_sync = nullptr;               // queue head of the compression launch
+    int *d_sync = nullptr;               // ready counters, queue head, status
+    int ncounters = 0;
     int nitems = 0;
-    // The work list: the mip items of stage s at [stage_first[s],
-    // stage_first[s + 1]), then the compression items from stage_first.back().
+    // The work list.  Staged schedule: the mip items of stage s at
+    // [stage_first[s], stage_first[s + 1]), then the compression items from
+    // stage_first.back().  One-launch schedule: stage_first = {0}, every item
+    // in the persistent launch.
     std::vector<int> stage_first;
     bool bc1_tiers = false;              // some BC1 level is not at icbc level 8
     int workgroups = 0;
@@ -59,6 +62,8 @@ namespace {
 
 using ntx::BatchItem;
 using ntx::BatchLevel;
+
+constexpr int kSpinLimit = 1 << 22;  // polls (each a sleep plus an atomic read): seconds, far beyond any real wait
 
 
 int count_mipmaps(int w, int h)
@@ -84,6 +89,7 @@ ntx_batch_tuning default_tuning()
     t.mip_tile_width = 64;
     t.kernel_variant = 1;
     t.workgroups = 0;
+    t.schedule = 0;
     return t;
 }
 
@@ -93,7 +99,7 @@ bool valid_tuning(const ntx_batch_tuning &t)
     return t.bc1_blocks_per_item >= 16 && t.bc1_blocks_per_item % 16 == 0 && t.channel_jobs_per_item >= 256 &&
            t.channel_jobs_per_item % 256 == 0 && t.mip_tiles_per_item >= 1 &&
            (tw == 16 || tw == 32 || tw == 64 || tw == 128 || tw == 256) && t.kernel_variant >= 0 &&
-           t.kernel_variant < ntx::batch_kernel_variants() && t.workgroups >= 0;
+           t.kernel_variant < ntx::batch_kernel_variants() && t.workgroups >= 0 && (t.schedule == 0 || t.schedule == 1);
 }
 
 int log2i(int v)
@@ -111,14 +117,18 @@ void release_graph(ntx_hip_batch *b)
     b->graph = nullptr;
 }
 
-// The work list for the current tuning: for each stage s the mip items that
-// make level s + 1 of every texture (one launch per stage), then the
-// compression items of every level (one launch), stage by stage with BC1
-// before channel jobs, each group in decreasing texture size.
+// The work list for the current tuning.  Staged schedule: for each stage s
+// the mip items that make level s + 1 of every texture (one launch per
+// stage), then the compression items of every level (one launch).  One-launch
+// schedule: stage s holds the mip items that make level s + 1 of every
+// texture, then the compression items of level s, so every item depends only
+// on items earlier in the list.  Compression items go stage by stage with BC1
+// before channel jobs; each group in decreasing texture size.
 ntx_status build_items(ntx_hip_batch *b)
 {
     ntx_hip_context *ctx = b->ctx;
     const ntx_batch_tuning &tu = b->tuning;
+    const bool one_launch = tu.schedule == 1;
     const int tw = tu.mip_tile_width, th = 256 / tu.mip_tile_width;
     const int n = int(b->tex.size());
     std::vector<int> order(n);
@@ -129,20 +139,20 @@ ntx_status build_items(ntx_hip_batch *b)
     int stages = 0;
     for (int t = 0; t < n; t++) stages = std::max(stages, b->first[t + 1] - b->first[t]);
     std::vector<BatchItem> items;
-    b->stage_first.clear();
-    for (int s = 0; s < stages; s++) {
-        b->stage_first.push_back(int(items.size()));
+    for (auto &lv : b->levels) lv.target = 0;
+    auto mips = [&](int s) {
         for (int t : order) {
             if (s + 1 >= b->first[t + 1] - b->first[t]) continue;
             const int li = b->first[t] + s;
             const BatchLevel &dst = b->levels[li + 1];
             const int tiles = ((dst.w + tw - 1) / tw) * ((dst.h + th - 1) / th);
-            for (int k = 0; k < tiles; k += tu.mip_tiles_per_item)
+            for (int k = 0; k < tiles; k += tu.mip_tiles_per_item) {
                 items.push_back({ ntx::kItemMip, li, k, std::min(tu.mip_tiles_per_item, tiles - k) });
+                b->levels[li + 1].target++;
+            }
         }
-    }
-    b->stage_first.push_back(int(items.size()));
-    for (int s = 0; s < stages; s++) {
+    };
+    auto compression = [&](int s) {
         for (int pass = 0; pass < 2; pass++) {
             for (int t : order) {
                 if (s >= b->first[t + 1] - b->first[t]) continue;
@@ -159,6 +169,21 @@ ntx_status build_items(ntx_hip_batch *b)
                 }
             }
         }
+    };
+    b->stage_first.clear();
+    if (one_launch) {
+        b->stage_first.push_back(0);
+        for (int s = 0; s < stages; s++) {
+            mips(s);
+            compression(s);
+        }
+    } else {
+        for (int s = 0; s < stages; s++) {
+            b->stage_first.push_back(int(items.size()));
+            mips(s);
+        }
+        b->stage_first.push_back(int(items.size()));
+        for (int s = 0; s < stages; s++) compression(s);
     }
     b->nitems = int(items.size());
     if (items.size() > b->items_cap) {
@@ -172,7 +197,7 @@ ntx_status build_items(ntx_hip_batch *b)
         NTX_HIP(ctx, hipMemcpy(b->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice));
     NTX_HIP(ctx, hipMemcpy(b->d_levels, b->levels.data(), sizeof(BatchLevel) * b->levels.size(), hipMemcpyHostToDevice));
     int resident = 0;
-    NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, tu.kernel_variant, b->bc1_tiers, &resident));
+    NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, tu.kernel_variant, b->bc1_tiers, one_launch, &resident));
     int wgs = tu.workgroups > 0 ? std::min(tu.workgroups, resident) : resident;
     wgs = std::max(1, std::min(wgs, std::max(b->nitems - b->stage_first.back(), 1)));
     b->workgroups = wgs;
@@ -185,18 +210,22 @@ ntx::BatchArgs make_args(const ntx_hip_batch *b)
     a.items = b->d_items;
     a.levels = b->d_levels;
     a.tab = b->ctx->d_tab;
-    a.head = b->d_sync;
+    a.counters = b->d_sync;
+    a.head = b->d_sync + b->ncounters;
+    a.status = b->d_sync + b->ncounters + 1;
     a.nitems = b->nitems;
     a.tile_shift = log2i(b->tuning.mip_tile_width);
+    a.spin_limit = kSpinLimit;
     return a;
 }
 
-// The launches of one run, enqueued on the context's stream: the queue
-// reset, the mip stages, the compression items, the multi-image BC1 passes.
+// The launches of one run, enqueued on the context's stream: the counter
+// reset, the mip stages (staged schedule), the persistent launch, the
+// multi-image BC1 passes.
 hipError_t enqueue_run(ntx_hip_batch *b)
 {
     hipStream_t s = b->ctx->stream;
-    hipError_t e = ntx::launch_batch_reset(b->d_sync, 1, s);
+    hipError_t e = ntx::launch_batch_reset(b->d_sync, b->ncounters + 2, s);
     const ntx::BatchArgs all = make_args(b);
     const int nstages = int(b->stage_first.size()) - 1;
     for (int k = 0; e == hipSuccess && k < nstages; k++) {
@@ -210,7 +239,7 @@ hipError_t enqueue_run(ntx_hip_batch *b)
         ntx::BatchArgs a = all;
         a.items = b->d_items + c0;
         a.nitems = b->nitems - c0;
-        e = ntx::launch_batch(a, b->tuning.kernel_variant, b->bc1_tiers, b->workgroups, s);
+        e = ntx::launch_batch(a, b->tuning.kernel_variant, b->bc1_tiers, b->tuning.schedule == 1, b->workgroups, s);
     }
     if (e == hipSuccess && b->bc1_imgs > 0) {
         int *counts = b->d_bc1_bucket, *lists = counts + ntx::bc3_lists_offset();
@@ -282,6 +311,12 @@ ntx_status time_runs(ntx_hip_batch *b, int reps, float *ms)
         NTX_HIP(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
         if (r == 0 || t < best) best = t;
     }
+    int status = 0;
+    NTX_HIP(ctx, hipMemcpy(&status, b->d_sync + b->ncounters + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (status) {
+        ctx->err = "batch: a dependency wait did not complete";
+        return NTX_ERR_DEVICE;
+    }
     *ms = best;
     return NTX_OK;
 }
@@ -330,6 +365,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             lv.format = d.format;
             lv.alpha_mode = d.alpha_mode;
             lv.bc1_level = ntx::bc1_level(d.quality);
+            lv.ready = l == 0 ? -1 : b->ncounters++;
             if (d.format == NTX_FORMAT_BC1 && lv.bc1_level != 8) b->bc1_tiers = true;
             const bool zero_w = d.color_weights[0] == 0 && d.color_weights[1] == 0 && d.color_weights[2] == 0 &&
                                 d.color_weights[3] == 0;
@@ -350,7 +386,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
     hipError_t e = hipMalloc(&b->d_images, std::max<size_t>(img_bytes, 256));
     if (e == hipSuccess) e = hipMalloc(&b->d_out, std::max<size_t>(out_total, 256));
     if (e == hipSuccess) e = hipMalloc(&b->d_levels, sizeof(BatchLevel) * b->levels.size());
-    if (e == hipSuccess) e = hipMalloc(&b->d_sync, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->d_sync, sizeof(int) * size_t(b->ncounters + 2));
     if (e != hipSuccess) return cleanup(ntx::fail(ctx, e, "batch buffers"));
     for (int t = 0; t < n; t++) {
         const ntx_texture_desc &d = b->tex[t];
@@ -473,6 +509,12 @@ NTX_API ntx_status ntx_hip_batch_wait(ntx_hip_batch *b)
     ntx_status s = ntx::bind(ctx);
     if (s) return s;
     NTX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int status = 0;
+    NTX_HIP(ctx, hipMemcpy(&status, b->d_sync + b->ncounters + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (status) {
+        ctx->err = "batch: a dependency wait did not complete";
+        return NTX_ERR_DEVICE;
+    }
     return NTX_OK;
 }
 
@@ -524,24 +566,27 @@ NTX_API ntx_status ntx_hip_batch_autotune(ntx_hip_batch *b, int reps, ntx_batch_
     float best_ms = 0.0f;
     if ((s = time_runs(b, reps, &best_ms))) return s;
     int resident = 0;
-    // Coordinate descent: each field in turn over its candidates, others fixed.
-    for (int field = 0; field < 6; field++) {
+    // Coordinate descent: each field in turn over its candidates, others
+    // fixed; the schedule first, the decomposition under it after.
+    for (int field : { 6, 0, 1, 2, 3, 4, 5 }) {
         std::vector<int> cand;
         switch (field) {
+        case 6: cand = { 0, 1 }; break;                             // schedule
         case 0: cand = { 0, 1, 2 }; break;                          // kernel_variant
         case 1: cand = { 16, 32, 64, 128, 256 }; break;             // mip_tile_width
         case 2: cand = { 1, 2, 4, 8 }; break;                       // mip_tiles_per_item
         case 3: cand = { 16, 32, 64, 128, 256 }; break;             // bc1_blocks_per_item
         case 4: cand = { 256, 512, 1024, 2048, 4096 }; break;       // channel_jobs_per_item
         default:                                                    // workgroups
-            NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, best.kernel_variant, b->bc1_tiers, &resident));
+            NTX_HIP(ctx, ntx::batch_resident_workgroups(ctx->device, best.kernel_variant, b->bc1_tiers,
+                                                        best.schedule == 1, &resident));
             cand = { 0, resident * 3 / 4, resident / 2, resident / 4 };
             break;
         }
         for (int v : cand) {
             ntx_batch_tuning t = best;
             int *slot[] = { &t.kernel_variant, &t.mip_tile_width, &t.mip_tiles_per_item, &t.bc1_blocks_per_item,
-                            &t.channel_jobs_per_item, &t.workgroups };
+                            &t.channel_jobs_per_item, &t.workgroups, &t.schedule };
             if (*slot[field] == v) continue;
             *slot[field] = v;
             if (field == 5 && v <= 0 && best.workgroups == 0) continue;

@@ -1,7 +1,8 @@
 // ntx_batch.h -- device-side layout of a batch: many textures, each with its
 // own mip chain and block format, run from a work list built on the host
-// (batch_api.cpp): one launch per mip stage, then one persistent launch over
-// the compression items (batch.hip).
+// (batch_api.cpp) in one of two schedules (batch.hip): a launch per mip stage
+// then one persistent launch over the compression items, or everything in
+// one persistent launch whose mip items complete per-level ready counters.
 #pragma once
 
 #include <stdint.h>
@@ -29,6 +30,8 @@ struct BatchLevel {
     int alpha_mode;
     float cw[3];        // colour weights (BC1)
     int bc1_level;      // BC1: icbc quality level (1, 8 or 9)
+    int ready;          // one-launch schedule: counter that completes `data` (-1: complete before launch)
+    int target;         // its value once every item writing `data` has finished
     int next;           // index of the next level, -1 for the last one
     int mip_kind;       // 0: fastDownSample, 1: polyphase resize
     int wrap;
@@ -47,9 +50,12 @@ struct BatchArgs {
     const BatchItem *items;  // the launch's items
     const BatchLevel *levels;
     const Tables *tab;
-    int *head;        // queue position (compression launch)
+    int *counters;    // ready counters of the levels (one-launch schedule)
+    int *head;        // queue position of the persistent launch
+    int *status;      // bit 0: a dependency wait gave up
     int nitems;
     int tile_shift;   // mip tile: (1 << tile_shift) x (256 >> tile_shift) outputs
+    int spin_limit;   // polls before a dependency wait gives up
 };
 
 }  // namespace ntx

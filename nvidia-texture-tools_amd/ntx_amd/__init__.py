@@ -91,7 +91,7 @@ class BatchTuning(ctypes.Structure):
     """ntx_batch_tuning (include/ntx_hip.h)."""
     _fields_ = [("bc1_blocks_per_item", ctypes.c_int), ("channel_jobs_per_item", ctypes.c_int),
                 ("mip_tiles_per_item", ctypes.c_int), ("mip_tile_width", ctypes.c_int),
-                ("kernel_variant", ctypes.c_int), ("workgroups", ctypes.c_int)]
+                ("kernel_variant", ctypes.c_int), ("workgroups", ctypes.c_int), ("schedule", ctypes.c_int)]
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
@@ -553,7 +553,8 @@ class Context:
 
 class Batch:
     """Many textures, each with a mip chain and a block format, run as one
-    HIP graph (ntx_hip_batch_*: mip stages, compression queue, two-pass BC1).  `textures` is a sequence of dicts
+    HIP graph (ntx_hip_batch_*: mip stages and a compression queue, or one
+    persistent launch for both, then the two-pass BC1 encoder).  `textures` is a sequence of dicts
     with keys d_rgba (DeviceBuffer of planar FP32 level 0), width, height,
     format and optionally quality, alpha_mode, color_weights, mip_levels,
     filter, filter_width, filter_params, wrap."""

@@ -100,6 +100,11 @@ def test_batch_matches_oracle(gpu, oracle):
     dict(workgroups=3, mip_tile_width=16, mip_tiles_per_item=1, bc1_blocks_per_item=16, channel_jobs_per_item=256),
     dict(kernel_variant=0, mip_tile_width=256, mip_tiles_per_item=8, bc1_blocks_per_item=256),
     dict(kernel_variant=2, mip_tile_width=32, channel_jobs_per_item=4096),
+    dict(schedule=1),
+    dict(schedule=1, workgroups=1),
+    dict(schedule=1, workgroups=3, mip_tile_width=16, mip_tiles_per_item=1, bc1_blocks_per_item=16,
+         channel_jobs_per_item=256),
+    dict(schedule=1, kernel_variant=2, mip_tile_width=256, mip_tiles_per_item=8),
 ])
 def test_batch_tunings_are_bit_identical(gpu, tuning):
     texs, hosts = _make(gpu, MIXED, seed0=50)

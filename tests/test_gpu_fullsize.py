@@ -61,11 +61,12 @@ def test_c2_srgb_kaiser_bc3_chain(gpu, oracle):
         d.free()
 
 
-def test_c3_batch_matches_per_image_path(gpu):
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_c3_batch_matches_per_image_path(gpu, schedule):
     """256 textures of sizes in {64..2048}^2 (BC1 noise+rect, BC5 normal maps,
-    BC4 single channel) with full Triangle chains in one batch launch: every
-    level equal to the per-image entry points (which the other tests pin to
-    the oracle)."""
+    BC4 single channel) with full Triangle chains in one batch graph, in both
+    schedules (mip stages, or one persistent launch): every level equal to
+    the per-image entry points (which the other tests pin to the oracle)."""
     rng = np.random.default_rng(1001)
     choices = [64, 128, 256, 512, 1024, 2048]
     base = {N.FORMAT_BC1: synth.noise_rect_rgba(2048, 2048, seed=2),
@@ -80,6 +81,7 @@ def test_c3_batch_matches_per_image_path(gpu):
         texs.append(dict(d_rgba=gpu.alloc(planar.nbytes).upload(planar), width=w, height=h, format=fmt,
                          filter=N.FILTER_TRIANGLE, wrap=N.WRAP_MIRROR))
     with N.Batch(gpu, texs) as b:
+        b.set_tuning(schedule=schedule)
         b.run()
         bad = []
         for i, t in enumerate(texs):
