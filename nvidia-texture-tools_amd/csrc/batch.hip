@@ -131,19 +131,38 @@ __device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLev
         if (t < ny) s_first[256 + t] = src.ly[oy0 + t] - ry0;
         // Thread t stages region elements t + 256 k; their (wrapped) source
         // offsets are the same in every plane, and plane c + 1's values are
-        // fetched into registers while plane c is filtered.
+        // fetched into registers while plane c is filtered.  Element
+        // t + 256 k sits at row r + k q, column col + k m of the region
+        // (256 = q sw + m), stepped without a division per element; slots
+        // past the region take its last element.  A region inside the image
+        // needs no wrap (a branch uniform over the workgroup).
         constexpr int kSlots = kMipSrc / 256;
         const int nsrc = sw * sh;
+        const int q = 256 / sw, m = 256 - q * sw;
+        const bool inside = cx0 >= 0 && cx0 + sw <= W && ry0 >= 0 && ry0 + sh <= H;
         uint32_t off[kSlots];
         float v[kSlots];
+        {
+            int r = t / sw, col = t - r * sw;
 #pragma unroll
-        for (int k = 0; k < kSlots; k++) {
-            const int i = min(t + 256 * k, nsrc - 1);
-            const int r = i / sw, col = i - r * sw;
-            off[k] = uint32_t(ntx_mip::wrap_index(ry0 + r, H, src.wrap)) * uint32_t(W) +
-                     uint32_t(ntx_mip::wrap_index(cx0 + col, W, src.wrap));
-            v[k] = s[off[k]];
+            for (int k = 0; k < kSlots; k++) {
+                const bool in = t + 256 * k < nsrc;
+                const int rr = in ? r : sh - 1, cc = in ? col : sw - 1;
+                if (inside)
+                    off[k] = uint32_t(ry0 + rr) * uint32_t(W) + uint32_t(cx0 + cc);
+                else
+                    off[k] = uint32_t(ntx_mip::wrap_index(ry0 + rr, H, src.wrap)) * uint32_t(W) +
+                             uint32_t(ntx_mip::wrap_index(cx0 + cc, W, src.wrap));
+                r += q;
+                col += m;
+                if (col >= sw) {
+                    col -= sw;
+                    r++;
+                }
+            }
         }
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) v[k] = s[off[k]];
 #pragma unroll 1
         for (int c = 0; c < 4; c++) {
 #pragma unroll
@@ -155,13 +174,26 @@ __device__ void mip_item(const BatchArgs &a, const BatchItem &it, const BatchLev
 #pragma unroll
                 for (int k = 0; k < kSlots; k++) v[k] = pl[off[k]];
             }
-            for (int i = t; i < sh * nx; i += 256) {
-                const int r = i / nx, xi = i - r * nx;
+            if (nx == tw) {
+                // Full-width tile: the thread's column is t mod tw in every
+                // row it takes (256 is a multiple of tw).
+                const int xi = t & (tw - 1);
                 const float *kr = s_kx + xi * src.wx;
-                const float *row = s_src + r * sw + s_first[xi];
-                float sum = 0.0f;
-                for (int j = 0; j < src.wx; ++j) sum += kr[j] * row[j];
-                s_tmp[i] = sum;
+                const float *row = s_src + (t >> a.tile_shift) * sw + s_first[xi];
+                for (int i = t; i < sh * nx; i += 256, row += th * sw) {
+                    float sum = 0.0f;
+                    for (int j = 0; j < src.wx; ++j) sum += kr[j] * row[j];
+                    s_tmp[i] = sum;
+                }
+            } else {
+                for (int i = t; i < sh * nx; i += 256) {
+                    const int r = i / nx, xi = i - r * nx;
+                    const float *kr = s_kx + xi * src.wx;
+                    const float *row = s_src + r * sw + s_first[xi];
+                    float sum = 0.0f;
+                    for (int j = 0; j < src.wx; ++j) sum += kr[j] * row[j];
+                    s_tmp[i] = sum;
+                }
             }
             __syncthreads();
             if (live) {
@@ -215,18 +247,22 @@ __device__ void channel_item(const BatchItem &it, const BatchLevel &lv, int t)
     }
 }
 
-// The mip items of one stage (their sources complete before the launch):
-// workgroup b takes items b, b + grid, ...
+// The mip items of one stage (their sources complete before the launch), one
+// per workgroup.  Workgroups are dispatched round-robin over the eight XCDs,
+// each with its own L2: XCD x takes a contiguous run of items, so the halo
+// rows and columns that neighbouring tiles share are fetched into one L2.
 __global__ __launch_bounds__(256) void batch_mip_kernel(BatchArgs a)
 {
     __shared__ float s_src[kMipSrc], s_tmp[kMipTmp], s_taps[kMipTaps];
     __shared__ int s_first[512];
     const int t = threadIdx.x;
-    for (int idx = blockIdx.x; idx < a.nitems; idx += gridDim.x) {
+    const int nwg = int(gridDim.x), b = int(blockIdx.x);
+    const int xcd = b & 7, q = nwg >> 3, rem = nwg & 7;
+    const int idx = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + (b >> 3);
+    if (idx < a.nitems) {
         const BatchItem it = uniform_copy(a.items + idx);
         const BatchLevel lv = uniform_copy(a.levels + it.level);
         mip_item(a, it, lv, t, s_src, s_tmp, s_taps, s_first);
-        __syncthreads();  // the next item's taps overwrite this one's
     }
 }
 
@@ -338,6 +374,7 @@ hipError_t launch_batch(const BatchArgs &args, int variant, bool tiers, bool mip
 hipError_t launch_batch_mips(const BatchArgs &args, hipStream_t stream)
 {
     if (args.nitems <= 0) return hipSuccess;
+    // One workgroup per item (batch_mip_kernel's item mapping assumes it).
     hipLaunchKernelGGL(batch_mip_kernel, dim3(unsigned(args.nitems)), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
