@@ -434,7 +434,7 @@ NTX_API ntx_status ntx_hip_batch_create(ntx_hip_context *ctx, const ntx_texture_
             const BatchLevel &lv = b->levels[li];
             if (lv.format != NTX_FORMAT_BC1 || lv.bc1_level != 8) continue;
             const int blocks = ((lv.w + 3) / 4) * ((lv.h + 3) / 4);
-            if (int(imgs.size()) >= ntx::bc1_multi_max_images() || blocks >= ntx::bc1_multi_max_blocks() ||
+            if (int(imgs.size()) >= ntx::bc1_multi_max_images() || blocks > ntx::bc1_multi_max_blocks() ||
                 total + blocks >= (1ll << 28))
                 continue;
             b->bc1_multi[li] = 1;

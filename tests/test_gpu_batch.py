@@ -116,6 +116,24 @@ def test_batch_tunings_are_bit_identical(gpu, tuning):
         _check_batch(b, MIXED, refs)
 
 
+def test_batch_bc1_image_at_the_block_index_limit(gpu):
+    """A 4096^2 BC1 level has 2^20 blocks, the most the multi-image encoder's
+    list entries (image << 20 | block) can index: it runs there, beside a
+    small image, and equals the per-image entry point."""
+    specs = [(4096, 4096, N.FORMAT_BC1, N.FILTER_BOX, N.ALPHA_NONE, N.WRAP_MIRROR, 1),
+             (40, 24, N.FORMAT_BC1, N.FILTER_BOX, N.ALPHA_NONE, N.WRAP_MIRROR, 1)]
+    texs, hosts = _make(gpu, specs, seed0=60)
+    with N.Batch(gpu, texs) as b:
+        b.run()
+        for t, (w, h, fmt, *_r) in enumerate(specs):
+            out = gpu.alloc(N.compressed_size(fmt, w, h))
+            gpu.compress_device(fmt, texs[t]["d_rgba"], w, h, out)
+            assert np.array_equal(b.blocks(t, 0), out.download(np.uint8)), t
+            out.free()
+    for t in texs:
+        t["d_rgba"].free()
+
+
 def test_batch_graph_replays(gpu):
     """The captured graph re-reads level 0 on every replay: replays with
     host work between them, and after the inputs change, all match."""
