@@ -282,14 +282,17 @@ __device__ __forceinline__ Best lane_search4(LaneRows rows, int lane, int n, flo
         return best;
     }
     if (units) {
+        // Entries by 32-bit byte offsets from the table's (scalar) base, so
+        // each load takes one vector offset: ((x0.w * 17 + x1.w) * 17 + x2.w) * 16.
+        const char *w4b = reinterpret_cast<const char *>(w4);
         for (int c = 1; c <= n; c++) {
             const float4 x2 = rows[c][lane];
             for (int a = 0; a <= c; a++) {
                 const float4 x0 = rows[a][lane];
-                const int base = (int(x0.w) * 17) * 17 + int(x2.w);
+                const uint32_t base = ((uint32_t(x0.w) * 17u) * 17u + uint32_t(x2.w)) * 16u;
                 for (int b = a; b <= c; b++, p++) {
                     const float4 x1 = rows[b][lane];
-                    const float4 t = w4[base + int(x1.w) * 17];
+                    const float4 t = *reinterpret_cast<const float4 *>(w4b + (base + uint32_t(x1.w) * 272u));
                     best.offer(eval_terms<true>(x0, x1, x2, sum, t, mx, my, mz, nullptr), p);
                 }
             }
@@ -342,11 +345,14 @@ __device__ __forceinline__ Best lane_search3(LaneRows rows, int lane, int m, flo
         }
         return best;
     }
+    const char *w3b = reinterpret_cast<const char *>(w3);
     for (int b = 1; b <= m; b++) {
         const float4 x1 = rows[b][lane];
         for (int a = 0; a <= b; a++, p++) {
             const float4 x0 = rows[a][lane];
-            const float4 t = units ? w3[int(x0.w) * 17 + int(x1.w)] : cluster_terms<false>(x0, x1, x1, sum.w);
+            // As in lane_search4: a 32-bit byte offset (x0.w * 17 + x1.w) * 16.
+            const float4 t = units ? *reinterpret_cast<const float4 *>(w3b + (uint32_t(x0.w) * 272u + uint32_t(x1.w) * 16u))
+                                   : cluster_terms<false>(x0, x1, x1, sum.w);
             best.offer(eval_terms<false>(x0, x1, x1, sum, t, mx, my, mz, nullptr), p);
         }
     }
