@@ -678,6 +678,11 @@ __device__ __forceinline__ void palette8(uint32_t c0, uint32_t c1, uint32_t pk[4
     for (int i = 0; i < 4; i++) {
         pk[i] = q[i][0] | q[i][1] << 8 | q[i][2] << 16;
         nk[i] = 3 - i - 4 * int(__builtin_amdgcn_udot4(pk[i], pk[i], 0u, false));
+        // Opaque to the optimiser (no instruction): each key is then one
+        // shift-add of the texel's dot product with this constant, where
+        // entry 3's "0 - 4|p|^2" would otherwise be folded into a shift and
+        // a subtract per texel.
+        asm("" : "+v"(nk[i]));
     }
 }
 
